@@ -1,0 +1,152 @@
+/*
+ * vpmx.h -- C ABI of the MI355X (gfx950) VPM core.
+ *
+ * Drop-in boundary for the reference's vpm() path (papis84/vpm_free).  The Fortran
+ * module vpm_free_amd/fortran/vpm_lib.f90 keeps the reference's own entry points
+ * (vpm(), remesh_particles_3d(), the pm.inp parse and the public modules) and calls
+ * these functions through ISO_C_BINDING; Python reaches them through ctypes.
+ *
+ * Conventions
+ *   - FP64 at the ABI.  Particle arrays are the reference's Fortran arrays as plain
+ *     pointers: XP(3,NVR), QP(neqpm+1,NVR), UP(3,NVR), GP(3,NVR), column-major.
+ *   - Grids returned by vpmx_get_grid use the reference layout: RHS_pm(neqpm+1,NX,NY,NZ),
+ *     SOL_pm(neqpm,NX,NY,NZ) component-fastest, velvr*_pm(NX,NY,NZ).
+ *   - Every call returns int32 status, 0 = ok; vpmx_last_error() gives the message.
+ *     Failures the reference turns into STOP ('WRONG IDVPM', 'No such projection
+ *     function', IPMWRITE>10, ...) come back as status codes.
+ *   - One host thread per context, one HIP stream per context; calls are synchronous
+ *     at return.  One GPU, one process: the reference's MPI ranks collapse into one
+ *     device (BLOCKS = NBI*NBJ*NBK blocks on that device).
+ */
+#ifndef VPMX_H
+#define VPMX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPMX_MAXB 64
+
+enum {
+    VPMX_OK = 0,
+    VPMX_ERR_ARG = 1,      /* bad argument / parameter outside what the reference accepts   */
+    VPMX_ERR_STATE = 2,    /* call order (e.g. mode 2 before any mode 0 geometry)          */
+    VPMX_ERR_HIP = 3,      /* HIP runtime error                                              */
+    VPMX_ERR_UNSUP = 4,    /* configuration the reference accepts but this build does not    */
+    VPMX_ERR_IO = 5,       /* pm.inp missing or malformed                                     */
+    VPMX_ERR_NODEV = 6     /* no GPU / library loaded without a device                        */
+};
+
+/* pm.inp contents, parsed as vpm.f90:132-152 reads them (one record per line). */
+typedef struct {
+    double dpm[3];                 /* DXpm DYpm DZpm                  */
+    int32_t interf_iproj;          /* projection function 2..6 (4=M4')*/
+    int32_t ibctyp;                /* 2 = boundary-source Green sum    */
+    int32_t idvpm;                 /* 1 constant volume, 0 variable     */
+    double epsvol;
+    int32_t ncoarse;
+    int32_t nbi, nbj, nbk;         /* yaps blocks                       */
+    int32_t nremesh, ncell_rem;
+    int32_t iyntree, ilevmax;
+    int32_t ompthreads, idefine, iynslice, ipmwrite;
+    int32_t ipmwstart[10], ipmwsteps[10];
+} vpmx_params;
+
+/* Geometry of define_sizes (vpm.f90:477-660), kept by the context between calls. */
+typedef struct {
+    double xbound[6], dpm[3];      /* Xbound(6), Dpm(3)                         */
+    int32_t nn[3], nn_bl[6];       /* NN(3), NN_bl(6) (1-based node indices)    */
+    int32_t nblocks, nbi, nbj, nbk;
+    double xbound_bl[6 * VPMX_MAXB];
+    int32_t nnbl[3 * VPMX_MAXB], nnbl_bl[6 * VPMX_MAXB];
+    double xbound_coarse[6], dpm_coarse[3];
+    int32_t nn_coarse[3], nn_bl_coarse[6];
+    int32_t ilevmax;
+    int32_t defined;               /* 1 once the box is fixed (idefine=1)       */
+} vpmx_geom;
+
+typedef struct vpmx_ctx vpmx_ctx;
+
+/* ---------------------------------------------------------------- host-only calls */
+
+/* Parse pm.inp (vpm.f90:132-152).  No device needed. */
+int32_t vpmx_read_pm_inp(const char *path, vpmx_params *out);
+
+/* define_sizes (vpm.f90:477-660) + definepm (pmlib.f90:216-432) on the host, bit-exact
+ * FP64 sequence.  `g` carries the fixed box between calls (idefine=1 semantics:
+ * recomputed only when ntime==0, or every call when idefine==0).  No device needed. */
+int32_t vpmx_define_sizes(const vpmx_params *p, vpmx_geom *g, const double *xp, int64_t nvr, int32_t ntime);
+
+/* ------------------------------------------------------------------ context */
+int32_t vpmx_create(vpmx_ctx **out, int32_t device);
+void vpmx_destroy(vpmx_ctx *ctx);
+const char *vpmx_last_error(const vpmx_ctx *ctx);
+int32_t vpmx_set_params(vpmx_ctx *ctx, const vpmx_params *p);
+int32_t vpmx_get_geometry(const vpmx_ctx *ctx, vpmx_geom *out);
+/* precision: 0 = plain FP64 form.  Other values return VPMX_ERR_UNSUP. */
+int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
+
+/* vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...) (vpm.f90:54-475) on host arrays.
+ * WhatToDo 0: geometry (define_sizes); 2: project, yaps solve, velocity + stretching,
+ * interpolate (UP, GP overwritten); 1: project + solve + velocity (grids kept);
+ * 3: project + solve.  Other modes return VPMX_ERR_UNSUP. */
+int32_t vpmx_vpm(vpmx_ctx *ctx, double *xp, double *qp, double *up, double *gp, int64_t nvr, int32_t neqpm,
+                 int32_t whattodo, int32_t ntime, double ni);
+
+/* remesh_particles_3d(iflag=1) (vpm_remesh.f90:5-225), ncell_rem=1, two-phase so that
+ * the caller owns the reallocation of XPR/QPR: _count projects and compacts on the
+ * device and returns the new count, _fetch copies the particles out (k->j->i node order). */
+int32_t vpmx_remesh_count(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
+                          int32_t mrem, int64_t *nvr_out);
+int32_t vpmx_remesh_fetch(vpmx_ctx *ctx, double *xp_out, double *qp_out);
+
+/* Grids of the last mode-1/2/3 call, in the reference layout.
+ * which: 0 RHS_pm(neqpm+1,N) after project_vol3d, 1 SOL_pm(neqpm,N) after the yaps solve
+ * (stream function), 2 SOL_pm after calc_velocity_serial_3d (stretching),
+ * 3 velvrx_pm,velvry_pm,velvrz_pm (3 consecutive N grids). */
+int32_t vpmx_get_grid(vpmx_ctx *ctx, int32_t which, double *dst, int64_t capacity);
+
+/* ------------------------------------------------ device-resident particle state */
+/* Particles stay in HBM between calls; the step API runs the application's step
+ * (test.f90:83-150 shape) without host round trips. */
+int32_t vpmx_particles_upload(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t nvr, int32_t neqpm);
+int32_t vpmx_particles_count(const vpmx_ctx *ctx, int64_t *nvr);
+int32_t vpmx_particles_download(vpmx_ctx *ctx, double *xp, double *qp, double *up, double *gp);
+/* vpm(WhatToDo) on the resident particles (modes 0..3 as vpmx_vpm). */
+int32_t vpmx_vpm_resident(vpmx_ctx *ctx, int32_t whattodo, int32_t ntime);
+/* remesh_particles_3d(1) on the resident particles (replaces them). */
+int32_t vpmx_remesh_resident(vpmx_ctx *ctx, int32_t mrem);
+/* One RK2 (midpoint) step: 2 x vpm(2) + updates x += (u+uinf) dt, QP(1:3) -= GP dt,
+ * then vpm(0) and remesh_particles_3d(1) when do_remesh != 0. */
+int32_t vpmx_step_rk2(vpmx_ctx *ctx, double dt, const double uinf[3], int32_t ntime, int32_t mrem,
+                      int32_t do_remesh);
+/* FP64 diagnostics of the resident particles: circ = -sum QP(1:3),
+ * imp = 1/2 sum x x (-QP(1:3)). */
+int32_t vpmx_diagnostics(vpmx_ctx *ctx, double circ[3], double imp[3]);
+
+/* ----------------------------------------------------- stage entry points
+ * Single routines of the path on host arrays in the reference layout, for parity
+ * testing of each kernel in isolation. */
+/* solve_eq_3d / solve_eq_0_3d (pmsolve.f90:284-432) on window win (1-based, NN_bl
+ * order) of a grid (nn, xbound, dpm) with nc components: zero_bc=1 writes the zero-BC
+ * solution into out, else solves in place in sol with its window faces as BCs. */
+int32_t vpmx_stage_solve(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                         int32_t nc, const int32_t win[6], const double *rhs, double *sol, int32_t zero_bc,
+                         double *out);
+/* pmesh(iynbc=1, ibctyp=2, direct sums) (pmlib.f90:69-192): sol += boundary sums, then
+ * the Dirichlet solve over the whole grid. */
+int32_t vpmx_stage_pmesh_inf(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                             int32_t nc, const double *rhs, double *sol);
+/* project_particles_3D + project_vol3d (pmproject.f90:43-153) on grid (nn,xbound,dpm). */
+int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                           const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr,
+                           int32_t ncomp, int32_t itype, int32_t idvpm, double *rhs);
+
+/* Timing of the last vpm/step call per stage (ms), names as the reference's timers. */
+int32_t vpmx_stage_times(const vpmx_ctx *ctx, double *ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

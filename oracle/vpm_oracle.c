@@ -1,0 +1,950 @@
+/*
+ * vpm_oracle.c -- CPU restatement of the reference VPM path (papis84/vpm_free),
+ * FP64 throughout, single yaps block (BLOCKS = NBI*NBJ*NBK = 1).
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker and CPU baseline for the HIP library.
+ * The product (vpm_free_amd/) never links, loads or calls this file.
+ *
+ * Every routine follows the reference routine cited beside it, operation for
+ * operation where the floating-point order matters.  The MKL Poisson Library
+ * (d_*_Helmholtz_3D, q=0, 'DDDDDD'; Intel MKL, version not pinned by the reference,
+ * 2021.4.0 in this image) is restated by its published algorithm: sine transforms
+ * along x and y, then a tridiagonal (Thomas) solve along z, of the standard
+ * 7-point system -(u_{i-1}-2u_i+u_{i+1})/hx^2 - ... = f with Dirichlet values.
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   orc_projection_fun   closed forms, partition of unity, moments; reference projlib (oracle/_ref)
+ *   orc_project          reference projlib project_particles_3D / project_vol3d (oracle/_ref)
+ *   orc_definepm / orc_define_sizes   hand-derived sizes for the shipped pm.inp and ring boxes,
+ *                        divisibility/containment invariants
+ *   orc_dst1, orc_solve_window   scipy.fft.dst type 1, sine eigenmodes, residual of the 7-point system
+ *   orc_normalderiv      exact on quartic fields (5-point one-sided stencil), face/centroid layout
+ *   orc_boundsum         closed-form single-source values; free-space potential of a Gaussian blob
+ *   orc_pmesh_inf / orc_yaps3d   free-space potential of a Gaussian blob (discretisation-level);
+ *                        exact invariance when the coarse correction vanishes
+ *   orc_velocity         exact curl/stretching of polynomial fields
+ *   orc_m2p              exact interpolation of quadratics (M4'), brute force
+ *   orc_remesh           brute-force threshold scan, circulation/impulse conservation
+ *   orc_vpm2             Hill's spherical vortex closed form (test_applic.f90:2-77)
+ */
+#include "vpm_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IX(c, i, j, k, nc, nx, ny) \
+    ((size_t)(c) + (size_t)(nc) * ((size_t)(i) + (size_t)(nx) * ((size_t)(j) + (size_t)(ny) * (size_t)(k))))
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------ pm.inp */
+/* vpm.f90:132-152: positional free-format read, one record per line. */
+int orc_read_pm_inp(const char *path, orc_params *p) {
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    char line[512];
+    memset(p, 0, sizeof(*p));
+#define NEXT() do { if (!fgets(line, sizeof line, f)) { fclose(f); return -2; } } while (0)
+    NEXT(); if (sscanf(line, "%lf %lf %lf", &p->dpm[0], &p->dpm[1], &p->dpm[2]) != 3) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->interf_iproj) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->ibctyp) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->idvpm) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%lf", &p->epsvol) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->ncoarse) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d %d %d", &p->nbi, &p->nbj, &p->nbk) != 3) goto bad;
+    NEXT(); if (sscanf(line, "%d %d", &p->nremesh, &p->ncell_rem) != 2) goto bad;
+    NEXT(); if (sscanf(line, "%d %d", &p->iyntree, &p->ilevmax) != 2) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->ompthreads) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->idefine) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->iynslice) != 1) goto bad;
+    NEXT(); if (sscanf(line, "%d", &p->ipmwrite) != 1) goto bad;
+    if (p->ipmwrite > 10) { fclose(f); return -3; } /* vpm.f90:146 */
+    for (int i = 0; i < p->ipmwrite; i++) {
+        NEXT(); if (sscanf(line, "%d %d", &p->ipmwstart[i], &p->ipmwsteps[i]) != 2) goto bad;
+    }
+#undef NEXT
+    fclose(f);
+    return 0;
+bad:
+    fclose(f);
+    return -2;
+}
+
+/* ------------------------------------------------------------ projection_fun */
+/* pmproject.f90:391-431 */
+double orc_projection_fun(int itype, double x) {
+    double xabs = fabs(x), r = 0.0;
+    if (itype == 2) {
+        r = (xabs > 1) ? 0.0 : 1.0 - xabs;
+    } else if (itype == 3) {
+        if (xabs > 3.0 / 2.0) r = 0.0;
+        if (xabs >= 0.5 && xabs <= 3.0 / 2.0) r = 0.5 * (-xabs + 3.0 / 2.0) * (-xabs + 3.0 / 2.0);
+        if (xabs >= 0.0 && xabs < 0.5)
+            r = 0.5 * (xabs + 3.0 / 2.0) * (xabs + 3.0 / 2.0) - 3.0 / 2.0 * (xabs + 0.5) * (xabs + 0.5);
+    } else if (itype == 4) {
+        if (xabs > 2.0) r = 0.0;
+        if (xabs >= 1.0 && xabs <= 2.0) r = 0.5 * (2.0 - xabs) * (2.0 - xabs) * (1.0 - xabs);
+        if (xabs < 1.0) r = 1.0 - 2.5 * (xabs * xabs) + 1.5 * (xabs * xabs * xabs);
+    } else if (itype == 5) {
+        if (xabs <= 0.5) r = 1.0 - xabs * xabs;
+        if (xabs > 0.5 && xabs <= 1.5) r = 0.5 * (1.0 - xabs) * (2.0 - xabs);
+        if (xabs > 1.5) r = 0.0;
+    } else if (itype == 6) {
+        if (xabs <= 1.0) r = 1.0 - xabs * xabs;
+        if (xabs > 1.0 && xabs <= 2.0) r = 0.5 * (1.0 - xabs) * (2.0 - xabs);
+        if (xabs > 2.0) r = 0.0;
+    } else {
+        r = NAN; /* 'No such projection function' STOP, pmproject.f90:426-427 */
+    }
+    return r;
+}
+
+/* -------------------------------------------------------------------- definepm */
+/* pmlib.f90:216-432 (ND=3). */
+void orc_definepm(int itype, double xb[6], double dpm[3], int ndum, const int32_t nsize[3], int32_t nn[3],
+                  int32_t nn_bl[6]) {
+    int ndum_new[3] = {ndum, ndum, ndum};
+    if (itype == 1) {
+        for (int d = 0; d < 3; d++) {
+            xb[d] = xb[d] - ((double)ndum_new[d] * dpm[d]);
+            xb[d + 3] = xb[d + 3] + ((double)ndum_new[d] * dpm[d]);
+        }
+        for (int d = 0; d < 3; d++) nn[d] = (int32_t)lround(fabs(xb[d + 3] - xb[d]) / dpm[d]) + 1;
+    } else if (itype == 3) {
+        for (int d = 0; d < 3; d++) {
+            xb[d] = xb[d] - ((double)ndum_new[d] * dpm[d]);
+            xb[d + 3] = xb[d + 3] + ((double)ndum_new[d] * dpm[d]);
+        }
+        for (int d = 0; d < 3; d++) nn[d] = (int32_t)lround(fabs(xb[d + 3] - xb[d]) / dpm[d]) + 1;
+        for (int d = 0; d < 3; d++) ndum_new[d] = nsize[d] - ((nn[d] - 1) % nsize[d]);
+        for (int d = 0; d < 3; d++) {
+            if (ndum_new[d] % 2 == 0) {
+                ndum_new[d] = ndum_new[d] / 2;
+                xb[d] = xb[d] - ((double)ndum_new[d] * dpm[d]);
+                xb[d + 3] = xb[d + 3] + ((double)ndum_new[d] * dpm[d]);
+                ndum_new[d] = ndum_new[d] + ndum;
+            } else {
+                int nn1 = ndum_new[d] % 2;
+                int nn2 = ndum_new[d] / 2;
+                xb[d] = xb[d] - ((double)(nn1 + nn2) * dpm[d]);
+                xb[d + 3] = xb[d + 3] + ((double)nn2 * dpm[d]);
+            }
+            nn[d] = (int32_t)lround(fabs(xb[d + 3] - xb[d]) / dpm[d]) + 1;
+            nn_bl[d] = ndum + 1;
+            nn_bl[d + 3] = nn[d] - ndum;
+        }
+        return;
+    } else {
+        return; /* itype 2/4 are not reached by define_sizes in 3-D */
+    }
+    for (int d = 0; d < 3; d++) {
+        nn_bl[d] = ndum_new[d] + 1;
+        nn_bl[d + 3] = nn[d] - ndum_new[d];
+    }
+}
+
+/* ---------------------------------------------------------------- define_sizes */
+/* vpm.f90:477-660 with BLOCKS = NBI*NBJ*NBK (the reference sets BLOCKS = MPI ranks). */
+int orc_define_sizes(const orc_params *p, orc_geom *g, const double *xp, int64_t nvr, int ntime) {
+    int nb = p->nbi * p->nbj * p->nbk;
+    if (nb < 1 || nb > ORC_MAXB) return -1;
+    int redefine = (ntime == 0 && p->idefine == 1) || p->idefine == 0 || !g->defined;
+    double xb[6], dpm[3];
+    if (redefine) {
+        if (nvr <= 0) return -2;
+        double mn[3] = {xp[0], xp[1], xp[2]}, mx[3] = {xp[0], xp[1], xp[2]};
+        for (int64_t n = 1; n < nvr; n++)
+            for (int d = 0; d < 3; d++) {
+                double v = xp[3 * n + d];
+                if (v < mn[d]) mn[d] = v;
+                if (v > mx[d]) mx[d] = v;
+            }
+        for (int d = 0; d < 3; d++) {
+            xb[d] = mn[d] - (double)p->interf_iproj * p->dpm[d];
+            xb[d + 3] = mx[d] + (double)p->interf_iproj * p->dpm[d];
+            dpm[d] = p->dpm[d];
+        }
+        int32_t nsiz[3] = {p->nbi * p->ncoarse, p->nbj * p->ncoarse, p->nbk * p->ncoarse};
+        orc_definepm(3, xb, dpm, 0, nsiz, g->nn, g->nn_bl);
+        memcpy(g->xbound, xb, sizeof xb);
+        memcpy(g->dpm, dpm, sizeof dpm);
+        g->defined = 1;
+    }
+    memcpy(xb, g->xbound, sizeof xb);
+    memcpy(dpm, g->dpm, sizeof dpm);
+    g->nblocks = nb;
+    g->nbi = p->nbi; g->nbj = p->nbj; g->nbk = p->nbk;
+    int nxb = (int)lround((xb[3] - xb[0]) / dpm[0]);
+    int nyb = (int)lround((xb[4] - xb[1]) / dpm[1]);
+    int nzb = (int)lround((xb[5] - xb[2]) / dpm[2]);
+    int nxbl = nxb / p->nbi, nybl = nyb / p->nbj, nzbl = nzb / p->nbk;
+    int32_t nsiz_bl[3] = {p->ncoarse, p->ncoarse, p->ncoarse};
+    for (int k = 1; k <= p->nbk; k++)
+        for (int j = 1; j <= p->nbj; j++)
+            for (int i = 1; i <= p->nbi; i++) {
+                int b = (k - 1) * p->nbj * p->nbi + (j - 1) * p->nbi + i - 1;
+                double t[6];
+                t[0] = xb[0] + (double)((i - 1) * nxbl) * dpm[0];
+                t[3] = xb[0] + (double)(i * nxbl) * dpm[0];
+                t[1] = xb[1] + (double)((j - 1) * nybl) * dpm[1];
+                t[4] = xb[1] + (double)(j * nybl) * dpm[1];
+                t[2] = xb[2] + (double)((k - 1) * nzbl) * dpm[2];
+                t[5] = xb[2] + (double)(k * nzbl) * dpm[2];
+                double d2[3] = {dpm[0], dpm[1], dpm[2]};
+                orc_definepm(1, t, d2, p->ncoarse, nsiz_bl, &g->nnbl[3 * b], &g->nnbl_bl[6 * b]);
+                memcpy(&g->xbound_bl[6 * b], t, sizeof t);
+            }
+    memcpy(g->xbound_coarse, xb, sizeof xb);
+    for (int d = 0; d < 3; d++) g->dpm_coarse[d] = (double)p->ncoarse * dpm[d];
+    int32_t nsc[3];
+    for (int d = 0; d < 3; d++) nsc[d] = 1 << p->ilevmax;
+    orc_definepm(1, g->xbound_coarse, g->dpm_coarse, 4, nsc, g->nn_coarse, g->nn_bl_coarse);
+    g->ilevmax = p->ilevmax;
+    for (int lev = 0; lev <= p->ilevmax; lev++) {
+        int istep = 1 << lev;
+        if ((g->nnbl_bl[3] - g->nnbl_bl[0]) / istep == 0 || (g->nnbl_bl[4] - g->nnbl_bl[1]) / istep == 0 ||
+            (g->nnbl_bl[5] - g->nnbl_bl[2]) / istep == 0) {
+            g->ilevmax = lev - 1;
+            break;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------- FFT / DST */
+typedef struct {
+    int n, nf, fac[40];
+    double complex *tw;
+} fftplan;
+
+static void plan_init(fftplan *P, int n) {
+    P->n = n;
+    P->nf = 0;
+    int r = n;
+    while (r % 4 == 0) { P->fac[P->nf++] = 4; r /= 4; }
+    while (r % 2 == 0) { P->fac[P->nf++] = 2; r /= 2; }
+    for (int q = 3; r > 1;) {
+        if (r % q == 0) { P->fac[P->nf++] = q; r /= q; }
+        else q += 2;
+    }
+    if (P->nf == 0) P->fac[P->nf++] = 1;
+    P->tw = (double complex *)malloc(sizeof(double complex) * (size_t)n);
+    for (int t = 0; t < n; t++) {
+        double a = 2.0 * M_PI * (double)t / (double)n;
+        P->tw[t] = cos(a) - I * sin(a);
+    }
+}
+static void plan_free(fftplan *P) { free(P->tw); }
+
+/* Mixed-radix decimation-in-time: out[k] = sum_j in[j*istride] w_n^{jk}, w_n = exp(-2 pi i/n). */
+static void fft_rec(const fftplan *P, double complex *out, const double complex *in, size_t istride, int n,
+                    const int *fac, int twmul, double complex *scratch) {
+    int p = fac[0], m = n / p, N = P->n;
+    if (m == 1) {
+        for (int q = 0; q < p; q++) out[q] = in[(size_t)q * istride];
+    } else {
+        for (int q = 0; q < p; q++)
+            fft_rec(P, out + (size_t)q * m, in + (size_t)q * istride, istride * (size_t)p, m, fac + 1, twmul * p,
+                    scratch);
+    }
+    for (int s = 0; s < m; s++) {
+        for (int q = 0; q < p; q++) scratch[q] = out[(size_t)q * m + s] * P->tw[((long)q * s * twmul) % N];
+        for (int k = 0; k < p; k++) {
+            double complex acc = 0;
+            for (int q = 0; q < p; q++) acc += scratch[q] * P->tw[((long)q * k * (N / p)) % N];
+            out[(size_t)k * m + s] = acc;
+        }
+    }
+}
+
+/* DST-I of two real lines at once (inputs strided, outputs contiguous) through one complex FFT of the odd extension
+ * (length L = 2(m+1)): S_k = sum_{j=1..m} x_j sin(pi j k/(m+1)). */
+static void dst_pair(const fftplan *P, int m, const double *a, size_t sa, const double *b, size_t sb, double *oa,
+                     double *ob, double complex *zin, double complex *zout, double complex *scr) {
+    int L = 2 * (m + 1);
+    zin[0] = 0;
+    zin[m + 1] = 0;
+    for (int j = 1; j <= m; j++) {
+        double complex v = a[(size_t)(j - 1) * sa] + I * (b ? b[(size_t)(j - 1) * sb] : 0.0);
+        zin[j] = v;
+        zin[L - j] = -v;
+    }
+    fft_rec(P, zout, zin, 1, L, P->fac, 1, scr);
+    for (int k = 1; k <= m; k++) {
+        oa[k - 1] = -cimag(zout[k]) * 0.5;
+        if (b) ob[k - 1] = creal(zout[k]) * 0.5;
+    }
+}
+
+void orc_dst1(int m, const double *in, double *out) {
+    fftplan P;
+    plan_init(&P, 2 * (m + 1));
+    int L = 2 * (m + 1);
+    double complex *zi = malloc(sizeof(double complex) * L), *zo = malloc(sizeof(double complex) * L),
+                   *sc = malloc(sizeof(double complex) * L);
+    double *tmp = malloc(sizeof(double) * m);
+    memcpy(tmp, in, sizeof(double) * m);
+    dst_pair(&P, m, tmp, 1, NULL, 0, out, NULL, zi, zo, sc);
+    free(zi); free(zo); free(sc); free(tmp);
+    plan_free(&P);
+}
+
+/* In-place DST-I along one axis of w (mx,my,mz x-fastest); scale applied to results. */
+static void dst_axis(double *w, int mx, int my, int mz, int axis, double scale) {
+    int m = axis == 0 ? mx : (axis == 1 ? my : mz);
+    size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)mx : (size_t)mx * my);
+    int nl1 = axis == 0 ? my : mx, nl2 = axis == 2 ? my : mz;
+    size_t s1 = axis == 0 ? (size_t)mx : 1, s2 = axis == 2 ? (size_t)mx : (size_t)mx * my;
+    long nlines = (long)nl1 * nl2;
+    fftplan P;
+    plan_init(&P, 2 * (m + 1));
+    int L = 2 * (m + 1);
+#pragma omp parallel
+    {
+        double complex *zi = malloc(sizeof(double complex) * L), *zo = malloc(sizeof(double complex) * L),
+                       *sc = malloc(sizeof(double complex) * L);
+        double *la = malloc(sizeof(double) * m), *lb = malloc(sizeof(double) * m);
+#pragma omp for schedule(static)
+        for (long pr = 0; pr < (nlines + 1) / 2; pr++) {
+            long l0 = 2 * pr, l1 = 2 * pr + 1;
+            double *a = w + (size_t)(l0 % nl1) * s1 + (size_t)(l0 / nl1) * s2;
+            double *b = l1 < nlines ? w + (size_t)(l1 % nl1) * s1 + (size_t)(l1 / nl1) * s2 : NULL;
+            dst_pair(&P, m, a, stride, b, stride, la, lb, zi, zo, sc);
+            for (int k = 0; k < m; k++) a[(size_t)k * stride] = la[k] * scale;
+            if (b)
+                for (int k = 0; k < m; k++) b[(size_t)k * stride] = lb[k] * scale;
+        }
+        free(zi); free(zo); free(sc); free(la); free(lb);
+    }
+    plan_free(&P);
+}
+
+/* ------------------------------------------------------ Dirichlet Poisson solve */
+/* pmsolve.f90:284-432 with MKL's d_Helmholtz_3D(q=0,'DDDDDD') restated. */
+void orc_solve_window(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int comp,
+                      const int32_t win[6], const double *rhs, double *sol, int zero_bc, double *out) {
+    int s[3] = {win[0], win[1], win[2]}, f[3] = {win[3], win[4], win[5]};
+    int n[3], m[3];
+    double h[3];
+    for (int d = 0; d < 3; d++) {
+        double mn = xb[d] + (double)(s[d] - 1) * dpm[d];
+        double mx = xb[d] + (double)(f[d] - 1) * dpm[d];
+        n[d] = f[d] - s[d] + 1;
+        m[d] = n[d] - 2;
+        h[d] = (mx - mn) / (double)(n[d] - 1); /* MKL: h = (b-a)/n_intervals */
+    }
+    const double *bc = zero_bc ? NULL : sol;
+    double *dst = zero_bc ? out : sol;
+    int NX = nn[0], NY = nn[1];
+#define G(i, j, k) IX(comp, (s[0] - 1) + (i), (s[1] - 1) + (j), (s[2] - 1) + (k), nc, NX, NY)
+    size_t mt = (size_t)m[0] * m[1] * m[2];
+    double *w = malloc(sizeof(double) * (mt ? mt : 1));
+    double ih2[3] = {1.0 / (h[0] * h[0]), 1.0 / (h[1] * h[1]), 1.0 / (h[2] * h[2])};
+    /* f = -RHS; boundary values moved to the right-hand side (interior node equations) */
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= m[2]; k++)
+        for (int j = 1; j <= m[1]; j++)
+            for (int i = 1; i <= m[0]; i++) {
+                double v = rhs[G(i, j, k)];
+                if (bc) {
+                    if (i == 1) v -= bc[G(0, j, k)] * ih2[0];
+                    if (i == m[0]) v -= bc[G(m[0] + 1, j, k)] * ih2[0];
+                    if (j == 1) v -= bc[G(i, 0, k)] * ih2[1];
+                    if (j == m[1]) v -= bc[G(i, m[1] + 1, k)] * ih2[1];
+                    if (k == 1) v -= bc[G(i, j, 0)] * ih2[2];
+                    if (k == m[2]) v -= bc[G(i, j, m[2] + 1)] * ih2[2];
+                }
+                w[(size_t)(i - 1) + (size_t)m[0] * ((size_t)(j - 1) + (size_t)m[1] * (k - 1))] = v;
+            }
+    dst_axis(w, m[0], m[1], m[2], 0, 1.0);
+    dst_axis(w, m[0], m[1], m[2], 1, 1.0);
+    /* tridiagonal along z for each (kx, ky): (u_{k-1}-2u_k+u_{k+1})/hz^2 - (lx+ly) u_k = r_k */
+#pragma omp parallel
+    {
+        double *cp = malloc(sizeof(double) * (m[2] + 1)), *dp = malloc(sizeof(double) * (m[2] + 1));
+#pragma omp for schedule(static)
+        for (int ky = 1; ky <= m[1]; ky++)
+            for (int kx = 1; kx <= m[0]; kx++) {
+                double sx = sin(M_PI * kx / (2.0 * (m[0] + 1))), sy = sin(M_PI * ky / (2.0 * (m[1] + 1)));
+                double mu = 4.0 * sx * sx * ih2[0] + 4.0 * sy * sy * ih2[1];
+                double a = ih2[2], c = ih2[2], b = -2.0 * ih2[2] - mu;
+                double *col = w + (size_t)(kx - 1) + (size_t)m[0] * (ky - 1);
+                size_t st = (size_t)m[0] * m[1];
+                cp[0] = c / b;
+                dp[0] = col[0] / b;
+                for (int k = 1; k < m[2]; k++) {
+                    double den = b - a * cp[k - 1];
+                    cp[k] = c / den;
+                    dp[k] = (col[k * st] - a * dp[k - 1]) / den;
+                }
+                col[(size_t)(m[2] - 1) * st] = dp[m[2] - 1];
+                for (int k = m[2] - 2; k >= 0; k--) col[k * st] = dp[k] - cp[k] * col[(k + 1) * st];
+            }
+        free(cp); free(dp);
+    }
+    dst_axis(w, m[0], m[1], m[2], 1, 2.0 / (m[1] + 1));
+    dst_axis(w, m[0], m[1], m[2], 0, 2.0 / (m[0] + 1));
+    /* write the whole window: interior from the solve, boundary = Dirichlet values */
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < n[2]; k++)
+        for (int j = 0; j < n[1]; j++)
+            for (int i = 0; i < n[0]; i++) {
+                int bnd = (i == 0 || j == 0 || k == 0 || i == n[0] - 1 || j == n[1] - 1 || k == n[2] - 1);
+                double v;
+                if (bnd) v = bc ? bc[G(i, j, k)] : 0.0;
+                else v = w[(size_t)(i - 1) + (size_t)m[0] * ((size_t)(j - 1) + (size_t)m[1] * (k - 1))];
+                dst[G(i, j, k)] = v;
+            }
+#undef G
+    free(w);
+}
+
+/* ------------------------------------------------------- normal derivatives */
+/* calc_normalderiv_3d, pinfdomain.f90:294-547; sources at face-cell centroids. */
+int64_t orc_normalderiv(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, const double *s0,
+                        double *src, double *cxyz, double *ds) {
+    const double a1 = 25.0 / 12.0, a2 = -4.0, a3 = 3.0, a4 = -4.0 / 3.0, a5 = 0.25;
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    int64_t nb = 0;
+    double DX = dpm[0], DY = dpm[1], DZ = dpm[2];
+#define S(c, i, j, k) s0[IX(c, (i)-1, (j)-1, (k)-1, nc, NX, NY)]
+#define XN(i) (xb[0] + (double)((i)-1) * DX)
+#define YN(j) (xb[1] + (double)((j)-1) * DY)
+#define ZN(k) (xb[2] + (double)((k)-1) * DZ)
+#define C4(a, b, c, d) (0.25 * ((((a) + (b)) + (c)) + (d)))
+    /* stencil along x from i in direction sgn */
+#define DX5(c, i, j, k, sg) \
+    ((1.0 / DX) * ((((a1 * S(c, i, j, k) + a2 * S(c, (i) + (sg), j, k)) + a3 * S(c, (i) + 2 * (sg), j, k)) + \
+                    a4 * S(c, (i) + 3 * (sg), j, k)) + a5 * S(c, (i) + 4 * (sg), j, k)))
+#define DY5(c, i, j, k, sg) \
+    ((1.0 / DY) * ((((a1 * S(c, i, j, k) + a2 * S(c, i, (j) + (sg), k)) + a3 * S(c, i, (j) + 2 * (sg), k)) + \
+                    a4 * S(c, i, (j) + 3 * (sg), k)) + a5 * S(c, i, (j) + 4 * (sg), k)))
+#define DZ5(c, i, j, k, sg) \
+    ((1.0 / DZ) * ((((a1 * S(c, i, j, k) + a2 * S(c, i, j, (k) + (sg))) + a3 * S(c, i, j, (k) + 2 * (sg))) + \
+                    a4 * S(c, i, j, (k) + 3 * (sg))) + a5 * S(c, i, j, (k) + 4 * (sg))))
+    for (int face = 0; face < 2; face++) { /* XMIN, XMAX */
+        int i = face == 0 ? 1 : NX, sg = face == 0 ? 1 : -1;
+        for (int k = 1; k <= NZ - 1; k++)
+            for (int j = 1; j <= NY - 1; j++) {
+                for (int c = 0; c < nc; c++) {
+                    double p1 = DX5(c, i, j, k, sg), p2 = DX5(c, i, j + 1, k, sg);
+                    double p3 = DX5(c, i, j + 1, k + 1, sg), p4 = DX5(c, i, j, k + 1, sg);
+                    src[(size_t)nc * nb + c] = 0.25 * (((p1 + p2) + p3) + p4);
+                }
+                double X = XN(i);
+                cxyz[3 * nb + 0] = C4(X, X, X, X);
+                if (face == 0) {
+                    cxyz[3 * nb + 1] = C4(YN(j), YN(j), YN(j + 1), YN(j + 1));
+                    cxyz[3 * nb + 2] = C4(ZN(k), ZN(k + 1), ZN(k + 1), ZN(k));
+                } else {
+                    cxyz[3 * nb + 1] = C4(YN(j + 1), YN(j + 1), YN(j), YN(j));
+                    cxyz[3 * nb + 2] = C4(ZN(k), ZN(k + 1), ZN(k + 1), ZN(k));
+                }
+                ds[nb] = DY * DZ;
+                nb++;
+            }
+    }
+    for (int face = 0; face < 2; face++) { /* YMIN, YMAX */
+        int j = face == 0 ? 1 : NY, sg = face == 0 ? 1 : -1;
+        for (int k = 1; k <= NZ - 1; k++)
+            for (int i = 1; i <= NX - 1; i++) {
+                for (int c = 0; c < nc; c++) {
+                    double p1 = DY5(c, i, j, k, sg), p2 = DY5(c, i + 1, j, k, sg);
+                    double p3 = DY5(c, i + 1, j, k + 1, sg), p4 = DY5(c, i, j, k + 1, sg);
+                    src[(size_t)nc * nb + c] = 0.25 * (((p1 + p2) + p3) + p4);
+                }
+                double Y = YN(j);
+                cxyz[3 * nb + 1] = C4(Y, Y, Y, Y);
+                if (face == 0) {
+                    cxyz[3 * nb + 0] = C4(XN(i), XN(i), XN(i + 1), XN(i + 1));
+                    cxyz[3 * nb + 2] = C4(ZN(k), ZN(k + 1), ZN(k + 1), ZN(k));
+                } else {
+                    cxyz[3 * nb + 0] = C4(XN(i + 1), XN(i + 1), XN(i), XN(i));
+                    cxyz[3 * nb + 2] = C4(ZN(k), ZN(k + 1), ZN(k + 1), ZN(k));
+                }
+                ds[nb] = DX * DZ;
+                nb++;
+            }
+    }
+    for (int face = 0; face < 2; face++) { /* ZMIN, ZMAX */
+        int k = face == 0 ? 1 : NZ, sg = face == 0 ? 1 : -1;
+        for (int j = 1; j <= NY - 1; j++)
+            for (int i = 1; i <= NX - 1; i++) {
+                for (int c = 0; c < nc; c++) {
+                    double p1 = DZ5(c, i, j, k, sg), p2 = DZ5(c, i + 1, j, k, sg);
+                    double p3 = DZ5(c, i + 1, j + 1, k, sg), p4 = DZ5(c, i, j + 1, k, sg);
+                    src[(size_t)nc * nb + c] = 0.25 * (((p1 + p2) + p3) + p4);
+                }
+                double Z = ZN(k);
+                cxyz[3 * nb + 2] = C4(Z, Z, Z, Z);
+                if (face == 0) {
+                    cxyz[3 * nb + 0] = C4(XN(i), XN(i), XN(i + 1), XN(i + 1));
+                    cxyz[3 * nb + 1] = C4(YN(j), YN(j + 1), YN(j + 1), YN(j));
+                } else {
+                    cxyz[3 * nb + 0] = C4(XN(i + 1), XN(i + 1), XN(i), XN(i));
+                    cxyz[3 * nb + 1] = C4(YN(j), YN(j + 1), YN(j + 1), YN(j));
+                }
+                ds[nb] = DX * DY;
+                nb++;
+            }
+    }
+#undef S
+#undef DX5
+#undef DY5
+#undef DZ5
+    return nb;
+}
+
+/* -------------------------------------------------------------- boundary sums */
+/* Bounds3d (pmbound.f90:80-157) -> calc_boundinf_3d (pmbound.f90:415-515), ibctyp=2.
+ * Targets: X faces full, Y faces skip x-edges, Z faces skip x- and y-edges; per target the
+ * sources are accumulated in source order, as the reference's nv loop does. */
+void orc_boundsum(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                  const double *src, const double *cxyz, const double *ds, double *sol) {
+    const double PI = 4.0 * atan(1.0), PI4 = 4.0 * PI;
+    const double lg = log((sqrt(2.0) - 1) / (sqrt(2.0) + 1));
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    double DX = dpm[0], DY = dpm[1], DZ = dpm[2];
+    /* enumerate targets */
+    int64_t nt = 2LL * NY * NZ + 2LL * (NX - 2) * NZ + 2LL * (NX - 2) * (NY - 2);
+    int32_t *tg = malloc(sizeof(int32_t) * 4 * nt);
+    int64_t t = 0;
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int j = 1; j <= NY; j++) { tg[4*t] = f ? NX : 1; tg[4*t+1] = j; tg[4*t+2] = k; tg[4*t+3] = 2; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int i = 2; i <= NX - 1; i++) { tg[4*t] = i; tg[4*t+1] = f ? NY : 1; tg[4*t+2] = k; tg[4*t+3] = 1; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int j = 2; j <= NY - 1; j++)
+            for (int i = 2; i <= NX - 1; i++) { tg[4*t] = i; tg[4*t+1] = j; tg[4*t+2] = f ? NZ : 1; tg[4*t+3] = 3; t++; }
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < nt; q++) {
+        int i = tg[4 * q], j = tg[4 * q + 1], k = tg[4 * q + 2], pl = tg[4 * q + 3];
+        double X = xb[0] + (double)(i - 1) * DX, Y = xb[1] + (double)(j - 1) * DY, Z = xb[2] + (double)(k - 1) * DZ;
+        double hs = pl == 2 ? (DY + DZ) : (pl == 1 ? (DX + DZ) : (DX + DY));
+        double acc[8];
+        for (int c = 0; c < nc; c++) acc[c] = sol[IX(c, i - 1, j - 1, k - 1, nc, NX, NY)];
+        for (int64_t nv = 0; nv < nbound; nv++) {
+            double XR = cxyz[3 * nv] - X, YR = cxyz[3 * nv + 1] - Y, ZR = cxyz[3 * nv + 2] - Z;
+            double r = sqrt(XR * XR + YR * YR + ZR * ZR);
+            double DS = ds[nv];
+            if (r > 1e-05) {
+                for (int c = 0; c < nc; c++) acc[c] = acc[c] - src[(size_t)nc * nv + c] * DS / (PI4 * r);
+            } else {
+                for (int c = 0; c < nc; c++) acc[c] = acc[c] + src[(size_t)nc * nv + c] * hs * lg / PI4;
+            }
+        }
+        for (int c = 0; c < nc; c++) sol[IX(c, i - 1, j - 1, k - 1, nc, NX, NY)] = acc[c];
+    }
+    free(tg);
+}
+
+/* ------------------------------------------------------------------ pmesh */
+/* pmlib.f90:69-192 with iynbc=1, ibctyp=2, itree=0: infdomain_3d (pinfdomain.f90:86-158)
+ * then solve_eq_3d on the whole grid.  sol accumulates the boundary sums. */
+void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, const double *rhs,
+                   double *sol) {
+    size_t G = (size_t)nn[0] * nn[1] * nn[2];
+    double *s0 = calloc(G * nc, sizeof(double));
+    int32_t win[6] = {1, 1, 1, nn[0], nn[1], nn[2]};
+    for (int c = 0; c < nc; c++) orc_solve_window(xb, dpm, nn, nc, c, win, rhs, NULL, 1, s0);
+    int64_t maxb = 2LL * nn[0] * nn[1] + 2LL * nn[0] * nn[2] + 2LL * nn[1] * nn[2];
+    double *src = malloc(sizeof(double) * nc * maxb), *cx = malloc(sizeof(double) * 3 * maxb),
+           *ds = malloc(sizeof(double) * maxb);
+    int64_t nb = orc_normalderiv(xb, dpm, nn, nc, s0, src, cx, ds);
+    free(s0);
+    orc_boundsum(xb, dpm, nn, nc, nb, src, cx, ds, sol);
+    free(src); free(cx); free(ds);
+    for (int c = 0; c < nc; c++) orc_solve_window(xb, dpm, nn, nc, c, win, rhs, sol, 0, NULL);
+}
+
+/* --------------------------------------------------------------------- yaps3d */
+/* yaps3d.f90:1-453 for BLOCKS=1 (block nb=1, nb_i=nb_j=nb_k=1; no neighbour exchange). */
+int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs_bl, double *sol_bl) {
+    if (g->nblocks != 1) return -1;
+    const double *xbl = &g->xbound_bl[0];
+    const int32_t *nnbl = &g->nnbl[0], *nnbl_bl = &g->nnbl_bl[0];
+    size_t B = (size_t)nnbl[0] * nnbl[1] * nnbl[2];
+    memset(sol_bl, 0, sizeof(double) * B * nc);
+    /* 1. fine infinite-domain solve on the whole block */
+    orc_pmesh_inf(xbl, g->dpm, nnbl, nc, rhs_bl, sol_bl);
+    /* 2. sample the fine solution on the coarse grid (yaps3d.f90:104-138) */
+    int ncx = g->nn_coarse[0], ncy = g->nn_coarse[1], ncz = g->nn_coarse[2];
+    size_t GC = (size_t)ncx * ncy * ncz;
+    double *samp = calloc(GC * nc, sizeof(double));
+    unsigned char *map = calloc(GC, 1);
+    int cmap[6] = {-1, -1, -1, -1, -1, -1};
+    int ncs = p->ncoarse;
+    for (int k = 1; k <= nnbl[2]; k += ncs)
+        for (int j = 1; j <= nnbl[1]; j += ncs)
+            for (int i = 1; i <= nnbl[0]; i += ncs) {
+                double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
+                double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
+                double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
+                int in = (int)lround((X0 - g->xbound_coarse[0]) / g->dpm_coarse[0]) + 1;
+                int jn = (int)lround((X1 - g->xbound_coarse[1]) / g->dpm_coarse[1]) + 1;
+                int kn = (int)lround((X2 - g->xbound_coarse[2]) / g->dpm_coarse[2]) + 1;
+                if (i == 1) cmap[0] = in;
+                if (i == nnbl[0]) cmap[3] = in;
+                if (j == 1) cmap[1] = jn;
+                if (j == nnbl[1]) cmap[4] = jn;
+                if (k == 1) cmap[2] = kn;
+                if (k == nnbl[2]) cmap[5] = kn;
+                if (in < 1 || in > ncx || jn < 1 || jn > ncy || kn < 1 || kn > ncz) continue;
+                for (int c = 0; c < nc; c++)
+                    samp[IX(c, in - 1, jn - 1, kn - 1, nc, ncx, ncy)] = sol_bl[IX(c, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1])];
+                map[(size_t)(in - 1) + (size_t)ncx * ((jn - 1) + (size_t)ncy * (kn - 1))] = 1;
+            }
+    /* 3. coarse RHS = 7-point Laplacian of the sample on the footprint interior (yaps3d.f90:1990-2016) */
+    double *rhs_c = calloc(GC * nc, sizeof(double)), *sol_c = calloc(GC * nc, sizeof(double));
+    const double *dc = g->dpm_coarse;
+#define SC(c, i, j, k) samp[IX(c, (i)-1, (j)-1, (k)-1, nc, ncx, ncy)]
+    for (int k = cmap[2] + 1; k <= cmap[5] - 1; k++)
+        for (int j = cmap[1] + 1; j <= cmap[4] - 1; j++)
+            for (int i = cmap[0] + 1; i <= cmap[3] - 1; i++)
+                for (int c = 0; c < nc; c++) {
+                    double v = ((SC(c, i + 1, j, k) - 2 * SC(c, i, j, k)) + SC(c, i - 1, j, k)) / (dc[0] * dc[0]) +
+                               ((SC(c, i, j + 1, k) - 2 * SC(c, i, j, k)) + SC(c, i, j - 1, k)) / (dc[1] * dc[1]) +
+                               ((SC(c, i, j, k + 1) - 2 * SC(c, i, j, k)) + SC(c, i, j, k - 1)) / (dc[2] * dc[2]);
+                    rhs_c[IX(c, i - 1, j - 1, k - 1, nc, ncx, ncy)] = 0.0 + v;
+                }
+    /* 4. coarse infinite-domain solve */
+    double xbc[6];
+    memcpy(xbc, g->xbound_coarse, sizeof xbc);
+    orc_pmesh_inf(xbc, g->dpm_coarse, g->nn_coarse, nc, rhs_c, sol_c);
+    /* 5. interp_stencil on the 6 faces of the inner box (yaps3d.f90:340-395, 1914-1983) */
+    int NXs = nnbl_bl[0], NYs = nnbl_bl[1], NZs = nnbl_bl[2], NXf = nnbl_bl[3], NYf = nnbl_bl[4], NZf = nnbl_bl[5];
+    int64_t nface = 2LL * (NYf - NYs + 1) * (NZf - NZs + 1) + 2LL * (NXf - NXs - 1) * (NZf - NZs + 1) +
+                    2LL * (NXf - NXs - 1) * (NYf - NYs - 1);
+    int32_t *fl = malloc(sizeof(int32_t) * 3 * nface);
+    int64_t q = 0;
+    for (int f = 0; f < 2; f++)
+        for (int k = NZs; k <= NZf; k++)
+            for (int j = NYs; j <= NYf; j++) { fl[3*q] = f ? NXf : NXs; fl[3*q+1] = j; fl[3*q+2] = k; q++; }
+    for (int f = 0; f < 2; f++)
+        for (int k = NZs; k <= NZf; k++)
+            for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = f ? NYf : NYs; fl[3*q+2] = k; q++; }
+    for (int f = 0; f < 2; f++)
+        for (int j = NYs + 1; j <= NYf - 1; j++)
+            for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = j; fl[3*q+2] = f ? NZf : NZs; q++; }
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < nface; t++) {
+        int i = fl[3 * t], j = fl[3 * t + 1], k = fl[3 * t + 2];
+        double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
+        double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
+        double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
+        int in = (int)lround((X0 - g->xbound_coarse[0]) / dc[0]) + 1;
+        int jn = (int)lround((X1 - g->xbound_coarse[1]) / dc[1]) + 1;
+        int kn = (int)lround((X2 - g->xbound_coarse[2]) / dc[2]) + 1;
+        int nnb = 0;
+        if (!map[(size_t)(in - 2) + (size_t)ncx * ((jn - 2) + (size_t)ncy * (kn - 2))] ||
+            !map[(size_t)(in) + (size_t)ncx * ((jn) + (size_t)ncy * (kn))])
+            nnb = 1;
+        size_t o = IX(0, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1]);
+        for (int kc = kn - 1; kc <= kn + 1; kc++)
+            for (int jc = jn - 1; jc <= jn + 1; jc++)
+                for (int ic = in - 1; ic <= in + 1; ic++) {
+                    double xc = (X0 - g->xbound_coarse[0] - (double)(ic - 1) * dc[0]) / dc[0];
+                    double yc = (X1 - g->xbound_coarse[1] - (double)(jc - 1) * dc[1]) / dc[1];
+                    double zc = (X2 - g->xbound_coarse[2] - (double)(kc - 1) * dc[2]) / dc[2];
+                    double fw = orc_projection_fun(3, xc) * orc_projection_fun(3, yc) * orc_projection_fun(3, zc);
+                    for (int c = 0; c < nc; c++) {
+                        double smp = SC(c, ic, jc, kc);
+                        double add_sample = smp + 0.0;
+                        double add = nnb ? smp + 0.0 : 0.0;
+                        sol_bl[o + c] = sol_bl[o + c] + fw * ((sol_c[IX(c, ic - 1, jc - 1, kc - 1, nc, ncx, ncy)] - add_sample) + add);
+                    }
+                }
+    }
+#undef SC
+    free(fl); free(samp); free(map); free(rhs_c); free(sol_c);
+    /* 6. final Dirichlet solve on the inner box */
+    for (int c = 0; c < nc; c++) orc_solve_window(xbl, g->dpm, nnbl, nc, c, nnbl_bl, rhs_bl, sol_bl, 0, NULL);
+    return 0;
+}
+
+/* ------------------------------------------------------- velocity, stretching */
+/* calc_velocity_serial_3d (vpm_gcalc.f90:1-136). vel grids are (NX,NY,NZ), 1-component. */
+void orc_velocity(const orc_geom *g, int neqpm, const double *rhs, double *sol, double *vx, double *vy,
+                  double *vz, int idcalc) {
+    int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
+    int xs = g->nn_bl[0], ys = g->nn_bl[1], zs = g->nn_bl[2], xf = g->nn_bl[3], yf = g->nn_bl[4], zf = g->nn_bl[5];
+    double DX2 = 2 * g->dpm[0], DY2 = 2 * g->dpm[1], DZ2 = 2 * g->dpm[2];
+    int nr = neqpm + 1;
+#define SL(c, i, j, k) sol[IX(c, (i)-1, (j)-1, (k)-1, neqpm, NX, NY)]
+#define RH(c, i, j, k) rhs[IX(c, (i)-1, (j)-1, (k)-1, nr, NX, NY)]
+#define V(a, i, j, k) a[IX(0, (i)-1, (j)-1, (k)-1, 1, NX, NY)]
+    if (idcalc >= 0) {
+#pragma omp parallel for schedule(static)
+        for (int k = zs + 1; k <= zf - 1; k++)
+            for (int j = ys + 1; j <= yf - 1; j++)
+                for (int i = xs + 1; i <= xf - 1; i++) {
+                    double dx[3], dy[3], dz[3];
+                    for (int c = 0; c < 3; c++) {
+                        dx[c] = (SL(c, i + 1, j, k) - SL(c, i - 1, j, k)) / DX2;
+                        dy[c] = (SL(c, i, j + 1, k) - SL(c, i, j - 1, k)) / DY2;
+                        dz[c] = (SL(c, i, j, k + 1) - SL(c, i, j, k - 1)) / DZ2;
+                    }
+                    V(vx, i, j, k) = V(vx, i, j, k) + dy[2] - dz[1];
+                    V(vy, i, j, k) = V(vy, i, j, k) - (dx[2] - dz[0]);
+                    V(vz, i, j, k) = V(vz, i, j, k) + dx[1] - dy[0];
+                }
+        if (neqpm == 4) {
+#pragma omp parallel for schedule(static)
+            for (int k = zs + 1; k <= zf - 1; k++)
+                for (int j = ys + 1; j <= yf - 1; j++)
+                    for (int i = xs + 1; i <= xf - 1; i++) {
+                        V(vx, i, j, k) = V(vx, i, j, k) + (SL(3, i + 1, j, k) - SL(3, i - 1, j, k)) / DX2;
+                        V(vy, i, j, k) = V(vy, i, j, k) + (SL(3, i, j + 1, k) - SL(3, i, j - 1, k)) / DY2;
+                        V(vz, i, j, k) = V(vz, i, j, k) + (SL(3, i, j, k + 1) - SL(3, i, j, k - 1)) / DZ2;
+                    }
+        }
+    }
+    if (idcalc == 0) return;
+    memset(sol, 0, sizeof(double) * (size_t)neqpm * NX * NY * NZ);
+#pragma omp parallel for schedule(static)
+    for (int k = zs + 2; k <= zf - 2; k++)
+        for (int j = ys + 2; j <= yf - 2; j++)
+            for (int i = xs + 2; i <= xf - 2; i++) {
+                const double *comp[3] = {vx, vy, vz};
+                for (int m = 0; m < 3; m++) {
+                    const double *u = comp[m];
+                    double pi = V(u, i + 1, j, k), pj = V(u, i, j + 1, k), pk = V(u, i, j, k + 1);
+                    double mi = V(u, i - 1, j, k), mj = V(u, i, j - 1, k), mk = V(u, i, j, k - 1);
+                    double a = -(RH(0, i + 1, j, k) * pi - (RH(0, i - 1, j, k)) * mi) / DX2;
+                    double b = -(RH(1, i, j + 1, k) * pj - (RH(1, i, j - 1, k)) * mj) / DY2;
+                    double c = -(RH(2, i, j, k + 1) * pk - (RH(2, i, j, k - 1)) * mk) / DZ2;
+                    SL(m, i, j, k) = a + b + c;
+                }
+            }
+#undef SL
+#undef RH
+#undef V
+}
+
+/* ------------------------------------------------------------- interpolation */
+/* back_to_particles_3D itype=1 (vpm_time.f90:30-107); UP overwritten (particles_gath copies
+ * the scattered UP, which starts from zero). Nodes outside the grid are skipped. */
+void orc_m2p(const orc_geom *g, int neqpm, int iproj, const double *sol, const double *vx, const double *vy,
+             const double *vz, const double *xp, const double *qp, double *up, double *gp, int64_t nvr) {
+    int ips = iproj == 2 ? 0 : 1, ipf = iproj == 2 ? 1 : 2;
+    int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
+    const double *Xb = g->xbound, *D = g->dpm;
+#pragma omp parallel for schedule(static)
+    for (int64_t nv = 0; nv < nvr; nv++) {
+        const double *x = xp + 3 * nv;
+        int in = (int)((x[0] - Xb[0]) / D[0]) + 1, jn = (int)((x[1] - Xb[1]) / D[1]) + 1,
+            kn = (int)((x[2] - Xb[2]) / D[2]) + 1;
+        double u[3] = {0, 0, 0}, gl[3] = {0, 0, 0};
+        for (int k = kn - ips; k <= kn + ipf; k++)
+            for (int j = jn - ips; j <= jn + ipf; j++)
+                for (int i = in - ips; i <= in + ipf; i++) {
+                    double fx = orc_projection_fun(iproj, (x[0] - Xb[0] - (double)(i - 1) * D[0]) / D[0]);
+                    double fy = orc_projection_fun(iproj, (x[1] - Xb[1] - (double)(j - 1) * D[1]) / D[1]);
+                    double fz = orc_projection_fun(iproj, (x[2] - Xb[2] - (double)(k - 1) * D[2]) / D[2]);
+                    double f = fx * fy * fz;
+                    if (i < 1 || i > NX || j < 1 || j > NY || k < 1 || k > NZ) continue;
+                    size_t o = IX(0, i - 1, j - 1, k - 1, 1, NX, NY);
+                    u[0] = u[0] + f * vx[o];
+                    u[1] = u[1] + f * vy[o];
+                    u[2] = u[2] + f * vz[o];
+                    for (int c = 0; c < 3; c++) gl[c] = gl[c] + f * sol[(size_t)neqpm * o + c];
+                }
+        for (int c = 0; c < 3; c++) {
+            up[3 * nv + c] = u[c];
+            gp[3 * nv + c] = gl[c] * qp[(size_t)(neqpm + 1) * nv + neqpm];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ projection */
+/* project_particles_3D (pmproject.f90:43-98) + project_vol3d (pmproject.f90:110-153).
+ * rhs has ncomp = neq+1 components; the last one carries the volume. */
+int orc_project(const double xb[6], const double dpm[3], const int32_t nn[3], const int32_t nn_bl[6],
+                const double *xp, const double *qp, int64_t nvr, int ncomp, int itype, int idvpm, double *rhs) {
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    int ips = itype == 2 ? 0 : 1, ipf = itype == 2 ? 1 : 2;
+    size_t G = (size_t)NX * NY * NZ;
+    memset(rhs, 0, sizeof(double) * G * ncomp);
+    int64_t outside = 0;
+    for (int64_t nv = 0; nv < nvr; nv++) {
+        const double *x = xp + 3 * nv;
+        const double *q = qp + (size_t)ncomp * nv;
+        int in = (int)((x[0] - xb[0]) / dpm[0]) + 1, jn = (int)((x[1] - xb[1]) / dpm[1]) + 1,
+            kn = (int)((x[2] - xb[2]) / dpm[2]) + 1;
+        for (int k = kn - ips; k <= kn + ipf; k++)
+            for (int j = jn - ips; j <= jn + ipf; j++)
+                for (int i = in - ips; i <= in + ipf; i++) {
+                    double fx = orc_projection_fun(itype, (x[0] - xb[0] - (double)(i - 1) * dpm[0]) / dpm[0]);
+                    double fy = orc_projection_fun(itype, (x[1] - xb[1] - (double)(j - 1) * dpm[1]) / dpm[1]);
+                    double fz = orc_projection_fun(itype, (x[2] - xb[2] - (double)(k - 1) * dpm[2]) / dpm[2]);
+                    double f = fx * fy * fz;
+                    if (i < 1 || i > NX || j < 1 || j > NY || k < 1 || k > NZ) { outside++; continue; }
+                    size_t o = IX(0, i - 1, j - 1, k - 1, ncomp, NX, NY);
+                    for (int c = 0; c < ncomp; c++) rhs[o + c] = rhs[o + c] + f * q[c];
+                }
+    }
+    double DV = dpm[0] * dpm[1] * dpm[2];
+    if (idvpm == 1) {
+        for (size_t n = 0; n < G; n++) {
+            for (int c = 0; c < ncomp - 1; c++) rhs[n * ncomp + c] = rhs[n * ncomp + c] / DV;
+            rhs[n * ncomp + ncomp - 1] = DV;
+        }
+    } else if (idvpm == 0) {
+        int NXs = nn_bl[0], NYs = nn_bl[1], NZs = nn_bl[2], NXf = nn_bl[3], NYf = nn_bl[4], NZf = nn_bl[5];
+        for (int k = 1; k <= NZ; k++)
+            for (int j = 1; j <= NY; j++)
+                for (int i = 1; i <= NX; i++) {
+                    size_t o = IX(0, i - 1, j - 1, k - 1, ncomp, NX, NY);
+                    /* the reference tests j.ge.NYf here (pmproject.f90:125) */
+                    if (i < NXs || i > NXf || j < NYs || j >= NYf || k < NZs || k > NZf) {
+                        for (int c = 0; c < ncomp - 1; c++) rhs[o + c] = 0.0;
+                        rhs[o + ncomp - 1] = DV;
+                        continue;
+                    }
+                    for (int c = 0; c < ncomp - 1; c++) rhs[o + c] = rhs[o + c] / rhs[o + ncomp - 1];
+                }
+    } else {
+        return -1; /* 'WRONG IDVPM' */
+    }
+    return (int)(outside > 0);
+}
+
+/* ------------------------------------------------------------------- vpm mode 2 */
+/* vpm.f90:171-313 (WhatToDo=2) on one block. */
+int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp, const double *qp, double *up,
+             double *gp, int64_t nvr, double *rhs_out, double *psi_out, double *sol_out, double *vel_out) {
+    if (nvr == 0) return 0;
+    if (g->nblocks != 1 || neqpm != 3) return -1;
+    int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
+    size_t G = (size_t)NX * NY * NZ;
+    int nr = neqpm + 1;
+    double *rhs = calloc(G * nr, sizeof(double));
+    double *sol = calloc(G * neqpm, sizeof(double));
+    orc_project(g->xbound, g->dpm, g->nn, g->nn_bl, xp, qp, nvr, nr, p->interf_iproj, p->idvpm, rhs);
+    /* rhsscat_3d (vpm_mpi.f90:199-238): block = global window + ncoarse halo of zeros */
+    const int32_t *nnbl = g->nnbl, *bl = g->nnbl_bl;
+    size_t B = (size_t)nnbl[0] * nnbl[1] * nnbl[2];
+    double *rhs_bl = calloc(B * neqpm, sizeof(double)), *sol_bl = calloc(B * neqpm, sizeof(double));
+    int ixs = g->nn_bl[0], jxs = g->nn_bl[1], kxs = g->nn_bl[2];
+    for (int k = bl[2]; k <= bl[5]; k++)
+        for (int j = bl[1]; j <= bl[4]; j++)
+            for (int i = bl[0]; i <= bl[3]; i++)
+                for (int c = 0; c < neqpm; c++)
+                    rhs_bl[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])] =
+                        rhs[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, nr, NX, NY)];
+    orc_yaps3d(p, g, neqpm, rhs_bl, sol_bl);
+    /* solget_3d (vpm_mpi.f90:240-328) */
+    for (int k = bl[2]; k <= bl[5]; k++)
+        for (int j = bl[1]; j <= bl[4]; j++)
+            for (int i = bl[0]; i <= bl[3]; i++)
+                for (int c = 0; c < neqpm; c++)
+                    sol[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, neqpm, NX, NY)] =
+                        sol_bl[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])];
+    free(rhs_bl); free(sol_bl);
+    if (psi_out) memcpy(psi_out, sol, sizeof(double) * G * neqpm);
+    double *vel = calloc(3 * G, sizeof(double));
+    orc_velocity(g, neqpm, rhs, sol, vel, vel + G, vel + 2 * G, 1);
+    orc_m2p(g, neqpm, p->interf_iproj, sol, vel, vel + G, vel + 2 * G, xp, qp, up, gp, nvr);
+    if (rhs_out) memcpy(rhs_out, rhs, sizeof(double) * G * nr);
+    if (sol_out) memcpy(sol_out, sol, sizeof(double) * G * neqpm);
+    if (vel_out) memcpy(vel_out, vel, sizeof(double) * 3 * G);
+    free(vel); free(rhs); free(sol);
+    return 0;
+}
+
+/* ------------------------------------------------------------------- remesh */
+/* remesh_particles_3d(iflag=1), ncell_rem=1 (vpm_remesh.f90:5-225). */
+int64_t orc_remesh(const orc_params *p, const orc_geom *g, int neqpm, int mrem, const double *xp, const double *qp,
+                   int64_t nvr, double *xp_out, double *qp_out) {
+    if (p->ncell_rem != 1) return -1;
+    int32_t NN[3], NNb[6];
+    double D[3];
+    for (int d = 0; d < 3; d++) NN[d] = g->nn[d] * mrem;
+    for (int d = 0; d < 6; d++) NNb[d] = g->nn_bl[d] * mrem;
+    for (int d = 0; d < 3; d++) D[d] = (g->xbound[d + 3] - g->xbound[d]) / (double)(NN[d] - 1);
+    double DV = D[0] * D[1] * D[2];
+    int nr = neqpm + 1;
+    size_t G = (size_t)NN[0] * NN[1] * NN[2];
+    double *rhs = malloc(sizeof(double) * G * nr);
+    orc_project(g->xbound, D, NN, NNb, xp, qp, nvr, nr, p->interf_iproj, 2 /* no vol yet */, rhs);
+    for (size_t n = 0; n < G; n++) rhs[n * nr + nr - 1] = DV; /* RHS_pm(neqpm+1,:,:,:)=DVpm */
+    /* project_vol3d(RHS_pm, ..., IDVPM) */
+    {
+        int NX = NN[0], NY = NN[1], NZ = NN[2];
+        if (p->idvpm == 1) {
+            for (size_t n = 0; n < G; n++) {
+                for (int c = 0; c < nr - 1; c++) rhs[n * nr + c] = rhs[n * nr + c] / DV;
+                rhs[n * nr + nr - 1] = DV;
+            }
+        } else {
+            for (int k = 1; k <= NZ; k++)
+                for (int j = 1; j <= NY; j++)
+                    for (int i = 1; i <= NX; i++) {
+                        size_t o = IX(0, i - 1, j - 1, k - 1, nr, NX, NY);
+                        if (i < NNb[0] || i > NNb[3] || j < NNb[1] || j >= NNb[4] || k < NNb[2] || k > NNb[5]) {
+                            for (int c = 0; c < nr - 1; c++) rhs[o + c] = 0.0;
+                            rhs[o + nr - 1] = DV;
+                            continue;
+                        }
+                        for (int c = 0; c < nr - 1; c++) rhs[o + c] = rhs[o + c] / rhs[o + nr - 1];
+                    }
+        }
+    }
+    int h = p->interf_iproj / 2;
+    int xs = NNb[0] + h, xf = NNb[3] - h, ys = NNb[1] + h, yf = NNb[4] - h, zs = NNb[2] + h, zf = NNb[5] - h;
+    const double thr = (double)1e-09f; /* default-REAL literal, vpm_remesh.f90:184 */
+    int64_t np = 0;
+    for (int k = zs; k <= zf; k++)
+        for (int j = ys; j <= yf; j++)
+            for (int i = xs; i <= xf; i++) {
+                size_t o = IX(0, i - 1, j - 1, k - 1, nr, NN[0], NN[1]);
+                double wmag = sqrt(rhs[o] * rhs[o] + rhs[o + 1] * rhs[o + 1] + rhs[o + 2] * rhs[o + 2]);
+                if (wmag < thr) continue;
+                if (xp_out) {
+                    xp_out[3 * np + 0] = g->xbound[0] + D[0] * (double)(i - 1);
+                    xp_out[3 * np + 1] = g->xbound[1] + D[1] * (double)(j - 1);
+                    xp_out[3 * np + 2] = g->xbound[2] + D[2] * (double)(k - 1);
+                    for (int c = 0; c < neqpm; c++) qp_out[(size_t)nr * np + c] = rhs[o + c] * DV;
+                    qp_out[(size_t)nr * np + neqpm] = DV;
+                }
+                np++;
+            }
+    free(rhs);
+    return np;
+}
+
+void orc_diagnostics(int neqpm, const double *xp, const double *qp, int64_t nvr, double circ[3], double imp[3]) {
+    int nr = neqpm + 1;
+    double c0 = 0, c1 = 0, c2 = 0, i0 = 0, i1 = 0, i2 = 0;
+    for (int64_t n = 0; n < nvr; n++) {
+        const double *x = xp + 3 * n, *q = qp + (size_t)nr * n;
+        double w0 = -q[0], w1 = -q[1], w2 = -q[2];
+        c0 += w0; c1 += w1; c2 += w2;
+        i0 += x[1] * w2 - x[2] * w1;
+        i1 += x[2] * w0 - x[0] * w2;
+        i2 += x[0] * w1 - x[1] * w0;
+    }
+    circ[0] = c0; circ[1] = c1; circ[2] = c2;
+    imp[0] = 0.5 * i0; imp[1] = 0.5 * i1; imp[2] = 0.5 * i2;
+}

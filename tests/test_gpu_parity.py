@@ -1,0 +1,166 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle on the same
+seeded inputs.  Bar: relative L2 <= 1e-12 for FP64 fields and particle values, bit-exact
+for remesh node indices and particle counts."""
+import numpy as np
+import pytest
+
+from vpm_free_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b.ravel())
+    return np.linalg.norm((a - b).ravel()) / (nb if nb > 0 else 1.0)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx(vpmlib):
+    V = vpmlib
+    p = V.Params.make(0.125)
+    c = V.VPM(p)
+    yield c
+    c.close()
+
+
+def test_solve_zero_bc(ctx, oracle):
+    rng = np.random.default_rng(1)
+    nn = np.array([21, 18, 25], np.int32)
+    xb = np.array([-1.0, -0.5, -0.7, 0, 0, 0])
+    dpm = np.array([0.1, 0.07, 0.09])
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    win = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
+    say("case: solve zero-bc", nn.tolist())
+    ref = oracle.solve_window(xb, dpm, nn, win, rhs, zero_bc=True)
+    got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, zero_bc=True)
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_solve_window_bc(ctx, oracle):
+    rng = np.random.default_rng(2)
+    nn = np.array([30, 27, 24], np.int32)
+    xb = np.array([-1.0, -0.5, -0.7, 0, 0, 0])
+    dpm = np.array([0.05, 0.06, 0.07])
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    sol = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    win = np.array([4, 3, 5, 26, 25, 20], np.int32)
+    say("case: solve window bc", win.tolist())
+    ref = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
+    got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, sol=sol)
+    assert rel_l2(got, ref) <= TOL
+
+
+def _blob_rhs(nn, xb, dpm, centre=(0.1, -0.05, 0.02), s=0.15):
+    x = xb[0] + np.arange(nn[0]) * dpm[0]
+    y = xb[1] + np.arange(nn[1]) * dpm[1]
+    z = xb[2] + np.arange(nn[2]) * dpm[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r2 = (X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2
+    g = np.exp(-r2 / s ** 2)
+    return np.stack([g, -0.5 * g * X, 0.3 * g * Y], axis=-1)
+
+
+def test_pmesh_inf(ctx, oracle):
+    nn = np.array([33, 29, 31], np.int32)
+    dpm = np.array([1 / 16, 1 / 16, 1 / 16])
+    xb = np.array([-1.0, -0.875, -0.9375, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm)
+    sol = np.zeros_like(rhs)
+    say("case: pmesh_inf", nn.tolist())
+    ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    got = ctx.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
+    assert rel_l2(got, ref) <= TOL
+
+
+def test_project(ctx, oracle):
+    rng = np.random.default_rng(3)
+    nn = np.array([24, 20, 22], np.int32)
+    dpm = np.array([0.1, 0.1, 0.1])
+    xb = np.array([-1.0, -1.0, -1.0, 0, 0, 0])
+    n = 5000
+    xp = np.stack([rng.uniform(-0.8, 1.2, n), rng.uniform(-0.8, 0.8, n), rng.uniform(-0.8, 1.0, n)], axis=1)
+    qp = np.concatenate([rng.standard_normal((n, 3)), np.full((n, 1), 1e-3)], axis=1)
+    nnb = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
+    say("case: project", n)
+    ref = oracle.project(xb, dpm, nn, nnb, xp, qp)
+    got = ctx.stage_project(xb, dpm, nn, nnb, xp, qp)
+    assert rel_l2(got, ref) <= TOL
+
+
+def _ring_case(h, sigma):
+    xp, qp = cases.single_ring(h=h, sigma=sigma)
+    return xp, qp
+
+
+@pytest.mark.parametrize("h,sigma", [(0.125, 0.2), (1 / 16, 0.1)])
+def test_vpm_mode2_ring(vpmlib, oracle, h, sigma):
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 ring h=%g sigma=%g n=%d" % (h, sigma, xp.shape[0]))
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    with V.VPM(V.Params.make(h)) as c:
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        assert bytes(c.geometry)[: 8 * 9 + 4 * 9] == bytes(go)[: 8 * 9 + 4 * 9]
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+        errs = {
+            "rhs": rel_l2(c.get_grid(0), f["rhs"]),
+            "psi": rel_l2(c.get_grid(1), f["psi"]),
+            "vel": rel_l2(c.get_grid(3), f["vel"]),
+            "sol": rel_l2(c.get_grid(2), f["sol"]),
+            "up": rel_l2(up, up_r),
+            "gp": rel_l2(gp, gp_r),
+        }
+    say("  rel L2:", errs)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
+def test_remesh_exact(vpmlib, oracle):
+    V = vpmlib
+    h = 0.125
+    xp, qp = _ring_case(h, 0.2)
+    rng = np.random.default_rng(5)
+    xp = xp + rng.uniform(-0.3, 0.3, xp.shape) * h  # off-node positions
+    say("case: remesh n=%d" % xp.shape[0])
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xp)
+    xr, qr = oracle.remesh(po, go, xp, qp)
+    with V.VPM(V.Params.make(h)) as c:
+        c.vpm(xp, qp, None, None, xp.shape[0], 3, 0, 0)
+        xg, qg = c.remesh_particles_3d(xp, qp, mrem=1)
+    assert xg.shape == xr.shape
+    assert np.array_equal(xg, xr)  # node positions bit-exact, k->j->i order
+    assert rel_l2(qg, qr) <= TOL
+
+
+def test_rk2_step_and_diagnostics(vpmlib, oracle):
+    V = vpmlib
+    h = 0.125
+    xp, qp = _ring_case(h, 0.2)
+    say("case: rk2 step n=%d" % xp.shape[0])
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xp)
+    x1, q1 = oracle.step_rk2(po, go, xp, qp, 0.01, ntime=1)
+    with V.VPM(V.Params.make(h)) as c:
+        c.upload(xp, qp)
+        c.vpm_resident(0, 0)
+        c.step_rk2(0.01, ntime=1)
+        xg, qg, _, _ = c.download()
+        circ, imp = c.diagnostics()
+    assert xg.shape == x1.shape
+    assert np.array_equal(xg, x1)
+    assert rel_l2(qg, q1) <= TOL
+    circ_r, imp_r = oracle.diagnostics(x1, q1)
+    assert np.allclose(circ, circ_r, rtol=0, atol=1e-13)
+    assert rel_l2(imp, imp_r) <= 1e-12

@@ -1,0 +1,249 @@
+// Grid kernels: fills, curl and stretching stencils, the yaps coarse-grid glue, and
+// layout conversions.
+//
+//   curl_kernel     u = curl(psi) by central differences   (vpm_gcalc.f90:17-40)
+//   stretch_kernel  SOL(m) = -sum_j d_j(RHS_j u_m)          (vpm_gcalc.f90:66-132)
+//   coarse_sample   fine solution every ncoarse nodes       (yaps3d.f90:104-138)
+//   coarse_laplacian 7-point Laplacian of the sample        (yaps3d.f90:1990-2016)
+//   face_interp     TSC coarse correction on inner faces    (yaps3d.f90:340-395, 1914-1983)
+#include "vpmx_ctx.h"
+#include "vpmx_internal.h"
+
+namespace vpmx {
+
+__global__ void k_fill(double *p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+int fill(Ctx *c, double *p, int64_t n, double v) {
+    if (n <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p, n, v);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// Node (i,j,k) 0-based.  Interior 1..n-2 gets the curl, the rest 0 (velvr*_pm = 0 before).
+__global__ void __launch_bounds__(256) k_curl(int nx, int ny, int nz, double dx2, double dy2, double dz2, GView psi,
+                                               GView vel) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    double u = 0.0, v = 0.0, w = 0.0;
+    if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+        const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
+        int64_t xp = psi.at(i + 1, j, k), xm = psi.at(i - 1, j, k);
+        int64_t yp = psi.at(i, j + 1, k), ym = psi.at(i, j - 1, k);
+        int64_t zp = psi.at(i, j, k + 1), zm = psi.at(i, j, k - 1);
+        double dpsidx1 = (p1[xp] - p1[xm]) / dx2, dpsidx2 = (p2[xp] - p2[xm]) / dx2;
+        double dpsidy0 = (p0[yp] - p0[ym]) / dy2, dpsidy2 = (p2[yp] - p2[ym]) / dy2;
+        double dpsidz0 = (p0[zp] - p0[zm]) / dz2, dpsidz1 = (p1[zp] - p1[zm]) / dz2;
+        u = dpsidy2 - dpsidz1;
+        v = -(dpsidx2 - dpsidz0);
+        w = dpsidx1 - dpsidy0;
+    }
+    int64_t o = vel.at(i, j, k);
+    vel.p[o] = u;
+    vel.p[vel.cs + o] = v;
+    vel.p[2 * vel.cs + o] = w;
+}
+int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel) {
+    int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_curl, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2], 2 * h[0],
+                       2 * h[1], 2 * h[2], psi, vel);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// Interior 2..n-3 (0-based) gets the stretching term, the rest 0 (SOL_pm = 0 before).
+__global__ void __launch_bounds__(256) k_stretch(int nx, int ny, int nz, double dx2, double dy2, double dz2,
+                                                  GView rhs, GView vel, GView out) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    double r[3] = {0.0, 0.0, 0.0};
+    if (i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3) {
+        const double *R0 = rhs.p, *R1 = rhs.p + rhs.cs, *R2 = rhs.p + 2 * rhs.cs;
+        int64_t rxp = rhs.at(i + 1, j, k), rxm = rhs.at(i - 1, j, k);
+        int64_t ryp = rhs.at(i, j + 1, k), rym = rhs.at(i, j - 1, k);
+        int64_t rzp = rhs.at(i, j, k + 1), rzm = rhs.at(i, j, k - 1);
+        int64_t vxp = vel.at(i + 1, j, k), vxm = vel.at(i - 1, j, k);
+        int64_t vyp = vel.at(i, j + 1, k), vym = vel.at(i, j - 1, k);
+        int64_t vzp = vel.at(i, j, k + 1), vzm = vel.at(i, j, k - 1);
+        double a0p = R0[rxp], a0m = R0[rxm], a1p = R1[ryp], a1m = R1[rym], a2p = R2[rzp], a2m = R2[rzm];
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const double *u = vel.p + m * vel.cs;
+            double wdudx = -(a0p * u[vxp] - (a0m)*u[vxm]) / dx2;
+            double wdvdy = -(a1p * u[vyp] - (a1m)*u[vym]) / dy2;
+            double wdwdz = -(a2p * u[vzp] - (a2m)*u[vzm]) / dz2;
+            r[m] = wdudx + wdvdy + wdwdz;
+        }
+    }
+    int64_t o = out.at(i, j, k);
+    out.p[o] = r[0];
+    out.p[out.cs + o] = r[1];
+    out.p[2 * out.cs + o] = r[2];
+}
+int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out) {
+    int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_stretch, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2],
+                       2 * h[0], 2 * h[1], 2 * h[2], rhs, vel, out);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// coarse(c, cmap_x[a], cmap_y[b], cmap_z[cc]) = fine(c, a*nc, b*nc, cc*nc)  (0-based)
+__global__ void k_coarse_sample(int ncomp, int sx, int sy, int sz, int nc, const int *cmap, GView fine, GView coarse) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)sx * sy * sz) return;
+    int a = (int)(t % sx), b = (int)((t / sx) % sy), cc = (int)(t / ((int64_t)sx * sy));
+    int ci = cmap[a] - 1, cj = cmap[sx + b] - 1, ck = cmap[sx + sy + cc] - 1;
+    int64_t of = fine.at(a * nc, b * nc, cc * nc), oc = coarse.at(ci, cj, ck);
+    for (int q = 0; q < ncomp; q++) coarse.p[q * coarse.cs + oc] = fine.p[q * fine.cs + of];
+}
+int coarse_sample(Ctx *c, int ncomp, const int ns[3], int nc, const int *cmap_dev, GView fine, GView coarse,
+                  const int cn[3]) {
+    (void)cn;
+    int64_t nt = (int64_t)ns[0] * ns[1] * ns[2];
+    hipLaunchKernelGGL(k_coarse_sample, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, ncomp, ns[0], ns[1],
+                       ns[2], nc, cmap_dev, fine, coarse);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// rhs(i,j,k) for lo < i,j,k < hi (1-based, footprint interior); 0 elsewhere.
+__global__ void k_coarse_laplacian(int ncomp, int nx, int ny, int nz, int lx, int ly, int lz, int hx, int hy, int hz,
+                                   double c0, double c1, double c2, GView s, GView rhs) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx) + 1, j = (int)((t / nx) % ny) + 1, k = (int)(t / ((int64_t)nx * ny)) + 1;
+    int64_t o = rhs.at(i - 1, j - 1, k - 1);
+    bool in = i > lx && i < hx && j > ly && j < hy && k > lz && k < hz;
+    for (int q = 0; q < ncomp; q++) {
+        double v = 0.0;
+        if (in) {
+            const double *S = s.p + q * s.cs;
+            double sc = S[s.at(i - 1, j - 1, k - 1)];
+            double ax = ((S[s.at(i, j - 1, k - 1)] - 2 * sc) + S[s.at(i - 2, j - 1, k - 1)]) / c0;
+            double ay = ((S[s.at(i - 1, j, k - 1)] - 2 * sc) + S[s.at(i - 1, j - 2, k - 1)]) / c1;
+            double az = ((S[s.at(i - 1, j - 1, k)] - 2 * sc) + S[s.at(i - 1, j - 1, k - 2)]) / c2;
+            v = 0.0 + ((ax + ay) + az);
+        }
+        rhs.p[q * rhs.cs + o] = v;
+    }
+}
+int coarse_laplacian(Ctx *c, int ncomp, const int cn[3], const int lo[3], const int hi[3], const double hc[3],
+                     GView samp, GView rhs) {
+    int64_t nt = (int64_t)cn[0] * cn[1] * cn[2];
+    hipLaunchKernelGGL(k_coarse_laplacian, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, ncomp, cn[0],
+                       cn[1], cn[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], hc[0] * hc[0], hc[1] * hc[1],
+                       hc[2] * hc[2], samp, rhs);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+__device__ __forceinline__ double tsc(double x) {
+#pragma clang fp contract(off)
+    double a = fabs(x);
+    if (a > 1.5) return 0.0;
+    if (a >= 0.5) return 0.5 * (-a + 1.5) * (-a + 1.5);
+    return 0.5 * (a + 1.5) * (a + 1.5) - 1.5 * (a + 0.5) * (a + 0.5);
+}
+
+// interp_stencil for every node of the six faces of the inner box, in face order.
+__global__ void __launch_bounds__(256) k_face_interp(int ncomp, InterpArgs a, int64_t nface, GView csol, GView samp,
+                                                      GView sol) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nface) return;
+    const int XS = a.nbl[0], YS = a.nbl[1], ZS = a.nbl[2], XF = a.nbl[3], YF = a.nbl[4], ZF = a.nbl[5];
+    const int64_t f12 = (int64_t)(YF - YS + 1) * (ZF - ZS + 1), f34 = (int64_t)(XF - XS - 1) * (ZF - ZS + 1);
+    int i, j, k;
+    if (t < 2 * f12) {
+        int f = (int)(t / f12); int64_t r = t % f12;
+        i = f ? XF : XS; j = YS + (int)(r % (YF - YS + 1)); k = ZS + (int)(r / (YF - YS + 1));
+    } else if (t < 2 * (f12 + f34)) {
+        int64_t u = t - 2 * f12; int f = (int)(u / f34); int64_t r = u % f34;
+        j = f ? YF : YS; i = XS + 1 + (int)(r % (XF - XS - 1)); k = ZS + (int)(r / (XF - XS - 1));
+    } else {
+        int64_t u = t - 2 * (f12 + f34); int64_t f56 = (int64_t)(XF - XS - 1) * (YF - YS - 1);
+        int f = (int)(u / f56); int64_t r = u % f56;
+        k = f ? ZF : ZS; i = XS + 1 + (int)(r % (XF - XS - 1)); j = YS + 1 + (int)(r / (XF - XS - 1));
+    }
+    double X0 = a.xbl[0] + (double)(i - 1) * a.h[0];
+    double X1 = a.xbl[1] + (double)(j - 1) * a.h[1];
+    double X2 = a.xbl[2] + (double)(k - 1) * a.h[2];
+    int in = (int)round((X0 - a.xc[0]) / a.hc[0]) + 1;
+    int jn = (int)round((X1 - a.xc[1]) / a.hc[1]) + 1;
+    int kn = (int)round((X2 - a.xc[2]) / a.hc[2]) + 1;
+    // map_nodes of the block: coarse nodes inside its sampled footprint
+    auto inmap = [&](int p, int q, int r) {
+        return p >= a.fp_lo[0] && p <= a.fp_hi[0] && q >= a.fp_lo[1] && q <= a.fp_hi[1] && r >= a.fp_lo[2] &&
+               r <= a.fp_hi[2];
+    };
+    int nnb = (!inmap(in - 1, jn - 1, kn - 1) || !inmap(in + 1, jn + 1, kn + 1)) ? 1 : 0;
+    int64_t o = sol.at(i - 1, j - 1, k - 1);
+    double acc[3];
+    for (int q = 0; q < ncomp; q++) acc[q] = sol.p[q * sol.cs + o];
+    for (int kc = kn - 1; kc <= kn + 1; kc++)
+        for (int jc = jn - 1; jc <= jn + 1; jc++)
+            for (int ic = in - 1; ic <= in + 1; ic++) {
+                double xc = (X0 - a.xc[0] - (double)(ic - 1) * a.hc[0]) / a.hc[0];
+                double yc = (X1 - a.xc[1] - (double)(jc - 1) * a.hc[1]) / a.hc[1];
+                double zc = (X2 - a.xc[2] - (double)(kc - 1) * a.hc[2]) / a.hc[2];
+                double fw = tsc(xc) * tsc(yc) * tsc(zc);
+                bool ok = ic >= 1 && ic <= a.cn[0] && jc >= 1 && jc <= a.cn[1] && kc >= 1 && kc <= a.cn[2];
+                int64_t oc = ok ? csol.at(ic - 1, jc - 1, kc - 1) : 0;
+                for (int q = 0; q < ncomp; q++) {
+                    double cs = ok ? csol.p[q * csol.cs + oc] : 0.0;
+                    double sm = ok ? samp.p[q * samp.cs + oc] : 0.0;
+                    double add_sample = sm + 0.0;
+                    double add = nnb ? sm + 0.0 : 0.0;
+                    acc[q] = acc[q] + fw * ((cs - add_sample) + add);
+                }
+            }
+    for (int q = 0; q < ncomp; q++) sol.p[q * sol.cs + o] = acc[q];
+}
+int face_interp(Ctx *c, int ncomp, const InterpArgs &a, GView csol, GView samp, GView sol) {
+    int64_t nx = a.nbl[3] - a.nbl[0] + 1, ny = a.nbl[4] - a.nbl[1] + 1, nz = a.nbl[5] - a.nbl[2] + 1;
+    int64_t nface = 2 * (ny * nz + (nx - 2) * nz + (nx - 2) * (ny - 2));
+    hipLaunchKernelGGL(k_face_interp, dim3((unsigned)((nface + 255) / 256)), dim3(256), 0, c->stream, ncomp, a, nface,
+                       csol, samp, sol);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// dst[c + ncd*(i + nx*(j + ny*k))] = src(c,i,j,k) for c < ncs; extra component = fill_last.
+__global__ void k_soa_to_aos(int ncs, int ncd, int nx, int ny, int nz, GView src, double fl, double *dst) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    int64_t o = src.at(i, j, k);
+    for (int q = 0; q < ncd; q++) dst[(int64_t)ncd * t + q] = q < ncs ? src.p[q * src.cs + o] : fl;
+}
+int soa_to_aos(Ctx *c, int ncs, int ncd, const int n[3], GView src, double fl, double *dst) {
+    int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, ncs, ncd, n[0], n[1],
+                       n[2], src, fl, dst);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+__global__ void k_aos_to_soa(int nc, int nx, int ny, int nz, const double *src, GView dst) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    int64_t o = dst.at(i, j, k);
+    for (int q = 0; q < nc; q++) dst.p[q * dst.cs + o] = src[(int64_t)nc * t + q];
+}
+int aos_to_soa(Ctx *c, int nc, const int n[3], const double *src, GView dst) {
+    int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, nc, n[0], n[1], n[2],
+                       src, dst);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+}  // namespace vpmx

@@ -1,0 +1,475 @@
+// Particle-side kernels: cell sort, gather-form projection (P2M), interpolation (M2P),
+// particle updates, reductions and the remesh compaction.
+//
+// P2M follows project_particles_3D + project_vol3d (pmproject.f90:43-153) but is
+// reorganised as a deterministic gather: particles are counting-sorted by cell (ties
+// kept in particle order), and every node sums the particles of the 4x4x4 cells whose
+// stencil covers it, so no floating-point atomics are used and the result does not
+// depend on scheduling.
+#include "vpmx_internal.h"
+#include "vpmx_ctx.h"
+
+namespace vpmx {
+
+// Projection functions of projection_fun (pmproject.f90:391-431).
+__device__ __forceinline__ double proj_fun(int itype, double x) {
+#pragma clang fp contract(off)
+    double a = fabs(x), r = 0.0;
+    if (itype == 4) {
+        if (a < 1.0) r = 1.0 - 2.5 * (a * a) + 1.5 * (a * a * a);
+        else if (a <= 2.0) r = 0.5 * (2.0 - a) * (2.0 - a) * (1.0 - a);
+    } else if (itype == 3) {
+        if (a < 0.5) r = 0.5 * (a + 1.5) * (a + 1.5) - 1.5 * (a + 0.5) * (a + 0.5);
+        else if (a <= 1.5) r = 0.5 * (-a + 1.5) * (-a + 1.5);
+    } else if (itype == 2) {
+        r = (a > 1.0) ? 0.0 : 1.0 - a;
+    } else if (itype == 5) {
+        if (a <= 0.5) r = 1.0 - a * a;
+        else if (a <= 1.5) r = 0.5 * (1.0 - a) * (2.0 - a);
+    } else if (itype == 6) {
+        if (a <= 1.0) r = 1.0 - a * a;
+        else if (a <= 2.0) r = 0.5 * (1.0 - a) * (2.0 - a);
+    }
+    return r;
+}
+
+// Node offset of the stencil relative to the particle's cell: nodes inode-ips .. inode+ipf.
+__host__ __device__ inline int stencil_ips(int itype) { return itype == 2 ? 0 : 1; }
+__host__ __device__ inline int stencil_ipf(int itype) { return itype == 2 ? 1 : 2; }
+
+// ------------------------------------------------------------------ scans
+// Three-phase exclusive scan of int32 counts (n entries) into out (n+1 entries).
+__global__ void k_scan_blocks(const int *in, int *out, int *bsum, int64_t n) {
+    __shared__ int s[1024];
+    int64_t base = (int64_t)blockIdx.x * 1024;
+    int t = threadIdx.x;
+    int v[4];
+    int loc = 0;
+    for (int q = 0; q < 4; q++) {
+        int64_t idx = base + t * 4 + q;
+        v[q] = idx < n ? in[idx] : 0;
+        loc += v[q];
+    }
+    s[t] = loc;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int add = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += add;
+        __syncthreads();
+    }
+    int run = s[t] - loc;
+    for (int q = 0; q < 4; q++) {
+        int64_t idx = base + t * 4 + q;
+        if (idx < n) out[idx] = run;
+        run += v[q];
+    }
+    if (t == 255) bsum[blockIdx.x] = s[255];
+}
+__global__ void k_scan_sums(int *bsum, int64_t nb, int *total) {
+    __shared__ int s[1024];
+    int t = threadIdx.x;
+    int carry = 0;
+    for (int64_t base = 0; base < nb; base += 1024) {
+        int64_t idx = base + t;
+        int v = idx < nb ? bsum[idx] : 0;
+        s[t] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = t >= off ? s[t - off] : 0;
+            __syncthreads();
+            s[t] += add;
+            __syncthreads();
+        }
+        if (idx < nb) bsum[idx] = carry + s[t] - v;
+        int blk = s[1023];
+        __syncthreads();
+        carry += blk;
+    }
+    if (t == 0) *total = carry;
+}
+__global__ void k_scan_add(int *out, const int *bsum, int64_t n, const int *total) {
+    int64_t idx = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
+    int add = bsum[blockIdx.x];
+    for (int q = 0; q < 4; q++)
+        if (idx + q < n) out[idx + q] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+}
+
+int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n) {
+    int st = VPMX_OK;
+    int64_t nb = (n + 1023) / 1024;
+    int *bsum = (int *)ensure(c, c->scan_tmp, sizeof(int) * (size_t)(nb + 2), &st);
+    if (st) return st;
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, c->stream, in, out, bsum, n);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, bsum, nb, bsum + nb);
+    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, c->stream, out, bsum, n, bsum + nb);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ cell sort
+__global__ void k_zero_int(int *p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Cell of a particle: inode = int((x - X0)/DX) + 1 (pmproject.f90:59-61), kept when
+// inode is in [-1, n+1] in every direction (any node of its stencil can be on the grid).
+__global__ void k_cell_keys(Grid g, const double *xp, int64_t n, int cnx, int cny, int cnz, int *keys, int *count) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int ci[3];
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+        double t = (xp[3 * p + d] - g.x0[d]) / g.h[d];
+        if (!(t > -3.0 && t < (double)g.n[d] + 3.0)) { ok = false; ci[d] = 0; continue; }
+        int inode = (int)t + 1;
+        ci[d] = inode + 1;
+        if (ci[d] < 0 || ci[d] > g.n[d] + 2) ok = false;
+    }
+    int key = -1;
+    if (ok) {
+        key = ci[0] + cnx * (ci[1] + cny * ci[2]);
+        atomicAdd(&count[key], 1);
+    }
+    keys[p] = key;
+}
+__global__ void k_cell_scatter(const int *keys, int64_t n, const int *start, int *fill, int *perm) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int key = keys[p];
+    if (key < 0) return;
+    int pos = start[key] + atomicAdd(&fill[key], 1);
+    perm[pos] = (int)p;
+}
+// Order the particles of each cell by particle index (the scatter above is unordered).
+__global__ void k_cell_order(const int *start, const int *count, int64_t ncells, int *perm) {
+    int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= ncells) return;
+    int cnt = count[cidx];
+    if (cnt < 2) return;
+    int s = start[cidx];
+    for (int a = 1; a < cnt; a++) {
+        int v = perm[s + a];
+        int b = a - 1;
+        while (b >= 0 && perm[s + b] > v) {
+            perm[s + b + 1] = perm[s + b];
+            b--;
+        }
+        perm[s + b + 1] = v;
+    }
+}
+
+int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState &s) {
+    int st = VPMX_OK;
+    s.cn[0] = g.n[0] + 3; s.cn[1] = g.n[1] + 3; s.cn[2] = g.n[2] + 3;
+    s.ncells = (int64_t)s.cn[0] * s.cn[1] * s.cn[2];
+    int *keys = (int *)ensure(c, s.keys, sizeof(int) * (size_t)(n > 0 ? n : 1), &st); if (st) return st;
+    int *perm = (int *)ensure(c, s.perm, sizeof(int) * (size_t)(n > 0 ? n : 1), &st); if (st) return st;
+    int *count = (int *)ensure(c, s.count, sizeof(int) * (size_t)s.ncells, &st); if (st) return st;
+    int *start = (int *)ensure(c, s.start, sizeof(int) * (size_t)(s.ncells + 1), &st); if (st) return st;
+    int *fill = (int *)ensure(c, s.fill, sizeof(int) * (size_t)s.ncells, &st); if (st) return st;
+    unsigned gc = (unsigned)((s.ncells + 255) / 256), gp = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, count, s.ncells);
+    hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, fill, s.ncells);
+    if (n > 0)
+        hipLaunchKernelGGL(k_cell_keys, dim3(gp), dim3(256), 0, c->stream, g, xp, n, s.cn[0], s.cn[1], s.cn[2], keys,
+                           count);
+    VPMX_TRY(exclusive_scan(c, count, start, s.ncells));
+    if (n > 0) hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(256), 0, c->stream, keys, n, start, fill, perm);
+    hipLaunchKernelGGL(k_cell_order, dim3(gc), dim3(256), 0, c->stream, start, count, s.ncells, perm);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ P2M gather
+// One thread per view node.  View node (a,b,c) is grid node (i,j,k) = (a-wlo0+1, ...)
+// (1-based); nodes outside the grid window are written 0 (the block halo of rhsscat_3d).
+template <int NC>
+__global__ void __launch_bounds__(256) k_p2m_gather(Grid g, const int *start, const int *perm, int cnx, int cny,
+                                                     const double *xp, const double *qp, int qstride, int itype,
+                                                     double div, GView out, int vnx, int vny, int vnz, int wlo0,
+                                                     int wlo1, int wlo2) {
+#pragma clang fp contract(off)
+    int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nv = (int64_t)vnx * vny * vnz;
+    if (tid >= nv) return;
+    int a = (int)(tid % vnx);
+    int b = (int)((tid / vnx) % vny);
+    int cc = (int)(tid / ((int64_t)vnx * vny));
+    int i = a - wlo0 + 1, j = b - wlo1 + 1, k = cc - wlo2 + 1;
+    double acc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++) acc[q] = 0.0;
+    if (i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2]) {
+        const int ips = stencil_ips(itype), ipf = stencil_ipf(itype);
+        const double X = g.x0[0] + 0.0, Y = g.x0[1], Z = g.x0[2];
+        // node coordinate offsets (i-1)*DX as in x = (XP - XMIN - (i-1)*DX)/DX
+        const double ox = (double)(i - 1) * g.h[0], oy = (double)(j - 1) * g.h[1], oz = (double)(k - 1) * g.h[2];
+        for (int ck = k - ipf; ck <= k + ips; ck++) {
+            for (int cj = j - ipf; cj <= j + ips; cj++) {
+                int64_t row = (int64_t)cnx * ((cj + 1) + (int64_t)cny * (ck + 1));
+                int p0 = start[row + (i - ipf + 1)];
+                int p1 = start[row + (i + ips + 1) + 1];
+                for (int s = p0; s < p1; s++) {
+                    int p = perm[s];
+                    double px = xp[3 * (int64_t)p], py = xp[3 * (int64_t)p + 1], pz = xp[3 * (int64_t)p + 2];
+                    double fx = proj_fun(itype, (px - X - ox) / g.h[0]);
+                    double fy = proj_fun(itype, (py - Y - oy) / g.h[1]);
+                    double fz = proj_fun(itype, (pz - Z - oz) / g.h[2]);
+                    double f = fx * fy * fz;
+                    const double *q = qp + (int64_t)qstride * p;
+#pragma unroll
+                    for (int r = 0; r < NC; r++) acc[r] = acc[r] + f * q[r];
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NC; r++) acc[r] = acc[r] / div;
+    }
+    int64_t o = out.at(a, b, cc);
+#pragma unroll
+    for (int r = 0; r < NC; r++) out.p[r * out.cs + o] = acc[r];
+}
+
+int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
+               int itype, double div, GView out, const int vn[3], const int wlo[3]) {
+    int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
+    unsigned grid = (unsigned)((nv + 255) / 256);
+    const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
+    if (ncomp == 3)
+        hipLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1], xp,
+                           qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+    else if (ncomp == 4)
+        hipLaunchKernelGGL(k_p2m_gather<4>, dim3(grid), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1], xp,
+                           qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+    else
+        return fail(c, VPMX_ERR_UNSUP, "p2m_gather: ncomp must be 3 or 4");
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ M2P
+// back_to_particles_3D itype 1 (vpm_time.f90:30-107): UP = sum W vel, GP = (sum W SOL)*QP(neqpm+1).
+__global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const double *qp, int qstride, int itype,
+                                              GView vel, GView sol, double *up, double *gp, int64_t n) {
+#pragma clang fp contract(off)
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int ips = stencil_ips(itype), ipf = stencil_ipf(itype);
+    double px = xp[3 * p], py = xp[3 * p + 1], pz = xp[3 * p + 2];
+    double u0 = 0, u1 = 0, u2 = 0, g0 = 0, g1 = 0, g2 = 0;
+    double tx = (px - g.x0[0]) / g.h[0], ty = (py - g.x0[1]) / g.h[1], tz = (pz - g.x0[2]) / g.h[2];
+    bool inside = (tx > -3.0 && tx < g.n[0] + 3.0) && (ty > -3.0 && ty < g.n[1] + 3.0) && (tz > -3.0 && tz < g.n[2] + 3.0);
+    if (inside) {
+        int in = (int)tx + 1, jn = (int)ty + 1, kn = (int)tz + 1;
+        double fx[4], fy[4], fz[4];
+        for (int q = 0; q < 4; q++) {
+            fx[q] = proj_fun(itype, (px - g.x0[0] - (double)(in - ips + q - 1) * g.h[0]) / g.h[0]);
+            fy[q] = proj_fun(itype, (py - g.x0[1] - (double)(jn - ips + q - 1) * g.h[1]) / g.h[1]);
+            fz[q] = proj_fun(itype, (pz - g.x0[2] - (double)(kn - ips + q - 1) * g.h[2]) / g.h[2]);
+        }
+        int w = ips + ipf + 1;
+        for (int c = 0; c < w; c++) {
+            int k = kn - ips + c;
+            for (int b = 0; b < w; b++) {
+                int j = jn - ips + b;
+                for (int a = 0; a < w; a++) {
+                    int i = in - ips + a;
+                    double f = fx[a] * fy[b] * fz[c];
+                    if (i < 1 || i > g.n[0] || j < 1 || j > g.n[1] || k < 1 || k > g.n[2]) continue;
+                    int64_t ov = vel.at(i - 1, j - 1, k - 1), os = sol.at(i - 1, j - 1, k - 1);
+                    u0 = u0 + f * vel.p[ov];
+                    u1 = u1 + f * vel.p[vel.cs + ov];
+                    u2 = u2 + f * vel.p[2 * vel.cs + ov];
+                    g0 = g0 + f * sol.p[os];
+                    g1 = g1 + f * sol.p[sol.cs + os];
+                    g2 = g2 + f * sol.p[2 * sol.cs + os];
+                }
+            }
+        }
+    }
+    double vol = qp[(int64_t)qstride * p + (qstride - 1)];
+    up[3 * p] = u0; up[3 * p + 1] = u1; up[3 * p + 2] = u2;
+    gp[3 * p] = g0 * vol; gp[3 * p + 1] = g1 * vol; gp[3 * p + 2] = g2 * vol;
+}
+
+int m2p_gather(Ctx *c, const Grid &g, const SortState &, const double *xp, const double *qp, int qstride, int itype,
+               GView vel, GView sol, double *up, double *gp, int64_t n) {
+    if (n <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, xp, qp, qstride, itype,
+                       vel, sol, up, gp, n);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ updates
+// x = x0 + (u + uinf)*dt ; q(1:3) = q0(1:3) - g*dt  (test.f90:104-117 with FACDEF = 1)
+__global__ void k_euler(double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, double dt,
+                        double ux, double uy, double uz, const double *x0, const double *q0) {
+#pragma clang fp contract(off)
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double ui[3] = {ux, uy, uz};
+    for (int d = 0; d < 3; d++) xp[3 * p + d] = x0[3 * p + d] + (up[3 * p + d] + ui[d]) * dt;
+    for (int d = 0; d < 3; d++) qp[(int64_t)qs * p + d] = q0[(int64_t)qs * p + d] - gp[3 * p + d] * dt;
+    for (int d = 3; d < qs; d++) qp[(int64_t)qs * p + d] = q0[(int64_t)qs * p + d];
+}
+int euler_update(Ctx *c, double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, double dt,
+                 const double uinf[3], const double *x0, const double *q0) {
+    if (n <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_euler, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xp, qp, qs, up, gp, n, dt,
+                       uinf[0], uinf[1], uinf[2], x0, q0);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ reductions
+// Deterministic two-pass reductions: fixed per-block tree, then one block over partials.
+template <int NV, int MODE>  // MODE 0: sum, MODE 1: bbox (min 0..2, max 3..5)
+__global__ void __launch_bounds__(256) k_reduce_particles(const double *xp, const double *qp, int qs, int64_t n,
+                                                           double *part) {
+#pragma clang fp contract(off)
+    __shared__ double s[NV][256];
+    int t = threadIdx.x;
+    double v[NV];
+    for (int q = 0; q < NV; q++) v[q] = MODE == 1 ? (q < 3 ? 1e308 : -1e308) : 0.0;
+    for (int64_t p = (int64_t)blockIdx.x * 256 + t; p < n; p += (int64_t)gridDim.x * 256) {
+        if (MODE == 1) {
+            for (int d = 0; d < 3; d++) {
+                double x = xp[3 * p + d];
+                v[d] = fmin(v[d], x);
+                v[3 + d] = fmax(v[3 + d], x);
+            }
+        } else {
+            const double *x = xp + 3 * p, *q = qp + (int64_t)qs * p;
+            double w0 = -q[0], w1 = -q[1], w2 = -q[2];
+            v[0] += w0; v[1] += w1; v[2] += w2;
+            v[3] += x[1] * w2 - x[2] * w1;
+            v[4] += x[2] * w0 - x[0] * w2;
+            v[5] += x[0] * w1 - x[1] * w0;
+        }
+    }
+    for (int q = 0; q < NV; q++) s[q][t] = v[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int q = 0; q < NV; q++) {
+                if (MODE == 1) s[q][t] = q < 3 ? fmin(s[q][t], s[q][t + off]) : fmax(s[q][t], s[q][t + off]);
+                else s[q][t] = s[q][t] + s[q][t + off];
+            }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < NV; q++) part[(int64_t)blockIdx.x * NV + q] = s[q][0];
+}
+template <int NV, int MODE>
+__global__ void k_reduce_final(const double *part, int nb, double *out) {
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    for (int q = 0; q < NV; q++) {
+        double v = part[q];
+        for (int b = 1; b < nb; b++) {
+            double x = part[(int64_t)b * NV + q];
+            if (MODE == 1) v = q < 3 ? fmin(v, x) : fmax(v, x);
+            else v = v + x;
+        }
+        out[q] = v;
+    }
+}
+
+static int reduce6(Ctx *c, int mode, const double *xp, const double *qp, int qs, int64_t n, double out[6]) {
+    int st = VPMX_OK;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    double *part = (double *)ensure(c, c->red_tmp, sizeof(double) * (size_t)(6 * nb + 6), &st);
+    if (st) return st;
+    if (mode == 1) {
+        hipLaunchKernelGGL((k_reduce_particles<6, 1>), dim3(nb), dim3(256), 0, c->stream, xp, qp, qs, n, part);
+        hipLaunchKernelGGL((k_reduce_final<6, 1>), dim3(1), dim3(64), 0, c->stream, part, nb, part + 6 * nb);
+    } else {
+        hipLaunchKernelGGL((k_reduce_particles<6, 0>), dim3(nb), dim3(256), 0, c->stream, xp, qp, qs, n, part);
+        hipLaunchKernelGGL((k_reduce_final<6, 0>), dim3(1), dim3(64), 0, c->stream, part, nb, part + 6 * nb);
+    }
+    VPMX_HIP(c, hipGetLastError());
+    VPMX_HIP(c, hipMemcpyAsync(out, part + 6 * nb, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3]) {
+    double o[6];
+    VPMX_TRY(reduce6(c, 1, xp, nullptr, 0, n, o));
+    for (int d = 0; d < 3; d++) { mn[d] = o[d]; mx[d] = o[3 + d]; }
+    return VPMX_OK;
+}
+int diagnostics(Ctx *c, const double *xp, const double *qp, int qs, int64_t n, double circ[3], double imp[3]) {
+    double o[6];
+    VPMX_TRY(reduce6(c, 0, xp, qp, qs, n, o));
+    for (int d = 0; d < 3; d++) { circ[d] = o[d]; imp[d] = 0.5 * o[3 + d]; }
+    return VPMX_OK;
+}
+
+// ------------------------------------------------------------------ remesh
+// remesh_particles_3d (vpm_remesh.f90:137-195), ncell_rem=1: nodes k->j->i over [lo,hi],
+// kept when sqrt(w1^2+w2^2+w3^2) >= thr; the kept nodes keep that order (prefix sum).
+__global__ void k_remesh_flag(GView w, int lx, int ly, int lz, int nx, int ny, int nz, double thr, int *flag) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nt = (int64_t)nx * ny * nz;
+    if (t >= nt) return;
+    int a = (int)(t % nx), b = (int)((t / nx) % ny), cc = (int)(t / ((int64_t)nx * ny));
+    int64_t o = w.at(lx - 1 + a, ly - 1 + b, lz - 1 + cc);
+    double r0 = w.p[o], r1 = w.p[w.cs + o], r2 = w.p[2 * w.cs + o];
+    double wmag = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+    flag[t] = (wmag < thr) ? 0 : 1;
+}
+__global__ void k_remesh_write(GView w, int lx, int ly, int lz, int nx, int ny, int nz, const int *flag,
+                               const int *pos, double x0, double y0, double z0, double hx, double hy, double hz,
+                               double dv, int neqpm, double *xo, double *qo) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nt = (int64_t)nx * ny * nz;
+    if (t >= nt || !flag[t]) return;
+    int a = (int)(t % nx), b = (int)((t / nx) % ny), cc = (int)(t / ((int64_t)nx * ny));
+    int i = lx + a, j = ly + b, k = lz + cc;
+    int64_t o = w.at(i - 1, j - 1, k - 1);
+    int64_t p = pos[t];
+    xo[3 * p] = x0 + hx * (double)(i - 1);
+    xo[3 * p + 1] = y0 + hy * (double)(j - 1);
+    xo[3 * p + 2] = z0 + hz * (double)(k - 1);
+    int nr = neqpm + 1;
+    for (int q = 0; q < neqpm; q++) qo[(int64_t)nr * p + q] = w.p[q * w.cs + o] * dv;
+    qo[(int64_t)nr * p + neqpm] = dv;
+}
+
+int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr, const double x0[3],
+                   const double h[3], double dv, int neqpm, double **xp_out, double **qp_out, DBuf &bx, DBuf &bq,
+                   int64_t *count) {
+    (void)vn;
+    int st = VPMX_OK;
+    int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+    if (nx <= 0 || ny <= 0 || nz <= 0) { *count = 0; return VPMX_OK; }
+    int64_t nt = (int64_t)nx * ny * nz;
+    int *flag = (int *)ensure(c, c->rm_flag, sizeof(int) * (size_t)nt, &st); if (st) return st;
+    int *pos = (int *)ensure(c, c->rm_pos, sizeof(int) * (size_t)(nt + 1), &st); if (st) return st;
+    unsigned gr = (unsigned)((nt + 255) / 256);
+    hipLaunchKernelGGL(k_remesh_flag, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, thr, flag);
+    VPMX_TRY(exclusive_scan(c, flag, pos, nt));
+    int total = 0;
+    VPMX_HIP(c, hipMemcpyAsync(&total, pos + nt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    int nr = neqpm + 1;
+    double *xo = (double *)ensure(c, bx, sizeof(double) * 3 * (size_t)(total > 0 ? total : 1), &st); if (st) return st;
+    double *qo = (double *)ensure(c, bq, sizeof(double) * nr * (size_t)(total > 0 ? total : 1), &st); if (st) return st;
+    hipLaunchKernelGGL(k_remesh_write, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, flag, pos,
+                       x0[0], x0[1], x0[2], h[0], h[1], h[2], dv, neqpm, xo, qo);
+    VPMX_HIP(c, hipGetLastError());
+    *xp_out = xo;
+    *qp_out = qo;
+    *count = total;
+    return VPMX_OK;
+}
+
+}  // namespace vpmx

@@ -1,0 +1,387 @@
+// Dirichlet 7-point Poisson solve by sine transforms (replaces the MKL Poisson Library
+// call of solve_eq_3d / solve_eq_0_3d, pmsolve.f90:284-432).
+//
+// The system is the one MKL's d_Helmholtz_3D(q=0,'DDDDDD') solves: for every interior
+// node, sum_axes (u_{-} - 2u + u_{+})/h^2 = RHS, u given on the window faces.  Face
+// values are folded into the right-hand side, then the interior problem is diagonalised
+// by DST-I along x, y and z:  u_hat = rhs_hat / -(lx + ly + lz),
+// l_a(k) = 4 sin^2(pi k / (2(m_a+1))) / h_a^2.
+//
+// Each pass transforms a batch of lines per work-group in LDS.  A DST-I of length m is
+// read off the complex FFT of length L = 2(m+1) of the odd extension; two real lines
+// share one complex FFT (real and imaginary parts).  The FFT is a mixed-radix Stockham
+// autosort (any factorisation of L; radices 2, 3, 4 specialised, others generic).
+// The z pass does forward transform, diagonal scaling and inverse transform without
+// leaving LDS, so one solve costs 5 sweeps over the interior.
+#include <cmath>
+#include <map>
+
+#include "vpmx_ctx.h"
+#include "vpmx_internal.h"
+
+namespace vpmx {
+
+enum { DST_XF = 0, DST_Y = 1, DST_Z = 2, DST_XI = 3 };
+
+struct DstArgs {
+    int m;              // line length
+    int L;              // FFT length 2(m+1)
+    int npairs;         // line pairs per work-group
+    int nstages;
+    int radix[24];
+    int64_t nlines;     // lines per component
+    int mx, my, mz;     // interior extents
+    int64_t wcs;        // component stride of the work array
+    double *w;          // work array (mx,my,mz) per component
+    const double2 *tw;  // exp(-2 pi i t / L), t < L
+    // fold / store
+    GView rhs, bc, out;
+    int has_bc;
+    double ih2[3];
+    double scale;
+    const double *lam[3];  // eigenvalues (z pass)
+    double norm;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
+// buffer that holds the result.
+__device__ double2 *fft_lds(double2 *A, double2 *B, const DstArgs &a) {
+    const int L = a.L;
+    int Ns = 1;
+    for (int s = 0; s < a.nstages; s++) {
+        const int R = a.radix[s];
+        const int nbf = L / R;
+        const int total = a.npairs * nbf;
+        const int twstep = L / (Ns * R);
+        for (int t = threadIdx.x; t < total; t += blockDim.x) {
+            const int pair = t / nbf, j = t - pair * nbf;
+            const int k0 = j % Ns;
+            const double2 *src = A + pair * L;
+            double2 *dst = B + pair * L;
+            const int idxD = (j / Ns) * Ns * R + k0;
+            if (R == 4) {
+                double2 v0 = src[j], v1 = src[j + nbf], v2 = src[j + 2 * nbf], v3 = src[j + 3 * nbf];
+                if (k0) {
+                    v1 = cmul(v1, a.tw[k0 * twstep]);
+                    v2 = cmul(v2, a.tw[2 * k0 * twstep]);
+                    v3 = cmul(v3, a.tw[3 * k0 * twstep]);
+                }
+                double2 s02 = cadd(v0, v2), d02 = csub(v0, v2), s13 = cadd(v1, v3), d13 = csub(v1, v3);
+                // -i * d13
+                double2 md = make_double2(d13.y, -d13.x);
+                dst[idxD] = cadd(s02, s13);
+                dst[idxD + Ns] = cadd(d02, md);
+                dst[idxD + 2 * Ns] = csub(s02, s13);
+                dst[idxD + 3 * Ns] = csub(d02, md);
+            } else if (R == 2) {
+                double2 v0 = src[j], v1 = src[j + nbf];
+                if (k0) v1 = cmul(v1, a.tw[k0 * twstep]);
+                dst[idxD] = cadd(v0, v1);
+                dst[idxD + Ns] = csub(v0, v1);
+            } else if (R == 3) {
+                double2 v0 = src[j], v1 = src[j + nbf], v2 = src[j + 2 * nbf];
+                if (k0) {
+                    v1 = cmul(v1, a.tw[k0 * twstep]);
+                    v2 = cmul(v2, a.tw[2 * k0 * twstep]);
+                }
+                const double c3 = -0.5, s3 = 0.86602540378443864676;
+                double2 t1 = cadd(v1, v2), t2 = csub(v1, v2);
+                double2 base = make_double2(v0.x + c3 * t1.x, v0.y + c3 * t1.y);
+                // -i*s3*t2 = (s3*t2.y, -s3*t2.x)
+                double2 rot = make_double2(s3 * t2.y, -s3 * t2.x);
+                dst[idxD] = cadd(v0, t1);
+                dst[idxD + Ns] = cadd(base, rot);
+                dst[idxD + 2 * Ns] = csub(base, rot);
+            } else {
+                const int Lr = L / R;
+                for (int k = 0; k < R; k++) {
+                    double2 acc = make_double2(0.0, 0.0);
+                    for (int r = 0; r < R; r++) {
+                        double2 v = src[j + r * nbf];
+                        int ti = (r * k0 * twstep + ((r * k) % R) * Lr) % L;
+                        acc = cadd(acc, cmul(v, a.tw[ti]));
+                    }
+                    dst[idxD + k * Ns] = acc;
+                }
+            }
+        }
+        __syncthreads();
+        double2 *tmp = A;
+        A = B;
+        B = tmp;
+        Ns *= R;
+    }
+    return A;
+}
+
+// Address of point pt (0-based interior index) of line gl in the work array.
+__device__ __forceinline__ int64_t line_addr(int mode, const DstArgs &a, int64_t gl, int pt) {
+    if (mode == DST_XF || mode == DST_XI) return gl * a.mx + pt;  // lines are rows (j,k)
+    if (mode == DST_Y) {                                          // lines (i,k), stride mx
+        int64_t i = gl % a.mx, k = gl / a.mx;
+        return i + (int64_t)a.mx * ((int64_t)pt + (int64_t)a.my * k);
+    }
+    return gl + (int64_t)a.mx * a.my * pt;  // z: lines (i,j), stride mx*my
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_dst_pass(DstArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int L = a.L, m = a.m, nl = 2 * a.npairs;
+    double2 *A = lds, *B = lds + a.npairs * L;
+    const int comp = blockIdx.y;
+    const int64_t l0 = (int64_t)blockIdx.x * nl;
+    double *w = a.w + comp * a.wcs;
+    // clear the two zero entries of each odd extension
+    for (int t = threadIdx.x; t < a.npairs; t += blockDim.x) {
+        A[t * L] = make_double2(0.0, 0.0);
+        A[t * L + m + 1] = make_double2(0.0, 0.0);
+    }
+    // load
+    const int ne = nl * m;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        int line, pt;
+        if (MODE == DST_XF || MODE == DST_XI) { line = e / m; pt = e - line * m; }
+        else { pt = e / nl; line = e - pt * nl; }
+        int64_t gl = l0 + line;
+        double v = 0.0;
+        if (gl < a.nlines) {
+            if (MODE == DST_XF) {
+                int j = (int)(gl % a.my), k = (int)(gl / a.my);
+                int i = pt;
+                // window node (i+1, j+1, k+1)
+                int64_t o = a.rhs.at(i + 1, j + 1, k + 1);
+                v = a.rhs.p[comp * a.rhs.cs + o];
+                if (a.has_bc) {
+                    const double *bp = a.bc.p + comp * a.bc.cs;
+                    if (i == 0) v -= bp[a.bc.at(0, j + 1, k + 1)] * a.ih2[0];
+                    if (i == a.mx - 1) v -= bp[a.bc.at(a.mx + 1, j + 1, k + 1)] * a.ih2[0];
+                    if (j == 0) v -= bp[a.bc.at(i + 1, 0, k + 1)] * a.ih2[1];
+                    if (j == a.my - 1) v -= bp[a.bc.at(i + 1, a.my + 1, k + 1)] * a.ih2[1];
+                    if (k == 0) v -= bp[a.bc.at(i + 1, j + 1, 0)] * a.ih2[2];
+                    if (k == a.mz - 1) v -= bp[a.bc.at(i + 1, j + 1, a.mz + 1)] * a.ih2[2];
+                }
+            } else {
+                v = w[line_addr(MODE, a, gl, pt)];
+            }
+        }
+        double *Ad = (double *)A;
+        int pair = line >> 1, ri = line & 1;
+        Ad[2 * (pair * L + pt + 1) + ri] = v;
+        Ad[2 * (pair * L + L - pt - 1) + ri] = -v;
+    }
+    __syncthreads();
+    double2 *R = fft_lds(A, B, a);
+    if (MODE == DST_Z) {
+        // S_k = (-Im Z_k / 2, Re Z_k / 2); scale by norm / -(lx+ly+lz); odd extension again
+        double2 *C = (R == A) ? B : A;
+        for (int t = threadIdx.x; t < a.npairs * (m + 2); t += blockDim.x) {
+            int pair = t / (m + 2), k = t - pair * (m + 2);
+            if (k == 0 || k == m + 1) {
+                C[pair * L + k] = make_double2(0.0, 0.0);
+                continue;
+            }
+            double2 z = R[pair * L + k];
+            double sa = -0.5 * z.y, sb = 0.5 * z.x;
+            int64_t ga = l0 + 2 * pair, gb = ga + 1;
+            double lz = a.lam[2][k - 1];
+            double fa = 0.0, fb = 0.0;
+            if (ga < a.nlines) {
+                int ix = (int)(ga % a.mx), iy = (int)(ga / a.mx);
+                fa = a.norm / (-(a.lam[0][ix] + a.lam[1][iy] + lz));
+            }
+            if (gb < a.nlines) {
+                int ix = (int)(gb % a.mx), iy = (int)(gb / a.mx);
+                fb = a.norm / (-(a.lam[0][ix] + a.lam[1][iy] + lz));
+            }
+            double2 v = make_double2(sa * fa, sb * fb);
+            C[pair * L + k] = v;
+            C[pair * L + L - k] = make_double2(-v.x, -v.y);
+        }
+        __syncthreads();
+        double2 *O = (C == A) ? B : A;
+        R = fft_lds(C, O, a);
+    }
+    // store
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        int line, pt;
+        if (MODE == DST_XF || MODE == DST_XI) { line = e / m; pt = e - line * m; }
+        else { pt = e / nl; line = e - pt * nl; }
+        int64_t gl = l0 + line;
+        if (gl >= a.nlines) continue;
+        int pair = line >> 1;
+        double2 z = R[pair * L + pt + 1];
+        double v = (line & 1) ? 0.5 * z.x : -0.5 * z.y;
+        if (MODE == DST_XI) {
+            int j = (int)(gl % a.my), k = (int)(gl / a.my);
+            a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = v * a.scale;
+        } else {
+            w[line_addr(MODE, a, gl, pt)] = v * a.scale;
+        }
+    }
+}
+
+// Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
+__global__ void k_zero_faces(GView out, int nx, int ny, int nz, int ncomp) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t fxy = (int64_t)nx * ny, fxz = (int64_t)nx * nz, fyz = (int64_t)ny * nz;
+    int64_t tot = 2 * (fxy + fxz + fyz);
+    if (t >= tot) return;
+    int i, j, k;
+    if (t < 2 * fyz) { int f = (int)(t / fyz); int64_t r = t % fyz; i = f ? nx - 1 : 0; j = (int)(r % ny); k = (int)(r / ny); }
+    else if (t < 2 * (fyz + fxz)) { int64_t u = t - 2 * fyz; int f = (int)(u / fxz); int64_t r = u % fxz; j = f ? ny - 1 : 0; i = (int)(r % nx); k = (int)(r / nx); }
+    else { int64_t u = t - 2 * (fyz + fxz); int f = (int)(u / fxy); int64_t r = u % fxy; k = f ? nz - 1 : 0; i = (int)(r % nx); j = (int)(r / nx); }
+    int64_t o = out.at(i, j, k);
+    for (int c = 0; c < ncomp; c++) out.p[c * out.cs + o] = 0.0;
+}
+
+// ------------------------------------------------------------------ host side
+struct FftPlan {
+    int L = 0, nstages = 0, radix[24];
+    double2 *tw = nullptr;
+};
+
+static int get_plan(Ctx *c, int L, FftPlan **out) {
+    auto it = c->fft_plans.find(L);
+    if (it != c->fft_plans.end()) { *out = (FftPlan *)it->second; return VPMX_OK; }
+    FftPlan *p = new FftPlan();
+    p->L = L;
+    int r = L;
+    while (r % 4 == 0) { p->radix[p->nstages++] = 4; r /= 4; }
+    while (r % 2 == 0) { p->radix[p->nstages++] = 2; r /= 2; }
+    for (int q = 3; r > 1;) {
+        if (r % q == 0) {
+            if (p->nstages >= 24) { delete p; return fail(c, VPMX_ERR_UNSUP, "FFT length has too many factors"); }
+            p->radix[p->nstages++] = q; r /= q;
+        } else q += 2;
+    }
+    for (int s = 0; s < p->nstages; s++)
+        if (p->radix[s] > 64) {
+            int R = p->radix[s];
+            delete p;
+            return fail(c, VPMX_ERR_UNSUP, "DST length 2(m+1)=" + std::to_string(L) + " has a prime factor " +
+                                               std::to_string(R) + " > 64");
+        }
+    std::vector<double2> h(L);
+    for (int t = 0; t < L; t++) {
+        long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)L;
+        h[t] = make_double2((double)cosl(ang), (double)(-sinl(ang)));
+    }
+    VPMX_HIP(c, hipMalloc(&p->tw, sizeof(double2) * L));
+    VPMX_HIP(c, hipMemcpy(p->tw, h.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
+    c->fft_plans[L] = p;
+    *out = p;
+    return VPMX_OK;
+}
+
+void free_fft_plans(Ctx *c) {
+    for (auto &kv : c->fft_plans) {
+        FftPlan *p = (FftPlan *)kv.second;
+        if (p->tw) (void)hipFree(p->tw);
+        delete p;
+    }
+    c->fft_plans.clear();
+}
+
+static int eigen(Ctx *c, int m, double h, DBuf &buf, const double **out) {
+    int st = VPMX_OK;
+    double *d = (double *)ensure(c, buf, sizeof(double) * (size_t)m, &st);
+    if (st) return st;
+    std::vector<double> v(m);
+    for (int k = 1; k <= m; k++) {
+        double s = std::sin(M_PI * (double)k / (2.0 * (double)(m + 1)));
+        v[k - 1] = 4.0 * s * s / (h * h);
+    }
+    VPMX_HIP(c, hipMemcpyAsync(d, v.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    *out = d;
+    return VPMX_OK;
+}
+
+template <int MODE>
+static int launch_pass(Ctx *c, DstArgs a, int ncomp) {
+    size_t lds = sizeof(double2) * 2 * (size_t)a.npairs * a.L;
+    unsigned nblk = (unsigned)((a.nlines + 2 * a.npairs - 1) / (2 * a.npairs));
+    hipLaunchKernelGGL(k_dst_pass<MODE>, dim3(nblk, ncomp), dim3(256), lds, c->stream, a);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+static void setup_axis(DstArgs &a, FftPlan *p, int m) {
+    a.m = m;
+    a.L = p->L;
+    a.nstages = p->nstages;
+    for (int s = 0; s < p->nstages; s++) a.radix[s] = p->radix[s];
+    a.tw = p->tw;
+    int np = (int)(65536 / (sizeof(double2) * 2 * (size_t)p->L));
+    if (np > 8) np = 8;
+    if (np < 1) np = 1;
+    a.npairs = np;
+}
+
+int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out) {
+    int st = VPMX_OK;
+    int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
+    if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(c, VPMX_ERR_ARG, "Poisson window needs >= 3 nodes per axis");
+    if (bc && bc->p != out.p) return fail(c, VPMX_ERR_ARG, "Poisson: BC view must be the output view");
+    for (int d = 0; d < 3; d++)
+        if (2 * (int64_t)(m[d] + 1) * sizeof(double2) * 2 > 65536)
+            return fail(c, VPMX_ERR_UNSUP, "Poisson: line length too long for LDS");
+    int64_t mt = (int64_t)m[0] * m[1] * m[2];
+    double *w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)mt * ncomp, &st);
+    if (st) return st;
+    FftPlan *px, *py, *pz;
+    VPMX_TRY(get_plan(c, 2 * (m[0] + 1), &px));
+    VPMX_TRY(get_plan(c, 2 * (m[1] + 1), &py));
+    VPMX_TRY(get_plan(c, 2 * (m[2] + 1), &pz));
+    DstArgs a{};
+    a.mx = m[0]; a.my = m[1]; a.mz = m[2];
+    a.wcs = mt;
+    a.w = w;
+    a.rhs = rhs;
+    a.out = out;
+    a.has_bc = bc ? 1 : 0;
+    if (bc) a.bc = *bc;
+    for (int d = 0; d < 3; d++) a.ih2[d] = 1.0 / (h[d] * h[d]);
+    VPMX_TRY(eigen(c, m[0], h[0], c->lam[0], &a.lam[0]));
+    VPMX_TRY(eigen(c, m[1], h[1], c->lam[1], &a.lam[1]));
+    VPMX_TRY(eigen(c, m[2], h[2], c->lam[2], &a.lam[2]));
+    a.norm = (2.0 / (m[0] + 1)) * (2.0 / (m[1] + 1)) * (2.0 / (m[2] + 1));
+    // x forward (with boundary folding)
+    setup_axis(a, px, m[0]);
+    a.nlines = (int64_t)m[1] * m[2];
+    a.scale = 1.0;
+    VPMX_TRY(launch_pass<DST_XF>(c, a, ncomp));
+    // y forward
+    setup_axis(a, py, m[1]);
+    a.nlines = (int64_t)m[0] * m[2];
+    VPMX_TRY(launch_pass<DST_Y>(c, a, ncomp));
+    // z forward + diagonal + inverse
+    setup_axis(a, pz, m[2]);
+    a.nlines = (int64_t)m[0] * m[1];
+    VPMX_TRY(launch_pass<DST_Z>(c, a, ncomp));
+    // y inverse
+    setup_axis(a, py, m[1]);
+    a.nlines = (int64_t)m[0] * m[2];
+    VPMX_TRY(launch_pass<DST_Y>(c, a, ncomp));
+    // x inverse -> output interior
+    setup_axis(a, px, m[0]);
+    a.nlines = (int64_t)m[1] * m[2];
+    VPMX_TRY(launch_pass<DST_XI>(c, a, ncomp));
+    if (!bc) {
+        int64_t tot = 2 * ((int64_t)wn[0] * wn[1] + (int64_t)wn[0] * wn[2] + (int64_t)wn[1] * wn[2]);
+        hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, out, wn[0], wn[1],
+                           wn[2], ncomp);
+        VPMX_HIP(c, hipGetLastError());
+    }
+    return VPMX_OK;
+}
+
+}  // namespace vpmx

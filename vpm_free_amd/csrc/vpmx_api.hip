@@ -1,0 +1,710 @@
+// Context, pipeline orchestration and the extern "C" entry points of libvpmx.
+//
+// vpm(WhatToDo=2) (vpm.f90:171-313) on one device, one yaps block:
+//   project_particles (P2M into the block RHS; rhsscat_3d's window and zero halo)
+//   yaps3d (yaps3d.f90:1-453): fine infinite-domain pmesh on the block, coarse sampling,
+//     coarse Laplacian, coarse infinite-domain pmesh, TSC face correction, inner solve
+//   solget_3d is a view: the global grid is the inner box of the block arrays
+//   calc_velocity_serial_3d(1), back_to_particles_3D(itype=1)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vpmx_ctx.h"
+#include "vpmx_internal.h"
+
+namespace vpmx {
+
+int fail(Ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st) {
+    if (bytes == 0) bytes = 8;
+    if (b.bytes >= bytes) return b.p;
+    if (b.p) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        *st = fail(c, VPMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        b.p = nullptr;
+        return nullptr;
+    }
+    b.bytes = bytes;
+    return b.p;
+}
+
+static void release(DBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+static GView view(double *p, const int n[3], int ox = 0, int oy = 0, int oz = 0, const int *alloc = nullptr) {
+    const int *a = alloc ? alloc : n;
+    GView v;
+    v.p = p;
+    v.py = a[0];
+    v.pz = (int64_t)a[0] * a[1];
+    v.cs = (int64_t)a[0] * a[1] * a[2];
+    v.ox = ox;
+    v.oy = oy;
+    v.oz = oz;
+    return v;
+}
+
+static Grid make_grid(const int32_t n[3], const double x0[3], const double h[3]) {
+    Grid g;
+    for (int d = 0; d < 3; d++) {
+        g.n[d] = n[d];
+        g.x0[d] = x0[d];
+        g.h[d] = h[d];
+    }
+    return g;
+}
+
+// MKL-style spacing of a solve window: h = (xmax_calc - xmin_calc)/(n-1) (pmsolve.f90:297-305).
+static void window_h(const double x0[3], const double dpm[3], const int32_t win[6], double h[3]) {
+    for (int d = 0; d < 3; d++) {
+        double mn = x0[d] + (double)(win[d] - 1) * dpm[d];
+        double mx = x0[d] + (double)(win[d + 3] - 1) * dpm[d];
+        h[d] = (mx - mn) / (double)(win[d + 3] - win[d]);
+    }
+}
+
+// ---------------------------------------------------------------- timing
+static void collect_times(Ctx *c, const bool *used) {
+    if (!c->ev_ok) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (int s = 0; s < ST_COUNT; s++) {
+        if (!used[s]) continue;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]) == hipSuccess) c->stage_ms[s] += ms;
+    }
+}
+
+// ---------------------------------------------------------------- pmesh (iynbc=1)
+// pmesh(iynbc=1, ibctyp=2) on grid g: SOL_0 = zero-BC solve, sources from its normal
+// derivatives, SOL(faces) += boundary sums, then the Dirichlet solve of the whole grid.
+static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine) {
+    int st = VPMX_OK;
+    const int n[3] = {g.n[0], g.n[1], g.n[2]};
+    int64_t G = (int64_t)n[0] * n[1] * n[2];
+    double *s0 = (double *)ensure(c, s0buf, sizeof(double) * 3 * (size_t)G, &st);
+    if (st) return st;
+    GView s0v = view(s0, n);
+    int32_t win[6] = {1, 1, 1, n[0], n[1], n[2]};
+    double h[3];
+    window_h(g.x0, g.h, win, h);
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v));
+    int64_t nbmax = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
+    double *sig = (double *)ensure(c, c->src_sig, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
+    double *cxyz = (double *)ensure(c, c->src_cxyz, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
+    double *wgt = (double *)ensure(c, c->src_w, sizeof(double) * 4 * (size_t)nbmax, &st); if (st) return st;
+    int64_t nb = 0;
+    VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
+    VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine));
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol));
+    return VPMX_OK;
+}
+
+// ---------------------------------------------------------------- yaps (1 block)
+static int yaps_single(Ctx *c, bool *used) {
+    int st = VPMX_OK;
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
+    double *rhs_bl = (double *)c->rhs_bl.p;
+    double *sol_bl = (double *)ensure(c, c->sol_bl, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    GView rv = view(rhs_bl, nb), sv = view(sol_bl, nb);
+    Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
+    // 1. fine infinite-domain solve over the whole block
+    mark(c, ST_FINE, 0);
+    VPMX_TRY(fill(c, sol_bl, 3 * B, 0.0));
+    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true));
+    mark(c, ST_FINE, 1);
+    used[ST_FINE] = used[ST_BOUNDS] = true;
+    // 2. coarse correction
+    mark(c, ST_COARSE, 0);
+    const int cn[3] = {G.nn_coarse[0], G.nn_coarse[1], G.nn_coarse[2]};
+    int64_t GC = (int64_t)cn[0] * cn[1] * cn[2];
+    double *samp = (double *)ensure(c, c->samp, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    double *rhsc = (double *)ensure(c, c->rhs_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    double *solc = (double *)ensure(c, c->sol_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    const int ncs = P.ncoarse;
+    int ns[3];
+    std::vector<int> cmap;
+    int fp_lo[3], fp_hi[3];
+    for (int d = 0; d < 3; d++) {
+        ns[d] = (nb[d] - 1) / ncs + 1;
+        for (int a = 0; a < ns[d]; a++) {
+            int i = 1 + a * ncs;
+            double X = G.xbound_bl[d] + (double)(i - 1) * G.dpm[d];
+            int node = (int)std::lround((X - G.xbound_coarse[d]) / G.dpm_coarse[d]) + 1;
+            if (node < 1 || node > cn[d]) return fail(c, VPMX_ERR_STATE, "coarse sample outside the coarse grid");
+            cmap.push_back(node);
+        }
+        fp_lo[d] = cmap[cmap.size() - ns[d]];
+        fp_hi[d] = cmap.back();
+        // the reference records NN_coarse_map at i = 1 and i = NNbl; with (NNbl-1) a multiple
+        // of ncoarse the last sample is i = NNbl
+    }
+    int *cmap_d = (int *)ensure(c, c->cmap, sizeof(int) * cmap.size(), &st); if (st) return st;
+    VPMX_HIP(c, hipMemcpyAsync(cmap_d, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice, c->stream));
+    GView smv = view(samp, cn), rcv = view(rhsc, cn), scv = view(solc, cn);
+    VPMX_TRY(fill(c, samp, 3 * GC, 0.0));
+    VPMX_TRY(coarse_sample(c, 3, ns, ncs, cmap_d, sv, smv, cn));
+    VPMX_TRY(coarse_laplacian(c, 3, cn, fp_lo, fp_hi, G.dpm_coarse, smv, rcv));
+    VPMX_TRY(fill(c, solc, 3 * GC, 0.0));
+    Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
+    VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));  // cmap host vector goes out of scope
+    mark(c, ST_COARSE, 1);
+    used[ST_COARSE] = true;
+    // 3. TSC correction of the inner-box faces
+    mark(c, ST_BC, 0);
+    InterpArgs ia;
+    for (int d = 0; d < 6; d++) ia.nbl[d] = G.nnbl_bl[d];
+    for (int d = 0; d < 3; d++) {
+        ia.xbl[d] = G.xbound_bl[d];
+        ia.h[d] = G.dpm[d];
+        ia.xc[d] = G.xbound_coarse[d];
+        ia.hc[d] = G.dpm_coarse[d];
+        ia.cn[d] = cn[d];
+        ia.fp_lo[d] = fp_lo[d];
+        ia.fp_hi[d] = fp_hi[d];
+    }
+    VPMX_TRY(face_interp(c, 3, ia, scv, smv, sv));
+    mark(c, ST_BC, 1);
+    used[ST_BC] = true;
+    // 4. final solve on the inner box
+    mark(c, ST_FINAL, 0);
+    const int32_t *w = G.nnbl_bl;
+    int wn[3] = {w[3] - w[0] + 1, w[4] - w[1] + 1, w[5] - w[2] + 1};
+    double h[3];
+    window_h(&G.xbound_bl[0], G.dpm, w, h);
+    GView rin = view(rhs_bl, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    GView sin = view(sol_bl, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin));
+    mark(c, ST_FINAL, 1);
+    used[ST_FINAL] = true;
+    return VPMX_OK;
+}
+
+static int check_ready(Ctx *c, int neqpm) {
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters: call vpmx_set_params or vpm mode 0 first");
+    if (!c->geom.defined) return fail(c, VPMX_ERR_STATE, "no geometry: vpm(WhatToDo=0) must run first");
+    if (c->geom.nblocks != 1)
+        return fail(c, VPMX_ERR_UNSUP, "NBI*NBJ*NBK > 1 yaps blocks are not supported by this build");
+    if (neqpm != 3) return fail(c, VPMX_ERR_UNSUP, "neqpm must be 3");
+    const vpmx_params &P = c->prm;
+    if (P.interf_iproj < 2 || P.interf_iproj > 4)
+        return fail(c, VPMX_ERR_ARG, "No such projection function (interf_iproj must be 2, 3 or 4)");
+    if (P.idvpm != 1) {
+        if (P.idvpm != 0) return fail(c, VPMX_ERR_ARG, "WRONG IDVPM");
+        return fail(c, VPMX_ERR_UNSUP, "IDVPM=0 (variable volume) is not supported by this build");
+    }
+    if (P.ibctyp != 2) return fail(c, VPMX_ERR_UNSUP, "only ibctyp=2 (boundary-source Green sums) is supported");
+    if (P.iyntree != 0) return fail(c, VPMX_ERR_UNSUP, "iyntree=1 (tree boundary sums) is not supported; set 0");
+    return VPMX_OK;
+}
+
+// Projection + yaps solve (+ velocity/stretching + interpolation for mode 2) on the
+// particles (xp, qp) already on the device.
+static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, double *up, double *gp, int64_t n) {
+    int st = VPMX_OK;
+    VPMX_TRY(check_ready(c, c->neqpm));
+    if (n == 0) return VPMX_OK;  // vpm.f90:172
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    bool used[ST_COUNT] = {};
+    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = 0.0;
+    c->last_pairs = 0.0;
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
+    double *rhs_bl = (double *)ensure(c, c->rhs_bl, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    // project_particles (vpm.f90:408-444) straight into the block layout of rhsscat_3d
+    mark(c, ST_PROJ, 0);
+    Grid gg = make_grid(G.nn, G.xbound, G.dpm);
+    VPMX_TRY(sort_particles(c, gg, xp, n, c->sort));
+    const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
+    int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
+    VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo));
+    mark(c, ST_PROJ, 1);
+    used[ST_PROJ] = true;
+    VPMX_TRY(yaps_single(c, used));
+    c->have_grids = true;
+    if (whattodo == 3) { collect_times(c, used); return VPMX_OK; }
+    // calc_velocity_serial_3d on the global grid = inner box of the block arrays
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    double *vel = (double *)ensure(c, c->vel, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    double *str = (double *)ensure(c, c->str, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    const int32_t *w = G.nnbl_bl;
+    GView psi = view((double *)c->sol_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    GView rg = view(rhs_bl, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    mark(c, ST_FD, 0);
+    VPMX_TRY(curl_kernel(c, gn, G.dpm, psi, view(vel, gn)));
+    if (whattodo == 2) VPMX_TRY(stretch_kernel(c, gn, G.dpm, rg, view(vel, gn), view(str, gn)));
+    mark(c, ST_FD, 1);
+    used[ST_FD] = true;
+    if (whattodo == 2) {
+        mark(c, ST_BACK, 0);
+        VPMX_TRY(m2p_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, P.interf_iproj, view(vel, gn), view(str, gn), up, gp,
+                            n));
+        mark(c, ST_BACK, 1);
+        used[ST_BACK] = true;
+    }
+    collect_times(c, used);
+    return VPMX_OK;
+}
+
+// remesh_particles_3d(1) on device particles; result in c->rm_xp/rm_qp/rm_count.
+static int run_remesh(Ctx *c, const double *xp, const double *qp, int64_t n, int mrem) {
+    int st = VPMX_OK;
+    VPMX_TRY(check_ready(c, c->neqpm));
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    if (P.ncell_rem != 1) return fail(c, VPMX_ERR_UNSUP, "ncell_rem must be 1");
+    if (mrem < 1) return fail(c, VPMX_ERR_ARG, "mrem must be >= 1");
+    bool used[ST_COUNT] = {};
+    mark(c, ST_REMESH, 0);
+    int32_t NN[3], NNb[6];
+    double D[3];
+    for (int d = 0; d < 3; d++) NN[d] = G.nn[d] * mrem;
+    for (int d = 0; d < 6; d++) NNb[d] = G.nn_bl[d] * mrem;
+    for (int d = 0; d < 3; d++) D[d] = (G.xbound[d + 3] - G.xbound[d]) / (double)(NN[d] - 1);
+    double DV = D[0] * D[1] * D[2];
+    int64_t GG = (int64_t)NN[0] * NN[1] * NN[2];
+    double *grid = (double *)ensure(c, c->rm_grid, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    Grid gr = make_grid(NN, G.xbound, D);
+    VPMX_TRY(sort_particles(c, gr, xp, n, c->sort));
+    int wlo[3] = {0, 0, 0};
+    VPMX_TRY(p2m_gather(c, gr, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(grid, NN), NN, wlo));
+    int half = P.interf_iproj / 2;
+    int lo[3] = {NNb[0] + half, NNb[1] + half, NNb[2] + half}, hi[3] = {NNb[3] - half, NNb[4] - half, NNb[5] - half};
+    const double thr = (double)1e-09f;  // default-REAL literal (vpm_remesh.f90:184)
+    int64_t cnt = 0;
+    double *xo = nullptr, *qo = nullptr;
+    VPMX_TRY(remesh_compact(c, view(grid, NN), NN, lo, hi, thr, G.xbound, D, DV, c->neqpm, &xo, &qo, c->rm_x, c->rm_q,
+                            &cnt));
+    mark(c, ST_REMESH, 1);
+    used[ST_REMESH] = true;
+    collect_times(c, used);
+    c->rm_xp = xo;
+    c->rm_qp = qo;
+    c->rm_count = cnt;
+    return VPMX_OK;
+}
+
+static int upload_particles(Ctx *c, const double *xp, const double *qp, int64_t n, int neqpm) {
+    int st = VPMX_OK;
+    if (n < 0 || n > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "particle count out of range");
+    int nr = neqpm + 1;
+    double *dx = (double *)ensure(c, c->xp, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *dq = (double *)ensure(c, c->qp, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
+    ensure(c, c->up, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    ensure(c, c->gp, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    if (n) {
+        VPMX_HIP(c, hipMemcpyAsync(dx, xp, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(dq, qp, sizeof(double) * nr * n, hipMemcpyHostToDevice, c->stream));
+    }
+    c->np = n;
+    c->neqpm = neqpm;
+    return VPMX_OK;
+}
+
+static int define_from_device(Ctx *c, int ntime) {
+    // define_sizes needs the particle bounding box only when it redefines the box
+    const vpmx_params &P = c->prm;
+    bool redefine = (ntime == 0 && P.idefine == 1) || P.idefine == 0 || !c->geom.defined;
+    if (!redefine) {
+        int r = vpmx_define_sizes(&c->prm, &c->geom, nullptr, 0, ntime);
+        return r == VPMX_OK ? r : fail(c, r, "define_sizes failed");
+    }
+    if (c->np <= 0) return fail(c, VPMX_ERR_ARG, "define_sizes: no particles");
+    double mn[3], mx[3];
+    VPMX_TRY(particle_bbox(c, (const double *)c->xp.p, c->np, mn, mx));
+    // the bbox is all define_sizes reads from the particles: two corner points suffice
+    double pts[6] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
+    int r = vpmx_define_sizes(&c->prm, &c->geom, pts, 2, ntime);
+    return r == VPMX_OK ? r : fail(c, r, "define_sizes failed");
+}
+
+}  // namespace vpmx
+
+using namespace vpmx;
+
+static Ctx *C(vpmx_ctx *p) { return static_cast<Ctx *>(p); }
+static const Ctx *C(const vpmx_ctx *p) { return static_cast<const Ctx *>(p); }
+
+extern "C" {
+
+int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
+    if (!out) return VPMX_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VPMX_ERR_NODEV;
+    if (device < 0 || device >= ndev) return VPMX_ERR_ARG;
+    Ctx *c = new Ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return VPMX_ERR_HIP;
+    }
+    c->ev_ok = true;
+    for (int i = 0; i < 2 * ST_COUNT; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) c->ev_ok = false;
+    *out = c;
+    return VPMX_OK;
+}
+
+void vpmx_destroy(vpmx_ctx *p) {
+    Ctx *c = C(p);
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DBuf *bufs[] = {&c->xp, &c->qp, &c->up, &c->gp, &c->x0, &c->q0, &c->rhs_bl, &c->sol_bl, &c->sol0, &c->vel,
+                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->samp, &c->rhs_c, &c->sol_c,
+                    &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
+                    &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
+                    &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill};
+    for (DBuf *b : bufs) release(*b);
+    free_fft_plans(c);
+    for (int i = 0; i < 2 * ST_COUNT; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    for (int k = 0; k < K_COUNT; k++)
+        for (hipEvent_t e : c->kev[k]) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *vpmx_last_error(const vpmx_ctx *p) { return p ? C(p)->err.c_str() : "null context"; }
+
+int32_t vpmx_set_params(vpmx_ctx *p, const vpmx_params *prm) {
+    Ctx *c = C(p);
+    if (!c || !prm) return VPMX_ERR_ARG;
+    if (prm->ipmwrite > 10) return fail(c, VPMX_ERR_ARG, "IPMWRITE > 10");
+    c->prm = *prm;
+    c->have_prm = true;
+    return VPMX_OK;
+}
+
+int32_t vpmx_get_geometry(const vpmx_ctx *p, vpmx_geom *out) {
+    if (!p || !out) return VPMX_ERR_ARG;
+    *out = C(p)->geom;
+    return VPMX_OK;
+}
+
+int32_t vpmx_set_precision(vpmx_ctx *p, int32_t mode) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    if (mode != 0) return fail(c, VPMX_ERR_UNSUP, "only the plain FP64 form (0) is available");
+    c->precision = mode;
+    return VPMX_OK;
+}
+
+int32_t vpmx_vpm(vpmx_ctx *p, double *xp, double *qp, double *up, double *gp, int64_t nvr, int32_t neqpm,
+                 int32_t whattodo, int32_t ntime, double ni) {
+    (void)ni;
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
+    if (whattodo == 0) {
+        int r = vpmx_define_sizes(&c->prm, &c->geom, xp, nvr, ntime);
+        if (r) return fail(c, r, "define_sizes failed");
+        c->have_grids = false;
+        return VPMX_OK;
+    }
+    if (whattodo < 1 || whattodo > 3) return fail(c, VPMX_ERR_UNSUP, "WhatToDo 4 and 5 are not supported");
+    if (nvr == 0) return VPMX_OK;
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
+    VPMX_TRY(run_vpm(c, whattodo, (const double *)c->xp.p, (const double *)c->qp.p, (double *)c->up.p,
+                     (double *)c->gp.p, nvr));
+    if (whattodo == 2) {
+        if (up) VPMX_HIP(c, hipMemcpyAsync(up, c->up.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+        if (gp) VPMX_HIP(c, hipMemcpyAsync(gp, c->gp.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_remesh_count(vpmx_ctx *p, const double *xp, const double *qp, int64_t nvr, int32_t neqpm, int32_t mrem,
+                          int64_t *nvr_out) {
+    Ctx *c = C(p);
+    if (!c || !nvr_out) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
+    VPMX_TRY(run_remesh(c, (const double *)c->xp.p, (const double *)c->qp.p, nvr, mrem));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    *nvr_out = c->rm_count;
+    return VPMX_OK;
+}
+
+int32_t vpmx_remesh_fetch(vpmx_ctx *p, double *xp_out, double *qp_out) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    if (c->rm_count < 0) return fail(c, VPMX_ERR_STATE, "no remesh result to fetch");
+    int nr = c->neqpm + 1;
+    if (c->rm_count > 0) {
+        if (xp_out)
+            VPMX_HIP(c, hipMemcpyAsync(xp_out, c->rm_xp, sizeof(double) * 3 * c->rm_count, hipMemcpyDeviceToHost,
+                                       c->stream));
+        if (qp_out)
+            VPMX_HIP(c, hipMemcpyAsync(qp_out, c->rm_qp, sizeof(double) * nr * c->rm_count, hipMemcpyDeviceToHost,
+                                       c->stream));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_get_grid(vpmx_ctx *p, int32_t which, double *dst, int64_t capacity) {
+    Ctx *c = C(p);
+    if (!c || !dst) return VPMX_ERR_ARG;
+    if (!c->have_grids) return fail(c, VPMX_ERR_STATE, "no grids: run vpm mode 1, 2 or 3 first");
+    const vpmx_geom &G = c->geom;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    const int32_t *w = G.nnbl_bl;
+    int st = VPMX_OK;
+    int ncd = which == 0 ? c->neqpm + 1 : 3;
+    if (capacity < ncd * GG) return fail(c, VPMX_ERR_ARG, "get_grid: destination too small");
+    double *stage = (double *)ensure(c, c->stage_a, sizeof(double) * ncd * (size_t)GG, &st);
+    if (st) return st;
+    const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
+    if (which == 0) {
+        VPMX_TRY(soa_to_aos(c, 3, 4, gn, view((double *)c->rhs_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb), DV, stage));
+    } else if (which == 1) {
+        VPMX_TRY(soa_to_aos(c, 3, 3, gn, view((double *)c->sol_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb), 0.0, stage));
+    } else if (which == 2) {
+        if (!c->str.p) return fail(c, VPMX_ERR_STATE, "no stretching grid");
+        VPMX_TRY(soa_to_aos(c, 3, 3, gn, view((double *)c->str.p, gn), 0.0, stage));
+    } else if (which == 3) {
+        if (!c->vel.p) return fail(c, VPMX_ERR_STATE, "no velocity grid");
+        VPMX_HIP(c, hipMemcpyAsync(stage, c->vel.p, sizeof(double) * 3 * GG, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        return fail(c, VPMX_ERR_ARG, "get_grid: which must be 0..3");
+    }
+    VPMX_HIP(c, hipMemcpyAsync(dst, stage, sizeof(double) * ncd * GG, hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_particles_upload(vpmx_ctx *p, const double *xp, const double *qp, int64_t nvr, int32_t neqpm) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    if (neqpm != 3) return fail(c, VPMX_ERR_UNSUP, "neqpm must be 3");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_particles_count(const vpmx_ctx *p, int64_t *nvr) {
+    if (!p || !nvr) return VPMX_ERR_ARG;
+    *nvr = C(p)->np;
+    return VPMX_OK;
+}
+
+int32_t vpmx_particles_download(vpmx_ctx *p, double *xp, double *qp, double *up, double *gp) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    int64_t n = c->np;
+    int nr = c->neqpm + 1;
+    if (n > 0) {
+        if (xp) VPMX_HIP(c, hipMemcpyAsync(xp, c->xp.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        if (qp) VPMX_HIP(c, hipMemcpyAsync(qp, c->qp.p, sizeof(double) * nr * n, hipMemcpyDeviceToHost, c->stream));
+        if (up) VPMX_HIP(c, hipMemcpyAsync(up, c->up.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+        if (gp) VPMX_HIP(c, hipMemcpyAsync(gp, c->gp.p, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_vpm_resident(vpmx_ctx *p, int32_t whattodo, int32_t ntime) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
+    if (whattodo == 0) {
+        VPMX_TRY(define_from_device(c, ntime));
+        c->have_grids = false;
+        return VPMX_OK;
+    }
+    if (whattodo < 1 || whattodo > 3) return fail(c, VPMX_ERR_UNSUP, "WhatToDo 4 and 5 are not supported");
+    VPMX_TRY(run_vpm(c, whattodo, (const double *)c->xp.p, (const double *)c->qp.p, (double *)c->up.p,
+                     (double *)c->gp.p, c->np));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+static int replace_with_remesh(Ctx *c) {
+    // swap the compacted particles in as the resident set
+    std::swap(c->xp, c->rm_x);
+    std::swap(c->qp, c->rm_q);
+    c->np = c->rm_count;
+    int st = VPMX_OK;
+    ensure(c, c->up, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
+    ensure(c, c->gp, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
+    c->rm_count = -1;
+    c->rm_xp = c->rm_qp = nullptr;
+    return VPMX_OK;
+}
+
+int32_t vpmx_remesh_resident(vpmx_ctx *p, int32_t mrem) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    VPMX_TRY(run_remesh(c, (const double *)c->xp.p, (const double *)c->qp.p, c->np, mrem));
+    VPMX_TRY(replace_with_remesh(c));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_step_rk2(vpmx_ctx *p, double dt, const double uinf[3], int32_t ntime, int32_t mrem, int32_t do_remesh) {
+    Ctx *c = C(p);
+    if (!c || !uinf) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    int st = VPMX_OK;
+    int64_t n = c->np;
+    int nr = c->neqpm + 1;
+    double *x0 = (double *)ensure(c, c->x0, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
+    if (n > 0) {
+        VPMX_HIP(c, hipMemcpyAsync(x0, xp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(q0, qp, sizeof(double) * nr * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    double acc[ST_COUNT] = {};
+    // stage 1 at (x0, q0)
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
+    // stage 2 at the midpoint
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, dt, uinf, x0, q0));
+    if (do_remesh) {
+        VPMX_TRY(define_from_device(c, ntime));
+        VPMX_TRY(run_remesh(c, xp, qp, n, mrem));
+        for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
+        VPMX_TRY(replace_with_remesh(c));
+    }
+    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = acc[s];
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_diagnostics(vpmx_ctx *p, double circ[3], double imp[3]) {
+    Ctx *c = C(p);
+    if (!c || !circ || !imp) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipSetDevice(c->device));
+    if (c->np <= 0) {
+        for (int d = 0; d < 3; d++) circ[d] = imp[d] = 0.0;
+        return VPMX_OK;
+    }
+    return diagnostics(c, (const double *)c->xp.p, (const double *)c->qp.p, c->neqpm + 1, c->np, circ, imp);
+}
+
+int32_t vpmx_stage_times(const vpmx_ctx *p, double *ms, int32_t n) {
+    if (!p || !ms) return VPMX_ERR_ARG;
+    const Ctx *c = C(p);
+    for (int i = 0; i < n && i < ST_COUNT; i++) ms[i] = c->stage_ms[i];
+    return VPMX_OK;
+}
+
+int32_t vpmx_stage_solve(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3], int32_t nc,
+                         const int32_t win[6], const double *rhs, double *sol, int32_t zero_bc, double *out) {
+    Ctx *c = C(p);
+    if (!c || !xbound || !dpm || !nn || !win || !rhs) return VPMX_ERR_ARG;
+    if (nc < 1 || nc > 4) return fail(c, VPMX_ERR_ARG, "nc must be 1..4");
+    if (zero_bc ? !out : !sol) return fail(c, VPMX_ERR_ARG, "missing output array");
+    for (int d = 0; d < 3; d++)
+        if (win[d] < 1 || win[d + 3] > nn[d] || win[d + 3] - win[d] < 2) return fail(c, VPMX_ERR_ARG, "bad window");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    int st = VPMX_OK;
+    const int n[3] = {nn[0], nn[1], nn[2]};
+    int64_t G = (int64_t)n[0] * n[1] * n[2];
+    double *ra = (double *)ensure(c, c->stage_a, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    double *rs = (double *)ensure(c, c->stage_b, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    double *ss = (double *)ensure(c, c->stage_c, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    VPMX_HIP(c, hipMemcpyAsync(ra, rhs, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(aos_to_soa(c, nc, n, ra, view(rs, n)));
+    const double *src = zero_bc ? out : sol;
+    VPMX_HIP(c, hipMemcpyAsync(ra, src, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(aos_to_soa(c, nc, n, ra, view(ss, n)));
+    int wn[3] = {win[3] - win[0] + 1, win[4] - win[1] + 1, win[5] - win[2] + 1};
+    double h[3];
+    window_h(xbound, dpm, win, h);
+    GView rv = view(rs, wn, win[0] - 1, win[1] - 1, win[2] - 1, n);
+    GView sv = view(ss, wn, win[0] - 1, win[1] - 1, win[2] - 1, n);
+    VPMX_TRY(poisson_dirichlet(c, nc, wn, h, rv, zero_bc ? nullptr : &sv, sv));
+    VPMX_TRY(soa_to_aos(c, nc, nc, n, view(ss, n), 0.0, ra));
+    VPMX_HIP(c, hipMemcpyAsync(zero_bc ? out : sol, ra, sizeof(double) * nc * G, hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_stage_pmesh_inf(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                             int32_t nc, const double *rhs, double *sol) {
+    Ctx *c = C(p);
+    if (!c || !xbound || !dpm || !nn || !rhs || !sol) return VPMX_ERR_ARG;
+    if (nc != 3) return fail(c, VPMX_ERR_ARG, "nc must be 3");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    int st = VPMX_OK;
+    const int n[3] = {nn[0], nn[1], nn[2]};
+    int64_t G = (int64_t)n[0] * n[1] * n[2];
+    double *ra = (double *)ensure(c, c->stage_a, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    double *rs = (double *)ensure(c, c->stage_b, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    double *ss = (double *)ensure(c, c->stage_c, sizeof(double) * nc * (size_t)G, &st); if (st) return st;
+    VPMX_HIP(c, hipMemcpyAsync(ra, rhs, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(aos_to_soa(c, nc, n, ra, view(rs, n)));
+    VPMX_HIP(c, hipMemcpyAsync(ra, sol, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(aos_to_soa(c, nc, n, ra, view(ss, n)));
+    Grid g = make_grid(nn, xbound, dpm);
+    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, false));
+    VPMX_TRY(soa_to_aos(c, nc, nc, n, view(ss, n), 0.0, ra));
+    VPMX_HIP(c, hipMemcpyAsync(sol, ra, sizeof(double) * nc * G, hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_stage_project(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                           const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr, int32_t ncomp,
+                           int32_t itype, int32_t idvpm, double *rhs) {
+    Ctx *c = C(p);
+    if (!c || !xbound || !dpm || !nn || !xp || !qp || !rhs) return VPMX_ERR_ARG;
+    (void)nn_bl;
+    if (ncomp != 4) return fail(c, VPMX_ERR_ARG, "ncomp must be 4 (3 strengths + volume)");
+    if (itype < 2 || itype > 6) return fail(c, VPMX_ERR_ARG, "No such projection function");
+    if (idvpm != 1) return fail(c, VPMX_ERR_UNSUP, "IDVPM=0 is not supported by this build");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    int st = VPMX_OK;
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, ncomp - 1));
+    const int n[3] = {nn[0], nn[1], nn[2]};
+    int64_t G = (int64_t)n[0] * n[1] * n[2];
+    double *gs = (double *)ensure(c, c->stage_b, sizeof(double) * 3 * (size_t)G, &st); if (st) return st;
+    double *ga = (double *)ensure(c, c->stage_a, sizeof(double) * 4 * (size_t)G, &st); if (st) return st;
+    Grid g = make_grid(nn, xbound, dpm);
+    VPMX_TRY(sort_particles(c, g, (const double *)c->xp.p, nvr, c->sort));
+    const double DV = dpm[0] * dpm[1] * dpm[2];
+    int wlo[3] = {0, 0, 0};
+    VPMX_TRY(p2m_gather(c, g, c->sort, (const double *)c->xp.p, (const double *)c->qp.p, ncomp, 3, itype, DV,
+                        view(gs, n), n, wlo));
+    VPMX_TRY(soa_to_aos(c, 3, 4, n, view(gs, n), DV, ga));
+    VPMX_HIP(c, hipMemcpyAsync(rhs, ga, sizeof(double) * 4 * G, hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Internal declarations shared by the HIP translation units of libvpmx.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/vpmx.h"
+
+namespace vpmx {
+
+// Node grid view of a component-planar (SoA) FP64 array:
+// value(c, i, j, k) = p[c * cs + (k + oz) * pz + (j + oy) * py + (i + ox)], 0-based node indices.
+struct GView {
+    double *p;
+    int64_t cs, py, pz;
+    int ox, oy, oz;
+    __host__ __device__ inline int64_t at(int i, int j, int k) const {
+        return (int64_t)(k + oz) * pz + (int64_t)(j + oy) * py + (i + ox);
+    }
+};
+
+// Geometry of a uniform node grid.
+struct Grid {
+    int n[3];
+    double x0[3];
+    double h[3];
+};
+
+// Device scratch that only grows.
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Ctx;
+
+// ---------------------------------------------------------------- error helpers
+int fail(Ctx *c, int code, const std::string &msg);
+#define VPMX_HIP(c, expr)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return ::vpmx::fail((c), VPMX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+#define VPMX_TRY(expr)           \
+    do {                         \
+        int s_ = (expr);         \
+        if (s_ != VPMX_OK) return s_; \
+    } while (0)
+
+void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st);
+
+// ---------------------------------------------------------------- launchers
+// Particles / projection (k_particles.hip)
+struct SortState {
+    DBuf keys, perm, count, start, fill;
+    int64_t ncells = 0;
+    int cn[3] = {0, 0, 0};  // padded cell grid dims
+};
+// Sort particles (xp AoS 3xn) into cells of grid g; cells are node indices inode in [-1, n+1].
+int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState &s);
+// Gather-form M-type projection: out(view over nodes of `g`, covering node window
+// [win0, win1) in each dim) = sum_p W * qp(c, p) / div for c < ncomp; nodes of the view
+// outside the window are written 0.
+int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
+               int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3]);
+int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
+               int itype, GView vel, GView sol, double *up, double *gp, int64_t n);
+int euler_update(Ctx *c, double *xp, double *qp, int qstride, const double *up, const double *gp, int64_t n,
+                 double dt, const double uinf[3], const double *x0, const double *q0);
+int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3]);
+int diagnostics(Ctx *c, const double *xp, const double *qp, int qstride, int64_t n, double circ[3], double imp[3]);
+int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr,
+                   const double x0[3], const double h[3], double dv, int neqpm, double **xp_out, double **qp_out,
+                   DBuf &bx, DBuf &bq, int64_t *count);
+
+// Poisson (k_poisson.hip)
+// Dirichlet 7-point solve on a window of nodes [0, wn) of the views (window = whole view
+// extent described by wn).  rhs: RHS view; bc: view holding Dirichlet values on the
+// window faces (nullptr for zero BCs); out: view receiving the solution (interior; the
+// faces are written with the BC values or zero).  h: node spacings.
+int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
+                      GView out);
+
+// Boundary (k_boundary.hip)
+int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt,
+                  int64_t *nbound);
+int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t nbound, const double *sig, const double *cxyz,
+                  const double *wgt, GView sol, bool fine);
+
+// Grid kernels (k_grid.hip)
+int fill(Ctx *c, double *p, int64_t n, double v);
+int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel);
+int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out);
+int coarse_sample(Ctx *c, int ncomp, const int nsamp[3], int nc, const int *cmap_dev, GView fine, GView coarse,
+                  const int cn[3]);
+int coarse_laplacian(Ctx *c, int ncomp, const int cn[3], const int lo[3], const int hi[3], const double hc[3],
+                     GView samp, GView rhs);
+struct InterpArgs {
+    int nbl[6];           // inner box NNbl_bl (1-based)
+    double xbl[3], h[3];  // block origin and fine spacing
+    double xc[3], hc[3];  // coarse origin, spacing
+    int cn[3];            // coarse dims
+    int fp_lo[3], fp_hi[3];  // footprint of the block on the coarse grid (1-based coarse index)
+};
+int face_interp(Ctx *c, int ncomp, const InterpArgs &a, GView coarse_sol, GView samp, GView sol_bl);
+int soa_to_aos(Ctx *c, int ncomp_src, int ncomp_dst, const int n[3], GView src, double fill_last, double *dst);
+int aos_to_soa(Ctx *c, int ncomp, const int n[3], const double *src, GView dst);
+
+}  // namespace vpmx

@@ -1,0 +1,309 @@
+"""Python host mirror of the reference's vpm() interface over the libvpmx C ABI.
+
+The reference entry points (vpm.f90:54-475 `vpm`, vpm_remesh.f90:5-225
+`remesh_particles_3d`, the pm.inp parse at vpm.f90:132-152) are exposed with the same
+names, argument meaning and call modes.  Particle arrays are numpy float64 arrays laid
+out like the Fortran arrays: XP(3,NVR) is a C-contiguous array of shape (NVR, 3),
+QP(neqpm+1,NVR) has shape (NVR, neqpm+1).
+
+Everything runs in libvpmx.so on the GPU; there is no CPU fallback.  If the library is
+not built, importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libvpmx.so")
+
+MAXB = 64
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_UNSUP, ERR_IO, ERR_NODEV = range(7)
+STAGES = ["Proj", "Poisson fine", "Bounds", "pmesh_coarse", "pm_bc2", "pmesh_final", "fd", "back", "remeshing"]
+
+
+class VpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"vpmx error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """pm.inp contents (vpm.f90:132-152)."""
+    _fields_ = [
+        ("dpm", C.c_double * 3), ("interf_iproj", C.c_int32), ("ibctyp", C.c_int32), ("idvpm", C.c_int32),
+        ("epsvol", C.c_double), ("ncoarse", C.c_int32), ("nbi", C.c_int32), ("nbj", C.c_int32),
+        ("nbk", C.c_int32), ("nremesh", C.c_int32), ("ncell_rem", C.c_int32), ("iyntree", C.c_int32),
+        ("ilevmax", C.c_int32), ("ompthreads", C.c_int32), ("idefine", C.c_int32), ("iynslice", C.c_int32),
+        ("ipmwrite", C.c_int32), ("ipmwstart", C.c_int32 * 10), ("ipmwsteps", C.c_int32 * 10),
+    ]
+
+    @classmethod
+    def make(cls, dpm, interf_iproj=4, ibctyp=2, idvpm=1, epsvol=0.0, ncoarse=8, blocks=(1, 1, 1),
+             nremesh=1, ncell_rem=1, iyntree=0, ilevmax=1, ompthreads=1, idefine=1, iynslice=0, ipmwrite=0):
+        p = cls()
+        d = np.broadcast_to(np.asarray(dpm, dtype=np.float64), (3,))
+        p.dpm[:] = [float(v) for v in d]
+        p.interf_iproj, p.ibctyp, p.idvpm, p.epsvol = interf_iproj, ibctyp, idvpm, epsvol
+        p.ncoarse = ncoarse
+        p.nbi, p.nbj, p.nbk = blocks
+        p.nremesh, p.ncell_rem, p.iyntree, p.ilevmax = nremesh, ncell_rem, iyntree, ilevmax
+        p.ompthreads, p.idefine, p.iynslice, p.ipmwrite = ompthreads, idefine, iynslice, ipmwrite
+        return p
+
+
+class Geom(C.Structure):
+    """Geometry of define_sizes (vpm.f90:477-660)."""
+    _fields_ = [
+        ("xbound", C.c_double * 6), ("dpm", C.c_double * 3), ("nn", C.c_int32 * 3), ("nn_bl", C.c_int32 * 6),
+        ("nblocks", C.c_int32), ("nbi", C.c_int32), ("nbj", C.c_int32), ("nbk", C.c_int32),
+        ("xbound_bl", C.c_double * (6 * MAXB)), ("nnbl", C.c_int32 * (3 * MAXB)),
+        ("nnbl_bl", C.c_int32 * (6 * MAXB)), ("xbound_coarse", C.c_double * 6), ("dpm_coarse", C.c_double * 3),
+        ("nn_coarse", C.c_int32 * 3), ("nn_bl_coarse", C.c_int32 * 6), ("ilevmax", C.c_int32),
+        ("defined", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def load_library():
+    """Load libvpmx.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m vpm_free_amd.build` (no CPU fallback exists)")
+    lib = C.CDLL(LIB_PATH)
+    d, i32, i64 = C.POINTER(C.c_double), C.c_int32, C.c_int64
+    vp = C.c_void_p
+    sig = {
+        "vpmx_read_pm_inp": [C.c_char_p, C.POINTER(Params)],
+        "vpmx_define_sizes": [C.POINTER(Params), C.POINTER(Geom), d, i64, i32],
+        "vpmx_create": [C.POINTER(vp), i32],
+        "vpmx_set_params": [vp, C.POINTER(Params)],
+        "vpmx_get_geometry": [vp, C.POINTER(Geom)],
+        "vpmx_set_precision": [vp, i32],
+        "vpmx_vpm": [vp, d, d, d, d, i64, i32, i32, i32, C.c_double],
+        "vpmx_remesh_count": [vp, d, d, i64, i32, i32, C.POINTER(i64)],
+        "vpmx_remesh_fetch": [vp, d, d],
+        "vpmx_get_grid": [vp, i32, d, i64],
+        "vpmx_particles_upload": [vp, d, d, i64, i32],
+        "vpmx_particles_count": [vp, C.POINTER(i64)],
+        "vpmx_particles_download": [vp, d, d, d, d],
+        "vpmx_vpm_resident": [vp, i32, i32],
+        "vpmx_remesh_resident": [vp, i32],
+        "vpmx_step_rk2": [vp, C.c_double, d, i32, i32, i32],
+        "vpmx_diagnostics": [vp, d, d],
+        "vpmx_stage_solve": [vp, d, d, C.POINTER(i32), i32, C.POINTER(i32), d, d, i32, d],
+        "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
+        "vpmx_stage_project": [vp, d, d, C.POINTER(i32), C.POINTER(i32), d, d, i64, i32, i32, i32, d],
+        "vpmx_stage_times": [vp, d, i32],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = i32
+    lib.vpmx_destroy.argtypes = [vp]
+    lib.vpmx_destroy.restype = None
+    lib.vpmx_last_error.argtypes = [vp]
+    lib.vpmx_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _ip(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def read_pm_inp(path):
+    """Parse pm.inp exactly as vpm.f90:132-152 reads it."""
+    lib = load_library()
+    p = Params()
+    r = lib.vpmx_read_pm_inp(str(path).encode(), C.byref(p))
+    if r != OK:
+        raise VpmError(r, f"cannot read {path}")
+    return p
+
+
+def define_sizes(params, xp, ntime=0, geom=None):
+    """define_sizes (vpm.f90:477-660) on the host; returns the (updated) Geom."""
+    lib = load_library()
+    g = geom if geom is not None else Geom()
+    xp = _f64(xp)
+    r = lib.vpmx_define_sizes(C.byref(params), C.byref(g), _dp(xp), xp.shape[0] if xp.ndim == 2 else 0, ntime)
+    if r != OK:
+        raise VpmError(r, "define_sizes")
+    return g
+
+
+def geom_dict(g):
+    """Plain-Python view of a Geom (first block only when BLOCKS=1)."""
+    nb = g.nblocks
+    return {
+        "xbound": list(g.xbound), "dpm": list(g.dpm), "nn": list(g.nn), "nn_bl": list(g.nn_bl),
+        "nblocks": nb, "xbound_bl": [list(g.xbound_bl[6 * b:6 * b + 6]) for b in range(nb)],
+        "nnbl": [list(g.nnbl[3 * b:3 * b + 3]) for b in range(nb)],
+        "nnbl_bl": [list(g.nnbl_bl[6 * b:6 * b + 6]) for b in range(nb)],
+        "xbound_coarse": list(g.xbound_coarse), "dpm_coarse": list(g.dpm_coarse),
+        "nn_coarse": list(g.nn_coarse), "nn_bl_coarse": list(g.nn_bl_coarse), "ilevmax": g.ilevmax,
+    }
+
+
+class VPM:
+    """One device context: vpm() modes, remesh_particles_3d(), device-resident stepping."""
+
+    def __init__(self, params, device=0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        r = self._lib.vpmx_create(C.byref(h), device)
+        if r != OK:
+            raise VpmError(r, "vpmx_create failed (no GPU?)")
+        self._h = h
+        self.params = params
+        self._check(self._lib.vpmx_set_params(self._h, C.byref(params)))
+        self.neqpm = 3
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vpmx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, r):
+        if r != OK:
+            raise VpmError(r, self._lib.vpmx_last_error(self._h).decode(errors="replace"))
+
+    # -- reference interface -------------------------------------------------------
+    def vpm(self, XP, QP, UP, GP, NVR, neqpm, WhatToDo, NTIME=0, NI=0.0):
+        """vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...): UP/GP are overwritten in mode 2."""
+        XP = _f64(XP)
+        QP = _f64(QP)
+        for a in (UP, GP):
+            if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous):
+                raise ValueError("UP/GP must be C-contiguous float64 arrays")
+        self._check(self._lib.vpmx_vpm(self._h, _dp(XP), _dp(QP), _dp(UP), _dp(GP), int(NVR), int(neqpm),
+                                       int(WhatToDo), int(NTIME), float(NI)))
+
+    def remesh_particles_3d(self, XP, QP, mrem=1):
+        """remesh_particles_3d(1): returns the new (XP, QP)."""
+        XP = _f64(XP)
+        QP = _f64(QP)
+        n = C.c_int64()
+        self._check(self._lib.vpmx_remesh_count(self._h, _dp(XP), _dp(QP), XP.shape[0], QP.shape[1] - 1, mrem,
+                                                C.byref(n)))
+        xo = np.empty((n.value, 3))
+        qo = np.empty((n.value, QP.shape[1]))
+        self._check(self._lib.vpmx_remesh_fetch(self._h, _dp(xo), _dp(qo)))
+        return xo, qo
+
+    @property
+    def geometry(self):
+        g = Geom()
+        self._check(self._lib.vpmx_get_geometry(self._h, C.byref(g)))
+        return g
+
+    def get_grid(self, which):
+        """0: RHS_pm(4,N) ; 1: SOL_pm after the solve ; 2: SOL_pm stretching ; 3: vel (3,N)."""
+        g = self.geometry
+        nx, ny, nz = g.nn
+        if which == 0:
+            out = np.empty((nz, ny, nx, 4))
+        elif which in (1, 2):
+            out = np.empty((nz, ny, nx, 3))
+        elif which == 3:
+            out = np.empty((3, nz, ny, nx))
+        else:
+            raise ValueError(which)
+        self._check(self._lib.vpmx_get_grid(self._h, which, _dp(out), out.size))
+        return out
+
+    def stage_times(self):
+        ms = np.zeros(len(STAGES))
+        self._check(self._lib.vpmx_stage_times(self._h, _dp(ms), len(STAGES)))
+        return dict(zip(STAGES, ms.tolist()))
+
+    # -- device-resident particles -------------------------------------------------
+    def upload(self, XP, QP):
+        XP, QP = _f64(XP), _f64(QP)
+        self._check(self._lib.vpmx_particles_upload(self._h, _dp(XP), _dp(QP), XP.shape[0], QP.shape[1] - 1))
+
+    def count(self):
+        n = C.c_int64()
+        self._check(self._lib.vpmx_particles_count(self._h, C.byref(n)))
+        return n.value
+
+    def download(self):
+        n = self.count()
+        xp, qp, up, gp = np.empty((n, 3)), np.empty((n, self.neqpm + 1)), np.empty((n, 3)), np.empty((n, 3))
+        self._check(self._lib.vpmx_particles_download(self._h, _dp(xp), _dp(qp), _dp(up), _dp(gp)))
+        return xp, qp, up, gp
+
+    def vpm_resident(self, WhatToDo, NTIME=0):
+        self._check(self._lib.vpmx_vpm_resident(self._h, WhatToDo, NTIME))
+
+    def remesh_resident(self, mrem=1):
+        self._check(self._lib.vpmx_remesh_resident(self._h, mrem))
+
+    def step_rk2(self, dt, uinf=(0.0, 0.0, 0.0), ntime=1, mrem=1, remesh=True):
+        u = np.asarray(uinf, dtype=np.float64)
+        self._check(self._lib.vpmx_step_rk2(self._h, float(dt), _dp(u), ntime, mrem, 1 if remesh else 0))
+
+    def diagnostics(self):
+        circ, imp = np.zeros(3), np.zeros(3)
+        self._check(self._lib.vpmx_diagnostics(self._h, _dp(circ), _dp(imp)))
+        return circ, imp
+
+    # -- single routines of the path -------------------------------------------------
+    def stage_solve(self, xbound, dpm, nn, nc, win, rhs, sol=None, zero_bc=False):
+        xb, dp = _f64(xbound), _f64(dpm)
+        nn_a, nn_p = _ip(nn)
+        w_a, w_p = _ip(win)
+        rhs = _f64(rhs)
+        out = np.array(sol if sol is not None else np.zeros_like(rhs), dtype=np.float64, order="C")
+        if zero_bc:
+            self._check(self._lib.vpmx_stage_solve(self._h, _dp(xb), _dp(dp), nn_p, nc, w_p, _dp(rhs), None, 1,
+                                                   _dp(out)))
+        else:
+            self._check(self._lib.vpmx_stage_solve(self._h, _dp(xb), _dp(dp), nn_p, nc, w_p, _dp(rhs), _dp(out), 0,
+                                                   None))
+        return out
+
+    def stage_pmesh_inf(self, xbound, dpm, nn, rhs, sol):
+        xb, dp = _f64(xbound), _f64(dpm)
+        nn_a, nn_p = _ip(nn)
+        rhs = _f64(rhs)
+        out = np.array(sol, dtype=np.float64, order="C")
+        self._check(self._lib.vpmx_stage_pmesh_inf(self._h, _dp(xb), _dp(dp), nn_p, 3, _dp(rhs), _dp(out)))
+        return out
+
+    def stage_project(self, xbound, dpm, nn, nn_bl, XP, QP, itype=4, idvpm=1):
+        xb, dp = _f64(xbound), _f64(dpm)
+        nn_a, nn_p = _ip(nn)
+        nb_a, nb_p = _ip(nn_bl)
+        XP, QP = _f64(XP), _f64(QP)
+        nx, ny, nz = [int(v) for v in nn]
+        out = np.empty((nz, ny, nx, QP.shape[1]))
+        self._check(self._lib.vpmx_stage_project(self._h, _dp(xb), _dp(dp), nn_p, nb_p, _dp(XP), _dp(QP),
+                                                 XP.shape[0], QP.shape[1], itype, idvpm, _dp(out)))
+        return out
