@@ -1,0 +1,185 @@
+"""Benchmark: VPM steps/s on one MI355X, two coaxial leapfrogging rings on the 256^3 grid.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+A step is the application's step with RK2: two vpm(WhatToDo=2) evaluations (projection,
+yaps infinite-domain Poisson solve, curl + stretching, interpolation), the particle
+updates, vpm(WhatToDo=0) and remesh_particles_3d(1) -- all on the device, particles
+resident in HBM.  Workload: rings R=1 and R=1.2, Gaussian core sigma=0.08, axial
+separation 0.5, box [-2,2]^3, h=1/64 (N=273 nodes per axis, 289^3 yaps block, 43^3
+coarse grid), particles at cell centres where |w|>1e-6, remeshed once before timing
+(test.f90:50-55), dt=0.01.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "VPM steps/s + HBM GB/s, 256³/2M particles; mixed-FP32 vs FP64 @1e-5 L2"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+FLOPS_PER_PAIR = 20.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--h", type=float, default=1.0 / 64)
+    ap.add_argument("--dt", type=float, default=0.01)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=256,
+                    help="boundary-sum target stride of the bounded CPU sample")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(h):
+    from vpm_free_amd import cases
+    return cases.leapfrog_rings(h=h)
+
+
+def cpu_baseline(h, xp, qp, stride):
+    """Oracle (C/OpenMP restatement of the reference path) on the host cores: one
+    vpm(WhatToDo=2) with the boundary sums evaluated on every stride-th boundary node,
+    scaled back to all nodes, plus one remesh; step = 2 x vpm2 + remesh."""
+    from oracle import oracle as O
+    p = O.Params.make(h)
+    g = O.define_sizes(p, xp)
+    O.set_boundsum_stride(stride)
+    O.timers(reset=True)
+    t0 = time.perf_counter()
+    O.vpm2(p, g, xp, qp)
+    wall_vpm = time.perf_counter() - t0
+    t = O.timers(reset=True)
+    O.set_boundsum_stride(1)
+    vpm_est = wall_vpm - t[2] + t[2] * stride
+    t0 = time.perf_counter()
+    O.remesh(p, g, xp, qp)
+    wall_rm = time.perf_counter() - t0
+    step = 2.0 * vpm_est + wall_rm
+    return {
+        "value": 1.0 / step,
+        "unit": "steps/s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": (f"oracle vpm(2) on the same 256^3 leapfrog workload with boundary sums on 1/{stride} "
+                   f"of the boundary nodes (measured {wall_vpm:.1f}s, boundary part {t[2]:.2f}s scaled x{stride}) "
+                   f"+ one full remesh ({wall_rm:.1f}s); step = 2 vpm(2) + remesh = {step:.1f}s"),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+                                   "and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+    from vpm_free_amd import vpm as V
+
+    t0 = time.perf_counter()
+    xp, qp = workload(args.h)
+    log(f"workload: {xp.shape[0]} particles generated in {time.perf_counter() - t0:.1f}s")
+    params = V.Params.make(args.h)
+    ctx = V.VPM(params)
+    ctx.upload(xp, qp)
+    ctx.vpm_resident(0, 0)  # define_sizes (NTIME=0): box fixed from here on (idefine=1)
+    ctx.remesh_resident(1)  # initial remesh, as the reference driver does before its loop
+    g = V.geom_dict(ctx.geometry)
+    nb = g["nnbl"][0]
+    pairs_fine = (nb[0] * nb[1] * nb[2] - (nb[0] - 2) * (nb[1] - 2) * (nb[2] - 2)) * \
+        2 * ((nb[1] - 1) * (nb[2] - 1) + (nb[0] - 1) * (nb[2] - 1) + (nb[0] - 1) * (nb[1] - 1))
+    n0 = ctx.count()
+    log(f"grid N={g['nn']} block={nb} coarse={g['nn_coarse']} particles after remesh={n0}")
+    xp, qp, _, _ = ctx.download()  # the CPU baseline starts from the same remeshed cloud
+    circ0, imp0 = ctx.diagnostics()
+
+    ntime = 1
+    for _ in range(args.warmup):
+        ctx.step_rk2(args.dt, ntime=ntime)
+        ntime += 1
+        log(f"warmup step done, particles={ctx.count()}")
+
+    stage_tot = {}
+    nparts = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ctx.step_rk2(args.dt, ntime=ntime)
+        ntime += 1
+        for k, v in ctx.stage_times().items():
+            stage_tot[k] = stage_tot.get(k, 0.0) + v
+        nparts.append(ctx.count())
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = 1000.0 * elapsed / args.steps
+    steps_per_s = args.steps / elapsed
+    circ1, imp1 = ctx.diagnostics()
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step; particles {nparts[0]}..{nparts[-1]}")
+    log("stage ms/step: " + ", ".join(f"{k}={v / args.steps:.2f}" for k, v in stage_tot.items()))
+
+    # dominant kernel: the fine-block FP64 boundary sum (one launch per vpm(2), two per step)
+    launches = 2 * args.steps
+    bs_ms = stage_tot["Bounds"] / launches
+    achieved = FLOPS_PER_PAIR * pairs_fine / (bs_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "k_boundsum<1> (FP64 direct Green's-function boundary sum, VALU)",
+        "pairs_per_launch": pairs_fine,
+        "ms_per_launch": round(bs_ms, 3),
+    }
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline sample ...")
+        cpu = cpu_baseline(args.h, xp, qp, args.cpu_stride)
+        log(f"cpu baseline: {cpu['value']:.4g} steps/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(steps_per_s, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "two coaxial leapfrogging vortex rings (R=1, 1.2; sigma=0.08; sep 0.5), box [-2,2]^3, "
+                        "h=1/64 (256^3 class: N=273, yaps block 289^3, coarse 43^3), RK2 dt=0.01 + remesh each step",
+            "grid_nodes": g["nn"],
+            "block_nodes": nb,
+            "particles": int(np.mean(nparts)),
+            "precision": "plain FP64 form",
+        },
+        "particles_per_s": round(float(np.mean(nparts)) * steps_per_s, 1),
+        "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_tot.items()},
+        "diagnostics": {"circulation_drift": float(np.max(np.abs(circ1 - circ0))),
+                        "impulse_rel_drift": float(np.linalg.norm(imp1 - imp0) / np.linalg.norm(imp0))},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

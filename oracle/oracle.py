@@ -90,6 +90,8 @@ def lib():
             "orc_dst1": (None, [C.c_int, D, D]),
             "orc_num_threads": (C.c_int, []),
             "orc_set_threads": (None, [C.c_int]),
+            "orc_timers": (None, [D, C.c_int]),
+            "orc_set_boundsum_stride": (None, [C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -117,6 +119,18 @@ def set_threads(n):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def timers(reset=False):
+    """Seconds spent in [projection, Poisson solves, boundary sums, whole vpm2/remesh]."""
+    t = np.zeros(4)
+    lib().orc_timers(_d(t), 1 if reset else 0)
+    return t
+
+
+def set_boundsum_stride(s):
+    """Evaluate only every s-th boundary target (bounded timing samples of big cases)."""
+    lib().orc_set_boundsum_stride(int(s))
 
 
 def projection_fun(itype, x):

@@ -41,6 +41,25 @@
 #define IX(c, i, j, k, nc, nx, ny) \
     ((size_t)(c) + (size_t)(nc) * ((size_t)(i) + (size_t)(nx) * ((size_t)(j) + (size_t)(ny) * (size_t)(k))))
 
+/* Stage timers (seconds) and the target stride of the boundary sums, for timing a bounded
+ * sample of a large case: 0 projection, 1 Poisson solves, 2 boundary sums, 3 whole vpm2/remesh. */
+static double g_timer[4];
+static int g_bs_stride = 1;
+static double now(void) {
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+void orc_timers(double *out, int reset) {
+    for (int i = 0; i < 4; i++) {
+        if (out) out[i] = g_timer[i];
+        if (reset) g_timer[i] = 0.0;
+    }
+}
+void orc_set_boundsum_stride(int s) { g_bs_stride = s > 0 ? s : 1; }
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();
@@ -97,12 +116,12 @@ double orc_projection_fun(int itype, double x) {
         r = (xabs > 1) ? 0.0 : 1.0 - xabs;
     } else if (itype == 3) {
         if (xabs > 3.0 / 2.0) r = 0.0;
-        if (xabs >= 0.5 && xabs <= 3.0 / 2.0) r = 0.5 * (-xabs + 3.0 / 2.0) * (-xabs + 3.0 / 2.0);
+        if (xabs >= 0.5 && xabs <= 3.0 / 2.0) r = 0.5 * ((-xabs + 3.0 / 2.0) * (-xabs + 3.0 / 2.0));
         if (xabs >= 0.0 && xabs < 0.5)
-            r = 0.5 * (xabs + 3.0 / 2.0) * (xabs + 3.0 / 2.0) - 3.0 / 2.0 * (xabs + 0.5) * (xabs + 0.5);
+            r = 0.5 * ((xabs + 3.0 / 2.0) * (xabs + 3.0 / 2.0)) - 3.0 / 2.0 * ((xabs + 0.5) * (xabs + 0.5));
     } else if (itype == 4) {
         if (xabs > 2.0) r = 0.0;
-        if (xabs >= 1.0 && xabs <= 2.0) r = 0.5 * (2.0 - xabs) * (2.0 - xabs) * (1.0 - xabs);
+        if (xabs >= 1.0 && xabs <= 2.0) r = 0.5 * ((2.0 - xabs) * (2.0 - xabs)) * (1.0 - xabs);
         if (xabs < 1.0) r = 1.0 - 2.5 * (xabs * xabs) + 1.5 * (xabs * xabs * xabs);
     } else if (itype == 5) {
         if (xabs <= 0.5) r = 1.0 - xabs * xabs;
@@ -352,6 +371,7 @@ void orc_solve_window(const double xb[6], const double dpm[3], const int32_t nn[
         m[d] = n[d] - 2;
         h[d] = (mx - mn) / (double)(n[d] - 1); /* MKL: h = (b-a)/n_intervals */
     }
+    double t0 = now();
     const double *bc = zero_bc ? NULL : sol;
     double *dst = zero_bc ? out : sol;
     int NX = nn[0], NY = nn[1];
@@ -416,6 +436,7 @@ void orc_solve_window(const double xb[6], const double dpm[3], const int32_t nn[
             }
 #undef G
     free(w);
+    g_timer[1] += now() - t0;
 }
 
 /* ------------------------------------------------------- normal derivatives */
@@ -537,8 +558,10 @@ void orc_boundsum(const double xb[6], const double dpm[3], const int32_t nn[3], 
     for (int f = 0; f < 2; f++)
         for (int j = 2; j <= NY - 1; j++)
             for (int i = 2; i <= NX - 1; i++) { tg[4*t] = i; tg[4*t+1] = j; tg[4*t+2] = f ? NZ : 1; tg[4*t+3] = 3; t++; }
+    double t0 = now();
+    int64_t stride = g_bs_stride;
 #pragma omp parallel for schedule(dynamic, 64)
-    for (int64_t q = 0; q < nt; q++) {
+    for (int64_t q = 0; q < nt; q += stride) {
         int i = tg[4 * q], j = tg[4 * q + 1], k = tg[4 * q + 2], pl = tg[4 * q + 3];
         double X = xb[0] + (double)(i - 1) * DX, Y = xb[1] + (double)(j - 1) * DY, Z = xb[2] + (double)(k - 1) * DZ;
         double hs = pl == 2 ? (DY + DZ) : (pl == 1 ? (DX + DZ) : (DX + DY));
@@ -557,6 +580,7 @@ void orc_boundsum(const double xb[6], const double dpm[3], const int32_t nn[3], 
         for (int c = 0; c < nc; c++) sol[IX(c, i - 1, j - 1, k - 1, nc, NX, NY)] = acc[c];
     }
     free(tg);
+    g_timer[2] += now() - t0;
 }
 
 /* ------------------------------------------------------------------ pmesh */
@@ -787,6 +811,7 @@ int orc_project(const double xb[6], const double dpm[3], const int32_t nn[3], co
     size_t G = (size_t)NX * NY * NZ;
     memset(rhs, 0, sizeof(double) * G * ncomp);
     int64_t outside = 0;
+    double t0 = now();
     for (int64_t nv = 0; nv < nvr; nv++) {
         const double *x = xp + 3 * nv;
         const double *q = qp + (size_t)ncomp * nv;
@@ -804,6 +829,7 @@ int orc_project(const double xb[6], const double dpm[3], const int32_t nn[3], co
                     for (int c = 0; c < ncomp; c++) rhs[o + c] = rhs[o + c] + f * q[c];
                 }
     }
+    g_timer[0] += now() - t0;
     double DV = dpm[0] * dpm[1] * dpm[2];
     if (idvpm == 1) {
         for (size_t n = 0; n < G; n++) {
@@ -836,6 +862,7 @@ int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp
              double *gp, int64_t nvr, double *rhs_out, double *psi_out, double *sol_out, double *vel_out) {
     if (nvr == 0) return 0;
     if (g->nblocks != 1 || neqpm != 3) return -1;
+    double t0 = now();
     int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
     size_t G = (size_t)NX * NY * NZ;
     int nr = neqpm + 1;
@@ -870,6 +897,7 @@ int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp
     if (sol_out) memcpy(sol_out, sol, sizeof(double) * G * neqpm);
     if (vel_out) memcpy(vel_out, vel, sizeof(double) * 3 * G);
     free(vel); free(rhs); free(sol);
+    g_timer[3] += now() - t0;
     return 0;
 }
 
@@ -878,6 +906,7 @@ int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp
 int64_t orc_remesh(const orc_params *p, const orc_geom *g, int neqpm, int mrem, const double *xp, const double *qp,
                    int64_t nvr, double *xp_out, double *qp_out) {
     if (p->ncell_rem != 1) return -1;
+    double t0 = now();
     int32_t NN[3], NNb[6];
     double D[3];
     for (int d = 0; d < 3; d++) NN[d] = g->nn[d] * mrem;
@@ -931,6 +960,7 @@ int64_t orc_remesh(const orc_params *p, const orc_geom *g, int neqpm, int mrem, 
                 np++;
             }
     free(rhs);
+    g_timer[3] += now() - t0;
     return np;
 }
 

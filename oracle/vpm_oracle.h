@@ -98,6 +98,8 @@ void orc_diagnostics(int neqpm, const double *xp, const double *qp, int64_t nvr,
 /* helpers exposed for tests */
 void orc_dst1(int m, const double *in, double *out); /* S_k = sum_j x_j sin(pi j k/(m+1)) */
 int orc_num_threads(void);
+void orc_timers(double *out, int reset);
+void orc_set_boundsum_stride(int s);
 void orc_set_threads(int n);
 
 #endif

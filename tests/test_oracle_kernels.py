@@ -1,0 +1,360 @@
+"""Pins of the CPU oracle's building blocks (runs without a GPU).
+
+Each check is chosen so that a plausible mistake (dropped term, wrong sign or index,
+transposed operand) fails: closed forms, moments, library routines (scipy DST-I),
+residuals of the linear system, exactness on polynomials, single-source closed forms,
+and the reference's own projection code compiled from pmproject.f90 (oracle/_ref).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+import scipy.fft as sf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ------------------------------------------------------------------ projection functions
+def test_m4prime_closed_form(oracle):
+    f = lambda x: oracle.projection_fun(4, x)
+    assert f(0.0) == 1.0
+    assert f(1.0) == 0.0 and f(2.0) == 0.0 and f(2.5) == 0.0
+    assert abs(f(0.5) - 0.5625) < 1e-16
+    assert abs(f(1.5) + 0.0625) < 1e-16
+    assert f(-0.7) == f(0.7)
+
+
+@pytest.mark.parametrize("itype,moments", [(2, (1, 0)), (3, (1, 0)), (4, (1, 0, 0))])
+def test_projection_moments(oracle, itype, moments):
+    """Partition of unity, first moment and (M4') second moment: sum_i (x-i)^m W(x-i)."""
+    rng = np.random.default_rng(itype)
+    for x in rng.uniform(0, 1, 50):
+        offs = np.arange(-3, 5) - x
+        w = np.array([oracle.projection_fun(itype, o) for o in offs])
+        for m, want in enumerate(moments):
+            assert abs(np.sum(offs ** m * w) - want) < 1e-14, (itype, m, x)
+
+
+def _ref_projlib():
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_projlib.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    return C.CDLL(path)
+
+
+def test_projection_fun_matches_reference(oracle):
+    """projection_fun_ compiled from the reference's pmproject.f90."""
+    L = _ref_projlib()
+    L.projection_fun_.restype = C.c_double
+    L.projection_fun_.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-2.6, 2.6, 400), [0, 0.5, 1, 1.5, 2, -1, -2]])
+    for itype in (2, 3, 4, 5, 6):
+        for x in xs:
+            it, xx = C.c_int(itype), C.c_double(x)
+            assert L.projection_fun_(C.byref(it), C.byref(xx)) == oracle.projection_fun(itype, x)
+
+
+def test_project_matches_reference_projlib(oracle):
+    """project_particles_3D + project_vol3d of the reference (oracle/_ref) vs orc_project."""
+    L = _ref_projlib()
+    D = C.POINTER(C.c_double)
+    I = C.POINTER(C.c_int)
+    rng = np.random.default_rng(7)
+    nn = np.array([20, 17, 23], np.int32)
+    nnb = np.array([1, 1, 1, 20, 17, 23], np.int32)
+    xb = np.array([-1.0, -0.8, -1.1, 0.9, 0.8, 1.1])
+    dpm = np.array([0.1, 0.1, 0.1])
+    n = 3000
+    xp = np.stack([rng.uniform(-0.7, 0.6, n), rng.uniform(-0.5, 0.5, n), rng.uniform(-0.8, 0.8, n)], axis=1)
+    qp = np.concatenate([rng.standard_normal((n, 3)), np.full((n, 1), 1e-3)], axis=1)
+    for itype in (2, 3, 4):
+        ref = np.zeros((nn[2], nn[1], nn[0], 4))
+        eps, idvpm, nd = C.c_double(0.0), C.c_int(1), C.c_int(3)
+        L._QMprojlibPprojlibinit(xb.ctypes.data_as(D), dpm.ctypes.data_as(D), nn.ctypes.data_as(I),
+                                 nnb.ctypes.data_as(I), C.byref(eps), C.byref(idvpm), C.byref(nd))
+        types = np.full(n, itype, np.int32)
+        ieq = np.arange(1, 5, dtype=np.int32)
+        qinf = np.zeros(4)
+        ipar, isize, neq = C.c_int(n), C.c_int(4), C.c_int(4)
+        L._QMprojlibPproject_particles_3d(ref.ctypes.data_as(D), qp.ctypes.data_as(D), xp.ctypes.data_as(D),
+                                          types.ctypes.data_as(I), C.byref(ipar), C.byref(isize),
+                                          ieq.ctypes.data_as(I), C.byref(neq), qinf.ctypes.data_as(D),
+                                          C.byref(ipar))
+        iflag = C.c_int(1)
+        L._QMprojlibPproject_vol3d(ref.ctypes.data_as(D), C.byref(isize), ieq.ctypes.data_as(I), C.byref(neq),
+                                   C.byref(iflag))
+        got = oracle.project(xb, dpm, nn, nnb, xp, qp, itype=itype, idvpm=1)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err <= 1e-15, (itype, err)
+
+
+def test_project_conserves_moments(oracle):
+    """M4' projection: total strength and first moments transfer exactly."""
+    rng = np.random.default_rng(11)
+    nn = np.array([30, 30, 30], np.int32)
+    nnb = np.array([1, 1, 1, 30, 30, 30], np.int32)
+    xb = np.array([-1.5, -1.5, -1.5, 1.4, 1.4, 1.4])
+    dpm = np.array([0.1, 0.1, 0.1])
+    n = 2000
+    xp = rng.uniform(-1.0, 1.0, (n, 3))
+    qp = np.concatenate([rng.standard_normal((n, 3)), np.full((n, 1), 1e-3)], axis=1)
+    rhs = oracle.project(xb, dpm, nn, nnb, xp, qp)
+    dv = 0.1 ** 3
+    tot = rhs[..., :3].reshape(-1, 3).sum(0) * dv
+    assert np.allclose(tot, qp[:, :3].sum(0), rtol=0, atol=1e-12)
+    x = xb[0] + np.arange(30) * 0.1
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    for d, G in enumerate((X, Y, Z)):
+        mom = (rhs[..., 0] * G).sum() * dv
+        assert abs(mom - (qp[:, 0] * xp[:, d]).sum()) < 1e-11
+    assert np.all(rhs[..., 3] == dv)
+
+
+# ------------------------------------------------------------------ Poisson solve
+@pytest.mark.parametrize("m", [7, 41, 95, 147, 271])
+def test_dst1_vs_scipy(oracle, m):
+    x = np.random.default_rng(m).standard_normal(m)
+    assert np.abs(oracle.dst1(x) - sf.dst(x, type=1) / 2).max() < 1e-12 * np.abs(x).sum()
+
+
+def _lap7(u, h):
+    """7-point Laplacian at interior nodes of u (nz,ny,nx)."""
+    c = u[1:-1, 1:-1, 1:-1]
+    return ((u[1:-1, 1:-1, 2:] - 2 * c + u[1:-1, 1:-1, :-2]) / h[0] ** 2 +
+            (u[1:-1, 2:, 1:-1] - 2 * c + u[1:-1, :-2, 1:-1]) / h[1] ** 2 +
+            (u[2:, 1:-1, 1:-1] - 2 * c + u[:-2, 1:-1, 1:-1]) / h[2] ** 2)
+
+
+def test_solve_residual_and_bc(oracle):
+    rng = np.random.default_rng(3)
+    nn = np.array([23, 19, 27], np.int32)
+    xb = np.array([0.0, 0.0, 0.0, 0, 0, 0])
+    dpm = np.array([0.05, 0.07, 0.06])
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 2))
+    sol = rng.standard_normal((nn[2], nn[1], nn[0], 2))
+    win = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
+    u = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
+    for c in range(2):
+        res = _lap7(u[..., c], dpm) - rhs[1:-1, 1:-1, 1:-1, c]
+        assert np.abs(res).max() < 1e-9 * np.abs(rhs).max()
+        inner = np.zeros(u.shape[:3], bool)
+        inner[1:-1, 1:-1, 1:-1] = True
+        assert np.array_equal(u[..., c][~inner], sol[..., c][~inner])
+
+
+def test_solve_sine_eigenmode(oracle):
+    nn = np.array([18, 22, 15], np.int32)
+    h = np.array([0.1, 0.08, 0.12])
+    m = nn - 2
+    k = (3, 5, 2)
+    ii = [np.arange(1, mm + 1) for mm in m]
+    modes = [np.sin(np.pi * k[d] * ii[d] / (m[d] + 1)) for d in range(3)]
+    f = np.zeros((nn[2], nn[1], nn[0], 1))
+    f[1:-1, 1:-1, 1:-1, 0] = modes[2][:, None, None] * modes[1][None, :, None] * modes[0][None, None, :]
+    lam = sum(4 * np.sin(np.pi * k[d] / (2 * (m[d] + 1))) ** 2 / h[d] ** 2 for d in range(3))
+    u = oracle.solve_window(np.zeros(6), h, nn, [1, 1, 1, *nn], f, zero_bc=True)
+    assert np.abs(u - f / (-lam)).max() < 1e-13 * np.abs(f / lam).max()
+
+
+def test_solve_vs_scipy_dstn_window(oracle):
+    """Sub-window solve with Dirichlet faces against an independent scipy.fft.dstn solve."""
+    rng = np.random.default_rng(9)
+    nn = np.array([26, 24, 22], np.int32)
+    dpm = np.array([0.1, 0.1, 0.1])
+    xb = np.array([-1.0, 2.0, 0.5, 0, 0, 0])
+    win = np.array([3, 2, 4, 24, 21, 20], np.int32)
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 1))
+    sol = rng.standard_normal((nn[2], nn[1], nn[0], 1))
+    u = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
+    s = [w - 1 for w in win[:3]]
+    e = [w for w in win[3:]]
+    R = rhs[s[2]:e[2], s[1]:e[1], s[0]:e[0], 0].copy()
+    B = sol[s[2]:e[2], s[1]:e[1], s[0]:e[0], 0].copy()
+    h = dpm
+    f = R[1:-1, 1:-1, 1:-1].copy()
+    f[:, :, 0] -= B[1:-1, 1:-1, 0] / h[0] ** 2
+    f[:, :, -1] -= B[1:-1, 1:-1, -1] / h[0] ** 2
+    f[:, 0, :] -= B[1:-1, 0, 1:-1] / h[1] ** 2
+    f[:, -1, :] -= B[1:-1, -1, 1:-1] / h[1] ** 2
+    f[0, :, :] -= B[0, 1:-1, 1:-1] / h[2] ** 2
+    f[-1, :, :] -= B[-1, 1:-1, 1:-1] / h[2] ** 2
+    m = f.shape
+    fh = sf.dstn(f, type=1)
+    lam = np.zeros(m)
+    for ax, hh in zip((2, 1, 0), h):
+        kk = np.arange(1, m[ax] + 1)
+        l = 4 * np.sin(np.pi * kk / (2 * (m[ax] + 1))) ** 2 / hh ** 2
+        shape = [1, 1, 1]
+        shape[ax] = m[ax]
+        lam = lam + l.reshape(shape)
+    ref = sf.idstn(fh / (-lam), type=1)
+    got = u[s[2] + 1:e[2] - 1, s[1] + 1:e[1] - 1, s[0] + 1:e[0] - 1, 0]
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+# ------------------------------------------------------------------ boundary sources
+def test_normalderiv_exact_on_quartics(oracle):
+    """The one-sided 5-point stencil is exact for degree-4 polynomials along the normal;
+    sigma is the mean of the 4 corner derivatives, oriented along the outward normal."""
+    nn = np.array([12, 10, 11], np.int32)
+    h = np.array([0.1, 0.125, 0.2])
+    xb = np.array([-0.5, -0.4, -1.0, 0, 0, 0])
+    x = xb[0] + np.arange(nn[0]) * h[0]
+    y = xb[1] + np.arange(nn[1]) * h[1]
+    z = xb[2] + np.arange(nn[2]) * h[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    f = np.stack([X ** 4 + Y * Z, Y ** 3 - 2 * X ** 2, Z ** 4 - X * Y ** 2], axis=-1)
+    fx = np.stack([4 * X ** 3, -4 * X, -Y ** 2], axis=-1)
+    fy = np.stack([Z, 3 * Y ** 2, -2 * X * Y], axis=-1)
+    fz = np.stack([Y, 0 * Z, 4 * Z ** 3], axis=-1)
+    src, cxyz, ds = oracle.normalderiv(xb, h, nn, f)
+    NX, NY, NZ = nn
+    assert src.shape[0] == 2 * ((NY - 1) * (NZ - 1) + (NX - 1) * (NZ - 1) + (NX - 1) * (NY - 1))
+    q = 0
+    for axis, (grad, n_a) in enumerate(((fx, NX), (fy, NY), (fz, NZ))):
+        for hi in (0, 1):
+            sgn = 1.0 if hi else -1.0  # outward normal
+            cnt = {0: (NY - 1) * (NZ - 1), 1: (NX - 1) * (NZ - 1), 2: (NX - 1) * (NY - 1)}[axis]
+            for t in range(cnt):
+                if axis == 0:
+                    j, k = t % (NY - 1), t // (NY - 1)
+                    i = NX - 1 if hi else 0
+                    corners = [(k, j, i), (k, j + 1, i), (k + 1, j + 1, i), (k + 1, j, i)]
+                    area = h[1] * h[2]
+                elif axis == 1:
+                    i, k = t % (NX - 1), t // (NX - 1)
+                    j = NY - 1 if hi else 0
+                    corners = [(k, j, i), (k, j, i + 1), (k + 1, j, i + 1), (k + 1, j, i)]
+                    area = h[0] * h[2]
+                else:
+                    i, j = t % (NX - 1), t // (NX - 1)
+                    k = NZ - 1 if hi else 0
+                    corners = [(k, j, i), (k, j, i + 1), (k, j + 1, i + 1), (k, j + 1, i)]
+                    area = h[0] * h[1]
+                want = sgn * np.mean([grad[c] for c in corners], axis=0)
+                assert np.allclose(src[q], want, rtol=1e-9, atol=1e-9), (axis, hi, t)
+                cen = np.mean([[x[c[2]], y[c[1]], z[c[0]]] for c in corners], axis=0)
+                assert np.allclose(cxyz[q], cen, rtol=0, atol=1e-14)
+                assert ds[q] == area
+                q += 1
+
+
+def test_boundsum_single_source_closed_form(oracle):
+    nn = np.array([9, 8, 10], np.int32)
+    h = np.array([0.25, 0.25, 0.25])
+    xb = np.array([0.0, 0.0, 0.0, 0, 0, 0])
+    cx = np.array([[0.7, -0.3, 1.1]])
+    src = np.array([[2.0, -1.0, 0.5]])
+    ds = np.array([0.0625])
+    sol = np.zeros((nn[2], nn[1], nn[0], 3))
+    out = oracle.boundsum(xb, h, nn, src, cx, ds, sol)
+    x = np.arange(nn[0]) * 0.25
+    y = np.arange(nn[1]) * 0.25
+    z = np.arange(nn[2]) * 0.25
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r = np.sqrt((X - 0.7) ** 2 + (Y + 0.3) ** 2 + (Z - 1.1) ** 2)
+    bnd = np.zeros(r.shape, bool)
+    bnd[0], bnd[-1], bnd[:, 0], bnd[:, -1], bnd[:, :, 0], bnd[:, :, -1] = [True] * 6
+    for c in range(3):
+        want = np.where(bnd, -src[0, c] * ds[0] / (4 * np.pi * r), 0.0)
+        assert np.allclose(out[..., c], want, rtol=1e-14, atol=0)
+
+
+def _gaussian(nn, xb, h, s=0.2, centre=(0.05, -0.02, 0.01)):
+    x = xb[0] + np.arange(nn[0]) * h[0]
+    y = xb[1] + np.arange(nn[1]) * h[1]
+    z = xb[2] + np.arange(nn[2]) * h[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r = np.sqrt((X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2)
+    w = np.exp(-r ** 2 / s ** 2) / (np.pi ** 1.5 * s ** 3)  # unit total strength
+    from scipy.special import erf
+    with np.errstate(invalid="ignore", divide="ignore"):
+        psi = np.where(r > 1e-12, erf(r / s) / (4 * np.pi * np.maximum(r, 1e-300)), 1 / (2 * np.pi ** 1.5 * s))
+    return w, psi
+
+
+def test_pmesh_inf_free_space_potential(oracle):
+    """Infinite-domain solve of lap(psi) = -w for a Gaussian blob reproduces the
+    free-space potential erf(r/s)/(4 pi r) to discretisation accuracy, including the
+    boundary values given by the Green's-function sums."""
+    h = np.array([1 / 16] * 3)
+    nn = np.array([33, 33, 33], np.int32)
+    xb = np.array([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0])
+    w, psi = _gaussian(nn, xb, h)
+    rhs = np.stack([-w, 2 * w, 0 * w], axis=-1)
+    out = oracle.pmesh_inf(xb, h, nn, rhs, np.zeros_like(rhs))
+    e0 = np.abs(out[..., 0] - psi).max() / psi.max()
+    e1 = np.abs(out[..., 1] + 2 * psi).max() / (2 * psi.max())
+    assert e0 < 2e-2 and e1 < 2e-2, (e0, e1)
+    assert np.abs(out[..., 2]).max() < 1e-14
+    # boundary values alone (sum of the face sources; the midpoint rule of the 1/r
+    # kernel on the source's own face is first order there) close to the exact far field
+    face = out[0, :, :, 0]
+    assert np.abs(face - psi[0]).max() / psi[0].max() < 3e-2
+
+
+# ------------------------------------------------------------------ velocity / stretching
+def test_velocity_and_stretching_exact_on_polynomials(oracle):
+    p = oracle.Params.make(0.1)
+    pts = np.array([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])
+    g = oracle.define_sizes(p, pts)
+    nx, ny, nz = g.nn
+    x = g.xbound[0] + np.arange(nx) * g.dpm[0]
+    y = g.xbound[1] + np.arange(ny) * g.dpm[1]
+    z = g.xbound[2] + np.arange(nz) * g.dpm[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    psi = np.stack([Y * Z + X ** 2, X * Z - Y ** 2 * 0.5, X * Y + 2 * Z ** 2], axis=-1)
+    # u = curl psi
+    u = np.stack([X - X, Y - Y, Z - Z], axis=0)
+    u[0] = X - X  # d psi3/dy - d psi2/dz = X - X
+    u[1] = Y - Y  # -(d psi3/dx - d psi1/dz) = -(Y - Y)
+    u[2] = Z - Z  # d psi2/dx - d psi1/dy = Z - Z
+    psi = np.stack([Y ** 2 * 0.5 + X * Z, X * Y - Z ** 2, X * Z + Y], axis=-1)
+    u = np.stack([1 - (-2 * Z), -(Z - X), Y - Y], axis=0)  # curl of psi
+    rhs = np.stack([0.3 + X, -0.2 * Y, 0.1 * Z, np.full(X.shape, 1e-3)], axis=-1)
+    sol = psi.copy()
+    vel = np.zeros((3, nz, ny, nx))
+    oracle.lib().orc_velocity(C.byref(g), 3, oracle._d(np.ascontiguousarray(rhs)), oracle._d(sol),
+                              oracle._d(vel[0]), oracle._d(vel[1]), oracle._d(vel[2]), 1)
+    inner = (slice(1, -1),) * 3
+    for m in range(3):
+        assert np.allclose(vel[m][inner], u[m][inner], rtol=0, atol=1e-12), m
+        assert np.all(vel[m][0] == 0) and np.all(vel[m][:, :, -1] == 0)
+    # stretching: SOL_m = sum_j d_j(w_j u_m), w = -RHS  (exact for products of linear fields)
+    w = -rhs[..., :3]
+    want = np.zeros((nz, ny, nx, 3))
+    for m in range(3):
+        want[..., m] = (np.gradient(w[..., 0] * u[m], x, axis=2, edge_order=2) +
+                        np.gradient(w[..., 1] * u[m], y, axis=1, edge_order=2) +
+                        np.gradient(w[..., 2] * u[m], z, axis=0, edge_order=2))
+    i2 = (slice(2, -2),) * 3
+    assert np.allclose(sol[i2], want[i2], rtol=0, atol=1e-11)
+    assert np.all(sol[1] == 0) and np.all(sol[:, :, -2] == 0)
+
+
+def test_m2p_reproduces_quadratics(oracle):
+    p = oracle.Params.make(0.1)
+    pts = np.array([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]])
+    g = oracle.define_sizes(p, pts)
+    nx, ny, nz = g.nn
+    x = g.xbound[0] + np.arange(nx) * g.dpm[0]
+    y = g.xbound[1] + np.arange(ny) * g.dpm[1]
+    z = g.xbound[2] + np.arange(nz) * g.dpm[2]
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    q = lambda a, b, c: 1 + a - 2 * b + 0.5 * c + a * b - c * c + 0.3 * a * a
+    vel = np.stack([q(X, Y, Z), q(Y, Z, X), q(Z, X, Y)], axis=0)
+    sol = np.stack([q(X, Z, Y), q(Z, Y, X), q(Y, X, Z)], axis=-1)
+    rng = np.random.default_rng(4)
+    n = 500
+    xp = rng.uniform(-0.9, 0.9, (n, 3))
+    qp = np.concatenate([np.zeros((n, 3)), rng.uniform(0.5, 2.0, (n, 1))], axis=1)
+    up, gp = np.zeros((n, 3)), np.zeros((n, 3))
+    oracle.lib().orc_m2p(C.byref(g), 3, 4, oracle._d(np.ascontiguousarray(sol)), oracle._d(vel[0]),
+                         oracle._d(vel[1]), oracle._d(vel[2]), oracle._d(xp), oracle._d(qp), oracle._d(up),
+                         oracle._d(gp), n)
+    a, b, c = xp[:, 0], xp[:, 1], xp[:, 2]
+    assert np.allclose(up, np.stack([q(a, b, c), q(b, c, a), q(c, a, b)], 1), rtol=0, atol=1e-12)
+    want_g = np.stack([q(a, c, b), q(c, b, a), q(b, a, c)], 1) * qp[:, 3:4]
+    assert np.allclose(gp, want_g, rtol=0, atol=1e-12)

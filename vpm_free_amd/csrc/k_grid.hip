@@ -149,8 +149,8 @@ __device__ __forceinline__ double tsc(double x) {
 #pragma clang fp contract(off)
     double a = fabs(x);
     if (a > 1.5) return 0.0;
-    if (a >= 0.5) return 0.5 * (-a + 1.5) * (-a + 1.5);
-    return 0.5 * (a + 1.5) * (a + 1.5) - 1.5 * (a + 0.5) * (a + 0.5);
+    if (a >= 0.5) return 0.5 * ((-a + 1.5) * (-a + 1.5));
+    return 0.5 * ((a + 1.5) * (a + 1.5)) - 1.5 * ((a + 0.5) * (a + 0.5));
 }
 
 // interp_stencil for every node of the six faces of the inner box, in face order.

@@ -17,10 +17,10 @@ __device__ __forceinline__ double proj_fun(int itype, double x) {
     double a = fabs(x), r = 0.0;
     if (itype == 4) {
         if (a < 1.0) r = 1.0 - 2.5 * (a * a) + 1.5 * (a * a * a);
-        else if (a <= 2.0) r = 0.5 * (2.0 - a) * (2.0 - a) * (1.0 - a);
+        else if (a <= 2.0) r = 0.5 * ((2.0 - a) * (2.0 - a)) * (1.0 - a);
     } else if (itype == 3) {
-        if (a < 0.5) r = 0.5 * (a + 1.5) * (a + 1.5) - 1.5 * (a + 0.5) * (a + 0.5);
-        else if (a <= 1.5) r = 0.5 * (-a + 1.5) * (-a + 1.5);
+        if (a < 0.5) r = 0.5 * ((a + 1.5) * (a + 1.5)) - 1.5 * ((a + 0.5) * (a + 0.5));
+        else if (a <= 1.5) r = 0.5 * ((-a + 1.5) * (-a + 1.5));
     } else if (itype == 2) {
         r = (a > 1.0) ? 0.0 : 1.0 - a;
     } else if (itype == 5) {

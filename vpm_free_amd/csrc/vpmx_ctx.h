@@ -65,8 +65,13 @@ struct Ctx : vpmx_ctx {
 };
 
 void free_fft_plans(Ctx *c);
+// Stage timing: the stream is drained before a start marker so that the marker's
+// timestamp is taken when the stage's first kernel is issued, not while earlier kernels
+// of the stream are still running (a marker does not wait for them to finish).
 inline void mark(Ctx *c, int stage, int end) {
-    if (c->ev_ok) (void)hipEventRecord(c->ev[2 * stage + end], c->stream);
+    if (!c->ev_ok) return;
+    if (!end) (void)hipStreamSynchronize(c->stream);
+    (void)hipEventRecord(c->ev[2 * stage + end], c->stream);
 }
 int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n);
 
