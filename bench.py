@@ -49,6 +49,23 @@ def workload(h):
     return cases.leapfrog_rings(h=h)
 
 
+def stencil_nodes(xp, g):
+    """Distinct grid nodes covered by the particles' 4x4x4 M4' stencils (nodes inode-1..inode+2)."""
+    n = g["nn"]
+    x0, h = np.array(g["xbound"][:3]), np.array(g["dpm"])
+    cell = np.floor((xp - x0) / h).astype(np.int64)  # inode - 1
+    occ = np.zeros((n[2], n[1], n[0]), dtype=bool)
+    ok = np.all((cell >= 0) & (cell < np.array(n)), axis=1)
+    c = cell[ok]
+    occ[c[:, 2], c[:, 1], c[:, 0]] = True
+    for ax in range(3):
+        out = occ.copy()
+        for s in (-1, 1, 2):
+            out |= np.roll(occ, s, axis=ax)
+        occ = out
+    return int(occ.sum())
+
+
 def cpu_baseline(h, xp, qp, stride):
     """Oracle (C/OpenMP restatement of the reference path) on the host cores: one
     vpm(WhatToDo=2) with the boundary sums evaluated on every stride-th boundary node,
@@ -112,6 +129,7 @@ def main():
 
     stage_tot = {}
     nparts = []
+    ctx.kernel_times(reset=True)
     t0 = time.perf_counter()
     for s in range(args.steps):
         ctx.step_rk2(args.dt, ntime=ntime)
@@ -127,10 +145,32 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step; particles {nparts[0]}..{nparts[-1]}")
     log("stage ms/step: " + ", ".join(f"{k}={v / args.steps:.2f}" for k, v in stage_tot.items()))
 
-    # dominant kernel: the fine-block FP64 boundary sum (one launch per vpm(2), two per step)
-    launches = 2 * args.steps
-    bs_ms = stage_tot["Bounds"] / launches
-    achieved = FLOPS_PER_PAIR * pairs_fine / (bs_ms * 1e-3) / 1e12
+    # dominant kernel: the fine-block FP64 boundary sum (one launch per vpm(2), two per step),
+    # timed by its dispatch's own start/stop events on the library's stream
+    kt = ctx.kernel_times()
+    bs_total, launches = kt["boundsum_fine"]
+    bs_ms = bs_total / launches
+    # FP64 work of one launch as the library executes it: 3 FMA per pair + the distinct 1/r
+    # table entries (20 flops each); the 20-flops-per-pair direct formulation is reported beside it
+    work = ctx.kernel_work()["boundsum_fine"]
+    achieved = work / (bs_ms * 1e-3) / 1e12
+    xf, _, _, _ = ctx.download()
+    G = g["nn"][0] * g["nn"][1] * g["nn"][2]
+    npart = float(np.mean(nparts))
+    T = stencil_nodes(xf, g)
+    hbm = {}
+    p2m_ms, p2m_n = kt["p2m"]
+    m2p_ms, m2p_n = kt["m2p"]
+    if p2m_n:
+        b = npart * 6 * 8 + G * 3 * 8
+        gbs = b / (p2m_ms / p2m_n * 1e-3) / 1e9
+        hbm["p2m"] = {"ms": round(p2m_ms / p2m_n, 4), "algorithmic_bytes": int(b), "GB/s": round(gbs, 1),
+                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if m2p_n:
+        b = npart * 4 * 8 + npart * 6 * 8 + T * 6 * 8
+        gbs = b / (m2p_ms / m2p_n * 1e-3) / 1e9
+        hbm["m2p"] = {"ms": round(m2p_ms / m2p_n, 4), "algorithmic_bytes": int(b), "stencil_nodes": int(T),
+                      "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 3),
@@ -138,8 +178,11 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
         "traffic": None,
-        "kernel": "k_boundsum<1> (FP64 direct Green's-function boundary sum, VALU)",
+        "kernel": "k_bs_lattice (FP64 Green's-function boundary sum, 1/r per lattice offset in LDS, VALU FMA)",
+        "flops_per_launch": work,
         "pairs_per_launch": pairs_fine,
+        "pairs_per_s": round(pairs_fine / (bs_ms * 1e-3), 1),
+        "direct_form_equiv_tflops": round(FLOPS_PER_PAIR * pairs_fine / (bs_ms * 1e-3) / 1e12, 2),
         "ms_per_launch": round(bs_ms, 3),
     }
     ctx.close()
@@ -176,6 +219,7 @@ def main():
         "diagnostics": {"circulation_drift": float(np.max(np.abs(circ1 - circ0))),
                         "impulse_rel_drift": float(np.linalg.norm(imp1 - imp0) / np.linalg.norm(imp0))},
         "roofline": roofline,
+        "hbm_kernels": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

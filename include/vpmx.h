@@ -143,8 +143,16 @@ int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double d
                            const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr,
                            int32_t ncomp, int32_t itype, int32_t idvpm, double *rhs);
 
-/* Timing of the last vpm/step call per stage (ms), names as the reference's timers. */
+/* Timing of the last vpm/step call per stage (ms, host clock between stream drains), in
+ * the order of the reference's timer labels: Proj, Poisson fine, Bounds, pmesh_coarse,
+ * pm_bc2, pmesh_final, fd, back, remeshing. */
 int32_t vpmx_stage_times(const vpmx_ctx *ctx, double *ms, int32_t n);
+/* Accumulated device time (ms, from each dispatch's start/stop events) and launch counts
+ * of the timed kernel classes: 0 fine-block boundary sum.  reset != 0 zeroes them. */
+int32_t vpmx_kernel_times(vpmx_ctx *ctx, double *ms_total, int64_t *launches, int32_t n, int32_t reset);
+/* Algorithmic work of the last launch of each timed class (FP64 flops for the boundary
+ * sum: 6 per source-target pair plus 20 per distinct 1/r evaluated). */
+int32_t vpmx_kernel_work(const vpmx_ctx *ctx, double *work, int32_t n);
 
 #ifdef __cplusplus
 }

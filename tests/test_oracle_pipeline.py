@@ -230,3 +230,16 @@ def test_create_without_gpu_fails_loudly(vpmlib):
         pass
     with pytest.raises(vpmlib.VpmError):
         vpmlib.VPM(vpmlib.Params.make(0.1))
+
+
+def test_io_formats_roundtrip(oracle, vpmlib, tmp_path):
+    from vpm_free_amd import io
+    rng = np.random.default_rng(1)
+    xp, w = rng.standard_normal((17, 3)), rng.standard_normal((17, 3))
+    io.write_particles_bin(tmp_path / "particles.bin", xp, w, 0.5 ** 3)
+    x2, w2, v = io.read_particles_bin(tmp_path / "particles.bin")
+    assert np.array_equal(x2, xp) and np.array_equal(w2, w) and v == 0.125
+    io.write_pm_inp(tmp_path / "pm.inp", 1 / 16, ncoarse=4, blocks=(1, 1, 1), ipmwrite=2, windows=[(5, 1), (9, 3)])
+    p = vpmlib.read_pm_inp(str(tmp_path / "pm.inp"))
+    assert list(p.dpm) == [1 / 16] * 3 and p.ncoarse == 4 and p.ipmwrite == 2
+    assert list(p.ipmwstart[:2]) == [5, 9] and list(p.ipmwsteps[:2]) == [1, 3]

@@ -77,6 +77,11 @@ def build_fortran():
     _run([FLANG, "-O2", "-fPIC", "-shared", "-module-dir", mod, "-o", out + ".tmp", src,
           "-L" + LIB, "-lvpmx", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
+    # example application (the reference driver's loop) linked against the drop-in
+    drv = os.path.join(LIB, "ring_driver")
+    _run([FLANG, "-O2", "-I", mod, "-o", drv + ".tmp", os.path.join(FSRC, "ring_driver.f90"),
+          "-L" + LIB, "-lvpm_lib", "-lvpmx", "-Wl,-rpath,$ORIGIN"])
+    os.replace(drv + ".tmp", drv)
     return out
 
 

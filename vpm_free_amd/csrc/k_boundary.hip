@@ -10,6 +10,8 @@
 // sources stream through LDS in tiles, and the source range is split over a second
 // grid dimension whose partial sums are added in a fixed order afterwards, so the
 // result is deterministic.  1/r is v_rsq_f64 refined by one third-order step.
+#include <hip/hip_ext.h>
+
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -237,6 +239,275 @@ __global__ void k_boundsum_exact(Grid g, int64_t nt, int64_t ns, const double *c
     for (int c = 0; c < 3; c++) sol.p[c * sol.cs + o] = acc[c];
 }
 
+// ---------------------------------------------------------------- lattice form
+// Sources (face-cell centres) and targets (face nodes) sit on regular lattices.  For a
+// target line and a source line running along the same axis C, the distance of a pair
+// depends only on the two line positions and on the index offset e = tc - sc along C:
+//   r^2 = K(line_t, line_s) + ((2e - 1) hC / 2)^2
+// (coordinates in half-cells: node i -> 2(i-1), cell j -> 2(j-1)+1).  Every pair's term
+// is still summed directly; a wave computes each distinct 1/r of its line once per
+// source line into an LDS table and reads it back for all pairs with that offset.
+struct LatPair {
+    int At, ht, As, hs;     // target / source face normal axes and hi flags
+    int C, Bt, Sa;          // line axis, target line-index axis, source line-index axis
+    int tlo[3], thi[3];     // target node ranges (1-based) on the target face
+    int nlines;             // target lines
+    int nsl;                // source lines
+    int nsc;                // sources per source line (cells along C)
+    int fs;                 // source face id 0..5 (partial-sum slice)
+    int64_t soff;           // index of the source face's first source
+    int64_t sstride_c;      // source index stride along C
+    int64_t sstride_l;      // source index stride along the source line axis
+    // resolved per pair on the host (no runtime-indexed arrays in the kernel)
+    int blo, clo, chi;      // first target line index, target range along C
+    int Pt, Ps, plane;      // plane coordinates (half-cells) and the target plane node index
+    double hA, hS, hC, hB;  // spacings along At, As, C, Bt
+};
+
+constexpr int LW = 4;  // waves (target lines) per work-group
+
+__device__ __forceinline__ int64_t target_index(const Grid &g, int face_axis, int hi, int i, int j, int k) {
+    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    const int64_t fX = (int64_t)NY * NZ, fY = (int64_t)(NX - 2) * NZ, fZ = (int64_t)(NX - 2) * (NY - 2);
+    if (face_axis == 0) return hi * fX + (int64_t)(k - 1) * NY + (j - 1);
+    if (face_axis == 1) return 2 * fX + hi * fY + (int64_t)(k - 1) * (NX - 2) + (i - 2);
+    return 2 * fX + 2 * fY + hi * fZ + (int64_t)(j - 2) * (NX - 2) + (i - 2);
+}
+
+template <int M>
+__global__ void __launch_bounds__(64 * LW) k_bs_lattice(Grid g, const LatPair *__restrict__ pairs, int nsp,
+                                                         int tabn, int nsplit, const double *__restrict__ a,
+                                                         int64_t ns, double *__restrict__ part, int64_t nt) {
+    extern __shared__ double lds[];
+    const LatPair P = pairs[blockIdx.y];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double *tab = lds + w * tabn;
+    double *aw = lds + LW * tabn;  // aw[3][nsp]
+    const int line = blockIdx.x * LW + w;
+    const bool active = line < P.nlines;
+    const int b = P.blo + line;
+    const int clo = P.clo, chi = P.chi;
+    const int emin = clo - nsp;
+    const double hA = P.hA, hS = P.hS, hC = P.hC;
+    const int Pt = P.Pt, Ps = P.Ps;
+    const int split = blockIdx.z;
+    const int l0 = (int)((int64_t)P.nsl * split / nsplit), l1 = (int)((int64_t)P.nsl * (split + 1) / nsplit);
+    double acc[M][3];
+#pragma unroll
+    for (int m = 0; m < M; m++) acc[m][0] = acc[m][1] = acc[m][2] = 0.0;
+    const int tc0 = clo + lane * M;
+    for (int sl = l0; sl < l1; sl++) {
+        const int aidx = sl + 1;  // source line index (cell) along the source line axis
+        __syncthreads();
+        // source weights of this line (zero-padded to nsp)
+        for (int q = threadIdx.x; q < nsp; q += 64 * LW) {
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+            if (q < P.nsc) {
+                int64_t s = P.soff + (int64_t)sl * P.sstride_l + (int64_t)q * P.sstride_c;
+                v0 = a[s];
+                v1 = a[ns + s];
+                v2 = a[2 * ns + s];
+            }
+            aw[q] = v0;
+            aw[nsp + q] = v1;
+            aw[2 * nsp + q] = v2;
+        }
+        // K for (target line b, source line aidx)
+        double d1, d2;
+        if (P.At != P.As) {
+            d1 = (double)(Pt - (2 * (aidx - 1) + 1)) * hA * 0.5;
+            d2 = (double)(2 * (b - 1) - Ps) * hS * 0.5;
+        } else {
+            d1 = (double)(Pt - Ps) * hA * 0.5;
+            d2 = (double)(2 * (b - 1) - (2 * (aidx - 1) + 1)) * P.hB * 0.5;
+        }
+        const double K = d1 * d1 + d2 * d2;
+        if (active)
+            for (int q = lane; q < tabn; q += 64) {
+                const int e = emin + q;
+                const double dc = (double)(2 * e - 1) * hC * 0.5;
+                tab[q] = rsqrt_f64(K + dc * dc);
+            }
+        __syncthreads();
+        if (!active) continue;
+        // sc runs over 1..nsp in blocks of M; window r[q] = tab[tc0 - s0 - (M-1) + q - emin]
+        for (int s0 = 1; s0 <= nsp; s0 += M) {
+            const int base = tc0 - s0 - (M - 1) - emin;
+            double r[2 * M - 1];
+#pragma unroll
+            for (int q = 0; q < 2 * M - 1; q++) r[q] = tab[base + q];
+#pragma unroll
+            for (int d = 0; d < M; d++) {
+                const double w0 = aw[s0 - 1 + d], w1 = aw[nsp + s0 - 1 + d], w2 = aw[2 * nsp + s0 - 1 + d];
+#pragma unroll
+                for (int m = 0; m < M; m++) {
+                    const double t = r[m - d + M - 1];
+                    acc[m][0] = __builtin_fma(w0, t, acc[m][0]);
+                    acc[m][1] = __builtin_fma(w1, t, acc[m][1]);
+                    acc[m][2] = __builtin_fma(w2, t, acc[m][2]);
+                }
+            }
+        }
+    }
+    if (!active) return;
+    double *pp = part + (int64_t)(P.fs * nsplit + split) * 3 * nt;
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        const int tc = tc0 + m;
+        if (tc > chi) continue;
+        const int pl = P.plane;
+        const int ii = P.At == 0 ? pl : (P.Bt == 0 ? b : tc);
+        const int jj = P.At == 1 ? pl : (P.Bt == 1 ? b : tc);
+        const int kk = P.At == 2 ? pl : (P.Bt == 2 ? b : tc);
+        const int64_t t = target_index(g, P.At, P.ht, ii, jj, kk);
+        pp[t] = acc[m][0];
+        pp[nt + t] = acc[m][1];
+        pp[2 * nt + t] = acc[m][2];
+    }
+}
+
+// SOL(target) = SOL(target) + sum over (source face, split) partials in a fixed order.
+__global__ void k_bs_lattice_reduce(Grid g, int64_t nt, int nslices, const double *part, GView sol) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int i, j, k, pl;
+    target_node(g, t, &i, &j, &k, &pl);
+    int64_t o = sol.at(i - 1, j - 1, k - 1);
+    for (int c = 0; c < 3; c++) {
+        double v = sol.p[c * sol.cs + o];
+        for (int s = 0; s < nslices; s++) v = v + part[(int64_t)s * 3 * nt + c * nt + t];
+        sol.p[c * sol.cs + o] = v;
+    }
+}
+
+static void make_pairs(const Grid &g, std::vector<LatPair> &out) {
+    const int N[3] = {g.n[0], g.n[1], g.n[2]};
+    // source face offsets in the reference order
+    int64_t fsz[3] = {(int64_t)(N[1] - 1) * (N[2] - 1), (int64_t)(N[0] - 1) * (N[2] - 1),
+                      (int64_t)(N[0] - 1) * (N[1] - 1)};
+    int64_t off[6];
+    int64_t o = 0;
+    for (int f = 0; f < 6; f++) {
+        off[f] = o;
+        o += fsz[f / 2];
+    }
+    for (int ft = 0; ft < 6; ft++)
+        for (int fs = 0; fs < 6; fs++) {
+            LatPair P{};
+            P.At = ft / 2; P.ht = ft & 1; P.As = fs / 2; P.hs = fs & 1;
+            if (P.At != P.As) {
+                P.C = 3 - P.At - P.As;
+                P.Bt = P.As;
+                P.Sa = P.At;
+            } else {
+                int u = P.At == 0 ? 1 : 0, v = P.At == 2 ? 1 : 2;
+                P.C = v;
+                P.Bt = u;
+                P.Sa = u;
+            }
+            // target ranges: X faces full; Y faces x in [2,NX-1]; Z faces x in [2,NX-1], y in [2,NY-1]
+            for (int d = 0; d < 3; d++) { P.tlo[d] = 1; P.thi[d] = N[d]; }
+            if (P.At >= 1) { P.tlo[0] = 2; P.thi[0] = N[0] - 1; }
+            if (P.At == 2) { P.tlo[1] = 2; P.thi[1] = N[1] - 1; }
+            P.tlo[P.At] = P.thi[P.At] = P.ht ? N[P.At] : 1;
+            P.nlines = P.thi[P.Bt] - P.tlo[P.Bt] + 1;
+            P.nsl = N[P.Sa] - 1;
+            P.nsc = N[P.C] - 1;
+            P.fs = fs;
+            P.soff = off[fs];
+            // source in-plane axes p < q: index = (q_cell-1)*(N_p-1) + (p_cell-1)
+            int p = P.As == 0 ? 1 : 0, q = P.As == 2 ? 1 : 2;
+            int64_t sp = 1, sq = N[p] - 1;
+            P.sstride_c = (P.C == p) ? sp : sq;
+            P.sstride_l = (P.Sa == p) ? sp : sq;
+            P.blo = P.tlo[P.Bt];
+            P.clo = P.tlo[P.C];
+            P.chi = P.thi[P.C];
+            P.Pt = P.ht ? 2 * (N[P.At] - 1) : 0;
+            P.Ps = P.hs ? 2 * (N[P.As] - 1) : 0;
+            P.plane = P.ht ? N[P.At] : 1;
+            P.hA = g.h[P.At];
+            P.hS = g.h[P.As];
+            P.hC = g.h[P.C];
+            P.hB = g.h[P.Bt];
+            out.push_back(P);
+        }
+}
+
+template <int M>
+static int launch_lattice(Ctx *c, const Grid &g, const LatPair *pairs_d, int maxlines, int nsp, int tabn, int nsplit,
+                          const double *a, int64_t ns, double *part, int64_t nt, bool fine) {
+    size_t lds = sizeof(double) * ((size_t)LW * tabn + 3 * (size_t)nsp);
+    dim3 grid((unsigned)((maxlines + LW - 1) / LW), 36, (unsigned)nsplit);
+    if (fine) {
+        hipExtLaunchKernelGGL(k_bs_lattice<M>, grid, dim3(64 * LW), (uint32_t)lds, c->stream, c->kev0[K_BOUNDSUM],
+                              c->kev1[K_BOUNDSUM], 0, g, pairs_d, nsp, tabn, nsplit, a, ns, part, nt);
+        kernel_timed(c, K_BOUNDSUM);
+    } else {
+        hipLaunchKernelGGL(k_bs_lattice<M>, grid, dim3(64 * LW), lds, c->stream, g, pairs_d, nsp, tabn, nsplit, a, ns,
+                           part, nt);
+    }
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+static int boundary_sums_lattice(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, bool fine) {
+    int st = VPMX_OK;
+    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    int64_t nt = 2 * ((int64_t)NY * NZ + (int64_t)(NX - 2) * NZ + (int64_t)(NX - 2) * (NY - 2));
+    int nmax = NX > NY ? NX : NY;
+    if (NZ > nmax) nmax = NZ;
+    const int M = (nmax + 63) / 64;
+    if (M > 8) return fail(c, VPMX_ERR_UNSUP, "lattice boundary sums: faces longer than 512 nodes");
+    std::vector<LatPair> pairs;
+    make_pairs(g, pairs);
+    int maxlines = 0, maxsc = 0;
+    for (auto &P : pairs) {
+        if (P.nlines > maxlines) maxlines = P.nlines;
+        if (P.nsc > maxsc) maxsc = P.nsc;
+    }
+    const int nsp = (maxsc + M - 1) / M * M;
+    const int tabn = 64 * M + nsp;
+    size_t lds = sizeof(double) * ((size_t)LW * tabn + 3 * (size_t)nsp);
+    if (lds > 65536) return fail(c, VPMX_ERR_UNSUP, "lattice boundary sums: LDS table too large");
+    // enough work-groups to fill the device several times over
+    int64_t base_wg = (int64_t)((maxlines + LW - 1) / LW) * 36;
+    int nsplit = (int)((4096 + base_wg - 1) / base_wg);
+    if (nsplit < 1) nsplit = 1;
+    if (nsplit > 8) nsplit = 8;
+    LatPair *pd = (LatPair *)ensure(c, fine ? c->lat_pairs : c->lat_pairs_c, sizeof(LatPair) * pairs.size(), &st);
+    if (st) return st;
+    VPMX_HIP(c, hipMemcpyAsync(pd, pairs.data(), sizeof(LatPair) * pairs.size(), hipMemcpyHostToDevice, c->stream));
+    double *part = (double *)ensure(c, c->bs_part, sizeof(double) * 3 * (size_t)nt * 6 * nsplit, &st);
+    if (st) return st;
+    if (fine) {
+        // FP64 work of the launch: 3 FMA (6 flops) per source-target pair, plus 20 flops per
+        // table entry (r^2, v_rsq_f64 and its refinement) -- every wave builds one table of
+        // tabn entries per (target line, source line)
+        double tables = 0.0;
+        for (auto &P : pairs) tables += (double)P.nlines * P.nsl * tabn;
+        c->k_work[K_BOUNDSUM] = 6.0 * (double)nt * (double)ns + 20.0 * tables;
+        mark(c, ST_BOUNDS, 0);
+    }
+    switch (M) {
+        case 1: st = launch_lattice<1>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 2: st = launch_lattice<2>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 3: st = launch_lattice<3>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 4: st = launch_lattice<4>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 5: st = launch_lattice<5>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 6: st = launch_lattice<6>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        case 7: st = launch_lattice<7>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+        default: st = launch_lattice<8>(c, g, pd, maxlines, nsp, tabn, nsplit, a, ns, part, nt, fine); break;
+    }
+    if (st) return st;
+    if (fine) mark(c, ST_BOUNDS, 1);
+    hipLaunchKernelGGL(k_bs_lattice_reduce, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, g, nt,
+                       6 * nsplit, part, sol);
+    VPMX_HIP(c, hipGetLastError());
+    if (fine) c->last_pairs += (double)nt * (double)ns;
+    return VPMX_OK;
+}
+
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *sig, const double *cxyz,
                   const double *wgt, GView sol, bool fine) {
     if (ncomp != 3) return fail(c, VPMX_ERR_UNSUP, "boundary sums: 3 components");
@@ -252,6 +523,7 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
         VPMX_HIP(c, hipGetLastError());
         return VPMX_OK;
     }
+    if (c->bs_mode == 0) return boundary_sums_lattice(c, g, ns, a, sol, fine);
     int64_t tblocks = (nt + BS_THREADS * BS_TPT - 1) / (BS_THREADS * BS_TPT);
     // split sources so the launch has >= ~2048 work-groups, each with >= 1024 sources
     int64_t nsplit = (2048 + tblocks - 1) / tblocks;
@@ -267,8 +539,9 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
     if (st) return st;
     if (fine) {
         mark(c, ST_BOUNDS, 0);
-        hipLaunchKernelGGL(k_boundsum<1>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0, c->stream, g,
-                           nt, ns, chunk, cxyz, a, part);
+        hipExtLaunchKernelGGL(k_boundsum<1>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0, c->stream,
+                              c->kev0[K_BOUNDSUM], c->kev1[K_BOUNDSUM], 0, g, nt, ns, chunk, cxyz, a, part);
+        kernel_timed(c, K_BOUNDSUM);
         mark(c, ST_BOUNDS, 1);
     } else {
         hipLaunchKernelGGL(k_boundsum<0>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0, c->stream, g,

@@ -6,6 +6,8 @@
 // kept in particle order), and every node sums the particles of the 4x4x4 cells whose
 // stencil covers it, so no floating-point atomics are used and the result does not
 // depend on scheduling.
+#include <hip/hip_ext.h>
+
 #include "vpmx_internal.h"
 #include "vpmx_ctx.h"
 
@@ -234,11 +236,16 @@ __global__ void __launch_bounds__(256) k_p2m_gather(Grid g, const int *start, co
 }
 
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
-               int itype, double div, GView out, const int vn[3], const int wlo[3]) {
+               int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass) {
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
-    if (ncomp == 3)
+    if (ncomp == 3 && kclass >= 0) {
+        hipExtLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, c->kev0[kclass], c->kev1[kclass], 0,
+                              g, start, perm, s.cn[0], s.cn[1], xp, qp, qstride, itype, div, out, vn[0], vn[1], vn[2],
+                              wlo[0], wlo[1], wlo[2]);
+        kernel_timed(c, kclass);
+    } else if (ncomp == 3)
         hipLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1], xp,
                            qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
     else if (ncomp == 4)
@@ -298,8 +305,9 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
 int m2p_gather(Ctx *c, const Grid &g, const SortState &, const double *xp, const double *qp, int qstride, int itype,
                GView vel, GView sol, double *up, double *gp, int64_t n) {
     if (n <= 0) return VPMX_OK;
-    hipLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, xp, qp, qstride, itype,
-                       vel, sol, up, gp, n);
+    hipExtLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->kev0[K_M2P],
+                          c->kev1[K_M2P], 0, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
+    kernel_timed(c, K_M2P);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

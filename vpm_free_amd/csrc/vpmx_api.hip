@@ -8,7 +8,9 @@
 //   calc_velocity_serial_3d(1), back_to_particles_3D(itype=1)
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -80,15 +82,11 @@ static void window_h(const double x0[3], const double dpm[3], const int32_t win[
 }
 
 // ---------------------------------------------------------------- timing
-static void collect_times(Ctx *c, const bool *used) {
-    if (!c->ev_ok) return;
-    (void)hipStreamSynchronize(c->stream);
-    for (int s = 0; s < ST_COUNT; s++) {
-        if (!used[s]) continue;
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]) == hipSuccess) c->stage_ms[s] += ms;
-    }
+double wall_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
+static void collect_times(Ctx *c, const bool *) { (void)hipStreamSynchronize(c->stream); }
 
 // ---------------------------------------------------------------- pmesh (iynbc=1)
 // pmesh(iynbc=1, ibctyp=2) on grid g: SOL_0 = zero-BC solve, sources from its normal
@@ -237,7 +235,8 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
     VPMX_TRY(sort_particles(c, gg, xp, n, c->sort));
     const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
     int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
-    VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo));
+    VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo,
+                        K_P2M));
     mark(c, ST_PROJ, 1);
     used[ST_PROJ] = true;
     VPMX_TRY(yaps_single(c, used));
@@ -276,6 +275,7 @@ static int run_remesh(Ctx *c, const double *xp, const double *qp, int64_t n, int
     if (P.ncell_rem != 1) return fail(c, VPMX_ERR_UNSUP, "ncell_rem must be 1");
     if (mrem < 1) return fail(c, VPMX_ERR_ARG, "mrem must be >= 1");
     bool used[ST_COUNT] = {};
+    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = 0.0;
     mark(c, ST_REMESH, 0);
     int32_t NN[3], NNb[6];
     double D[3];
@@ -360,9 +360,11 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
         delete c;
         return VPMX_ERR_HIP;
     }
-    c->ev_ok = true;
-    for (int i = 0; i < 2 * ST_COUNT; i++)
-        if (hipEventCreate(&c->ev[i]) != hipSuccess) c->ev_ok = false;
+    c->kev_ok = true;
+    const char *bm = getenv("VPMX_BOUNDSUM");
+    if (bm && std::string(bm) == "direct") c->bs_mode = 1;
+    for (int k = 0; k < K_COUNT; k++)
+        if (hipEventCreate(&c->kev0[k]) != hipSuccess || hipEventCreate(&c->kev1[k]) != hipSuccess) c->kev_ok = false;
     *out = c;
     return VPMX_OK;
 }
@@ -373,16 +375,16 @@ void vpmx_destroy(vpmx_ctx *p) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->xp, &c->qp, &c->up, &c->gp, &c->x0, &c->q0, &c->rhs_bl, &c->sol_bl, &c->sol0, &c->vel,
-                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->samp, &c->rhs_c, &c->sol_c,
+                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->samp, &c->rhs_c, &c->sol_c,
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill};
     for (DBuf *b : bufs) release(*b);
     free_fft_plans(c);
-    for (int i = 0; i < 2 * ST_COUNT; i++)
-        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-    for (int k = 0; k < K_COUNT; k++)
-        for (hipEvent_t e : c->kev[k]) (void)hipEventDestroy(e);
+    for (int k = 0; k < K_COUNT; k++) {
+        if (c->kev0[k]) (void)hipEventDestroy(c->kev0[k]);
+        if (c->kev1[k]) (void)hipEventDestroy(c->kev1[k]);
+    }
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -614,6 +616,28 @@ int32_t vpmx_diagnostics(vpmx_ctx *p, double circ[3], double imp[3]) {
         return VPMX_OK;
     }
     return diagnostics(c, (const double *)c->xp.p, (const double *)c->qp.p, c->neqpm + 1, c->np, circ, imp);
+}
+
+int32_t vpmx_kernel_times(vpmx_ctx *p, double *ms_total, int64_t *launches, int32_t n, int32_t reset) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    for (int k = 0; k < n && k < K_COUNT; k++) {
+        if (ms_total) ms_total[k] = c->k_ms[k];
+        if (launches) launches[k] = c->k_launches[k];
+    }
+    if (reset)
+        for (int k = 0; k < K_COUNT; k++) {
+            c->k_ms[k] = 0.0;
+            c->k_launches[k] = 0;
+        }
+    return VPMX_OK;
+}
+
+int32_t vpmx_kernel_work(const vpmx_ctx *p, double *work, int32_t n) {
+    if (!p || !work) return VPMX_ERR_ARG;
+    const Ctx *c = C(p);
+    for (int k = 0; k < n && k < K_COUNT; k++) work[k] = c->k_work[k];
+    return VPMX_OK;
 }
 
 int32_t vpmx_stage_times(const vpmx_ctx *p, double *ms, int32_t n) {

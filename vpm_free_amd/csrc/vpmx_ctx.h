@@ -41,7 +41,8 @@ struct Ctx : vpmx_ctx {
 
     // grids
     DBuf rhs_bl, sol_bl, sol0, vel, str;
-    DBuf src_sig, src_cxyz, src_w, bs_part;
+    DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c;
+    int bs_mode = 0;  // boundary sums: 0 lattice tables (default), 1 direct LDS source tiles
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
@@ -54,24 +55,37 @@ struct Ctx : vpmx_ctx {
     int64_t rm_count = -1;
     double *rm_xp = nullptr, *rm_qp = nullptr;
 
-    // timing
-    hipEvent_t ev[2 * ST_COUNT] = {};
-    bool ev_ok = false;
+    // timing: stages by host wall clock between stream drains; kernels of class K_* by
+    // the dispatch's own start/stop events (hipExtLaunchKernel)
+    bool timing = true;
+    double stage_t0[ST_COUNT] = {};
     double stage_ms[ST_COUNT] = {};
     double last_pairs = 0.0;
-    bool profiling = false;
-    std::vector<hipEvent_t> kev[K_COUNT];
-    size_t kev_used[K_COUNT] = {};
+    hipEvent_t kev0[K_COUNT] = {}, kev1[K_COUNT] = {};
+    bool kev_ok = false;
+    double k_ms[K_COUNT] = {};
+    int64_t k_launches[K_COUNT] = {};
+    double k_work[K_COUNT] = {};  // algorithmic work of the last timed launch (flops or bytes)
 };
 
 void free_fft_plans(Ctx *c);
-// Stage timing: the stream is drained before a start marker so that the marker's
-// timestamp is taken when the stage's first kernel is issued, not while earlier kernels
-// of the stream are still running (a marker does not wait for them to finish).
+double wall_ms();
+// Stage timing: drain the stream at both ends so the interval covers exactly the stage's
+// kernels (enqueueing is asynchronous).
 inline void mark(Ctx *c, int stage, int end) {
-    if (!c->ev_ok) return;
-    if (!end) (void)hipStreamSynchronize(c->stream);
-    (void)hipEventRecord(c->ev[2 * stage + end], c->stream);
+    if (!c->timing) return;
+    (void)hipStreamSynchronize(c->stream);
+    if (!end) c->stage_t0[stage] = wall_ms();
+    else c->stage_ms[stage] += wall_ms() - c->stage_t0[stage];
+}
+// Add the duration of the last timed dispatch of class k (waits for it).
+inline void kernel_timed(Ctx *c, int k) {
+    if (!c->kev_ok) return;
+    float ms = 0.f;
+    if (hipEventSynchronize(c->kev1[k]) == hipSuccess && hipEventElapsedTime(&ms, c->kev0[k], c->kev1[k]) == hipSuccess) {
+        c->k_ms[k] += ms;
+        c->k_launches[k] += 1;
+    }
 }
 int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n);
 

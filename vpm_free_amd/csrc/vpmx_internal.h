@@ -64,7 +64,7 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
 // [win0, win1) in each dim) = sum_p W * qp(c, p) / div for c < ncomp; nodes of the view
 // outside the window are written 0.
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
-               int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3]);
+               int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass = -1);
 int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
                int itype, GView vel, GView sol, double *up, double *gp, int64_t n);
 int euler_update(Ctx *c, double *xp, double *qp, int qstride, const double *up, const double *gp, int64_t n,

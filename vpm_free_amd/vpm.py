@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libvpmx.so")
 MAXB = 64
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_UNSUP, ERR_IO, ERR_NODEV = range(7)
 STAGES = ["Proj", "Poisson fine", "Bounds", "pmesh_coarse", "pm_bc2", "pmesh_final", "fd", "back", "remeshing"]
+KERNELS = ["boundsum_fine", "p2m", "m2p", "dst"]
 
 
 class VpmError(RuntimeError):
@@ -99,6 +100,8 @@ def load_library():
         "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
         "vpmx_stage_project": [vp, d, d, C.POINTER(i32), C.POINTER(i32), d, d, i64, i32, i32, i32, d],
         "vpmx_stage_times": [vp, d, i32],
+        "vpmx_kernel_times": [vp, d, C.POINTER(i64), i32, i32],
+        "vpmx_kernel_work": [vp, d, i32],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -242,6 +245,19 @@ class VPM:
         ms = np.zeros(len(STAGES))
         self._check(self._lib.vpmx_stage_times(self._h, _dp(ms), len(STAGES)))
         return dict(zip(STAGES, ms.tolist()))
+
+    def kernel_times(self, reset=False):
+        """{class: (total device ms, launches)} from the dispatches' own events."""
+        ms = np.zeros(len(KERNELS))
+        n = (C.c_int64 * len(KERNELS))()
+        self._check(self._lib.vpmx_kernel_times(self._h, _dp(ms), n, len(KERNELS), 1 if reset else 0))
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
+
+    def kernel_work(self):
+        """{class: algorithmic work of its last launch} (FP64 flops for boundsum_fine)."""
+        w = np.zeros(len(KERNELS))
+        self._check(self._lib.vpmx_kernel_work(self._h, _dp(w), len(KERNELS)))
+        return dict(zip(KERNELS, w.tolist()))
 
     # -- device-resident particles -------------------------------------------------
     def upload(self, XP, QP):
