@@ -164,3 +164,19 @@ def test_rk2_step_and_diagnostics(vpmlib, oracle):
     circ_r, imp_r = oracle.diagnostics(x1, q1)
     assert np.allclose(circ, circ_r, rtol=0, atol=1e-13)
     assert rel_l2(imp, imp_r) <= 1e-12
+
+
+@pytest.mark.parametrize("nn", [(35, 23, 27), (38, 12, 20), (47, 45, 13)])
+def test_solve_all_radix_classes(ctx, oracle, nn):
+    """DST lengths 2(m+1) with prime factors 17, 11, 13, 37, 5, 23 and 3 (every radix class)."""
+    rng = np.random.default_rng(sum(nn))
+    nn = np.array(nn, np.int32)
+    xb = np.array([0.3, -0.2, 0.1, 0, 0, 0])
+    dpm = np.array([0.05, 0.04, 0.06])
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    sol = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    win = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
+    say("case: solve radix classes", nn.tolist())
+    ref = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
+    got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, sol=sol)
+    assert rel_l2(got, ref) <= TOL

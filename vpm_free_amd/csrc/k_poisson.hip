@@ -228,6 +228,263 @@ __global__ void __launch_bounds__(256) k_dst_pass(DstArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ single-buffer passes
+// Same transform as k_dst_pass with one LDS buffer: every stage loads each butterfly's R
+// inputs into registers, synchronises, and writes the outputs to their Stockham
+// positions.  Lines are mapped to threads without 64-bit divisions, and the number of
+// lines per work-group is a power of two.
+constexpr int DST_EMAX = 2304;  // complex line elements held in LDS per work-group (36 KB)
+constexpr int DST_LDS_MAX = 4096;  // complex elements of LDS per work-group, lines + twiddles (64 KB)
+constexpr int DST_THREADS = 512;
+
+// In-register butterflies of the specialised radices (2, 3, 4).
+template <int R>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const double c3 = -0.5, s3 = 0.86602540378443864676;
+        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        double2 base = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+        double2 rot = make_double2(s3 * t2.y, -s3 * t2.x);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(base, rot);
+        v[2] = csub(base, rot);
+    } else if constexpr (R == 4) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]), s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        double2 md = make_double2(d13.y, -d13.x);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, md);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, md);
+    }
+}
+
+// One Stockham stage in place: inputs to registers, barrier, outputs to LDS, barrier.
+// Radices above 4 form each output after the barrier (v holds the inputs) so that only
+// the R inputs are live.
+template <int R>
+__device__ void stage_reg(double2 *buf, const double2 *__restrict__ tw, const DstArgs &a, int Ns) {
+    constexpr int BPT = (DST_EMAX / R + DST_THREADS - 1) / DST_THREADS;
+    const int L = a.L, nbf = L / R, total = a.npairs * nbf, twstep = L / (Ns * R);
+    double2 v[BPT][R];
+    int dst[BPT];
+    double2 wr[R > 4 ? R : 1];  // exp(-2 pi i q / R)
+    if constexpr (R > 4) {
+#pragma unroll
+        for (int q = 0; q < R; q++) wr[q] = tw[q * (L / R)];
+    }
+#pragma unroll
+    for (int bi = 0; bi < BPT; bi++) {
+        const int t = threadIdx.x + bi * DST_THREADS;
+        dst[bi] = -1;
+        if (t < total) {
+            const int pair = t / nbf, j = t - pair * nbf;
+            const int k0 = j % Ns;
+            const double2 *src = buf + pair * L;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[bi][r] = src[j + r * nbf];
+            if (k0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[bi][r] = cmul(v[bi][r], tw[r * k0 * twstep]);
+            }
+            if constexpr (R <= 4) dft_reg<R>(v[bi]);
+            dst[bi] = pair * L + (j - k0) * R + k0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < BPT; bi++)
+        if (dst[bi] >= 0) {
+            if constexpr (R <= 4) {
+#pragma unroll
+                for (int k = 0; k < R; k++) buf[dst[bi] + k * Ns] = v[bi][k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    double2 acc = v[bi][0];
+#pragma unroll
+                    for (int r = 1; r < R; r++) acc = cadd(acc, cmul(v[bi][r], wr[(r * k) % R]));
+                    buf[dst[bi] + k * Ns] = acc;
+                }
+            }
+        }
+    __syncthreads();
+}
+
+// MAXR bounds the radices compiled into a kernel, so the registers of the common 2/3/4
+// stages are not sized for a large prime stage.
+template <int MAXR>
+__device__ void fft_reg(double2 *buf, const double2 *tw, const DstArgs &a) {
+    int Ns = 1;
+    for (int s = 0; s < a.nstages; s++) {
+        const int R = a.radix[s];
+        if (R == 4) stage_reg<4>(buf, tw, a, Ns);
+        else if (R == 2) stage_reg<2>(buf, tw, a, Ns);
+        else if (R == 3) stage_reg<3>(buf, tw, a, Ns);
+        else if constexpr (MAXR >= 7) {
+            if (R == 5) stage_reg<5>(buf, tw, a, Ns);
+            else if (R == 7) stage_reg<7>(buf, tw, a, Ns);
+            else if constexpr (MAXR >= 17) {
+                if (R == 11) stage_reg<11>(buf, tw, a, Ns);
+                else if (R == 13) stage_reg<13>(buf, tw, a, Ns);
+                else if (R == 17) stage_reg<17>(buf, tw, a, Ns);
+                else if constexpr (MAXR >= 37) {
+                    if (R == 19) stage_reg<19>(buf, tw, a, Ns);
+                    else if (R == 23) stage_reg<23>(buf, tw, a, Ns);
+                    else if (R == 29) stage_reg<29>(buf, tw, a, Ns);
+                    else if (R == 31) stage_reg<31>(buf, tw, a, Ns);
+                    else stage_reg<37>(buf, tw, a, Ns);
+                }
+            }
+        }
+        Ns *= R;
+    }
+}
+
+template <int MODE, int MAXR>
+__global__ void __launch_bounds__(DST_THREADS) k_dst_pass2(DstArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    __shared__ double lamxy[16];
+    const int L = a.L, m = a.m, np = a.npairs, nl = 2 * np;
+    double2 *A = lds;
+    double2 *twl = lds + np * L;  // twiddle table exp(-2 pi i t / L) in LDS
+    const int comp = blockIdx.y;
+    const int64_t l0 = (int64_t)blockIdx.x * nl;
+    double *w = a.w + comp * a.wcs;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < L; t += DST_THREADS) twl[t] = a.tw[t];
+    for (int t = tid; t < np; t += DST_THREADS) {
+        A[t * L] = make_double2(0.0, 0.0);
+        A[t * L + m + 1] = make_double2(0.0, 0.0);
+    }
+    if (MODE == DST_Z && tid < nl) {
+        const int64_t gl = l0 + tid;
+        double v = 0.0;
+        if (gl < a.nlines) {
+            const int ix = (int)(gl % a.mx), iy = (int)(gl / a.mx);
+            v = a.lam[0][ix] + a.lam[1][iy];
+        }
+        lamxy[tid] = v;
+    }
+    double *Ad = (double *)A;
+    if (MODE == DST_XF || MODE == DST_XI) {
+        for (int line = 0; line < nl; line++) {
+            const int64_t gl = l0 + line;
+            const int pair = line >> 1, ri = line & 1;
+            if (gl >= a.nlines) {
+                for (int pt = tid; pt < m; pt += DST_THREADS) {
+                    Ad[2 * (pair * L + pt + 1) + ri] = 0.0;
+                    Ad[2 * (pair * L + L - pt - 1) + ri] = -0.0;
+                }
+                continue;
+            }
+            const int j = (int)(gl % a.my), k = (int)(gl / a.my);
+            if (MODE == DST_XF) {
+                const double *rp = a.rhs.p + comp * a.rhs.cs;
+                const double *bp = a.bc.p + comp * a.bc.cs;
+                for (int pt = tid; pt < m; pt += DST_THREADS) {
+                    double v = rp[a.rhs.at(pt + 1, j + 1, k + 1)];
+                    if (a.has_bc) {
+                        if (pt == 0) v -= bp[a.bc.at(0, j + 1, k + 1)] * a.ih2[0];
+                        if (pt == a.mx - 1) v -= bp[a.bc.at(a.mx + 1, j + 1, k + 1)] * a.ih2[0];
+                        if (j == 0) v -= bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
+                        if (j == a.my - 1) v -= bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
+                        if (k == 0) v -= bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
+                        if (k == a.mz - 1) v -= bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
+                    }
+                    Ad[2 * (pair * L + pt + 1) + ri] = v;
+                    Ad[2 * (pair * L + L - pt - 1) + ri] = -v;
+                }
+            } else {
+                const double *src = w + gl * a.mx;
+                for (int pt = tid; pt < m; pt += DST_THREADS) {
+                    const double v = src[pt];
+                    Ad[2 * (pair * L + pt + 1) + ri] = v;
+                    Ad[2 * (pair * L + L - pt - 1) + ri] = -v;
+                }
+            }
+        }
+    } else {
+        const int line = tid & (nl - 1);
+        const int pair = line >> 1, ri = line & 1;
+        const int64_t gl = l0 + line;
+        const bool ok = gl < a.nlines;
+        int64_t base = 0, stride;
+        if (MODE == DST_Y) {
+            stride = a.mx;
+            if (ok) base = (gl % a.mx) + (int64_t)a.mx * a.my * (gl / a.mx);
+        } else {
+            stride = (int64_t)a.mx * a.my;
+            base = gl;
+        }
+        const int step = DST_THREADS / nl;
+        for (int pt = tid / nl; pt < m; pt += step) {
+            const double v = ok ? w[base + pt * stride] : 0.0;
+            Ad[2 * (pair * L + pt + 1) + ri] = v;
+            Ad[2 * (pair * L + L - pt - 1) + ri] = -v;
+        }
+    }
+    __syncthreads();
+    fft_reg<MAXR>(A, twl, a);
+    if (MODE == DST_Z) {
+        // S_k = (-Im Z_k / 2, Re Z_k / 2) scaled by norm / -(lx+ly+lz), odd extension again (in place:
+        // entry k is read and rewritten by one thread, entries L-k are not read)
+        for (int t = tid; t < np * (m + 2); t += DST_THREADS) {
+            const int pair = t / (m + 2), k = t - pair * (m + 2);
+            if (k == 0 || k == m + 1) {
+                A[pair * L + k] = make_double2(0.0, 0.0);
+                continue;
+            }
+            const double2 z = A[pair * L + k];
+            const double lz = a.lam[2][k - 1];
+            const double fa = a.norm / (-(lamxy[2 * pair] + lz)), fb = a.norm / (-(lamxy[2 * pair + 1] + lz));
+            const double2 v = make_double2(-0.5 * z.y * fa, 0.5 * z.x * fb);
+            A[pair * L + k] = v;
+            A[pair * L + L - k] = make_double2(-v.x, -v.y);
+        }
+        __syncthreads();
+        fft_reg<MAXR>(A, twl, a);
+    }
+    if (MODE == DST_XF || MODE == DST_XI) {
+        for (int line = 0; line < nl; line++) {
+            const int64_t gl = l0 + line;
+            if (gl >= a.nlines) break;
+            const int pair = line >> 1;
+            const int j = (int)(gl % a.my), k = (int)(gl / a.my);
+            for (int pt = tid; pt < m; pt += DST_THREADS) {
+                const double2 z = A[pair * L + pt + 1];
+                const double v = (line & 1) ? 0.5 * z.x : -0.5 * z.y;
+                if (MODE == DST_XI) a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = v * a.scale;
+                else w[gl * a.mx + pt] = v * a.scale;
+            }
+        }
+    } else {
+        const int line = tid & (nl - 1);
+        const int pair = line >> 1;
+        const int64_t gl = l0 + line;
+        if (gl < a.nlines) {
+            int64_t base, stride;
+            if (MODE == DST_Y) {
+                stride = a.mx;
+                base = (gl % a.mx) + (int64_t)a.mx * a.my * (gl / a.mx);
+            } else {
+                stride = (int64_t)a.mx * a.my;
+                base = gl;
+            }
+            const int step = DST_THREADS / nl;
+            for (int pt = tid / nl; pt < m; pt += step) {
+                const double2 z = A[pair * L + pt + 1];
+                const double v = (line & 1) ? 0.5 * z.x : -0.5 * z.y;
+                w[base + pt * stride] = v * a.scale;
+            }
+        }
+    }
+}
+
 // Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
 __global__ void k_zero_faces(GView out, int nx, int ny, int nz, int ncomp) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,28 +545,69 @@ void free_fft_plans(Ctx *c) {
         delete p;
     }
     c->fft_plans.clear();
+    for (auto &kv : c->eig_cache) (void)hipFree(kv.second);
+    c->eig_cache.clear();
 }
 
-static int eigen(Ctx *c, int m, double h, DBuf &buf, const double **out) {
-    int st = VPMX_OK;
-    double *d = (double *)ensure(c, buf, sizeof(double) * (size_t)m, &st);
-    if (st) return st;
+// Eigenvalues 4 sin^2(pi k / (2(m+1))) / h^2 of the 1-D Dirichlet second difference,
+// uploaded once per (m, h) and kept for the context's lifetime.
+static int eigen(Ctx *c, int m, double h, DBuf &, const double **out) {
+    auto key = std::make_pair(m, h);
+    auto it = c->eig_cache.find(key);
+    if (it != c->eig_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
     std::vector<double> v(m);
     for (int k = 1; k <= m; k++) {
         double s = std::sin(M_PI * (double)k / (2.0 * (double)(m + 1)));
         v[k - 1] = 4.0 * s * s / (h * h);
     }
-    VPMX_HIP(c, hipMemcpyAsync(d, v.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->stream));
-    VPMX_HIP(c, hipStreamSynchronize(c->stream));
-    *out = d;
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * m));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    c->eig_cache[key] = d;
+    *out = (const double *)d;
     return VPMX_OK;
+}
+
+// Largest radix of the current plan, rounded to the radix class of k_dst_pass2 (0: the
+// plan needs the two-buffer kernel).
+static int radix_class(const DstArgs &a) {
+    int mx = 0;
+    for (int s = 0; s < a.nstages; s++) mx = a.radix[s] > mx ? a.radix[s] : mx;
+    if (a.L > DST_EMAX || 2 * a.L > DST_LDS_MAX) return 0;  // lines + twiddle table must fit
+    if (mx <= 4) return 4;
+    if (mx <= 7) return 7;
+    if (mx <= 17) return 17;
+    if (mx <= 37) return 37;
+    return 0;
 }
 
 template <int MODE>
 static int launch_pass(Ctx *c, DstArgs a, int ncomp) {
-    size_t lds = sizeof(double2) * 2 * (size_t)a.npairs * a.L;
-    unsigned nblk = (unsigned)((a.nlines + 2 * a.npairs - 1) / (2 * a.npairs));
-    hipLaunchKernelGGL(k_dst_pass<MODE>, dim3(nblk, ncomp), dim3(256), lds, c->stream, a);
+    const int rc = c->dst_v1 ? 0 : radix_class(a);
+    if (rc == 0) {
+        int np = (int)(65536 / (sizeof(double2) * 2 * (size_t)a.L));
+        a.npairs = np > 8 ? 8 : (np < 1 ? 1 : np);
+        size_t lds = sizeof(double2) * 2 * (size_t)a.npairs * a.L;
+        unsigned nblk = (unsigned)((a.nlines + 2 * a.npairs - 1) / (2 * a.npairs));
+        hipLaunchKernelGGL(k_dst_pass<MODE>, dim3(nblk, ncomp), dim3(256), lds, c->stream, a);
+    } else {
+        // power-of-two line pairs per work-group, at most DST_EMAX complex entries in LDS
+        int np = 1;
+        while (np < 8 && 2 * np * a.L <= DST_EMAX && (2 * np + 1) * a.L <= DST_LDS_MAX) np *= 2;
+        a.npairs = np;
+        size_t lds = sizeof(double2) * (size_t)(np + 1) * a.L;
+        unsigned nblk = (unsigned)((a.nlines + 2 * np - 1) / (2 * np));
+        dim3 grid(nblk, ncomp);
+        switch (rc) {
+            case 4: hipLaunchKernelGGL((k_dst_pass2<MODE, 4>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+            case 7: hipLaunchKernelGGL((k_dst_pass2<MODE, 7>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+            case 17: hipLaunchKernelGGL((k_dst_pass2<MODE, 17>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+            default: hipLaunchKernelGGL((k_dst_pass2<MODE, 37>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+        }
+    }
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
@@ -320,10 +618,7 @@ static void setup_axis(DstArgs &a, FftPlan *p, int m) {
     a.nstages = p->nstages;
     for (int s = 0; s < p->nstages; s++) a.radix[s] = p->radix[s];
     a.tw = p->tw;
-    int np = (int)(65536 / (sizeof(double2) * 2 * (size_t)p->L));
-    if (np > 8) np = 8;
-    if (np < 1) np = 1;
-    a.npairs = np;
+    a.npairs = 1;
 }
 
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out) {

@@ -363,6 +363,8 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     c->kev_ok = true;
     const char *bm = getenv("VPMX_BOUNDSUM");
     if (bm && std::string(bm) == "direct") c->bs_mode = 1;
+    const char *dm = getenv("VPMX_DST");
+    if (dm && std::string(dm) == "v1") c->dst_v1 = true;
     for (int k = 0; k < K_COUNT; k++)
         if (hipEventCreate(&c->kev0[k]) != hipSuccess || hipEventCreate(&c->kev1[k]) != hipSuccess) c->kev_ok = false;
     *out = c;

@@ -43,6 +43,8 @@ struct Ctx : vpmx_ctx {
     DBuf rhs_bl, sol_bl, sol0, vel, str;
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c;
     int bs_mode = 0;  // boundary sums: 0 lattice tables (default), 1 direct LDS source tiles
+    bool dst_v1 = false;  // DST passes: force the two-buffer kernel
+    std::map<std::pair<int, double>, void *> eig_cache;  // DST-I eigenvalues per (m, h)
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
