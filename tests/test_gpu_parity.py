@@ -180,3 +180,22 @@ def test_solve_all_radix_classes(ctx, oracle, nn):
     ref = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
     got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, sol=sol)
     assert rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("mode", ["spectral", "lattice", "direct"])
+def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
+    """The three evaluations of the fine boundary sums (spectral convolution + batched
+    products, lattice 1/r tables, direct LDS source tiles) against the oracle's direct sum,
+    on a non-cubic block (every face pair has its own geometry)."""
+    V = vpmlib
+    monkeypatch.setenv("VPMX_BOUNDSUM", mode)
+    nn = np.array([65, 57, 49], np.int32)
+    dpm = np.array([1 / 32, 1 / 30, 1 / 28])
+    xb = np.array([-1.0, -0.95, -0.85, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.02, -0.03, 0.01), s=0.25)
+    sol = np.zeros_like(rhs)
+    say("case: boundary sums", mode, nn.tolist())
+    ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    with V.VPM(V.Params.make(0.125)) as c:
+        got = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
+    assert rel_l2(got, ref) <= TOL

@@ -1,0 +1,159 @@
+// In-LDS complex FFTs shared by the sine-transform Poisson passes and the spectral
+// boundary sums: mixed-radix Stockham autosort over `npairs` sequences of length L held
+// back to back in LDS, one work-group of DST_THREADS threads, twiddles exp(-2 pi i t / L)
+// read from an LDS table.  Args is any struct with fields L, npairs, nstages, radix[].
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace vpmx {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// ------------------------------------------------------------------ register-staged stages
+// Every stage loads each butterfly's R inputs into registers, synchronises, and writes the
+// outputs to their Stockham positions.  npairs * L must not exceed DST_EMAX.
+constexpr int DST_EMAX = 2304;  // complex line elements held in LDS per work-group (36 KB)
+constexpr int DST_LDS_MAX = 4096;  // complex elements of LDS per work-group, lines + twiddles (64 KB)
+constexpr int DST_THREADS = 512;
+
+// In-register butterflies of the specialised radices (2, 3, 4).
+template <int R>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
+    if constexpr (R == 2) {
+        double2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const double c3 = -0.5, s3 = 0.86602540378443864676;
+        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        double2 base = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+        double2 rot = make_double2(s3 * t2.y, -s3 * t2.x);
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(base, rot);
+        v[2] = csub(base, rot);
+    } else if constexpr (R == 4) {
+        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]), s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        double2 md = make_double2(d13.y, -d13.x);
+        v[0] = cadd(s02, s13);
+        v[1] = cadd(d02, md);
+        v[2] = csub(s02, s13);
+        v[3] = csub(d02, md);
+    }
+}
+
+// Odd prime R-point DFT of v written to buf[d + k*Ns], k < R, with the conjugate symmetry of
+// the roots: for r < R/2, p_r = v_r + v_{R-r} and m_r = v_r - v_{R-r}; with
+// exp(-2 pi i r k / R) = c - i s,
+//   X_k   = v_0 + sum_r (c p_r.x + s m_r.y,  c p_r.y - s m_r.x)
+//   X_R-k = v_0 + sum_r (c p_r.x - s m_r.y,  c p_r.y + s m_r.x)
+// so two outputs cost 4 FMA per r.  c and s come from the LDS twiddle table.
+template <int R>
+__device__ __forceinline__ void dft_odd_store(double2 (&v)[R], double2 *buf, int d, int Ns,
+                                              const double2 *__restrict__ tw, int L) {
+    constexpr int H = (R - 1) / 2;
+    double2 p[H], mm[H];
+    double2 x0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        p[r - 1] = cadd(v[r], v[R - r]);
+        mm[r - 1] = csub(v[r], v[R - r]);
+        x0 = cadd(x0, p[r - 1]);
+    }
+    buf[d] = x0;
+    const int Lr = L / R;
+    // k is not unrolled: the H twiddles of one k are loaded per iteration, not all H*H at once
+#pragma unroll 1
+    for (int k = 1; k <= H; k++) {
+        double X = v[0].x, Y = v[0].y, U = 0.0, V = 0.0;
+        int q = 0;  // (r * k) mod R
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            q += k;
+            if (q >= R) q -= R;
+            const double2 w = tw[q * Lr];  // (c, -s)
+            const double c = w.x, s = -w.y;
+            X += c * p[r - 1].x;
+            Y += c * p[r - 1].y;
+            U += s * mm[r - 1].y;
+            V += s * mm[r - 1].x;
+        }
+        buf[d + k * Ns] = make_double2(X + U, Y - V);
+        buf[d + (R - k) * Ns] = make_double2(X - U, Y + V);
+    }
+}
+
+// One Stockham stage in place: inputs to registers, barrier, outputs to LDS, barrier.
+// Radices above 4 form their outputs after the barrier (v holds the twiddled inputs).
+template <int R, class Args>
+__device__ void stage_reg(double2 *buf, const double2 *__restrict__ tw, const Args &a, int Ns) {
+    constexpr int BPT = (DST_EMAX / R + DST_THREADS - 1) / DST_THREADS;
+    const int L = a.L, nbf = L / R, total = a.npairs * nbf, twstep = L / (Ns * R);
+    double2 v[BPT][R];
+    int dst[BPT];
+#pragma unroll
+    for (int bi = 0; bi < BPT; bi++) {
+        const int t = threadIdx.x + bi * DST_THREADS;
+        dst[bi] = -1;
+        if (t < total) {
+            const int pair = t / nbf, j = t - pair * nbf;
+            const int k0 = j % Ns;
+            const double2 *src = buf + pair * L;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[bi][r] = src[j + r * nbf];
+            if (k0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[bi][r] = cmul(v[bi][r], tw[r * k0 * twstep]);
+            }
+            if constexpr (R <= 4) dft_reg<R>(v[bi]);
+            dst[bi] = pair * L + (j - k0) * R + k0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bi = 0; bi < BPT; bi++)
+        if (dst[bi] >= 0) {
+            if constexpr (R <= 4) {
+#pragma unroll
+                for (int k = 0; k < R; k++) buf[dst[bi] + k * Ns] = v[bi][k];
+            } else {
+                dft_odd_store<R>(v[bi], buf, dst[bi], Ns, tw, L);
+            }
+        }
+    __syncthreads();
+}
+
+// MAXR bounds the radices compiled into a kernel, so the registers of the common 2/3/4
+// stages are not sized for a large prime stage.
+template <int MAXR, class Args>
+__device__ void fft_reg(double2 *buf, const double2 *tw, const Args &a) {
+    int Ns = 1;
+    for (int s = 0; s < a.nstages; s++) {
+        const int R = a.radix[s];
+        if (R == 4) stage_reg<4, Args>(buf, tw, a, Ns);
+        else if (R == 2) stage_reg<2, Args>(buf, tw, a, Ns);
+        else if (R == 3) stage_reg<3, Args>(buf, tw, a, Ns);
+        else if constexpr (MAXR >= 7) {
+            if (R == 5) stage_reg<5, Args>(buf, tw, a, Ns);
+            else if (R == 7) stage_reg<7, Args>(buf, tw, a, Ns);
+            else if constexpr (MAXR >= 17) {
+                if (R == 11) stage_reg<11, Args>(buf, tw, a, Ns);
+                else if (R == 13) stage_reg<13, Args>(buf, tw, a, Ns);
+                else if (R == 17) stage_reg<17, Args>(buf, tw, a, Ns);
+                else if constexpr (MAXR >= 37) {
+                    if (R == 19) stage_reg<19, Args>(buf, tw, a, Ns);
+                    else if (R == 23) stage_reg<23, Args>(buf, tw, a, Ns);
+                    else if (R == 29) stage_reg<29, Args>(buf, tw, a, Ns);
+                    else if (R == 31) stage_reg<31, Args>(buf, tw, a, Ns);
+                    else stage_reg<37, Args>(buf, tw, a, Ns);
+                }
+            }
+        }
+        Ns *= R;
+    }
+}
+
+}  // namespace vpmx

@@ -12,6 +12,7 @@
 // result is deterministic.  1/r is v_rsq_f64 refined by one third-order step.
 #include <hip/hip_ext.h>
 
+#include "bnodes.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -107,22 +108,6 @@ int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, doubl
     VPMX_HIP(c, hipGetLastError());
     *nbound = nb;
     return VPMX_OK;
-}
-
-// Boundary node t (reference face order) -> 1-based (i,j,k) and plane code.
-__device__ __forceinline__ void target_node(const Grid &g, int64_t t, int *i, int *j, int *k, int *pl) {
-    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
-    const int64_t fX = (int64_t)NY * NZ, fY = (int64_t)(NX - 2) * NZ, fZ = (int64_t)(NX - 2) * (NY - 2);
-    if (t < 2 * fX) {
-        int f = (int)(t / fX); int64_t r = t % fX;
-        *i = f ? NX : 1; *j = (int)(r % NY) + 1; *k = (int)(r / NY) + 1; *pl = 2;
-    } else if (t < 2 * (fX + fY)) {
-        int64_t u = t - 2 * fX; int f = (int)(u / fY); int64_t r = u % fY;
-        *j = f ? NY : 1; *i = (int)(r % (NX - 2)) + 2; *k = (int)(r / (NX - 2)) + 1; *pl = 1;
-    } else {
-        int64_t u = t - 2 * (fX + fY); int f = (int)(u / fZ); int64_t r = u % fZ;
-        *k = f ? NZ : 1; *i = (int)(r % (NX - 2)) + 2; *j = (int)(r / (NX - 2)) + 2; *pl = 3;
-    }
 }
 
 __device__ __forceinline__ double rsqrt_f64(double r2) {
@@ -523,7 +508,21 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
         VPMX_HIP(c, hipGetLastError());
         return VPMX_OK;
     }
-    if (c->bs_mode == 0) return boundary_sums_lattice(c, g, ns, a, sol, fine);
+    if (fine && c->bs_mode == 2) {
+        mark(c, ST_BOUNDS, 0);
+        if (c->kev_ok) VPMX_HIP(c, hipEventRecord(c->kev0[K_BOUNDSUM], c->stream));
+        const int st = boundary_sums_spectral(c, g, ns, a, sol);
+        if (st == VPMX_OK) {
+            if (c->kev_ok) VPMX_HIP(c, hipEventRecord(c->kev1[K_BOUNDSUM], c->stream));
+            kernel_timed(c, K_BOUNDSUM);
+            mark(c, ST_BOUNDS, 1);
+            c->last_pairs += (double)nt * (double)ns;
+            return VPMX_OK;
+        }
+        if (st != VPMX_ERR_UNSUP) return st;
+        mark(c, ST_BOUNDS, 1);  // faces too long for the in-LDS transforms: lattice kernel
+    }
+    if (c->bs_mode != 1) return boundary_sums_lattice(c, g, ns, a, sol, fine);
     int64_t tblocks = (nt + BS_THREADS * BS_TPT - 1) / (BS_THREADS * BS_TPT);
     // split sources so the launch has >= ~2048 work-groups, each with >= 1024 sources
     int64_t nsplit = (2048 + tblocks - 1) / tblocks;

@@ -1,0 +1,486 @@
+// Spectral evaluation of the infinite-domain boundary sums (Bounds3d -> calc_boundinf_3d,
+// pmbound.f90:80-157, 415-515, ibctyp=2).  Same sum as the direct kernels,
+//   SOL(c,t) += sum_s a_c(s) / |x_t - x_s|,   a = -sigma * ds / (4 pi),
+// over the face-cell sources s of the six block faces and every boundary node t, exact up
+// to rounding.
+//
+// Targets on face B, sources on face A.  Both faces contain an in-plane axis C (for
+// perpendicular faces the one axis they share, for parallel faces the higher in-plane
+// axis), along which the kernel depends only on the offset e = kt - ks between the target
+// node and the source cell:  |x_t - x_s|^2 = K(p, q) + ((e - 1/2) h_C)^2, with p the
+// target's other coordinate and q the source's.  So each (B, A) pair is a 1-D convolution
+// along C -- real FFTs of length L >= 2 n_C - 2, which leaves the needed outputs free of
+// wrap-around -- and a dense sum over q, which for every frequency w is a complex matrix
+// product
+//   O^(p, col; w) = sum_q G^(p, q; w) A^(q, col; w),
+// batched over the L/2+1 frequencies.  Pairs with the same geometry up to a reflection of
+// p or q share one G^: on a cubic block three matrices (perpendicular, same face, opposite
+// face) serve all 36 pairs, whose pairs x components are the columns.  G^ depends only on
+// the block geometry and is kept across calls.
+//
+// The products run in rocBLAS (complex FP64, atomics off, so the results do not depend on
+// scheduling); the transforms run in LDS with the Stockham FFT of the Poisson passes.  A
+// final pass adds, for every boundary node, the six pair contributions of its owner face
+// in a fixed order (X faces own their edges, Y faces their z-edges).
+#include <rocblas/rocblas.h>
+
+#include <cmath>
+#include <vector>
+
+#include "bnodes.h"
+#include "fft_lds.h"
+#include "vpmx_ctx.h"
+#include "vpmx_internal.h"
+
+namespace vpmx {
+
+constexpr int SPEC_MAXR = 7;  // FFT lengths are chosen 7-smooth
+
+struct SpecFft {  // FFT shape for fft_reg
+    int L, npairs, nstages, radix[24];
+};
+
+struct SpecTab {  // one Green's-function matrix family
+    int type;     // 0: perpendicular faces, 1: parallel faces
+    int M, K;     // target coordinates p < M, source coordinates q < K
+    int nC, L, nw;
+    double hT, hS, hC, d;
+};
+
+struct SpecCol {  // one column: a source face line family, one component
+    int64_t src_off;  // first source of face A in the source array
+    int ustride;      // row length of face A's source array (its lower in-plane axis)
+    int s_is_u;       // q runs along A's lower in-plane axis (else C does)
+    int flipq;        // q -> K-1-q
+    int comp;
+};
+
+struct SpecPair {  // where the contribution of source face A to target face B sits
+    int64_t base;  // offset of component 0 in Ores ([col][p][kt])
+    int M, nC, tax, cax, flipp, pad;
+};
+
+// ------------------------------------------------------------------ kernels
+__device__ __forceinline__ int residue_offset(int r, int L, int nC, bool *ok) {
+    // circular index r of the kernel -> offset e in [-(nC-2), nC-1]
+    *ok = true;
+    if (r <= nC - 1) return r;
+    if (r >= L - (nC - 2)) return r - L;
+    *ok = false;
+    return 0;
+}
+
+// Two real lines packed as one complex FFT -> their half spectra, k < nw.
+__device__ __forceinline__ double2 unpack_half(const double2 *A, int L, int pr, int k, int odd) {
+    const double2 z = A[pr * L + k], zc = A[pr * L + (k == 0 ? 0 : L - k)];
+    if (!odd) return make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y - zc.y));
+    return make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
+}
+
+// G^[w][q][p] for p in the block's lines, one q per blockIdx.y.
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t, const double2 *tw, double2 *G) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int L = f.L, np = f.npairs, nl = 2 * np;
+    double2 *A = lds, *twl = lds + np * L;
+    double *Ad = (double *)A;
+    const int tid = threadIdx.x, p0 = blockIdx.x * nl, q = blockIdx.y;
+    for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
+    for (int i = tid; i < nl * L; i += DST_THREADS) {
+        const int l = i / L, r = i - l * L, p = p0 + l;
+        bool ok;
+        const int e = residue_offset(r, L, t.nC, &ok);
+        double v = 0.0;
+        if (ok && p < t.M) {
+            const double dc = ((double)e - 0.5) * t.hC;
+            double r2;
+            if (t.type == 0) {
+                const double x = (double)p * t.hT, y = ((double)q + 0.5) * t.hS;
+                r2 = (x * x + y * y) + dc * dc;
+            } else {
+                const double x = ((double)(p - q) - 0.5) * t.hT;
+                r2 = (x * x + t.d * t.d) + dc * dc;
+            }
+            v = 1.0 / sqrt(r2);
+        }
+        Ad[2 * ((l >> 1) * L + r) + (l & 1)] = v;
+    }
+    __syncthreads();
+    fft_reg<SPEC_MAXR>(A, twl, f);
+    for (int i = tid; i < t.nw * nl; i += DST_THREADS) {
+        const int k = i / nl, l = i - k * nl, p = p0 + l;
+        if (p < t.M) G[((int64_t)k * t.K + q) * t.M + p] = unpack_half(A, L, l >> 1, k, l & 1);
+    }
+}
+
+// A^ of every column: Bm[col][w][q], lines q of the block, one column per blockIdx.y.
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, const double2 *tw,
+                                                         const SpecCol *cols, const double *a, int64_t ns,
+                                                         double2 *Bm) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int L = f.L, np = f.npairs, nl = 2 * np;
+    double2 *A = lds, *twl = lds + np * L;
+    double *Ad = (double *)A;
+    const int tid = threadIdx.x, q0 = blockIdx.x * nl, col = blockIdx.y;
+    const SpecCol cd = cols[col];
+    const double *src = a + (int64_t)cd.comp * ns + cd.src_off;
+    const int ncell = t.nC - 1;  // source cells along C
+    for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
+    if (cd.s_is_u) {
+        // q along the face's contiguous axis: neighbouring threads take neighbouring lines
+        for (int i = tid; i < nl * L; i += DST_THREADS) {
+            const int r = i / nl, l = i - r * nl, q = q0 + l;
+            double v = 0.0;
+            if (r < ncell && q < t.K) {
+                const int qs = cd.flipq ? t.K - 1 - q : q;
+                v = src[(int64_t)r * cd.ustride + qs];
+            }
+            Ad[2 * ((l >> 1) * L + r) + (l & 1)] = v;
+        }
+    } else {
+        for (int i = tid; i < nl * L; i += DST_THREADS) {
+            const int l = i / L, r = i - l * L, q = q0 + l;
+            double v = 0.0;
+            if (r < ncell && q < t.K) {
+                const int qs = cd.flipq ? t.K - 1 - q : q;
+                v = src[(int64_t)qs * cd.ustride + r];
+            }
+            Ad[2 * ((l >> 1) * L + r) + (l & 1)] = v;
+        }
+    }
+    __syncthreads();
+    fft_reg<SPEC_MAXR>(A, twl, f);
+    double2 *out = Bm + (int64_t)col * t.nw * t.K;
+    for (int i = tid; i < t.nw * nl; i += DST_THREADS) {
+        const int k = i / nl, l = i - k * nl, q = q0 + l;
+        if (q < t.K) out[(int64_t)k * t.K + q] = unpack_half(A, L, l >> 1, k, l & 1);
+    }
+}
+
+// Inverse real FFT of O^[col][w][p] over w, first nC outputs -> Ores[col][p][kt].
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, const double2 *tw, const double2 *Cm,
+                                                         double *Ores, int64_t ores_off) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int L = f.L, np = f.npairs, nl = 2 * np;
+    double2 *A = lds, *twl = lds + np * L;
+    const int tid = threadIdx.x, p0 = blockIdx.x * nl, col = blockIdx.y;
+    const double2 *in = Cm + (int64_t)col * t.nw * t.M;
+    for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
+    // Z_k = X_k + i Y_k with the Hermitian halves of lines p (X) and p+1 (Y)
+    for (int i = tid; i < t.nw * np; i += DST_THREADS) {
+        const int k = i / np, pr = i - k * np, p = p0 + 2 * pr;
+        const double2 X = p < t.M ? in[(int64_t)k * t.M + p] : make_double2(0.0, 0.0);
+        const double2 Y = p + 1 < t.M ? in[(int64_t)k * t.M + p + 1] : make_double2(0.0, 0.0);
+        if (k == 0 || 2 * k == L) {
+            A[pr * L + k] = make_double2(X.x, Y.x);
+        } else {
+            A[pr * L + k] = make_double2(X.x - Y.y, X.y + Y.x);
+            A[pr * L + L - k] = make_double2(X.x + Y.y, Y.x - X.y);
+        }
+    }
+    __syncthreads();
+    fft_reg<SPEC_MAXR>(A, twl, f);
+    // z_n = FFT(Z)[(L - n) mod L] / L
+    const double invL = 1.0 / (double)L;
+    double *out = Ores + ores_off + (int64_t)col * t.M * t.nC;
+    for (int i = tid; i < nl * t.nC; i += DST_THREADS) {
+        const int l = i / t.nC, n = i - l * t.nC, p = p0 + l;
+        if (p >= t.M) continue;
+        const double2 w = A[(l >> 1) * L + (n == 0 ? 0 : L - n)];
+        out[(int64_t)p * t.nC + n] = ((l & 1) ? w.y : w.x) * invL;
+    }
+}
+
+// SOL(t) += sum over the six source faces (fixed order) of the owner face's pair outputs.
+__global__ void k_bsx_gather(Grid g, int64_t nt, const SpecPair *pairs, const double *Ores, GView sol) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    int i, j, k, pl;
+    target_node(g, t, &i, &j, &k, &pl);
+    const int B = pl == 2 ? (i == g.n[0] ? 1 : 0) : (pl == 1 ? 2 + (j == g.n[1] ? 1 : 0) : 4 + (k == g.n[2] ? 1 : 0));
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (int A = 0; A < 6; A++) {
+        const SpecPair P = pairs[B * 6 + A];
+        int p = P.tax == 0 ? i - 1 : (P.tax == 1 ? j - 1 : k - 1);
+        const int kt = P.cax == 0 ? i - 1 : (P.cax == 1 ? j - 1 : k - 1);
+        if (P.flipp) p = P.M - 1 - p;
+        const int64_t cst = (int64_t)P.M * P.nC;
+        const double *o = Ores + P.base + (int64_t)p * P.nC + kt;
+        acc0 += o[0];
+        acc1 += o[cst];
+        acc2 += o[2 * cst];
+    }
+    const int64_t n = sol.at(i - 1, j - 1, k - 1);
+    sol.p[n] += acc0;
+    sol.p[sol.cs + n] += acc1;
+    sol.p[2 * sol.cs + n] += acc2;
+}
+
+// ------------------------------------------------------------------ host plan
+struct SpecTable {
+    SpecTab t{};
+    SpecFft f{};
+    const double2 *tw = nullptr;
+    double2 *G = nullptr;
+    std::vector<SpecCol> cols;
+    SpecCol *cols_d = nullptr;
+    int64_t b_off = 0, c_off = 0, o_off = 0;  // column 0 in Bm, Cm (complex) and Ores (real)
+};
+
+struct SpecPlan {
+    int n[3] = {0, 0, 0};
+    double h[3] = {0, 0, 0};
+    std::vector<SpecTable> tabs;
+    SpecPair *pairs_d = nullptr;
+    double2 *Bm = nullptr, *Cm = nullptr;
+    double *Ores = nullptr;
+    double flops = 0.0;
+};
+
+static void free_plan(SpecPlan *P) {
+    if (!P) return;
+    for (auto &T : P->tabs) {
+        if (T.G) (void)hipFree(T.G);
+        if (T.cols_d) (void)hipFree(T.cols_d);
+    }
+    if (P->pairs_d) (void)hipFree(P->pairs_d);
+    if (P->Bm) (void)hipFree(P->Bm);
+    if (P->Cm) (void)hipFree(P->Cm);
+    if (P->Ores) (void)hipFree(P->Ores);
+    delete P;
+}
+
+void free_bspec(Ctx *c) {
+    free_plan((SpecPlan *)c->bsx);
+    c->bsx = nullptr;
+    if (c->blas) (void)rocblas_destroy_handle((rocblas_handle)c->blas);
+    c->blas = nullptr;
+}
+
+static bool smooth7(int n) {
+    for (int p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+static int spec_length(int nC) {
+    int L = 2 * nC - 2;
+    if (L < 4) L = 4;
+    while (!(L % 2 == 0 && smooth7(L))) L++;
+    return L;
+}
+
+static bool same_tab(const SpecTab &a, const SpecTab &b) {
+    return a.type == b.type && a.M == b.M && a.K == b.K && a.nC == b.nC && a.L == b.L && a.hT == b.hT &&
+           a.hS == b.hS && a.hC == b.hC && a.d == b.d;
+}
+
+static int fft_setup(Ctx *c, int L, SpecFft *f, const double2 **tw) {
+    FftShape fs;
+    VPMX_TRY(fft_shape(c, L, &fs));
+    f->L = L;
+    f->nstages = fs.nstages;
+    for (int s = 0; s < fs.nstages; s++) {
+        if (fs.radix[s] > SPEC_MAXR) return fail(c, VPMX_ERR_UNSUP, "spectral boundary sums: FFT radix");
+        f->radix[s] = fs.radix[s];
+    }
+    int np = 1;
+    while (np < 8 && 2 * np * L <= DST_EMAX && (2 * np + 1) * L <= DST_LDS_MAX) np *= 2;
+    f->npairs = np;
+    *tw = fs.tw;
+    return VPMX_OK;
+}
+
+static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
+    SpecPlan *P = new SpecPlan();
+    for (int d = 0; d < 3; d++) {
+        P->n[d] = g.n[d];
+        P->h[d] = g.h[d];
+    }
+    const int *n = g.n;
+    const double *h = g.h;
+    // sources of face f start at face_off[f]; face storage is (lower in-plane axis) inner
+    int64_t face_off[6], off = 0;
+    for (int f = 0; f < 6; f++) {
+        const int ax = f / 2, u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+        face_off[f] = off;
+        off += (int64_t)(n[u] - 1) * (n[v] - 1);
+    }
+    struct PairRef {
+        int tab, col0, tax, cax, flipp;
+    };
+    PairRef refs[36];
+    for (int B = 0; B < 6; B++)
+        for (int A = 0; A < 6; A++) {
+            const int At = B / 2, sB = B & 1, As = A / 2, sA = A & 1;
+            SpecTab t{};
+            int tax, sax, cax, flipp, flipq;
+            if (At != As) {
+                cax = 3 - At - As;
+                tax = As;
+                sax = At;
+                t.type = 0;
+                t.M = n[As];
+                t.K = n[At] - 1;
+                t.hT = h[As];
+                t.hS = h[At];
+                t.d = 0.0;
+                flipp = sA;
+                flipq = sB;
+            } else {
+                const int u = At == 0 ? 1 : 0, v = At == 2 ? 1 : 2;
+                cax = v;
+                tax = sax = u;
+                t.type = 1;
+                t.M = n[u];
+                t.K = n[u] - 1;
+                t.hT = t.hS = h[u];
+                t.d = sA == sB ? 0.0 : (double)(n[At] - 1) * h[At];
+                flipp = flipq = 0;
+            }
+            t.nC = n[cax];
+            t.hC = h[cax];
+            t.L = spec_length(t.nC);
+            t.nw = t.L / 2 + 1;
+            if (t.L > DST_EMAX || 2 * t.L > DST_LDS_MAX) {
+                free_plan(P);
+                return VPMX_ERR_UNSUP;
+            }
+            int ti = -1;
+            for (size_t q = 0; q < P->tabs.size(); q++)
+                if (same_tab(P->tabs[q].t, t)) ti = (int)q;
+            if (ti < 0) {
+                SpecTable T;
+                T.t = t;
+                P->tabs.push_back(T);
+                ti = (int)P->tabs.size() - 1;
+            }
+            SpecTable &T = P->tabs[ti];
+            const int uA = As == 0 ? 1 : 0;
+            refs[B * 6 + A] = {ti, (int)T.cols.size(), tax, cax, flipp};
+            for (int comp = 0; comp < 3; comp++)
+                T.cols.push_back({face_off[A], n[uA] - 1, sax == uA ? 1 : 0, flipq, comp});
+        }
+    // buffers
+    int64_t nb = 0, ncm = 0, no = 0;
+    for (auto &T : P->tabs) {
+        const int64_t N = (int64_t)T.cols.size();
+        T.b_off = nb;
+        T.c_off = ncm;
+        T.o_off = no;
+        nb += N * T.t.nw * T.t.K;
+        ncm += N * T.t.nw * T.t.M;
+        no += N * T.t.M * T.t.nC;
+    }
+    int st = VPMX_OK;
+    auto alloc = [&](void **p, size_t bytes) {
+        if (st) return;
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) st = fail(c, VPMX_ERR_HIP, std::string("spectral boundary sums: hipMalloc: ") + hipGetErrorString(e));
+    };
+    alloc((void **)&P->Bm, sizeof(double2) * nb);
+    alloc((void **)&P->Cm, sizeof(double2) * ncm);
+    alloc((void **)&P->Ores, sizeof(double) * no);
+    alloc((void **)&P->pairs_d, sizeof(SpecPair) * 36);
+    for (auto &T : P->tabs) {
+        alloc((void **)&T.G, sizeof(double2) * (size_t)T.t.nw * T.t.K * T.t.M);
+        alloc((void **)&T.cols_d, sizeof(SpecCol) * T.cols.size());
+    }
+    if (st) {
+        free_plan(P);
+        return st;
+    }
+    SpecPair hp[36];
+    for (int q = 0; q < 36; q++) {
+        const PairRef &R = refs[q];
+        const SpecTable &T = P->tabs[R.tab];
+        hp[q] = {T.o_off + (int64_t)R.col0 * T.t.M * T.t.nC, T.t.M, T.t.nC, R.tax, R.cax, R.flipp, 0};
+    }
+    hipError_t e = hipMemcpyAsync(P->pairs_d, hp, sizeof(hp), hipMemcpyHostToDevice, c->stream);
+    for (auto &T : P->tabs) {
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(T.cols_d, T.cols.data(), sizeof(SpecCol) * T.cols.size(), hipMemcpyHostToDevice,
+                               c->stream);
+        if (e == hipSuccess) st = fft_setup(c, T.t.L, &T.f, &T.tw);
+        if (st) break;
+        const size_t lds = sizeof(double2) * (size_t)(T.f.npairs + 1) * T.t.L;
+        dim3 grid((unsigned)((T.t.M + 2 * T.f.npairs - 1) / (2 * T.f.npairs)), (unsigned)T.t.K);
+        hipLaunchKernelGGL(k_bsx_table, grid, dim3(DST_THREADS), lds, c->stream, T.f, T.t, T.tw, T.G);
+        if (e == hipSuccess) e = hipGetLastError();
+        P->flops += 8.0 * T.t.M * T.t.K * (double)T.cols.size() * T.t.nw;
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host arrays hp / cols go out of scope
+    if (!st && e != hipSuccess) st = fail(c, VPMX_ERR_HIP, std::string("spectral boundary sums: ") + hipGetErrorString(e));
+    if (st) {
+        free_plan(P);
+        return st;
+    }
+    *out = P;
+    return VPMX_OK;
+}
+
+static int blas_check(Ctx *c, rocblas_status s, const char *what) {
+    if (s == rocblas_status_success) return VPMX_OK;
+    return fail(c, VPMX_ERR_HIP, std::string("rocBLAS ") + what + ": " + rocblas_status_to_string(s));
+}
+
+int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol) {
+    SpecPlan *P = (SpecPlan *)c->bsx;
+    if (P && (P->n[0] != g.n[0] || P->n[1] != g.n[1] || P->n[2] != g.n[2] || P->h[0] != g.h[0] ||
+              P->h[1] != g.h[1] || P->h[2] != g.h[2])) {
+        VPMX_HIP(c, hipStreamSynchronize(c->stream));
+        free_plan(P);
+        c->bsx = P = nullptr;
+    }
+    if (!P) {
+        int st = build_plan(c, g, &P);
+        if (st) return st;
+        c->bsx = P;
+    }
+    if (!c->blas) {
+        rocblas_handle hb;
+        VPMX_TRY(blas_check(c, rocblas_create_handle(&hb), "create_handle"));
+        c->blas = hb;
+        VPMX_TRY(blas_check(c, rocblas_set_atomics_mode(hb, rocblas_atomics_not_allowed), "set_atomics_mode"));
+        VPMX_TRY(blas_check(c, rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host), "set_pointer_mode"));
+    }
+    rocblas_handle hb = (rocblas_handle)c->blas;
+    VPMX_TRY(blas_check(c, rocblas_set_stream(hb, c->stream), "set_stream"));
+    for (auto &T : P->tabs) {
+        const size_t lds = sizeof(double2) * (size_t)(T.f.npairs + 1) * T.t.L;
+        const int nl = 2 * T.f.npairs, ncol = (int)T.cols.size();
+        hipLaunchKernelGGL(k_bsx_fwd, dim3((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
+                           c->stream, T.f, T.t, T.tw, T.cols_d, a, ns, P->Bm + T.b_off);
+        VPMX_HIP(c, hipGetLastError());
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+        VPMX_TRY(blas_check(
+            c,
+            rocblas_zgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M, ncol, T.t.K, &one,
+                                          (const rocblas_double_complex *)T.G, T.t.M, (rocblas_stride)T.t.M * T.t.K,
+                                          (const rocblas_double_complex *)(P->Bm + T.b_off), T.t.nw * T.t.K,
+                                          (rocblas_stride)T.t.K, &zero, (rocblas_double_complex *)(P->Cm + T.c_off),
+                                          T.t.nw * T.t.M, (rocblas_stride)T.t.M, T.t.nw),
+            "zgemm_strided_batched"));
+        hipLaunchKernelGGL(k_bsx_inv, dim3((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
+                           c->stream, T.f, T.t, T.tw, P->Cm + T.c_off, P->Ores, T.o_off);
+        VPMX_HIP(c, hipGetLastError());
+    }
+    const int64_t nt = 2 * ((int64_t)g.n[1] * g.n[2] + (int64_t)(g.n[0] - 2) * g.n[2] +
+                            (int64_t)(g.n[0] - 2) * (g.n[1] - 2));
+    hipLaunchKernelGGL(k_bsx_gather, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, g, nt, P->pairs_d,
+                       P->Ores, sol);
+    VPMX_HIP(c, hipGetLastError());
+    // FP64 work of the call: the complex products, plus ~5 L log2 L flops per complex FFT
+    double fft = 0.0;
+    for (auto &T : P->tabs) {
+        const double per = 5.0 * T.t.L * std::log2((double)T.t.L);
+        fft += per * (double)T.cols.size() * (T.t.K + T.t.M) / 2.0;
+    }
+    c->k_work[K_BOUNDSUM] = P->flops + fft;
+    return VPMX_OK;
+}
+
+}  // namespace vpmx

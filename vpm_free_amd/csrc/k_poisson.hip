@@ -16,6 +16,7 @@
 #include <cmath>
 #include <map>
 
+#include "fft_lds.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -42,12 +43,6 @@ struct DstArgs {
     const double *lam[3];  // eigenvalues (z pass)
     double norm;
 };
-
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
 // Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
 // buffer that holds the result.
@@ -233,117 +228,6 @@ __global__ void __launch_bounds__(256) k_dst_pass(DstArgs a) {
 // inputs into registers, synchronises, and writes the outputs to their Stockham
 // positions.  Lines are mapped to threads without 64-bit divisions, and the number of
 // lines per work-group is a power of two.
-constexpr int DST_EMAX = 2304;  // complex line elements held in LDS per work-group (36 KB)
-constexpr int DST_LDS_MAX = 4096;  // complex elements of LDS per work-group, lines + twiddles (64 KB)
-constexpr int DST_THREADS = 512;
-
-// In-register butterflies of the specialised radices (2, 3, 4).
-template <int R>
-__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
-    if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else if constexpr (R == 3) {
-        const double c3 = -0.5, s3 = 0.86602540378443864676;
-        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        double2 base = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
-        double2 rot = make_double2(s3 * t2.y, -s3 * t2.x);
-        v[0] = cadd(v[0], t1);
-        v[1] = cadd(base, rot);
-        v[2] = csub(base, rot);
-    } else if constexpr (R == 4) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]), s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-        double2 md = make_double2(d13.y, -d13.x);
-        v[0] = cadd(s02, s13);
-        v[1] = cadd(d02, md);
-        v[2] = csub(s02, s13);
-        v[3] = csub(d02, md);
-    }
-}
-
-// One Stockham stage in place: inputs to registers, barrier, outputs to LDS, barrier.
-// Radices above 4 form each output after the barrier (v holds the inputs) so that only
-// the R inputs are live.
-template <int R>
-__device__ void stage_reg(double2 *buf, const double2 *__restrict__ tw, const DstArgs &a, int Ns) {
-    constexpr int BPT = (DST_EMAX / R + DST_THREADS - 1) / DST_THREADS;
-    const int L = a.L, nbf = L / R, total = a.npairs * nbf, twstep = L / (Ns * R);
-    double2 v[BPT][R];
-    int dst[BPT];
-    double2 wr[R > 4 ? R : 1];  // exp(-2 pi i q / R)
-    if constexpr (R > 4) {
-#pragma unroll
-        for (int q = 0; q < R; q++) wr[q] = tw[q * (L / R)];
-    }
-#pragma unroll
-    for (int bi = 0; bi < BPT; bi++) {
-        const int t = threadIdx.x + bi * DST_THREADS;
-        dst[bi] = -1;
-        if (t < total) {
-            const int pair = t / nbf, j = t - pair * nbf;
-            const int k0 = j % Ns;
-            const double2 *src = buf + pair * L;
-#pragma unroll
-            for (int r = 0; r < R; r++) v[bi][r] = src[j + r * nbf];
-            if (k0) {
-#pragma unroll
-                for (int r = 1; r < R; r++) v[bi][r] = cmul(v[bi][r], tw[r * k0 * twstep]);
-            }
-            if constexpr (R <= 4) dft_reg<R>(v[bi]);
-            dst[bi] = pair * L + (j - k0) * R + k0;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int bi = 0; bi < BPT; bi++)
-        if (dst[bi] >= 0) {
-            if constexpr (R <= 4) {
-#pragma unroll
-                for (int k = 0; k < R; k++) buf[dst[bi] + k * Ns] = v[bi][k];
-            } else {
-#pragma unroll
-                for (int k = 0; k < R; k++) {
-                    double2 acc = v[bi][0];
-#pragma unroll
-                    for (int r = 1; r < R; r++) acc = cadd(acc, cmul(v[bi][r], wr[(r * k) % R]));
-                    buf[dst[bi] + k * Ns] = acc;
-                }
-            }
-        }
-    __syncthreads();
-}
-
-// MAXR bounds the radices compiled into a kernel, so the registers of the common 2/3/4
-// stages are not sized for a large prime stage.
-template <int MAXR>
-__device__ void fft_reg(double2 *buf, const double2 *tw, const DstArgs &a) {
-    int Ns = 1;
-    for (int s = 0; s < a.nstages; s++) {
-        const int R = a.radix[s];
-        if (R == 4) stage_reg<4>(buf, tw, a, Ns);
-        else if (R == 2) stage_reg<2>(buf, tw, a, Ns);
-        else if (R == 3) stage_reg<3>(buf, tw, a, Ns);
-        else if constexpr (MAXR >= 7) {
-            if (R == 5) stage_reg<5>(buf, tw, a, Ns);
-            else if (R == 7) stage_reg<7>(buf, tw, a, Ns);
-            else if constexpr (MAXR >= 17) {
-                if (R == 11) stage_reg<11>(buf, tw, a, Ns);
-                else if (R == 13) stage_reg<13>(buf, tw, a, Ns);
-                else if (R == 17) stage_reg<17>(buf, tw, a, Ns);
-                else if constexpr (MAXR >= 37) {
-                    if (R == 19) stage_reg<19>(buf, tw, a, Ns);
-                    else if (R == 23) stage_reg<23>(buf, tw, a, Ns);
-                    else if (R == 29) stage_reg<29>(buf, tw, a, Ns);
-                    else if (R == 31) stage_reg<31>(buf, tw, a, Ns);
-                    else stage_reg<37>(buf, tw, a, Ns);
-                }
-            }
-        }
-        Ns *= R;
-    }
-}
-
 template <int MODE, int MAXR>
 __global__ void __launch_bounds__(DST_THREADS) k_dst_pass2(DstArgs a) {
 #pragma clang fp contract(off)
@@ -535,6 +419,16 @@ static int get_plan(Ctx *c, int L, FftPlan **out) {
     VPMX_HIP(c, hipMemcpy(p->tw, h.data(), sizeof(double2) * L, hipMemcpyHostToDevice));
     c->fft_plans[L] = p;
     *out = p;
+    return VPMX_OK;
+}
+
+int fft_shape(Ctx *c, int L, FftShape *out) {
+    FftPlan *p = nullptr;
+    VPMX_TRY(get_plan(c, L, &p));
+    out->L = p->L;
+    out->nstages = p->nstages;
+    for (int s = 0; s < p->nstages; s++) out->radix[s] = p->radix[s];
+    out->tw = p->tw;
     return VPMX_OK;
 }
 

@@ -363,6 +363,7 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     c->kev_ok = true;
     const char *bm = getenv("VPMX_BOUNDSUM");
     if (bm && std::string(bm) == "direct") c->bs_mode = 1;
+    if (bm && std::string(bm) == "lattice") c->bs_mode = 0;
     const char *dm = getenv("VPMX_DST");
     if (dm && std::string(dm) == "v1") c->dst_v1 = true;
     for (int k = 0; k < K_COUNT; k++)
@@ -382,6 +383,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill};
     for (DBuf *b : bufs) release(*b);
+    free_bspec(c);
     free_fft_plans(c);
     for (int k = 0; k < K_COUNT; k++) {
         if (c->kev0[k]) (void)hipEventDestroy(c->kev0[k]);
@@ -698,7 +700,7 @@ int32_t vpmx_stage_pmesh_inf(vpmx_ctx *p, const double xbound[6], const double d
     VPMX_HIP(c, hipMemcpyAsync(ra, sol, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
     VPMX_TRY(aos_to_soa(c, nc, n, ra, view(ss, n)));
     Grid g = make_grid(nn, xbound, dpm);
-    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, false));
+    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, true));
     VPMX_TRY(soa_to_aos(c, nc, nc, n, view(ss, n), 0.0, ra));
     VPMX_HIP(c, hipMemcpyAsync(sol, ra, sizeof(double) * nc * G, hipMemcpyDeviceToHost, c->stream));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));

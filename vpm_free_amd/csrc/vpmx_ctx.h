@@ -42,7 +42,9 @@ struct Ctx : vpmx_ctx {
     // grids
     DBuf rhs_bl, sol_bl, sol0, vel, str;
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c;
-    int bs_mode = 0;  // boundary sums: 0 lattice tables (default), 1 direct LDS source tiles
+    int bs_mode = 2;  // fine boundary sums: 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
+    void *bsx = nullptr;   // spectral boundary-sum plan of the current block geometry
+    void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
     bool dst_v1 = false;  // DST passes: force the two-buffer kernel
     std::map<std::pair<int, double>, void *> eig_cache;  // DST-I eigenvalues per (m, h)
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;

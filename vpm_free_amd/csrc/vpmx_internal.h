@@ -83,11 +83,23 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
                       GView out);
 
+// Complex FFT plan of length L (Stockham radices, device twiddle table exp(-2 pi i t / L)),
+// cached per context.
+struct FftShape {
+    int L, nstages, radix[24];
+    const double2 *tw;
+};
+int fft_shape(Ctx *c, int L, FftShape *out);
+
 // Boundary (k_boundary.hip)
 int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt,
                   int64_t *nbound);
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t nbound, const double *sig, const double *cxyz,
                   const double *wgt, GView sol, bool fine);
+// Spectral evaluation of the same sums (k_bspec.hip); returns VPMX_ERR_UNSUP when the
+// block's faces are too long for its in-LDS FFTs, so the caller can use the lattice kernel.
+int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol);
+void free_bspec(Ctx *c);
 
 // Grid kernels (k_grid.hip)
 int fill(Ctx *c, double *p, int64_t n, double v);
