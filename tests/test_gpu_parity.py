@@ -199,3 +199,21 @@ def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
     with V.VPM(V.Params.make(0.125)) as c:
         got = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(got, ref) <= TOL
+
+
+@pytest.mark.parametrize("nn", [(145, 40, 44), (289, 21, 19)])
+def test_solve_many_batches(ctx, oracle, nn):
+    """Windows with more line batches than resident work-groups (every work-group loops
+    over several batches) and x lines long enough that a thread pre-processes several
+    slot pairs; Dirichlet values on all faces."""
+    rng = np.random.default_rng(sum(nn) + 7)
+    nn = np.array(nn, np.int32)
+    xb = np.array([-0.4, 0.2, -0.1, 0, 0, 0])
+    dpm = np.array([1 / 64, 1 / 50, 1 / 40])
+    rhs = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    sol = rng.standard_normal((nn[2], nn[1], nn[0], 3))
+    win = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
+    say("case: solve many batches", nn.tolist())
+    ref = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
+    got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, sol=sol)
+    assert rel_l2(got, ref) <= TOL

@@ -156,4 +156,84 @@ __device__ void fft_reg(double2 *buf, const double2 *tw, const Args &a) {
     }
 }
 
+// ------------------------------------------------------------------ one wave per sequence
+// FFTs of length L <= WAVE_LMAX where each sequence belongs to one 64-lane wave: the
+// stages need no work-group barrier (a wave's LDS operations execute in order; the wave
+// barrier only keeps the compiler from moving them).  Per stage: R radix, Ns span, mg a
+// multiplier with j / Ns == umulhi(j, mg) for j < 2^16 and Ns > 1 (mg = floor(2^32 / Ns) + 1;
+// the first stage, Ns = 1, divides by nothing).
+constexpr int WAVE_LMAX = 288;
+
+struct WavePlan {
+    int L, nstages;
+    int R[12], Ns[12];
+    unsigned mg[12];
+};
+
+template <int R, int BPL>
+__device__ __forceinline__ void wave_stage(double2 *buf, const double2 *__restrict__ tw, int L, int Ns, unsigned mg,
+                                           int lane) {
+    const int nbf = L / R, twstep = L / (Ns * R);
+    double2 v[BPL][R];
+    int dst[BPL];
+#pragma unroll
+    for (int i = 0; i < BPL; i++) {
+        const int j = lane + 64 * i;
+        dst[i] = -1;
+        if (j < nbf) {
+            const int q = Ns == 1 ? j : (int)__umulhi((unsigned)j, mg), k0 = j - q * Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i][r] = buf[j + r * nbf];
+            if (k0) {
+#pragma unroll
+                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+            }
+            if constexpr (R <= 4) dft_reg<R>(v[i]);
+            dst[i] = q * Ns * R + k0;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < BPL; i++)
+        if (dst[i] >= 0) {
+            if constexpr (R <= 4) {
+#pragma unroll
+                for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
+            } else {
+                dft_odd_store<R>(v[i], buf, dst[i], Ns, tw, L);
+            }
+        }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int R>
+constexpr int wave_bpl() { return (WAVE_LMAX / R + 63) / 64; }
+
+template <int MAXR>
+__device__ void wave_fft(double2 *buf, const double2 *tw, const WavePlan &P, int lane) {
+    for (int s = 0; s < P.nstages; s++) {
+        const int R = P.R[s], Ns = P.Ns[s];
+        const unsigned mg = P.mg[s];
+        if (R == 4) wave_stage<4, wave_bpl<4>()>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 2) wave_stage<2, wave_bpl<2>()>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 3) wave_stage<3, wave_bpl<3>()>(buf, tw, P.L, Ns, mg, lane);
+        else if constexpr (MAXR >= 7) {
+            if (R == 5) wave_stage<5, wave_bpl<5>()>(buf, tw, P.L, Ns, mg, lane);
+            else if (R == 7) wave_stage<7, wave_bpl<7>()>(buf, tw, P.L, Ns, mg, lane);
+            else if constexpr (MAXR >= 17) {
+                if (R == 11) wave_stage<11, wave_bpl<11>()>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 13) wave_stage<13, wave_bpl<13>()>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 17) wave_stage<17, wave_bpl<17>()>(buf, tw, P.L, Ns, mg, lane);
+                else if constexpr (MAXR >= 37) {
+                    if (R == 19) wave_stage<19, wave_bpl<19>()>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 23) wave_stage<23, wave_bpl<23>()>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 29) wave_stage<29, wave_bpl<29>()>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 31) wave_stage<31, wave_bpl<31>()>(buf, tw, P.L, Ns, mg, lane);
+                    else wave_stage<37, wave_bpl<37>()>(buf, tw, P.L, Ns, mg, lane);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace vpmx

@@ -42,6 +42,7 @@ struct DstArgs {
     double scale;
     const double *lam[3];  // eigenvalues (z pass)
     double norm;
+    const double *sn;      // sin(j pi / N), j <= N/2 (half-length passes)
 };
 
 // Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
@@ -369,6 +370,413 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_pass2(DstArgs a) {
     }
 }
 
+// ------------------------------------------------------------------ half-length passes
+// DST-I of length m through a complex FFT of length N = m + 1 instead of 2(m + 1) (the
+// sine-transform algorithm of Press et al., Numerical Recipes, sec. 12.3): with
+//   y_0 = 0,  y_j = sin(j pi / N) (x_j + x_{N-j}) + (x_j - x_{N-j}) / 2,   Y = FFT_N(y),
+// the transform F_k = sum_j x_j sin(pi j k / N) is
+//   F_{2k} = -Im Y_k,   F_{2k+1} = F_{2k-1} + Re Y_k,   F_1 = Re Y_0 / 2,
+// (Y with the exp(+) sign; our FFT has exp(-), which flips Im).  Two real lines share one
+// complex FFT, so a line costs N * 8 bytes of LDS: 16 lines of a 289-node block fit in
+// 42 KB, and the strided y and z passes read and write 128-byte runs (16 neighbouring
+// lines).  The odd outputs are a prefix sum, done by one wave per line pair.
+
+// y from x in place; slot j of pair p holds (x_j of line 2p, x_j of line 2p+1).
+// With fold (the x forward pass), slots 1 and N-1 first lose the x-end Dirichlet terms
+// fold[2l] * ih2 and fold[2l+1] * ih2 of each line l (fold in LDS).
+__device__ __forceinline__ void dst_pre(double2 *A, const double *__restrict__ sn, int N, int np,
+                                        const double *fold = nullptr, double ih2 = 0.0) {
+    const int H = N / 2;  // pairs (j, N-j), j = 1..H (j = N/2 maps to itself)
+    for (int t = threadIdx.x; t < np * H; t += DST_THREADS) {
+        const int p = t / H, j = t - p * H + 1;
+        double2 *B = A + p * N;
+        double2 x = B[j], y = B[N - j];
+        if (fold && j == 1) {
+            const double *fa = fold + 4 * p, *fb = fa + 2;  // lines 2p and 2p+1
+            if (N == 2) {  // one interior point: both x ends fold into slot 1
+                x = make_double2((x.x - fa[0] * ih2) - fa[1] * ih2, (x.y - fb[0] * ih2) - fb[1] * ih2);
+                y = x;
+            } else {
+                x = make_double2(x.x - fa[0] * ih2, x.y - fb[0] * ih2);
+                y = make_double2(y.x - fa[1] * ih2, y.y - fb[1] * ih2);
+            }
+        }
+        const double sj = sn[j];
+        const double2 sp = make_double2(sj * (x.x + y.x), sj * (x.y + y.y));
+        const double2 am = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
+        B[j] = cadd(sp, am);
+        B[N - j] = csub(sp, am);
+    }
+    for (int p = threadIdx.x; p < np; p += DST_THREADS) A[p * N] = make_double2(0.0, 0.0);
+}
+
+constexpr int DST_PPT = (DST_EMAX / 2 + 8 + DST_THREADS - 1) / DST_THREADS;
+
+// FFT output -> F_j in slot j (j = 1..N-1) of every pair.  Ends with a barrier.
+__device__ __forceinline__ void dst_post(double2 *A, int N, int np) {
+    const int H = N / 2 + 1;  // k = 0..N/2
+    double2 ev[DST_PPT], od[DST_PPT];
+    int base[DST_PPT], kk[DST_PPT];
+#pragma unroll
+    for (int u = 0; u < DST_PPT; u++) {
+        const int t = threadIdx.x + u * DST_THREADS;
+        base[u] = -1;
+        kk[u] = 0;
+        ev[u] = od[u] = make_double2(0.0, 0.0);
+        if (t < np * H) {
+            const int p = t / H, k = t - p * H;
+            const double2 z = A[p * N + k], zc = A[p * N + (k == 0 ? 0 : N - k)];
+            // line 2p: Y_k = (z + conj zc) / 2; line 2p+1: Y_k = (z - conj zc) / 2i
+            ev[u] = make_double2(-0.5 * (z.y - zc.y), 0.5 * (z.x - zc.x));  // F_{2k} = -Im Y_k (exp(+))
+            od[u] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));   // Re Y_k
+            if (k == 0) od[u] = make_double2(0.5 * od[u].x, 0.5 * od[u].y);
+            base[u] = p * N;
+            kk[u] = k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < DST_PPT; u++) {
+        if (base[u] < 0) continue;
+        const int k = kk[u];
+        if (k >= 1 && 2 * k <= N - 1) A[base[u] + 2 * k] = ev[u];
+        if (2 * k + 1 <= N - 1) A[base[u] + 2 * k + 1] = od[u];
+    }
+    __syncthreads();
+    // odd slots: running sum over k, one wave per pair: each lane sums a contiguous chunk,
+    // the chunk totals are scanned across the wave
+    const int K = N / 2, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int C = (K + 63) / 64;
+    for (int p = wave; p < np; p += DST_THREADS / 64) {
+        double2 *B = A + p * N;
+        const int k0 = lane * C < K ? lane * C : K, k1 = k0 + C < K ? k0 + C : K;
+        double2 s = make_double2(0.0, 0.0);
+        for (int k = k0; k < k1; k++) s = cadd(s, B[2 * k + 1]);
+        double2 inc = s;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double ox = __shfl_up(inc.x, d, 64), oy = __shfl_up(inc.y, d, 64);
+            if (lane >= d) inc = make_double2(inc.x + ox, inc.y + oy);
+        }
+        double2 run = make_double2(__shfl_up(inc.x, 1, 64), __shfl_up(inc.y, 1, 64));
+        if (lane == 0) run = make_double2(0.0, 0.0);
+        for (int k = k0; k < k1; k++) {
+            run = cadd(run, B[2 * k + 1]);
+            B[2 * k + 1] = run;
+        }
+    }
+    __syncthreads();
+}
+
+// Values of one batch of lines held by a thread between the prefetch and the LDS write.
+constexpr int DST_PF = (2 * DST_EMAX + DST_THREADS - 1) / DST_THREADS;
+
+// Element u of this thread in a batch of nl lines: x passes walk rows (neighbouring
+// threads take neighbouring points), y and z passes walk columns (neighbouring threads
+// take neighbouring lines, 8*nl contiguous bytes per point).
+template <int MODE>
+__device__ __forceinline__ void dst_elem(int m, int nl, int u, int *line, int *pt) {
+    if (MODE == DST_XF || MODE == DST_XI) {
+        const int e = threadIdx.x + u * DST_THREADS;
+        *line = e / m;
+        *pt = e - *line * m;
+    } else {
+        *line = threadIdx.x & (nl - 1);
+        *pt = threadIdx.x / nl + u * (DST_THREADS / nl);
+    }
+}
+
+// Global offset (within a component) of point pt of line gl of the work array.
+template <int MODE>
+__device__ __forceinline__ int64_t dst_addr(const DstArgs &a, int gl, int pt) {
+    if (MODE == DST_XF || MODE == DST_XI) return (int64_t)gl * a.mx + pt;
+    if (MODE == DST_Y) return (int64_t)(gl % a.mx) + (int64_t)a.mx * a.my * (gl / a.mx) + (int64_t)pt * a.mx;
+    return (int64_t)gl + (int64_t)pt * a.mx * a.my;
+}
+
+// Issue the loads of batch b into registers (no wait).  For the x forward pass, thread
+// l < nl also fetches the Dirichlet values beyond both x ends of line l: fx = (x0, x1).
+template <int MODE>
+__device__ __forceinline__ void dst_prefetch(const DstArgs &a, const double *w, int comp, int b, int nl,
+                                             double (&v)[DST_PF], double (&fx)[2]) {
+#pragma unroll
+    for (int u = 0; u < DST_PF; u++) {
+        int line, pt;
+        dst_elem<MODE>(a.m, nl, u, &line, &pt);
+        const int gl = b * nl + line;
+        v[u] = 0.0;
+        if (line < nl && pt < a.m && gl < a.nlines) {
+            if (MODE == DST_XF) {
+                const int j = gl % a.my, k = gl / a.my;
+                v[u] = a.rhs.p[comp * a.rhs.cs + a.rhs.at(pt + 1, j + 1, k + 1)];
+            } else {
+                v[u] = w[dst_addr<MODE>(a, gl, pt)];
+            }
+        }
+    }
+    if (MODE == DST_XF) {
+        fx[0] = fx[1] = 0.0;
+        const int gl = b * nl + (int)threadIdx.x;
+        if (a.has_bc && (int)threadIdx.x < nl && gl < a.nlines) {
+            const int j = gl % a.my, k = gl / a.my;
+            const double *bp = a.bc.p + comp * a.bc.cs;
+            fx[0] = bp[a.bc.at(0, j + 1, k + 1)];
+            fx[1] = bp[a.bc.at(a.mx + 1, j + 1, k + 1)];
+        }
+    }
+}
+
+// Persistent, software-pipelined passes: each work-group walks batches b, b + gridDim.x,
+// ...; the loads of the next batch are in flight while the current one is transformed.
+template <int MODE, int MAXR>
+__global__ void __launch_bounds__(DST_THREADS) k_dst_pass3(DstArgs a) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    __shared__ double lamxy[16];
+    __shared__ double foldx[32];
+    const int N = a.L, m = a.m, np = a.npairs, nl = 2 * np;
+    const int nbatch = (int)((a.nlines + nl - 1) / nl);
+    double2 *A = lds;
+    double2 *twl = lds + np * N;  // exp(-2 pi i t / N)
+    double *Ad = (double *)A;
+    const int comp = blockIdx.y;
+    double *w = a.w + comp * a.wcs;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < N; t += DST_THREADS) twl[t] = a.tw[t];
+    double v[DST_PF], fx[2];
+    int b = blockIdx.x;
+    if (b < nbatch) dst_prefetch<MODE>(a, w, comp, b, nl, v, fx);
+    for (; b < nbatch; b += gridDim.x) {
+        // registers -> LDS slot pt+1 (real part: even line, imaginary part: odd line)
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++) {
+            int line, pt;
+            dst_elem<MODE>(m, nl, u, &line, &pt);
+            if (line >= nl || pt >= m) continue;
+            double val = v[u];
+            if (MODE == DST_XF && a.has_bc) {
+                const int gl = b * nl + line;
+                if (gl < a.nlines) {
+                    const int j = gl % a.my, k = gl / a.my;
+                    const double *bp = a.bc.p + comp * a.bc.cs;
+                    if (j == 0) val -= bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
+                    if (j == a.my - 1) val -= bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
+                    if (k == 0) val -= bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
+                    if (k == a.mz - 1) val -= bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
+                }
+            }
+            Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] = val;
+        }
+        if (MODE == DST_XF && tid < nl) {
+            foldx[2 * tid] = fx[0];
+            foldx[2 * tid + 1] = fx[1];
+        }
+        if (MODE == DST_Z && tid < nl) {
+            const int gl = b * nl + tid;
+            double lv = 0.0;
+            if (gl < a.nlines) lv = a.lam[0][gl % a.mx] + a.lam[1][gl / a.mx];
+            lamxy[tid] = lv;
+        }
+        __syncthreads();
+        if (b + (int)gridDim.x < nbatch) dst_prefetch<MODE>(a, w, comp, b + gridDim.x, nl, v, fx);
+        if (MODE == DST_XF && a.has_bc) dst_pre(A, a.sn, N, np, foldx, a.ih2[0]);
+        else dst_pre(A, a.sn, N, np);
+        __syncthreads();
+        fft_reg<MAXR>(A, twl, a);
+        dst_post(A, N, np);
+        if (MODE == DST_Z) {
+            for (int t = tid; t < np * m; t += DST_THREADS) {
+                const int p = t / m, j = t - p * m + 1;
+                const double lz = a.lam[2][j - 1];
+                const double fa = a.norm / (-(lamxy[2 * p] + lz)), fb = a.norm / (-(lamxy[2 * p + 1] + lz));
+                const double2 z = A[p * N + j];
+                A[p * N + j] = make_double2(z.x * fa, z.y * fb);
+            }
+            __syncthreads();
+            dst_pre(A, a.sn, N, np);
+            __syncthreads();
+            fft_reg<MAXR>(A, twl, a);
+            dst_post(A, N, np);
+        }
+        // LDS slot pt+1 -> global
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++) {
+            int line, pt;
+            dst_elem<MODE>(m, nl, u, &line, &pt);
+            const int gl = b * nl + line;
+            if (line >= nl || pt >= m || gl >= a.nlines) continue;
+            const double val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * a.scale;
+            if (MODE == DST_XI) {
+                const int j = gl % a.my, k = gl / a.my;
+                a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = val;
+            } else {
+                w[dst_addr<MODE>(a, gl, pt)] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ one wave per line pair
+// The passes of lines up to WAVE_LMAX - 1 points: 16 lines (8 pairs) per batch, wave w
+// transforms pair w alone (pre-processing, FFT, odd-output scan, and for z the diagonal
+// and the inverse), so a batch needs three work-group barriers -- after the LDS fill,
+// before the stores, after the stores.  Batches are pipelined as in k_dst_pass3.
+
+__device__ __forceinline__ void wave_pre(double2 *B, const double *__restrict__ sn, int N, int lane,
+                                         const double *fold, double ih2) {
+    const int H = N / 2;
+    for (int j = 1 + lane; j <= H; j += 64) {
+        double2 x = B[j], y = B[N - j];
+        if (fold && j == 1) {  // fold = (x0, x1) of line 2p, then of line 2p+1
+            if (N == 2) {
+                x = make_double2((x.x - fold[0] * ih2) - fold[1] * ih2, (x.y - fold[2] * ih2) - fold[3] * ih2);
+                y = x;
+            } else {
+                x = make_double2(x.x - fold[0] * ih2, x.y - fold[2] * ih2);
+                y = make_double2(y.x - fold[1] * ih2, y.y - fold[3] * ih2);
+            }
+        }
+        const double sj = sn[j];
+        const double2 sp = make_double2(sj * (x.x + y.x), sj * (x.y + y.y));
+        const double2 am = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
+        B[j] = cadd(sp, am);
+        B[N - j] = csub(sp, am);
+    }
+    if (lane == 0) B[0] = make_double2(0.0, 0.0);
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void wave_post(double2 *B, int N, int lane) {
+    constexpr int PPL = (WAVE_LMAX / 2 + 1 + 63) / 64;
+    const int H = N / 2 + 1;
+    double2 ev[PPL], od[PPL];
+#pragma unroll
+    for (int i = 0; i < PPL; i++) {
+        const int k = lane + 64 * i;
+        ev[i] = od[i] = make_double2(0.0, 0.0);
+        if (k < H) {
+            const double2 z = B[k], zc = B[k == 0 ? 0 : N - k];
+            ev[i] = make_double2(-0.5 * (z.y - zc.y), 0.5 * (z.x - zc.x));
+            od[i] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
+            if (k == 0) od[i] = make_double2(0.5 * od[i].x, 0.5 * od[i].y);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < PPL; i++) {
+        const int k = lane + 64 * i;
+        if (k >= H) continue;
+        if (k >= 1 && 2 * k <= N - 1) B[2 * k] = ev[i];
+        if (2 * k + 1 <= N - 1) B[2 * k + 1] = od[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int K = N / 2, C = (K + 63) / 64;
+    const int k0 = lane * C < K ? lane * C : K, k1 = k0 + C < K ? k0 + C : K;
+    double2 s = make_double2(0.0, 0.0);
+    for (int k = k0; k < k1; k++) s = cadd(s, B[2 * k + 1]);
+    double2 inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double ox = __shfl_up(inc.x, d, 64), oy = __shfl_up(inc.y, d, 64);
+        if (lane >= d) inc = make_double2(inc.x + ox, inc.y + oy);
+    }
+    double2 run = make_double2(__shfl_up(inc.x, 1, 64), __shfl_up(inc.y, 1, 64));
+    if (lane == 0) run = make_double2(0.0, 0.0);
+    for (int k = k0; k < k1; k++) {
+        run = cadd(run, B[2 * k + 1]);
+        B[2 * k + 1] = run;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <int MODE, int MAXR>
+__global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    __shared__ double lamxy[16];
+    __shared__ double foldx[32];
+    constexpr int nl = 16;
+    const int N = P.L, m = a.m;
+    const int nbatch = (int)((a.nlines + nl - 1) / nl);
+    double2 *A = lds;
+    double2 *twl = lds + 8 * N;  // exp(-2 pi i t / N)
+    double *Ad = (double *)A;
+    const int comp = blockIdx.y;
+    double *w = a.w + comp * a.wcs;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double2 *B = A + wave * N;
+    for (int t = tid; t < N; t += DST_THREADS) twl[t] = a.tw[t];
+    double v[DST_PF], fx[2];
+    int b = blockIdx.x;
+    if (b < nbatch) dst_prefetch<MODE>(a, w, comp, b, nl, v, fx);
+    for (; b < nbatch; b += gridDim.x) {
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++) {
+            int line, pt;
+            dst_elem<MODE>(m, nl, u, &line, &pt);
+            if (line >= nl || pt >= m) continue;
+            double val = v[u];
+            if (MODE == DST_XF && a.has_bc) {
+                const int gl = b * nl + line;
+                if (gl < a.nlines) {
+                    const int j = gl % a.my, k = gl / a.my;
+                    const double *bp = a.bc.p + comp * a.bc.cs;
+                    if (j == 0) val -= bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
+                    if (j == a.my - 1) val -= bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
+                    if (k == 0) val -= bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
+                    if (k == a.mz - 1) val -= bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
+                }
+            }
+            Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] = val;
+        }
+        if (MODE == DST_XF && tid < nl) {
+            foldx[2 * tid] = fx[0];
+            foldx[2 * tid + 1] = fx[1];
+        }
+        if (MODE == DST_Z && tid < nl) {
+            const int gl = b * nl + tid;
+            double lv = 0.0;
+            if (gl < a.nlines) lv = a.lam[0][gl % a.mx] + a.lam[1][gl / a.mx];
+            lamxy[tid] = lv;
+        }
+        __syncthreads();
+        if (b + (int)gridDim.x < nbatch) dst_prefetch<MODE>(a, w, comp, b + gridDim.x, nl, v, fx);
+        wave_pre(B, a.sn, N, lane, (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr, a.ih2[0]);
+        wave_fft<MAXR>(B, twl, P, lane);
+        wave_post(B, N, lane);
+        if (MODE == DST_Z) {
+            const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
+            for (int j = 1 + lane; j <= m; j += 64) {
+                const double lz = a.lam[2][j - 1];
+                const double fa = a.norm / (-(la + lz)), fb = a.norm / (-(lb + lz));
+                const double2 z = B[j];
+                B[j] = make_double2(z.x * fa, z.y * fb);
+            }
+            __builtin_amdgcn_wave_barrier();
+            wave_pre(B, a.sn, N, lane, nullptr, 0.0);
+            wave_fft<MAXR>(B, twl, P, lane);
+            wave_post(B, N, lane);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++) {
+            int line, pt;
+            dst_elem<MODE>(m, nl, u, &line, &pt);
+            const int gl = b * nl + line;
+            if (line >= nl || pt >= m || gl >= a.nlines) continue;
+            const double val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * a.scale;
+            if (MODE == DST_XI) {
+                const int j = gl % a.my, k = gl / a.my;
+                a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = val;
+            } else {
+                w[dst_addr<MODE>(a, gl, pt)] = val;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
 __global__ void k_zero_faces(GView out, int nx, int ny, int nz, int ncomp) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -441,6 +849,8 @@ void free_fft_plans(Ctx *c) {
     c->fft_plans.clear();
     for (auto &kv : c->eig_cache) (void)hipFree(kv.second);
     c->eig_cache.clear();
+    for (auto &kv : c->sin_cache) (void)hipFree(kv.second);
+    c->sin_cache.clear();
 }
 
 // Eigenvalues 4 sin^2(pi k / (2(m+1))) / h^2 of the 1-D Dirichlet second difference,
@@ -480,7 +890,7 @@ static int radix_class(const DstArgs &a) {
 
 template <int MODE>
 static int launch_pass(Ctx *c, DstArgs a, int ncomp) {
-    const int rc = c->dst_v1 ? 0 : radix_class(a);
+    const int rc = c->dst_kernel == 1 ? 0 : radix_class(a);
     if (rc == 0) {
         int np = (int)(65536 / (sizeof(double2) * 2 * (size_t)a.L));
         a.npairs = np > 8 ? 8 : (np < 1 ? 1 : np);
@@ -515,6 +925,99 @@ static void setup_axis(DstArgs &a, FftPlan *p, int m) {
     a.npairs = 1;
 }
 
+// sin(j pi / N), j = 0..N/2, uploaded once per N.
+static int sine_table(Ctx *c, int N, const double **out) {
+    auto it = c->sin_cache.find(N);
+    if (it != c->sin_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    std::vector<double> v(N / 2 + 1);
+    for (int j = 0; j <= N / 2; j++)
+        v[j] = (double)sinl(3.141592653589793238462643383279502884L * (long double)j / (long double)N);
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * v.size()));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    c->sin_cache[N] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
+}
+
+template <int MODE>
+static int launch_pass3(Ctx *c, DstArgs a, int ncomp, int rc) {
+    int np = 1;
+    while (np < 8 && 2 * np * a.L <= DST_EMAX && (2 * np + 1) * a.L <= DST_LDS_MAX) np *= 2;
+    a.npairs = np;
+    const size_t lds = sizeof(double2) * (size_t)(np + 1) * a.L;
+    // persistent grid: the work-groups the device holds at once, batches looped inside
+    const int64_t nbatch = (a.nlines + 2 * np - 1) / (2 * np);
+    int64_t resident = (int64_t)c->num_cu * c->dst_wg_per_cu / (ncomp > 0 ? ncomp : 1);
+    if (resident < 1) resident = 1;
+    dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
+    switch (rc) {
+        case 4: hipLaunchKernelGGL((k_dst_pass3<MODE, 4>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+        case 7: hipLaunchKernelGGL((k_dst_pass3<MODE, 7>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+        case 17: hipLaunchKernelGGL((k_dst_pass3<MODE, 17>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+        default: hipLaunchKernelGGL((k_dst_pass3<MODE, 37>), grid, dim3(DST_THREADS), lds, c->stream, a); break;
+    }
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// One pass along an axis of interior length m: the half-length kernel (FFT of m+1) when its
+// plan fits, else the odd-extension kernels (FFT of 2(m+1)).
+struct AxisPlan {
+    int m;
+    FftPlan *full, *half;
+    const double *sn;
+};
+
+template <int MODE>
+static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) {
+    WavePlan P{};
+    P.L = fp->L;
+    P.nstages = fp->nstages;
+    int ns = 1;
+    for (int s = 0; s < fp->nstages; s++) {
+        P.R[s] = fp->radix[s];
+        P.Ns[s] = ns;
+        P.mg[s] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns + 1ull) : 0u;
+        ns *= fp->radix[s];
+    }
+    a.npairs = 8;
+    const size_t lds = sizeof(double2) * (size_t)9 * a.L;
+    const int64_t nbatch = (a.nlines + 15) / 16;
+    int64_t resident = (int64_t)c->num_cu * c->dst_wg_per_cu / (ncomp > 0 ? ncomp : 1);
+    if (resident < 1) resident = 1;
+    dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
+    switch (rc) {
+        case 4: hipLaunchKernelGGL((k_dst_wave<MODE, 4>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        case 7: hipLaunchKernelGGL((k_dst_wave<MODE, 7>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        case 17: hipLaunchKernelGGL((k_dst_wave<MODE, 17>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        default: hipLaunchKernelGGL((k_dst_wave<MODE, 37>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+    }
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+template <int MODE>
+static int run_pass(Ctx *c, DstArgs a, int ncomp, const AxisPlan &P) {
+    if (c->dst_kernel == 4 && P.m + 1 <= WAVE_LMAX && P.half->nstages <= 12) {
+        setup_axis(a, P.half, P.m);
+        a.sn = P.sn;
+        const int rc = radix_class(a);
+        if (rc) return launch_wave<MODE>(c, a, ncomp, rc, P.half);
+    }
+    if (c->dst_kernel >= 3) {
+        setup_axis(a, P.half, P.m);
+        a.sn = P.sn;
+        const int rc = radix_class(a);
+        if (rc) return launch_pass3<MODE>(c, a, ncomp, rc);
+    }
+    setup_axis(a, P.full, P.m);
+    return launch_pass<MODE>(c, a, ncomp);
+}
+
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out) {
     int st = VPMX_OK;
     int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
@@ -526,10 +1029,13 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     int64_t mt = (int64_t)m[0] * m[1] * m[2];
     double *w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)mt * ncomp, &st);
     if (st) return st;
-    FftPlan *px, *py, *pz;
-    VPMX_TRY(get_plan(c, 2 * (m[0] + 1), &px));
-    VPMX_TRY(get_plan(c, 2 * (m[1] + 1), &py));
-    VPMX_TRY(get_plan(c, 2 * (m[2] + 1), &pz));
+    AxisPlan ax[3];
+    for (int d = 0; d < 3; d++) {
+        ax[d].m = m[d];
+        VPMX_TRY(get_plan(c, 2 * (m[d] + 1), &ax[d].full));
+        VPMX_TRY(get_plan(c, m[d] + 1, &ax[d].half));
+        VPMX_TRY(sine_table(c, m[d] + 1, &ax[d].sn));
+    }
     DstArgs a{};
     a.mx = m[0]; a.my = m[1]; a.mz = m[2];
     a.wcs = mt;
@@ -543,27 +1049,22 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     VPMX_TRY(eigen(c, m[1], h[1], c->lam[1], &a.lam[1]));
     VPMX_TRY(eigen(c, m[2], h[2], c->lam[2], &a.lam[2]));
     a.norm = (2.0 / (m[0] + 1)) * (2.0 / (m[1] + 1)) * (2.0 / (m[2] + 1));
-    // x forward (with boundary folding)
-    setup_axis(a, px, m[0]);
-    a.nlines = (int64_t)m[1] * m[2];
     a.scale = 1.0;
-    VPMX_TRY(launch_pass<DST_XF>(c, a, ncomp));
-    // y forward
-    setup_axis(a, py, m[1]);
-    a.nlines = (int64_t)m[0] * m[2];
-    VPMX_TRY(launch_pass<DST_Y>(c, a, ncomp));
-    // z forward + diagonal + inverse
-    setup_axis(a, pz, m[2]);
-    a.nlines = (int64_t)m[0] * m[1];
-    VPMX_TRY(launch_pass<DST_Z>(c, a, ncomp));
-    // y inverse
-    setup_axis(a, py, m[1]);
-    a.nlines = (int64_t)m[0] * m[2];
-    VPMX_TRY(launch_pass<DST_Y>(c, a, ncomp));
-    // x inverse -> output interior
-    setup_axis(a, px, m[0]);
+    // x forward (with boundary folding)
     a.nlines = (int64_t)m[1] * m[2];
-    VPMX_TRY(launch_pass<DST_XI>(c, a, ncomp));
+    VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0]));
+    // y forward
+    a.nlines = (int64_t)m[0] * m[2];
+    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1]));
+    // z forward + diagonal + inverse
+    a.nlines = (int64_t)m[0] * m[1];
+    VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2]));
+    // y inverse
+    a.nlines = (int64_t)m[0] * m[2];
+    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1]));
+    // x inverse -> output interior
+    a.nlines = (int64_t)m[1] * m[2];
+    VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0]));
     if (!bc) {
         int64_t tot = 2 * ((int64_t)wn[0] * wn[1] + (int64_t)wn[0] * wn[2] + (int64_t)wn[1] * wn[2]);
         hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, out, wn[0], wn[1],

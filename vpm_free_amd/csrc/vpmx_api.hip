@@ -365,7 +365,14 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     if (bm && std::string(bm) == "direct") c->bs_mode = 1;
     if (bm && std::string(bm) == "lattice") c->bs_mode = 0;
     const char *dm = getenv("VPMX_DST");
-    if (dm && std::string(dm) == "v1") c->dst_v1 = true;
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        c->num_cu = ncu;
+    const char *wpc = getenv("VPMX_DST_WG_PER_CU");
+    if (wpc && atoi(wpc) > 0) c->dst_wg_per_cu = atoi(wpc);
+    if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
+    if (dm && std::string(dm) == "v2") c->dst_kernel = 2;
+    if (dm && std::string(dm) == "v3") c->dst_kernel = 3;
     for (int k = 0; k < K_COUNT; k++)
         if (hipEventCreate(&c->kev0[k]) != hipSuccess || hipEventCreate(&c->kev1[k]) != hipSuccess) c->kev_ok = false;
     *out = c;

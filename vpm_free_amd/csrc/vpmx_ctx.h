@@ -45,7 +45,11 @@ struct Ctx : vpmx_ctx {
     int bs_mode = 2;  // fine boundary sums: 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx = nullptr;   // spectral boundary-sum plan of the current block geometry
     void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
-    bool dst_v1 = false;  // DST passes: force the two-buffer kernel
+    int num_cu = 256;       // compute units of the device
+    int dst_wg_per_cu = 1;  // resident work-groups per CU assumed by the persistent DST passes
+    int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length
+                          // work-group FFTs, 2 odd extension, 1 two-buffer
+    std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N
     std::map<std::pair<int, double>, void *> eig_cache;  // DST-I eigenvalues per (m, h)
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;
     DBuf dst_work, lam[3];
