@@ -35,6 +35,7 @@ def parse():
     ap.add_argument("--h", type=float, default=1.0 / 64)
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", type=int, default=0, help="0 plain FP64 form, 1 mixed form")
     ap.add_argument("--cpu-stride", type=int, default=256,
                     help="boundary-sum target stride of the bounded CPU sample")
     return ap.parse_args()
@@ -109,6 +110,7 @@ def main():
     log(f"workload: {xp.shape[0]} particles generated in {time.perf_counter() - t0:.1f}s")
     params = V.Params.make(args.h)
     ctx = V.VPM(params)
+    ctx.set_precision(args.precision)
     ctx.upload(xp, qp)
     ctx.vpm_resident(0, 0)  # define_sizes (NTIME=0): box fixed from here on (idefine=1)
     ctx.remesh_resident(1)  # initial remesh, as the reference driver does before its loop

@@ -217,3 +217,34 @@ def test_solve_many_batches(ctx, oracle, nn):
     ref = oracle.solve_window(xb, dpm, nn, win, rhs, sol=sol)
     got = ctx.stage_solve(xb, dpm, nn, 3, win, rhs, sol=sol)
     assert rel_l2(got, ref) <= TOL
+
+
+MIXED_TOL = 1e-5
+
+
+@pytest.mark.parametrize("h,sigma", [(1 / 16, 0.1)])
+def test_vpm_mode2_mixed(vpmlib, oracle, h, sigma):
+    """Mixed form (FP32 Poisson passes, FP64 boundary sums) against the FP64 oracle: every
+    field and the particle velocities within relative L2 1e-5."""
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 mixed ring h=%g sigma=%g n=%d" % (h, sigma, xp.shape[0]))
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    with V.VPM(V.Params.make(h)) as c:
+        c.set_precision(1)
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+        errs = {
+            "psi": rel_l2(c.get_grid(1), f["psi"]),
+            "vel": rel_l2(c.get_grid(3), f["vel"]),
+            "sol": rel_l2(c.get_grid(2), f["sol"]),
+            "up": rel_l2(up, up_r),
+            "gp": rel_l2(gp, gp_r),
+        }
+    say("  rel L2 vs FP64:", errs)
+    for k, v in errs.items():
+        assert v <= MIXED_TOL, (k, v)
+    assert errs["up"] > 0.0  # the FP32 passes ran

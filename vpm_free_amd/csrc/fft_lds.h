@@ -12,6 +12,22 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Scalar type and constructor of a complex vector type.
+template <class C> struct Cx;
+template <> struct Cx<double2> {
+    using R = double;
+    static __device__ __forceinline__ double2 mk(double x, double y) { return make_double2(x, y); }
+};
+template <> struct Cx<float2> {
+    using R = float;
+    static __device__ __forceinline__ float2 mk(float x, float y) { return make_float2(x, y); }
+};
 
 // ------------------------------------------------------------------ register-staged stages
 // Every stage loads each butterfly's R inputs into registers, synchronises, and writes the
@@ -21,23 +37,24 @@ constexpr int DST_LDS_MAX = 4096;  // complex elements of LDS per work-group, li
 constexpr int DST_THREADS = 512;
 
 // In-register butterflies of the specialised radices (2, 3, 4).
-template <int R>
-__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
+template <int R, class C>
+__device__ __forceinline__ void dft_reg(C (&v)[R]) {
+    using T = typename Cx<C>::R;
     if constexpr (R == 2) {
-        double2 a = v[0], b = v[1];
+        C a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
     } else if constexpr (R == 3) {
-        const double c3 = -0.5, s3 = 0.86602540378443864676;
-        double2 t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
-        double2 base = make_double2(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
-        double2 rot = make_double2(s3 * t2.y, -s3 * t2.x);
+        const T c3 = (T)-0.5, s3 = (T)0.86602540378443864676;
+        C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+        C base = Cx<C>::mk(v[0].x + c3 * t1.x, v[0].y + c3 * t1.y);
+        C rot = Cx<C>::mk(s3 * t2.y, -s3 * t2.x);
         v[0] = cadd(v[0], t1);
         v[1] = cadd(base, rot);
         v[2] = csub(base, rot);
     } else if constexpr (R == 4) {
-        double2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]), s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-        double2 md = make_double2(d13.y, -d13.x);
+        C s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]), s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        C md = Cx<C>::mk(d13.y, -d13.x);
         v[0] = cadd(s02, s13);
         v[1] = cadd(d02, md);
         v[2] = csub(s02, s13);
@@ -51,12 +68,12 @@ __device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
 //   X_k   = v_0 + sum_r (c p_r.x + s m_r.y,  c p_r.y - s m_r.x)
 //   X_R-k = v_0 + sum_r (c p_r.x - s m_r.y,  c p_r.y + s m_r.x)
 // so two outputs cost 4 FMA per r.  c and s come from the LDS twiddle table.
-template <int R>
-__device__ __forceinline__ void dft_odd_store(double2 (&v)[R], double2 *buf, int d, int Ns,
-                                              const double2 *__restrict__ tw, int L) {
+template <int R, class C>
+__device__ __forceinline__ void dft_odd_store(C (&v)[R], C *buf, int d, int Ns, const C *__restrict__ tw, int L) {
+    using T = typename Cx<C>::R;
     constexpr int H = (R - 1) / 2;
-    double2 p[H], mm[H];
-    double2 x0 = v[0];
+    C p[H], mm[H];
+    C x0 = v[0];
 #pragma unroll
     for (int r = 1; r <= H; r++) {
         p[r - 1] = cadd(v[r], v[R - r]);
@@ -68,21 +85,21 @@ __device__ __forceinline__ void dft_odd_store(double2 (&v)[R], double2 *buf, int
     // k is not unrolled: the H twiddles of one k are loaded per iteration, not all H*H at once
 #pragma unroll 1
     for (int k = 1; k <= H; k++) {
-        double X = v[0].x, Y = v[0].y, U = 0.0, V = 0.0;
+        T X = v[0].x, Y = v[0].y, U = 0, V = 0;
         int q = 0;  // (r * k) mod R
 #pragma unroll
         for (int r = 1; r <= H; r++) {
             q += k;
             if (q >= R) q -= R;
-            const double2 w = tw[q * Lr];  // (c, -s)
-            const double c = w.x, s = -w.y;
+            const C w = tw[q * Lr];  // (c, -s)
+            const T c = w.x, s = -w.y;
             X += c * p[r - 1].x;
             Y += c * p[r - 1].y;
             U += s * mm[r - 1].y;
             V += s * mm[r - 1].x;
         }
-        buf[d + k * Ns] = make_double2(X + U, Y - V);
-        buf[d + (R - k) * Ns] = make_double2(X - U, Y + V);
+        buf[d + k * Ns] = Cx<C>::mk(X + U, Y - V);
+        buf[d + (R - k) * Ns] = Cx<C>::mk(X - U, Y + V);
     }
 }
 
@@ -170,11 +187,10 @@ struct WavePlan {
     unsigned mg[12];
 };
 
-template <int R, int BPL>
-__device__ __forceinline__ void wave_stage(double2 *buf, const double2 *__restrict__ tw, int L, int Ns, unsigned mg,
-                                           int lane) {
+template <int R, int BPL, class C>
+__device__ __forceinline__ void wave_stage(C *buf, const C *__restrict__ tw, int L, int Ns, unsigned mg, int lane) {
     const int nbf = L / R, twstep = L / (Ns * R);
-    double2 v[BPL][R];
+    C v[BPL][R];
     int dst[BPL];
 #pragma unroll
     for (int i = 0; i < BPL; i++) {
@@ -209,27 +225,27 @@ __device__ __forceinline__ void wave_stage(double2 *buf, const double2 *__restri
 template <int R>
 constexpr int wave_bpl() { return (WAVE_LMAX / R + 63) / 64; }
 
-template <int MAXR>
-__device__ void wave_fft(double2 *buf, const double2 *tw, const WavePlan &P, int lane) {
+template <int MAXR, class C>
+__device__ void wave_fft(C *buf, const C *tw, const WavePlan &P, int lane) {
     for (int s = 0; s < P.nstages; s++) {
         const int R = P.R[s], Ns = P.Ns[s];
         const unsigned mg = P.mg[s];
-        if (R == 4) wave_stage<4, wave_bpl<4>()>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 2) wave_stage<2, wave_bpl<2>()>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 3) wave_stage<3, wave_bpl<3>()>(buf, tw, P.L, Ns, mg, lane);
+        if (R == 4) wave_stage<4, wave_bpl<4>(), C>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 2) wave_stage<2, wave_bpl<2>(), C>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 3) wave_stage<3, wave_bpl<3>(), C>(buf, tw, P.L, Ns, mg, lane);
         else if constexpr (MAXR >= 7) {
-            if (R == 5) wave_stage<5, wave_bpl<5>()>(buf, tw, P.L, Ns, mg, lane);
-            else if (R == 7) wave_stage<7, wave_bpl<7>()>(buf, tw, P.L, Ns, mg, lane);
+            if (R == 5) wave_stage<5, wave_bpl<5>(), C>(buf, tw, P.L, Ns, mg, lane);
+            else if (R == 7) wave_stage<7, wave_bpl<7>(), C>(buf, tw, P.L, Ns, mg, lane);
             else if constexpr (MAXR >= 17) {
-                if (R == 11) wave_stage<11, wave_bpl<11>()>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 13) wave_stage<13, wave_bpl<13>()>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 17) wave_stage<17, wave_bpl<17>()>(buf, tw, P.L, Ns, mg, lane);
+                if (R == 11) wave_stage<11, wave_bpl<11>(), C>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 13) wave_stage<13, wave_bpl<13>(), C>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 17) wave_stage<17, wave_bpl<17>(), C>(buf, tw, P.L, Ns, mg, lane);
                 else if constexpr (MAXR >= 37) {
-                    if (R == 19) wave_stage<19, wave_bpl<19>()>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 23) wave_stage<23, wave_bpl<23>()>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 29) wave_stage<29, wave_bpl<29>()>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 31) wave_stage<31, wave_bpl<31>()>(buf, tw, P.L, Ns, mg, lane);
-                    else wave_stage<37, wave_bpl<37>()>(buf, tw, P.L, Ns, mg, lane);
+                    if (R == 19) wave_stage<19, wave_bpl<19>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 23) wave_stage<23, wave_bpl<23>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 29) wave_stage<29, wave_bpl<29>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 31) wave_stage<31, wave_bpl<31>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else wave_stage<37, wave_bpl<37>(), C>(buf, tw, P.L, Ns, mg, lane);
                 }
             }
         }

@@ -15,6 +15,7 @@
 // leaving LDS, so one solve costs 5 sweeps over the interior.
 #include <cmath>
 #include <map>
+#include <type_traits>
 
 #include "fft_lds.h"
 #include "vpmx_ctx.h"
@@ -471,15 +472,16 @@ __device__ __forceinline__ void dst_post(double2 *A, int N, int np) {
 // Values of one batch of lines held by a thread between the prefetch and the LDS write.
 constexpr int DST_PF = (2 * DST_EMAX + DST_THREADS - 1) / DST_THREADS;
 
-// Element u of this thread in a batch of nl lines: x passes walk rows (neighbouring
-// threads take neighbouring points), y and z passes walk columns (neighbouring threads
-// take neighbouring lines, 8*nl contiguous bytes per point).
+// Element u of this thread in a batch of nl lines (nl a power of two): x passes give each
+// line DST_THREADS / nl neighbouring threads, which walk its points (the line index is
+// fixed per thread); y and z passes walk columns (neighbouring threads take neighbouring
+// lines, 8*nl contiguous bytes per point).  Both cover m <= DST_PF * DST_THREADS / nl.
 template <int MODE>
 __device__ __forceinline__ void dst_elem(int m, int nl, int u, int *line, int *pt) {
     if (MODE == DST_XF || MODE == DST_XI) {
-        const int e = threadIdx.x + u * DST_THREADS;
-        *line = e / m;
-        *pt = e - *line * m;
+        const int tpl = DST_THREADS / nl;
+        *line = threadIdx.x / tpl;
+        *pt = (threadIdx.x & (tpl - 1)) + u * tpl;
     } else {
         *line = threadIdx.x & (nl - 1);
         *pt = threadIdx.x / nl + u * (DST_THREADS / nl);
@@ -496,19 +498,19 @@ __device__ __forceinline__ int64_t dst_addr(const DstArgs &a, int gl, int pt) {
 
 // Issue the loads of batch b into registers (no wait).  For the x forward pass, thread
 // l < nl also fetches the Dirichlet values beyond both x ends of line l: fx = (x0, x1).
-template <int MODE>
-__device__ __forceinline__ void dst_prefetch(const DstArgs &a, const double *w, int comp, int b, int nl,
-                                             double (&v)[DST_PF], double (&fx)[2]) {
+template <int MODE, class T>
+__device__ __forceinline__ void dst_prefetch(const DstArgs &a, const T *w, int comp, int b, int nl,
+                                             T (&v)[DST_PF], double (&fx)[2]) {
 #pragma unroll
     for (int u = 0; u < DST_PF; u++) {
         int line, pt;
         dst_elem<MODE>(a.m, nl, u, &line, &pt);
         const int gl = b * nl + line;
-        v[u] = 0.0;
+        v[u] = (T)0;
         if (line < nl && pt < a.m && gl < a.nlines) {
             if (MODE == DST_XF) {
                 const int j = gl % a.my, k = gl / a.my;
-                v[u] = a.rhs.p[comp * a.rhs.cs + a.rhs.at(pt + 1, j + 1, k + 1)];
+                v[u] = (T)a.rhs.p[comp * a.rhs.cs + a.rhs.at(pt + 1, j + 1, k + 1)];
             } else {
                 v[u] = w[dst_addr<MODE>(a, gl, pt)];
             }
@@ -623,43 +625,49 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_pass3(DstArgs a) {
 // and the inverse), so a batch needs three work-group barriers -- after the LDS fill,
 // before the stores, after the stores.  Batches are pipelined as in k_dst_pass3.
 
-__device__ __forceinline__ void wave_pre(double2 *B, const double *__restrict__ sn, int N, int lane,
-                                         const double *fold, double ih2) {
+template <class C>
+__device__ __forceinline__ void wave_pre(C *B, const double *__restrict__ sn, int N, int lane, const double *fold,
+                                         double ih2) {
+    using T = typename Cx<C>::R;
     const int H = N / 2;
     for (int j = 1 + lane; j <= H; j += 64) {
-        double2 x = B[j], y = B[N - j];
+        C x = B[j], y = B[N - j];
         if (fold && j == 1) {  // fold = (x0, x1) of line 2p, then of line 2p+1
+            const T f0 = (T)(fold[0] * ih2), f1 = (T)(fold[1] * ih2), f2 = (T)(fold[2] * ih2), f3 = (T)(fold[3] * ih2);
             if (N == 2) {
-                x = make_double2((x.x - fold[0] * ih2) - fold[1] * ih2, (x.y - fold[2] * ih2) - fold[3] * ih2);
+                x = Cx<C>::mk((x.x - f0) - f1, (x.y - f2) - f3);
                 y = x;
             } else {
-                x = make_double2(x.x - fold[0] * ih2, x.y - fold[2] * ih2);
-                y = make_double2(y.x - fold[1] * ih2, y.y - fold[3] * ih2);
+                x = Cx<C>::mk(x.x - f0, x.y - f2);
+                y = Cx<C>::mk(y.x - f1, y.y - f3);
             }
         }
-        const double sj = sn[j];
-        const double2 sp = make_double2(sj * (x.x + y.x), sj * (x.y + y.y));
-        const double2 am = make_double2(0.5 * (x.x - y.x), 0.5 * (x.y - y.y));
+        const T sj = (T)sn[j], h = (T)0.5;
+        const C sp = Cx<C>::mk(sj * (x.x + y.x), sj * (x.y + y.y));
+        const C am = Cx<C>::mk(h * (x.x - y.x), h * (x.y - y.y));
         B[j] = cadd(sp, am);
         B[N - j] = csub(sp, am);
     }
-    if (lane == 0) B[0] = make_double2(0.0, 0.0);
+    if (lane == 0) B[0] = Cx<C>::mk(0, 0);
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void wave_post(double2 *B, int N, int lane) {
+template <class C>
+__device__ __forceinline__ void wave_post(C *B, int N, int lane) {
+    using T = typename Cx<C>::R;
     constexpr int PPL = (WAVE_LMAX / 2 + 1 + 63) / 64;
     const int H = N / 2 + 1;
-    double2 ev[PPL], od[PPL];
+    const T h = (T)0.5;
+    C ev[PPL], od[PPL];
 #pragma unroll
     for (int i = 0; i < PPL; i++) {
         const int k = lane + 64 * i;
-        ev[i] = od[i] = make_double2(0.0, 0.0);
+        ev[i] = od[i] = Cx<C>::mk(0, 0);
         if (k < H) {
-            const double2 z = B[k], zc = B[k == 0 ? 0 : N - k];
-            ev[i] = make_double2(-0.5 * (z.y - zc.y), 0.5 * (z.x - zc.x));
-            od[i] = make_double2(0.5 * (z.x + zc.x), 0.5 * (z.y + zc.y));
-            if (k == 0) od[i] = make_double2(0.5 * od[i].x, 0.5 * od[i].y);
+            const C z = B[k], zc = B[k == 0 ? 0 : N - k];
+            ev[i] = Cx<C>::mk(-h * (z.y - zc.y), h * (z.x - zc.x));
+            od[i] = Cx<C>::mk(h * (z.x + zc.x), h * (z.y + zc.y));
+            if (k == 0) od[i] = Cx<C>::mk(h * od[i].x, h * od[i].y);
         }
     }
     __builtin_amdgcn_wave_barrier();
@@ -671,18 +679,18 @@ __device__ __forceinline__ void wave_post(double2 *B, int N, int lane) {
         if (2 * k + 1 <= N - 1) B[2 * k + 1] = od[i];
     }
     __builtin_amdgcn_wave_barrier();
-    const int K = N / 2, C = (K + 63) / 64;
-    const int k0 = lane * C < K ? lane * C : K, k1 = k0 + C < K ? k0 + C : K;
-    double2 s = make_double2(0.0, 0.0);
+    const int K = N / 2, Cn = (K + 63) / 64;
+    const int k0 = lane * Cn < K ? lane * Cn : K, k1 = k0 + Cn < K ? k0 + Cn : K;
+    C s = Cx<C>::mk(0, 0);
     for (int k = k0; k < k1; k++) s = cadd(s, B[2 * k + 1]);
-    double2 inc = s;
+    C inc = s;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const double ox = __shfl_up(inc.x, d, 64), oy = __shfl_up(inc.y, d, 64);
-        if (lane >= d) inc = make_double2(inc.x + ox, inc.y + oy);
+        const T ox = __shfl_up(inc.x, d, 64), oy = __shfl_up(inc.y, d, 64);
+        if (lane >= d) inc = Cx<C>::mk(inc.x + ox, inc.y + oy);
     }
-    double2 run = make_double2(__shfl_up(inc.x, 1, 64), __shfl_up(inc.y, 1, 64));
-    if (lane == 0) run = make_double2(0.0, 0.0);
+    C run = Cx<C>::mk(__shfl_up(inc.x, 1, 64), __shfl_up(inc.y, 1, 64));
+    if (lane == 0) run = Cx<C>::mk(0, 0);
     for (int k = k0; k < k1; k++) {
         run = cadd(run, B[2 * k + 1]);
         B[2 * k + 1] = run;
@@ -690,45 +698,61 @@ __device__ __forceinline__ void wave_post(double2 *B, int N, int lane) {
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int MODE, int MAXR>
+// T: arithmetic and work-array type (double: plain FP64 form; float: the mixed form, whose
+// RHS, Dirichlet values and output stay FP64 and whose inner passes run in FP32).
+template <int MODE, int MAXR, class T>
 __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
 #pragma clang fp contract(off)
-    extern __shared__ double2 lds[];
+    using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    extern __shared__ double2 lds_raw[];
+    C *lds = (C *)lds_raw;
     __shared__ double lamxy[16];
     __shared__ double foldx[32];
     constexpr int nl = 16;
     const int N = P.L, m = a.m;
     const int nbatch = (int)((a.nlines + nl - 1) / nl);
-    double2 *A = lds;
-    double2 *twl = lds + 8 * N;  // exp(-2 pi i t / N)
-    double *Ad = (double *)A;
+    C *A = lds;
+    C *twl = lds + 8 * N;  // exp(-2 pi i t / N)
+    T *Ad = (T *)A;
     const int comp = blockIdx.y;
-    double *w = a.w + comp * a.wcs;
+    T *w = (T *)a.w + comp * a.wcs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    double2 *B = A + wave * N;
-    for (int t = tid; t < N; t += DST_THREADS) twl[t] = a.tw[t];
-    double v[DST_PF], fx[2];
+    C *B = A + wave * N;
+    for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
+    T v[DST_PF];
+    double fx[2];
     int b = blockIdx.x;
-    if (b < nbatch) dst_prefetch<MODE>(a, w, comp, b, nl, v, fx);
+    if (b < nbatch) dst_prefetch<MODE, T>(a, w, comp, b, nl, v, fx);
     for (; b < nbatch; b += gridDim.x) {
 #pragma unroll
         for (int u = 0; u < DST_PF; u++) {
             int line, pt;
             dst_elem<MODE>(m, nl, u, &line, &pt);
             if (line >= nl || pt >= m) continue;
-            double val = v[u];
-            if (MODE == DST_XF && a.has_bc) {
-                const int gl = b * nl + line;
-                if (gl < a.nlines) {
-                    const int j = gl % a.my, k = gl / a.my;
-                    const double *bp = a.bc.p + comp * a.bc.cs;
-                    if (j == 0) val -= bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
-                    if (j == a.my - 1) val -= bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
-                    if (k == 0) val -= bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
-                    if (k == a.mz - 1) val -= bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
+            Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] = v[u];
+        }
+        if (MODE == DST_XF && a.has_bc) {
+            // lines on the y or z faces of the window (a thread's line is fixed in x passes):
+            // fold their Dirichlet neighbours; the x ends are folded in wave_pre
+            int line, pt;
+            dst_elem<MODE>(m, nl, 0, &line, &pt);
+            const int gl = b * nl + line;
+            const int j = gl % a.my, k = gl / a.my;
+            if (gl < a.nlines && (j == 0 || j == a.my - 1 || k == 0 || k == a.mz - 1)) {
+                const double *bp = a.bc.p + comp * a.bc.cs;
+#pragma unroll 1
+                for (int u = 0; u < DST_PF; u++) {
+                    dst_elem<MODE>(m, nl, u, &line, &pt);
+                    if (pt >= m) continue;
+                    double f = 0.0;
+                    if (j == 0) f += bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
+                    if (j == a.my - 1) f += bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
+                    if (k == 0) f += bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
+                    if (k == a.mz - 1) f += bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
+                    T &slot = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)];
+                    slot = (T)((double)slot - f);
                 }
             }
-            Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] = val;
         }
         if (MODE == DST_XF && tid < nl) {
             foldx[2 * tid] = fx[0];
@@ -741,7 +765,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             lamxy[tid] = lv;
         }
         __syncthreads();
-        if (b + (int)gridDim.x < nbatch) dst_prefetch<MODE>(a, w, comp, b + gridDim.x, nl, v, fx);
+        if (b + (int)gridDim.x < nbatch) dst_prefetch<MODE, T>(a, w, comp, b + gridDim.x, nl, v, fx);
         wave_pre(B, a.sn, N, lane, (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr, a.ih2[0]);
         wave_fft<MAXR>(B, twl, P, lane);
         wave_post(B, N, lane);
@@ -749,9 +773,9 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
             for (int j = 1 + lane; j <= m; j += 64) {
                 const double lz = a.lam[2][j - 1];
-                const double fa = a.norm / (-(la + lz)), fb = a.norm / (-(lb + lz));
-                const double2 z = B[j];
-                B[j] = make_double2(z.x * fa, z.y * fb);
+                const T fa = (T)(a.norm / (-(la + lz))), fb = (T)(a.norm / (-(lb + lz)));
+                const C z = B[j];
+                B[j] = Cx<C>::mk(z.x * fa, z.y * fb);
             }
             __builtin_amdgcn_wave_barrier();
             wave_pre(B, a.sn, N, lane, nullptr, 0.0);
@@ -765,10 +789,10 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             dst_elem<MODE>(m, nl, u, &line, &pt);
             const int gl = b * nl + line;
             if (line >= nl || pt >= m || gl >= a.nlines) continue;
-            const double val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * a.scale;
+            const T val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * (T)a.scale;
             if (MODE == DST_XI) {
                 const int j = gl % a.my, k = gl / a.my;
-                a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = val;
+                a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = (double)val;
             } else {
                 w[dst_addr<MODE>(a, gl, pt)] = val;
             }
@@ -951,7 +975,8 @@ static int launch_pass3(Ctx *c, DstArgs a, int ncomp, int rc) {
     const size_t lds = sizeof(double2) * (size_t)(np + 1) * a.L;
     // persistent grid: the work-groups the device holds at once, batches looped inside
     const int64_t nbatch = (a.nlines + 2 * np - 1) / (2 * np);
-    int64_t resident = (int64_t)c->num_cu * c->dst_wg_per_cu / (ncomp > 0 ? ncomp : 1);
+    const int per_cu = c->dst_wg_per_cu > 0 ? c->dst_wg_per_cu : 1;
+    int64_t resident = (int64_t)c->num_cu * per_cu / (ncomp > 0 ? ncomp : 1);
     if (resident < 1) resident = 1;
     dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
     switch (rc) {
@@ -972,7 +997,7 @@ struct AxisPlan {
     const double *sn;
 };
 
-template <int MODE>
+template <int MODE, class T>
 static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) {
     WavePlan P{};
     P.L = fp->L;
@@ -985,28 +1010,45 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         ns *= fp->radix[s];
     }
     a.npairs = 8;
-    const size_t lds = sizeof(double2) * (size_t)9 * a.L;
+    const size_t lds = 2 * sizeof(T) * (size_t)9 * a.L;
     const int64_t nbatch = (a.nlines + 15) / 16;
-    int64_t resident = (int64_t)c->num_cu * c->dst_wg_per_cu / (ncomp > 0 ? ncomp : 1);
+    const void *fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T>
+                   : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T>
+                   : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T> : (const void *)k_dst_wave<MODE, 37, T>;
+    // persistent grid: as many work-groups as the device holds at once
+    int per_cu = c->dst_wg_per_cu;
+    if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
+        per_cu = 1;
+    if (per_cu < 1) per_cu = 1;
+    int64_t resident = (int64_t)c->num_cu * per_cu / (ncomp > 0 ? ncomp : 1);
     if (resident < 1) resident = 1;
     dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
     switch (rc) {
-        case 4: hipLaunchKernelGGL((k_dst_wave<MODE, 4>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-        case 7: hipLaunchKernelGGL((k_dst_wave<MODE, 7>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-        case 17: hipLaunchKernelGGL((k_dst_wave<MODE, 17>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-        default: hipLaunchKernelGGL((k_dst_wave<MODE, 37>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        case 4: hipLaunchKernelGGL((k_dst_wave<MODE, 4, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        case 7: hipLaunchKernelGGL((k_dst_wave<MODE, 7, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        case 17: hipLaunchKernelGGL((k_dst_wave<MODE, 17, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
+        default: hipLaunchKernelGGL((k_dst_wave<MODE, 37, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
     }
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
 
+// The one-wave-per-pair kernel takes lines of up to WAVE_LMAX - 1 points whose FFT of m+1
+// has radices up to 37.
+static int wave_class(const AxisPlan &P) {
+    if (P.m + 1 > WAVE_LMAX || P.half->nstages > 12) return 0;
+    DstArgs a{};
+    setup_axis(a, P.half, P.m);
+    return radix_class(a);
+}
+
 template <int MODE>
-static int run_pass(Ctx *c, DstArgs a, int ncomp, const AxisPlan &P) {
-    if (c->dst_kernel == 4 && P.m + 1 <= WAVE_LMAX && P.half->nstages <= 12) {
+static int run_pass(Ctx *c, DstArgs a, int ncomp, const AxisPlan &P, bool f32) {
+    const int wc = wave_class(P);
+    if (f32 || (c->dst_kernel == 4 && wc)) {
         setup_axis(a, P.half, P.m);
         a.sn = P.sn;
-        const int rc = radix_class(a);
-        if (rc) return launch_wave<MODE>(c, a, ncomp, rc, P.half);
+        return f32 ? launch_wave<MODE, float>(c, a, ncomp, wc, P.half) : launch_wave<MODE, double>(c, a, ncomp, wc, P.half);
     }
     if (c->dst_kernel >= 3) {
         setup_axis(a, P.half, P.m);
@@ -1050,21 +1092,23 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     VPMX_TRY(eigen(c, m[2], h[2], c->lam[2], &a.lam[2]));
     a.norm = (2.0 / (m[0] + 1)) * (2.0 / (m[1] + 1)) * (2.0 / (m[2] + 1));
     a.scale = 1.0;
+    // mixed form: the five passes in FP32 when every axis takes the one-wave kernel
+    const bool f32 = c->precision == 1 && wave_class(ax[0]) && wave_class(ax[1]) && wave_class(ax[2]);
     // x forward (with boundary folding)
     a.nlines = (int64_t)m[1] * m[2];
-    VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0]));
+    VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0], f32));
     // y forward
     a.nlines = (int64_t)m[0] * m[2];
-    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1]));
+    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
     // z forward + diagonal + inverse
     a.nlines = (int64_t)m[0] * m[1];
-    VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2]));
+    VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
     // y inverse
     a.nlines = (int64_t)m[0] * m[2];
-    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1]));
+    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
     // x inverse -> output interior
     a.nlines = (int64_t)m[1] * m[2];
-    VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0]));
+    VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0], f32));
     if (!bc) {
         int64_t tot = 2 * ((int64_t)wn[0] * wn[1] + (int64_t)wn[0] * wn[2] + (int64_t)wn[1] * wn[2]);
         hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, out, wn[0], wn[1],

@@ -420,7 +420,7 @@ int32_t vpmx_get_geometry(const vpmx_ctx *p, vpmx_geom *out) {
 int32_t vpmx_set_precision(vpmx_ctx *p, int32_t mode) {
     Ctx *c = C(p);
     if (!c) return VPMX_ERR_ARG;
-    if (mode != 0) return fail(c, VPMX_ERR_UNSUP, "only the plain FP64 form (0) is available");
+    if (mode != 0 && mode != 1) return fail(c, VPMX_ERR_ARG, "precision: 0 plain FP64 form, 1 mixed form");
     c->precision = mode;
     return VPMX_OK;
 }

@@ -46,7 +46,7 @@ struct Ctx : vpmx_ctx {
     void *bsx = nullptr;   // spectral boundary-sum plan of the current block geometry
     void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
     int num_cu = 256;       // compute units of the device
-    int dst_wg_per_cu = 1;  // resident work-groups per CU assumed by the persistent DST passes
+    int dst_wg_per_cu = 0;  // resident work-groups per CU of the persistent DST passes (0: occupancy query)
     int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length
                           // work-group FFTs, 2 odd extension, 1 two-buffer
     std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N

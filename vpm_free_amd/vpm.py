@@ -179,6 +179,10 @@ class VPM:
         self._check(self._lib.vpmx_set_params(self._h, C.byref(params)))
         self.neqpm = 3
 
+    def set_precision(self, mode):
+        """0: plain FP64 form; 1: mixed form (FP32 Poisson passes, FP64 boundary sums)."""
+        self._check(self._lib.vpmx_set_precision(self._h, int(mode)))
+
     def close(self):
         if getattr(self, "_h", None):
             self._lib.vpmx_destroy(self._h)
