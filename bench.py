@@ -1,4 +1,6 @@
-"""Benchmark: VPM steps/s on one MI355X, two coaxial leapfrogging rings on the 256^3 grid.
+"""Benchmark: VPM steps/s on one MI355X, two coaxial leapfrogging rings on the 256^3 grid,
+mixed form (FP32 sine-transform Poisson passes, FP64 boundary sums and diagnostics) beside
+the plain FP64 form.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
@@ -8,7 +10,8 @@ updates, vpm(WhatToDo=0) and remesh_particles_3d(1) -- all on the device, partic
 resident in HBM.  Workload: rings R=1 and R=1.2, Gaussian core sigma=0.08, axial
 separation 0.5, box [-2,2]^3, h=1/64 (N=273 nodes per axis, 289^3 yaps block, 43^3
 coarse grid), particles at cell centres where |w|>1e-6, remeshed once before timing
-(test.f90:50-55), dt=0.01.  Prints one JSON line.
+(test.f90:50-55), dt=0.01.  Both forms start from the same remeshed particles; K steps of
+each are timed.  Prints one JSON line; `value` is the mixed form's steps/s.
 """
 import argparse
 import json
@@ -22,22 +25,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "VPM steps/s + HBM GB/s, 256³/2M particles; mixed-FP32 vs FP64 @1e-5 L2"
-FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-FLOPS_PER_PAIR = 20.0
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--h", type=float, default=1.0 / 64)
     ap.add_argument("--dt", type=float, default=0.01)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--precision", type=int, default=0, help="0 plain FP64 form, 1 mixed form")
     ap.add_argument("--cpu-stride", type=int, default=256,
                     help="boundary-sum target stride of the bounded CPU sample")
+    ap.add_argument("--forms", default="fp64,mixed", help="forms to time: fp64, mixed or both")
     return ap.parse_args()
 
 
@@ -67,6 +68,10 @@ def stencil_nodes(xp, g):
     return int(occ.sum())
 
 
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
 def cpu_baseline(h, xp, qp, stride):
     """Oracle (C/OpenMP restatement of the reference path) on the host cores: one
     vpm(WhatToDo=2) with the boundary sums evaluated on every stride-th boundary node,
@@ -91,10 +96,74 @@ def cpu_baseline(h, xp, qp, stride):
         "unit": "steps/s",
         "cores": O.num_threads(),
         "kind": "port",
-        "sample": (f"oracle vpm(2) on the same 256^3 leapfrog workload with boundary sums on 1/{stride} "
+        "sample": (f"oracle vpm(2) on the same 256^3 leapfrog particles with boundary sums on 1/{stride} "
                    f"of the boundary nodes (measured {wall_vpm:.1f}s, boundary part {t[2]:.2f}s scaled x{stride}) "
                    f"+ one full remesh ({wall_rm:.1f}s); step = 2 vpm(2) + remesh = {step:.1f}s"),
     }
+
+
+def time_form(ctx, precision, xp0, qp0, args, g):
+    """Upload the starting particles, run W warmup steps and K timed steps in one form."""
+    ctx.set_precision(precision)
+    ctx.upload(xp0, qp0)
+    circ0, imp0 = ctx.diagnostics()
+    ntime = 1
+    for _ in range(args.warmup):
+        ctx.step_rk2(args.dt, ntime=ntime)
+        ntime += 1
+    stage_tot, nparts = {}, []
+    ctx.kernel_times(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.step_rk2(args.dt, ntime=ntime)
+        ntime += 1
+        for k, v in ctx.stage_times().items():
+            stage_tot[k] = stage_tot.get(k, 0.0) + v
+        nparts.append(ctx.count())
+    elapsed = time.perf_counter() - t0
+    kt = ctx.kernel_times()
+    work = ctx.kernel_work()
+    circ1, imp1 = ctx.diagnostics()
+    xf, _, _, _ = ctx.download()
+    npart = float(np.mean(nparts))
+    G = g["nn"][0] * g["nn"][1] * g["nn"][2]
+    res = {
+        "steps_per_s": args.steps / elapsed,
+        "ms_per_step": 1000.0 * elapsed / args.steps,
+        "particles": int(npart),
+        "particles_range": [int(nparts[0]), int(nparts[-1])],
+        "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_tot.items()},
+        "diagnostics": {"circulation_drift": float(np.max(np.abs(circ1 - circ0))),
+                        "impulse_rel_drift": float(np.linalg.norm(imp1 - imp0) / np.linalg.norm(imp0))},
+        "kernels": {},
+    }
+    # per-kernel device time from each dispatch's start/stop events on the library's stream
+    dst_ms, dst_n = kt["dst_y_fine"]
+    if dst_n:
+        b = work["dst_y_fine"]  # read + write of the work array: 2 x lines x points x comps x word
+        gbs = b / (dst_ms / dst_n * 1e-3) / 1e9
+        res["kernels"]["dst_y_fine"] = {"ms": round(dst_ms / dst_n, 4), "launches": int(dst_n),
+                                        "algorithmic_bytes": int(b), "GB/s": round(gbs, 1),
+                                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    bs_ms, bs_n = kt["boundsum_fine"]
+    if bs_n:
+        fl = work["boundsum_fine"]
+        res["kernels"]["boundsum_fine"] = {"ms": round(bs_ms / bs_n, 4), "launches": int(bs_n), "flops": fl,
+                                           "TFLOP/s": round(fl / (bs_ms / bs_n * 1e-3) / 1e12, 3)}
+    p2m_ms, p2m_n = kt["p2m"]
+    if p2m_n:
+        b = npart * 6 * 8 + G * 3 * 8  # FP64 positions + strengths read, FP64 RHS written
+        gbs = b / (p2m_ms / p2m_n * 1e-3) / 1e9
+        res["kernels"]["p2m"] = {"ms": round(p2m_ms / p2m_n, 4), "launches": int(p2m_n), "algorithmic_bytes": int(b),
+                                 "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    m2p_ms, m2p_n = kt["m2p"]
+    if m2p_n:
+        T = stencil_nodes(xf, g)
+        b = npart * 4 * 8 + npart * 6 * 8 + T * 6 * 8
+        gbs = b / (m2p_ms / m2p_n * 1e-3) / 1e9
+        res["kernels"]["m2p"] = {"ms": round(m2p_ms / m2p_n, 4), "launches": int(m2p_n), "algorithmic_bytes": int(b),
+                                 "stencil_nodes": int(T), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return res
 
 
 def main():
@@ -108,120 +177,89 @@ def main():
     t0 = time.perf_counter()
     xp, qp = workload(args.h)
     log(f"workload: {xp.shape[0]} particles generated in {time.perf_counter() - t0:.1f}s")
-    params = V.Params.make(args.h)
-    ctx = V.VPM(params)
-    ctx.set_precision(args.precision)
+    ctx = V.VPM(V.Params.make(args.h))
     ctx.upload(xp, qp)
     ctx.vpm_resident(0, 0)  # define_sizes (NTIME=0): box fixed from here on (idefine=1)
     ctx.remesh_resident(1)  # initial remesh, as the reference driver does before its loop
     g = V.geom_dict(ctx.geometry)
     nb = g["nnbl"][0]
-    pairs_fine = (nb[0] * nb[1] * nb[2] - (nb[0] - 2) * (nb[1] - 2) * (nb[2] - 2)) * \
-        2 * ((nb[1] - 1) * (nb[2] - 1) + (nb[0] - 1) * (nb[2] - 1) + (nb[0] - 1) * (nb[1] - 1))
-    n0 = ctx.count()
-    log(f"grid N={g['nn']} block={nb} coarse={g['nn_coarse']} particles after remesh={n0}")
-    xp, qp, _, _ = ctx.download()  # the CPU baseline starts from the same remeshed cloud
-    circ0, imp0 = ctx.diagnostics()
+    xp0, qp0, _, _ = ctx.download()
+    log(f"grid N={g['nn']} block={nb} coarse={g['nn_coarse']} particles after remesh={xp0.shape[0]}")
 
-    ntime = 1
-    for _ in range(args.warmup):
-        ctx.step_rk2(args.dt, ntime=ntime)
-        ntime += 1
-        log(f"warmup step done, particles={ctx.count()}")
+    # accuracy of the mixed form: one vpm(2) on the same particles in both forms
+    ctx.set_precision(0)
+    ctx.vpm_resident(2, 1)
+    _, _, up64, gp64 = ctx.download()
+    ctx.set_precision(1)
+    ctx.upload(xp0, qp0)
+    ctx.vpm_resident(2, 1)
+    _, _, up32, gp32 = ctx.download()
+    acc = {"up_rel_l2": rel_l2(up32, up64), "gp_rel_l2": rel_l2(gp32, gp64)}
+    log(f"mixed vs FP64 on one vpm(2): UP rel L2 {acc['up_rel_l2']:.3e}, GP rel L2 {acc['gp_rel_l2']:.3e}")
 
-    stage_tot = {}
-    nparts = []
-    ctx.kernel_times(reset=True)
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        ctx.step_rk2(args.dt, ntime=ntime)
-        ntime += 1
-        for k, v in ctx.stage_times().items():
-            stage_tot[k] = stage_tot.get(k, 0.0) + v
-        nparts.append(ctx.count())
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_per_step = 1000.0 * elapsed / args.steps
-    steps_per_s = args.steps / elapsed
-    circ1, imp1 = ctx.diagnostics()
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step; particles {nparts[0]}..{nparts[-1]}")
-    log("stage ms/step: " + ", ".join(f"{k}={v / args.steps:.2f}" for k, v in stage_tot.items()))
-
-    # dominant kernel: the fine-block FP64 boundary sum (one launch per vpm(2), two per step),
-    # timed by its dispatch's own start/stop events on the library's stream
-    kt = ctx.kernel_times()
-    bs_total, launches = kt["boundsum_fine"]
-    bs_ms = bs_total / launches
-    # FP64 work of one launch as the library executes it: 3 FMA per pair + the distinct 1/r
-    # table entries (20 flops each); the 20-flops-per-pair direct formulation is reported beside it
-    work = ctx.kernel_work()["boundsum_fine"]
-    achieved = work / (bs_ms * 1e-3) / 1e12
-    xf, _, _, _ = ctx.download()
-    G = g["nn"][0] * g["nn"][1] * g["nn"][2]
-    npart = float(np.mean(nparts))
-    T = stencil_nodes(xf, g)
-    hbm = {}
-    p2m_ms, p2m_n = kt["p2m"]
-    m2p_ms, m2p_n = kt["m2p"]
-    if p2m_n:
-        b = npart * 6 * 8 + G * 3 * 8
-        gbs = b / (p2m_ms / p2m_n * 1e-3) / 1e9
-        hbm["p2m"] = {"ms": round(p2m_ms / p2m_n, 4), "algorithmic_bytes": int(b), "GB/s": round(gbs, 1),
-                      "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    if m2p_n:
-        b = npart * 4 * 8 + npart * 6 * 8 + T * 6 * 8
-        gbs = b / (m2p_ms / m2p_n * 1e-3) / 1e9
-        hbm["m2p"] = {"ms": round(m2p_ms / m2p_n, 4), "algorithmic_bytes": int(b), "stencil_nodes": int(T),
-                      "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    roofline = {
-        "bound": "mfma",
-        "achieved": round(achieved, 3),
-        "peak": FP64_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-        "traffic": None,
-        "kernel": "k_bs_lattice (FP64 Green's-function boundary sum, 1/r per lattice offset in LDS, VALU FMA)",
-        "flops_per_launch": work,
-        "pairs_per_launch": pairs_fine,
-        "pairs_per_s": round(pairs_fine / (bs_ms * 1e-3), 1),
-        "direct_form_equiv_tflops": round(FLOPS_PER_PAIR * pairs_fine / (bs_ms * 1e-3) / 1e12, 2),
-        "ms_per_launch": round(bs_ms, 3),
-    }
+    forms = {}
+    for name in args.forms.split(","):
+        forms[name] = time_form(ctx, 1 if name == "mixed" else 0, xp0, qp0, args, g)
+        f = forms[name]
+        log(f"{name}: {f['ms_per_step']:.2f} ms/step over {args.steps} steps; particles "
+            f"{f['particles_range'][0]}..{f['particles_range'][1]}")
+        log("  stage ms/step: " + ", ".join(f"{k}={v:.2f}" for k, v in f["stage_ms_per_step"].items()))
+        log("  kernels: " + json.dumps(f["kernels"]))
     ctx.close()
+
+    mixed = "mixed" in forms
+    head = forms["mixed"] if mixed else forms["fp64"]
+    # dominant kernel: the y pass of the 289^3 block's sine transforms (HBM-bound)
+    dk = head["kernels"].get("dst_y_fine", {})
+    roofline = {
+        "bound": "hbm",
+        "achieved": dk.get("GB/s"),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": dk.get("frac"),
+        "traffic": None,
+        "kernel": "k_dst_wave<DST_Y,4,%s>: y pass of the 289^3 block's sine-transform Poisson solve (3 comps)"
+                  % ("float" if mixed else "double"),
+        "ms_per_launch": dk.get("ms"),
+        "algorithmic_bytes_per_launch": dk.get("algorithmic_bytes"),
+    }
 
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline sample ...")
-        cpu = cpu_baseline(args.h, xp, qp, args.cpu_stride)
+        cpu = cpu_baseline(args.h, xp0, qp0, args.cpu_stride)
         log(f"cpu baseline: {cpu['value']:.4g} steps/s on {cpu['cores']} threads")
 
     out = {
         "metric": METRIC,
-        "value": round(steps_per_s, 4),
+        "value": round(head["steps_per_s"], 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(head["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 (mixed form: FP32 Poisson passes; FP64 particles, projection, boundary sums, diagnostics)"
+                 if mixed else "f64",
         "data": "synthetic",
         "config": {
             "workload": "two coaxial leapfrogging vortex rings (R=1, 1.2; sigma=0.08; sep 0.5), box [-2,2]^3, "
                         "h=1/64 (256^3 class: N=273, yaps block 289^3, coarse 43^3), RK2 dt=0.01 + remesh each step",
             "grid_nodes": g["nn"],
             "block_nodes": nb,
-            "particles": int(np.mean(nparts)),
-            "precision": "plain FP64 form",
+            "particles": head["particles"],
+            "form": "mixed" if mixed else "fp64",
         },
-        "particles_per_s": round(float(np.mean(nparts)) * steps_per_s, 1),
-        "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in stage_tot.items()},
-        "diagnostics": {"circulation_drift": float(np.max(np.abs(circ1 - circ0))),
-                        "impulse_rel_drift": float(np.linalg.norm(imp1 - imp0) / np.linalg.norm(imp0))},
+        "particles_per_s": round(head["particles"] * head["steps_per_s"], 1),
+        "mixed_vs_fp64": {
+            "speedup": round(forms["mixed"]["steps_per_s"] / forms["fp64"]["steps_per_s"], 3)
+            if mixed and "fp64" in forms else None,
+            "accuracy_one_vpm2": acc,
+        },
+        "forms": forms,
         "roofline": roofline,
-        "hbm_kernels": hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

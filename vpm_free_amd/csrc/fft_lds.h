@@ -179,7 +179,8 @@ __device__ void fft_reg(double2 *buf, const double2 *tw, const Args &a) {
 // barrier only keeps the compiler from moving them).  Per stage: R radix, Ns span, mg a
 // multiplier with j / Ns == umulhi(j, mg) for j < 2^16 and Ns > 1 (mg = floor(2^32 / Ns) + 1;
 // the first stage, Ns = 1, divides by nothing).
-constexpr int WAVE_LMAX = 288;
+constexpr int WAVE_LMAX = 288;      // double sequences (16 per work-group of 8 waves)
+constexpr int WAVE_LMAX_F = 576;    // float sequences (same LDS bytes)
 
 struct WavePlan {
     int L, nstages;
@@ -222,30 +223,30 @@ __device__ __forceinline__ void wave_stage(C *buf, const C *__restrict__ tw, int
     __builtin_amdgcn_wave_barrier();
 }
 
-template <int R>
-constexpr int wave_bpl() { return (WAVE_LMAX / R + 63) / 64; }
+template <int R, int LMAX>
+constexpr int wave_bpl() { return (LMAX / R + 63) / 64; }
 
-template <int MAXR, class C>
+template <int MAXR, class C, int LMAX = (sizeof(C) == 16 ? WAVE_LMAX : WAVE_LMAX_F)>
 __device__ void wave_fft(C *buf, const C *tw, const WavePlan &P, int lane) {
     for (int s = 0; s < P.nstages; s++) {
         const int R = P.R[s], Ns = P.Ns[s];
         const unsigned mg = P.mg[s];
-        if (R == 4) wave_stage<4, wave_bpl<4>(), C>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 2) wave_stage<2, wave_bpl<2>(), C>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 3) wave_stage<3, wave_bpl<3>(), C>(buf, tw, P.L, Ns, mg, lane);
+        if (R == 4) wave_stage<4, wave_bpl<4, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 2) wave_stage<2, wave_bpl<2, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 3) wave_stage<3, wave_bpl<3, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
         else if constexpr (MAXR >= 7) {
-            if (R == 5) wave_stage<5, wave_bpl<5>(), C>(buf, tw, P.L, Ns, mg, lane);
-            else if (R == 7) wave_stage<7, wave_bpl<7>(), C>(buf, tw, P.L, Ns, mg, lane);
+            if (R == 5) wave_stage<5, wave_bpl<5, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+            else if (R == 7) wave_stage<7, wave_bpl<7, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
             else if constexpr (MAXR >= 17) {
-                if (R == 11) wave_stage<11, wave_bpl<11>(), C>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 13) wave_stage<13, wave_bpl<13>(), C>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 17) wave_stage<17, wave_bpl<17>(), C>(buf, tw, P.L, Ns, mg, lane);
+                if (R == 11) wave_stage<11, wave_bpl<11, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 13) wave_stage<13, wave_bpl<13, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 17) wave_stage<17, wave_bpl<17, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
                 else if constexpr (MAXR >= 37) {
-                    if (R == 19) wave_stage<19, wave_bpl<19>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 23) wave_stage<23, wave_bpl<23>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 29) wave_stage<29, wave_bpl<29>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 31) wave_stage<31, wave_bpl<31>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else wave_stage<37, wave_bpl<37>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    if (R == 19) wave_stage<19, wave_bpl<19, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 23) wave_stage<23, wave_bpl<23, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 29) wave_stage<29, wave_bpl<29, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 31) wave_stage<31, wave_bpl<31, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    else wave_stage<37, wave_bpl<37, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
                 }
             }
         }

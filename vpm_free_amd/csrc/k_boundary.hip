@@ -424,10 +424,10 @@ static int launch_lattice(Ctx *c, const Grid &g, const LatPair *pairs_d, int max
                           const double *a, int64_t ns, double *part, int64_t nt, bool fine) {
     size_t lds = sizeof(double) * ((size_t)LW * tabn + 3 * (size_t)nsp);
     dim3 grid((unsigned)((maxlines + LW - 1) / LW), 36, (unsigned)nsplit);
-    if (fine) {
-        hipExtLaunchKernelGGL(k_bs_lattice<M>, grid, dim3(64 * LW), (uint32_t)lds, c->stream, c->kev0[K_BOUNDSUM],
-                              c->kev1[K_BOUNDSUM], 0, g, pairs_d, nsp, tabn, nsplit, a, ns, part, nt);
-        kernel_timed(c, K_BOUNDSUM);
+    hipEvent_t e0, e1;
+    if (fine && kernel_events(c, K_BOUNDSUM, &e0, &e1)) {
+        hipExtLaunchKernelGGL(k_bs_lattice<M>, grid, dim3(64 * LW), (uint32_t)lds, c->stream, e0, e1, 0, g, pairs_d,
+                              nsp, tabn, nsplit, a, ns, part, nt);
     } else {
         hipLaunchKernelGGL(k_bs_lattice<M>, grid, dim3(64 * LW), lds, c->stream, g, pairs_d, nsp, tabn, nsplit, a, ns,
                            part, nt);
@@ -510,16 +510,18 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
     }
     if (fine && c->bs_mode == 2) {
         mark(c, ST_BOUNDS, 0);
-        if (c->kev_ok) VPMX_HIP(c, hipEventRecord(c->kev0[K_BOUNDSUM], c->stream));
+        hipEvent_t e0, e1;
+        const bool timed = kernel_events(c, K_BOUNDSUM, &e0, &e1);
+        if (timed) VPMX_HIP(c, hipEventRecord(e0, c->stream));
         const int st = boundary_sums_spectral(c, g, ns, a, sol);
         if (st == VPMX_OK) {
-            if (c->kev_ok) VPMX_HIP(c, hipEventRecord(c->kev1[K_BOUNDSUM], c->stream));
-            kernel_timed(c, K_BOUNDSUM);
+            if (timed) VPMX_HIP(c, hipEventRecord(e1, c->stream));
             mark(c, ST_BOUNDS, 1);
             c->last_pairs += (double)nt * (double)ns;
             return VPMX_OK;
         }
         if (st != VPMX_ERR_UNSUP) return st;
+        if (timed) c->kev_pending[K_BOUNDSUM] -= 1;  // nothing ran: drop the interval
         mark(c, ST_BOUNDS, 1);  // faces too long for the in-LDS transforms: lattice kernel
     }
     if (c->bs_mode != 1) return boundary_sums_lattice(c, g, ns, a, sol, fine);
@@ -538,9 +540,13 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
     if (st) return st;
     if (fine) {
         mark(c, ST_BOUNDS, 0);
-        hipExtLaunchKernelGGL(k_boundsum<1>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0, c->stream,
-                              c->kev0[K_BOUNDSUM], c->kev1[K_BOUNDSUM], 0, g, nt, ns, chunk, cxyz, a, part);
-        kernel_timed(c, K_BOUNDSUM);
+        hipEvent_t e0, e1;
+        if (kernel_events(c, K_BOUNDSUM, &e0, &e1))
+            hipExtLaunchKernelGGL(k_boundsum<1>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0,
+                                  c->stream, e0, e1, 0, g, nt, ns, chunk, cxyz, a, part);
+        else
+            hipLaunchKernelGGL(k_boundsum<1>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0,
+                               c->stream, g, nt, ns, chunk, cxyz, a, part);
         mark(c, ST_BOUNDS, 1);
     } else {
         hipLaunchKernelGGL(k_boundsum<0>, dim3((unsigned)tblocks, (unsigned)nsplit), dim3(BS_THREADS), 0, c->stream, g,

@@ -240,11 +240,10 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
-    if (ncomp == 3 && kclass >= 0) {
-        hipExtLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, c->kev0[kclass], c->kev1[kclass], 0,
-                              g, start, perm, s.cn[0], s.cn[1], xp, qp, qstride, itype, div, out, vn[0], vn[1], vn[2],
-                              wlo[0], wlo[1], wlo[2]);
-        kernel_timed(c, kclass);
+    hipEvent_t e0, e1;
+    if (ncomp == 3 && kclass >= 0 && kernel_events(c, kclass, &e0, &e1)) {
+        hipExtLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, e0, e1, 0, g, start, perm, s.cn[0],
+                              s.cn[1], xp, qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
     } else if (ncomp == 3)
         hipLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1], xp,
                            qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
@@ -305,9 +304,13 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
 int m2p_gather(Ctx *c, const Grid &g, const SortState &, const double *xp, const double *qp, int qstride, int itype,
                GView vel, GView sol, double *up, double *gp, int64_t n) {
     if (n <= 0) return VPMX_OK;
-    hipExtLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->kev0[K_M2P],
-                          c->kev1[K_M2P], 0, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
-    kernel_timed(c, K_M2P);
+    hipEvent_t e0, e1;
+    if (kernel_events(c, K_M2P, &e0, &e1))
+        hipExtLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, e0, e1, 0, g, xp, qp,
+                              qstride, itype, vel, sol, up, gp, n);
+    else
+        hipLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, xp, qp, qstride,
+                           itype, vel, sol, up, gp, n);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

@@ -127,6 +127,7 @@ static int yaps_single(Ctx *c, bool *used) {
     // 1. fine infinite-domain solve over the whole block
     mark(c, ST_FINE, 0);
     VPMX_TRY(fill(c, sol_bl, 3 * B, 0.0));
+    c->dst_timed_m = gb.n[1] - 2;  // the block's y lines are the timed sine-transform passes
     VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true));
     mark(c, ST_FINE, 1);
     used[ST_FINE] = used[ST_BOUNDS] = true;
@@ -373,8 +374,7 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
     if (dm && std::string(dm) == "v2") c->dst_kernel = 2;
     if (dm && std::string(dm) == "v3") c->dst_kernel = 3;
-    for (int k = 0; k < K_COUNT; k++)
-        if (hipEventCreate(&c->kev0[k]) != hipSuccess || hipEventCreate(&c->kev1[k]) != hipSuccess) c->kev_ok = false;
+
     *out = c;
     return VPMX_OK;
 }
@@ -393,8 +393,8 @@ void vpmx_destroy(vpmx_ctx *p) {
     free_bspec(c);
     free_fft_plans(c);
     for (int k = 0; k < K_COUNT; k++) {
-        if (c->kev0[k]) (void)hipEventDestroy(c->kev0[k]);
-        if (c->kev1[k]) (void)hipEventDestroy(c->kev1[k]);
+        for (hipEvent_t e : c->kev0[k]) (void)hipEventDestroy(e);
+        for (hipEvent_t e : c->kev1[k]) (void)hipEventDestroy(e);
     }
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -632,6 +632,8 @@ int32_t vpmx_diagnostics(vpmx_ctx *p, double circ[3], double imp[3]) {
 int32_t vpmx_kernel_times(vpmx_ctx *p, double *ms_total, int64_t *launches, int32_t n, int32_t reset) {
     Ctx *c = C(p);
     if (!c) return VPMX_ERR_ARG;
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    resolve_kernel_times(c);
     for (int k = 0; k < n && k < K_COUNT; k++) {
         if (ms_total) ms_total[k] = c->k_ms[k];
         if (launches) launches[k] = c->k_launches[k];

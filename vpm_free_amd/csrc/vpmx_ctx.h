@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <vector>
 
 #include "vpmx_internal.h"
 
@@ -23,6 +24,7 @@ enum Stage {
 };
 
 // Kernel classes timed with events when profiling is on.
+// K_DST: the y passes of the fine block's sine transforms.
 enum KClass { K_BOUNDSUM = 0, K_P2M = 1, K_M2P = 2, K_DST = 3, K_COUNT = 4 };
 
 struct Ctx : vpmx_ctx {
@@ -63,37 +65,66 @@ struct Ctx : vpmx_ctx {
     int64_t rm_count = -1;
     double *rm_xp = nullptr, *rm_qp = nullptr;
 
-    // timing: stages by host wall clock between stream drains; kernels of class K_* by
-    // the dispatch's own start/stop events (hipExtLaunchKernel)
+    // timing: stages by host wall clock between stream drains; kernels of class K_* by the
+    // dispatch's own start/stop events (hipExtLaunchKernel), read back at the next drain
     bool timing = true;
     double stage_t0[ST_COUNT] = {};
     double stage_ms[ST_COUNT] = {};
     double last_pairs = 0.0;
-    hipEvent_t kev0[K_COUNT] = {}, kev1[K_COUNT] = {};
+    std::vector<hipEvent_t> kev0[K_COUNT], kev1[K_COUNT];
+    int kev_pending[K_COUNT] = {};
     bool kev_ok = false;
     double k_ms[K_COUNT] = {};
     int64_t k_launches[K_COUNT] = {};
     double k_work[K_COUNT] = {};  // algorithmic work of the last timed launch (flops or bytes)
+    int dst_timed_m = -1;         // interior length of the DST passes timed as K_DST (fine block)
 };
 
 void free_fft_plans(Ctx *c);
 double wall_ms();
 // Stage timing: drain the stream at both ends so the interval covers exactly the stage's
 // kernels (enqueueing is asynchronous).
+// Add the durations of the recorded dispatches of every class (the stream is drained).
+inline void resolve_kernel_times(Ctx *c) {
+    for (int k = 0; k < K_COUNT; k++) {
+        for (int i = 0; i < c->kev_pending[k]; i++) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, c->kev0[k][i], c->kev1[k][i]) == hipSuccess) {
+                c->k_ms[k] += ms;
+                c->k_launches[k] += 1;
+            }
+        }
+        c->kev_pending[k] = 0;
+    }
+}
+// Stage timing: drain the stream at both ends so the interval covers exactly the stage's
+// kernels (enqueueing is asynchronous).
 inline void mark(Ctx *c, int stage, int end) {
     if (!c->timing) return;
     (void)hipStreamSynchronize(c->stream);
+    resolve_kernel_times(c);
     if (!end) c->stage_t0[stage] = wall_ms();
     else c->stage_ms[stage] += wall_ms() - c->stage_t0[stage];
 }
-// Add the duration of the last timed dispatch of class k (waits for it).
-inline void kernel_timed(Ctx *c, int k) {
-    if (!c->kev_ok) return;
-    float ms = 0.f;
-    if (hipEventSynchronize(c->kev1[k]) == hipSuccess && hipEventElapsedTime(&ms, c->kev0[k], c->kev1[k]) == hipSuccess) {
-        c->k_ms[k] += ms;
-        c->k_launches[k] += 1;
+// Start/stop events for the next timed dispatch of class k (false: time nothing).
+inline bool kernel_events(Ctx *c, int k, hipEvent_t *e0, hipEvent_t *e1) {
+    if (!c->kev_ok) return false;
+    const int i = c->kev_pending[k];
+    if (i >= 64) return false;  // more than 64 undrained dispatches: leave the rest untimed
+    if (i == (int)c->kev0[k].size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return false;
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return false;
+        }
+        c->kev0[k].push_back(a);
+        c->kev1[k].push_back(b);
     }
+    *e0 = c->kev0[k][i];
+    *e1 = c->kev1[k][i];
+    c->kev_pending[k] = i + 1;
+    return true;
 }
 int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n);
 
