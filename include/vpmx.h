@@ -84,7 +84,10 @@ void vpmx_destroy(vpmx_ctx *ctx);
 const char *vpmx_last_error(const vpmx_ctx *ctx);
 int32_t vpmx_set_params(vpmx_ctx *ctx, const vpmx_params *p);
 int32_t vpmx_get_geometry(const vpmx_ctx *ctx, vpmx_geom *out);
-/* precision: 0 = plain FP64 form.  Other values return VPMX_ERR_UNSUP. */
+/* Arithmetic form (no reference counterpart: the reference is FP64 throughout).
+ * 0 = plain FP64 form (default); 1 = mixed form: the Poisson transform passes in FP32, the
+ * Green's-function boundary sums, projection, interpolation, remesh and diagnostics in FP64.
+ * VPMX_PRECISION=mixed in the environment selects 1 at vpmx_create. */
 int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
 
 /* vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...) (vpm.f90:54-475) on host arrays.
@@ -148,10 +151,13 @@ int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double d
  * pm_bc2, pmesh_final, fd, back, remeshing. */
 int32_t vpmx_stage_times(const vpmx_ctx *ctx, double *ms, int32_t n);
 /* Accumulated device time (ms, from each dispatch's start/stop events) and launch counts
- * of the timed kernel classes: 0 fine-block boundary sum.  reset != 0 zeroes them. */
+ * of the timed kernel classes: 0 fine-block boundary sum (Bounds3d, pmbound.f90:80-157),
+ * 1 projection (pmproject.f90:43-153), 2 interpolation (vpm_time.f90:30-107), 3 the y passes
+ * of the fine block's Poisson solves (pmsolve.f90:284-432).  reset != 0 zeroes them. */
 int32_t vpmx_kernel_times(vpmx_ctx *ctx, double *ms_total, int64_t *launches, int32_t n, int32_t reset);
-/* Algorithmic work of the last launch of each timed class (FP64 flops for the boundary
- * sum: 6 per source-target pair plus 20 per distinct 1/r evaluated). */
+/* Algorithmic work of the last launch of each timed class: FP64 flops for the boundary sum
+ * (spectral form: complex products + transforms; lattice form: 6 per pair + 20 per
+ * distinct 1/r), bytes read + written for the Poisson passes. */
 int32_t vpmx_kernel_work(const vpmx_ctx *ctx, double *work, int32_t n);
 
 #ifdef __cplusplus

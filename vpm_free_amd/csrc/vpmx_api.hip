@@ -371,6 +371,8 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
         c->num_cu = ncu;
     const char *wpc = getenv("VPMX_DST_WG_PER_CU");
     if (wpc && atoi(wpc) > 0) c->dst_wg_per_cu = atoi(wpc);
+    const char *pm = getenv("VPMX_PRECISION");
+    if (pm && std::string(pm) == "mixed") c->precision = 1;
     if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
     if (dm && std::string(dm) == "v2") c->dst_kernel = 2;
     if (dm && std::string(dm) == "v3") c->dst_kernel = 3;
