@@ -253,4 +253,74 @@ __device__ void wave_fft(C *buf, const C *tw, const WavePlan &P, int lane) {
     }
 }
 
+
+// ------------------------------------------------------------------ compile-time plans
+// The same Stockham stages with the length and radices known at compile time: spans,
+// strides, twiddle steps and divisions by Ns become constants, which removes most of the
+// integer work of the runtime-plan stages.  Same stage order and arithmetic as wave_fft,
+// so results are identical.
+template <int R, int L, int Ns, class C>
+__device__ __forceinline__ void wave_stage_s(C *buf, const C *__restrict__ tw, int lane) {
+    constexpr int nbf = L / R, twstep = L / (Ns * R), BPL = (nbf + 63) / 64;
+    C v[BPL][R];
+    int dst[BPL];
+#pragma unroll
+    for (int i = 0; i < BPL; i++) {
+        const int j = lane + 64 * i;
+        dst[i] = -1;
+        if ((i + 1) * 64 <= nbf || j < nbf) {
+            const int q = j / Ns, k0 = j - q * Ns;
+#pragma unroll
+            for (int r = 0; r < R; r++) v[i][r] = buf[j + r * nbf];
+            if constexpr (Ns > 1) {
+                // no branch on k0 == 0: tw[0] = (1, 0) leaves the value unchanged
+#pragma unroll
+                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+            }
+            if constexpr (R <= 4) dft_reg<R>(v[i]);
+            dst[i] = q * Ns * R + k0;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < BPL; i++)
+        if (dst[i] >= 0) {
+            if constexpr (R <= 4) {
+#pragma unroll
+                for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
+            } else {
+                dft_odd_store<R>(v[i], buf, dst[i], Ns, tw, L);
+            }
+        }
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <class C, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void wave_fft_s(C *buf, const C *tw, int lane) {
+    wave_stage_s<R, L, Ns, C>(buf, tw, lane);
+    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, L, Ns * R, Rest...>(buf, tw, lane);
+}
+
+// Lengths with a compile-time plan (radices in the planner's order: 4s, 2s, odd primes).
+// FIX = -1 selects the runtime plan.
+template <int FIX> struct FixedFft;
+template <> struct FixedFft<0> {  // 576 = 4.4.4.3.3 (FP32, 289-node block)
+    static constexpr int L = 576;
+    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
+        wave_fft_s<C, 576, 1, 4, 4, 4, 3, 3>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<2> {  // 288 = 4.4.2.3.3 (FP64, 289-node block)
+    static constexpr int L = 288;
+    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
+        wave_fft_s<C, 288, 1, 4, 4, 2, 3, 3>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<3> {  // 272 = 4.4.17 (FP64, 273-node inner box)
+    static constexpr int L = 272;
+    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
+        wave_fft_s<C, 272, 1, 4, 4, 17>(b, tw, lane);
+    }
+};
+
 }  // namespace vpmx

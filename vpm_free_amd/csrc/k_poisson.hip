@@ -748,7 +748,7 @@ __device__ __forceinline__ void wave_post_odd(C *B, int m, int lane) {
 // T: arithmetic and work-array type (double: plain FP64 form, half-length transforms;
 // float: the mixed form, odd-extension transforms, whose RHS, Dirichlet values and output
 // stay FP64 and whose inner passes run in FP32).  P.L is the FFT length (m+1 or 2(m+1)).
-template <int MODE, int MAXR, class T>
+template <int MODE, int MAXR, class T, int FIX>
 __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
 #pragma clang fp contract(off)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
@@ -818,7 +818,8 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if (ODD) wave_pre_odd(B, m, lane, fold, a.ih2[0]);
         else wave_pre(B, a.sn, N, lane, fold, a.ih2[0]);
-        wave_fft<MAXR>(B, twl, P, lane);
+        if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
+            else wave_fft<MAXR>(B, twl, P, lane);
         if (ODD) wave_post_odd(B, m, lane);
         else wave_post(B, N, lane);
         if (MODE == DST_Z) {
@@ -832,7 +833,8 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             __builtin_amdgcn_wave_barrier();
             if (ODD) wave_pre_odd(B, m, lane, nullptr, 0.0);
             else wave_pre(B, a.sn, N, lane, nullptr, 0.0);
-            wave_fft<MAXR>(B, twl, P, lane);
+            if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
+            else wave_fft<MAXR>(B, twl, P, lane);
             if (ODD) wave_post_odd(B, m, lane);
             else wave_post(B, N, lane);
         }
@@ -1066,9 +1068,30 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     a.npairs = 8;
     const size_t lds = 2 * sizeof(T) * (size_t)9 * a.L;
     const int64_t nbatch = (a.nlines + 15) / 16;
-    const void *fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T>
-                   : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T>
-                   : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T> : (const void *)k_dst_wave<MODE, 37, T>;
+    // compile-time plans for the lengths of the common block sizes
+    auto same = [&](const int *rr, int n) {
+        if (fp->nstages != n) return false;
+        for (int q = 0; q < n; q++)
+            if (fp->radix[q] != rr[q]) return false;
+        return true;
+    };
+    static const int r576[] = {4, 4, 4, 3, 3}, r288[] = {4, 4, 2, 3, 3}, r272[] = {4, 4, 17};
+    int fix = -1;
+    if (sizeof(T) == 4 && fp->L == 576 && same(r576, 5)) fix = 0;
+    // (544 = 4.4.2.17 in FP32 keeps the runtime plan: its compile-time form needs more registers
+    //  and halves the resident work-groups)
+    if (sizeof(T) == 8 && fp->L == 288 && same(r288, 5)) fix = 2;
+    if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
+    const void *fn;
+    switch (fix) {
+        case 0: fn = (const void *)k_dst_wave<MODE, 4, T, 0>; break;
+        case 2: fn = (const void *)k_dst_wave<MODE, 4, T, 2>; break;
+        case 3: fn = (const void *)k_dst_wave<MODE, 17, T, 3>; break;
+        default:
+            fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
+               : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
+               : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
+    }
     // persistent grid: as many work-groups as the device holds at once
     int per_cu = c->dst_wg_per_cu;
     if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
@@ -1078,22 +1101,13 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (resident < 1) resident = 1;
     dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
     hipEvent_t e0, e1;
+    void *args[] = {&a, &P};
     if (MODE == DST_Y && a.m == c->dst_timed_m && kernel_events(c, K_DST, &e0, &e1)) {
         // the fine block's y passes: timed by their own dispatch events; work = bytes moved
         c->k_work[K_DST] = 2.0 * sizeof(T) * (double)a.nlines * a.m * ncomp;
-        switch (rc) {
-            case 4: hipExtLaunchKernelGGL((k_dst_wave<MODE, 4, T>), grid, dim3(DST_THREADS), lds, c->stream, e0, e1, 0, a, P); break;
-            case 7: hipExtLaunchKernelGGL((k_dst_wave<MODE, 7, T>), grid, dim3(DST_THREADS), lds, c->stream, e0, e1, 0, a, P); break;
-            case 17: hipExtLaunchKernelGGL((k_dst_wave<MODE, 17, T>), grid, dim3(DST_THREADS), lds, c->stream, e0, e1, 0, a, P); break;
-            default: hipExtLaunchKernelGGL((k_dst_wave<MODE, 37, T>), grid, dim3(DST_THREADS), lds, c->stream, e0, e1, 0, a, P); break;
-        }
+        VPMX_HIP(c, hipExtLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream, e0, e1, 0));
     } else {
-        switch (rc) {
-            case 4: hipLaunchKernelGGL((k_dst_wave<MODE, 4, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-            case 7: hipLaunchKernelGGL((k_dst_wave<MODE, 7, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-            case 17: hipLaunchKernelGGL((k_dst_wave<MODE, 17, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-            default: hipLaunchKernelGGL((k_dst_wave<MODE, 37, T>), grid, dim3(DST_THREADS), lds, c->stream, a, P); break;
-        }
+        VPMX_HIP(c, hipLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream));
     }
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
