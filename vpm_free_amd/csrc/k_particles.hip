@@ -165,6 +165,7 @@ __global__ void k_cell_order(const int *start, const int *count, int64_t ncells,
 
 int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState &s) {
     int st = VPMX_OK;
+    c->sort_n = n;
     s.cn[0] = g.n[0] + 3; s.cn[1] = g.n[1] + 3; s.cn[2] = g.n[2] + 3;
     s.ncells = (int64_t)s.cn[0] * s.cn[1] * s.cn[2];
     int *keys = (int *)ensure(c, s.keys, sizeof(int) * (size_t)(n > 0 ? n : 1), &st); if (st) return st;
@@ -235,12 +236,151 @@ __global__ void __launch_bounds__(256) k_p2m_gather(Grid g, const int *start, co
     for (int r = 0; r < NC; r++) out.p[r * out.cs + o] = acc[r];
 }
 
+// M4' (itype 4) weight, the expression of proj_fun.
+__device__ __forceinline__ double m4p(double x) {
+#pragma clang fp contract(off)
+    const double a = fabs(x);
+    double r = 0.0;
+    if (a < 1.0) r = 1.0 - 2.5 * (a * a) + 1.5 * (a * a * a);
+    else if (a <= 2.0) r = 0.5 * ((2.0 - a) * (2.0 - a)) * (1.0 - a);
+    return r;
+}
+
+// Particle data in cell order (positions, NC strengths), for coalesced reads in k_p2m_block.
+template <int NC>
+__global__ void k_sorted_copy(const int *perm, const int *total, int64_t n, const double *xp, const double *qp,
+                              int qstride, double *xs, double *qs) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || s >= (int64_t)*total) return;
+    const int64_t p = perm[s];
+#pragma unroll
+    for (int d = 0; d < 3; d++) xs[3 * s + d] = xp[3 * p + d];
+#pragma unroll
+    for (int r = 0; r < NC; r++) qs[NC * s + r] = qp[(int64_t)qstride * p + r];
+}
+
+// M4' projection, one thread per block of BX x BY x BZ view nodes.  A particle in cell ci
+// (inode) reaches nodes ci-1..ci+2, so the block's nodes i0..i0+BX-1 collect the cells
+// i0-2..i0+BX; cells are visited in (k, j, i) order and the particles of a cell in their
+// sorted order, so every node sums its particles in the same order as k_p2m_gather (and
+// gets the same bits).  The x extent of each cell's reach is resolved at compile time;
+// in y and z both block nodes are evaluated (M4' is exactly 0 beyond its support).
+template <int NC, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256) k_p2m_block(Grid g, const int *start, int cnx, int cny, const double *xs,
+                                                   const double *qs, double div, GView out, int vnx, int vny, int vnz,
+                                                   int wlo0, int wlo1, int wlo2) {
+#pragma clang fp contract(off)
+    const int nbx = (vnx + BX - 1) / BX, nby = (vny + BY - 1) / BY, nbz = (vnz + BZ - 1) / BZ;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nbx * nby * nbz) return;
+    const int bx = (int)(t % nbx), by = (int)((t / nbx) % nby), bz = (int)(t / ((int64_t)nbx * nby));
+    const int a0 = bx * BX, b0 = by * BY, c0 = bz * BZ;
+    const int i0 = a0 - wlo0 + 1, j0 = b0 - wlo1 + 1, k0 = c0 - wlo2 + 1;
+    double acc[BZ][BY][BX][NC];
+#pragma unroll
+    for (int c = 0; c < BZ; c++)
+#pragma unroll
+        for (int b = 0; b < BY; b++)
+#pragma unroll
+            for (int a = 0; a < BX; a++)
+#pragma unroll
+                for (int r = 0; r < NC; r++) acc[c][b][a][r] = 0.0;
+    const double X = g.x0[0], Y = g.x0[1], Z = g.x0[2], hx = g.h[0], hy = g.h[1], hz = g.h[2];
+    double ox[BX], oy[BY], oz[BZ];
+#pragma unroll
+    for (int a = 0; a < BX; a++) ox[a] = (double)(i0 + a - 1) * hx;
+#pragma unroll
+    for (int b = 0; b < BY; b++) oy[b] = (double)(j0 + b - 1) * hy;
+#pragma unroll
+    for (int c = 0; c < BZ; c++) oz[c] = (double)(k0 + c - 1) * hz;
+    for (int cz = 0; cz < BZ + 3; cz++) {
+        const int ck = k0 - 2 + cz;
+        if (ck < -1 || ck > g.n[2] + 1) continue;
+        for (int cy = 0; cy < BY + 3; cy++) {
+            const int cj = j0 - 2 + cy;
+            if (cj < -1 || cj > g.n[1] + 1) continue;
+            const int64_t row = (int64_t)cnx * ((cj + 1) + (int64_t)cny * (ck + 1));
+#pragma unroll
+            for (int cx = 0; cx < BX + 3; cx++) {
+                const int ci = i0 - 2 + cx;
+                if (ci < -1 || ci > g.n[0] + 1) continue;
+                constexpr int dummy = 0;
+                (void)dummy;
+                const int xlo = cx - 3 > 0 ? cx - 3 : 0, xhi = cx < BX - 1 ? cx : BX - 1;
+                const int p0 = start[row + ci + 1], p1 = start[row + ci + 2];
+                for (int sidx = p0; sidx < p1; sidx++) {
+                    const double px = xs[3 * (int64_t)sidx], py = xs[3 * (int64_t)sidx + 1],
+                                 pz = xs[3 * (int64_t)sidx + 2];
+                    double q[NC];
+#pragma unroll
+                    for (int r = 0; r < NC; r++) q[r] = qs[NC * (int64_t)sidx + r];
+                    double fx[BX], fy[BY], fz[BZ];
+#pragma unroll
+                    for (int a = 0; a < BX; a++) fx[a] = (a >= xlo && a <= xhi) ? m4p((px - X - ox[a]) / hx) : 0.0;
+#pragma unroll
+                    for (int b = 0; b < BY; b++) fy[b] = m4p((py - Y - oy[b]) / hy);
+#pragma unroll
+                    for (int c = 0; c < BZ; c++) fz[c] = m4p((pz - Z - oz[c]) / hz);
+#pragma unroll
+                    for (int c = 0; c < BZ; c++)
+#pragma unroll
+                        for (int b = 0; b < BY; b++)
+#pragma unroll
+                            for (int a = 0; a < BX; a++) {
+                                if (a < xlo || a > xhi) continue;
+                                const double f = fx[a] * fy[b] * fz[c];
+#pragma unroll
+                                for (int r = 0; r < NC; r++) acc[c][b][a][r] = acc[c][b][a][r] + f * q[r];
+                            }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < BZ; c++)
+#pragma unroll
+        for (int b = 0; b < BY; b++)
+#pragma unroll
+            for (int a = 0; a < BX; a++) {
+                const int va = a0 + a, vb = b0 + b, vc = c0 + c;
+                if (va >= vnx || vb >= vny || vc >= vnz) continue;
+                const int i = i0 + a, j = j0 + b, k = k0 + c;
+                const bool in = i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+                const int64_t o = out.at(va, vb, vc);
+#pragma unroll
+                for (int r = 0; r < NC; r++) out.p[r * out.cs + o] = in ? acc[c][b][a][r] / div : 0.0;
+            }
+}
+
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
                int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass) {
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
     hipEvent_t e0, e1;
+    if (ncomp == 3 && itype == 4 && c->p2m_block) {
+        // cell-ordered copy of the particles, then one thread per 4x2x2 node block
+        constexpr int BX = 4, BY = 2, BZ = 2;
+        int st = VPMX_OK;
+        const int64_t n = c->sort_n;
+        double *xs = (double *)ensure(c, c->sort.xs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
+        if (st) return st;
+        double *qs = (double *)ensure(c, c->sort.qs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
+        if (st) return st;
+        if (n > 0)
+            hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
+                               start + s.ncells, n, xp, qp, qstride, xs, qs);
+        const int64_t nblk = (int64_t)((vn[0] + BX - 1) / BX) * ((vn[1] + BY - 1) / BY) * ((vn[2] + BZ - 1) / BZ);
+        const unsigned gb = (unsigned)((nblk + 255) / 256);
+        if (kclass >= 0 && kernel_events(c, kclass, &e0, &e1))
+            hipExtLaunchKernelGGL((k_p2m_block<3, BX, BY, BZ>), dim3(gb), dim3(256), 0, c->stream, e0, e1, 0, g, start,
+                                  s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+        else
+            hipLaunchKernelGGL((k_p2m_block<3, BX, BY, BZ>), dim3(gb), dim3(256), 0, c->stream, g, start, s.cn[0],
+                               s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+        VPMX_HIP(c, hipGetLastError());
+        return VPMX_OK;
+    }
     if (ncomp == 3 && kclass >= 0 && kernel_events(c, kclass, &e0, &e1)) {
         hipExtLaunchKernelGGL(k_p2m_gather<3>, dim3(grid), dim3(256), 0, c->stream, e0, e1, 0, g, start, perm, s.cn[0],
                               s.cn[1], xp, qp, qstride, itype, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);

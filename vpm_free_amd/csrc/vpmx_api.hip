@@ -371,6 +371,8 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
         c->num_cu = ncu;
     const char *wpc = getenv("VPMX_DST_WG_PER_CU");
     if (wpc && atoi(wpc) > 0) c->dst_wg_per_cu = atoi(wpc);
+    const char *pb = getenv("VPMX_P2M");
+    if (pb && std::string(pb) == "node") c->p2m_block = false;
     const char *pm = getenv("VPMX_PRECISION");
     if (pm && std::string(pm) == "mixed") c->precision = 1;
     if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
@@ -390,7 +392,8 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->samp, &c->rhs_c, &c->sol_c,
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
-                    &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill};
+                    &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
+                    &c->sort.xs, &c->sort.qs};
     for (DBuf *b : bufs) release(*b);
     free_bspec(c);
     free_fft_plans(c);

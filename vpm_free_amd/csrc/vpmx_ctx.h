@@ -58,6 +58,8 @@ struct Ctx : vpmx_ctx {
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
     DBuf stage_a, stage_b, stage_c;
     SortState sort;
+    int64_t sort_n = 0;     // particles of the last sort
+    bool p2m_block = true;  // projection: node-block kernel (else one thread per node)
     std::map<int, void *> fft_plans;
     bool have_grids = false;  // RHS/SOL/vel of the last mode 1-3 call are valid
 

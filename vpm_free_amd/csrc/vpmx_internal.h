@@ -55,6 +55,7 @@ void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st);
 // Particles / projection (k_particles.hip)
 struct SortState {
     DBuf keys, perm, count, start, fill;
+    DBuf xs, qs;  // particle positions / strengths in cell order (projection)
     int64_t ncells = 0;
     int cn[3] = {0, 0, 0};  // padded cell grid dims
 };
