@@ -68,6 +68,20 @@ def stencil_nodes(xp, g):
     return int(occ.sum())
 
 
+def measured_traffic(mixed):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "dst_y_pass_289_traffic.md")
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch"):
+                kv = dict(t.split("=") for t in line.split()[1:])
+                return int(kv["float" if mixed else "double"])
+    except (OSError, KeyError, ValueError):
+        return None
+    return None
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
@@ -217,7 +231,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": dk.get("frac"),
-        "traffic": None,
+        "traffic": measured_traffic(mixed),
+        "traffic_source": "profiles/dst_y_pass_289_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         "kernel": "k_dst_wave<DST_Y,4,%s>: y pass of the 289^3 block's sine-transform Poisson solve (3 comps)"
                   % ("float" if mixed else "double"),
         "ms_per_launch": dk.get("ms"),
