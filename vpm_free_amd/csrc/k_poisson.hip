@@ -35,6 +35,7 @@ struct DstArgs {
     int radix[24];
     int64_t nlines;     // lines per component
     int mx, my, mz;     // interior extents
+    int px;             // row pitch of the work array (elements, >= mx)
     int64_t wcs;        // component stride of the work array
     double *w;          // work array (mx,my,mz) per component
     const double2 *tw;  // exp(-2 pi i t / L), t < L
@@ -491,11 +492,21 @@ __device__ __forceinline__ void dst_elem(int m, int nl, int u, int *line, int *p
 }
 
 // Global offset (within a component) of point pt of line gl of the work array.
+// Rows of the work array have pitch px >= mx; y lines are (i, k) and z lines (i, j) with i
+// running over the pitch, so a batch of neighbouring lines starts on a row-aligned x.
 template <int MODE>
 __device__ __forceinline__ int64_t dst_addr(const DstArgs &a, int gl, int pt) {
-    if (MODE == DST_XF || MODE == DST_XI) return (int64_t)gl * a.mx + pt;
-    if (MODE == DST_Y) return (int64_t)(gl % a.mx) + (int64_t)a.mx * a.my * (gl / a.mx) + (int64_t)pt * a.mx;
-    return (int64_t)gl + (int64_t)pt * a.mx * a.my;
+    if (MODE == DST_XF || MODE == DST_XI) return (int64_t)gl * a.px + pt;
+    if (MODE == DST_Y) return (int64_t)(gl % a.px) + (int64_t)a.px * a.my * (gl / a.px) + (int64_t)pt * a.px;
+    return (int64_t)gl + (int64_t)pt * a.px * a.my;
+}
+
+// Line gl exists (padding lines of the y and z passes, x >= mx, are skipped).
+template <int MODE>
+__device__ __forceinline__ bool dst_line_ok(const DstArgs &a, int gl) {
+    if (gl >= a.nlines) return false;
+    if (MODE == DST_Y || MODE == DST_Z) return gl % a.px < a.mx;
+    return true;
 }
 
 // Issue the loads of batch b into registers (no wait).  For the x forward pass, thread
@@ -509,7 +520,7 @@ __device__ __forceinline__ void dst_prefetch(const DstArgs &a, const T *w, int c
         dst_elem<MODE>(a.m, nl, u, &line, &pt);
         const int gl = b * nl + line;
         v[u] = (T)0;
-        if (line < nl && pt < a.m && gl < a.nlines) {
+        if (line < nl && pt < a.m && dst_line_ok<MODE>(a, gl)) {
             if (MODE == DST_XF) {
                 const int j = gl % a.my, k = gl / a.my;
                 v[u] = (T)a.rhs.p[comp * a.rhs.cs + a.rhs.at(pt + 1, j + 1, k + 1)];
@@ -770,9 +781,14 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
     T v[DST_PF];
     double fx[2];
-    int b = blockIdx.x;
+    // batch of iteration `it`: the two halves of a batch pair (neighbouring lines, one
+    // 128-byte row segment in FP32) go to work-groups w and w+8, which share an XCD (and its
+    // L2); gridDim.x is a multiple of 16
+    const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
+    auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
+    int it = 0, b = batch_of(0);
     if (b < nbatch) dst_prefetch<MODE, T>(a, w, comp, b, nl, v, fx);
-    for (; b < nbatch; b += gridDim.x) {
+    for (; b < nbatch; b = batch_of(++it)) {
 #pragma unroll
         for (int u = 0; u < DST_PF; u++) {
             int line, pt;
@@ -787,7 +803,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             dst_elem<MODE>(m, nl, 0, &line, &pt);
             const int gl = b * nl + line;
             const int j = gl % a.my, k = gl / a.my;
-            if (gl < a.nlines && (j == 0 || j == a.my - 1 || k == 0 || k == a.mz - 1)) {
+            if (dst_line_ok<MODE>(a, gl) && (j == 0 || j == a.my - 1 || k == 0 || k == a.mz - 1)) {
                 const double *bp = a.bc.p + comp * a.bc.cs;
 #pragma unroll 1
                 for (int u = 0; u < DST_PF; u++) {
@@ -810,11 +826,12 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
         if (MODE == DST_Z && tid < nl) {
             const int gl = b * nl + tid;
             double lv = 0.0;
-            if (gl < a.nlines) lv = a.lam[0][gl % a.mx] + a.lam[1][gl / a.mx];
+            if (dst_line_ok<MODE>(a, gl)) lv = a.lam[0][gl % a.px] + a.lam[1][gl / a.px];
             lamxy[tid] = lv;
         }
         __syncthreads();
-        if (b + (int)gridDim.x < nbatch) dst_prefetch<MODE, T>(a, w, comp, b + gridDim.x, nl, v, fx);
+        const int bn = batch_of(it + 1);
+        if (bn < nbatch) dst_prefetch<MODE, T>(a, w, comp, bn, nl, v, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if (ODD) wave_pre_odd(B, m, lane, fold, a.ih2[0]);
         else wave_pre(B, a.sn, N, lane, fold, a.ih2[0]);
@@ -844,7 +861,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             int line, pt;
             dst_elem<MODE>(m, nl, u, &line, &pt);
             const int gl = b * nl + line;
-            if (line >= nl || pt >= m || gl >= a.nlines) continue;
+            if (line >= nl || pt >= m || !dst_line_ok<MODE>(a, gl)) continue;
             const T val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * (T)a.scale;
             if (MODE == DST_XI) {
                 const int j = gl % a.my, k = gl / a.my;
@@ -1098,13 +1115,16 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         per_cu = 1;
     if (per_cu < 1) per_cu = 1;
     int64_t resident = (int64_t)c->num_cu * per_cu / (ncomp > 0 ? ncomp : 1);
-    if (resident < 1) resident = 1;
-    dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
+    // multiple of 16 work-groups (batch pairs on one XCD), no more than the batches need
+    resident = resident / 16 * 16;
+    if (resident < 16) resident = 16;
+    const int64_t need = (nbatch + 15) / 16 * 16;
+    dim3 grid((unsigned)(need < resident ? need : resident), ncomp);
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
     if (MODE == DST_Y && a.m == c->dst_timed_m && kernel_events(c, K_DST, &e0, &e1)) {
         // the fine block's y passes: timed by their own dispatch events; work = bytes moved
-        c->k_work[K_DST] = 2.0 * sizeof(T) * (double)a.nlines * a.m * ncomp;
+        c->k_work[K_DST] = 2.0 * sizeof(T) * (double)a.mx * a.mz * a.m * ncomp;
         VPMX_HIP(c, hipExtLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream, e0, e1, 0));
     } else {
         VPMX_HIP(c, hipLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream));
@@ -1153,9 +1173,7 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     for (int d = 0; d < 3; d++)
         if (2 * (int64_t)(m[d] + 1) * sizeof(double2) * 2 > 65536)
             return fail(c, VPMX_ERR_UNSUP, "Poisson: line length too long for LDS");
-    int64_t mt = (int64_t)m[0] * m[1] * m[2];
-    double *w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)mt * ncomp, &st);
-    if (st) return st;
+    double *w = nullptr;
     AxisPlan ax[3];
     for (int d = 0; d < 3; d++) {
         ax[d].m = m[d];
@@ -1165,8 +1183,7 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     }
     DstArgs a{};
     a.mx = m[0]; a.my = m[1]; a.mz = m[2];
-    a.wcs = mt;
-    a.w = w;
+    a.px = m[0];
     a.rhs = rhs;
     a.out = out;
     a.has_bc = bc ? 1 : 0;
@@ -1179,17 +1196,28 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     a.scale = 1.0;
     // mixed form: the five passes in FP32 when every axis takes the one-wave kernel
     const bool f32 = c->precision == 1 && wave_class(ax[0], true) && wave_class(ax[1], true) && wave_class(ax[2], true);
+    // one-wave kernels on every axis: rows of the work array padded to 128 bytes
+    const bool wave_all = f32 || (c->dst_kernel == 4 && wave_class(ax[0], false) && wave_class(ax[1], false) &&
+                                  wave_class(ax[2], false));
+    if (wave_all) {
+        const int q = f32 ? 32 : 16;
+        a.px = (m[0] + q - 1) / q * q;
+    }
+    a.wcs = (int64_t)a.px * m[1] * m[2];
+    w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)a.wcs * ncomp, &st);
+    if (st) return st;
+    a.w = w;
     // x forward (with boundary folding)
     a.nlines = (int64_t)m[1] * m[2];
     VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0], f32));
     // y forward
-    a.nlines = (int64_t)m[0] * m[2];
+    a.nlines = (int64_t)a.px * m[2];
     VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
     // z forward + diagonal + inverse
-    a.nlines = (int64_t)m[0] * m[1];
+    a.nlines = (int64_t)a.px * m[1];
     VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
     // y inverse
-    a.nlines = (int64_t)m[0] * m[2];
+    a.nlines = (int64_t)a.px * m[2];
     VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
     // x inverse -> output interior
     a.nlines = (int64_t)m[1] * m[2];
