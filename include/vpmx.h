@@ -91,9 +91,12 @@ int32_t vpmx_get_geometry(const vpmx_ctx *ctx, vpmx_geom *out);
 int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
 
 /* vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...) (vpm.f90:54-475) on host arrays.
- * WhatToDo 0: geometry (define_sizes); 2: project, yaps solve, velocity + stretching,
- * interpolate (UP, GP overwritten); 1: project + solve + velocity (grids kept);
- * 3: project + solve.  Other modes return VPMX_ERR_UNSUP. */
+ * WhatToDo 0: geometry (define_sizes), velocity grids zeroed; 2: project, yaps solve,
+ * velocity + stretching, interpolate (UP, GP overwritten); 1: project + solve + velocity
+ * (grids kept); 3: project + solve; 4: project, stretching from the velocity grids of the
+ * last 0/1/5 call, interpolate (vpm.f90:316-343); 5: project, diffusion -NI lap(w)
+ * (vpm_gcalc.f90:138-210), UP = 0 and GP = interpolated diffusion * QP(4)
+ * (vpm_time.f90:109-151), velocity grids zeroed (vpm.f90:345-373). */
 int32_t vpmx_vpm(vpmx_ctx *ctx, double *xp, double *qp, double *up, double *gp, int64_t nvr, int32_t neqpm,
                  int32_t whattodo, int32_t ntime, double ni);
 

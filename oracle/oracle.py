@@ -92,6 +92,9 @@ def lib():
             "orc_set_threads": (None, [C.c_int]),
             "orc_timers": (None, [D, C.c_int]),
             "orc_set_boundsum_stride": (None, [C.c_int]),
+            "orc_diffuse": (None, [C.POINTER(Geom), C.c_int, C.c_double, D, D]),
+            "orc_vpm45": (i32, [C.POINTER(Params), C.POINTER(Geom), C.c_int, C.c_int, C.c_double, D, D, D, D, i64,
+                                D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -233,6 +236,31 @@ def vpm2(params, geom, xp, qp, neqpm=3, fields=False):
     if fields:
         return up, gp, {"rhs": rhs, "psi": psi, "sol": sol, "vel": vel}
     return up, gp
+
+
+def diffuse(geom, ni, rhs):
+    """diffuse_vort_3d on rhs (G x 4, node-major, x fastest); returns SOL (G x 3)."""
+    rhs = _f(rhs)
+    G = rhs.shape[0] if rhs.ndim == 2 else int(np.prod(rhs.shape[:-1]))
+    out = np.zeros((G, 3))
+    lib().orc_diffuse(C.byref(geom), 3, ni, _d(rhs), _d(out))
+    return out
+
+
+def vpm45(params, geom, xp, qp, mode, ni=0.0, vel=None, neqpm=3):
+    """vpm WhatToDo 4 (stretching from the velocity grids `vel`, 3 x G) or 5 (diffusion).
+    Returns UP, GP and the SOL grid."""
+    xp, qp = _f(xp), _f(qp)
+    n = xp.shape[0]
+    up, gp = np.zeros((n, 3)), np.zeros((n, 3))
+    G = int(geom.nn[0]) * int(geom.nn[1]) * int(geom.nn[2])
+    sol = np.zeros((G, neqpm))
+    v = _f(vel) if vel is not None else None
+    r = lib().orc_vpm45(C.byref(params), C.byref(geom), neqpm, mode, ni, _d(xp), _d(qp), _d(up), _d(gp),
+                        n, _d(v) if v is not None else None, _d(sol))
+    if r != 0:
+        raise RuntimeError("orc_vpm45 failed")
+    return up, gp, sol
 
 
 def remesh(params, geom, xp, qp, neqpm=3, mrem=1):

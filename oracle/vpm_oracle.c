@@ -26,6 +26,11 @@
  *   orc_m2p              exact interpolation of quadratics (M4'), brute force
  *   orc_remesh           brute-force threshold scan, circulation/impulse conservation
  *   orc_vpm2             Hill's spherical vortex closed form (test_applic.f90:2-77)
+ *   orc_diffuse          exact on quadratic fields (the 7-point Laplacian of a quadratic is its
+ *                        constant Laplacian)
+ *   orc_vpm45            composition of pinned parts (projection, stretching, diffusion, M2P);
+ *                        mode-4/5 semantics (UP zeroed in 5) read off vpm.f90/vpm_time.f90 --
+ *                        parity unpinned beyond that composition
  */
 #include "vpm_oracle.h"
 
@@ -898,6 +903,66 @@ int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp
     if (vel_out) memcpy(vel_out, vel, sizeof(double) * 3 * G);
     free(vel); free(rhs); free(sol);
     g_timer[3] += now() - t0;
+    return 0;
+}
+
+/* ------------------------------------------------------------- modes 4 and 5 */
+/* diffuse_vort_3d (vpm_gcalc.f90:138-210), neqpm = 3: SOL(1:3) = -NI lap(RHS(1:3)) on the
+ * nodes NXs_bl+1..NXf_bl-1 (and y, z), 0 elsewhere. */
+void orc_diffuse(const orc_geom *g, int neqpm, double ni, const double *rhs, double *sol) {
+    int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
+    int xs = g->nn_bl[0], ys = g->nn_bl[1], zs = g->nn_bl[2], xf = g->nn_bl[3], yf = g->nn_bl[4], zf = g->nn_bl[5];
+    double DX2 = g->dpm[0] * g->dpm[0], DY2 = g->dpm[1] * g->dpm[1], DZ2 = g->dpm[2] * g->dpm[2];
+    int nr = neqpm + 1;
+    memset(sol, 0, sizeof(double) * (size_t)neqpm * NX * NY * NZ);
+#define SL(c, i, j, k) sol[IX(c, (i)-1, (j)-1, (k)-1, neqpm, NX, NY)]
+#define RH(c, i, j, k) rhs[IX(c, (i)-1, (j)-1, (k)-1, nr, NX, NY)]
+#pragma omp parallel for schedule(static)
+    for (int k = zs + 1; k <= zf - 1; k++)
+        for (int j = ys + 1; j <= yf - 1; j++)
+            for (int i = xs + 1; i <= xf - 1; i++) {
+                double VIS = ni;  /* neqpm = 3 */
+                for (int c = 0; c < 3; c++) {
+                    double dwxdx = (RH(c, i + 1, j, k) - 2 * RH(c, i, j, k) + RH(c, i - 1, j, k)) / DX2;
+                    double dwydy = (RH(c, i, j + 1, k) - 2 * RH(c, i, j, k) + RH(c, i, j - 1, k)) / DY2;
+                    double dwzdz = (RH(c, i, j, k + 1) - 2 * RH(c, i, j, k) + RH(c, i, j, k - 1)) / DZ2;
+                    SL(c, i, j, k) = -VIS * (dwxdx + dwydy + dwzdz);
+                }
+            }
+#undef SL
+#undef RH
+}
+
+/* vpm WhatToDo 4 (vpm.f90:316-343) and 5 (vpm.f90:345-373): projection, then for 4 the
+ * stretching term from the velocity grids vel (3 x G, from an earlier call) and
+ * back_to_particles itype 1; for 5 the diffusion term and back_to_particles itype 2
+ * (vpm_time.f90:109-151: UP = 0, GP = sum W SOL * QP(4)).  sol_out: the SOL grid. */
+int orc_vpm45(const orc_params *p, const orc_geom *g, int neqpm, int mode, double ni, const double *xp,
+              const double *qp, double *up, double *gp, int64_t nvr, const double *vel, double *sol_out) {
+    if (nvr == 0) return 0;
+    if (g->nblocks != 1 || neqpm != 3 || (mode != 4 && mode != 5)) return -1;
+    int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
+    size_t G = (size_t)NX * NY * NZ;
+    int nr = neqpm + 1;
+    double *rhs = calloc(G * nr, sizeof(double));
+    double *sol = calloc(G * neqpm, sizeof(double));
+    orc_project(g->xbound, g->dpm, g->nn, g->nn_bl, xp, qp, nvr, nr, p->interf_iproj, p->idvpm, rhs);
+    if (mode == 4) {
+        double *v = calloc(3 * G, sizeof(double));
+        if (vel) memcpy(v, vel, sizeof(double) * 3 * G);
+        orc_velocity(g, neqpm, rhs, sol, v, v + G, v + 2 * G, -1);
+        orc_m2p(g, neqpm, p->interf_iproj, sol, v, v + G, v + 2 * G, xp, qp, up, gp, nvr);
+        free(v);
+    } else {
+        double *z = calloc(3 * G, sizeof(double));
+        orc_diffuse(g, neqpm, ni, rhs, sol);
+        orc_m2p(g, neqpm, p->interf_iproj, sol, z, z + G, z + 2 * G, xp, qp, up, gp, nvr);
+        for (int64_t nv = 0; nv < nvr; nv++)
+            for (int c = 0; c < 3; c++) up[3 * nv + c] = 0.0;
+        free(z);
+    }
+    if (sol_out) memcpy(sol_out, sol, sizeof(double) * G * neqpm);
+    free(rhs); free(sol);
     return 0;
 }
 

@@ -100,6 +100,9 @@ void orc_dst1(int m, const double *in, double *out); /* S_k = sum_j x_j sin(pi j
 int orc_num_threads(void);
 void orc_timers(double *out, int reset);
 void orc_set_boundsum_stride(int s);
+void orc_diffuse(const orc_geom *g, int neqpm, double ni, const double *rhs, double *sol);
+int orc_vpm45(const orc_params *p, const orc_geom *g, int neqpm, int mode, double ni, const double *xp,
+              const double *qp, double *up, double *gp, int64_t nvr, const double *vel, double *sol_out);
 void orc_set_threads(int n);
 
 #endif

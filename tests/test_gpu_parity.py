@@ -248,3 +248,32 @@ def test_vpm_mode2_mixed(vpmlib, oracle, h, sigma):
     for k, v in errs.items():
         assert v <= MIXED_TOL, (k, v)
     assert errs["up"] > 0.0  # the FP32 passes ran
+
+
+def test_vpm_modes_4_and_5(vpmlib, oracle):
+    """WhatToDo 4 (stretching from the velocity grids of a preceding mode-1 call, then
+    interpolation) and 5 (diffusion -NI lap(w), UP = 0, GP interpolated) against the oracle."""
+    V = vpmlib
+    h = 0.125
+    xp, qp = _ring_case(h, 0.2)
+    rng = np.random.default_rng(11)
+    xm = xp + rng.uniform(-0.2, 0.2, xp.shape) * h  # particles moved after the mode-1 call
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xp)
+    _, _, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    up4_r, gp4_r, _ = oracle.vpm45(po, go, xm, qp, 4, vel=f["vel"])
+    up5_r, gp5_r, _ = oracle.vpm45(po, go, xm, qp, 5, ni=0.02)
+    say("case: vpm modes 4 and 5, n=%d" % xp.shape[0])
+    with V.VPM(V.Params.make(h)) as c:
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 1, 1)
+        up4, gp4 = np.full_like(xp, 7.0), np.full_like(xp, 7.0)
+        c.vpm(xm, qp, up4, gp4, xm.shape[0], 3, 4, 1)
+        up5, gp5 = np.full_like(xp, 7.0), np.full_like(xp, 7.0)
+        c.vpm(xm, qp, up5, gp5, xm.shape[0], 3, 5, 1, NI=0.02)
+    errs = {"up4": rel_l2(up4, up4_r), "gp4": rel_l2(gp4, gp4_r), "gp5": rel_l2(gp5, gp5_r)}
+    say("  rel L2:", errs)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+    assert np.all(up5 == 0.0)

@@ -358,3 +358,30 @@ def test_m2p_reproduces_quadratics(oracle):
     assert np.allclose(up, np.stack([q(a, b, c), q(b, c, a), q(c, a, b)], 1), rtol=0, atol=1e-12)
     want_g = np.stack([q(a, c, b), q(c, b, a), q(b, a, c)], 1) * qp[:, 3:4]
     assert np.allclose(gp, want_g, rtol=0, atol=1e-12)
+
+
+def test_diffuse_exact_on_quadratics(oracle):
+    """diffuse_vort_3d (vpm_gcalc.f90:138-210): -NI times the 7-point Laplacian, exact on
+    quadratics (constant Laplacian), zero outside the NN_bl interior."""
+    from vpm_free_amd import cases
+    h = 0.125
+    xp, _ = cases.single_ring(h=h, sigma=0.2)
+    p = oracle.Params.make(h)
+    g = oracle.define_sizes(p, xp)
+    nn = [int(v) for v in g.nn]
+    G = nn[0] * nn[1] * nn[2]
+    x0, d = np.array(g.xbound[:3]), np.array(g.dpm)
+    k, j, i = np.meshgrid(np.arange(nn[2]), np.arange(nn[1]), np.arange(nn[0]), indexing="ij")
+    X, Y, Z = x0[0] + i * d[0], x0[1] + j * d[1], x0[2] + k * d[2]
+    rhs = np.zeros((G, 4))
+    rhs[:, 0] = (X ** 2 + 2 * Y ** 2 - Z * X).ravel()   # lap = 6
+    rhs[:, 1] = (3 * Z ** 2 - X * Y).ravel()            # lap = 6
+    rhs[:, 2] = (X * Y + Y * Z).ravel()                 # lap = 0
+    ni = 0.01
+    sol = oracle.diffuse(g, ni, rhs)
+    bl = [int(v) for v in g.nn_bl]
+    inner = ((i >= bl[0]) & (i <= bl[3] - 2) & (j >= bl[1]) & (j <= bl[4] - 2) & (k >= bl[2]) & (k <= bl[5] - 2)).ravel()
+    np.testing.assert_allclose(sol[inner, 0], -ni * 6.0, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(sol[inner, 1], -ni * 6.0, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(sol[inner, 2], 0.0, rtol=0, atol=1e-9)
+    assert np.all(sol[~inner] == 0.0)

@@ -94,6 +94,41 @@ int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView v
     return VPMX_OK;
 }
 
+// diffuse_vort_3d (vpm_gcalc.f90:138-210), neqpm = 3: out = -NI * lap(RHS) per component on the
+// interior nodes (7-point second differences, the reference's term order), 0 elsewhere.
+__global__ void __launch_bounds__(256) k_diffuse(int nx, int ny, int nz, double dx2, double dy2, double dz2,
+                                                 double vis, GView rhs, GView out) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    double r[3] = {0.0, 0.0, 0.0};
+    if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
+        const int64_t c0 = rhs.at(i, j, k);
+        const int64_t xp = rhs.at(i + 1, j, k), xm = rhs.at(i - 1, j, k);
+        const int64_t yp = rhs.at(i, j + 1, k), ym = rhs.at(i, j - 1, k);
+        const int64_t zp = rhs.at(i, j, k + 1), zm = rhs.at(i, j, k - 1);
+#pragma unroll
+        for (int m = 0; m < 3; m++) {
+            const double *R = rhs.p + m * rhs.cs;
+            const double dwxdx = (R[xp] - 2 * R[c0] + R[xm]) / dx2;
+            const double dwydy = (R[yp] - 2 * R[c0] + R[ym]) / dy2;
+            const double dwzdz = (R[zp] - 2 * R[c0] + R[zm]) / dz2;
+            r[m] = -vis * (dwxdx + dwydy + dwzdz);
+        }
+    }
+    const int64_t o = out.at(i, j, k);
+    for (int m = 0; m < 3; m++) out.p[m * out.cs + o] = r[m];
+}
+
+int diffuse_kernel(Ctx *c, const int n[3], const double h[3], double vis, GView rhs, GView out) {
+    int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_diffuse, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2],
+                       h[0] * h[0], h[1] * h[1], h[2] * h[2], vis, rhs, out);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
 // coarse(c, cmap_x[a], cmap_y[b], cmap_z[cc]) = fine(c, a*nc, b*nc, cc*nc)  (0-based)
 __global__ void k_coarse_sample(int ncomp, int sx, int sy, int sz, int nc, const int *cmap, GView fine, GView coarse) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

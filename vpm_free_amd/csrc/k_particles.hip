@@ -398,6 +398,7 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
 
 // ------------------------------------------------------------------ M2P
 // back_to_particles_3D itype 1 (vpm_time.f90:30-107): UP = sum W vel, GP = (sum W SOL)*QP(neqpm+1).
+template <int ITYPEB>
 __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const double *qp, int qstride, int itype,
                                               GView vel, GView sol, double *up, double *gp, int64_t n) {
 #pragma clang fp contract(off)
@@ -426,9 +427,11 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
                     double f = fx[a] * fy[b] * fz[c];
                     if (i < 1 || i > g.n[0] || j < 1 || j > g.n[1] || k < 1 || k > g.n[2]) continue;
                     int64_t ov = vel.at(i - 1, j - 1, k - 1), os = sol.at(i - 1, j - 1, k - 1);
-                    u0 = u0 + f * vel.p[ov];
-                    u1 = u1 + f * vel.p[vel.cs + ov];
-                    u2 = u2 + f * vel.p[2 * vel.cs + ov];
+                    if (ITYPEB == 1) {
+                        u0 = u0 + f * vel.p[ov];
+                        u1 = u1 + f * vel.p[vel.cs + ov];
+                        u2 = u2 + f * vel.p[2 * vel.cs + ov];
+                    }
                     g0 = g0 + f * sol.p[os];
                     g1 = g1 + f * sol.p[sol.cs + os];
                     g2 = g2 + f * sol.p[2 * sol.cs + os];
@@ -442,15 +445,17 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
 }
 
 int m2p_gather(Ctx *c, const Grid &g, const SortState &, const double *xp, const double *qp, int qstride, int itype,
-               GView vel, GView sol, double *up, double *gp, int64_t n) {
+               GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb) {
     if (n <= 0) return VPMX_OK;
     hipEvent_t e0, e1;
-    if (kernel_events(c, K_M2P, &e0, &e1))
-        hipExtLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, e0, e1, 0, g, xp, qp,
-                              qstride, itype, vel, sol, up, gp, n);
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (itypeb == 2)
+        hipLaunchKernelGGL(k_m2p<2>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
+    else if (kernel_events(c, K_M2P, &e0, &e1))
+        hipExtLaunchKernelGGL(k_m2p<1>, grid, dim3(256), 0, c->stream, e0, e1, 0, g, xp, qp, qstride, itype, vel, sol,
+                              up, gp, n);
     else
-        hipLaunchKernelGGL(k_m2p, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, xp, qp, qstride,
-                           itype, vel, sol, up, gp, n);
+        hipLaunchKernelGGL(k_m2p<1>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

@@ -218,7 +218,8 @@ static int check_ready(Ctx *c, int neqpm) {
 
 // Projection + yaps solve (+ velocity/stretching + interpolation for mode 2) on the
 // particles (xp, qp) already on the device.
-static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, double *up, double *gp, int64_t n) {
+static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, double *up, double *gp, int64_t n,
+                   double ni = 0.0) {
     int st = VPMX_OK;
     VPMX_TRY(check_ready(c, c->neqpm));
     if (n == 0) return VPMX_OK;  // vpm.f90:172
@@ -240,6 +241,31 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
                         K_P2M));
     mark(c, ST_PROJ, 1);
     used[ST_PROJ] = true;
+    if (whattodo == 4 || whattodo == 5) {
+        // no Poisson solve (vpm.f90:316-373): mode 4 stretching from the velocity grids of the
+        // last mode 0/1/5 call, mode 5 diffusion of the projected vorticity
+        const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+        const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+        const bool fresh = c->vel.bytes < sizeof(double) * 3 * (size_t)GG;
+        double *vel = (double *)ensure(c, c->vel, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+        double *str = (double *)ensure(c, c->str, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+        if (fresh || whattodo == 5) VPMX_TRY(fill(c, vel, 3 * GG, 0.0));  // velvr*_pm = 0
+        const int32_t *w = G.nnbl_bl;
+        GView rg = view(rhs_bl, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+        mark(c, ST_FD, 0);
+        if (whattodo == 4) VPMX_TRY(stretch_kernel(c, gn, G.dpm, rg, view(vel, gn), view(str, gn)));
+        else VPMX_TRY(diffuse_kernel(c, gn, G.dpm, ni, rg, view(str, gn)));
+        mark(c, ST_FD, 1);
+        used[ST_FD] = true;
+        mark(c, ST_BACK, 0);
+        VPMX_TRY(m2p_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, P.interf_iproj, view(vel, gn), view(str, gn), up, gp,
+                            n, whattodo == 5 ? 2 : 1));
+        mark(c, ST_BACK, 1);
+        used[ST_BACK] = true;
+        c->have_grids = true;
+        collect_times(c, used);
+        return VPMX_OK;
+    }
     VPMX_TRY(yaps_single(c, used));
     c->have_grids = true;
     if (whattodo == 3) { collect_times(c, used); return VPMX_OK; }
@@ -432,23 +458,26 @@ int32_t vpmx_set_precision(vpmx_ctx *p, int32_t mode) {
 
 int32_t vpmx_vpm(vpmx_ctx *p, double *xp, double *qp, double *up, double *gp, int64_t nvr, int32_t neqpm,
                  int32_t whattodo, int32_t ntime, double ni) {
-    (void)ni;
     Ctx *c = C(p);
     if (!c) return VPMX_ERR_ARG;
     VPMX_HIP(c, hipSetDevice(c->device));
     if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
+    c->ni = ni;
     if (whattodo == 0) {
         int r = vpmx_define_sizes(&c->prm, &c->geom, xp, nvr, ntime);
         if (r) return fail(c, r, "define_sizes failed");
         c->have_grids = false;
+        // velvr*_pm = 0 after WhatToDo = 0 (vpm.f90:154-166)
+        if (c->vel.p) VPMX_TRY(fill(c, (double *)c->vel.p, (int64_t)(c->vel.bytes / sizeof(double)), 0.0));
+        VPMX_HIP(c, hipStreamSynchronize(c->stream));
         return VPMX_OK;
     }
-    if (whattodo < 1 || whattodo > 3) return fail(c, VPMX_ERR_UNSUP, "WhatToDo 4 and 5 are not supported");
+    if (whattodo < 1 || whattodo > 5) return fail(c, VPMX_ERR_ARG, "WhatToDo must be 0..5");
     if (nvr == 0) return VPMX_OK;
     VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
     VPMX_TRY(run_vpm(c, whattodo, (const double *)c->xp.p, (const double *)c->qp.p, (double *)c->up.p,
-                     (double *)c->gp.p, nvr));
-    if (whattodo == 2) {
+                     (double *)c->gp.p, nvr, ni));
+    if (whattodo == 2 || whattodo == 4 || whattodo == 5) {
         if (up) VPMX_HIP(c, hipMemcpyAsync(up, c->up.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
         if (gp) VPMX_HIP(c, hipMemcpyAsync(gp, c->gp.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
     }
@@ -557,11 +586,13 @@ int32_t vpmx_vpm_resident(vpmx_ctx *p, int32_t whattodo, int32_t ntime) {
     if (whattodo == 0) {
         VPMX_TRY(define_from_device(c, ntime));
         c->have_grids = false;
+        if (c->vel.p) VPMX_TRY(fill(c, (double *)c->vel.p, (int64_t)(c->vel.bytes / sizeof(double)), 0.0));
+        VPMX_HIP(c, hipStreamSynchronize(c->stream));
         return VPMX_OK;
     }
-    if (whattodo < 1 || whattodo > 3) return fail(c, VPMX_ERR_UNSUP, "WhatToDo 4 and 5 are not supported");
+    if (whattodo < 1 || whattodo > 5) return fail(c, VPMX_ERR_ARG, "WhatToDo must be 0..5");
     VPMX_TRY(run_vpm(c, whattodo, (const double *)c->xp.p, (const double *)c->qp.p, (double *)c->up.p,
-                     (double *)c->gp.p, c->np));
+                     (double *)c->gp.p, c->np, c->ni));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));
     return VPMX_OK;
 }

@@ -40,6 +40,7 @@ struct Ctx : vpmx_ctx {
     DBuf xp, qp, up, gp, x0, q0;
     int64_t np = 0;
     int neqpm = 3;
+    double ni = 0.0;  // kinematic viscosity NI of the diffusion mode (resident calls)
 
     // grids
     DBuf rhs_bl, sol_bl, sol0, vel, str;

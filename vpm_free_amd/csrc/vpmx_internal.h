@@ -66,8 +66,9 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
 // outside the window are written 0.
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
                int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass = -1);
+// itypeb 1: UP = sum W vel, GP = (sum W sol) QP(4); itypeb 2 (diffusion): UP = 0, GP as above
 int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
-               int itype, GView vel, GView sol, double *up, double *gp, int64_t n);
+               int itype, GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb = 1);
 int euler_update(Ctx *c, double *xp, double *qp, int qstride, const double *up, const double *gp, int64_t n,
                  double dt, const double uinf[3], const double *x0, const double *q0);
 int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3]);
@@ -106,6 +107,7 @@ void free_bspec(Ctx *c);
 int fill(Ctx *c, double *p, int64_t n, double v);
 int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel);
 int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out);
+int diffuse_kernel(Ctx *c, const int n[3], const double h[3], double vis, GView rhs, GView out);
 int coarse_sample(Ctx *c, int ncomp, const int nsamp[3], int nc, const int *cmap_dev, GView fine, GView coarse,
                   const int cn[3]);
 int coarse_laplacian(Ctx *c, int ncomp, const int cn[3], const int lo[3], const int hi[3], const double hc[3],

@@ -320,6 +320,36 @@ contains
             endif
             return
         endif
+        if (WhatToDo == 4 .or. WhatToDo == 5) then
+            ! 4: stretching from the velocity grids of the last 0/1/5 call (vpm.f90:316-343);
+            ! 5: diffusion, UP = 0, GP = interpolated -NI lap(w) * QP(4) (vpm.f90:345-373)
+            call vpmx_check(vpmx_vpm(vpmx_ctx_h, c_loc(XP_in), c_loc(QP_in), c_loc(UP_in), c_loc(GP_in), &
+                            int(NVR_in, c_int64_t), int(neqpm, c_int32_t), int(WhatToDo, c_int32_t), &
+                            int(NTIME_pm, c_int32_t), NI), 'vpm')
+            if (WhatToDo == 4) then
+                if (IPMWRITE > 0) then
+                    do i = 1, IPMWRITE
+                        if (NTIME_pm >= IPMWSTART(i) .and. NTIME_pm <= (IPMWSTART(i) + IPMWSTEPS(i))) then
+                            call fetch_rhs()
+                            call writesol
+                        endif
+                    enddo
+                endif
+                if (iynslice /= 0) then
+                    call fetch_rhs()
+                    call writesolXavatar(iynslice)
+                endif
+            else
+                if (allocated(velvrx_pm)) deallocate(velvrx_pm, velvry_pm, velvrz_pm)
+                allocate(velvrx_pm(NXpm,NYpm,NZpm), velvry_pm(NXpm,NYpm,NZpm), velvrz_pm(NXpm,NYpm,NZpm))
+                velvrx_pm = 0.d0; velvry_pm = 0.d0; velvrz_pm = 0.d0
+                nullify(velx); nullify(vely); nullify(velz)
+                velx => velvrx_pm; vely => velvry_pm; velz => velvrz_pm
+            endif
+            iwrite = 0
+            if (allocated(RHS_pm)) deallocate(RHS_pm)
+            return
+        endif
         write(*,*) 'vpm_lib: WhatToDo = ', WhatToDo, ' is not supported by this build'
         error stop 'vpm_lib: WhatToDo'
     end subroutine vpm
