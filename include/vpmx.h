@@ -144,6 +144,11 @@ int32_t vpmx_stage_solve(vpmx_ctx *ctx, const double xbound[6], const double dpm
  * the Dirichlet solve over the whole grid. */
 int32_t vpmx_stage_pmesh_inf(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
                              int32_t nc, const double *rhs, double *sol);
+/* The same with pmesh's itree / levmax arguments (pmlib.f90:69): itree = 1 takes the
+ * boundary sums from the face-cell tree of build_level_nbound_3d + tree_calc_3d
+ * (pinfdomain.f90:875-1530, pmbound.f90:980-1250) with levmax levels. */
+int32_t vpmx_stage_pmesh_inf_lev(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                                 int32_t nc, const double *rhs, double *sol, int32_t itree, int32_t levmax);
 /* project_particles_3D + project_vol3d (pmproject.f90:43-153) on grid (nn,xbound,dpm). */
 int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
                            const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr,

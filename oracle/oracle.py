@@ -93,6 +93,9 @@ def lib():
             "orc_timers": (None, [D, C.c_int]),
             "orc_set_boundsum_stride": (None, [C.c_int]),
             "orc_diffuse": (None, [C.POINTER(Geom), C.c_int, C.c_double, D, D]),
+            "orc_set_tree": (None, [C.c_int]),
+            "orc_set_tree_factor": (None, [C.c_double]),
+            "orc_boundsum_tree": (None, [D, D, I32P, C.c_int, i64, D, D, D, D, C.c_int]),
             "orc_vpm45": (i32, [C.POINTER(Params), C.POINTER(Geom), C.c_int, C.c_int, C.c_double, D, D, D, D, i64,
                                 D, D]),
         }
@@ -236,6 +239,25 @@ def vpm2(params, geom, xp, qp, neqpm=3, fields=False):
     if fields:
         return up, gp, {"rhs": rhs, "psi": psi, "sol": sol, "vel": vel}
     return up, gp
+
+
+def set_tree(levmax):
+    """Boundary sums of pmesh_inf by the tree with this levmax (>= 0), or direct (-1)."""
+    lib().orc_set_tree(int(levmax))
+
+
+def set_tree_factor(f):
+    lib().orc_set_tree_factor(float(f))
+
+
+def boundsum_tree(xbound, dpm, nn, src, cxyz, ds, sol, levmax):
+    xb, dp = _f(xbound), _f(dpm)
+    nn = np.asarray(nn, np.int32)
+    src, cxyz, ds = _f(src), _f(cxyz), _f(ds)
+    out = np.array(sol, dtype=np.float64, order="C")
+    lib().orc_boundsum_tree(_d(xb), _d(dp), _i(nn), src.shape[1], src.shape[0], _d(src), _d(cxyz), _d(ds), _d(out),
+                            int(levmax))
+    return out
 
 
 def diffuse(geom, ni, rhs):

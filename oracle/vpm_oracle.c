@@ -588,9 +588,212 @@ void orc_boundsum(const double xb[6], const double dpm[3], const int32_t nn[3], 
     g_timer[2] += now() - t0;
 }
 
+/* ------------------------------------------------------------- tree boundary */
+/* build_level_nbound_3d (pinfdomain.f90:875-1530): per face a hierarchy of cell groups,
+ * level lev grouping 2^lev x 2^lev level-0 cells (centroid = mean of the level-0 centroids,
+ * area = sum, source = area-weighted mean); groups that do not fit the face are "dummy"
+ * cells that only point at their two children.  Arrays are 1-based like the reference:
+ * entry [lev][i], i = 1..nbound_lev[lev]. */
+typedef struct {
+    int levmax;
+    int nlev[16];
+    double *xs[16], *ys[16], *zs[16], *dsl[16], *src[16];  /* src: nc per cell */
+    int32_t *t[16];                                           /* ilev_t: 6 per cell (1..6) */
+    int nc;
+} orc_tree;
+
+#define TL(lv, i, m) tr->t[lv][6 * (size_t)(i) + (m) - 1]
+
+/* Refinement factor of the descent (10 in the reference); tests raise it to reach every leaf. */
+static double g_tree_factor = 10.0;
+void orc_set_tree_factor(double f) { g_tree_factor = f; }
+
+static void tree_face(orc_tree *tr, int lev_unused, const double *xs_tmp, const double *ds_tmp, const double *s_tmp,
+                      int S1, int F1, int S2, int F2, int NNrow, int d1, int nn_lev[3][16], int *nbound_lev) {
+    (void)lev_unused;
+    int nc = tr->nc;
+    int ncount[16];
+    for (int lev = 0; lev <= tr->levmax; lev++) {
+        int istep = 1 << lev;
+        int n2s = S2, n2f = F2 - istep, n1s = S1, n1f = F1 - istep;
+        int icount = nbound_lev[lev];
+        int lf = nbound_lev[0];
+        for (int k2 = n2s; k2 <= n2f; k2 += istep) {
+            for (int k1 = n1s; k1 <= n1f; k1 += istep) {
+                double xc = 0, yc = 0, zc = 0, sc = 0, sourcec[8] = {0};
+                for (int mm = 1; mm <= istep; mm++)
+                    for (int nn = 1; nn <= istep; nn++) {
+                        int leaf = lf + (k2 - S2) * NNrow + (k1 - S1) + (mm - 1) * NNrow + nn; /* 1-based */
+                        double x = xs_tmp[3 * (size_t)(leaf - 1)], y = xs_tmp[3 * (size_t)(leaf - 1) + 1],
+                               z = xs_tmp[3 * (size_t)(leaf - 1) + 2], sv = ds_tmp[leaf - 1];
+                        xc = x + xc; yc = y + yc; zc = z + zc; sc = sv + sc;
+                        for (int c = 0; c < nc; c++) sourcec[c] = s_tmp[(size_t)nc * (leaf - 1) + c] * sv + sourcec[c];
+                    }
+                icount++;
+                double fl = (double)(float)(istep * istep);
+                tr->xs[lev][icount] = xc / fl;
+                tr->ys[lev][icount] = yc / fl;
+                tr->zs[lev][icount] = zc / fl;
+                tr->dsl[lev][icount] = sc;
+                for (int c = 0; c < nc; c++) tr->src[lev][(size_t)nc * icount + c] = sourcec[c] / sc;
+                int npre = nn_lev[d1][lev];
+                if (lev > 0) {
+                    int nnl = nn_lev[d1][lev - 1], nj;
+                    if (k1 == n1s && icount - 1 == nbound_lev[lev]) nj = nbound_lev[lev - 1] + 1;
+                    else if (k1 == n1s && icount - 1 > nbound_lev[lev]) {
+                        nj = TL(lev, icount - 1, 3) + 1;
+                        if (TL(lev, icount - 1, 6) < 0) nj = TL(lev, icount - 1, 2) + 1;
+                    } else nj = TL(lev, icount - 1, 2) + 1;
+                    TL(lev, icount, 1) = nj;
+                    TL(lev, icount, 2) = nj + 1;
+                    TL(lev, icount, 3) = nj + nnl + 1;
+                    TL(lev, icount, 4) = nj + nnl;
+                    TL(lev, icount, 5) = 4;
+                }
+                TL(lev, icount, 6) = npre;
+            }
+            if ((F1 - S1) % istep != 0 || F1 - S1 <= istep) {
+                if (lev == 0) { fprintf(stderr, "tree: level 0 dummy\n"); abort(); }
+                icount++;
+                int nj = TL(lev, icount - 1, 2) + 1, nnl = nn_lev[d1][lev - 1];
+                TL(lev, icount, 1) = nj;
+                TL(lev, icount, 2) = nj + nnl;
+                TL(lev, icount, 5) = 2;
+                TL(lev, icount, 6) = -1;
+            }
+        }
+        if ((F2 - S2) % istep != 0 || F2 - S2 <= istep) {
+            if (lev == 0) { fprintf(stderr, "tree: level 0 dummy\n"); abort(); }
+            for (int k1 = n1s; k1 <= n1f; k1 += istep) {
+                if (k1 == n1f && (F1 - S1) % istep != 0) continue;
+                icount++;
+                int nj;
+                if (k1 == n1s && TL(lev, icount - 1, 6) > 0) nj = TL(lev, icount - 1, 3) + 1;
+                else nj = TL(lev, icount - 1, 2) + 1;
+                TL(lev, icount, 1) = nj;
+                TL(lev, icount, 2) = nj + 1;
+                TL(lev, icount, 5) = 2;
+                TL(lev, icount, 6) = -2;
+            }
+        }
+        ncount[lev] = icount;
+    }
+    for (int lev = 0; lev <= tr->levmax; lev++) nbound_lev[lev] = ncount[lev];
+}
+
+/* Sources: nb face cells in the reference face order (src nc per cell, cxyz centroids, ds).
+ * Node extents of the grid nn (NXs = 1, NXf = NX, ...). */
+static void tree_build(orc_tree *tr, int levmax, const int32_t nn[3], int nc, int64_t nb, const double *src,
+                       const double *cxyz, const double *ds) {
+    memset(tr, 0, sizeof *tr);
+    tr->levmax = levmax;
+    tr->nc = nc;
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    int NNX = NX - 1, NNY = NY - 1, NNZ = NZ - 1;
+    int nn_lev[3][16];
+    for (int lev = 0; lev <= levmax; lev++) {
+        int istep = 1 << lev;
+        nn_lev[0][lev] = NNX / istep + (NNX % istep != 0);
+        nn_lev[1][lev] = NNY / istep + (NNY % istep != 0);
+        nn_lev[2][lev] = NNZ / istep + (NNZ % istep != 0);
+        tr->xs[lev] = calloc((size_t)nb + 2, sizeof(double));
+        tr->ys[lev] = calloc((size_t)nb + 2, sizeof(double));
+        tr->zs[lev] = calloc((size_t)nb + 2, sizeof(double));
+        tr->dsl[lev] = calloc((size_t)nb + 2, sizeof(double));
+        tr->src[lev] = calloc(((size_t)nb + 2) * nc, sizeof(double));
+        tr->t[lev] = calloc(((size_t)nb + 2) * 6, sizeof(int32_t));
+    }
+    int nbl[16] = {0};
+    /* XMIN, XMAX: outer k, inner j, rows of NNY cells; YMIN, YMAX and ZMIN, ZMAX: inner i */
+    tree_face(tr, 0, cxyz, ds, src, 1, NY, 1, NZ, NNY, 1, nn_lev, nbl);
+    tree_face(tr, 0, cxyz, ds, src, 1, NY, 1, NZ, NNY, 1, nn_lev, nbl);
+    tree_face(tr, 0, cxyz, ds, src, 1, NX, 1, NZ, NNX, 0, nn_lev, nbl);
+    tree_face(tr, 0, cxyz, ds, src, 1, NX, 1, NZ, NNX, 0, nn_lev, nbl);
+    tree_face(tr, 0, cxyz, ds, src, 1, NX, 1, NY, NNX, 0, nn_lev, nbl);
+    tree_face(tr, 0, cxyz, ds, src, 1, NX, 1, NY, NNX, 0, nn_lev, nbl);
+    if (nbl[0] != nb) { fprintf(stderr, "tree: nbound_lev problem %d %lld\n", nbl[0], (long long)nb); abort(); }
+    for (int lev = 0; lev <= levmax; lev++) tr->nlev[lev] = nbl[lev];
+}
+
+static void tree_free(orc_tree *tr) {
+    for (int lev = 0; lev <= tr->levmax; lev++) {
+        free(tr->xs[lev]); free(tr->ys[lev]); free(tr->zs[lev]); free(tr->dsl[lev]); free(tr->src[lev]);
+        free(tr->t[lev]);
+    }
+}
+
+/* tree_calc_3d (pmbound.f90:1139-1237): SOURCE -= s DS / r over the cells reached by the
+ * descent (a cell is refined when r < 10 sqrt(DS) above level 0, and dummies always). */
+static void tree_calc(const orc_tree *tr, int nv, int nlev, double X, double Y, double Z, double *S) {
+    int list[4], nleaf_n, use_list;
+    if (nlev == tr->levmax) { nleaf_n = tr->nlev[nlev]; use_list = 0; }
+    else if (TL(nlev + 1, nv, 6) < 0) { list[0] = TL(nlev + 1, nv, 1); list[1] = TL(nlev + 1, nv, 2); nleaf_n = 2; use_list = 1; }
+    else { for (int q = 0; q < 4; q++) list[q] = TL(nlev + 1, nv, q + 1); nleaf_n = 4; use_list = 1; }
+    const double lg = log((sqrt(2.0) - 1) / (sqrt(2.0) + 1));
+    for (int q = 0; q < nleaf_n; q++) {
+        int nleaf = use_list ? list[q] : q + 1;
+        if (TL(nlev, nleaf, 6) == 0) { fprintf(stderr, "Something is wrong with the tree %d %d\n", nleaf, nlev); abort(); }
+        double XR = tr->xs[nlev][nleaf] - X, YR = tr->ys[nlev][nleaf] - Y, ZR = tr->zs[nlev][nleaf] - Z;
+        double r = sqrt(XR * XR + YR * YR + ZR * ZR);
+        double DS = tr->dsl[nlev][nleaf];
+        if ((r < g_tree_factor * sqrt(DS) && nlev > 0) || TL(nlev, nleaf, 6) < 0) {
+            tree_calc(tr, nleaf, nlev - 1, X, Y, Z, S);
+        } else {
+            const double *sb = tr->src[nlev] + (size_t)tr->nc * nleaf;
+            if (r > 1e-05) {
+                for (int c = 0; c < tr->nc; c++) S[c] = S[c] - sb[c] * DS / r;
+            } else {
+                for (int c = 0; c < tr->nc; c++) S[c] = S[c] + sb[c] * 2 * sqrt(DS) * lg;
+            }
+        }
+    }
+}
+#undef TL
+
+/* Bounds3d_lev -> calc_boundinf_3d_lev (pmbound.f90:980-1137), ibctyp=2: targets in the
+ * order of the direct sums; SOL += SOURCE / (4 pi). */
+void orc_boundsum_tree(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                       const double *src, const double *cxyz, const double *ds, double *sol, int levmax) {
+    orc_tree tr;
+    tree_build(&tr, levmax, nn, nc, nbound, src, cxyz, ds);
+    const double PI4 = 4.0 * (4.0 * atan(1.0));
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    int64_t nt = 2LL * NY * NZ + 2LL * (NX - 2) * NZ + 2LL * (NX - 2) * (NY - 2);
+    int32_t *tg = malloc(sizeof(int32_t) * 3 * nt);
+    int64_t t = 0;
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int j = 1; j <= NY; j++) { tg[3*t] = f ? NX : 1; tg[3*t+1] = j; tg[3*t+2] = k; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int i = 2; i <= NX - 1; i++) { tg[3*t] = i; tg[3*t+1] = f ? NY : 1; tg[3*t+2] = k; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int j = 2; j <= NY - 1; j++)
+            for (int i = 2; i <= NX - 1; i++) { tg[3*t] = i; tg[3*t+1] = j; tg[3*t+2] = f ? NZ : 1; t++; }
+    double t0 = now();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < nt; q++) {
+        int i = tg[3 * q], j = tg[3 * q + 1], k = tg[3 * q + 2];
+        double X = xb[0] + (double)(i - 1) * dpm[0], Y = xb[1] + (double)(j - 1) * dpm[1],
+               Z = xb[2] + (double)(k - 1) * dpm[2];
+        double S[8] = {0};
+        tree_calc(&tr, 1, levmax, X, Y, Z, S);
+        for (int c = 0; c < nc; c++) {
+            size_t o = IX(c, i - 1, j - 1, k - 1, nc, NX, NY);
+            sol[o] = sol[o] + S[c] / PI4;
+        }
+    }
+    g_timer[2] += now() - t0;
+    free(tg);
+    tree_free(&tr);
+}
+
 /* ------------------------------------------------------------------ pmesh */
 /* pmlib.f90:69-192 with iynbc=1, ibctyp=2, itree=0: infdomain_3d (pinfdomain.f90:86-158)
  * then solve_eq_3d on the whole grid.  sol accumulates the boundary sums. */
+static int g_tree_levmax = -1; /* >= 0: boundary sums by the tree with this levmax */
+void orc_set_tree(int levmax) { g_tree_levmax = levmax; }
+
 void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, const double *rhs,
                    double *sol) {
     size_t G = (size_t)nn[0] * nn[1] * nn[2];
@@ -602,7 +805,8 @@ void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3],
            *ds = malloc(sizeof(double) * maxb);
     int64_t nb = orc_normalderiv(xb, dpm, nn, nc, s0, src, cx, ds);
     free(s0);
-    orc_boundsum(xb, dpm, nn, nc, nb, src, cx, ds, sol);
+    if (g_tree_levmax >= 0) orc_boundsum_tree(xb, dpm, nn, nc, nb, src, cx, ds, sol, g_tree_levmax);
+    else orc_boundsum(xb, dpm, nn, nc, nb, src, cx, ds, sol);
     free(src); free(cx); free(ds);
     for (int c = 0; c < nc; c++) orc_solve_window(xb, dpm, nn, nc, c, win, rhs, sol, 0, NULL);
 }
@@ -615,7 +819,10 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
     const int32_t *nnbl = &g->nnbl[0], *nnbl_bl = &g->nnbl_bl[0];
     size_t B = (size_t)nnbl[0] * nnbl[1] * nnbl[2];
     memset(sol_bl, 0, sizeof(double) * B * nc);
-    /* 1. fine infinite-domain solve on the whole block */
+    /* 1. fine infinite-domain solve on the whole block (tree with lmax = ilevmax when
+     *    iyntree = 1, yaps3d.f90:47-48) */
+    const int saved_tree = g_tree_levmax;
+    if (p->iyntree == 1) g_tree_levmax = g->ilevmax;
     orc_pmesh_inf(xbl, g->dpm, nnbl, nc, rhs_bl, sol_bl);
     /* 2. sample the fine solution on the coarse grid (yaps3d.f90:104-138) */
     int ncx = g->nn_coarse[0], ncy = g->nn_coarse[1], ncz = g->nn_coarse[2];
@@ -660,7 +867,9 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
     /* 4. coarse infinite-domain solve */
     double xbc[6];
     memcpy(xbc, g->xbound_coarse, sizeof xbc);
+    if (p->iyntree == 1) g_tree_levmax = g->ilevmax - 1;  /* yaps3d.f90:288-294 */
     orc_pmesh_inf(xbc, g->dpm_coarse, g->nn_coarse, nc, rhs_c, sol_c);
+    g_tree_levmax = saved_tree;
     /* 5. interp_stencil on the 6 faces of the inner box (yaps3d.f90:340-395, 1914-1983) */
     int NXs = nnbl_bl[0], NYs = nnbl_bl[1], NZs = nnbl_bl[2], NXf = nnbl_bl[3], NYf = nnbl_bl[4], NZf = nnbl_bl[5];
     int64_t nface = 2LL * (NYf - NYs + 1) * (NZf - NZs + 1) + 2LL * (NXf - NXs - 1) * (NZf - NZs + 1) +

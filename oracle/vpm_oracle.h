@@ -100,6 +100,12 @@ void orc_dst1(int m, const double *in, double *out); /* S_k = sum_j x_j sin(pi j
 int orc_num_threads(void);
 void orc_timers(double *out, int reset);
 void orc_set_boundsum_stride(int s);
+/* Boundary sums of orc_pmesh_inf by the tree (build_level_nbound_3d + tree_calc_3d) with this
+ * levmax (>= 0), or direct (-1). */
+void orc_set_tree(int levmax);
+void orc_set_tree_factor(double f);
+void orc_boundsum_tree(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                       const double *src, const double *cxyz, const double *ds, double *sol, int levmax);
 void orc_diffuse(const orc_geom *g, int neqpm, double ni, const double *rhs, double *sol);
 int orc_vpm45(const orc_params *p, const orc_geom *g, int neqpm, int mode, double ni, const double *xp,
               const double *qp, double *up, double *gp, int64_t nvr, const double *vel, double *sol_out);

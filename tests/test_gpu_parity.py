@@ -201,6 +201,52 @@ def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
     assert rel_l2(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("nn,levmax", [((65, 57, 49), 3), ((43, 43, 43), 2), ((33, 29, 31), 3)])
+def test_boundary_sum_tree(vpmlib, oracle, nn, levmax):
+    """Tree form of pmesh's boundary sums (pm.inp tree = 1) against the oracle's tree:
+    faces of whole groups; dummy rows and columns with the corner cell the links leave
+    out; faces whose finer rows hold two cells beyond the whole groups (shifted links)."""
+    V = vpmlib
+    nn = np.array(nn, np.int32)
+    dpm = np.array([1 / 32, 1 / 30, 1 / 28])
+    xb = np.array([-1.0, -0.95, -0.85, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.02, -0.03, 0.01), s=0.25)
+    sol = np.zeros_like(rhs)
+    say("case: tree boundary sums", nn.tolist(), "levmax", levmax)
+    try:
+        oracle.set_tree(levmax)
+        ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    finally:
+        oracle.set_tree(-1)
+    direct = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    with V.VPM(V.Params.make(0.125)) as c:
+        got = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol, itree=1, levmax=levmax)
+    err = rel_l2(got, ref)
+    say("  rel L2 vs oracle tree %.2e (tree vs direct %.2e)" % (err, rel_l2(ref, direct)))
+    assert rel_l2(ref, direct) > 1e3 * TOL  # the tree's sums are not the direct ones
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("h,sigma", [(0.125, 0.2), (1 / 16, 0.1)])
+def test_vpm_mode2_tree(vpmlib, oracle, h, sigma):
+    """vpm(2) with the shipped pm.inp tree settings (tree = 1, 3 levels): the block solve
+    takes ilevmax levels, the coarse solve ilevmax - 1 (yaps3d.f90:46-48, 286-292)."""
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 tree h=%g n=%d" % (h, xp.shape[0]))
+    po = oracle.Params.make(h, iyntree=1, ilevmax=3)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    with V.VPM(V.Params.make(h, iyntree=1, ilevmax=3)) as c:
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+        errs = {"sol": rel_l2(c.get_grid(2), f["sol"]), "up": rel_l2(up, up_r), "gp": rel_l2(gp, gp_r)}
+    say("  rel L2:", errs)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
 @pytest.mark.parametrize("nn", [(145, 40, 44), (289, 21, 19)])
 def test_solve_many_batches(ctx, oracle, nn):
     """Windows with more line batches than resident work-groups (every work-group loops

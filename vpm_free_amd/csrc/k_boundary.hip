@@ -494,11 +494,17 @@ static int boundary_sums_lattice(Ctx *c, const Grid &g, int64_t ns, const double
 }
 
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *sig, const double *cxyz,
-                  const double *wgt, GView sol, bool fine) {
+                  const double *wgt, GView sol, bool fine, int levmax) {
     if (ncomp != 3) return fail(c, VPMX_ERR_UNSUP, "boundary sums: 3 components");
     const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
     int64_t nt = 2 * ((int64_t)NY * NZ + (int64_t)(NX - 2) * NZ + (int64_t)(NX - 2) * (NY - 2));
     const double *a = wgt, *ds = wgt + 3 * ns;
+    if (levmax >= 0) {  // Bounds3d_lev (infdomain_3d with itree = 1, pinfdomain.f90:128-150)
+        if (fine) mark(c, ST_BOUNDS, 0);
+        VPMX_TRY(boundary_sums_tree(c, g, ns, sig, cxyz, ds, sol, levmax, fine));
+        if (fine) mark(c, ST_BOUNDS, 1);
+        return VPMX_OK;
+    }
     double hmin = g.h[0] < g.h[1] ? g.h[0] : g.h[1];
     if (g.h[2] < hmin) hmin = g.h[2];
     // nearest source-target distance is h_min/sqrt(2) (face-cell centroid vs node)

@@ -90,8 +90,9 @@ static void collect_times(Ctx *c, const bool *) { (void)hipStreamSynchronize(c->
 
 // ---------------------------------------------------------------- pmesh (iynbc=1)
 // pmesh(iynbc=1, ibctyp=2) on grid g: SOL_0 = zero-BC solve, sources from its normal
-// derivatives, SOL(faces) += boundary sums, then the Dirichlet solve of the whole grid.
-static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine) {
+// derivatives, SOL(faces) += boundary sums (tree form with levmax levels when levmax >= 0,
+// pmlib.f90:69 itree/levmax), then the Dirichlet solve of the whole grid.
+static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax) {
     int st = VPMX_OK;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     int64_t G = (int64_t)n[0] * n[1] * n[2];
@@ -108,7 +109,7 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     double *wgt = (double *)ensure(c, c->src_w, sizeof(double) * 4 * (size_t)nbmax, &st); if (st) return st;
     int64_t nb = 0;
     VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
-    VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine));
+    VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax));
     VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol));
     return VPMX_OK;
 }
@@ -128,7 +129,11 @@ static int yaps_single(Ctx *c, bool *used) {
     mark(c, ST_FINE, 0);
     VPMX_TRY(fill(c, sol_bl, 3 * B, 0.0));
     c->dst_timed_m = gb.n[1] - 2;  // the block's y lines are the timed sine-transform passes
-    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true));
+    // tree levels: ilevmax on the block, ilevmax-1 on the coarse grid (yaps3d.f90:46-48, 286-292)
+    const bool tree = P.iyntree == 1;
+    if (tree && G.ilevmax < 1)
+        return fail(c, VPMX_ERR_UNSUP, "tree = 1 needs at least one level on the coarse grid (ilevmax >= 1)");
+    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true, tree ? G.ilevmax : -1));
     mark(c, ST_FINE, 1);
     used[ST_FINE] = used[ST_BOUNDS] = true;
     // 2. coarse correction
@@ -164,7 +169,7 @@ static int yaps_single(Ctx *c, bool *used) {
     VPMX_TRY(coarse_laplacian(c, 3, cn, fp_lo, fp_hi, G.dpm_coarse, smv, rcv));
     VPMX_TRY(fill(c, solc, 3 * GC, 0.0));
     Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
-    VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false));
+    VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false, tree ? G.ilevmax - 1 : -1));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));  // cmap host vector goes out of scope
     mark(c, ST_COARSE, 1);
     used[ST_COARSE] = true;
@@ -212,7 +217,6 @@ static int check_ready(Ctx *c, int neqpm) {
         return fail(c, VPMX_ERR_UNSUP, "IDVPM=0 (variable volume) is not supported by this build");
     }
     if (P.ibctyp != 2) return fail(c, VPMX_ERR_UNSUP, "only ibctyp=2 (boundary-source Green sums) is supported");
-    if (P.iyntree != 0) return fail(c, VPMX_ERR_UNSUP, "iyntree=1 (tree boundary sums) is not supported; set 0");
     return VPMX_OK;
 }
 
@@ -422,6 +426,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sort.xs, &c->sort.qs};
     for (DBuf *b : bufs) release(*b);
     free_bspec(c);
+    free_tree(c);
     free_fft_plans(c);
     for (int k = 0; k < K_COUNT; k++) {
         for (hipEvent_t e : c->kev0[k]) (void)hipEventDestroy(e);
@@ -730,6 +735,11 @@ int32_t vpmx_stage_solve(vpmx_ctx *p, const double xbound[6], const double dpm[3
 
 int32_t vpmx_stage_pmesh_inf(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
                              int32_t nc, const double *rhs, double *sol) {
+    return vpmx_stage_pmesh_inf_lev(p, xbound, dpm, nn, nc, rhs, sol, 0, 1);
+}
+
+int32_t vpmx_stage_pmesh_inf_lev(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
+                                 int32_t nc, const double *rhs, double *sol, int32_t itree, int32_t levmax) {
     Ctx *c = C(p);
     if (!c || !xbound || !dpm || !nn || !rhs || !sol) return VPMX_ERR_ARG;
     if (nc != 3) return fail(c, VPMX_ERR_ARG, "nc must be 3");
@@ -745,7 +755,7 @@ int32_t vpmx_stage_pmesh_inf(vpmx_ctx *p, const double xbound[6], const double d
     VPMX_HIP(c, hipMemcpyAsync(ra, sol, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
     VPMX_TRY(aos_to_soa(c, nc, n, ra, view(ss, n)));
     Grid g = make_grid(nn, xbound, dpm);
-    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, true));
+    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, true, itree == 1 ? levmax : -1));
     VPMX_TRY(soa_to_aos(c, nc, nc, n, view(ss, n), 0.0, ra));
     VPMX_HIP(c, hipMemcpyAsync(sol, ra, sizeof(double) * nc * G, hipMemcpyDeviceToHost, c->stream));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));

@@ -48,6 +48,7 @@ struct Ctx : vpmx_ctx {
     int bs_mode = 2;  // fine boundary sums: 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx = nullptr;   // spectral boundary-sum plan of the current block geometry
     void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
+    void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
     int num_cu = 256;       // compute units of the device
     int dst_wg_per_cu = 0;  // resident work-groups per CU of the persistent DST passes (0: occupancy query)
     int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length

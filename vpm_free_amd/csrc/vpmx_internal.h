@@ -96,12 +96,18 @@ int fft_shape(Ctx *c, int L, FftShape *out);
 // Boundary (k_boundary.hip)
 int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt,
                   int64_t *nbound);
+// levmax >= 0: the tree form with that many levels (pm.inp tree = 1), else the direct sums.
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t nbound, const double *sig, const double *cxyz,
-                  const double *wgt, GView sol, bool fine);
+                  const double *wgt, GView sol, bool fine, int levmax = -1);
 // Spectral evaluation of the same sums (k_bspec.hip); returns VPMX_ERR_UNSUP when the
 // block's faces are too long for its in-LDS FFTs, so the caller can use the lattice kernel.
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol);
 void free_bspec(Ctx *c);
+// Tree form (k_tree.hip): sig (3 x nbound), cxyz (3 x nbound), ds (nbound) of normal_derivs.
+int boundary_sums_tree(Ctx *c, const Grid &g, int64_t nbound, const double *sig, const double *cxyz, const double *ds,
+                       GView sol, int levmax, bool fine);
+int64_t tree_entries(Ctx *c, const int n[3], int levmax);
+void free_tree(Ctx *c);
 
 // Grid kernels (k_grid.hip)
 int fill(Ctx *c, double *p, int64_t n, double v);
