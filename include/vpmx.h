@@ -42,7 +42,7 @@ enum {
 typedef struct {
     double dpm[3];                 /* DXpm DYpm DZpm                  */
     int32_t interf_iproj;          /* projection function 2..6 (4=M4')*/
-    int32_t ibctyp;                /* 2 = boundary-source Green sum    */
+    int32_t ibctyp;                /* 2 point-source, 3 panel Green sums */
     int32_t idvpm;                 /* 1 constant volume, 0 variable     */
     double epsvol;
     int32_t ncoarse;
@@ -144,11 +144,14 @@ int32_t vpmx_stage_solve(vpmx_ctx *ctx, const double xbound[6], const double dpm
  * the Dirichlet solve over the whole grid. */
 int32_t vpmx_stage_pmesh_inf(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
                              int32_t nc, const double *rhs, double *sol);
-/* The same with pmesh's itree / levmax arguments (pmlib.f90:69): itree = 1 takes the
- * boundary sums from the face-cell tree of build_level_nbound_3d + tree_calc_3d
- * (pinfdomain.f90:875-1530, pmbound.f90:980-1250) with levmax levels. */
+/* The same with pmesh's ibctyp / itree / levmax arguments (pmlib.f90:69): ibctyp 2 point
+ * sources (calc_boundinf_3d), 3 constant-source panels (calc_boundinf_3d_s, FSOUR_A4,
+ * pmbound.f90:520-648, 1541-1632); itree = 1 takes the sums from the face-cell tree of
+ * build_level_nbound_3d + tree_calc_3d / tree_calc_3d_s (pinfdomain.f90:875-1530,
+ * pmbound.f90:980-1481) with levmax levels. */
 int32_t vpmx_stage_pmesh_inf_lev(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
-                                 int32_t nc, const double *rhs, double *sol, int32_t itree, int32_t levmax);
+                                 int32_t nc, const double *rhs, double *sol, int32_t ibctyp, int32_t itree,
+                                 int32_t levmax);
 /* project_particles_3D + project_vol3d (pmproject.f90:43-153) on grid (nn,xbound,dpm). */
 int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double dpm[3], const int32_t nn[3],
                            const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr,

@@ -260,6 +260,35 @@ def boundsum_tree(xbound, dpm, nn, src, cxyz, ds, sol, levmax):
     return out
 
 
+def set_ibctyp(ibctyp):
+    """Boundary sums of pmesh_inf with point sources (2) or panels (3, FSOUR_A4)."""
+    lib().orc_set_ibctyp(int(ibctyp))
+
+
+def fsour_a4(xo, rg, iplane, dpm, area=0.0):
+    lib().orc_fsour_a4.restype = C.c_double
+    return lib().orc_fsour_a4(_d(_f(xo)), _d(_f(rg)), int(iplane), _d(_f(dpm)), C.c_double(area))
+
+
+def boundsum_s(xbound, dpm, nn, src, cxyz, sol):
+    xb, dp = _f(xbound), _f(dpm)
+    nn = np.asarray(nn, np.int32)
+    src, cxyz = _f(src), _f(cxyz)
+    out = np.array(sol, dtype=np.float64, order="C")
+    lib().orc_boundsum_s(_d(xb), _d(dp), _i(nn), src.shape[1], src.shape[0], _d(src), _d(cxyz), _d(out))
+    return out
+
+
+def boundsum_tree_s(xbound, dpm, nn, src, cxyz, ds, sol, levmax):
+    xb, dp = _f(xbound), _f(dpm)
+    nn = np.asarray(nn, np.int32)
+    src, cxyz, ds = _f(src), _f(cxyz), _f(ds)
+    out = np.array(sol, dtype=np.float64, order="C")
+    lib().orc_boundsum_tree_s(_d(xb), _d(dp), _i(nn), src.shape[1], src.shape[0], _d(src), _d(cxyz), _d(ds),
+                              _d(out), int(levmax))
+    return out
+
+
 def diffuse(geom, ni, rhs):
     """diffuse_vort_3d on rhs (G x 4, node-major, x fastest); returns SOL (G x 3)."""
     rhs = _f(rhs)

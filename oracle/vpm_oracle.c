@@ -748,6 +748,131 @@ static void tree_calc(const orc_tree *tr, int nv, int nlev, double X, double Y, 
         }
     }
 }
+
+/* ------------------------------------------------- panel sources (ibctyp = 3) */
+/* Frame of the TARGET plane in calc_boundinf_3d_s / calc_boundinf_3d_lev_s
+ * (pmbound.f90:537-620, 1271-1337): every source panel is taken to lie parallel to the
+ * plane being evaluated.  iplane = +-1 Y planes, +-2 X planes, +-3 Z planes, negative on
+ * the max faces (Bounds3d, pmbound.f90:80-157). */
+typedef struct {
+    double E1[3], E2[3], E3[3], S[4], T[4], D[4], SINB[4], COSB[4], DIAG, AREA;
+} panel_frame;
+
+static void panel_frame_init(panel_frame *f, int iplane, const double dpm[3]) {
+    memset(f, 0, sizeof *f);
+    const double si = iplane > 0 ? 1.0 : -1.0;
+    const double DX = dpm[0], DY = dpm[1], DZ = dpm[2];
+    double a = 0, b = 0;
+    f->COSB[1] = 1.0; f->COSB[3] = -1.0;
+    f->SINB[0] = 1.0; f->SINB[2] = -1.0;
+    switch (iplane < 0 ? -iplane : iplane) {
+    case 1: f->DIAG = sqrt(DX * DX + DZ * DZ); f->E1[0] = 1.0 * si; f->E2[2] = 1.0; f->E3[1] = -1.0 * si;
+            f->AREA = DX * DZ; a = DX; b = DZ; break;
+    case 2: f->DIAG = sqrt(DY * DY + DZ * DZ); f->E1[2] = 1.0; f->E2[1] = -1.0 * si; f->E3[0] = -1.0 * si;
+            f->AREA = DY * DZ; a = DY; b = DZ; break;
+    default: f->DIAG = sqrt(DX * DX + DY * DY); f->E1[0] = 1.0; f->E2[1] = -1.0 * si; f->E3[2] = -1.0 * si;
+            f->AREA = DX * DY; a = DX; b = DY; break;
+    }
+    f->S[0] = -0.5 * a; f->S[1] = -0.5 * a; f->S[2] = 0.5 * a; f->S[3] = 0.5 * a;
+    f->T[0] = -0.5 * b; f->T[1] = 0.5 * b; f->T[2] = 0.5 * b; f->T[3] = -0.5 * b;
+    f->D[0] = b; f->D[1] = a; f->D[2] = b; f->D[3] = a;
+}
+
+/* FSOUR_A4 (pmbound.f90:1541-1632) with NSIDE = 0, ISING = 0, EPSS = 1e-14: potential of a
+ * flat constant-source quadrilateral (corners S,T about RG in the frame E1,E2,E3) at XO;
+ * far field (RO/DIAG > 10) -AREA/(4 pi RO). */
+static double fsour_a4(const double XO[3], const double RG[3], const panel_frame *f, double AREA, double PI4) {
+    const double EPSS = 1e-14, TINY = 1e-10;
+    const double dx = XO[0] - RG[0], dy = XO[1] - RG[1], dz = XO[2] - RG[2];
+    double X = dx * f->E1[0] + dy * f->E1[1] + dz * f->E1[2];
+    double Y = dx * f->E2[0] + dy * f->E2[1] + dz * f->E2[2];
+    double Z = dx * f->E3[0] + dy * f->E3[1] + dz * f->E3[2];
+    const double RO = sqrt(X * X + Y * Y + Z * Z);
+    const double RATIO = RO / f->DIAG;
+    double FIS = 0;
+    if (RATIO > 10) {
+        const double AREA1 = AREA / PI4;
+        return -AREA1 / RO;
+    }
+    double UL0 = 0.0, UL1 = 0.0, UL2, FIL = 0.0;
+    double ZP = 1.0;
+    if (Z < 0.0) ZP = -1.0;
+    double AZ = fabs(Z);
+    AZ = AZ > EPSS ? AZ : EPSS;
+    Z = AZ * ZP;
+    const double ZK = Z * Z;
+    double R[4], E[4], H[4];
+    for (int K = 0; K < 4; K++) {
+        const double XK = (X - f->S[K]) * (X - f->S[K]);
+        const double YK = (Y - f->T[K]) * (Y - f->T[K]);
+        H[K] = (X - f->S[K]) * (Y - f->T[K]);
+        R[K] = sqrt(XK + YK + ZK);
+        E[K] = ZK + XK;
+    }
+    for (int K = 0; K < 4; K++) {
+        const int K1 = K, K2 = K == 3 ? 0 : K + 1;
+        const double A2 = R[K1] + R[K2] + f->D[K];
+        const double A1 = R[K1] + R[K2] - f->D[K];
+        if (fabs(A1 * A2) < TINY) continue;
+        const double AK = log(A1 / A2);
+        const double TOO = -(X - f->S[K1]) * f->SINB[K] + (Y - f->T[K1]) * f->COSB[K];
+        UL0 = UL0 + (f->T[K2] - f->T[K1]) * AK / f->D[K];
+        UL1 = UL1 - (f->S[K2] - f->S[K1]) * AK / f->D[K];
+        FIL = FIL - TOO * AK;
+    }
+    (void)UL0; (void)UL1;
+    AZ = AZ > TINY ? AZ : TINY;
+    Z = AZ * ZP;
+    UL2 = 0.0;
+    for (int K = 0; K < 4; K++) {
+        if (fabs(f->COSB[K]) >= TINY) {
+            const int K1 = K, K2 = K == 3 ? 0 : K + 1;
+            UL2 = UL2 + atan((f->SINB[K] * E[K1] - f->COSB[K] * H[K1]) / (Z * f->COSB[K] * R[K1])) -
+                  atan((f->SINB[K] * E[K2] - f->COSB[K] * H[K2]) / (Z * f->COSB[K] * R[K2]));
+        }
+    }
+    FIS = FIS + (FIL + Z * UL2) / PI4;
+    return FIS;
+}
+
+double orc_fsour_a4(const double XO[3], const double RG[3], int iplane, const double dpm[3], double area) {
+    panel_frame f;
+    panel_frame_init(&f, iplane, dpm);
+    const double PI4 = 4.0 * (4.0 * atan(1.0));
+    return fsour_a4(XO, RG, &f, area > 0 ? area : f.AREA, PI4);
+}
+
+/* tree_calc_3d_s (pmbound.f90:1346-1481): refine while r/sqrt(DS) < 10 above level 0, and
+ * always at dummies of kind -1 (dummies of kind -2 are taken as cells of zero area); a cell
+ * taken below the top level adds -s DS / (4 pi r); a top-level cell adds FSOUR_A4 (with its
+ * own area) times the source of cell nv — the first top-level cell, as the reference passes
+ * nv = 1 at the top. */
+static void tree_calc_s(const orc_tree *tr, int nv, int nlev, const double XO[3], const panel_frame *f, double PI4,
+                        double *S) {
+    int list[4], nleaf_n, use_list;
+    if (nlev == tr->levmax) { nleaf_n = tr->nlev[nlev]; use_list = 0; }
+    else if (TL(nlev + 1, nv, 6) < 0) { list[0] = TL(nlev + 1, nv, 1); list[1] = TL(nlev + 1, nv, 2); nleaf_n = 2; use_list = 1; }
+    else { for (int q = 0; q < 4; q++) list[q] = TL(nlev + 1, nv, q + 1); nleaf_n = 4; use_list = 1; }
+    for (int q = 0; q < nleaf_n; q++) {
+        int nleaf = use_list ? list[q] : q + 1;
+        if (TL(nlev, nleaf, 6) == 0) { fprintf(stderr, "Something is wrong with the tree %d %d\n", nleaf, nlev); abort(); }
+        double RG[3] = {tr->xs[nlev][nleaf], tr->ys[nlev][nleaf], tr->zs[nlev][nleaf]};
+        double a = XO[0] - RG[0], b = XO[1] - RG[1], c3 = XO[2] - RG[2];
+        double r = sqrt(a * a + b * b + c3 * c3);
+        double DS = tr->dsl[nlev][nleaf];
+        double RATIO = r / sqrt(DS);
+        if ((RATIO < g_tree_factor && nlev > 0) || TL(nlev, nleaf, 6) == -1) {
+            tree_calc_s(tr, nleaf, nlev - 1, XO, f, PI4, S);
+        } else if (nlev != tr->levmax) {
+            const double *sb = tr->src[nlev] + (size_t)tr->nc * nleaf;
+            for (int c = 0; c < tr->nc; c++) S[c] = S[c] - sb[c] * DS / (PI4 * r);
+        } else {
+            double FIS = fsour_a4(XO, RG, f, DS, PI4);
+            const double *sv = tr->src[nlev] + (size_t)tr->nc * nv;
+            for (int c = 0; c < tr->nc; c++) S[c] = S[c] + sv[c] * FIS;
+        }
+    }
+}
 #undef TL
 
 /* Bounds3d_lev -> calc_boundinf_3d_lev (pmbound.f90:980-1137), ibctyp=2: targets in the
@@ -788,11 +913,89 @@ void orc_boundsum_tree(const double xb[6], const double dpm[3], const int32_t nn
     tree_free(&tr);
 }
 
+
+/* Target nodes of Bounds3d in call order with their plane code (pmbound.f90:80-157):
+ * XMIN (2), XMAX (-2), YMIN (1), YMAX (-1), ZMIN (3), ZMAX (-3). */
+static int64_t bound_targets(const int32_t nn[3], int32_t **out) {
+    int NX = nn[0], NY = nn[1], NZ = nn[2];
+    int64_t nt = 2LL * NY * NZ + 2LL * (NX - 2) * NZ + 2LL * (NX - 2) * (NY - 2);
+    int32_t *tg = malloc(sizeof(int32_t) * 4 * nt);
+    int64_t t = 0;
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int j = 1; j <= NY; j++) { tg[4*t] = f ? NX : 1; tg[4*t+1] = j; tg[4*t+2] = k; tg[4*t+3] = f ? -2 : 2; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int k = 1; k <= NZ; k++)
+            for (int i = 2; i <= NX - 1; i++) { tg[4*t] = i; tg[4*t+1] = f ? NY : 1; tg[4*t+2] = k; tg[4*t+3] = f ? -1 : 1; t++; }
+    for (int f = 0; f < 2; f++)
+        for (int j = 2; j <= NY - 1; j++)
+            for (int i = 2; i <= NX - 1; i++) { tg[4*t] = i; tg[4*t+1] = j; tg[4*t+2] = f ? NZ : 1; tg[4*t+3] = f ? -3 : 3; t++; }
+    *out = tg;
+    return nt;
+}
+
+/* Bounds3d -> calc_boundinf_3d_s (pmbound.f90:520-648), ibctyp = 3: SOL += FIS * sigma over
+ * every source, FIS = FSOUR_A4 in the target plane's frame, sources in order. */
+void orc_boundsum_s(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                    const double *src, const double *cxyz, double *sol) {
+    const double PI4 = 4.0 * (4.0 * atan(1.0));
+    int NX = nn[0], NY = nn[1];
+    int32_t *tg;
+    int64_t nt = bound_targets(nn, &tg);
+    double t0 = now();
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t q = 0; q < nt; q++) {
+        int i = tg[4 * q], j = tg[4 * q + 1], k = tg[4 * q + 2], pl = tg[4 * q + 3];
+        panel_frame f;
+        panel_frame_init(&f, pl, dpm);
+        double XO[3] = {xb[0] + (double)(i - 1) * dpm[0], xb[1] + (double)(j - 1) * dpm[1], xb[2] + (double)(k - 1) * dpm[2]};
+        double acc[8];
+        for (int c = 0; c < nc; c++) acc[c] = sol[IX(c, i - 1, j - 1, k - 1, nc, NX, NY)];
+        for (int64_t nv = 0; nv < nbound; nv++) {
+            double FIS = fsour_a4(XO, &cxyz[3 * nv], &f, f.AREA, PI4);
+            for (int c = 0; c < nc; c++) acc[c] = acc[c] + FIS * src[(size_t)nc * nv + c];
+        }
+        for (int c = 0; c < nc; c++) sol[IX(c, i - 1, j - 1, k - 1, nc, NX, NY)] = acc[c];
+    }
+    g_timer[2] += now() - t0;
+    free(tg);
+}
+
+/* Bounds3d_lev -> calc_boundinf_3d_lev_s (pmbound.f90:1255-1345): SOL += SOURCE. */
+void orc_boundsum_tree_s(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                         const double *src, const double *cxyz, const double *ds, double *sol, int levmax) {
+    orc_tree tr;
+    tree_build(&tr, levmax, nn, nc, nbound, src, cxyz, ds);
+    const double PI4 = 4.0 * (4.0 * atan(1.0));
+    int NX = nn[0], NY = nn[1];
+    int32_t *tg;
+    int64_t nt = bound_targets(nn, &tg);
+    double t0 = now();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < nt; q++) {
+        int i = tg[4 * q], j = tg[4 * q + 1], k = tg[4 * q + 2], pl = tg[4 * q + 3];
+        panel_frame f;
+        panel_frame_init(&f, pl, dpm);
+        double XO[3] = {xb[0] + (double)(i - 1) * dpm[0], xb[1] + (double)(j - 1) * dpm[1], xb[2] + (double)(k - 1) * dpm[2]};
+        double S[8] = {0};
+        tree_calc_s(&tr, 1, levmax, XO, &f, PI4, S);
+        for (int c = 0; c < nc; c++) {
+            size_t o = IX(c, i - 1, j - 1, k - 1, nc, NX, NY);
+            sol[o] = sol[o] + S[c];
+        }
+    }
+    g_timer[2] += now() - t0;
+    free(tg);
+    tree_free(&tr);
+}
+
 /* ------------------------------------------------------------------ pmesh */
 /* pmlib.f90:69-192 with iynbc=1, ibctyp=2, itree=0: infdomain_3d (pinfdomain.f90:86-158)
  * then solve_eq_3d on the whole grid.  sol accumulates the boundary sums. */
 static int g_tree_levmax = -1; /* >= 0: boundary sums by the tree with this levmax */
 void orc_set_tree(int levmax) { g_tree_levmax = levmax; }
+static int g_ibctyp = 2;       /* 2: point sources (calc_boundinf_3d*), 3: panels (*_s) */
+void orc_set_ibctyp(int ibctyp) { g_ibctyp = ibctyp; }
 
 void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, const double *rhs,
                    double *sol) {
@@ -805,7 +1008,10 @@ void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3],
            *ds = malloc(sizeof(double) * maxb);
     int64_t nb = orc_normalderiv(xb, dpm, nn, nc, s0, src, cx, ds);
     free(s0);
-    if (g_tree_levmax >= 0) orc_boundsum_tree(xb, dpm, nn, nc, nb, src, cx, ds, sol, g_tree_levmax);
+    if (g_ibctyp == 3) {
+        if (g_tree_levmax >= 0) orc_boundsum_tree_s(xb, dpm, nn, nc, nb, src, cx, ds, sol, g_tree_levmax);
+        else orc_boundsum_s(xb, dpm, nn, nc, nb, src, cx, sol);
+    } else if (g_tree_levmax >= 0) orc_boundsum_tree(xb, dpm, nn, nc, nb, src, cx, ds, sol, g_tree_levmax);
     else orc_boundsum(xb, dpm, nn, nc, nb, src, cx, ds, sol);
     free(src); free(cx); free(ds);
     for (int c = 0; c < nc; c++) orc_solve_window(xb, dpm, nn, nc, c, win, rhs, sol, 0, NULL);
@@ -821,7 +1027,8 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
     memset(sol_bl, 0, sizeof(double) * B * nc);
     /* 1. fine infinite-domain solve on the whole block (tree with lmax = ilevmax when
      *    iyntree = 1, yaps3d.f90:47-48) */
-    const int saved_tree = g_tree_levmax;
+    const int saved_tree = g_tree_levmax, saved_bc = g_ibctyp;
+    g_ibctyp = p->ibctyp == 3 ? 3 : 2;
     if (p->iyntree == 1) g_tree_levmax = g->ilevmax;
     orc_pmesh_inf(xbl, g->dpm, nnbl, nc, rhs_bl, sol_bl);
     /* 2. sample the fine solution on the coarse grid (yaps3d.f90:104-138) */
@@ -870,6 +1077,7 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
     if (p->iyntree == 1) g_tree_levmax = g->ilevmax - 1;  /* yaps3d.f90:288-294 */
     orc_pmesh_inf(xbc, g->dpm_coarse, g->nn_coarse, nc, rhs_c, sol_c);
     g_tree_levmax = saved_tree;
+    g_ibctyp = saved_bc;
     /* 5. interp_stencil on the 6 faces of the inner box (yaps3d.f90:340-395, 1914-1983) */
     int NXs = nnbl_bl[0], NYs = nnbl_bl[1], NZs = nnbl_bl[2], NXf = nnbl_bl[3], NYf = nnbl_bl[4], NZf = nnbl_bl[5];
     int64_t nface = 2LL * (NYf - NYs + 1) * (NZf - NZs + 1) + 2LL * (NXf - NXs - 1) * (NZf - NZs + 1) +

@@ -106,6 +106,15 @@ void orc_set_tree(int levmax);
 void orc_set_tree_factor(double f);
 void orc_boundsum_tree(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
                        const double *src, const double *cxyz, const double *ds, double *sol, int levmax);
+/* Panel sources (ibctyp = 3): FSOUR_A4 in the frame of target plane iplane (area <= 0: the
+ * plane's cell area), the direct sums calc_boundinf_3d_s and the tree sums
+ * calc_boundinf_3d_lev_s; orc_set_ibctyp selects them in orc_pmesh_inf. */
+double orc_fsour_a4(const double XO[3], const double RG[3], int iplane, const double dpm[3], double area);
+void orc_boundsum_s(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                    const double *src, const double *cxyz, double *sol);
+void orc_boundsum_tree_s(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
+                         const double *src, const double *cxyz, const double *ds, double *sol, int levmax);
+void orc_set_ibctyp(int ibctyp);
 void orc_diffuse(const orc_geom *g, int neqpm, double ni, const double *rhs, double *sol);
 int orc_vpm45(const orc_params *p, const orc_geom *g, int neqpm, int mode, double ni, const double *xp,
               const double *qp, double *up, double *gp, int64_t nvr, const double *vel, double *sol_out);

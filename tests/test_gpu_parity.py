@@ -247,6 +247,60 @@ def test_vpm_mode2_tree(vpmlib, oracle, h, sigma):
         assert v <= TOL, (k, v)
 
 
+@pytest.mark.parametrize("nn,itree,levmax,origin", [((33, 29, 31), 0, 1, False), ((65, 57, 49), 0, 1, False),
+                                                     ((65, 57, 49), 1, 3, False), ((43, 43, 43), 1, 2, True),
+                                                     ((33, 29, 31), 1, 3, False)])
+def test_boundary_sum_panels(vpmlib, oracle, nn, itree, levmax, origin):
+    """Constant-source panel sums (pm.inp ibctyp = 3, FSOUR_A4 in the target plane's frame)
+    against the oracle: direct form (point-source sum + near-field panel corrections) and
+    tree form (tree_calc_3d_s, with its top-level cells weighted by the first cell's source;
+    with `origin` the box holds the origin, where the reference's top-level dummies sit)."""
+    V = vpmlib
+    nn = np.array(nn, np.int32)
+    dpm = np.array([1 / 32, 1 / 30, 1 / 28])
+    if origin:
+        xb = np.array([-(nn[0] - 1) * dpm[0] / 2, -(nn[1] - 1) * dpm[1] / 2 + 0.01, -0.3, 0, 0, 0])
+    else:
+        xb = np.array([-1.0, -0.95, -0.85, 0, 0, 0])
+    c0 = xb[:3] + (nn - 1) * dpm / 2
+    rhs = _blob_rhs(nn, xb, dpm, centre=tuple(c0 + np.array([0.02, -0.03, 0.01])), s=0.25)
+    sol = np.zeros_like(rhs)
+    say("case: panel boundary sums", nn.tolist(), "itree", itree, "levmax", levmax, "origin", origin)
+    try:
+        oracle.set_ibctyp(3)
+        oracle.set_tree(levmax if itree else -1)
+        ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    finally:
+        oracle.set_ibctyp(2)
+        oracle.set_tree(-1)
+    point = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    with V.VPM(V.Params.make(0.125)) as c:
+        got = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol, itree=itree, levmax=levmax, ibctyp=3)
+    err = rel_l2(got, ref)
+    say("  rel L2 vs oracle %.2e (panels vs points %.2e)" % (err, rel_l2(ref, point)))
+    assert rel_l2(ref, point) > 1e3 * TOL
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("h,sigma,tree", [(0.125, 0.2, 0), (0.125, 0.2, 1), (1 / 16, 0.1, 1)])
+def test_vpm_mode2_panels(vpmlib, oracle, h, sigma, tree):
+    """vpm(2) with pm.inp ibctyp = 3 (direct and tree forms) against the oracle."""
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 panels h=%g tree=%d n=%d" % (h, tree, xp.shape[0]))
+    po = oracle.Params.make(h, ibctyp=3, iyntree=tree, ilevmax=3)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    with V.VPM(V.Params.make(h, ibctyp=3, iyntree=tree, ilevmax=3)) as c:
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+        errs = {"sol": rel_l2(c.get_grid(2), f["sol"]), "up": rel_l2(up, up_r), "gp": rel_l2(gp, gp_r)}
+    say("  rel L2:", errs)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
 @pytest.mark.parametrize("nn", [(145, 40, 44), (289, 21, 19)])
 def test_solve_many_batches(ctx, oracle, nn):
     """Windows with more line batches than resident work-groups (every work-group loops

@@ -25,7 +25,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["vpmx_api.hip", "k_particles.hip", "k_poisson.hip", "k_boundary.hip", "k_bspec.hip", "k_grid.hip",
-               "k_tree.hip"]
+               "k_tree.hip", "k_panel.hip"]
 CXX_SOURCES = ["geometry.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -494,14 +494,18 @@ static int boundary_sums_lattice(Ctx *c, const Grid &g, int64_t ns, const double
 }
 
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *sig, const double *cxyz,
-                  const double *wgt, GView sol, bool fine, int levmax) {
+                  const double *wgt, GView sol, bool fine, int levmax, int ibctyp, const GView *scratch) {
     if (ncomp != 3) return fail(c, VPMX_ERR_UNSUP, "boundary sums: 3 components");
     const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
     int64_t nt = 2 * ((int64_t)NY * NZ + (int64_t)(NX - 2) * NZ + (int64_t)(NX - 2) * (NY - 2));
     const double *a = wgt, *ds = wgt + 3 * ns;
+    if (ibctyp == 3) {  // panel sources: calc_boundinf_3d_s / calc_boundinf_3d_lev_s
+        if (levmax < 0 && !scratch) return fail(c, VPMX_ERR_STATE, "panel sums need a scratch grid");
+        return boundary_sums_panels(c, g, ns, sig, cxyz, ds, sol, fine, levmax, scratch ? *scratch : sol);
+    }
     if (levmax >= 0) {  // Bounds3d_lev (infdomain_3d with itree = 1, pinfdomain.f90:128-150)
         if (fine) mark(c, ST_BOUNDS, 0);
-        VPMX_TRY(boundary_sums_tree(c, g, ns, sig, cxyz, ds, sol, levmax, fine));
+        VPMX_TRY(boundary_sums_tree(c, g, ns, sig, cxyz, ds, sol, levmax, fine, false));
         if (fine) mark(c, ST_BOUNDS, 1);
         return VPMX_OK;
     }
@@ -514,6 +518,12 @@ int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t ns, const double *si
         VPMX_HIP(c, hipGetLastError());
         return VPMX_OK;
     }
+    return point_sums(c, g, ns, cxyz, a, sol, fine);
+}
+
+int point_sums(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, const double *a, GView sol, bool fine) {
+    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    int64_t nt = 2 * ((int64_t)NY * NZ + (int64_t)(NX - 2) * NZ + (int64_t)(NX - 2) * (NY - 2));
     if (fine && c->bs_mode == 2) {
         mark(c, ST_BOUNDS, 0);
         hipEvent_t e0, e1;

@@ -7,7 +7,8 @@
 //                 face extents are not multiples of 2^lev), because the sums must be the
 //                 reference's sums.
 //   tree sums   : Bounds3d_lev -> calc_boundinf_3d_lev -> tree_calc_3d
-//                 (pmbound.f90:980-1250), ibctyp=2.
+//                 (pmbound.f90:980-1250), ibctyp=2; calc_boundinf_3d_lev_s -> tree_calc_3d_s
+//                 (pmbound.f90:1255-1481), ibctyp=3 (panel sources, k_tree_eval_s).
 //
 // Device form.  The forest is flattened on the host into one preorder list per geometry:
 // entry e is a (level, cell) node, its children follow it, and skip[e] is the first entry
@@ -28,6 +29,7 @@
 #include <map>
 
 #include "bnodes.h"
+#include "panel.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -56,7 +58,7 @@ struct TreePlan {
     TreeTiles tiles{};
     int4 *desc = nullptr;     // per entry: base, pitch, level, kind
     int *skip = nullptr;      // per entry
-    double4 *pos = nullptr;   // per entry: centroid, refinement radius 10 sqrt(DS)
+    double4 *pos = nullptr;   // per entry: centroid, refinement parameter (k_tree_entries)
     double4 *val = nullptr;   // per entry: DS, source per component
 };
 
@@ -189,16 +191,26 @@ int flatten(Ctx *c, const std::vector<std::vector<TreeLink>> &lv, int levmax, st
 }
 
 // Entry aggregates from the level-0 cells (build_level_nbound_3d's sums, same order:
-// rows of the group outer, cells of a row inner; area-weighted source).
+// rows of the group outer, cells of a row inner; area-weighted source).  pos.w encodes the
+// refinement test of the traversal:
+//   point sources (tree_calc_3d):   refine iff r < pos.w  — 10 sqrt(DS) above level 0, -1 at
+//                                   level 0 (never), +inf at dummies (always);
+//   panels (tree_calc_3d_s):        refine iff r / pos.w < 10 — sqrt(DS) above level 0, NaN at
+//                                   level 0 (never), -1 at dummies of kind -1 (always), 0 at
+//                                   dummies of kind -2 (never: the reference takes them as
+//                                   cells of zero area, r / 0 = inf).
+// Dummies sit at the origin with zero area and source, as the reference's zeroed level
+// arrays leave them.
 __global__ void k_tree_entries(int64_t nent, const int4 *__restrict__ desc, int64_t nb, const double *__restrict__ cxyz,
                                const double *__restrict__ sig, const double *__restrict__ ds, double4 *__restrict__ pos,
-                               double4 *__restrict__ val) {
+                               double4 *__restrict__ val, int panels) {
 #pragma clang fp contract(off)
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nent) return;
     const int4 d = desc[e];
-    if (d.w < 0) {  // dummy: always refined, no data
-        pos[e] = make_double4(0.0, 0.0, 0.0, __builtin_inf());
+    if (d.w < 0) {
+        const double w = panels ? (d.w == -1 ? -1.0 : 0.0) : __builtin_inf();
+        pos[e] = make_double4(0.0, 0.0, 0.0, w);
         val[e] = make_double4(0.0, 0.0, 0.0, 0.0);
         return;
     }
@@ -218,8 +230,10 @@ __global__ void k_tree_entries(int64_t nent, const int4 *__restrict__ desc, int6
         }
     }
     const double fl = (double)(s * s);
-    // level 0 is never refined (r >= -1 always)
-    pos[e] = make_double4(xc / fl, yc / fl, zc / fl, d.z > 0 ? 10.0 * sqrt(sc) : -1.0);
+    double w;
+    if (panels) w = d.z > 0 ? sqrt(sc) : __builtin_nan("");
+    else w = d.z > 0 ? 10.0 * sqrt(sc) : -1.0;
+    pos[e] = make_double4(xc / fl, yc / fl, zc / fl, w);
     val[e] = make_double4(sc, q0 / sc, q1 / sc, q2 / sc);
 }
 
@@ -294,6 +308,81 @@ __global__ void __launch_bounds__(64 * TR_WAVES)
     sol.p[2 * sol.cs + o] = sol.p[2 * sol.cs + o] + S2 / PI4;
 }
 
+// tree_calc_3d_s (pmbound.f90:1346-1481) under calc_boundinf_3d_lev_s (:1255-1345):
+// refine iff r / sqrt(DS) < 10 above level 0 or at a dummy of kind -1; a cell taken below the
+// top level adds -s DS / (4 pi r); a top-level cell adds FSOUR_A4 (the target plane's frame,
+// the cell's area) times the source of the FIRST top-level cell, as the reference passes
+// nv = 1 at the top.  SOL += SOURCE.
+__global__ void __launch_bounds__(64 * TR_WAVES)
+    k_tree_eval_s(Grid g, TreeTiles tt, int nent, int levmax, const int4 *__restrict__ desc,
+                  const double4 *__restrict__ pos, const double4 *__restrict__ val, const int *__restrict__ skip,
+                  GView sol) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * TR_WAVES + (threadIdx.x >> 6);
+    if (w >= tt.off[6]) return;
+    int q = 0;
+    while (w >= tt.off[q + 1]) q++;
+    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    const int tl = w - tt.off[q];
+    const int u = (tl % tt.nu[q]) * 8 + (lane & 7);
+    const int v = (tl / tt.nu[q]) * 8 + (lane >> 3);
+    const int hi = q & 1;
+    int i, j, k, iplane;
+    bool ok;
+    if (q < 2) {
+        i = hi ? NX : 1; j = u + 1; k = v + 1; iplane = hi ? -2 : 2;
+        ok = u < NY && v < NZ;
+    } else if (q < 4) {
+        j = hi ? NY : 1; i = u + 2; k = v + 1; iplane = hi ? -1 : 1;
+        ok = u < NX - 2 && v < NZ;
+    } else {
+        k = hi ? NZ : 1; i = u + 2; j = v + 2; iplane = hi ? -3 : 3;
+        ok = u < NX - 2 && v < NY - 2;
+    }
+    const PanelFrame fr = panel_frame(iplane, g.h);
+    const double XO[3] = {g.x0[0] + (double)(i - 1) * g.h[0], g.x0[1] + (double)(j - 1) * g.h[1],
+                          g.x0[2] + (double)(k - 1) * g.h[2]};
+    const double PI4 = 4.0 * (4.0 * atan(1.0));
+    const double4 first = val[0];  // the first top-level cell
+    double S0 = 0.0, S1 = 0.0, S2 = 0.0;
+    int until = ok ? 0 : 0x7fffffff;
+    int e = 0;
+    while (e < nent) {
+        const double4 p = pos[e];
+        const double4 a = val[e];
+        const int lev = desc[e].z;
+        const int sk = skip[e];
+        bool refine = false;
+        if (e >= until) {
+            const double d[3] = {XO[0] - p.x, XO[1] - p.y, XO[2] - p.z};
+            const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            if (r / p.w < 10) {
+                refine = true;
+            } else {
+                if (lev != levmax) {
+                    const double PR = PI4 * r;
+                    S0 = S0 - a.y * a.x / PR;
+                    S1 = S1 - a.z * a.x / PR;
+                    S2 = S2 - a.w * a.x / PR;
+                } else {
+                    const double FIS = fsour_a4(fr, d, a.x, PI4);
+                    S0 = S0 + first.y * FIS;
+                    S1 = S1 + first.z * FIS;
+                    S2 = S2 + first.w * FIS;
+                }
+                until = sk;
+            }
+        }
+        e = __builtin_amdgcn_readfirstlane(__any(refine) ? e + 1 : sk);
+    }
+    if (!ok) return;
+    const int64_t o = sol.at(i - 1, j - 1, k - 1);
+    sol.p[o] = sol.p[o] + S0;
+    sol.p[sol.cs + o] = sol.p[sol.cs + o] + S1;
+    sol.p[2 * sol.cs + o] = sol.p[2 * sol.cs + o] + S2;
+}
+
 void free_plan(TreePlan *p) {
     if (!p) return;
     (void)hipFree(p->desc);
@@ -351,14 +440,20 @@ int get_plan(Ctx *c, const Grid &g, int levmax, int64_t nb, TreePlan **out) {
 }  // namespace
 
 int boundary_sums_tree(Ctx *c, const Grid &g, int64_t nb, const double *sig, const double *cxyz, const double *ds,
-                       GView sol, int levmax, bool fine) {
+                       GView sol, int levmax, bool fine, bool panels) {
     if (g.n[0] < 3 || g.n[1] < 3 || g.n[2] < 3) return fail(c, VPMX_ERR_ARG, "tree: grid too small");
     TreePlan *p = nullptr;
     VPMX_TRY(get_plan(c, g, levmax, nb, &p));
     hipLaunchKernelGGL(k_tree_entries, dim3((unsigned)((p->nent + 255) / 256)), dim3(256), 0, c->stream, p->nent,
-                       p->desc, nb, cxyz, sig, ds, p->pos, p->val);
+                       p->desc, nb, cxyz, sig, ds, p->pos, p->val, panels ? 1 : 0);
     VPMX_HIP(c, hipGetLastError());
     const unsigned blocks = (unsigned)((p->tiles.off[6] + TR_WAVES - 1) / TR_WAVES);
+    if (panels) {
+        hipLaunchKernelGGL(k_tree_eval_s, dim3(blocks), dim3(64 * TR_WAVES), 0, c->stream, g, p->tiles, (int)p->nent,
+                           levmax, p->desc, p->pos, p->val, p->skip, sol);
+        VPMX_HIP(c, hipGetLastError());
+        return VPMX_OK;
+    }
     hipEvent_t e0, e1;
     if (fine && kernel_events(c, K_BOUNDSUM, &e0, &e1))
         hipExtLaunchKernelGGL(k_tree_eval, dim3(blocks), dim3(64 * TR_WAVES), 0, c->stream, e0, e1, 0, g, p->tiles,

@@ -89,10 +89,11 @@ double wall_ms() {
 static void collect_times(Ctx *c, const bool *) { (void)hipStreamSynchronize(c->stream); }
 
 // ---------------------------------------------------------------- pmesh (iynbc=1)
-// pmesh(iynbc=1, ibctyp=2) on grid g: SOL_0 = zero-BC solve, sources from its normal
-// derivatives, SOL(faces) += boundary sums (tree form with levmax levels when levmax >= 0,
-// pmlib.f90:69 itree/levmax), then the Dirichlet solve of the whole grid.
-static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax) {
+// pmesh(iynbc=1) on grid g: SOL_0 = zero-BC solve, sources from its normal derivatives,
+// SOL(faces) += boundary sums (ibctyp 2 point sources, 3 panels; tree form with levmax
+// levels when levmax >= 0, pmlib.f90:69 ibctyp/itree/levmax), then the Dirichlet solve of
+// the whole grid.
+static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax, int ibctyp) {
     int st = VPMX_OK;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     int64_t G = (int64_t)n[0] * n[1] * n[2];
@@ -109,7 +110,8 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     double *wgt = (double *)ensure(c, c->src_w, sizeof(double) * 4 * (size_t)nbmax, &st); if (st) return st;
     int64_t nb = 0;
     VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
-    VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax));
+    // SOL_0 is dead once the sources exist: scratch of the panel sums
+    VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax, ibctyp, &s0v));
     VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol));
     return VPMX_OK;
 }
@@ -133,7 +135,7 @@ static int yaps_single(Ctx *c, bool *used) {
     const bool tree = P.iyntree == 1;
     if (tree && G.ilevmax < 1)
         return fail(c, VPMX_ERR_UNSUP, "tree = 1 needs at least one level on the coarse grid (ilevmax >= 1)");
-    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true, tree ? G.ilevmax : -1));
+    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true, tree ? G.ilevmax : -1, P.ibctyp));
     mark(c, ST_FINE, 1);
     used[ST_FINE] = used[ST_BOUNDS] = true;
     // 2. coarse correction
@@ -169,7 +171,7 @@ static int yaps_single(Ctx *c, bool *used) {
     VPMX_TRY(coarse_laplacian(c, 3, cn, fp_lo, fp_hi, G.dpm_coarse, smv, rcv));
     VPMX_TRY(fill(c, solc, 3 * GC, 0.0));
     Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
-    VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false, tree ? G.ilevmax - 1 : -1));
+    VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false, tree ? G.ilevmax - 1 : -1, P.ibctyp));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));  // cmap host vector goes out of scope
     mark(c, ST_COARSE, 1);
     used[ST_COARSE] = true;
@@ -216,7 +218,8 @@ static int check_ready(Ctx *c, int neqpm) {
         if (P.idvpm != 0) return fail(c, VPMX_ERR_ARG, "WRONG IDVPM");
         return fail(c, VPMX_ERR_UNSUP, "IDVPM=0 (variable volume) is not supported by this build");
     }
-    if (P.ibctyp != 2) return fail(c, VPMX_ERR_UNSUP, "only ibctyp=2 (boundary-source Green sums) is supported");
+    if (P.ibctyp != 2 && P.ibctyp != 3)
+        return fail(c, VPMX_ERR_UNSUP, "ibctyp must be 2 (point sources) or 3 (constant-source panels)");
     return VPMX_OK;
 }
 
@@ -419,7 +422,7 @@ void vpmx_destroy(vpmx_ctx *p) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->xp, &c->qp, &c->up, &c->gp, &c->x0, &c->q0, &c->rhs_bl, &c->sol_bl, &c->sol0, &c->vel,
-                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->samp, &c->rhs_c, &c->sol_c,
+                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->panel_a, &c->samp, &c->rhs_c, &c->sol_c,
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
@@ -735,11 +738,12 @@ int32_t vpmx_stage_solve(vpmx_ctx *p, const double xbound[6], const double dpm[3
 
 int32_t vpmx_stage_pmesh_inf(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
                              int32_t nc, const double *rhs, double *sol) {
-    return vpmx_stage_pmesh_inf_lev(p, xbound, dpm, nn, nc, rhs, sol, 0, 1);
+    return vpmx_stage_pmesh_inf_lev(p, xbound, dpm, nn, nc, rhs, sol, 2, 0, 1);
 }
 
 int32_t vpmx_stage_pmesh_inf_lev(vpmx_ctx *p, const double xbound[6], const double dpm[3], const int32_t nn[3],
-                                 int32_t nc, const double *rhs, double *sol, int32_t itree, int32_t levmax) {
+                                 int32_t nc, const double *rhs, double *sol, int32_t ibctyp, int32_t itree,
+                                 int32_t levmax) {
     Ctx *c = C(p);
     if (!c || !xbound || !dpm || !nn || !rhs || !sol) return VPMX_ERR_ARG;
     if (nc != 3) return fail(c, VPMX_ERR_ARG, "nc must be 3");
@@ -755,7 +759,8 @@ int32_t vpmx_stage_pmesh_inf_lev(vpmx_ctx *p, const double xbound[6], const doub
     VPMX_HIP(c, hipMemcpyAsync(ra, sol, sizeof(double) * nc * G, hipMemcpyHostToDevice, c->stream));
     VPMX_TRY(aos_to_soa(c, nc, n, ra, view(ss, n)));
     Grid g = make_grid(nn, xbound, dpm);
-    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, true, itree == 1 ? levmax : -1));
+    if (ibctyp != 2 && ibctyp != 3) return fail(c, VPMX_ERR_ARG, "ibctyp must be 2 or 3");
+    VPMX_TRY(pmesh_inf(c, g, view(rs, n), view(ss, n), c->sol0, true, itree == 1 ? levmax : -1, ibctyp));
     VPMX_TRY(soa_to_aos(c, nc, nc, n, view(ss, n), 0.0, ra));
     VPMX_HIP(c, hipMemcpyAsync(sol, ra, sizeof(double) * nc * G, hipMemcpyDeviceToHost, c->stream));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));

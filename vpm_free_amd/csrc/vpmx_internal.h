@@ -96,16 +96,26 @@ int fft_shape(Ctx *c, int L, FftShape *out);
 // Boundary (k_boundary.hip)
 int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt,
                   int64_t *nbound);
-// levmax >= 0: the tree form with that many levels (pm.inp tree = 1), else the direct sums.
+// levmax >= 0: the tree form with that many levels (pm.inp tree = 1), else the direct sums;
+// ibctyp 2 point sources, 3 constant-source panels (needs `scratch`, a grid of sol's shape
+// whose boundary nodes it may overwrite, for the direct form).
 int boundary_sums(Ctx *c, int ncomp, const Grid &g, int64_t nbound, const double *sig, const double *cxyz,
-                  const double *wgt, GView sol, bool fine, int levmax = -1);
+                  const double *wgt, GView sol, bool fine, int levmax = -1, int ibctyp = 2,
+                  const GView *scratch = nullptr);
+// SOL(t) += sum_s a(c, s) / |x_t - x_s| over the face-cell centres (spectral / lattice /
+// direct evaluation by the context's setting).
+int point_sums(Ctx *c, const Grid &g, int64_t nbound, const double *cxyz, const double *a, GView sol, bool fine);
+// Panel sources (k_panel.hip), direct or tree form.
+int boundary_sums_panels(Ctx *c, const Grid &g, int64_t nb, const double *sig, const double *cxyz, const double *ds,
+                         GView sol, bool fine, int levmax, GView scratch);
 // Spectral evaluation of the same sums (k_bspec.hip); returns VPMX_ERR_UNSUP when the
 // block's faces are too long for its in-LDS FFTs, so the caller can use the lattice kernel.
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol);
 void free_bspec(Ctx *c);
 // Tree form (k_tree.hip): sig (3 x nbound), cxyz (3 x nbound), ds (nbound) of normal_derivs.
+// panels: tree_calc_3d_s (ibctyp = 3) instead of tree_calc_3d.
 int boundary_sums_tree(Ctx *c, const Grid &g, int64_t nbound, const double *sig, const double *cxyz, const double *ds,
-                       GView sol, int levmax, bool fine);
+                       GView sol, int levmax, bool fine, bool panels);
 int64_t tree_entries(Ctx *c, const int n[3], int levmax);
 void free_tree(Ctx *c);
 

@@ -98,7 +98,7 @@ def load_library():
         "vpmx_diagnostics": [vp, d, d],
         "vpmx_stage_solve": [vp, d, d, C.POINTER(i32), i32, C.POINTER(i32), d, d, i32, d],
         "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
-        "vpmx_stage_pmesh_inf_lev": [vp, d, d, C.POINTER(i32), i32, d, d, i32, i32],
+        "vpmx_stage_pmesh_inf_lev": [vp, d, d, C.POINTER(i32), i32, d, d, i32, i32, i32],
         "vpmx_stage_project": [vp, d, d, C.POINTER(i32), C.POINTER(i32), d, d, i64, i32, i32, i32, d],
         "vpmx_stage_times": [vp, d, i32],
         "vpmx_kernel_times": [vp, d, C.POINTER(i64), i32, i32],
@@ -310,14 +310,14 @@ class VPM:
                                                    None))
         return out
 
-    def stage_pmesh_inf(self, xbound, dpm, nn, rhs, sol, itree=0, levmax=1):
-        """pmesh(iynbc=1, ibctyp=2) (pmlib.f90:69) with its itree / levmax arguments."""
+    def stage_pmesh_inf(self, xbound, dpm, nn, rhs, sol, itree=0, levmax=1, ibctyp=2):
+        """pmesh(iynbc=1) (pmlib.f90:69) with its ibctyp / itree / levmax arguments."""
         xb, dp = _f64(xbound), _f64(dpm)
         nn_a, nn_p = _ip(nn)
         rhs = _f64(rhs)
         out = np.array(sol, dtype=np.float64, order="C")
         self._check(self._lib.vpmx_stage_pmesh_inf_lev(self._h, _dp(xb), _dp(dp), nn_p, 3, _dp(rhs), _dp(out),
-                                                       int(itree), int(levmax)))
+                                                       int(ibctyp), int(itree), int(levmax)))
         return out
 
     def stage_project(self, xbound, dpm, nn, nn_bl, XP, QP, itype=4, idvpm=1):
