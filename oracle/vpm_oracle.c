@@ -1018,59 +1018,87 @@ void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3],
 }
 
 /* --------------------------------------------------------------------- yaps3d */
-/* yaps3d.f90:1-453 for BLOCKS=1 (block nb=1, nb_i=nb_j=nb_k=1; no neighbour exchange). */
+/* yaps3d.f90:1-453 for BLOCKS = NBI*NBJ*NBK blocks, the work of every rank in turn.  Block
+ * b (0-based, b = (k-1)*NBI*NBJ + (j-1)*NBI + i-1) has nnbl[3b..] nodes; its arrays sit one
+ * after another in rhs_bl / sol_bl (nc components, component fastest).
+ *   1. each block's infinite-domain solve (yaps3d.f90:36-56);
+ *   2. samples of every block on the coarse grid and its map_nodes / NN_coarse_map
+ *      (:104-138);
+ *   3. coarse RHS = sum over blocks, in block order, of the 7-point Laplacian of the
+ *      block's samples on its footprint interior (:179-196, 1990-2016);
+ *   4. the coarse infinite-domain solve (:283-299);
+ *   5. per block, interp_stencil on its inner-box faces (:302-395, 1914-1983): TSC of the
+ *      coarse solution minus the samples of the blocks whose footprint covers the stencil,
+ *      plus those neighbours' fine solutions at the node (BBound of mapnodes_bl,
+ *      :457-1912: a neighbour sends every node of the receiver's inner faces that lies in
+ *      its own block grid; nodes it does not reach stay 0);
+ *   6. per block, the Dirichlet solve of its inner box (:397-418). */
 int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs_bl, double *sol_bl) {
-    if (g->nblocks != 1) return -1;
-    const double *xbl = &g->xbound_bl[0];
-    const int32_t *nnbl = &g->nnbl[0], *nnbl_bl = &g->nnbl_bl[0];
-    size_t B = (size_t)nnbl[0] * nnbl[1] * nnbl[2];
-    memset(sol_bl, 0, sizeof(double) * B * nc);
-    /* 1. fine infinite-domain solve on the whole block (tree with lmax = ilevmax when
-     *    iyntree = 1, yaps3d.f90:47-48) */
+    const int NBL = g->nblocks;
+    if (NBL < 1 || NBL > ORC_MAXB || g->nbi * g->nbj * g->nbk != NBL) return -1;
+    size_t boff[ORC_MAXB + 1];
+    boff[0] = 0;
+    for (int b = 0; b < NBL; b++)
+        boff[b + 1] = boff[b] + (size_t)g->nnbl[3 * b] * g->nnbl[3 * b + 1] * g->nnbl[3 * b + 2] * nc;
+    memset(sol_bl, 0, sizeof(double) * boff[NBL]);
     const int saved_tree = g_tree_levmax, saved_bc = g_ibctyp;
     g_ibctyp = p->ibctyp == 3 ? 3 : 2;
+    /* 1. fine infinite-domain solves (tree with lmax = ilevmax when iyntree = 1, :47-48) */
     if (p->iyntree == 1) g_tree_levmax = g->ilevmax;
-    orc_pmesh_inf(xbl, g->dpm, nnbl, nc, rhs_bl, sol_bl);
-    /* 2. sample the fine solution on the coarse grid (yaps3d.f90:104-138) */
+    for (int b = 0; b < NBL; b++)
+        orc_pmesh_inf(&g->xbound_bl[6 * b], g->dpm, &g->nnbl[3 * b], nc, rhs_bl + boff[b], sol_bl + boff[b]);
+    /* 2. samples */
     int ncx = g->nn_coarse[0], ncy = g->nn_coarse[1], ncz = g->nn_coarse[2];
     size_t GC = (size_t)ncx * ncy * ncz;
-    double *samp = calloc(GC * nc, sizeof(double));
-    unsigned char *map = calloc(GC, 1);
-    int cmap[6] = {-1, -1, -1, -1, -1, -1};
+    double *samp = calloc(GC * nc * NBL, sizeof(double));
+    unsigned char *map = calloc(GC * NBL, 1);
+    int cmap[6 * ORC_MAXB];
+    for (int q = 0; q < 6 * NBL; q++) cmap[q] = -1;
     int ncs = p->ncoarse;
-    for (int k = 1; k <= nnbl[2]; k += ncs)
-        for (int j = 1; j <= nnbl[1]; j += ncs)
-            for (int i = 1; i <= nnbl[0]; i += ncs) {
-                double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
-                double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
-                double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
-                int in = (int)lround((X0 - g->xbound_coarse[0]) / g->dpm_coarse[0]) + 1;
-                int jn = (int)lround((X1 - g->xbound_coarse[1]) / g->dpm_coarse[1]) + 1;
-                int kn = (int)lround((X2 - g->xbound_coarse[2]) / g->dpm_coarse[2]) + 1;
-                if (i == 1) cmap[0] = in;
-                if (i == nnbl[0]) cmap[3] = in;
-                if (j == 1) cmap[1] = jn;
-                if (j == nnbl[1]) cmap[4] = jn;
-                if (k == 1) cmap[2] = kn;
-                if (k == nnbl[2]) cmap[5] = kn;
-                if (in < 1 || in > ncx || jn < 1 || jn > ncy || kn < 1 || kn > ncz) continue;
-                for (int c = 0; c < nc; c++)
-                    samp[IX(c, in - 1, jn - 1, kn - 1, nc, ncx, ncy)] = sol_bl[IX(c, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1])];
-                map[(size_t)(in - 1) + (size_t)ncx * ((jn - 1) + (size_t)ncy * (kn - 1))] = 1;
-            }
-    /* 3. coarse RHS = 7-point Laplacian of the sample on the footprint interior (yaps3d.f90:1990-2016) */
+    for (int b = 0; b < NBL; b++) {
+        const double *xbl = &g->xbound_bl[6 * b];
+        const int32_t *nnbl = &g->nnbl[3 * b];
+        for (int k = 1; k <= nnbl[2]; k += ncs)
+            for (int j = 1; j <= nnbl[1]; j += ncs)
+                for (int i = 1; i <= nnbl[0]; i += ncs) {
+                    double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
+                    double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
+                    double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
+                    int in = (int)lround((X0 - g->xbound_coarse[0]) / g->dpm_coarse[0]) + 1;
+                    int jn = (int)lround((X1 - g->xbound_coarse[1]) / g->dpm_coarse[1]) + 1;
+                    int kn = (int)lround((X2 - g->xbound_coarse[2]) / g->dpm_coarse[2]) + 1;
+                    if (i == 1) cmap[6 * b + 0] = in;
+                    if (i == nnbl[0]) cmap[6 * b + 3] = in;
+                    if (j == 1) cmap[6 * b + 1] = jn;
+                    if (j == nnbl[1]) cmap[6 * b + 4] = jn;
+                    if (k == 1) cmap[6 * b + 2] = kn;
+                    if (k == nnbl[2]) cmap[6 * b + 5] = kn;
+                    if (in < 1 || in > ncx || jn < 1 || jn > ncy || kn < 1 || kn > ncz) continue;
+                    for (int c = 0; c < nc; c++)
+                        samp[GC * nc * b + IX(c, in - 1, jn - 1, kn - 1, nc, ncx, ncy)] =
+                            sol_bl[boff[b] + IX(c, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1])];
+                    map[GC * b + (size_t)(in - 1) + (size_t)ncx * ((jn - 1) + (size_t)ncy * (kn - 1))] = 1;
+                }
+    }
+    /* 3. coarse RHS */
     double *rhs_c = calloc(GC * nc, sizeof(double)), *sol_c = calloc(GC * nc, sizeof(double));
     const double *dc = g->dpm_coarse;
-#define SC(c, i, j, k) samp[IX(c, (i)-1, (j)-1, (k)-1, nc, ncx, ncy)]
-    for (int k = cmap[2] + 1; k <= cmap[5] - 1; k++)
-        for (int j = cmap[1] + 1; j <= cmap[4] - 1; j++)
-            for (int i = cmap[0] + 1; i <= cmap[3] - 1; i++)
-                for (int c = 0; c < nc; c++) {
-                    double v = ((SC(c, i + 1, j, k) - 2 * SC(c, i, j, k)) + SC(c, i - 1, j, k)) / (dc[0] * dc[0]) +
-                               ((SC(c, i, j + 1, k) - 2 * SC(c, i, j, k)) + SC(c, i, j - 1, k)) / (dc[1] * dc[1]) +
-                               ((SC(c, i, j, k + 1) - 2 * SC(c, i, j, k)) + SC(c, i, j, k - 1)) / (dc[2] * dc[2]);
-                    rhs_c[IX(c, i - 1, j - 1, k - 1, nc, ncx, ncy)] = 0.0 + v;
-                }
+    for (int b = 0; b < NBL; b++) {
+        const double *sb = samp + GC * nc * b;
+        const int *cm = &cmap[6 * b];
+#define SC(c, i, j, k) sb[IX(c, (i)-1, (j)-1, (k)-1, nc, ncx, ncy)]
+        for (int k = cm[2] + 1; k <= cm[5] - 1; k++)
+            for (int j = cm[1] + 1; j <= cm[4] - 1; j++)
+                for (int i = cm[0] + 1; i <= cm[3] - 1; i++)
+                    for (int c = 0; c < nc; c++) {
+                        double v = ((SC(c, i + 1, j, k) - 2 * SC(c, i, j, k)) + SC(c, i - 1, j, k)) / (dc[0] * dc[0]) +
+                                   ((SC(c, i, j + 1, k) - 2 * SC(c, i, j, k)) + SC(c, i, j - 1, k)) / (dc[1] * dc[1]) +
+                                   ((SC(c, i, j, k + 1) - 2 * SC(c, i, j, k)) + SC(c, i, j, k - 1)) / (dc[2] * dc[2]);
+                        size_t o = IX(c, i - 1, j - 1, k - 1, nc, ncx, ncy);
+                        rhs_c[o] = rhs_c[o] + v;
+                    }
+#undef SC
+    }
     /* 4. coarse infinite-domain solve */
     double xbc[6];
     memcpy(xbc, g->xbound_coarse, sizeof xbc);
@@ -1078,54 +1106,89 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
     orc_pmesh_inf(xbc, g->dpm_coarse, g->nn_coarse, nc, rhs_c, sol_c);
     g_tree_levmax = saved_tree;
     g_ibctyp = saved_bc;
-    /* 5. interp_stencil on the 6 faces of the inner box (yaps3d.f90:340-395, 1914-1983) */
-    int NXs = nnbl_bl[0], NYs = nnbl_bl[1], NZs = nnbl_bl[2], NXf = nnbl_bl[3], NYf = nnbl_bl[4], NZf = nnbl_bl[5];
-    int64_t nface = 2LL * (NYf - NYs + 1) * (NZf - NZs + 1) + 2LL * (NXf - NXs - 1) * (NZf - NZs + 1) +
-                    2LL * (NXf - NXs - 1) * (NYf - NYs - 1);
-    int32_t *fl = malloc(sizeof(int32_t) * 3 * nface);
-    int64_t q = 0;
-    for (int f = 0; f < 2; f++)
-        for (int k = NZs; k <= NZf; k++)
-            for (int j = NYs; j <= NYf; j++) { fl[3*q] = f ? NXf : NXs; fl[3*q+1] = j; fl[3*q+2] = k; q++; }
-    for (int f = 0; f < 2; f++)
-        for (int k = NZs; k <= NZf; k++)
-            for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = f ? NYf : NYs; fl[3*q+2] = k; q++; }
-    for (int f = 0; f < 2; f++)
-        for (int j = NYs + 1; j <= NYf - 1; j++)
-            for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = j; fl[3*q+2] = f ? NZf : NZs; q++; }
+    /* 5. interp_stencil; neighbours' fine solutions as they are after step 1 */
+    double *fine = malloc(sizeof(double) * boff[NBL]);
+    memcpy(fine, sol_bl, sizeof(double) * boff[NBL]);
+    const int NBI = g->nbi, NBJ = g->nbj, NBK = g->nbk;
+    for (int b = 0; b < NBL; b++) {
+        const double *xbl = &g->xbound_bl[6 * b];
+        const int32_t *nnbl = &g->nnbl[3 * b], *nnbl_bl = &g->nnbl_bl[6 * b];
+        const int nb_i = b % NBI + 1, nb_j = (b / NBI) % NBJ + 1, nb_k = b / (NBI * NBJ) + 1;
+        double *sb = sol_bl + boff[b];
+        int NXs = nnbl_bl[0], NYs = nnbl_bl[1], NZs = nnbl_bl[2], NXf = nnbl_bl[3], NYf = nnbl_bl[4], NZf = nnbl_bl[5];
+        int64_t nface = 2LL * (NYf - NYs + 1) * (NZf - NZs + 1) + 2LL * (NXf - NXs - 1) * (NZf - NZs + 1) +
+                        2LL * (NXf - NXs - 1) * (NYf - NYs - 1);
+        int32_t *fl = malloc(sizeof(int32_t) * 3 * nface);
+        int64_t q = 0;
+        for (int f = 0; f < 2; f++)
+            for (int k = NZs; k <= NZf; k++)
+                for (int j = NYs; j <= NYf; j++) { fl[3*q] = f ? NXf : NXs; fl[3*q+1] = j; fl[3*q+2] = k; q++; }
+        for (int f = 0; f < 2; f++)
+            for (int k = NZs; k <= NZf; k++)
+                for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = f ? NYf : NYs; fl[3*q+2] = k; q++; }
+        for (int f = 0; f < 2; f++)
+            for (int j = NYs + 1; j <= NYf - 1; j++)
+                for (int i = NXs + 1; i <= NXf - 1; i++) { fl[3*q] = i; fl[3*q+1] = j; fl[3*q+2] = f ? NZf : NZs; q++; }
 #pragma omp parallel for schedule(static)
-    for (int64_t t = 0; t < nface; t++) {
-        int i = fl[3 * t], j = fl[3 * t + 1], k = fl[3 * t + 2];
-        double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
-        double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
-        double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
-        int in = (int)lround((X0 - g->xbound_coarse[0]) / dc[0]) + 1;
-        int jn = (int)lround((X1 - g->xbound_coarse[1]) / dc[1]) + 1;
-        int kn = (int)lround((X2 - g->xbound_coarse[2]) / dc[2]) + 1;
-        int nnb = 0;
-        if (!map[(size_t)(in - 2) + (size_t)ncx * ((jn - 2) + (size_t)ncy * (kn - 2))] ||
-            !map[(size_t)(in) + (size_t)ncx * ((jn) + (size_t)ncy * (kn))])
-            nnb = 1;
-        size_t o = IX(0, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1]);
-        for (int kc = kn - 1; kc <= kn + 1; kc++)
-            for (int jc = jn - 1; jc <= jn + 1; jc++)
-                for (int ic = in - 1; ic <= in + 1; ic++) {
-                    double xc = (X0 - g->xbound_coarse[0] - (double)(ic - 1) * dc[0]) / dc[0];
-                    double yc = (X1 - g->xbound_coarse[1] - (double)(jc - 1) * dc[1]) / dc[1];
-                    double zc = (X2 - g->xbound_coarse[2] - (double)(kc - 1) * dc[2]) / dc[2];
-                    double fw = orc_projection_fun(3, xc) * orc_projection_fun(3, yc) * orc_projection_fun(3, zc);
-                    for (int c = 0; c < nc; c++) {
-                        double smp = SC(c, ic, jc, kc);
-                        double add_sample = smp + 0.0;
-                        double add = nnb ? smp + 0.0 : 0.0;
-                        sol_bl[o + c] = sol_bl[o + c] + fw * ((sol_c[IX(c, ic - 1, jc - 1, kc - 1, nc, ncx, ncy)] - add_sample) + add);
+        for (int64_t t = 0; t < nface; t++) {
+            int i = fl[3 * t], j = fl[3 * t + 1], k = fl[3 * t + 2];
+            double X0 = xbl[0] + (double)(i - 1) * g->dpm[0];
+            double X1 = xbl[1] + (double)(j - 1) * g->dpm[1];
+            double X2 = xbl[2] + (double)(k - 1) * g->dpm[2];
+            int in = (int)lround((X0 - g->xbound_coarse[0]) / dc[0]) + 1;
+            int jn = (int)lround((X1 - g->xbound_coarse[1]) / dc[1]) + 1;
+            int kn = (int)lround((X2 - g->xbound_coarse[2]) / dc[2]) + 1;
+            int nnb[ORC_MAXB];
+            for (int bc = 0; bc < NBL; bc++) {
+                const unsigned char *mb = map + GC * bc;
+                nnb[bc] = (!mb[(size_t)(in - 2) + (size_t)ncx * ((jn - 2) + (size_t)ncy * (kn - 2))] ||
+                           !mb[(size_t)(in) + (size_t)ncx * ((jn) + (size_t)ncy * (kn))]) ? 1 : 0;
+            }
+            size_t o = IX(0, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1]);
+            for (int kc = kn - 1; kc <= kn + 1; kc++)
+                for (int jc = jn - 1; jc <= jn + 1; jc++)
+                    for (int ic = in - 1; ic <= in + 1; ic++) {
+                        size_t oc = IX(0, ic - 1, jc - 1, kc - 1, nc, ncx, ncy);
+                        double add[8] = {0}, add_sample[8] = {0};
+                        for (int bc = 0; bc < NBL; bc++)
+                            for (int c = 0; c < nc; c++) {
+                                double smp = samp[GC * nc * bc + oc + c];
+                                add_sample[c] = smp + add_sample[c];
+                                if (nnb[bc]) add[c] = smp + add[c];
+                            }
+                        double xc = (X0 - g->xbound_coarse[0] - (double)(ic - 1) * dc[0]) / dc[0];
+                        double yc = (X1 - g->xbound_coarse[1] - (double)(jc - 1) * dc[1]) / dc[1];
+                        double zc = (X2 - g->xbound_coarse[2] - (double)(kc - 1) * dc[2]) / dc[2];
+                        double fw = orc_projection_fun(3, xc) * orc_projection_fun(3, yc) * orc_projection_fun(3, zc);
+                        for (int c = 0; c < nc; c++)
+                            sb[o + c] = sb[o + c] + fw * ((sol_c[oc + c] - add_sample[c]) + add[c]);
                     }
-                }
+            for (int k_nb = nb_k - 1; k_nb <= nb_k + 1; k_nb++)
+                for (int j_nb = nb_j - 1; j_nb <= nb_j + 1; j_nb++)
+                    for (int i_nb = nb_i - 1; i_nb <= nb_i + 1; i_nb++) {
+                        if (i_nb < 1 || i_nb > NBI || j_nb < 1 || j_nb > NBJ || k_nb < 1 || k_nb > NBK) continue;
+                        int bc = (k_nb - 1) * NBI * NBJ + (j_nb - 1) * NBI + i_nb - 1;
+                        if (nnb[bc] || bc == b) continue;
+                        /* the node in block bc's grid (all blocks share the lattice) */
+                        const double *xc = &g->xbound_bl[6 * bc];
+                        const int32_t *nc3 = &g->nnbl[3 * bc];
+                        int ii = i + (int)lround((xbl[0] - xc[0]) / g->dpm[0]);
+                        int jj = j + (int)lround((xbl[1] - xc[1]) / g->dpm[1]);
+                        int kk = k + (int)lround((xbl[2] - xc[2]) / g->dpm[2]);
+                        if (ii < 1 || ii > nc3[0] || jj < 1 || jj > nc3[1] || kk < 1 || kk > nc3[2]) continue;
+                        const double *fb = fine + boff[bc];
+                        size_t on = IX(0, ii - 1, jj - 1, kk - 1, nc, nc3[0], nc3[1]);
+                        for (int c = 0; c < nc; c++) sb[o + c] = sb[o + c] + fb[on + c];
+                    }
+        }
+        free(fl);
     }
-#undef SC
-    free(fl); free(samp); free(map); free(rhs_c); free(sol_c);
-    /* 6. final Dirichlet solve on the inner box */
-    for (int c = 0; c < nc; c++) orc_solve_window(xbl, g->dpm, nnbl, nc, c, nnbl_bl, rhs_bl, sol_bl, 0, NULL);
+    free(fine); free(samp); free(map); free(rhs_c); free(sol_c);
+    /* 6. final Dirichlet solves on the inner boxes */
+    for (int b = 0; b < NBL; b++)
+        for (int c = 0; c < nc; c++)
+            orc_solve_window(&g->xbound_bl[6 * b], g->dpm, &g->nnbl[3 * b], nc, c, &g->nnbl_bl[6 * b], rhs_bl + boff[b],
+                             sol_bl + boff[b], 0, NULL);
     return 0;
 }
 
@@ -1279,11 +1342,58 @@ int orc_project(const double xb[6], const double dpm[3], const int32_t nn[3], co
 }
 
 /* ------------------------------------------------------------------- vpm mode 2 */
-/* vpm.f90:171-313 (WhatToDo=2) on one block. */
+/* rhsscat_3d (vpm_mpi.f90:199-238) for every block: the block's inner box takes the global
+ * window ixs = (nb_i-1)(NXf-NXs) + NN_bl(1) ..., the rest of the block is 0, and a block that
+ * is not first along an axis zeroes its min face on that axis. */
+static void rhsscat(const orc_geom *g, int neqpm, const double *rhs, int nr, double *rhs_bl, size_t *boff) {
+    int NX = g->nn[0], NY = g->nn[1];
+    boff[0] = 0;
+    for (int b = 0; b < g->nblocks; b++)
+        boff[b + 1] = boff[b] + (size_t)g->nnbl[3 * b] * g->nnbl[3 * b + 1] * g->nnbl[3 * b + 2] * neqpm;
+    memset(rhs_bl, 0, sizeof(double) * boff[g->nblocks]);
+    for (int b = 0; b < g->nblocks; b++) {
+        const int32_t *nnbl = &g->nnbl[3 * b], *bl = &g->nnbl_bl[6 * b];
+        int nb_i = b % g->nbi + 1, nb_j = (b / g->nbi) % g->nbj + 1, nb_k = b / (g->nbi * g->nbj) + 1;
+        int ixs = (nb_i - 1) * (bl[3] - bl[0]) + g->nn_bl[0];
+        int jxs = (nb_j - 1) * (bl[4] - bl[1]) + g->nn_bl[1];
+        int kxs = (nb_k - 1) * (bl[5] - bl[2]) + g->nn_bl[2];
+        double *rb = rhs_bl + boff[b];
+        for (int k = bl[2]; k <= bl[5]; k++)
+            for (int j = bl[1]; j <= bl[4]; j++)
+                for (int i = bl[0]; i <= bl[3]; i++) {
+                    int zero = (nb_i > 1 && i == bl[0]) || (nb_j > 1 && j == bl[1]) || (nb_k > 1 && k == bl[2]);
+                    for (int c = 0; c < neqpm; c++)
+                        rb[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])] =
+                            zero ? 0.0 : rhs[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, nr, NX, NY)];
+                }
+    }
+}
+
+/* solget_3d (vpm_mpi.f90:240-328): inner boxes into the global grid in block order (a later
+ * block overwrites the faces it shares with an earlier one). */
+static void solget(const orc_geom *g, int neqpm, const double *sol_bl, const size_t *boff, double *sol) {
+    int NX = g->nn[0], NY = g->nn[1];
+    for (int b = 0; b < g->nblocks; b++) {
+        const int32_t *nnbl = &g->nnbl[3 * b], *bl = &g->nnbl_bl[6 * b];
+        int nb_i = b % g->nbi + 1, nb_j = (b / g->nbi) % g->nbj + 1, nb_k = b / (g->nbi * g->nbj) + 1;
+        int ixs = (nb_i - 1) * (bl[3] - bl[0]) + g->nn_bl[0];
+        int jxs = (nb_j - 1) * (bl[4] - bl[1]) + g->nn_bl[1];
+        int kxs = (nb_k - 1) * (bl[5] - bl[2]) + g->nn_bl[2];
+        const double *sb = sol_bl + boff[b];
+        for (int k = bl[2]; k <= bl[5]; k++)
+            for (int j = bl[1]; j <= bl[4]; j++)
+                for (int i = bl[0]; i <= bl[3]; i++)
+                    for (int c = 0; c < neqpm; c++)
+                        sol[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, neqpm, NX, NY)] =
+                            sb[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])];
+    }
+}
+
+/* vpm.f90:171-313 (WhatToDo=2), every yaps block in turn. */
 int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp, const double *qp, double *up,
              double *gp, int64_t nvr, double *rhs_out, double *psi_out, double *sol_out, double *vel_out) {
     if (nvr == 0) return 0;
-    if (g->nblocks != 1 || neqpm != 3) return -1;
+    if (g->nblocks < 1 || neqpm != 3) return -1;
     double t0 = now();
     int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
     size_t G = (size_t)NX * NY * NZ;
@@ -1291,25 +1401,12 @@ int orc_vpm2(const orc_params *p, const orc_geom *g, int neqpm, const double *xp
     double *rhs = calloc(G * nr, sizeof(double));
     double *sol = calloc(G * neqpm, sizeof(double));
     orc_project(g->xbound, g->dpm, g->nn, g->nn_bl, xp, qp, nvr, nr, p->interf_iproj, p->idvpm, rhs);
-    /* rhsscat_3d (vpm_mpi.f90:199-238): block = global window + ncoarse halo of zeros */
-    const int32_t *nnbl = g->nnbl, *bl = g->nnbl_bl;
-    size_t B = (size_t)nnbl[0] * nnbl[1] * nnbl[2];
-    double *rhs_bl = calloc(B * neqpm, sizeof(double)), *sol_bl = calloc(B * neqpm, sizeof(double));
-    int ixs = g->nn_bl[0], jxs = g->nn_bl[1], kxs = g->nn_bl[2];
-    for (int k = bl[2]; k <= bl[5]; k++)
-        for (int j = bl[1]; j <= bl[4]; j++)
-            for (int i = bl[0]; i <= bl[3]; i++)
-                for (int c = 0; c < neqpm; c++)
-                    rhs_bl[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])] =
-                        rhs[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, nr, NX, NY)];
-    orc_yaps3d(p, g, neqpm, rhs_bl, sol_bl);
-    /* solget_3d (vpm_mpi.f90:240-328) */
-    for (int k = bl[2]; k <= bl[5]; k++)
-        for (int j = bl[1]; j <= bl[4]; j++)
-            for (int i = bl[0]; i <= bl[3]; i++)
-                for (int c = 0; c < neqpm; c++)
-                    sol[IX(c, ixs + (i - bl[0]) - 1, jxs + (j - bl[1]) - 1, kxs + (k - bl[2]) - 1, neqpm, NX, NY)] =
-                        sol_bl[IX(c, i - 1, j - 1, k - 1, neqpm, nnbl[0], nnbl[1])];
+    size_t boff[ORC_MAXB + 1], tot = 0;
+    for (int b = 0; b < g->nblocks; b++) tot += (size_t)g->nnbl[3 * b] * g->nnbl[3 * b + 1] * g->nnbl[3 * b + 2] * neqpm;
+    double *rhs_bl = calloc(tot, sizeof(double)), *sol_bl = calloc(tot, sizeof(double));
+    rhsscat(g, neqpm, rhs, nr, rhs_bl, boff);
+    if (orc_yaps3d(p, g, neqpm, rhs_bl, sol_bl) != 0) { free(rhs_bl); free(sol_bl); free(rhs); free(sol); return -1; }
+    solget(g, neqpm, sol_bl, boff, sol);
     free(rhs_bl); free(sol_bl);
     if (psi_out) memcpy(psi_out, sol, sizeof(double) * G * neqpm);
     double *vel = calloc(3 * G, sizeof(double));
@@ -1357,7 +1454,7 @@ void orc_diffuse(const orc_geom *g, int neqpm, double ni, const double *rhs, dou
 int orc_vpm45(const orc_params *p, const orc_geom *g, int neqpm, int mode, double ni, const double *xp,
               const double *qp, double *up, double *gp, int64_t nvr, const double *vel, double *sol_out) {
     if (nvr == 0) return 0;
-    if (g->nblocks != 1 || neqpm != 3 || (mode != 4 && mode != 5)) return -1;
+    if (neqpm != 3 || (mode != 4 && mode != 5)) return -1;
     int NX = g->nn[0], NY = g->nn[1], NZ = g->nn[2];
     size_t G = (size_t)NX * NY * NZ;
     int nr = neqpm + 1;

@@ -19,21 +19,26 @@ def rel_l2(a, b):
     return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
 
 
-def test_fortran_driver_matches_oracle(oracle, tmp_path):
+@pytest.mark.parametrize("h,sigma,opts", [
+    (0.125, 0.2, {}),
+    # the shipped pm.inp's solver options: 6 x 1 x 1 yaps blocks, tree boundary sums, 3 levels
+    (1 / 16, 0.1, dict(blocks=(6, 1, 1), iyntree=1, ilevmax=3)),
+])
+def test_fortran_driver_matches_oracle(oracle, tmp_path, h, sigma, opts):
     assert os.path.exists(DRIVER), "run python -m vpm_free_amd.build first"
-    h, dt, nsteps = 0.125, 0.01, 2
-    xp, qp = cases.single_ring(h=h, sigma=0.2)
+    dt, nsteps = 0.01, 2
+    xp, qp = cases.single_ring(h=h, sigma=sigma)
     vref = h ** 3
-    io.write_pm_inp(tmp_path / "pm.inp", h)
+    io.write_pm_inp(tmp_path / "pm.inp", h, **opts)
     io.write_particles_bin(tmp_path / "particles.bin", xp, -qp[:, :3] / vref, vref)
-    print("case: fortran ring_driver", nsteps, "steps", flush=True)
+    print("case: fortran ring_driver", nsteps, "steps", opts, flush=True)
     r = subprocess.run([DRIVER, str(nsteps), str(dt)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     # oracle: same sequence
     xr, w, vr = io.read_particles_bin(tmp_path / "particles.bin")
     q = io.particles_to_qp(xr, w, vr)
-    p = oracle.Params.make(h)
+    p = oracle.Params.make(h, **opts)
     g = oracle.define_sizes(p, xr, ntime=0)
     x, q = oracle.remesh(p, g, xr, q)
     first = None

@@ -301,6 +301,39 @@ def test_vpm_mode2_panels(vpmlib, oracle, h, sigma, tree):
         assert v <= TOL, (k, v)
 
 
+@pytest.mark.parametrize("h,sigma,blocks,tree,ibctyp", [
+    (0.125, 0.2, (2, 1, 1), 0, 2), (0.125, 0.2, (1, 2, 1), 0, 2), (0.125, 0.2, (2, 2, 2), 0, 2),
+    (0.125, 0.2, (3, 2, 1), 1, 2), (0.125, 0.2, (2, 1, 2), 0, 3), (1 / 16, 0.1, (6, 1, 1), 1, 2)])
+def test_vpm_mode2_blocks(vpmlib, oracle, h, sigma, blocks, tree, ibctyp):
+    """vpm(2) with NBI x NBJ x NBK yaps blocks on one device (rhsscat_3d, every block's fine
+    solve, the coarse problem from all blocks' samples, interp_stencil with the neighbours'
+    fine solutions, solget_3d) against the oracle; (6, 1, 1) with tree = 1, 3 levels is the
+    shipped pm.inp's combination."""
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 blocks %s h=%g tree=%d ibctyp=%d n=%d" % (blocks, h, tree, ibctyp, xp.shape[0]))
+    kw = dict(blocks=blocks, iyntree=tree, ilevmax=3, ibctyp=ibctyp)
+    po = oracle.Params.make(h, **kw)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    with V.VPM(V.Params.make(h, **kw)) as c:
+        up, gp = np.zeros_like(xp), np.zeros_like(xp)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+        assert bytes(c.geometry) == bytes(go)
+        c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+        errs = {
+            "rhs": rel_l2(c.get_grid(0), f["rhs"]),
+            "psi": rel_l2(c.get_grid(1), f["psi"]),
+            "vel": rel_l2(c.get_grid(3), f["vel"]),
+            "sol": rel_l2(c.get_grid(2), f["sol"]),
+            "up": rel_l2(up, up_r),
+            "gp": rel_l2(gp, gp_r),
+        }
+    say("  rel L2:", errs)
+    for k, v in errs.items():
+        assert v <= TOL, (k, v)
+
+
 @pytest.mark.parametrize("nn", [(145, 40, 44), (289, 21, 19)])
 def test_solve_many_batches(ctx, oracle, nn):
     """Windows with more line batches than resident work-groups (every work-group loops

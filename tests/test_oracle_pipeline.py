@@ -190,6 +190,80 @@ def hill(pts, a=1.0, U=1.0):
     return w, u, st
 
 
+def _yaps_blocks(oracle, p, g, rhs):
+    """rhsscat_3d into every block, orc_yaps3d over all blocks; the blocks' solutions."""
+    nb = g.nblocks
+    dims = [tuple(g.nnbl[3 * b:3 * b + 3]) for b in range(nb)]
+    bls = [tuple(g.nnbl_bl[6 * b:6 * b + 6]) for b in range(nb)]
+    rb = []
+    for b in range(nb):
+        nx, ny, nz = dims[b]
+        bl = bls[b]
+        bi, bj, bk = b % g.nbi + 1, (b // g.nbi) % g.nbj + 1, b // (g.nbi * g.nbj) + 1
+        x0 = (bi - 1) * (bl[3] - bl[0]) + g.nn_bl[0] - 1
+        y0 = (bj - 1) * (bl[4] - bl[1]) + g.nn_bl[1] - 1
+        z0 = (bk - 1) * (bl[5] - bl[2]) + g.nn_bl[2] - 1
+        a = np.zeros((nz, ny, nx, 3))
+        a[bl[2] - 1:bl[5], bl[1] - 1:bl[4], bl[0] - 1:bl[3]] = \
+            rhs[z0:z0 + bl[5] - bl[2] + 1, y0:y0 + bl[4] - bl[1] + 1, x0:x0 + bl[3] - bl[0] + 1, :3]
+        if bi > 1:
+            a[:, :, bl[0] - 1] = 0
+        if bj > 1:
+            a[:, bl[1] - 1, :] = 0
+        if bk > 1:
+            a[bl[2] - 1] = 0
+        rb.append(a)
+    rflat = np.ascontiguousarray(np.concatenate([a.ravel() for a in rb]))
+    sflat = np.zeros_like(rflat)
+    assert oracle.lib().orc_yaps3d(C.byref(p), C.byref(g), 3, oracle._d(rflat), oracle._d(sflat)) == 0
+    out, off = [], 0
+    for b in range(nb):
+        nx, ny, nz = dims[b]
+        out.append(sflat[off:off + nx * ny * nz * 3].reshape(nz, ny, nx, 3))
+        off += nx * ny * nz * 3
+    return out, bls
+
+
+@pytest.mark.parametrize("blocks", [(2, 1, 1), (2, 2, 1), (3, 1, 2)])
+def test_yaps_blocks_agree_on_shared_faces(oracle, blocks):
+    """Several yaps blocks: two blocks that share an inner-box face reach its nodes by
+    different routes (own fine solution + the neighbour's through BBound, the neighbour's
+    coarse samples removed) and must give the same values; a wrong neighbour offset or a
+    one-sided term breaks the symmetry.  Against one block, the several-block answer differs
+    only by the coarse correction's accuracy."""
+    h = 0.125
+    xp, qp = cases.single_ring(h=h, sigma=0.2)
+    p = oracle.Params.make(h, blocks=blocks)
+    g = oracle.define_sizes(p, xp)
+    rhs = oracle.project(g.xbound, g.dpm, g.nn, g.nn_bl, xp, qp)
+    sols, bls = _yaps_blocks(oracle, p, g, rhs)
+    pairs = 0
+    for b in range(g.nblocks):
+        bi, bj, bk = b % g.nbi, (b // g.nbi) % g.nbj, b // (g.nbi * g.nbj)
+        A, bl = sols[b], bls[b]
+        if bi + 1 < g.nbi:  # x neighbour
+            Bv, b2 = sols[b + 1], bls[b + 1]
+            fa, fb = A[bl[2] - 1:bl[5], bl[1] - 1:bl[4], bl[3] - 1], Bv[b2[2] - 1:b2[5], b2[1] - 1:b2[4], b2[0] - 1]
+            assert np.linalg.norm(fa - fb) <= 1e-14 * np.linalg.norm(fa)
+            pairs += 1
+        if bj + 1 < g.nbj:  # y neighbour
+            Bv, b2 = sols[b + g.nbi], bls[b + g.nbi]
+            fa, fb = A[bl[2] - 1:bl[5], bl[4] - 1, bl[0] - 1:bl[3]], Bv[b2[2] - 1:b2[5], b2[1] - 1, b2[0] - 1:b2[3]]
+            assert np.linalg.norm(fa - fb) <= 1e-14 * np.linalg.norm(fa)
+            pairs += 1
+        if bk + 1 < g.nbk:  # z neighbour
+            Bv, b2 = sols[b + g.nbi * g.nbj], bls[b + g.nbi * g.nbj]
+            fa, fb = A[bl[5] - 1, bl[1] - 1:bl[4], bl[0] - 1:bl[3]], Bv[b2[2] - 1, b2[1] - 1:b2[4], b2[0] - 1:b2[3]]
+            assert np.linalg.norm(fa - fb) <= 1e-14 * np.linalg.norm(fa)
+            pairs += 1
+    assert pairs >= 1
+    up, gp = oracle.vpm2(p, g, xp, qp)
+    p1 = oracle.Params.make(h)
+    up1, gp1 = oracle.vpm2(p1, oracle.define_sizes(p1, xp), xp, qp)
+    err = np.linalg.norm(up - up1) / np.linalg.norm(up1)
+    assert 1e-6 < err < 5e-2
+
+
 def test_vpm2_hills_vortex(oracle):
     h = 1 / 16
     pts = cases.cell_centres(-1.0, 1.0, h)

@@ -148,9 +148,11 @@ int coarse_sample(Ctx *c, int ncomp, const int ns[3], int nc, const int *cmap_de
     return VPMX_OK;
 }
 
-// rhs(i,j,k) for lo < i,j,k < hi (1-based, footprint interior); 0 elsewhere.
+// rhs(i,j,k) for lo < i,j,k < hi (1-based, footprint interior); 0 elsewhere.  accumulate:
+// rhs += the Laplacian on the footprint interior, other nodes untouched (the sum over
+// blocks of yaps3d.f90:190-195).
 __global__ void k_coarse_laplacian(int ncomp, int nx, int ny, int nz, int lx, int ly, int lz, int hx, int hy, int hz,
-                                   double c0, double c1, double c2, GView s, GView rhs) {
+                                   double c0, double c1, double c2, GView s, GView rhs, int accumulate) {
 #pragma clang fp contract(off)
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nx * ny * nz) return;
@@ -167,15 +169,16 @@ __global__ void k_coarse_laplacian(int ncomp, int nx, int ny, int nz, int lx, in
             double az = ((S[s.at(i - 1, j - 1, k)] - 2 * sc) + S[s.at(i - 1, j - 1, k - 2)]) / c2;
             v = 0.0 + ((ax + ay) + az);
         }
-        rhs.p[q * rhs.cs + o] = v;
+        if (!accumulate) rhs.p[q * rhs.cs + o] = v;
+        else if (in) rhs.p[q * rhs.cs + o] = rhs.p[q * rhs.cs + o] + v;
     }
 }
 int coarse_laplacian(Ctx *c, int ncomp, const int cn[3], const int lo[3], const int hi[3], const double hc[3],
-                     GView samp, GView rhs) {
+                     GView samp, GView rhs, bool accumulate) {
     int64_t nt = (int64_t)cn[0] * cn[1] * cn[2];
     hipLaunchKernelGGL(k_coarse_laplacian, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, ncomp, cn[0],
                        cn[1], cn[2], lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], hc[0] * hc[0], hc[1] * hc[1],
-                       hc[2] * hc[2], samp, rhs);
+                       hc[2] * hc[2], samp, rhs, accumulate ? 1 : 0);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
@@ -247,6 +250,174 @@ int face_interp(Ctx *c, int ncomp, const InterpArgs &a, GView csol, GView samp, 
     int64_t nface = 2 * (ny * nz + (nx - 2) * nz + (nx - 2) * (ny - 2));
     hipLaunchKernelGGL(k_face_interp, dim3((unsigned)((nface + 255) / 256)), dim3(256), 0, c->stream, ncomp, a, nface,
                        csol, samp, sol);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// ---------------------------------------------------------------- several yaps blocks
+// Node t of the six faces of an inner box [XS..XF] x [YS..YF] x [ZS..ZF] (1-based), in the
+// order of yaps3d.f90:320-395: x faces full, y faces without the x edges, z faces without
+// both.
+__device__ __forceinline__ void inner_face_node(const int nbl[6], int64_t t, int *i, int *j, int *k) {
+    const int XS = nbl[0], YS = nbl[1], ZS = nbl[2], XF = nbl[3], YF = nbl[4], ZF = nbl[5];
+    const int64_t f12 = (int64_t)(YF - YS + 1) * (ZF - ZS + 1), f34 = (int64_t)(XF - XS - 1) * (ZF - ZS + 1);
+    if (t < 2 * f12) {
+        int f = (int)(t / f12); int64_t r = t % f12;
+        *i = f ? XF : XS; *j = YS + (int)(r % (YF - YS + 1)); *k = ZS + (int)(r / (YF - YS + 1));
+    } else if (t < 2 * (f12 + f34)) {
+        int64_t u = t - 2 * f12; int f = (int)(u / f34); int64_t r = u % f34;
+        *j = f ? YF : YS; *i = XS + 1 + (int)(r % (XF - XS - 1)); *k = ZS + (int)(r / (XF - XS - 1));
+    } else {
+        int64_t u = t - 2 * (f12 + f34); int64_t f56 = (int64_t)(XF - XS - 1) * (YF - YS - 1);
+        int f = (int)(u / f56); int64_t r = u % f56;
+        *k = f ? ZF : ZS; *i = XS + 1 + (int)(r % (XF - XS - 1)); *j = YS + 1 + (int)(r / (XF - XS - 1));
+    }
+}
+
+// interp_stencil (yaps3d.f90:1914-1983) for the face nodes of every block in one launch.
+// Block b's arrays: fine + b * bstride (3 components of B nodes, SoA), samp + b * sstride
+// (3 x coarse grid).  fine holds every block's infinite-domain solution as it is before
+// this step; the new face values go to out (read nothing that this launch writes).
+__global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock *__restrict__ blk, GView csol,
+                                                         const double *__restrict__ samp,
+                                                         const double *__restrict__ fine, double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nface * a.nblocks) return;
+    const int b = (int)(t / a.nface);
+    const int64_t tl = t % a.nface;
+    const MbBlock &B = blk[b];
+    int i, j, k;
+    inner_face_node(B.nbl, tl, &i, &j, &k);
+    const double X0 = B.xbl[0] + (double)(i - 1) * a.h[0];
+    const double X1 = B.xbl[1] + (double)(j - 1) * a.h[1];
+    const double X2 = B.xbl[2] + (double)(k - 1) * a.h[2];
+    const int in = (int)round((X0 - a.xc[0]) / a.hc[0]) + 1;
+    const int jn = (int)round((X1 - a.xc[1]) / a.hc[1]) + 1;
+    const int kn = (int)round((X2 - a.xc[2]) / a.hc[2]) + 1;
+    // nnb: the block's sampled footprint lacks a corner of the 3x3x3 coarse stencil
+    uint64_t nnb = 0;
+    for (int bc = 0; bc < a.nblocks; bc++) {
+        const MbBlock &C = blk[bc];
+        const bool lo = in - 1 >= C.fp_lo[0] && in - 1 <= C.fp_hi[0] && jn - 1 >= C.fp_lo[1] && jn - 1 <= C.fp_hi[1] &&
+                        kn - 1 >= C.fp_lo[2] && kn - 1 <= C.fp_hi[2];
+        const bool hi = in + 1 >= C.fp_lo[0] && in + 1 <= C.fp_hi[0] && jn + 1 >= C.fp_lo[1] && jn + 1 <= C.fp_hi[1] &&
+                        kn + 1 >= C.fp_lo[2] && kn + 1 <= C.fp_hi[2];
+        if (!lo || !hi) nnb |= (uint64_t)1 << bc;
+    }
+    const int64_t B3 = (int64_t)B.n[0] * B.n[1] * B.n[2];
+    const int64_t o = ((int64_t)(k - 1) * B.n[1] + (j - 1)) * B.n[0] + (i - 1);
+    const double *own = fine + (int64_t)b * a.bstride;
+    double acc[3] = {own[o], own[B3 + o], own[2 * B3 + o]};
+    for (int kc = kn - 1; kc <= kn + 1; kc++)
+        for (int jc = jn - 1; jc <= jn + 1; jc++)
+            for (int ic = in - 1; ic <= in + 1; ic++) {
+                const double xc = (X0 - a.xc[0] - (double)(ic - 1) * a.hc[0]) / a.hc[0];
+                const double yc = (X1 - a.xc[1] - (double)(jc - 1) * a.hc[1]) / a.hc[1];
+                const double zc = (X2 - a.xc[2] - (double)(kc - 1) * a.hc[2]) / a.hc[2];
+                const double fw = tsc(xc) * tsc(yc) * tsc(zc);
+                const bool ok = ic >= 1 && ic <= a.cn[0] && jc >= 1 && jc <= a.cn[1] && kc >= 1 && kc <= a.cn[2];
+                const int64_t oc = ok ? csol.at(ic - 1, jc - 1, kc - 1) : 0;
+                for (int q = 0; q < 3; q++) {
+                    double add_sample = 0.0, add = 0.0;
+                    for (int bc = 0; bc < a.nblocks; bc++) {
+                        const double sm = ok ? samp[(int64_t)bc * a.sstride + q * csol.cs + oc] : 0.0;
+                        add_sample = sm + add_sample;
+                        if ((nnb >> bc) & 1) add = sm + add;
+                    }
+                    const double cs = ok ? csol.p[q * csol.cs + oc] : 0.0;
+                    acc[q] = acc[q] + fw * ((cs - add_sample) + add);
+                }
+            }
+    // neighbours whose footprint covers the stencil: their fine solution at the node (BBound)
+    for (int kb = B.bk - 1; kb <= B.bk + 1; kb++)
+        for (int jb = B.bj - 1; jb <= B.bj + 1; jb++)
+            for (int ib = B.bi - 1; ib <= B.bi + 1; ib++) {
+                if (ib < 1 || ib > a.nbi || jb < 1 || jb > a.nbj || kb < 1 || kb > a.nbk) continue;
+                const int bc = (kb - 1) * a.nbi * a.nbj + (jb - 1) * a.nbi + ib - 1;
+                if (((nnb >> bc) & 1) || bc == b) continue;
+                const MbBlock &C = blk[bc];
+                const int ii = i + (int)round((B.xbl[0] - C.xbl[0]) / a.h[0]);
+                const int jj = j + (int)round((B.xbl[1] - C.xbl[1]) / a.h[1]);
+                const int kk = k + (int)round((B.xbl[2] - C.xbl[2]) / a.h[2]);
+                if (ii < 1 || ii > C.n[0] || jj < 1 || jj > C.n[1] || kk < 1 || kk > C.n[2]) continue;
+                const double *fc = fine + (int64_t)bc * a.bstride;
+                const int64_t C3 = (int64_t)C.n[0] * C.n[1] * C.n[2];
+                const int64_t on = ((int64_t)(kk - 1) * C.n[1] + (jj - 1)) * C.n[0] + (ii - 1);
+                for (int q = 0; q < 3; q++) acc[q] = acc[q] + fc[q * C3 + on];
+            }
+    for (int q = 0; q < 3; q++) out[3 * t + q] = acc[q];
+}
+
+__global__ void k_face_store_mb(MbArgs a, const MbBlock *__restrict__ blk, const double *__restrict__ in_vals,
+                                double *__restrict__ sol) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nface * a.nblocks) return;
+    const int b = (int)(t / a.nface);
+    const MbBlock &B = blk[b];
+    int i, j, k;
+    inner_face_node(B.nbl, t % a.nface, &i, &j, &k);
+    const int64_t B3 = (int64_t)B.n[0] * B.n[1] * B.n[2];
+    const int64_t o = ((int64_t)(k - 1) * B.n[1] + (j - 1)) * B.n[0] + (i - 1);
+    double *sb = sol + (int64_t)b * a.bstride;
+    for (int q = 0; q < 3; q++) sb[q * B3 + o] = in_vals[3 * t + q];
+}
+
+int face_interp_blocks(Ctx *c, const MbArgs &a, const MbBlock *blk_dev, GView csol, const double *samp, double *sol,
+                       double *tmp) {
+    const int64_t nt = a.nface * a.nblocks;
+    hipLaunchKernelGGL(k_face_interp_mb, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, a, blk_dev, csol,
+                       samp, sol, tmp);
+    hipLaunchKernelGGL(k_face_store_mb, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, a, blk_dev, tmp,
+                       sol);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+// rhsscat_3d (vpm_mpi.f90:199-238): block = global window (global node g0 + block inner index
+// offset) on the inner box, 0 elsewhere and on the min faces listed in zmin.
+__global__ void k_rhsscat(int nx, int ny, int nz, Box6 nbl_, int gx, int gy, int gz, int zx, int zy, int zz,
+                          GView glob, GView blk) {
+    const int *nbl = nbl_.v;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    const int i = (int)(t % nx) + 1, j = (int)((t / nx) % ny) + 1, k = (int)(t / ((int64_t)nx * ny)) + 1;
+    const bool in = i >= nbl[0] && i <= nbl[3] && j >= nbl[1] && j <= nbl[4] && k >= nbl[2] && k <= nbl[5];
+    const bool zero = !in || (zx && i == nbl[0]) || (zy && j == nbl[1]) || (zz && k == nbl[2]);
+    const int64_t o = blk.at(i - 1, j - 1, k - 1);
+    const int64_t og = zero ? 0 : glob.at(gx + (i - nbl[0]) - 1, gy + (j - nbl[1]) - 1, gz + (k - nbl[2]) - 1);
+    for (int q = 0; q < 3; q++) blk.p[q * blk.cs + o] = zero ? 0.0 : glob.p[q * glob.cs + og];
+}
+
+// solget_3d (vpm_mpi.f90:240-328) for one block: its inner box into the global grid.
+__global__ void k_solget(int wx, int wy, int wz, Box6 nbl_, int gx, int gy, int gz, GView blk, GView glob) {
+    const int *nbl = nbl_.v;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)wx * wy * wz) return;
+    const int a = (int)(t % wx), b = (int)((t / wx) % wy), cc = (int)(t / ((int64_t)wx * wy));
+    const int64_t o = blk.at(nbl[0] - 1 + a, nbl[1] - 1 + b, nbl[2] - 1 + cc);
+    const int64_t og = glob.at(gx - 1 + a, gy - 1 + b, gz - 1 + cc);
+    for (int q = 0; q < 3; q++) glob.p[q * glob.cs + og] = blk.p[q * blk.cs + o];
+}
+
+int rhsscat_block(Ctx *c, const int n[3], const int nbl[6], const int g0[3], const int zmin[3], GView glob,
+                  GView blk) {
+    Box6 bx;
+    for (int d = 0; d < 6; d++) bx.v[d] = nbl[d];
+    const int64_t nt = (int64_t)n[0] * n[1] * n[2];
+    hipLaunchKernelGGL(k_rhsscat, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2], bx,
+                       g0[0], g0[1], g0[2], zmin[0], zmin[1], zmin[2], glob, blk);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+int solget_block(Ctx *c, const int nbl[6], const int g0[3], GView blk, GView glob) {
+    Box6 bx;
+    for (int d = 0; d < 6; d++) bx.v[d] = nbl[d];
+    const int w[3] = {nbl[3] - nbl[0] + 1, nbl[4] - nbl[1] + 1, nbl[5] - nbl[2] + 1};
+    const int64_t nt = (int64_t)w[0] * w[1] * w[2];
+    hipLaunchKernelGGL(k_solget, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, w[0], w[1], w[2], bx,
+                       g0[0], g0[1], g0[2], blk, glob);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

@@ -205,11 +205,165 @@ static int yaps_single(Ctx *c, bool *used) {
     return VPMX_OK;
 }
 
+// ---------------------------------------------------------------- yaps (several blocks)
+// yaps3d.f90:1-453 with BLOCKS = NBI*NBJ*NBK blocks on this device, the work of every rank
+// in turn: all fine infinite-domain solves; the coarse problem from every block's samples
+// (Laplacians summed in block order); the coarse solve; interp_stencil on every block's
+// inner-box faces in one launch, reading every block's fine solution before any face is
+// written (a block's faces are its neighbours' BBound nodes); the inner-box solves.
+// c->rhs_bl holds the blocks' RHS back to back (rhsscat_3d done by the caller).
+static int yaps_multi(Ctx *c, bool *used) {
+    int st = VPMX_OK;
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    const int NB = G.nblocks;
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    for (int b = 1; b < NB; b++)
+        for (int d = 0; d < 3; d++)
+            if (G.nnbl[3 * b + d] != nb[d]) return fail(c, VPMX_ERR_UNSUP, "yaps blocks of different sizes");
+    const int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
+    double *rhs_bl = (double *)c->rhs_bl.p;
+    double *sol_bl = (double *)ensure(c, c->sol_bl, sizeof(double) * 3 * (size_t)B * NB, &st); if (st) return st;
+    auto bview = [&](double *base, int b) { return view(base + (int64_t)b * 3 * B, nb); };
+    const bool tree = P.iyntree == 1;
+    if (tree && G.ilevmax < 1)
+        return fail(c, VPMX_ERR_UNSUP, "tree = 1 needs at least one level on the coarse grid (ilevmax >= 1)");
+    // 1. fine infinite-domain solves
+    mark(c, ST_FINE, 0);
+    VPMX_TRY(fill(c, sol_bl, 3 * B * NB, 0.0));
+    c->dst_timed_m = nb[1] - 2;
+    for (int b = 0; b < NB; b++) {
+        Grid gb = make_grid(&G.nnbl[3 * b], &G.xbound_bl[6 * b], G.dpm);
+        VPMX_TRY(pmesh_inf(c, gb, bview(rhs_bl, b), bview(sol_bl, b), c->sol0, true, tree ? G.ilevmax : -1, P.ibctyp));
+    }
+    mark(c, ST_FINE, 1);
+    used[ST_FINE] = used[ST_BOUNDS] = true;
+    // 2. samples of every block, coarse RHS, coarse solve
+    mark(c, ST_COARSE, 0);
+    const int cn[3] = {G.nn_coarse[0], G.nn_coarse[1], G.nn_coarse[2]};
+    const int64_t GC = (int64_t)cn[0] * cn[1] * cn[2];
+    double *samp = (double *)ensure(c, c->samp, sizeof(double) * 3 * (size_t)GC * NB, &st); if (st) return st;
+    double *rhsc = (double *)ensure(c, c->rhs_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    double *solc = (double *)ensure(c, c->sol_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    const int ncs = P.ncoarse;
+    int ns[3];
+    for (int d = 0; d < 3; d++) ns[d] = (nb[d] - 1) / ncs + 1;
+    std::vector<int> cmap((size_t)(ns[0] + ns[1] + ns[2]) * NB);
+    std::vector<MbBlock> blk(NB);
+    for (int b = 0; b < NB; b++) {
+        MbBlock &M = blk[b];
+        for (int d = 0; d < 6; d++) M.nbl[d] = G.nnbl_bl[6 * b + d];
+        int off = 0;
+        for (int d = 0; d < 3; d++) {
+            M.n[d] = nb[d];
+            M.xbl[d] = G.xbound_bl[6 * b + d];
+            for (int a = 0; a < ns[d]; a++) {
+                const int i = 1 + a * ncs;
+                const double X = G.xbound_bl[6 * b + d] + (double)(i - 1) * G.dpm[d];
+                const int node = (int)std::lround((X - G.xbound_coarse[d]) / G.dpm_coarse[d]) + 1;
+                if (node < 1 || node > cn[d]) return fail(c, VPMX_ERR_STATE, "coarse sample outside the coarse grid");
+                if (a > 0 && node != cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + a - 1] + 1)
+                    return fail(c, VPMX_ERR_STATE, "block samples are not consecutive coarse nodes");
+                cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + a] = node;
+            }
+            M.fp_lo[d] = cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off];
+            M.fp_hi[d] = cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + ns[d] - 1];
+            off += ns[d];
+        }
+        M.bi = b % G.nbi + 1;
+        M.bj = (b / G.nbi) % G.nbj + 1;
+        M.bk = b / (G.nbi * G.nbj) + 1;
+    }
+    int *cmap_d = (int *)ensure(c, c->cmap, sizeof(int) * cmap.size(), &st); if (st) return st;
+    MbBlock *blk_d = (MbBlock *)ensure(c, c->mb_blocks, sizeof(MbBlock) * (size_t)NB, &st); if (st) return st;
+    VPMX_HIP(c, hipMemcpyAsync(cmap_d, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice, c->stream));
+    VPMX_HIP(c, hipMemcpyAsync(blk_d, blk.data(), sizeof(MbBlock) * (size_t)NB, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(fill(c, samp, 3 * GC * NB, 0.0));
+    VPMX_TRY(fill(c, rhsc, 3 * GC, 0.0));
+    auto cview = [&](double *base, int b) { return view(base + (int64_t)b * 3 * GC, cn); };
+    for (int b = 0; b < NB; b++) {
+        const int *cm = cmap_d + (size_t)(ns[0] + ns[1] + ns[2]) * b;
+        VPMX_TRY(coarse_sample(c, 3, ns, ncs, cm, bview(sol_bl, b), cview(samp, b), cn));
+        VPMX_TRY(coarse_laplacian(c, 3, cn, blk[b].fp_lo, blk[b].fp_hi, G.dpm_coarse, cview(samp, b), view(rhsc, cn),
+                                  true));
+    }
+    VPMX_TRY(fill(c, solc, 3 * GC, 0.0));
+    Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
+    VPMX_TRY(pmesh_inf(c, gc, view(rhsc, cn), view(solc, cn), c->sol0_c, false, tree ? G.ilevmax - 1 : -1, P.ibctyp));
+    mark(c, ST_COARSE, 1);
+    used[ST_COARSE] = true;
+    // 3. interp_stencil on every block's inner-box faces
+    mark(c, ST_BC, 0);
+    MbArgs a;
+    for (int d = 0; d < 3; d++) {
+        a.h[d] = G.dpm[d];
+        a.xc[d] = G.xbound_coarse[d];
+        a.hc[d] = G.dpm_coarse[d];
+        a.cn[d] = cn[d];
+    }
+    a.nbi = G.nbi;
+    a.nbj = G.nbj;
+    a.nbk = G.nbk;
+    a.nblocks = NB;
+    a.bstride = 3 * B;
+    a.sstride = 3 * GC;
+    const int *w0 = G.nnbl_bl;
+    const int64_t fx = w0[3] - w0[0] + 1, fy = w0[4] - w0[1] + 1, fz = w0[5] - w0[2] + 1;
+    a.nface = 2 * (fy * fz + (fx - 2) * fz + (fx - 2) * (fy - 2));
+    double *tmp = (double *)ensure(c, c->mb_tmp, sizeof(double) * 3 * (size_t)(a.nface * NB), &st); if (st) return st;
+    VPMX_TRY(face_interp_blocks(c, a, blk_d, view(solc, cn), samp, sol_bl, tmp));
+    mark(c, ST_BC, 1);
+    used[ST_BC] = true;
+    // 4. inner-box solves
+    mark(c, ST_FINAL, 0);
+    for (int b = 0; b < NB; b++) {
+        const int32_t *w = &G.nnbl_bl[6 * b];
+        int wn[3] = {w[3] - w[0] + 1, w[4] - w[1] + 1, w[5] - w[2] + 1};
+        double h[3];
+        window_h(&G.xbound_bl[6 * b], G.dpm, w, h);
+        GView rin = view(rhs_bl + (int64_t)b * 3 * B, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+        GView sin = view(sol_bl + (int64_t)b * 3 * B, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+        VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));  // host tables go out of scope
+    mark(c, ST_FINAL, 1);
+    used[ST_FINAL] = true;
+    return VPMX_OK;
+}
+
+// Global origin (1-based global node) of block b's inner box (rhsscat_3d / solget_3d).
+static void block_origin(const vpmx_geom &G, int b, int g0[3]) {
+    const int bi = b % G.nbi, bj = (b / G.nbi) % G.nbj, bk = b / (G.nbi * G.nbj);
+    const int32_t *w = &G.nnbl_bl[6 * b];
+    g0[0] = bi * (w[3] - w[0]) + G.nn_bl[0];
+    g0[1] = bj * (w[4] - w[1]) + G.nn_bl[1];
+    g0[2] = bk * (w[5] - w[2]) + G.nn_bl[2];
+}
+
+// The global RHS / SOL grids: the block arrays' inner box for one block, own grids otherwise.
+static GView global_rhs(Ctx *c) {
+    const vpmx_geom &G = c->geom;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    if (G.nblocks > 1) return view((double *)c->rhs_g.p, gn);
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    const int32_t *w = G.nnbl_bl;
+    return view((double *)c->rhs_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+}
+static GView global_psi(Ctx *c) {
+    const vpmx_geom &G = c->geom;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    if (G.nblocks > 1) return view((double *)c->sol_g.p, gn);
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    const int32_t *w = G.nnbl_bl;
+    return view((double *)c->sol_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+}
+
 static int check_ready(Ctx *c, int neqpm) {
     if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters: call vpmx_set_params or vpm mode 0 first");
     if (!c->geom.defined) return fail(c, VPMX_ERR_STATE, "no geometry: vpm(WhatToDo=0) must run first");
-    if (c->geom.nblocks != 1)
-        return fail(c, VPMX_ERR_UNSUP, "NBI*NBJ*NBK > 1 yaps blocks are not supported by this build");
+    if (c->geom.nblocks < 1 || c->geom.nblocks > VPMX_MAXB ||
+        c->geom.nbi * c->geom.nbj * c->geom.nbk != c->geom.nblocks)
+        return fail(c, VPMX_ERR_ARG, "NBI*NBJ*NBK must be 1..64 yaps blocks");
     if (neqpm != 3) return fail(c, VPMX_ERR_UNSUP, "neqpm must be 3");
     const vpmx_params &P = c->prm;
     if (P.interf_iproj < 2 || P.interf_iproj > 4)
@@ -237,28 +391,43 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
     c->last_pairs = 0.0;
     const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
     int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
-    double *rhs_bl = (double *)ensure(c, c->rhs_bl, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
-    // project_particles (vpm.f90:408-444) straight into the block layout of rhsscat_3d
+    const bool multi = G.nblocks > 1;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    double *rhs_bl = (double *)ensure(c, c->rhs_bl, sizeof(double) * 3 * (size_t)B * G.nblocks, &st); if (st) return st;
     mark(c, ST_PROJ, 0);
     Grid gg = make_grid(G.nn, G.xbound, G.dpm);
     VPMX_TRY(sort_particles(c, gg, xp, n, c->sort));
     const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
-    int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
-    VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo,
-                        K_P2M));
+    if (!multi) {
+        // project_particles (vpm.f90:408-444) straight into the block layout of rhsscat_3d
+        int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
+        VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo,
+                            K_P2M));
+    } else {
+        // the global grid, then rhsscat_3d into every block (vpm_mpi.f90:199-238)
+        double *rhs_g = (double *)ensure(c, c->rhs_g, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+        const int wlo[3] = {0, 0, 0};
+        VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_g, gn), gn, wlo,
+                            K_P2M));
+        for (int b = 0; b < G.nblocks; b++) {
+            int g0[3];
+            block_origin(G, b, g0);
+            const int zmin[3] = {b % G.nbi > 0, (b / G.nbi) % G.nbj > 0, b / (G.nbi * G.nbj) > 0};
+            VPMX_TRY(rhsscat_block(c, nb, &G.nnbl_bl[6 * b], g0, zmin, view(rhs_g, gn),
+                                   view(rhs_bl + (int64_t)b * 3 * B, nb)));
+        }
+    }
     mark(c, ST_PROJ, 1);
     used[ST_PROJ] = true;
     if (whattodo == 4 || whattodo == 5) {
         // no Poisson solve (vpm.f90:316-373): mode 4 stretching from the velocity grids of the
         // last mode 0/1/5 call, mode 5 diffusion of the projected vorticity
-        const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
-        const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
         const bool fresh = c->vel.bytes < sizeof(double) * 3 * (size_t)GG;
         double *vel = (double *)ensure(c, c->vel, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
         double *str = (double *)ensure(c, c->str, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
         if (fresh || whattodo == 5) VPMX_TRY(fill(c, vel, 3 * GG, 0.0));  // velvr*_pm = 0
-        const int32_t *w = G.nnbl_bl;
-        GView rg = view(rhs_bl, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+        GView rg = global_rhs(c);
         mark(c, ST_FD, 0);
         if (whattodo == 4) VPMX_TRY(stretch_kernel(c, gn, G.dpm, rg, view(vel, gn), view(str, gn)));
         else VPMX_TRY(diffuse_kernel(c, gn, G.dpm, ni, rg, view(str, gn)));
@@ -273,17 +442,26 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
         collect_times(c, used);
         return VPMX_OK;
     }
-    VPMX_TRY(yaps_single(c, used));
+    if (!multi) {
+        VPMX_TRY(yaps_single(c, used));
+    } else {
+        VPMX_TRY(yaps_multi(c, used));
+        // solget_3d: inner boxes into the global grid in block order
+        double *sol_g = (double *)ensure(c, c->sol_g, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+        for (int b = 0; b < G.nblocks; b++) {
+            int g0[3];
+            block_origin(G, b, g0);
+            VPMX_TRY(solget_block(c, &G.nnbl_bl[6 * b], g0, view((double *)c->sol_bl.p + (int64_t)b * 3 * B, nb),
+                                  view(sol_g, gn)));
+        }
+    }
     c->have_grids = true;
     if (whattodo == 3) { collect_times(c, used); return VPMX_OK; }
-    // calc_velocity_serial_3d on the global grid = inner box of the block arrays
-    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
-    int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    // calc_velocity_serial_3d on the global grid
     double *vel = (double *)ensure(c, c->vel, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
     double *str = (double *)ensure(c, c->str, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
-    const int32_t *w = G.nnbl_bl;
-    GView psi = view((double *)c->sol_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
-    GView rg = view(rhs_bl, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    GView psi = global_psi(c);
+    GView rg = global_rhs(c);
     mark(c, ST_FD, 0);
     VPMX_TRY(curl_kernel(c, gn, G.dpm, psi, view(vel, gn)));
     if (whattodo == 2) VPMX_TRY(stretch_kernel(c, gn, G.dpm, rg, view(vel, gn), view(str, gn)));
@@ -422,7 +600,7 @@ void vpmx_destroy(vpmx_ctx *p) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->xp, &c->qp, &c->up, &c->gp, &c->x0, &c->q0, &c->rhs_bl, &c->sol_bl, &c->sol0, &c->vel,
-                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->panel_a, &c->samp, &c->rhs_c, &c->sol_c,
+                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->panel_a, &c->rhs_g, &c->sol_g, &c->mb_blocks, &c->mb_tmp, &c->samp, &c->rhs_c, &c->sol_c,
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
@@ -528,9 +706,7 @@ int32_t vpmx_get_grid(vpmx_ctx *p, int32_t which, double *dst, int64_t capacity)
     if (!c->have_grids) return fail(c, VPMX_ERR_STATE, "no grids: run vpm mode 1, 2 or 3 first");
     const vpmx_geom &G = c->geom;
     const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
-    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
     int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
-    const int32_t *w = G.nnbl_bl;
     int st = VPMX_OK;
     int ncd = which == 0 ? c->neqpm + 1 : 3;
     if (capacity < ncd * GG) return fail(c, VPMX_ERR_ARG, "get_grid: destination too small");
@@ -538,9 +714,9 @@ int32_t vpmx_get_grid(vpmx_ctx *p, int32_t which, double *dst, int64_t capacity)
     if (st) return st;
     const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
     if (which == 0) {
-        VPMX_TRY(soa_to_aos(c, 3, 4, gn, view((double *)c->rhs_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb), DV, stage));
+        VPMX_TRY(soa_to_aos(c, 3, 4, gn, global_rhs(c), DV, stage));
     } else if (which == 1) {
-        VPMX_TRY(soa_to_aos(c, 3, 3, gn, view((double *)c->sol_bl.p, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb), 0.0, stage));
+        VPMX_TRY(soa_to_aos(c, 3, 3, gn, global_psi(c), 0.0, stage));
     } else if (which == 2) {
         if (!c->str.p) return fail(c, VPMX_ERR_STATE, "no stretching grid");
         VPMX_TRY(soa_to_aos(c, 3, 3, gn, view((double *)c->str.p, gn), 0.0, stage));

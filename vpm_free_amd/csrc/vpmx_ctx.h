@@ -56,6 +56,7 @@ struct Ctx : vpmx_ctx {
     std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N
     std::map<std::pair<int, double>, void *> eig_cache;  // DST-I eigenvalues per (m, h)
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;
+    DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
     DBuf stage_a, stage_b, stage_c;

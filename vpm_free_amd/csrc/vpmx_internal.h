@@ -127,7 +127,7 @@ int diffuse_kernel(Ctx *c, const int n[3], const double h[3], double vis, GView 
 int coarse_sample(Ctx *c, int ncomp, const int nsamp[3], int nc, const int *cmap_dev, GView fine, GView coarse,
                   const int cn[3]);
 int coarse_laplacian(Ctx *c, int ncomp, const int cn[3], const int lo[3], const int hi[3], const double hc[3],
-                     GView samp, GView rhs);
+                     GView samp, GView rhs, bool accumulate = false);
 struct InterpArgs {
     int nbl[6];           // inner box NNbl_bl (1-based)
     double xbl[3], h[3];  // block origin and fine spacing
@@ -136,6 +136,32 @@ struct InterpArgs {
     int fp_lo[3], fp_hi[3];  // footprint of the block on the coarse grid (1-based coarse index)
 };
 int face_interp(Ctx *c, int ncomp, const InterpArgs &a, GView coarse_sol, GView samp, GView sol_bl);
+// Several yaps blocks (all blocks' arrays back to back, `bstride` doubles apart).
+struct MbBlock {
+    int nbl[6];              // inner box NNbl_bl (1-based)
+    int n[3];                // block grid NNbl
+    int fp_lo[3], fp_hi[3];  // sampled footprint on the coarse grid (1-based, inclusive)
+    int bi, bj, bk;          // block position (1-based)
+    double xbl[3];           // block origin
+};
+struct MbArgs {
+    double h[3], xc[3], hc[3];
+    int cn[3];
+    int nbi, nbj, nbk, nblocks;
+    int64_t bstride;  // doubles per block in the fine arrays (3 x B)
+    int64_t sstride;  // doubles per block in the sample array (3 x coarse nodes)
+    int64_t nface;    // inner-face nodes per block (all blocks equal)
+};
+int face_interp_blocks(Ctx *c, const MbArgs &a, const MbBlock *blk_dev, GView csol, const double *samp, double *sol,
+                       double *tmp);
+struct Box6 {
+    int v[6];
+};
+// rhsscat_3d / solget_3d for one block: n block dims, nbl inner box, g0 global node of the
+// inner box's first node (1-based), zmin min faces to zero (block not first on that axis).
+int rhsscat_block(Ctx *c, const int n[3], const int nbl[6], const int g0[3], const int zmin[3], GView glob,
+                  GView blk);
+int solget_block(Ctx *c, const int nbl[6], const int g0[3], GView blk, GView glob);
 int soa_to_aos(Ctx *c, int ncomp_src, int ncomp_dst, const int n[3], GView src, double fill_last, double *dst);
 int aos_to_soa(Ctx *c, int ncomp, const int n[3], const double *src, GView dst);
 
