@@ -352,14 +352,115 @@ __global__ void __launch_bounds__(256) k_p2m_block(Grid g, const int *start, int
             }
 }
 
+template <typename T>
+__device__ __forceinline__ T m4p_t(T x) {
+#pragma clang fp contract(off)
+    const T a = fabs(x);
+    T r = T(0);
+    if (a < T(1)) r = T(1) - T(2.5) * (a * a) + T(1.5) * (a * a * a);
+    else if (a <= T(2)) r = T(0.5) * ((T(2) - a) * (T(2) - a)) * (T(1) - a);
+    return r;
+}
+
+// M4' projection, one thread per 4 consecutive x-nodes of the view.  Node i collects the
+// particles of cells i-2..i+1 (a particle in cell ci reaches nodes ci-1..ci+2), so the nodes
+// i0..i0+3 collect cells i0-2..i0+4 of each of the 16 (y, z) cell rows around them, and in
+// cell order the particles of such a run of cells are one contiguous range.  The 32 range
+// loads are issued together, and particles are fetched four at a time, so a thread waits on
+// memory a few times per row instead of once per cell and once per particle.  Per particle
+// t = (x - x0)/h once per axis and the node weights W(t - (i-1)) (M4' vanishes outside a
+// node's four cells, so no masks).  Every node sums its particles in row order (z outer,
+// y inner), cell and particle order within the row: deterministic, no atomics.
+// T: arithmetic of the weights and the sums (double: FP64 form; float: mixed form, the
+// offsets still formed in FP64).
+// Particles per cell row (the start table is too large for the caches; the row counts stay
+// in L2 and let the projection skip the start loads of empty rows).
+__global__ void k_row_counts(const int *__restrict__ start, int cnx, int64_t nrows, int *__restrict__ rowcnt) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nrows) rowcnt[r] = start[(r + 1) * cnx] - start[r * cnx];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_p2m_rows(Grid g, const int *__restrict__ start,
+                                                  const int *__restrict__ rowcnt, int cnx, int cny,
+                                                  const double *__restrict__ xs, const double *__restrict__ qs, double div,
+                                                  GView out, int vnx, int vny, int vnz, int wlo0, int wlo1, int wlo2) {
+#pragma clang fp contract(off)
+    const int nq = (vnx + 3) >> 2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nq * vny * vnz) return;
+    const int qa = (int)(t % nq), vb = (int)((t / nq) % vny), vc = (int)(t / ((int64_t)nq * vny));
+    const int a0 = 4 * qa;
+    const int i0 = a0 - wlo0 + 1, j = vb - wlo1 + 1, k = vc - wlo2 + 1;
+    const double ihx = 1.0 / g.h[0], ihy = 1.0 / g.h[1], ihz = 1.0 / g.h[2];
+    const double X = g.x0[0], Y = g.x0[1], Z = g.x0[2];
+    T acc[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int r = 0; r < 3; r++) acc[a][r] = T(0);
+    const int clo = i0 - 2 > -1 ? i0 - 2 : -1;
+    const int chi = i0 + 4 < g.n[0] + 1 ? i0 + 4 : g.n[0] + 1;
+    int p0[16], p1[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int cj = j - 2 + (r & 3), ck = k - 2 + (r >> 2);
+        bool ok = clo <= chi && cj >= -1 && cj <= g.n[1] + 1 && ck >= -1 && ck <= g.n[2] + 1;
+        const int64_t rid = ok ? (cj + 1) + (int64_t)cny * (ck + 1) : 0;
+        ok = ok && rowcnt[rid] > 0;
+        const int64_t row = ok ? (int64_t)cnx * rid : 0;
+        p0[r] = ok ? start[row + clo + 1] : 0;
+        p1[r] = ok ? start[row + chi + 2] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int jr = j - 1, kr = k - 1;
+        for (int s0 = p0[r]; s0 < p1[r]; s0 += 4) {
+            double px[4], py[4], pz[4], q0[4], q1[4], q2[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int64_t sl = s0 + u < p1[r] ? s0 + u : s0;
+                px[u] = xs[3 * sl]; py[u] = xs[3 * sl + 1]; pz[u] = xs[3 * sl + 2];
+                q0[u] = qs[3 * sl]; q1[u] = qs[3 * sl + 1]; q2[u] = qs[3 * sl + 2];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                if (s0 + u >= p1[r]) break;
+                const T fy = m4p_t<T>((T)((py[u] - Y) * ihy - (double)jr));
+                const T fz = m4p_t<T>((T)((pz[u] - Z) * ihz - (double)kr));
+                const double tx = (px[u] - X) * ihx - (double)(i0 - 1);
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    const T f = (m4p_t<T>((T)(tx - (double)a)) * fy) * fz;
+                    acc[a][0] = acc[a][0] + f * (T)q0[u];
+                    acc[a][1] = acc[a][1] + f * (T)q1[u];
+                    acc[a][2] = acc[a][2] + f * (T)q2[u];
+                }
+            }
+        }
+    }
+    const bool jk_in = j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const int va = a0 + a;
+        if (va >= vnx) continue;
+        const int i = i0 + a;
+        const bool in = jk_in && i >= 1 && i <= g.n[0];
+        const int64_t o = out.at(va, vb, vc);
+#pragma unroll
+        for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = in ? (double)acc[a][r] / div : 0.0;
+    }
+}
+
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
-               int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass) {
+               int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32) {
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
     hipEvent_t e0, e1;
     if (ncomp == 3 && itype == 4 && c->p2m_block) {
-        // cell-ordered copy of the particles, then one thread per 4x2x2 node block
+        // cell-ordered copy of the particles, then the node-run kernel (or one thread per
+        // 4x2x2 node block)
         constexpr int BX = 4, BY = 2, BZ = 2;
         int st = VPMX_OK;
         const int64_t n = c->sort_n;
@@ -370,6 +471,33 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         if (n > 0)
             hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
                                start + s.ncells, n, xp, qp, qstride, xs, qs);
+        if (c->p2m_rows) {
+            const int64_t nth = (int64_t)((vn[0] + 3) / 4) * vn[1] * vn[2];
+            const unsigned gr = (unsigned)((nth + 255) / 256);
+            const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
+            int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
+            if (st) return st;
+            hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
+                               s.cn[0], nrows, rowcnt);
+            const bool timed = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
+            if (f32) {
+                if (timed)
+                    hipExtLaunchKernelGGL(k_p2m_rows<float>, dim3(gr), dim3(256), 0, c->stream, e0, e1, 0, g, start,
+                                          rowcnt, s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+                else
+                    hipLaunchKernelGGL(k_p2m_rows<float>, dim3(gr), dim3(256), 0, c->stream, g, start, rowcnt, s.cn[0], s.cn[1],
+                                       xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+            } else {
+                if (timed)
+                    hipExtLaunchKernelGGL(k_p2m_rows<double>, dim3(gr), dim3(256), 0, c->stream, e0, e1, 0, g, start,
+                                          rowcnt, s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+                else
+                    hipLaunchKernelGGL(k_p2m_rows<double>, dim3(gr), dim3(256), 0, c->stream, g, start, rowcnt,
+                                       s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+            }
+            VPMX_HIP(c, hipGetLastError());
+            return VPMX_OK;
+        }
         const int64_t nblk = (int64_t)((vn[0] + BX - 1) / BX) * ((vn[1] + BY - 1) / BY) * ((vn[2] + BZ - 1) / BZ);
         const unsigned gb = (unsigned)((nblk + 255) / 256);
         if (kclass >= 0 && kernel_events(c, kclass, &e0, &e1))
@@ -444,11 +572,176 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
     gp[3 * p] = g0 * vol; gp[3 * p + 1] = g1 * vol; gp[3 * p + 2] = g2 * vol;
 }
 
-int m2p_gather(Ctx *c, const Grid &g, const SortState &, const double *xp, const double *qp, int qstride, int itype,
+// back_to_particles_3D itype 1 by cell tiles: a work-group owns M2P_TX x M2P_TY x M2P_TZ
+// cells, stages the velocity and stretching values of the (TX+3)(TY+3)(TZ+3) nodes its
+// particles can reach in LDS (node-major, 0 outside the grid), then each thread takes one
+// of the tile's particles (cell order) and sums its stencil from LDS in the order of k_m2p.
+constexpr int M2P_TX = 8, M2P_TY = 8, M2P_TZ = 4;
+constexpr int M2P_NX = M2P_TX + 3, M2P_NY = M2P_TY + 3, M2P_NZ = M2P_TZ + 3;
+constexpr int M2P_ROWS = M2P_TY * M2P_TZ;
+
+// Tiles holding particles: flag per tile, then (after an exclusive scan) their list.
+// One 32-lane group per tile, one lane per cell row (loads in flight together).
+__global__ void k_m2p_tile_flags(int ntiles, const int *__restrict__ start, int cnx, int cny, int cnz,
+                                 int *__restrict__ flag) {
+    static_assert(M2P_ROWS == 32, "one lane per row of a 32-lane group");
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int tile = (int)(gt >> 5), r = (int)(gt & 31);
+    int cnt = 0;
+    if (tile < ntiles) {
+        const int ntx = (cnx + M2P_TX - 1) / M2P_TX, nty = (cny + M2P_TY - 1) / M2P_TY;
+        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+        const int cx0 = tx * M2P_TX, cx1 = min(cx0 + M2P_TX, cnx);
+        const int cyy = ty * M2P_TY + r % M2P_TY, czz = tz * M2P_TZ + r / M2P_TY;
+        if (cyy < cny && czz < cnz) {
+            const int64_t row = (int64_t)cnx * (cyy + (int64_t)cny * czz);
+            cnt = start[row + cx1] - start[row + cx0];
+        }
+    }
+    // any row of the group non-empty (lanes 0-31 and 32-63 are two groups of one wave)
+    const unsigned long long any = __ballot(cnt > 0);
+    const unsigned half = (threadIdx.x & 32) ? (unsigned)(any >> 32) : (unsigned)(any & 0xffffffffULL);
+    if (r == 0 && tile < ntiles) flag[tile] = half != 0 ? 1 : 0;
+}
+__global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const int *__restrict__ pos,
+                                int *__restrict__ list) {
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile < ntiles && flag[tile]) list[pos[tile]] = tile;
+}
+
+__global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict__ start, const int *__restrict__ perm,
+                                                  int cnx, int cny, int cnz, const int *__restrict__ list,
+                                                  const int *__restrict__ nlist, const double *__restrict__ xp,
+                                                  const double *__restrict__ qp, int qstride, int itype, GView vel,
+                                                  GView sol, double *__restrict__ up, double *__restrict__ gp) {
+#pragma clang fp contract(off)
+    __shared__ double nodes[M2P_NX * M2P_NY * M2P_NZ * 6];
+    __shared__ int rs[M2P_ROWS + 1], rp[M2P_ROWS];
+    const int ntx = (cnx + M2P_TX - 1) / M2P_TX, nty = (cny + M2P_TY - 1) / M2P_TY;
+    const int ntot = *nlist;
+    for (int li = blockIdx.x; li < ntot; li += gridDim.x) {
+    const int tile = list[li];
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    // cell indices (stored index = cell + 1) of the tile: [cx0, cx0 + TX) clipped to the grid
+    const int cx0 = tx * M2P_TX, cy0 = ty * M2P_TY, cz0 = tz * M2P_TZ;
+    const int cx1 = min(cx0 + M2P_TX, cnx);
+    const int tid = threadIdx.x;
+    if (tid < M2P_ROWS) {
+        const int cyy = cy0 + tid % M2P_TY, czz = cz0 + tid / M2P_TY;
+        int cnt = 0, p0 = 0;
+        if (cyy < cny && czz < cnz) {
+            const int64_t row = (int64_t)cnx * (cyy + (int64_t)cny * czz);
+            p0 = start[row + cx0];
+            cnt = start[row + cx1] - p0;
+        }
+        rp[tid] = p0;
+        rs[tid + 1] = cnt;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        rs[0] = 0;
+        for (int r = 0; r < M2P_ROWS; r++) rs[r + 1] += rs[r];
+    }
+    __syncthreads();
+    const int total = rs[M2P_ROWS];
+    // node region: a particle of stored cell cs (inode = cs - 1, the sort's cell) reaches the
+    // nodes inode-1 .. inode+2 = cs-2 .. cs+1, so the tile's cells [cx0, cx0+TX) reach global
+    // nodes (1-based) cx0-2 .. cx0+TX: region index a <-> node cx0 - 2 + a, a in [0, TX+3)
+    const int i0 = cx0 - 2, j0 = cy0 - 2, k0 = cz0 - 2;
+    for (int e = tid; e < M2P_NX * M2P_NY * M2P_NZ; e += blockDim.x) {
+        const int a = e % M2P_NX, b = (e / M2P_NX) % M2P_NY, cc = e / (M2P_NX * M2P_NY);
+        const int i = i0 + a, j = j0 + b, k = k0 + cc;
+        const bool in = i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+        double v[6] = {0, 0, 0, 0, 0, 0};
+        if (in) {
+            const int64_t ov = vel.at(i - 1, j - 1, k - 1), os = sol.at(i - 1, j - 1, k - 1);
+            v[0] = vel.p[ov]; v[1] = vel.p[vel.cs + ov]; v[2] = vel.p[2 * vel.cs + ov];
+            v[3] = sol.p[os]; v[4] = sol.p[sol.cs + os]; v[5] = sol.p[2 * sol.cs + os];
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) nodes[6 * e + q] = v[q];
+    }
+    __syncthreads();
+    const int ips = stencil_ips(itype), ipf = stencil_ipf(itype);
+    const int w = ips + ipf + 1;
+    for (int t = tid; t < total; t += blockDim.x) {
+        int r = 0;
+        while (rs[r + 1] <= t) r++;
+        const int sidx = rp[r] + (t - rs[r]);
+        const int64_t p = perm[sidx];
+        const double px = xp[3 * p], py = xp[3 * p + 1], pz = xp[3 * p + 2];
+        double u0 = 0, u1 = 0, u2 = 0, g0 = 0, g1 = 0, g2 = 0;
+        const double txx = (px - g.x0[0]) / g.h[0], tyy = (py - g.x0[1]) / g.h[1], tzz = (pz - g.x0[2]) / g.h[2];
+        const int in = (int)txx + 1, jn = (int)tyy + 1, kn = (int)tzz + 1;
+        double fx[4], fy[4], fz[4];
+        for (int q = 0; q < 4; q++) {
+            fx[q] = proj_fun(itype, (px - g.x0[0] - (double)(in - ips + q - 1) * g.h[0]) / g.h[0]);
+            fy[q] = proj_fun(itype, (py - g.x0[1] - (double)(jn - ips + q - 1) * g.h[1]) / g.h[1]);
+            fz[q] = proj_fun(itype, (pz - g.x0[2] - (double)(kn - ips + q - 1) * g.h[2]) / g.h[2]);
+        }
+        for (int c = 0; c < w; c++) {
+            const int k = kn - ips + c;
+            for (int b = 0; b < w; b++) {
+                const int j = jn - ips + b;
+                for (int a = 0; a < w; a++) {
+                    const int i = in - ips + a;
+                    const double f = fx[a] * fy[b] * fz[c];
+                    const double *nd = nodes + 6 * ((i - i0) + M2P_NX * ((j - j0) + M2P_NY * (k - k0)));
+                    u0 = u0 + f * nd[0];
+                    u1 = u1 + f * nd[1];
+                    u2 = u2 + f * nd[2];
+                    g0 = g0 + f * nd[3];
+                    g1 = g1 + f * nd[4];
+                    g2 = g2 + f * nd[5];
+                }
+            }
+        }
+        const double vol = qp[(int64_t)qstride * p + (qstride - 1)];
+        up[3 * p] = u0; up[3 * p + 1] = u1; up[3 * p + 2] = u2;
+        gp[3 * p] = g0 * vol; gp[3 * p + 1] = g1 * vol; gp[3 * p + 2] = g2 * vol;
+    }
+    __syncthreads();  // the next tile reuses the LDS
+    }
+}
+
+// Particles outside the cell grid reach no node: UP = GP = 0.
+__global__ void k_m2p_outside(const int *__restrict__ keys, int64_t n, double *up, double *gp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n || keys[p] >= 0) return;
+    for (int q = 0; q < 3; q++) up[3 * p + q] = 0.0, gp[3 * p + q] = 0.0;
+}
+
+int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int itype,
                GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb) {
     if (n <= 0) return VPMX_OK;
     hipEvent_t e0, e1;
     const dim3 grid((unsigned)((n + 255) / 256));
+    if (itypeb == 1 && c->m2p_tile && c->sort_n == n && s.cn[0] == g.n[0] + 3 && s.cn[1] == g.n[1] + 3 &&
+        s.cn[2] == g.n[2] + 3) {
+        const int ntx = (s.cn[0] + M2P_TX - 1) / M2P_TX, nty = (s.cn[1] + M2P_TY - 1) / M2P_TY,
+                  ntz = (s.cn[2] + M2P_TZ - 1) / M2P_TZ;
+        const int tiles = ntx * nty * ntz;
+        const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p, *keys = (const int *)s.keys.p;
+        int st = VPMX_OK;
+        int *flag = (int *)ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+        int *pos = (int *)ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles + 1), &st); if (st) return st;
+        int *list = (int *)ensure(c, c->tile_list, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+        const unsigned gt = (unsigned)((tiles + 255) / 256);
+        hipLaunchKernelGGL(k_m2p_outside, grid, dim3(256), 0, c->stream, keys, n, up, gp);
+        hipLaunchKernelGGL(k_m2p_tile_flags, dim3((unsigned)(((int64_t)tiles * 32 + 255) / 256)), dim3(256), 0,
+                           c->stream, tiles, start, s.cn[0], s.cn[1], s.cn[2], flag);
+        VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
+        hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, tiles, flag, pos, list);
+        const unsigned wg = (unsigned)(4 * c->num_cu);  // persistent: 4 resident work-groups per CU (LDS-bound)
+        if (kernel_events(c, K_M2P, &e0, &e1))
+            hipExtLaunchKernelGGL(k_m2p_tile, dim3(wg), dim3(256), 0, c->stream, e0, e1, 0, g, start, perm, s.cn[0],
+                                  s.cn[1], s.cn[2], list, pos + tiles, xp, qp, qstride, itype, vel, sol, up, gp);
+        else
+            hipLaunchKernelGGL(k_m2p_tile, dim3(wg), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1],
+                               s.cn[2], list, pos + tiles, xp, qp, qstride, itype, vel, sol, up, gp);
+        VPMX_HIP(c, hipGetLastError());
+        return VPMX_OK;
+    }
     if (itypeb == 2)
         hipLaunchKernelGGL(k_m2p<2>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
     else if (kernel_events(c, K_M2P, &e0, &e1))

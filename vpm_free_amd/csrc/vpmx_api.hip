@@ -403,13 +403,13 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
         // project_particles (vpm.f90:408-444) straight into the block layout of rhsscat_3d
         int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
         VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo,
-                            K_P2M));
+                            K_P2M, c->precision == 1));
     } else {
         // the global grid, then rhsscat_3d into every block (vpm_mpi.f90:199-238)
         double *rhs_g = (double *)ensure(c, c->rhs_g, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
         const int wlo[3] = {0, 0, 0};
         VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_g, gn), gn, wlo,
-                            K_P2M));
+                            K_P2M, c->precision == 1));
         for (int b = 0; b < G.nblocks; b++) {
             int g0[3];
             block_origin(G, b, g0);
@@ -584,6 +584,9 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     if (wpc && atoi(wpc) > 0) c->dst_wg_per_cu = atoi(wpc);
     const char *pb = getenv("VPMX_P2M");
     if (pb && std::string(pb) == "node") c->p2m_block = false;
+    if (pb && std::string(pb) == "block") c->p2m_rows = false;
+    const char *mb = getenv("VPMX_M2P");
+    if (mb && std::string(mb) == "particle") c->m2p_tile = false;
     const char *pm = getenv("VPMX_PRECISION");
     if (pm && std::string(pm) == "mixed") c->precision = 1;
     if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
@@ -600,7 +603,7 @@ void vpmx_destroy(vpmx_ctx *p) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     DBuf *bufs[] = {&c->xp, &c->qp, &c->up, &c->gp, &c->x0, &c->q0, &c->rhs_bl, &c->sol_bl, &c->sol0, &c->vel,
-                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->panel_a, &c->rhs_g, &c->sol_g, &c->mb_blocks, &c->mb_tmp, &c->samp, &c->rhs_c, &c->sol_c,
+                    &c->str, &c->src_sig, &c->src_cxyz, &c->src_w, &c->bs_part, &c->lat_pairs, &c->lat_pairs_c, &c->panel_a, &c->rhs_g, &c->sol_g, &c->mb_blocks, &c->mb_tmp, &c->tile_flag, &c->tile_pos, &c->tile_list, &c->samp, &c->rhs_c, &c->sol_c,
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,

@@ -59,10 +59,13 @@ struct Ctx : vpmx_ctx {
     DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
+    DBuf tile_flag, tile_pos, tile_list;  // interpolation: cell tiles holding particles
     DBuf stage_a, stage_b, stage_c;
     SortState sort;
     int64_t sort_n = 0;     // particles of the last sort
-    bool p2m_block = true;  // projection: node-block kernel (else one thread per node)
+    bool p2m_block = true;  // projection: node-run / node-block kernels (else one thread per node)
+    bool p2m_rows = true;   // projection: 4 x-nodes per thread, cell-row walks (else 4x2x2 node blocks)
+    bool m2p_tile = true;   // interpolation: cell tiles with the nodes staged in LDS
     std::map<int, void *> fft_plans;
     bool have_grids = false;  // RHS/SOL/vel of the last mode 1-3 call are valid
 

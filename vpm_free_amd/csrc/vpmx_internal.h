@@ -56,6 +56,7 @@ void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st);
 struct SortState {
     DBuf keys, perm, count, start, fill;
     DBuf xs, qs;  // particle positions / strengths in cell order (projection)
+    DBuf rowcnt;  // particles per cell row (projection)
     int64_t ncells = 0;
     int cn[3] = {0, 0, 0};  // padded cell grid dims
 };
@@ -64,8 +65,10 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
 // Gather-form M-type projection: out(view over nodes of `g`, covering node window
 // [win0, win1) in each dim) = sum_p W * qp(c, p) / div for c < ncomp; nodes of the view
 // outside the window are written 0.
+// f32: weights and sums in FP32 (mixed form), the grid stays FP64.
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
-               int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass = -1);
+               int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass = -1,
+               bool f32 = false);
 // itypeb 1: UP = sum W vel, GP = (sum W sol) QP(4); itypeb 2 (diffusion): UP = 0, GP as above
 int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
                int itype, GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb = 1);
