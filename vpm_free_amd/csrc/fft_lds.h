@@ -68,8 +68,8 @@ __device__ __forceinline__ void dft_reg(C (&v)[R]) {
 //   X_k   = v_0 + sum_r (c p_r.x + s m_r.y,  c p_r.y - s m_r.x)
 //   X_R-k = v_0 + sum_r (c p_r.x - s m_r.y,  c p_r.y + s m_r.x)
 // so two outputs cost 4 FMA per r.  c and s come from the LDS twiddle table.
-template <int R, class C>
-__device__ __forceinline__ void dft_odd_store(C (&v)[R], C *buf, int d, int Ns, const C *__restrict__ tw, int L) {
+template <int R, class C, class Buf>
+__device__ __forceinline__ void dft_odd_store(C (&v)[R], Buf buf, int d, int Ns, const C *__restrict__ tw, int L) {
     using T = typename Cx<C>::R;
     constexpr int H = (R - 1) / 2;
     C p[H], mm[H];
@@ -188,8 +188,23 @@ struct WavePlan {
     unsigned mg[12];
 };
 
-template <int R, int BPL, class C>
-__device__ __forceinline__ void wave_stage(C *buf, const C *__restrict__ tw, int L, int Ns, unsigned mg, int lane) {
+// LDS view of one wave's sequence with one unused slot after every 2^S elements (S = 0: no
+// slots).  With S = 4 for float2 and 3 for double2 (128 bytes between slots) the first
+// Stockham stages, which write at strides of R and R * Ns elements, spread a 64-lane write
+// over all LDS banks instead of putting 4 lanes on each.  The slots cost address arithmetic
+// and registers on every access, so a kernel takes them only where that pays (measured).
+template <int S>
+__host__ __device__ constexpr int pad_pos(int e) { return S ? e + (e >> S) : e; }
+template <int S>
+__host__ __device__ constexpr int pad_len(int n) { return S ? pad_pos<S>(n) + 1 : n; }
+template <class C, int S>
+struct PadBuf {
+    C *p;
+    __device__ __forceinline__ C &operator[](int e) const { return p[pad_pos<S>(e)]; }
+};
+
+template <int R, int BPL, class C, class Buf>
+__device__ __forceinline__ void wave_stage(Buf buf, const C *__restrict__ tw, int L, int Ns, unsigned mg, int lane) {
     const int nbf = L / R, twstep = L / (Ns * R);
     C v[BPL][R];
     int dst[BPL];
@@ -226,27 +241,27 @@ __device__ __forceinline__ void wave_stage(C *buf, const C *__restrict__ tw, int
 template <int R, int LMAX>
 constexpr int wave_bpl() { return (LMAX / R + 63) / 64; }
 
-template <int MAXR, class C, int LMAX = (sizeof(C) == 16 ? WAVE_LMAX : WAVE_LMAX_F)>
-__device__ void wave_fft(C *buf, const C *tw, const WavePlan &P, int lane) {
+template <int MAXR, class C, class Buf, int LMAX = (sizeof(C) == 16 ? WAVE_LMAX : WAVE_LMAX_F)>
+__device__ void wave_fft(Buf buf, const C *tw, const WavePlan &P, int lane) {
     for (int s = 0; s < P.nstages; s++) {
         const int R = P.R[s], Ns = P.Ns[s];
         const unsigned mg = P.mg[s];
-        if (R == 4) wave_stage<4, wave_bpl<4, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 2) wave_stage<2, wave_bpl<2, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-        else if (R == 3) wave_stage<3, wave_bpl<3, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+        if (R == 4) wave_stage<4, wave_bpl<4, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 2) wave_stage<2, wave_bpl<2, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+        else if (R == 3) wave_stage<3, wave_bpl<3, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
         else if constexpr (MAXR >= 7) {
-            if (R == 5) wave_stage<5, wave_bpl<5, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-            else if (R == 7) wave_stage<7, wave_bpl<7, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+            if (R == 5) wave_stage<5, wave_bpl<5, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+            else if (R == 7) wave_stage<7, wave_bpl<7, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
             else if constexpr (MAXR >= 17) {
-                if (R == 11) wave_stage<11, wave_bpl<11, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 13) wave_stage<13, wave_bpl<13, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                else if (R == 17) wave_stage<17, wave_bpl<17, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                if (R == 11) wave_stage<11, wave_bpl<11, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 13) wave_stage<13, wave_bpl<13, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                else if (R == 17) wave_stage<17, wave_bpl<17, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
                 else if constexpr (MAXR >= 37) {
-                    if (R == 19) wave_stage<19, wave_bpl<19, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 23) wave_stage<23, wave_bpl<23, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 29) wave_stage<29, wave_bpl<29, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else if (R == 31) wave_stage<31, wave_bpl<31, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
-                    else wave_stage<37, wave_bpl<37, LMAX>(), C>(buf, tw, P.L, Ns, mg, lane);
+                    if (R == 19) wave_stage<19, wave_bpl<19, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 23) wave_stage<23, wave_bpl<23, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 29) wave_stage<29, wave_bpl<29, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                    else if (R == 31) wave_stage<31, wave_bpl<31, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
+                    else wave_stage<37, wave_bpl<37, LMAX>(), C, Buf>(buf, tw, P.L, Ns, mg, lane);
                 }
             }
         }
@@ -259,8 +274,8 @@ __device__ void wave_fft(C *buf, const C *tw, const WavePlan &P, int lane) {
 // strides, twiddle steps and divisions by Ns become constants, which removes most of the
 // integer work of the runtime-plan stages.  Same stage order and arithmetic as wave_fft,
 // so results are identical.
-template <int R, int L, int Ns, class C>
-__device__ __forceinline__ void wave_stage_s(C *buf, const C *__restrict__ tw, int lane) {
+template <int R, int L, int Ns, class C, class Buf>
+__device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, int lane) {
     constexpr int nbf = L / R, twstep = L / (Ns * R), BPL = (nbf + 63) / 64;
     C v[BPL][R];
     int dst[BPL];
@@ -295,10 +310,10 @@ __device__ __forceinline__ void wave_stage_s(C *buf, const C *__restrict__ tw, i
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class C, int L, int Ns, int R, int... Rest>
-__device__ __forceinline__ void wave_fft_s(C *buf, const C *tw, int lane) {
-    wave_stage_s<R, L, Ns, C>(buf, tw, lane);
-    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, L, Ns * R, Rest...>(buf, tw, lane);
+template <class C, class Buf, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void wave_fft_s(Buf buf, const C *tw, int lane) {
+    wave_stage_s<R, L, Ns, C, Buf>(buf, tw, lane);
+    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, Buf, L, Ns * R, Rest...>(buf, tw, lane);
 }
 
 // Lengths with a compile-time plan (radices in the planner's order: 4s, 2s, odd primes).
@@ -306,20 +321,20 @@ __device__ __forceinline__ void wave_fft_s(C *buf, const C *tw, int lane) {
 template <int FIX> struct FixedFft;
 template <> struct FixedFft<0> {  // 576 = 4.4.4.3.3 (FP32, 289-node block)
     static constexpr int L = 576;
-    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
-        wave_fft_s<C, 576, 1, 4, 4, 4, 3, 3>(b, tw, lane);
+    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, 576, 1, 4, 4, 4, 3, 3>(b, tw, lane);
     }
 };
 template <> struct FixedFft<2> {  // 288 = 4.4.2.3.3 (FP64, 289-node block)
     static constexpr int L = 288;
-    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
-        wave_fft_s<C, 288, 1, 4, 4, 2, 3, 3>(b, tw, lane);
+    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, 288, 1, 4, 4, 2, 3, 3>(b, tw, lane);
     }
 };
 template <> struct FixedFft<3> {  // 272 = 4.4.17 (FP64, 273-node inner box)
     static constexpr int L = 272;
-    template <class C> static __device__ __forceinline__ void run(C *b, const C *tw, int lane) {
-        wave_fft_s<C, 272, 1, 4, 4, 17>(b, tw, lane);
+    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, 272, 1, 4, 4, 17>(b, tw, lane);
     }
 };
 

@@ -638,8 +638,8 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_pass3(DstArgs a) {
 // and the inverse), so a batch needs three work-group barriers -- after the LDS fill,
 // before the stores, after the stores.  Batches are pipelined as in k_dst_pass3.
 
-template <class C>
-__device__ __forceinline__ void wave_pre(C *B, const double *__restrict__ sn, int N, int lane, const double *fold,
+template <class C, class Buf>
+__device__ __forceinline__ void wave_pre(Buf B, const double *__restrict__ sn, int N, int lane, const double *fold,
                                          double ih2) {
     using T = typename Cx<C>::R;
     const int H = N / 2;
@@ -665,8 +665,8 @@ __device__ __forceinline__ void wave_pre(C *B, const double *__restrict__ sn, in
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class C>
-__device__ __forceinline__ void wave_post(C *B, int N, int lane) {
+template <class C, class Buf>
+__device__ __forceinline__ void wave_post(Buf B, int N, int lane) {
     using T = typename Cx<C>::R;
     constexpr int PPL = (WAVE_LMAX / 2 + 1 + 63) / 64;
     const int H = N / 2 + 1;
@@ -716,8 +716,8 @@ __device__ __forceinline__ void wave_post(C *B, int N, int lane) {
 // imaginary parts, F_k = -Im Z_k / 2 for the first and Re Z_k / 2 for the second.  No
 // sine weights and no running sums, so in FP32 its error stays at the FFT's own (about
 // 1e-7 relative, ten times below the half-length form's).
-template <class C>
-__device__ __forceinline__ void wave_pre_odd(C *B, int m, int lane, const double *fold, double ih2) {
+template <class C, class Buf>
+__device__ __forceinline__ void wave_pre_odd(Buf B, int m, int lane, const double *fold, double ih2) {
     using T = typename Cx<C>::R;
     const int L = 2 * (m + 1);
     if (fold && lane == 0) {  // x ends: fold = (x0, x1) of line 2p, then of line 2p+1
@@ -745,8 +745,8 @@ __device__ __forceinline__ void wave_pre_odd(C *B, int m, int lane, const double
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class C>
-__device__ __forceinline__ void wave_post_odd(C *B, int m, int lane) {
+template <class C, class Buf>
+__device__ __forceinline__ void wave_post_odd(Buf B, int m, int lane) {
     using T = typename Cx<C>::R;
     const T h = (T)0.5;
     for (int k = 1 + lane; k <= m; k += 64) {
@@ -754,6 +754,14 @@ __device__ __forceinline__ void wave_post_odd(C *B, int m, int lane) {
         B[k] = Cx<C>::mk(-h * z.y, h * z.x);
     }
     __builtin_amdgcn_wave_barrier();
+}
+
+// Padding of the wave kernels' LDS sequences (PadBuf shift, 0: none).  Measured on the 289^3
+// block: the y passes with compile-time plans gain 10-16% (their Stockham writes conflicted
+// most); the x passes, the FP32 z pass and the runtime plans lose more to the extra
+// registers and address arithmetic than the conflicts cost them.
+__host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
+    return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && dbl)) ? (dbl ? 3 : 4) : 0;
 }
 
 // T: arithmetic and work-array type (double: plain FP64 form, half-length transforms;
@@ -771,13 +779,21 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     constexpr bool ODD = sizeof(T) == 4;
     const int N = P.L, m = a.m;  // N: FFT length
     const int nbatch = (int)((a.nlines + nl - 1) / nl);
+    // eight padded sequences (one per wave), then the twiddles
+    constexpr int S = wave_pad_shift(sizeof(T) == 8, MODE, FIX);
+    const int Np = pad_len<S>(N);
     C *A = lds;
-    C *twl = lds + 8 * N;  // exp(-2 pi i t / N)
+    C *twl = lds + 8 * Np;  // exp(-2 pi i t / N)
     T *Ad = (T *)A;
+    // LDS scalar of point pt (element pt + 1) of line `line` in a batch: lines 2p and 2p+1 are
+    // the real and imaginary parts of sequence p
+    auto slot_of = [&](int line, int pt) { return 2 * ((line >> 1) * Np + pad_pos<S>(pt + 1)) + (line & 1); };
     const int comp = blockIdx.y;
     T *w = (T *)a.w + comp * a.wcs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    C *B = A + wave * N;
+    // (S = 0: the plain pointer, so those kernels compile exactly as unpadded code)
+    using Buf = typename std::conditional<S == 0, C *, PadBuf<C, S>>::type;
+    const Buf B{A + wave * Np};
     for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
     T v[DST_PF];
     double fx[2];
@@ -794,7 +810,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             int line, pt;
             dst_elem<MODE>(m, nl, u, &line, &pt);
             if (line >= nl || pt >= m) continue;
-            Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] = v[u];
+            Ad[slot_of(line, pt)] = v[u];
         }
         if (MODE == DST_XF && a.has_bc) {
             // lines on the y or z faces of the window (a thread's line is fixed in x passes):
@@ -814,7 +830,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
                     if (j == a.my - 1) f += bp[a.bc.at(pt + 1, a.my + 1, k + 1)] * a.ih2[1];
                     if (k == 0) f += bp[a.bc.at(pt + 1, j + 1, 0)] * a.ih2[2];
                     if (k == a.mz - 1) f += bp[a.bc.at(pt + 1, j + 1, a.mz + 1)] * a.ih2[2];
-                    T &slot = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)];
+                    T &slot = Ad[slot_of(line, pt)];
                     slot = (T)((double)slot - f);
                 }
             }
@@ -833,12 +849,12 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
         const int bn = batch_of(it + 1);
         if (bn < nbatch) dst_prefetch<MODE, T>(a, w, comp, bn, nl, v, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
-        if (ODD) wave_pre_odd(B, m, lane, fold, a.ih2[0]);
-        else wave_pre(B, a.sn, N, lane, fold, a.ih2[0]);
+        if (ODD) wave_pre_odd<C>(B, m, lane, fold, a.ih2[0]);
+        else wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
         if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
-        if (ODD) wave_post_odd(B, m, lane);
-        else wave_post(B, N, lane);
+        if (ODD) wave_post_odd<C>(B, m, lane);
+        else wave_post<C>(B, N, lane);
         if (MODE == DST_Z) {
             const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
             for (int j = 1 + lane; j <= m; j += 64) {
@@ -848,12 +864,12 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
                 B[j] = Cx<C>::mk(z.x * fa, z.y * fb);
             }
             __builtin_amdgcn_wave_barrier();
-            if (ODD) wave_pre_odd(B, m, lane, nullptr, 0.0);
-            else wave_pre(B, a.sn, N, lane, nullptr, 0.0);
+            if (ODD) wave_pre_odd<C>(B, m, lane, nullptr, 0.0);
+            else wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
             if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
-            if (ODD) wave_post_odd(B, m, lane);
-            else wave_post(B, N, lane);
+            if (ODD) wave_post_odd<C>(B, m, lane);
+            else wave_post<C>(B, N, lane);
         }
         __syncthreads();
 #pragma unroll
@@ -862,7 +878,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             dst_elem<MODE>(m, nl, u, &line, &pt);
             const int gl = b * nl + line;
             if (line >= nl || pt >= m || !dst_line_ok<MODE>(a, gl)) continue;
-            const T val = Ad[2 * ((line >> 1) * N + pt + 1) + (line & 1)] * (T)a.scale;
+            const T val = Ad[slot_of(line, pt)] * (T)a.scale;
             if (MODE == DST_XI) {
                 const int j = gl % a.my, k = gl / a.my;
                 a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = (double)val;
@@ -1083,7 +1099,6 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         ns *= fp->radix[s];
     }
     a.npairs = 8;
-    const size_t lds = 2 * sizeof(T) * (size_t)9 * a.L;
     const int64_t nbatch = (a.nlines + 15) / 16;
     // compile-time plans for the lengths of the common block sizes
     auto same = [&](const int *rr, int n) {
@@ -1109,6 +1124,11 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
                : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
                : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
     }
+    // eight padded sequences and the twiddles (the kernel's LDS layout)
+    using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+    const int S = wave_pad_shift(sizeof(T) == 8, MODE, fix);
+    const int Np = S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
+    const size_t lds = sizeof(C) * (size_t)(8 * Np + a.L);
     // persistent grid: as many work-groups as the device holds at once
     int per_cu = c->dst_wg_per_cu;
     if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
