@@ -182,6 +182,42 @@ def test_solve_all_radix_classes(ctx, oracle, nn):
     assert rel_l2(got, ref) <= TOL
 
 
+def _lap7(u, h):
+    """7-point Laplacian at the interior nodes of u (nz, ny, nx)."""
+    c = u[1:-1, 1:-1, 1:-1]
+    return ((u[1:-1, 1:-1, 2:] - 2 * c + u[1:-1, 1:-1, :-2]) / h[0] ** 2 +
+            (u[1:-1, 2:, 1:-1] - 2 * c + u[1:-1, :-2, 1:-1]) / h[1] ** 2 +
+            (u[2:, 1:-1, 1:-1] - 2 * c + u[:-2, 1:-1, 1:-1]) / h[2] ** 2)
+
+
+@pytest.mark.parametrize("n", [289, 273])
+def test_solve_full_size_plans(ctx, n):
+    """The block sizes of the 256-cubed workload, where the sine transforms run their
+    compile-time FFT plans (FP64: 288 = 4.4.2.3.3, 272 = 4.4.17; FP32: 576 = 4.4.4.3.3,
+    544 = 17.4.4.2), padded LDS sequences and register-capped kernels — sizes the oracle is
+    too slow for.  The FP64 solution satisfies the 7-point equations to rounding, and the
+    mixed form agrees with it to FP32 accuracy."""
+    rng = np.random.default_rng(n)
+    nn = np.array([n, n, n], np.int32)
+    h = np.array([1 / 64] * 3)
+    xb = np.array([-2.0, -2.0, -2.0, 0, 0, 0])
+    rhs = rng.standard_normal((n, n, n, 3))
+    sol = rng.standard_normal((n, n, n, 3))
+    win = np.array([1, 1, 1, n, n, n], np.int32)
+    say("case: full-size solve", n)
+    try:
+        ctx.set_precision(0)
+        u = ctx.stage_solve(xb, h, nn, 3, win, rhs, sol=sol)
+        ctx.set_precision(1)
+        um = ctx.stage_solve(xb, h, nn, 3, win, rhs, sol=sol)
+    finally:
+        ctx.set_precision(0)
+    for c in range(3):
+        res = _lap7(u[..., c], h) - rhs[1:-1, 1:-1, 1:-1, c]
+        assert np.abs(res).max() < 1e-8 * np.abs(rhs).max(), c
+    assert rel_l2(um, u) < 1e-6
+
+
 @pytest.mark.parametrize("mode", ["spectral", "lattice", "direct"])
 def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
     """The three evaluations of the fine boundary sums (spectral convolution + batched

@@ -331,6 +331,13 @@ template <> struct FixedFft<2> {  // 288 = 4.4.2.3.3 (FP64, 289-node block)
         wave_fft_s<C, Buf, 288, 1, 4, 4, 2, 3, 3>(b, tw, lane);
     }
 };
+template <> struct FixedFft<5> {  // 544 = 17.4.4.2 (FP32, 273-node inner box; the prime stage first needs
+                                  // the fewest registers of the orders tried)
+    static constexpr int L = 544;
+    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, 544, 1, 17, 4, 4, 2>(b, tw, lane);
+    }
+};
 template <> struct FixedFft<3> {  // 272 = 4.4.17 (FP64, 273-node inner box)
     static constexpr int L = 272;
     template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {

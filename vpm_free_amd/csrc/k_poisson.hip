@@ -756,6 +756,76 @@ __device__ __forceinline__ void wave_post_odd(Buf B, int m, int lane) {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Addressing in the one-wave kernels (16 lines per batch).  dst_elem gives every thread one
+// line (x passes: 32 threads per line; y and z passes: line tid mod 16) and the points
+// pt0 + 32u of it, so per batch a thread forms its line's offsets once: element u is read at
+// src + u * src_step, written at dst + u * dst_step, and sits in LDS at a fixed stride.
+constexpr int WV_NL = 16, WV_PS = DST_THREADS / WV_NL;
+struct WaveAddr {
+    int64_t src, src_step;  // work array, or the RHS view (x forward)
+    int64_t dst, dst_step;  // work array, or the output view (x inverse)
+    bool ok;                // the line exists
+};
+
+template <int MODE>
+__device__ __forceinline__ WaveAddr wave_addr(const DstArgs &a, int comp, int gl, int pt0) {
+    WaveAddr w;
+    w.ok = dst_line_ok<MODE>(a, gl);
+    if (MODE == DST_XF || MODE == DST_XI) {
+        const int j = gl % a.my, k = gl / a.my;
+        const int64_t wo = (int64_t)gl * a.px + pt0;
+        w.src = MODE == DST_XF ? comp * a.rhs.cs + a.rhs.at(pt0 + 1, j + 1, k + 1) : wo;
+        w.dst = MODE == DST_XI ? comp * a.out.cs + a.out.at(pt0 + 1, j + 1, k + 1) : wo;
+        w.src_step = w.dst_step = WV_PS;
+    } else {
+        w.src = w.dst = dst_addr<MODE>(a, gl, pt0);
+        w.src_step = w.dst_step = MODE == DST_Y ? (int64_t)WV_PS * a.px : (int64_t)WV_PS * a.px * a.my;
+    }
+    return w;
+}
+
+// Loads of batch b into registers (no wait), as dst_prefetch.
+template <int MODE, class T>
+__device__ __forceinline__ void wave_prefetch(const DstArgs &a, const T *w, int comp, int b, int line, int pt0,
+                                              int m, T (&v)[DST_PF], double (&fx)[2]) {
+    const WaveAddr ad = wave_addr<MODE>(a, comp, b * WV_NL + line, pt0);
+#pragma unroll
+    for (int u = 0; u < DST_PF; u++) {
+        v[u] = (T)0;
+        if (ad.ok && pt0 + u * WV_PS < m) {
+            if constexpr (MODE == DST_XF) v[u] = (T)a.rhs.p[ad.src + u * ad.src_step];
+            else v[u] = w[ad.src + u * ad.src_step];
+        }
+    }
+    if (MODE == DST_XF) {
+        fx[0] = fx[1] = 0.0;
+        const int gl = b * WV_NL + (int)threadIdx.x;
+        if (a.has_bc && (int)threadIdx.x < WV_NL && gl < a.nlines) {
+            const int j = gl % a.my, k = gl / a.my;
+            const double *bp = a.bc.p + comp * a.bc.cs;
+            fx[0] = bp[a.bc.at(0, j + 1, k + 1)];
+            fx[1] = bp[a.bc.at(a.mx + 1, j + 1, k + 1)];
+        }
+    }
+}
+
+// -n / d for d > 0 from the hardware reciprocal and Newton steps (within an ulp or two of
+// the quotient, at a fraction of the cost of the IEEE division sequence).
+__device__ __forceinline__ double neg_quot(double n, double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    double q = n * r;
+    q = fma(r, fma(-d, q, n), q);
+    return -q;
+}
+
+template <int FIX>
+__host__ __device__ constexpr int fix_len() {
+    if constexpr (FIX >= 0) return FixedFft<FIX>::L;
+    else return 0;
+}
+
 // Padding of the wave kernels' LDS sequences (PadBuf shift, 0: none).  Measured on the 289^3
 // block: the y passes with compile-time plans gain 10-16% (their Stockham writes conflicted
 // most); the x passes, the FP32 z pass and the runtime plans lose more to the extra
@@ -768,16 +838,20 @@ __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
 // float: the mixed form, odd-extension transforms, whose RHS, Dirichlet values and output
 // stay FP64 and whose inner passes run in FP32).  P.L is the FFT length (m+1 or 2(m+1)).
 template <int MODE, int MAXR, class T, int FIX>
-__global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
+__device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &P) {
 #pragma clang fp contract(off)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     extern __shared__ double2 lds_raw[];
     C *lds = (C *)lds_raw;
     __shared__ double lamxy[16];
     __shared__ double foldx[32];
-    constexpr int nl = 16;
+    __shared__ double lzs[MODE == DST_Z ? WAVE_LMAX : 1];  // lambda_z, read once per kernel
+    constexpr int nl = WV_NL;
     constexpr bool ODD = sizeof(T) == 4;
-    const int N = P.L, m = a.m;  // N: FFT length
+    // N: FFT length (2(m+1) in the odd-extension form, m+1 in the half-length form); both
+    // constants with a compile-time plan
+    const int N = FIX >= 0 ? fix_len<FIX>() : P.L;
+    const int m = FIX >= 0 ? (ODD ? N / 2 - 1 : N - 1) : a.m;
     const int nbatch = (int)((a.nlines + nl - 1) / nl);
     // eight padded sequences (one per wave), then the twiddles
     constexpr int S = wave_pad_shift(sizeof(T) == 8, MODE, FIX);
@@ -788,6 +862,11 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     // LDS scalar of point pt (element pt + 1) of line `line` in a batch: lines 2p and 2p+1 are
     // the real and imaginary parts of sequence p
     auto slot_of = [&](int line, int pt) { return 2 * ((line >> 1) * Np + pad_pos<S>(pt + 1)) + (line & 1); };
+    // this thread's line and first point (dst_elem), and the LDS stride of its elements
+    const int tline = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x / WV_PS : (int)threadIdx.x & (WV_NL - 1);
+    const int tpt0 = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x & (WV_PS - 1) : (int)threadIdx.x / WV_NL;
+    constexpr int SSTEP = 2 * (WV_PS + (S ? WV_PS >> S : 0));
+    const int slot0 = slot_of(tline, tpt0);
     const int comp = blockIdx.y;
     T *w = (T *)a.w + comp * a.wcs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -795,6 +874,8 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     using Buf = typename std::conditional<S == 0, C *, PadBuf<C, S>>::type;
     const Buf B{A + wave * Np};
     for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
+    if (MODE == DST_Z)
+        for (int t = tid; t < m; t += DST_THREADS) lzs[t] = a.lam[2][t];
     T v[DST_PF];
     double fx[2];
     // batch of iteration `it`: the two halves of a batch pair (neighbouring lines, one
@@ -803,15 +884,11 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
     auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
     int it = 0, b = batch_of(0);
-    if (b < nbatch) dst_prefetch<MODE, T>(a, w, comp, b, nl, v, fx);
+    if (b < nbatch) wave_prefetch<MODE, T>(a, w, comp, b, tline, tpt0, m, v, fx);
     for (; b < nbatch; b = batch_of(++it)) {
 #pragma unroll
-        for (int u = 0; u < DST_PF; u++) {
-            int line, pt;
-            dst_elem<MODE>(m, nl, u, &line, &pt);
-            if (line >= nl || pt >= m) continue;
-            Ad[slot_of(line, pt)] = v[u];
-        }
+        for (int u = 0; u < DST_PF; u++)
+            if (tpt0 + u * WV_PS < m) Ad[slot0 + u * SSTEP] = v[u];
         if (MODE == DST_XF && a.has_bc) {
             // lines on the y or z faces of the window (a thread's line is fixed in x passes):
             // fold their Dirichlet neighbours; the x ends are folded in wave_pre
@@ -847,7 +924,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
         }
         __syncthreads();
         const int bn = batch_of(it + 1);
-        if (bn < nbatch) dst_prefetch<MODE, T>(a, w, comp, bn, nl, v, fx);
+        if (bn < nbatch) wave_prefetch<MODE, T>(a, w, comp, bn, tline, tpt0, m, v, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if (ODD) wave_pre_odd<C>(B, m, lane, fold, a.ih2[0]);
         else wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
@@ -858,10 +935,17 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
         if (MODE == DST_Z) {
             const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
             for (int j = 1 + lane; j <= m; j += 64) {
-                const double lz = a.lam[2][j - 1];
-                const T fa = (T)(a.norm / (-(la + lz))), fb = (T)(a.norm / (-(lb + lz)));
+                const double lz = lzs[j - 1];
                 const C z = B[j];
-                B[j] = Cx<C>::mk(z.x * fa, z.y * fb);
+                if constexpr (ODD) {
+                    // FP32 form: the factors from the FP32 reciprocal (1 ulp)
+                    const float nz = (float)a.norm;
+                    const float fa = -nz * __builtin_amdgcn_rcpf((float)(la + lz));
+                    const float fb = -nz * __builtin_amdgcn_rcpf((float)(lb + lz));
+                    B[j] = Cx<C>::mk(z.x * fa, z.y * fb);
+                } else {
+                    B[j] = Cx<C>::mk(z.x * neg_quot(a.norm, la + lz), z.y * neg_quot(a.norm, lb + lz));
+                }
             }
             __builtin_amdgcn_wave_barrier();
             if (ODD) wave_pre_odd<C>(B, m, lane, nullptr, 0.0);
@@ -872,22 +956,33 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
             else wave_post<C>(B, N, lane);
         }
         __syncthreads();
+        const WaveAddr ad = wave_addr<MODE>(a, comp, b * nl + tline, tpt0);
 #pragma unroll
         for (int u = 0; u < DST_PF; u++) {
-            int line, pt;
-            dst_elem<MODE>(m, nl, u, &line, &pt);
-            const int gl = b * nl + line;
-            if (line >= nl || pt >= m || !dst_line_ok<MODE>(a, gl)) continue;
-            const T val = Ad[slot_of(line, pt)] * (T)a.scale;
-            if (MODE == DST_XI) {
-                const int j = gl % a.my, k = gl / a.my;
-                a.out.p[comp * a.out.cs + a.out.at(pt + 1, j + 1, k + 1)] = (double)val;
-            } else {
-                w[dst_addr<MODE>(a, gl, pt)] = val;
-            }
+            if (!ad.ok || tpt0 + u * WV_PS >= m) continue;
+            const T val = Ad[slot0 + u * SSTEP] * (T)a.scale;
+            if (MODE == DST_XI) a.out.p[ad.dst + u * ad.dst_step] = (double)val;
+            else w[ad.dst + u * ad.dst_step] = val;
         }
         __syncthreads();
     }
+}
+
+template <int MODE, int MAXR, class T, int FIX>
+__global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
+    dst_wave_body<MODE, MAXR, T, FIX>(a, P);
+}
+// the same with the registers capped for four waves per SIMD (two work-groups per CU); the
+// FP64 x and y passes of the 289-node block and the x inverse pass of the 273-node box gain
+// 8-15% from the doubled occupancy despite a few spilled registers, the other FP64 passes
+// lose more to the spills (measured)
+__host__ __device__ constexpr bool wave_w4(bool dbl, int mode, int fix) {
+    return dbl && ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI));
+}
+template <int MODE, int MAXR, class T, int FIX>
+__global__ void __launch_bounds__(DST_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
+k_dst_wave_w4(DstArgs a, WavePlan P) {
+    dst_wave_body<MODE, MAXR, T, FIX>(a, P);
 }
 
 // Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
@@ -1107,18 +1202,25 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             if (fp->radix[q] != rr[q]) return false;
         return true;
     };
-    static const int r576[] = {4, 4, 4, 3, 3}, r288[] = {4, 4, 2, 3, 3}, r272[] = {4, 4, 17};
+    static const int r576[] = {4, 4, 4, 3, 3}, r288[] = {4, 4, 2, 3, 3}, r272[] = {4, 4, 17}, r544[] = {4, 4, 2, 17};
     int fix = -1;
     if (sizeof(T) == 4 && fp->L == 576 && same(r576, 5)) fix = 0;
-    // (544 = 4.4.2.17 in FP32 keeps the runtime plan: its compile-time form needs more registers
-    //  and halves the resident work-groups)
     if (sizeof(T) == 8 && fp->L == 288 && same(r288, 5)) fix = 2;
     if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
+    if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
     const void *fn;
+    constexpr bool dbl = sizeof(T) == 8;
     switch (fix) {
         case 0: fn = (const void *)k_dst_wave<MODE, 4, T, 0>; break;
-        case 2: fn = (const void *)k_dst_wave<MODE, 4, T, 2>; break;
-        case 3: fn = (const void *)k_dst_wave<MODE, 17, T, 3>; break;
+        case 2:
+            if constexpr (wave_w4(dbl, MODE, 2)) fn = (const void *)k_dst_wave_w4<MODE, 4, T, 2>;
+            else fn = (const void *)k_dst_wave<MODE, 4, T, 2>;
+            break;
+        case 3:
+            if constexpr (wave_w4(dbl, MODE, 3)) fn = (const void *)k_dst_wave_w4<MODE, 17, T, 3>;
+            else fn = (const void *)k_dst_wave<MODE, 17, T, 3>;
+            break;
+        case 5: fn = (const void *)k_dst_wave<MODE, 17, T, 5>; break;
         default:
             fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
                : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
