@@ -784,18 +784,20 @@ __device__ __forceinline__ WaveAddr wave_addr(const DstArgs &a, int comp, int gl
     return w;
 }
 
-// Loads of batch b into registers (no wait), as dst_prefetch.
-template <int MODE, class T>
+// Loads of batch b into registers (no wait), as dst_prefetch.  Every load is issued (one
+// whose element does not exist reads element 0 instead), so no branch separates the loads and
+// none has to wait for another; `ok` says whether the line exists, and the fill keeps only
+// the points below m.  PT: double for the x forward pass (the RHS), converted at the fill.
+template <int MODE, class T, class PT>
 __device__ __forceinline__ void wave_prefetch(const DstArgs &a, const T *w, int comp, int b, int line, int pt0,
-                                              int m, T (&v)[DST_PF], double (&fx)[2]) {
+                                              int m, PT (&v)[DST_PF], bool &ok, double (&fx)[2]) {
     const WaveAddr ad = wave_addr<MODE>(a, comp, b * WV_NL + line, pt0);
+    ok = ad.ok;
 #pragma unroll
     for (int u = 0; u < DST_PF; u++) {
-        v[u] = (T)0;
-        if (ad.ok && pt0 + u * WV_PS < m) {
-            if constexpr (MODE == DST_XF) v[u] = (T)a.rhs.p[ad.src + u * ad.src_step];
-            else v[u] = w[ad.src + u * ad.src_step];
-        }
+        const int64_t o = (ad.ok && pt0 + u * WV_PS < m) ? ad.src + u * ad.src_step : 0;
+        if constexpr (MODE == DST_XF) v[u] = a.rhs.p[o];
+        else v[u] = w[o];
     }
     if (MODE == DST_XF) {
         fx[0] = fx[1] = 0.0;
@@ -876,7 +878,9 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
     if (MODE == DST_Z)
         for (int t = tid; t < m; t += DST_THREADS) lzs[t] = a.lam[2][t];
-    T v[DST_PF];
+    using PT = typename std::conditional<MODE == DST_XF, double, T>::type;
+    PT v[DST_PF];
+    bool vok = false;
     double fx[2];
     // batch of iteration `it`: the two halves of a batch pair (neighbouring lines, one
     // 128-byte row segment in FP32) go to work-groups w and w+8, which share an XCD (and its
@@ -884,11 +888,11 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
     auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
     int it = 0, b = batch_of(0);
-    if (b < nbatch) wave_prefetch<MODE, T>(a, w, comp, b, tline, tpt0, m, v, fx);
+    if (b < nbatch) wave_prefetch<MODE, T>(a, w, comp, b, tline, tpt0, m, v, vok, fx);
     for (; b < nbatch; b = batch_of(++it)) {
 #pragma unroll
         for (int u = 0; u < DST_PF; u++)
-            if (tpt0 + u * WV_PS < m) Ad[slot0 + u * SSTEP] = v[u];
+            if (tpt0 + u * WV_PS < m) Ad[slot0 + u * SSTEP] = vok ? (T)v[u] : (T)0;
         if (MODE == DST_XF && a.has_bc) {
             // lines on the y or z faces of the window (a thread's line is fixed in x passes):
             // fold their Dirichlet neighbours; the x ends are folded in wave_pre
@@ -924,7 +928,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
         }
         __syncthreads();
         const int bn = batch_of(it + 1);
-        if (bn < nbatch) wave_prefetch<MODE, T>(a, w, comp, bn, tline, tpt0, m, v, fx);
+        if (bn < nbatch) wave_prefetch<MODE, T>(a, w, comp, bn, tline, tpt0, m, v, vok, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if (ODD) wave_pre_odd<C>(B, m, lane, fold, a.ih2[0]);
         else wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
@@ -973,11 +977,12 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
     dst_wave_body<MODE, MAXR, T, FIX>(a, P);
 }
 // the same with the registers capped for four waves per SIMD (two work-groups per CU); the
-// FP64 x and y passes of the 289-node block and the x inverse pass of the 273-node box gain
-// 8-15% from the doubled occupancy despite a few spilled registers, the other FP64 passes
-// lose more to the spills (measured)
+// FP64 x and y passes of the 289-node block, the FP64 x inverse and the FP32 z pass of the
+// 273-node box gain from the doubled occupancy despite a few spilled registers, the other
+// passes lose more to the spills or already fit (measured)
 __host__ __device__ constexpr bool wave_w4(bool dbl, int mode, int fix) {
-    return dbl && ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI));
+    return dbl ? ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI))
+               : (fix == 5 && mode == DST_Z);
 }
 template <int MODE, int MAXR, class T, int FIX>
 __global__ void __launch_bounds__(DST_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
@@ -1208,23 +1213,30 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (sizeof(T) == 8 && fp->L == 288 && same(r288, 5)) fix = 2;
     if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
     if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
-    const void *fn;
+    const void *fn = nullptr;
     constexpr bool dbl = sizeof(T) == 8;
-    switch (fix) {
-        case 0: fn = (const void *)k_dst_wave<MODE, 4, T, 0>; break;
-        case 2:
+    // (each length's plan is instantiated only for the form that uses it)
+    if constexpr (dbl) {
+        if (fix == 2) {
             if constexpr (wave_w4(dbl, MODE, 2)) fn = (const void *)k_dst_wave_w4<MODE, 4, T, 2>;
             else fn = (const void *)k_dst_wave<MODE, 4, T, 2>;
-            break;
-        case 3:
+        } else if (fix == 3) {
             if constexpr (wave_w4(dbl, MODE, 3)) fn = (const void *)k_dst_wave_w4<MODE, 17, T, 3>;
             else fn = (const void *)k_dst_wave<MODE, 17, T, 3>;
-            break;
-        case 5: fn = (const void *)k_dst_wave<MODE, 17, T, 5>; break;
-        default:
-            fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
-               : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
-               : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
+        }
+    } else {
+        if (fix == 0) {
+            fn = (const void *)k_dst_wave<MODE, 4, T, 0>;
+        } else if (fix == 5) {
+            if constexpr (wave_w4(dbl, MODE, 5)) fn = (const void *)k_dst_wave_w4<MODE, 17, T, 5>;
+            else fn = (const void *)k_dst_wave<MODE, 17, T, 5>;
+        }
+    }
+    if (!fn) {
+        fix = -1;
+        fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
+           : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
+           : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
     }
     // eight padded sequences and the twiddles (the kernel's LDS layout)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
