@@ -976,18 +976,21 @@ template <int MODE, int MAXR, class T, int FIX>
 __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
     dst_wave_body<MODE, MAXR, T, FIX>(a, P);
 }
-// the same with the registers capped for four waves per SIMD (two work-groups per CU); the
-// FP64 x and y passes of the 289-node block, the FP64 x inverse and the FP32 z pass of the
-// 273-node box gain from the doubled occupancy despite a few spilled registers, the other
-// passes lose more to the spills or already fit (measured)
-__host__ __device__ constexpr bool wave_w4(bool dbl, int mode, int fix) {
-    return dbl ? ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI))
-               : (fix == 5 && mode == DST_Z);
-}
-template <int MODE, int MAXR, class T, int FIX>
-__global__ void __launch_bounds__(DST_THREADS) __attribute__((amdgpu_waves_per_eu(4)))
-k_dst_wave_w4(DstArgs a, WavePlan P) {
+// The same with the registers capped for W waves per SIMD (W = 4: 128 VGPRs, two 8-wave
+// work-groups per CU; W = 6: 80 VGPRs, three).  Measured on the 289- and 273-node blocks:
+// the FP64 x and y passes of the 289-node block, the FP64 x inverse and the FP32 z pass of
+// the 273-node box gain from four waves despite a few spilled registers, the FP32 x inverse
+// pass of the 289-node block from six; the other passes lose more to the spills than the
+// occupancy brings (the FP32 y pass at six waves: 8 % slower), or already fit.
+template <int W, int MODE, int MAXR, class T, int FIX>
+__global__ void __launch_bounds__(DST_THREADS) __attribute__((amdgpu_waves_per_eu(W)))
+k_dst_wave_cap(DstArgs a, WavePlan P) {
     dst_wave_body<MODE, MAXR, T, FIX>(a, P);
+}
+__host__ __device__ constexpr int wave_cap(bool dbl, int mode, int fix) {
+    if (dbl) return ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI)) ? 4 : 0;
+    if (fix == 0 && mode == DST_XI) return 6;
+    return (fix == 5 && mode == DST_Z) ? 4 : 0;
 }
 
 // Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
@@ -1216,19 +1219,22 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     const void *fn = nullptr;
     constexpr bool dbl = sizeof(T) == 8;
     // (each length's plan is instantiated only for the form that uses it)
+    constexpr int c2 = wave_cap(dbl, MODE, 2), c3 = wave_cap(dbl, MODE, 3), c0 = wave_cap(dbl, MODE, 0),
+                  c5 = wave_cap(dbl, MODE, 5);
     if constexpr (dbl) {
         if (fix == 2) {
-            if constexpr (wave_w4(dbl, MODE, 2)) fn = (const void *)k_dst_wave_w4<MODE, 4, T, 2>;
+            if constexpr (c2) fn = (const void *)k_dst_wave_cap<c2, MODE, 4, T, 2>;
             else fn = (const void *)k_dst_wave<MODE, 4, T, 2>;
         } else if (fix == 3) {
-            if constexpr (wave_w4(dbl, MODE, 3)) fn = (const void *)k_dst_wave_w4<MODE, 17, T, 3>;
+            if constexpr (c3) fn = (const void *)k_dst_wave_cap<c3, MODE, 17, T, 3>;
             else fn = (const void *)k_dst_wave<MODE, 17, T, 3>;
         }
     } else {
         if (fix == 0) {
-            fn = (const void *)k_dst_wave<MODE, 4, T, 0>;
+            if constexpr (c0) fn = (const void *)k_dst_wave_cap<c0, MODE, 4, T, 0>;
+            else fn = (const void *)k_dst_wave<MODE, 4, T, 0>;
         } else if (fix == 5) {
-            if constexpr (wave_w4(dbl, MODE, 5)) fn = (const void *)k_dst_wave_w4<MODE, 17, T, 5>;
+            if constexpr (c5) fn = (const void *)k_dst_wave_cap<c5, MODE, 17, T, 5>;
             else fn = (const void *)k_dst_wave<MODE, 17, T, 5>;
         }
     }
