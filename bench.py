@@ -68,13 +68,14 @@ def stencil_nodes(xp, g):
     return int(occ.sum())
 
 
-def measured_traffic(mixed):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
+def measured_traffic(mixed, axis):
+    """HBM bytes per launch of the y or z pass from the committed rocprofv3 PMC summary
     (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", "dst_y_pass_289_traffic.md")
+    path = os.path.join(ROOT, "profiles", "dst_passes_289_traffic.md")
+    key = "traffic_bytes_per_launch_" + axis
     try:
         for line in open(path):
-            if line.strip().startswith("traffic_bytes_per_launch"):
+            if line.strip().startswith(key + " "):
                 kv = dict(t.split("=") for t in line.split()[1:])
                 return int(kv["float" if mixed else "double"])
     except (OSError, KeyError, ValueError):
@@ -152,13 +153,15 @@ def time_form(ctx, precision, xp0, qp0, args, g):
         "kernels": {},
     }
     # per-kernel device time from each dispatch's start/stop events on the library's stream
-    dst_ms, dst_n = kt["dst_y_fine"]
-    if dst_n:
-        b = work["dst_y_fine"]  # read + write of the work array: 2 x lines x points x comps x word
-        gbs = b / (dst_ms / dst_n * 1e-3) / 1e9
-        res["kernels"]["dst_y_fine"] = {"ms": round(dst_ms / dst_n, 4), "launches": int(dst_n),
-                                        "algorithmic_bytes": int(b), "GB/s": round(gbs, 1),
-                                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    for name in ("dst_y_fine", "dst_z_fine"):
+        dst_ms, dst_n = kt[name]
+        if dst_n:
+            b = work[name]  # read + write of every interior point: 2 x points x comps x word
+            gbs = b / (dst_ms / dst_n * 1e-3) / 1e9
+            res["kernels"][name] = {"ms": round(dst_ms / dst_n, 4), "launches": int(dst_n),
+                                    "ms_per_step": round(dst_ms / args.steps, 4),
+                                    "algorithmic_bytes": int(b), "GB/s": round(gbs, 1),
+                                    "frac": round(gbs / HBM_PEAK_GBS, 4)}
     bs_ms, bs_n = kt["boundsum_fine"]
     if bs_n:
         fl = work["boundsum_fine"]
@@ -223,19 +226,25 @@ def main():
 
     mixed = "mixed" in forms
     head = forms["mixed"] if mixed else forms["fp64"]
-    # dominant kernel: the y pass of the 289^3 block's sine transforms (HBM-bound)
-    dk = head["kernels"].get("dst_y_fine", {})
+    # dominant kernel: the y or the z pass of the 289^3 block's sine transforms, whichever
+    # takes more device time per step (both read and write the block once per launch)
+    ks = head["kernels"]
+    dom = max(("dst_y_fine", "dst_z_fine"), key=lambda k: ks.get(k, {}).get("ms_per_step", -1.0))
+    dk = ks.get(dom, {})
+    axis = "y" if dom == "dst_y_fine" else "z"
     roofline = {
         "bound": "hbm",
         "achieved": dk.get("GB/s"),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": dk.get("frac"),
-        "traffic": measured_traffic(mixed),
-        "traffic_source": "profiles/dst_y_pass_289_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-        "kernel": "k_dst_wave<DST_Y,4,%s>: y pass of the 289^3 block's sine-transform Poisson solve (3 comps)"
-                  % ("float" if mixed else "double"),
+        "traffic": measured_traffic(mixed, axis),
+        "traffic_source": "profiles/dst_passes_289_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "kernel": "k_dst_wave<DST_%s,4,%s>: %s pass of the 289^3 block's sine-transform Poisson solve (3 comps%s)"
+                  % (axis.upper(), "float" if mixed else "double", axis,
+                     "" if axis == "y" else "; forward transform, diagonal and inverse transform"),
         "ms_per_launch": dk.get("ms"),
+        "launches_per_step": round(dk.get("launches", 0) / args.steps, 2),
         "algorithmic_bytes_per_launch": dk.get("algorithmic_bytes"),
     }
 

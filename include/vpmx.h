@@ -164,7 +164,8 @@ int32_t vpmx_stage_times(const vpmx_ctx *ctx, double *ms, int32_t n);
 /* Accumulated device time (ms, from each dispatch's start/stop events) and launch counts
  * of the timed kernel classes: 0 fine-block boundary sum (Bounds3d, pmbound.f90:80-157),
  * 1 projection (pmproject.f90:43-153), 2 interpolation (vpm_time.f90:30-107), 3 the y passes
- * of the fine block's Poisson solves (pmsolve.f90:284-432).  reset != 0 zeroes them. */
+ * and 4 the z passes (forward, diagonal, inverse) of the fine block's Poisson solves
+ * (pmsolve.f90:284-432).  reset != 0 zeroes them. */
 int32_t vpmx_kernel_times(vpmx_ctx *ctx, double *ms_total, int64_t *launches, int32_t n, int32_t reset);
 /* Algorithmic work of the last launch of each timed class: FP64 flops for the boundary sum
  * (spectral form: complex products + transforms; lattice form: 6 per pair + 20 per

@@ -23,73 +23,75 @@ int fill(Ctx *c, double *p, int64_t n, double v) {
 }
 
 // Node (i,j,k) 0-based.  Interior 1..n-2 gets the curl, the rest 0 (velvr*_pm = 0 before).
-__global__ void __launch_bounds__(256) k_curl(int nx, int ny, int nz, double dx2, double dy2, double dz2, GView psi,
+// One thread per node on a (x, y, z) launch (64 x 4 nodes per work-group, one z plane), so no
+// index divisions; the differences are scaled by 1/(2h) (within an ulp of the reference's
+// division by 2h).
+__global__ void __launch_bounds__(256) k_curl(int nx, int ny, int nz, double ix2, double iy2, double iz2, GView psi,
                                                GView vel) {
 #pragma clang fp contract(off)
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nx * ny * nz) return;
-    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6), k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
     double u = 0.0, v = 0.0, w = 0.0;
     if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2) {
         const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
-        int64_t xp = psi.at(i + 1, j, k), xm = psi.at(i - 1, j, k);
-        int64_t yp = psi.at(i, j + 1, k), ym = psi.at(i, j - 1, k);
-        int64_t zp = psi.at(i, j, k + 1), zm = psi.at(i, j, k - 1);
-        double dpsidx1 = (p1[xp] - p1[xm]) / dx2, dpsidx2 = (p2[xp] - p2[xm]) / dx2;
-        double dpsidy0 = (p0[yp] - p0[ym]) / dy2, dpsidy2 = (p2[yp] - p2[ym]) / dy2;
-        double dpsidz0 = (p0[zp] - p0[zm]) / dz2, dpsidz1 = (p1[zp] - p1[zm]) / dz2;
+        const int64_t o = psi.at(i, j, k);
+        const int64_t xp = o + 1, xm = o - 1, yp = o + psi.py, ym = o - psi.py, zp = o + psi.pz, zm = o - psi.pz;
+        double dpsidx1 = (p1[xp] - p1[xm]) * ix2, dpsidx2 = (p2[xp] - p2[xm]) * ix2;
+        double dpsidy0 = (p0[yp] - p0[ym]) * iy2, dpsidy2 = (p2[yp] - p2[ym]) * iy2;
+        double dpsidz0 = (p0[zp] - p0[zm]) * iz2, dpsidz1 = (p1[zp] - p1[zm]) * iz2;
         u = dpsidy2 - dpsidz1;
         v = -(dpsidx2 - dpsidz0);
         w = dpsidx1 - dpsidy0;
     }
-    int64_t o = vel.at(i, j, k);
+    const int64_t o = vel.at(i, j, k);
     vel.p[o] = u;
     vel.p[vel.cs + o] = v;
     vel.p[2 * vel.cs + o] = w;
 }
+
+static dim3 node_grid(const int n[3]) {
+    return dim3((unsigned)((n[0] + 63) / 64), (unsigned)((n[1] + 3) / 4), (unsigned)n[2]);
+}
+
 int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel) {
-    int64_t nt = (int64_t)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(k_curl, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2], 2 * h[0],
-                       2 * h[1], 2 * h[2], psi, vel);
+    if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_curl, node_grid(n), dim3(256), 0, c->stream, n[0], n[1], n[2], 1.0 / (2 * h[0]),
+                       1.0 / (2 * h[1]), 1.0 / (2 * h[2]), psi, vel);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
 
-// Interior 2..n-3 (0-based) gets the stretching term, the rest 0 (SOL_pm = 0 before).
-__global__ void __launch_bounds__(256) k_stretch(int nx, int ny, int nz, double dx2, double dy2, double dz2,
+// Interior 2..n-3 (0-based) gets the stretching term, the rest 0 (SOL_pm = 0 before).  Same
+// launch shape and scaling as k_curl.
+__global__ void __launch_bounds__(256) k_stretch(int nx, int ny, int nz, double ix2, double iy2, double iz2,
                                                   GView rhs, GView vel, GView out) {
 #pragma clang fp contract(off)
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nx * ny * nz) return;
-    int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63), j = blockIdx.y * 4 + (threadIdx.x >> 6), k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
     double r[3] = {0.0, 0.0, 0.0};
     if (i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3) {
         const double *R0 = rhs.p, *R1 = rhs.p + rhs.cs, *R2 = rhs.p + 2 * rhs.cs;
-        int64_t rxp = rhs.at(i + 1, j, k), rxm = rhs.at(i - 1, j, k);
-        int64_t ryp = rhs.at(i, j + 1, k), rym = rhs.at(i, j - 1, k);
-        int64_t rzp = rhs.at(i, j, k + 1), rzm = rhs.at(i, j, k - 1);
-        int64_t vxp = vel.at(i + 1, j, k), vxm = vel.at(i - 1, j, k);
-        int64_t vyp = vel.at(i, j + 1, k), vym = vel.at(i, j - 1, k);
-        int64_t vzp = vel.at(i, j, k + 1), vzm = vel.at(i, j, k - 1);
-        double a0p = R0[rxp], a0m = R0[rxm], a1p = R1[ryp], a1m = R1[rym], a2p = R2[rzp], a2m = R2[rzm];
+        const int64_t ro = rhs.at(i, j, k), vo = vel.at(i, j, k);
+        const double a0p = R0[ro + 1], a0m = R0[ro - 1], a1p = R1[ro + rhs.py], a1m = R1[ro - rhs.py];
+        const double a2p = R2[ro + rhs.pz], a2m = R2[ro - rhs.pz];
 #pragma unroll
         for (int m = 0; m < 3; m++) {
-            const double *u = vel.p + m * vel.cs;
-            double wdudx = -(a0p * u[vxp] - (a0m)*u[vxm]) / dx2;
-            double wdvdy = -(a1p * u[vyp] - (a1m)*u[vym]) / dy2;
-            double wdwdz = -(a2p * u[vzp] - (a2m)*u[vzm]) / dz2;
+            const double *u = vel.p + m * vel.cs + vo;
+            double wdudx = -(a0p * u[1] - (a0m)*u[-1]) * ix2;
+            double wdvdy = -(a1p * u[vel.py] - (a1m)*u[-vel.py]) * iy2;
+            double wdwdz = -(a2p * u[vel.pz] - (a2m)*u[-vel.pz]) * iz2;
             r[m] = wdudx + wdvdy + wdwdz;
         }
     }
-    int64_t o = out.at(i, j, k);
+    const int64_t o = out.at(i, j, k);
     out.p[o] = r[0];
     out.p[out.cs + o] = r[1];
     out.p[2 * out.cs + o] = r[2];
 }
 int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out) {
-    int64_t nt = (int64_t)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(k_stretch, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, n[0], n[1], n[2],
-                       2 * h[0], 2 * h[1], 2 * h[2], rhs, vel, out);
+    if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_stretch, node_grid(n), dim3(256), 0, c->stream, n[0], n[1], n[2], 1.0 / (2 * h[0]),
+                       1.0 / (2 * h[1]), 1.0 / (2 * h[2]), rhs, vel, out);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

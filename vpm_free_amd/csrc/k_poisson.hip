@@ -1262,9 +1262,11 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     dim3 grid((unsigned)(need < resident ? need : resident), ncomp);
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
-    if (MODE == DST_Y && a.m == c->dst_timed_m && kernel_events(c, K_DST, &e0, &e1)) {
-        // the fine block's y passes: timed by their own dispatch events; work = bytes moved
-        c->k_work[K_DST] = 2.0 * sizeof(T) * (double)a.mx * a.mz * a.m * ncomp;
+    constexpr int kc = MODE == DST_Y ? K_DST : K_DSTZ;
+    if ((MODE == DST_Y || MODE == DST_Z) && a.m == c->dst_timed_m && kernel_events(c, kc, &e0, &e1)) {
+        // the fine block's y and z passes: timed by their own dispatch events; work = bytes
+        // moved (read + write of every interior point)
+        c->k_work[kc] = 2.0 * sizeof(T) * (double)a.mx * (MODE == DST_Y ? a.mz : a.my) * a.m * ncomp;
         VPMX_HIP(c, hipExtLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream, e0, e1, 0));
     } else {
         VPMX_HIP(c, hipLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream));

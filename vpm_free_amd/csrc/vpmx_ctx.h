@@ -24,8 +24,9 @@ enum Stage {
 };
 
 // Kernel classes timed with events when profiling is on.
-// K_DST: the y passes of the fine block's sine transforms.
-enum KClass { K_BOUNDSUM = 0, K_P2M = 1, K_M2P = 2, K_DST = 3, K_COUNT = 4 };
+// K_DST: the y passes of the fine block's sine transforms; K_DSTZ: its z passes (forward,
+// diagonal, inverse).
+enum KClass { K_BOUNDSUM = 0, K_P2M = 1, K_M2P = 2, K_DST = 3, K_DSTZ = 4, K_COUNT = 5 };
 
 struct Ctx : vpmx_ctx {
     int device = 0;

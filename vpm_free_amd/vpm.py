@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libvpmx.so")
 MAXB = 64
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_UNSUP, ERR_IO, ERR_NODEV = range(7)
 STAGES = ["Proj", "Poisson fine", "Bounds", "pmesh_coarse", "pm_bc2", "pmesh_final", "fd", "back", "remeshing"]
-KERNELS = ["boundsum_fine", "p2m", "m2p", "dst_y_fine"]
+KERNELS = ["boundsum_fine", "p2m", "m2p", "dst_y_fine", "dst_z_fine"]
 
 
 class VpmError(RuntimeError):
