@@ -715,7 +715,9 @@ __device__ __forceinline__ void wave_post(Buf B, int N, int lane) {
 // of the odd extension (0, x_1..x_m, 0, -x_m..-x_1); with two lines packed as real and
 // imaginary parts, F_k = -Im Z_k / 2 for the first and Re Z_k / 2 for the second.  No
 // sine weights and no running sums, so in FP32 its error stays at the FFT's own (about
-// 1e-7 relative, ten times below the half-length form's).
+// 1e-7 relative, ten times below the half-length form's).  The kernel writes the mirror
+// image in its LDS fill and reads F_k in its drain; wave_pre_odd serves the x forward pass
+// with Dirichlet values, whose folds change points after the fill.
 template <class C, class Buf>
 __device__ __forceinline__ void wave_pre_odd(Buf B, int m, int lane, const double *fold, double ih2) {
     using T = typename Cx<C>::R;
@@ -745,16 +747,6 @@ __device__ __forceinline__ void wave_pre_odd(Buf B, int m, int lane, const doubl
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class C, class Buf>
-__device__ __forceinline__ void wave_post_odd(Buf B, int m, int lane) {
-    using T = typename Cx<C>::R;
-    const T h = (T)0.5;
-    for (int k = 1 + lane; k <= m; k += 64) {
-        const C z = B[k];
-        B[k] = Cx<C>::mk(-h * z.y, h * z.x);
-    }
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Addressing in the one-wave kernels (16 lines per batch).  dst_elem gives every thread one
 // line (x passes: 32 threads per line; y and z passes: line tid mod 16) and the points
@@ -869,6 +861,8 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     const int tpt0 = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x & (WV_PS - 1) : (int)threadIdx.x / WV_NL;
     constexpr int SSTEP = 2 * (WV_PS + (S ? WV_PS >> S : 0));
     const int slot0 = slot_of(tline, tpt0);
+    // odd-extension form: the mirror element L - (pt + 1) of point pt, stepping down by SSTEP
+    const int mslot0 = 2 * ((tline >> 1) * Np + pad_pos<S>(N - 1 - tpt0)) + (tline & 1);
     const int comp = blockIdx.y;
     T *w = (T *)a.w + comp * a.wcs;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -892,7 +886,12 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     for (; b < nbatch; b = batch_of(++it)) {
 #pragma unroll
         for (int u = 0; u < DST_PF; u++)
-            if (tpt0 + u * WV_PS < m) Ad[slot0 + u * SSTEP] = vok ? (T)v[u] : (T)0;
+            if (tpt0 + u * WV_PS < m) {
+                const T x = vok ? (T)v[u] : (T)0;
+                Ad[slot0 + u * SSTEP] = x;
+                // odd-extension form: the mirror image -x at element L - (pt + 1) as well
+                if constexpr (ODD) Ad[mslot0 - u * SSTEP] = -x;
+            }
         if (MODE == DST_XF && a.has_bc) {
             // lines on the y or z faces of the window (a thread's line is fixed in x passes):
             // fold their Dirichlet neighbours; the x ends are folded in wave_pre
@@ -930,41 +929,60 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
         const int bn = batch_of(it + 1);
         if (bn < nbatch) wave_prefetch<MODE, T>(a, w, comp, bn, tline, tpt0, m, v, vok, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
-        if (ODD) wave_pre_odd<C>(B, m, lane, fold, a.ih2[0]);
-        else wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
+        if constexpr (ODD) {
+            // the fill wrote the odd extension; with Dirichlet values the folds change
+            // points of it, so wave_pre_odd folds the x ends and writes the mirror again
+            if (MODE == DST_XF && a.has_bc) {
+                wave_pre_odd<C>(B, m, lane, fold, a.ih2[0]);
+            } else {
+                if (lane == 0) B[0] = B[m + 1] = Cx<C>::mk(0, 0);
+                __builtin_amdgcn_wave_barrier();
+            }
+        } else {
+            wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
+        }
         if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
-        if (ODD) wave_post_odd<C>(B, m, lane);
-        else wave_post<C>(B, N, lane);
+        if (!ODD) wave_post<C>(B, N, lane);
         if (MODE == DST_Z) {
             const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
             for (int j = 1 + lane; j <= m; j += 64) {
                 const double lz = lzs[j - 1];
                 const C z = B[j];
                 if constexpr (ODD) {
-                    // FP32 form: the factors from the FP32 reciprocal (1 ulp)
-                    const float nz = (float)a.norm;
+                    // odd-extension form: the transform's outputs F = (-Im Z/2, Re Z/2), the
+                    // diagonal (factors from the FP32 reciprocal, 1 ulp) and the odd extension
+                    // of the result for the inverse transform, in one pass
+                    const float nz = (float)a.norm, h = 0.5f;
                     const float fa = -nz * __builtin_amdgcn_rcpf((float)(la + lz));
                     const float fb = -nz * __builtin_amdgcn_rcpf((float)(lb + lz));
-                    B[j] = Cx<C>::mk(z.x * fa, z.y * fb);
+                    const C g = Cx<C>::mk((-h * z.y) * fa, (h * z.x) * fb);
+                    B[j] = g;
+                    B[N - j] = Cx<C>::mk(-g.x, -g.y);
                 } else {
                     B[j] = Cx<C>::mk(z.x * neg_quot(a.norm, la + lz), z.y * neg_quot(a.norm, lb + lz));
                 }
             }
-            __builtin_amdgcn_wave_barrier();
-            if (ODD) wave_pre_odd<C>(B, m, lane, nullptr, 0.0);
-            else wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
+            if constexpr (ODD) {
+                if (lane == 0) B[0] = B[m + 1] = Cx<C>::mk(0, 0);
+                __builtin_amdgcn_wave_barrier();
+            } else {
+                __builtin_amdgcn_wave_barrier();
+                wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
+            }
             if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
-            if (ODD) wave_post_odd<C>(B, m, lane);
-            else wave_post<C>(B, N, lane);
+            if (!ODD) wave_post<C>(B, N, lane);
         }
         __syncthreads();
         const WaveAddr ad = wave_addr<MODE>(a, comp, b * nl + tline, tpt0);
+        // odd-extension form: point k of line 2p is -Im Z_k / 2, of line 2p+1 Re Z_k / 2 (the
+        // other part of the same complex slot)
+        const T hs = ODD ? ((tline & 1) ? (T)0.5 : (T)-0.5) : (T)1;
 #pragma unroll
         for (int u = 0; u < DST_PF; u++) {
             if (!ad.ok || tpt0 + u * WV_PS >= m) continue;
-            const T val = Ad[slot0 + u * SSTEP] * (T)a.scale;
+            const T val = ODD ? (hs * Ad[(slot0 + u * SSTEP) ^ 1]) * (T)a.scale : Ad[slot0 + u * SSTEP] * (T)a.scale;
             if (MODE == DST_XI) a.out.p[ad.dst + u * ad.dst_step] = (double)val;
             else w[ad.dst + u * ad.dst_step] = val;
         }
