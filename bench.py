@@ -265,7 +265,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (mixed form: FP32 Poisson passes; FP64 particles, projection, boundary sums, diagnostics)"
+        "dtype": "f32 (mixed form: FP32 projection weights and sums, Poisson passes and interpolation; FP64 "
+                 "particles, grids in HBM, boundary sums, curl/stretching, diagnostics)"
                  if mixed else "f64",
         "data": "synthetic",
         "config": {

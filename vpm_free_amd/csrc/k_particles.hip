@@ -609,13 +609,16 @@ __global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const 
     if (tile < ntiles && flag[tile]) list[pos[tile]] = tile;
 }
 
+// T: arithmetic of the node values, weights and sums (double: FP64 form; float: mixed form,
+// positions and offsets still in FP64, UP and GP written in FP64).
+template <class T>
 __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict__ start, const int *__restrict__ perm,
                                                   int cnx, int cny, int cnz, const int *__restrict__ list,
                                                   const int *__restrict__ nlist, const double *__restrict__ xp,
                                                   const double *__restrict__ qp, int qstride, int itype, GView vel,
                                                   GView sol, double *__restrict__ up, double *__restrict__ gp) {
 #pragma clang fp contract(off)
-    __shared__ double nodes[M2P_NX * M2P_NY * M2P_NZ * 6];
+    __shared__ T nodes[M2P_NX * M2P_NY * M2P_NZ * 6];
     __shared__ int rs[M2P_ROWS + 1], rp[M2P_ROWS];
     const int ntx = (cnx + M2P_TX - 1) / M2P_TX, nty = (cny + M2P_TY - 1) / M2P_TY;
     const int ntot = *nlist;
@@ -659,7 +662,7 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
             v[3] = sol.p[os]; v[4] = sol.p[sol.cs + os]; v[5] = sol.p[2 * sol.cs + os];
         }
 #pragma unroll
-        for (int q = 0; q < 6; q++) nodes[6 * e + q] = v[q];
+        for (int q = 0; q < 6; q++) nodes[6 * e + q] = (T)v[q];
     }
     __syncthreads();
     const int ips = stencil_ips(itype), ipf = stencil_ipf(itype);
@@ -670,14 +673,14 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
         const int sidx = rp[r] + (t - rs[r]);
         const int64_t p = perm[sidx];
         const double px = xp[3 * p], py = xp[3 * p + 1], pz = xp[3 * p + 2];
-        double u0 = 0, u1 = 0, u2 = 0, g0 = 0, g1 = 0, g2 = 0;
+        T u0 = 0, u1 = 0, u2 = 0, g0 = 0, g1 = 0, g2 = 0;
         const double txx = (px - g.x0[0]) / g.h[0], tyy = (py - g.x0[1]) / g.h[1], tzz = (pz - g.x0[2]) / g.h[2];
         const int in = (int)txx + 1, jn = (int)tyy + 1, kn = (int)tzz + 1;
-        double fx[4], fy[4], fz[4];
+        T fx[4], fy[4], fz[4];
         for (int q = 0; q < 4; q++) {
-            fx[q] = proj_fun(itype, (px - g.x0[0] - (double)(in - ips + q - 1) * g.h[0]) / g.h[0]);
-            fy[q] = proj_fun(itype, (py - g.x0[1] - (double)(jn - ips + q - 1) * g.h[1]) / g.h[1]);
-            fz[q] = proj_fun(itype, (pz - g.x0[2] - (double)(kn - ips + q - 1) * g.h[2]) / g.h[2]);
+            fx[q] = (T)proj_fun(itype, (px - g.x0[0] - (double)(in - ips + q - 1) * g.h[0]) / g.h[0]);
+            fy[q] = (T)proj_fun(itype, (py - g.x0[1] - (double)(jn - ips + q - 1) * g.h[1]) / g.h[1]);
+            fz[q] = (T)proj_fun(itype, (pz - g.x0[2] - (double)(kn - ips + q - 1) * g.h[2]) / g.h[2]);
         }
         for (int c = 0; c < w; c++) {
             const int k = kn - ips + c;
@@ -685,8 +688,8 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
                 const int j = jn - ips + b;
                 for (int a = 0; a < w; a++) {
                     const int i = in - ips + a;
-                    const double f = fx[a] * fy[b] * fz[c];
-                    const double *nd = nodes + 6 * ((i - i0) + M2P_NX * ((j - j0) + M2P_NY * (k - k0)));
+                    const T f = fx[a] * fy[b] * fz[c];
+                    const T *nd = nodes + 6 * ((i - i0) + M2P_NX * ((j - j0) + M2P_NY * (k - k0)));
                     u0 = u0 + f * nd[0];
                     u1 = u1 + f * nd[1];
                     u2 = u2 + f * nd[2];
@@ -697,8 +700,8 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
             }
         }
         const double vol = qp[(int64_t)qstride * p + (qstride - 1)];
-        up[3 * p] = u0; up[3 * p + 1] = u1; up[3 * p + 2] = u2;
-        gp[3 * p] = g0 * vol; gp[3 * p + 1] = g1 * vol; gp[3 * p + 2] = g2 * vol;
+        up[3 * p] = (double)u0; up[3 * p + 1] = (double)u1; up[3 * p + 2] = (double)u2;
+        gp[3 * p] = (double)g0 * vol; gp[3 * p + 1] = (double)g1 * vol; gp[3 * p + 2] = (double)g2 * vol;
     }
     __syncthreads();  // the next tile reuses the LDS
     }
@@ -732,13 +735,19 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
                            c->stream, tiles, start, s.cn[0], s.cn[1], s.cn[2], flag);
         VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
         hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, tiles, flag, pos, list);
-        const unsigned wg = (unsigned)(4 * c->num_cu);  // persistent: 4 resident work-groups per CU (LDS-bound)
+        // persistent: the work-groups one CU holds (LDS-bound: 40 KB of FP64 node values, 20 KB
+        // of FP32 in the mixed form)
+        const bool f32 = c->precision == 1;
+        const void *fn = f32 ? (const void *)k_m2p_tile<float> : (const void *)k_m2p_tile<double>;
+        const unsigned wg = (unsigned)((f32 ? 8 : 4) * c->num_cu);
+        int gn = (int)s.cn[0], gny = (int)s.cn[1], gnz = (int)s.cn[2];
+        const int *nl = pos + tiles;
+        void *args[] = {(void *)&g, (void *)&start, (void *)&perm, &gn, &gny, &gnz, (void *)&list, (void *)&nl,
+                        (void *)&xp, (void *)&qp, &qstride, &itype, &vel, &sol, &up, &gp};
         if (kernel_events(c, K_M2P, &e0, &e1))
-            hipExtLaunchKernelGGL(k_m2p_tile, dim3(wg), dim3(256), 0, c->stream, e0, e1, 0, g, start, perm, s.cn[0],
-                                  s.cn[1], s.cn[2], list, pos + tiles, xp, qp, qstride, itype, vel, sol, up, gp);
+            VPMX_HIP(c, hipExtLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream, e0, e1, 0));
         else
-            hipLaunchKernelGGL(k_m2p_tile, dim3(wg), dim3(256), 0, c->stream, g, start, perm, s.cn[0], s.cn[1],
-                               s.cn[2], list, pos + tiles, xp, qp, qstride, itype, vel, sol, up, gp);
+            VPMX_HIP(c, hipLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream));
         VPMX_HIP(c, hipGetLastError());
         return VPMX_OK;
     }
