@@ -452,6 +452,207 @@ __global__ void __launch_bounds__(256) k_p2m_rows(Grid g, const int *__restrict_
     }
 }
 
+// M4' projection on node tiles.  A work-group takes 16 x 4 x 4 nodes (one per thread); the
+// particles that reach them lie in 19 x 7 x 7 cells (the particles of cell c are visited by
+// nodes c-1..c+2), 49 cell rows whose particles are contiguous in cell order.  The rows'
+// particles are staged in LDS, each with the M4' weights of the 4 x 4 x 4 nodes that visit
+// it (W((x - x0)/h - (i - 1)) for node i), formed once per tile, in chunks of whole rows that
+// fit; every thread then sums its node's 16 rows, the four cells of each, in the order of
+// k_p2m_rows (rows z-outer, y-inner, cells and particles in cell order): deterministic, no
+// atomics.  A row with more particles than a chunk holds is read straight from memory in the
+// same order.  Tiles without particles are written by k_p2m_zero_tiles.
+constexpr int PT_X = 16, PT_Y = 4, PT_Z = 4;
+constexpr int PC_X = PT_X + 3, PC_Y = PT_Y + 3, PC_Z = PT_Z + 3;
+constexpr int PT_ROWS = PC_Y * PC_Z;
+// The staged particles in LDS, one array per field (neighbouring lanes read neighbouring
+// particles, so a field read is a unit-stride access across the wave): w[4 d + q][p] the
+// weight of the q-th node that visits particle p along axis d (nodes c_d - 1 .. c_d + 2 for
+// the particle's cell c_d in the cell order), q[c][p] the strengths.
+template <class T, int MAXP>
+struct P2mStage {
+    T w[12][MAXP];
+    T q[3][MAXP];
+};
+
+// Tiles whose 49 cell rows hold any particle (a superset of the tiles with work); thread per
+// tile, so the row-count loads of many tiles are in flight together.
+__global__ void k_p2m_tile_flags(int ntiles, int ntx, int nty, int vnx, int vny, int vnz, Grid g, int cny,
+                                 const int *__restrict__ rowcnt, int wlo1, int wlo2, int *__restrict__ flag) {
+    const int tile = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tile >= ntiles) return;
+    const int ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int J0 = ty * PT_Y - wlo1 + 1, K0 = tz * PT_Z - wlo2 + 1;
+    int any = 0;
+    for (int r = 0; r < PT_ROWS; r++) {
+        const int cj = J0 - 2 + r % PC_Y, ck = K0 - 2 + r / PC_Y;
+        if (cj >= -1 && cj <= g.n[1] + 1 && ck >= -1 && ck <= g.n[2] + 1)
+            any |= rowcnt[(cj + 1) + (int64_t)cny * (ck + 1)] > 0;
+    }
+    flag[tile] = any;
+}
+
+// Zeros for the nodes of the view in tiles without particles (one thread per node).
+__global__ void k_p2m_zero_tiles(int ntx, int nty, int vnx, int vny, int vnz, const int *__restrict__ flag, GView out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)vnx * vny * vnz) return;
+    const int va = (int)(t % vnx), vb = (int)((t / vnx) % vny), vc = (int)(t / ((int64_t)vnx * vny));
+    const int tile = va / PT_X + ntx * (vb / PT_Y + nty * (vc / PT_Z));
+    if (flag[tile]) return;
+    const int64_t o = out.at(va, vb, vc);
+#pragma unroll
+    for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = 0.0;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict__ start,
+                                                  const int *__restrict__ rowcnt, int cnx, int cny,
+                                                  const double *__restrict__ xs, const double *__restrict__ qs,
+                                                  double div, GView out, int vnx, int vny, int vnz, int wlo0,
+                                                  int wlo1, int wlo2, const int *__restrict__ list,
+                                                  const int *__restrict__ nlist) {
+#pragma clang fp contract(off)
+    // particles per chunk (measured on the 273-node grid: 256 FP32 / 192 FP64 keep 5-6
+    // work-groups per CU, enough to cover the load latencies of the row tables and staging)
+    constexpr int MAXP = sizeof(T) == 4 ? 256 : 192;
+    __shared__ P2mStage<T, MAXP> S;
+    __shared__ int cst[PT_ROWS][PC_X + 1];  // sorted index of the first particle of each cell, and the row's end
+    __shared__ int rbase[PT_ROWS + 1];      // exclusive prefix of the rows' counts
+    __shared__ int rowok[PT_ROWS];
+    __shared__ int any;
+    const int tid = threadIdx.x;
+    const int ntx = (vnx + PT_X - 1) / PT_X, nty = (vny + PT_Y - 1) / PT_Y;
+    if ((int)blockIdx.x >= *nlist) return;
+    const int tile = list[blockIdx.x];
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int lx = tid & (PT_X - 1), ly = (tid / PT_X) & (PT_Y - 1), lz = tid / (PT_X * PT_Y);
+    const int va = tx * PT_X + lx, vb = ty * PT_Y + ly, vc = tz * PT_Z + lz;
+    // 1-based grid nodes: this thread's, and the tile's first
+    const int i = va - wlo0 + 1, j = vb - wlo1 + 1, k = vc - wlo2 + 1;
+    const int I0 = tx * PT_X - wlo0 + 1, J0 = ty * PT_Y - wlo1 + 1, K0 = tz * PT_Z - wlo2 + 1;
+    if (tid == 0) any = 0;
+    __syncthreads();
+    if (tid < PT_ROWS) {
+        const int cj = J0 - 2 + tid % PC_Y, ck = K0 - 2 + tid / PC_Y;
+        int ok = cj >= -1 && cj <= g.n[1] + 1 && ck >= -1 && ck <= g.n[2] + 1;
+        if (ok) ok = rowcnt[(cj + 1) + (int64_t)cny * (ck + 1)] > 0;
+        rowok[tid] = ok;
+        if (ok) any = 1;
+    }
+    __syncthreads();
+    T acc[3] = {T(0), T(0), T(0)};
+    if (any) {
+        for (int e = tid; e < PT_ROWS * (PC_X + 1); e += blockDim.x) {
+            const int r = e / (PC_X + 1), cidx = e - r * (PC_X + 1);
+            int v = 0;
+            if (rowok[r]) {
+                const int cj = J0 - 2 + r % PC_Y, ck = K0 - 2 + r / PC_Y;
+                const int64_t row = (int64_t)cnx * ((cj + 1) + (int64_t)cny * (ck + 1));
+                const int cx = min(max(I0 - 2 + cidx, -1), g.n[0] + 2);
+                v = start[row + cx + 1];
+            }
+            cst[r][cidx] = v;
+        }
+        __syncthreads();
+        if (tid < 64) {  // the rows' counts, scanned across the first wave
+            int v = tid < PT_ROWS ? cst[tid][PC_X] - cst[tid][0] : 0;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int o = __shfl_up(v, d, 64);
+                if (tid >= d) v += o;
+            }
+            if (tid < PT_ROWS) rbase[tid + 1] = v;
+            if (tid == 0) rbase[0] = 0;
+        }
+        __syncthreads();
+        const double ih[3] = {1.0 / g.h[0], 1.0 / g.h[1], 1.0 / g.h[2]};
+        int rb = 0;
+        while (rb < PT_ROWS) {
+            int re = rb;
+            while (re < PT_ROWS && rbase[re + 1] - rbase[rb] <= MAXP) re++;
+            const bool direct = re == rb;  // one row larger than a chunk
+            if (direct) re = rb + 1;
+            const int cnt = rbase[re] - rbase[rb];
+            if (!direct && cnt > 0) {
+                for (int e = tid; e < cnt; e += blockDim.x) {
+                    // the row of staged element e (binary search over the chunk's rows)
+                    int lo = rb, hi = re - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (rbase[mid] - rbase[rb] <= e) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const int64_t sidx = cst[lo][0] + (e - (rbase[lo] - rbase[rb]));
+                    // its cell along x (the last cell of the row starting at or before it)
+                    int cl = 0, ch = PC_X - 1;
+                    while (cl < ch) {
+                        const int mid = (cl + ch + 1) >> 1;
+                        if (cst[lo][mid] <= sidx) cl = mid;
+                        else ch = mid - 1;
+                    }
+                    // the first node that visits the particle along each axis: its cell - 1
+                    const int n0[3] = {I0 - 3 + cl, J0 - 3 + lo % PC_Y, K0 - 3 + lo / PC_Y};
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double t = (xs[3 * sidx + d] - g.x0[d]) * ih[d];
+#pragma unroll
+                        for (int q = 0; q < 4; q++) S.w[4 * d + q][e] = m4p_t<T>((T)(t - (double)(n0[d] + q - 1)));
+                        S.q[d][e] = (T)qs[3 * sidx + d];
+                    }
+                }
+            }
+            __syncthreads();
+            // this node's rows of the chunk: z-outer, y-inner, as in k_p2m_rows; in row (dy, dz)
+            // and cell dc of its 4 x 4 x 4 neighbourhood the node is the (3 - dc, 3 - dy, 3 - dz)-th
+            // visitor of the particles
+#pragma unroll
+            for (int dz = 0; dz < 4; dz++) {
+#pragma unroll
+                for (int dy = 0; dy < 4; dy++) {
+                    const int r = (ly + dy) + PC_Y * (lz + dz);
+                    if (r < rb || r >= re) continue;
+                    if (!direct) {
+                        const int off = rbase[r] - rbase[rb] - cst[r][0];
+#pragma unroll
+                        for (int dc = 0; dc < 4; dc++) {
+                            const int p0 = cst[r][lx + dc] + off, p1 = cst[r][lx + dc + 1] + off;
+                            for (int p = p0; p < p1; p++) {
+                                const T f = (S.w[3 - dc][p] * S.w[4 + 3 - dy][p]) * S.w[8 + 3 - dz][p];
+                                acc[0] = acc[0] + f * S.q[0][p];
+                                acc[1] = acc[1] + f * S.q[1][p];
+                                acc[2] = acc[2] + f * S.q[2][p];
+                            }
+                        }
+                    } else {
+                        for (int sp = cst[r][lx]; sp < cst[r][lx + 4]; sp++) {
+                            T w[3];
+#pragma unroll
+                            for (int d = 0; d < 3; d++) {
+                                const double t = (xs[3 * (int64_t)sp + d] - g.x0[d]) * ih[d];
+                                const int node = d == 0 ? i : (d == 1 ? j : k);
+                                w[d] = m4p_t<T>((T)(t - (double)(node - 1)));
+                            }
+                            const T f = (w[0] * w[1]) * w[2];
+                            acc[0] = acc[0] + f * (T)qs[3 * (int64_t)sp];
+                            acc[1] = acc[1] + f * (T)qs[3 * (int64_t)sp + 1];
+                            acc[2] = acc[2] + f * (T)qs[3 * (int64_t)sp + 2];
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // the next chunk reuses the LDS
+            rb = re;
+        }
+    }
+    if (va >= vnx || vb >= vny || vc >= vnz) return;
+    const bool in = i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
+    const int64_t o = out.at(va, vb, vc);
+#pragma unroll
+    for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = in ? (double)acc[r] / div : 0.0;
+}
+
+__global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const int *__restrict__ pos,
+                                int *__restrict__ list);
+
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
                int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32) {
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
@@ -471,6 +672,35 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         if (n > 0)
             hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
                                start + s.ncells, n, xp, qp, qstride, xs, qs);
+        if (c->p2m_tile) {
+            const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
+            int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
+            if (st) return st;
+            hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
+                               s.cn[0], nrows, rowcnt);
+            const int ntx = (vn[0] + PT_X - 1) / PT_X, nty = (vn[1] + PT_Y - 1) / PT_Y;
+            const int64_t tiles = (int64_t)ntx * nty * ((vn[2] + PT_Z - 1) / PT_Z);
+            int *flag = (int *)ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+            int *pos = (int *)ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles + 1), &st); if (st) return st;
+            int *list = (int *)ensure(c, c->tile_list, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+            const unsigned gt = (unsigned)((tiles + 255) / 256);
+            hipLaunchKernelGGL(k_p2m_tile_flags, dim3(gt), dim3(256), 0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1],
+                               vn[2], g, s.cn[1], rowcnt, wlo[1], wlo[2], flag);
+            VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
+            hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
+            hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx, nty,
+                               vn[0], vn[1], vn[2], flag, out);
+            const bool timed = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
+            const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
+            int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
+            const int *lst = list, *nl = pos + tiles;
+            void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, (void *)&xs, (void *)&qs, &div,
+                            &out, &vx, &vy, &vz, &w0, &w1, &w2, (void *)&lst, (void *)&nl};
+            if (timed) VPMX_HIP(c, hipExtLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream, e0, e1, 0));
+            else VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
+            VPMX_HIP(c, hipGetLastError());
+            return VPMX_OK;
+        }
         if (c->p2m_rows) {
             const int64_t nth = (int64_t)((vn[0] + 3) / 4) * vn[1] * vn[2];
             const unsigned gr = (unsigned)((nth + 255) / 256);

@@ -585,6 +585,7 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     const char *pb = getenv("VPMX_P2M");
     if (pb && std::string(pb) == "node") c->p2m_block = false;
     if (pb && std::string(pb) == "block") c->p2m_rows = false;
+    if (pb && (std::string(pb) == "rows" || std::string(pb) == "block" || std::string(pb) == "node")) c->p2m_tile = false;
     const char *mb = getenv("VPMX_M2P");
     if (mb && std::string(mb) == "particle") c->m2p_tile = false;
     const char *pm = getenv("VPMX_PRECISION");

@@ -66,6 +66,7 @@ struct Ctx : vpmx_ctx {
     int64_t sort_n = 0;     // particles of the last sort
     bool p2m_block = true;  // projection: node-run / node-block kernels (else one thread per node)
     bool p2m_rows = true;   // projection: 4 x-nodes per thread, cell-row walks (else 4x2x2 node blocks)
+    bool p2m_tile = true;   // projection: node tiles with the particles staged in LDS (else the above)
     bool m2p_tile = true;   // interpolation: cell tiles with the nodes staged in LDS
     std::map<int, void *> fft_plans;
     bool have_grids = false;  // RHS/SOL/vel of the last mode 1-3 call are valid
