@@ -669,6 +669,9 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         if (st) return st;
         double *qs = (double *)ensure(c, c->sort.qs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
         if (st) return st;
+        // the tile path is timed as a whole (copy, row counts, tile list, zeros, tiles)
+        const bool timed_all = c->p2m_tile && kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
+        if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
         if (n > 0)
             hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
                                start + s.ncells, n, xp, qp, qstride, xs, qs);
@@ -690,15 +693,14 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
             hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
             hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx, nty,
                                vn[0], vn[1], vn[2], flag, out);
-            const bool timed = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
             const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
             int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
             const int *lst = list, *nl = pos + tiles;
             void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, (void *)&xs, (void *)&qs, &div,
                             &out, &vx, &vy, &vz, &w0, &w1, &w2, (void *)&lst, (void *)&nl};
-            if (timed) VPMX_HIP(c, hipExtLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream, e0, e1, 0));
-            else VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
+            VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
             VPMX_HIP(c, hipGetLastError());
+            if (timed_all) VPMX_HIP(c, hipEventRecord(e1, c->stream));
             return VPMX_OK;
         }
         if (c->p2m_rows) {
@@ -960,6 +962,9 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         int *pos = (int *)ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles + 1), &st); if (st) return st;
         int *list = (int *)ensure(c, c->tile_list, sizeof(int) * (size_t)tiles, &st); if (st) return st;
         const unsigned gt = (unsigned)((tiles + 255) / 256);
+        // timed as a whole (tile list and the tiles)
+        const bool timed = kernel_events(c, K_M2P, &e0, &e1);
+        if (timed) VPMX_HIP(c, hipEventRecord(e0, c->stream));
         hipLaunchKernelGGL(k_m2p_outside, grid, dim3(256), 0, c->stream, keys, n, up, gp);
         hipLaunchKernelGGL(k_m2p_tile_flags, dim3((unsigned)(((int64_t)tiles * 32 + 255) / 256)), dim3(256), 0,
                            c->stream, tiles, start, s.cn[0], s.cn[1], s.cn[2], flag);
@@ -974,11 +979,9 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         const int *nl = pos + tiles;
         void *args[] = {(void *)&g, (void *)&start, (void *)&perm, &gn, &gny, &gnz, (void *)&list, (void *)&nl,
                         (void *)&xp, (void *)&qp, &qstride, &itype, &vel, &sol, &up, &gp};
-        if (kernel_events(c, K_M2P, &e0, &e1))
-            VPMX_HIP(c, hipExtLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream, e0, e1, 0));
-        else
-            VPMX_HIP(c, hipLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream));
+        VPMX_HIP(c, hipLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream));
         VPMX_HIP(c, hipGetLastError());
+        if (timed) VPMX_HIP(c, hipEventRecord(e1, c->stream));
         return VPMX_OK;
     }
     if (itypeb == 2)
