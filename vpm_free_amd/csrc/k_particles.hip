@@ -474,6 +474,42 @@ struct P2mStage {
     T q[3][MAXP];
 };
 
+// Cell-ordered particle records for the tiled projection: for sorted particle s (particle
+// perm[s]) R[16 s + 4 d + q] = W(t_d - (c_d + q - 2)), the M4' weight of the q-th node that
+// visits it along axis d (node c_d - 1 + q, c_d its cell as the sort computed it,
+// int((x - x0)/h) + 1, t_d = (x - x0)/h), and R[16 s + 12 + c] its strengths: 16-word records,
+// written and read with vector accesses.  Formed once per projection instead of once per tile
+// that stages the particle.
+template <class T>
+__global__ void k_p2m_sorted(const int *__restrict__ perm, const int *__restrict__ total, int64_t n, Grid g,
+                             const double *__restrict__ xp, const double *__restrict__ qp, int qstride,
+                             T *__restrict__ R) {
+#pragma clang fp contract(off)
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || s >= (int64_t)*total) return;
+    const int64_t p = perm[s];
+    T r[16];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double x = xp[3 * p + d];
+        const int cell = (int)((x - g.x0[d]) / g.h[d]) + 1;
+        const double t = (x - g.x0[d]) * (1.0 / g.h[d]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) r[4 * d + q] = m4p_t<T>((T)(t - (double)(cell + q - 2)));
+        r[12 + d] = (T)qp[(int64_t)qstride * p + d];
+    }
+    r[15] = T(0);
+    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+    constexpr int NV = 16 * (int)sizeof(T) / (int)sizeof(V);
+    V *dst = (V *)(R + 16 * s);
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        V x;
+        __builtin_memcpy(&x, r + v * (int)(sizeof(V) / sizeof(T)), sizeof(V));
+        dst[v] = x;
+    }
+}
+
 // Tiles whose 49 cell rows hold any particle (a superset of the tiles with work); thread per
 // tile, so the row-count loads of many tiles are in flight together.
 __global__ void k_p2m_tile_flags(int ntiles, int ntx, int nty, int vnx, int vny, int vnz, Grid g, int cny,
@@ -506,8 +542,8 @@ __global__ void k_p2m_zero_tiles(int ntx, int nty, int vnx, int vny, int vnz, co
 template <class T>
 __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict__ start,
                                                   const int *__restrict__ rowcnt, int cnx, int cny,
-                                                  const double *__restrict__ xs, const double *__restrict__ qs,
-                                                  double div, GView out, int vnx, int vny, int vnz, int wlo0,
+                                                  const T *__restrict__ R, double div, GView out, int vnx,
+                                                  int vny, int vnz, int wlo0,
                                                   int wlo1, int wlo2, const int *__restrict__ list,
                                                   const int *__restrict__ nlist) {
 #pragma clang fp contract(off)
@@ -564,7 +600,6 @@ __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict_
             if (tid == 0) rbase[0] = 0;
         }
         __syncthreads();
-        const double ih[3] = {1.0 / g.h[0], 1.0 / g.h[1], 1.0 / g.h[2]};
         int rb = 0;
         while (rb < PT_ROWS) {
             int re = rb;
@@ -582,22 +617,19 @@ __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict_
                         else hi = mid - 1;
                     }
                     const int64_t sidx = cst[lo][0] + (e - (rbase[lo] - rbase[rb]));
-                    // its cell along x (the last cell of the row starting at or before it)
-                    int cl = 0, ch = PC_X - 1;
-                    while (cl < ch) {
-                        const int mid = (cl + ch + 1) >> 1;
-                        if (cst[lo][mid] <= sidx) cl = mid;
-                        else ch = mid - 1;
-                    }
-                    // the first node that visits the particle along each axis: its cell - 1
-                    const int n0[3] = {I0 - 3 + cl, J0 - 3 + lo % PC_Y, K0 - 3 + lo / PC_Y};
+                    using V = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+                    constexpr int NV = 16 * (int)sizeof(T) / (int)sizeof(V), PV = (int)(sizeof(V) / sizeof(T));
+                    T r[16];
+                    const V *src = (const V *)(R + 16 * sidx);
 #pragma unroll
-                    for (int d = 0; d < 3; d++) {
-                        const double t = (xs[3 * sidx + d] - g.x0[d]) * ih[d];
-#pragma unroll
-                        for (int q = 0; q < 4; q++) S.w[4 * d + q][e] = m4p_t<T>((T)(t - (double)(n0[d] + q - 1)));
-                        S.q[d][e] = (T)qs[3 * sidx + d];
+                    for (int v = 0; v < NV; v++) {
+                        const V x = src[v];
+                        __builtin_memcpy(r + v * PV, &x, sizeof(V));
                     }
+#pragma unroll
+                    for (int q = 0; q < 12; q++) S.w[q][e] = r[q];
+#pragma unroll
+                    for (int c = 0; c < 3; c++) S.q[c][e] = r[12 + c];
                 }
             }
             __syncthreads();
@@ -623,18 +655,15 @@ __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict_
                             }
                         }
                     } else {
-                        for (int sp = cst[r][lx]; sp < cst[r][lx + 4]; sp++) {
-                            T w[3];
 #pragma unroll
-                            for (int d = 0; d < 3; d++) {
-                                const double t = (xs[3 * (int64_t)sp + d] - g.x0[d]) * ih[d];
-                                const int node = d == 0 ? i : (d == 1 ? j : k);
-                                w[d] = m4p_t<T>((T)(t - (double)(node - 1)));
+                        for (int dc = 0; dc < 4; dc++) {
+                            for (int64_t sp = cst[r][lx + dc]; sp < cst[r][lx + dc + 1]; sp++) {
+                                const T *w = R + 16 * sp;
+                                const T f = (w[3 - dc] * w[7 - dy]) * w[11 - dz];
+                                acc[0] = acc[0] + f * w[12];
+                                acc[1] = acc[1] + f * w[13];
+                                acc[2] = acc[2] + f * w[14];
                             }
-                            const T f = (w[0] * w[1]) * w[2];
-                            acc[0] = acc[0] + f * (T)qs[3 * (int64_t)sp];
-                            acc[1] = acc[1] + f * (T)qs[3 * (int64_t)sp + 1];
-                            acc[2] = acc[2] + f * (T)qs[3 * (int64_t)sp + 2];
                         }
                     }
                 }
@@ -669,16 +698,35 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         if (st) return st;
         double *qs = (double *)ensure(c, c->sort.qs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
         if (st) return st;
-        // the tile path is timed as a whole (copy, row counts, tile list, zeros, tiles)
-        const bool timed_all = c->p2m_tile && kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
-        if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
-        if (n > 0)
+        if (n > 0 && !c->p2m_tile)
             hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
                                start + s.ncells, n, xp, qp, qstride, xs, qs);
         if (c->p2m_tile) {
+            // buffers first (a growing one drains the stream), then the path, timed as a whole
+            // (records, row counts, tile list, zeros, tiles); the cell-ordered particle records
+            // use the space of the cell-ordered copy, which this path does not use
+            const size_t ws = f32 ? sizeof(float) : sizeof(double);
+            void *Rv = ensure(c, c->sort.xs, ws * 16 * (size_t)(n > 0 ? n : 1), &st);
+            if (st) return st;
             const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
             int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
             if (st) return st;
+            const int ntx0 = (vn[0] + PT_X - 1) / PT_X, nty0 = (vn[1] + PT_Y - 1) / PT_Y;
+            const int64_t tiles0 = (int64_t)ntx0 * nty0 * ((vn[2] + PT_Z - 1) / PT_Z);
+            if (!ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles0, &st)) return st;
+            if (!ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles0 + 1), &st)) return st;
+            if (!ensure(c, c->tile_list, sizeof(int) * (size_t)tiles0, &st)) return st;
+            const bool timed_all = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
+            if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
+            if (n > 0) {
+                const dim3 gs((unsigned)((n + 255) / 256));
+                if (f32)
+                    hipLaunchKernelGGL(k_p2m_sorted<float>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n, g,
+                                       xp, qp, qstride, (float *)Rv);
+                else
+                    hipLaunchKernelGGL(k_p2m_sorted<double>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n,
+                                       g, xp, qp, qstride, (double *)Rv);
+            }
             hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
                                s.cn[0], nrows, rowcnt);
             const int ntx = (vn[0] + PT_X - 1) / PT_X, nty = (vn[1] + PT_Y - 1) / PT_Y;
@@ -696,7 +744,7 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
             const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
             int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
             const int *lst = list, *nl = pos + tiles;
-            void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, (void *)&xs, (void *)&qs, &div,
+            void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, &Rv, &div,
                             &out, &vx, &vy, &vz, &w0, &w1, &w2, (void *)&lst, (void *)&nl};
             VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
             VPMX_HIP(c, hipGetLastError());

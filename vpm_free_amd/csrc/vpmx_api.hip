@@ -27,6 +27,9 @@ int fail(Ctx *c, int code, const std::string &msg) {
 void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st) {
     if (bytes == 0) bytes = 8;
     if (b.bytes >= bytes) return b.p;
+    // growing: an eighth more than asked, so slowly growing particle counts do not drain the
+    // stream and reallocate on every call
+    if (b.p) bytes += bytes / 8;
     if (b.p) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(b.p);
