@@ -157,9 +157,10 @@ int32_t vpmx_stage_project(vpmx_ctx *ctx, const double xbound[6], const double d
                            const int32_t nn_bl[6], const double *xp, const double *qp, int64_t nvr,
                            int32_t ncomp, int32_t itype, int32_t idvpm, double *rhs);
 
-/* Timing of the last vpm/step call per stage (ms, host clock between stream drains), in
- * the order of the reference's timer labels: Proj, Poisson fine, Bounds, pmesh_coarse,
- * pm_bc2, pmesh_final, fd, back, remeshing. */
+/* Timing of the last vpm/step call per stage (ms of device time between events recorded on
+ * the library's stream at the stage's start and end, summed over the stage's runs; read back
+ * when the call ends, so timing adds no stream drain), in the order of the reference's timer
+ * labels: Proj, Poisson fine, Bounds, pmesh_coarse, pm_bc2, pmesh_final, fd, back, remeshing. */
 int32_t vpmx_stage_times(const vpmx_ctx *ctx, double *ms, int32_t n);
 /* Accumulated device time (ms, from each dispatch's start/stop events) and launch counts
  * of the timed kernel classes: 0 fine-block boundary sum (Bounds3d, pmbound.f90:80-157),

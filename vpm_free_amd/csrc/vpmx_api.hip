@@ -89,7 +89,8 @@ double wall_ms() {
     using namespace std::chrono;
     return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
 }
-static void collect_times(Ctx *c, const bool *) { (void)hipStreamSynchronize(c->stream); }
+// (stage times are read back when the outermost call's TimingScope ends)
+static void collect_times(Ctx *, const bool *) {}
 
 // ---------------------------------------------------------------- pmesh (iynbc=1)
 // pmesh(iynbc=1) on grid g: SOL_0 = zero-BC solve, sources from its normal derivatives,
@@ -389,8 +390,8 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
     if (n == 0) return VPMX_OK;  // vpm.f90:172
     const vpmx_geom &G = c->geom;
     const vpmx_params &P = c->prm;
+    TimingScope timing(c);
     bool used[ST_COUNT] = {};
-    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = 0.0;
     c->last_pairs = 0.0;
     const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
     int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
@@ -489,8 +490,8 @@ static int run_remesh(Ctx *c, const double *xp, const double *qp, int64_t n, int
     const vpmx_params &P = c->prm;
     if (P.ncell_rem != 1) return fail(c, VPMX_ERR_UNSUP, "ncell_rem must be 1");
     if (mrem < 1) return fail(c, VPMX_ERR_ARG, "mrem must be >= 1");
+    TimingScope timing(c);
     bool used[ST_COUNT] = {};
-    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = 0.0;
     mark(c, ST_REMESH, 0);
     int32_t NN[3], NNb[6];
     double D[3];
@@ -619,6 +620,10 @@ void vpmx_destroy(vpmx_ctx *p) {
     for (int k = 0; k < K_COUNT; k++) {
         for (hipEvent_t e : c->kev0[k]) (void)hipEventDestroy(e);
         for (hipEvent_t e : c->kev1[k]) (void)hipEventDestroy(e);
+    }
+    for (int k = 0; k < ST_COUNT; k++) {
+        for (hipEvent_t e : c->sev0[k]) (void)hipEventDestroy(e);
+        for (hipEvent_t e : c->sev1[k]) (void)hipEventDestroy(e);
     }
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -821,26 +826,22 @@ int32_t vpmx_step_rk2(vpmx_ctx *p, double dt, const double uinf[3], int32_t ntim
     double *x0 = (double *)ensure(c, c->x0, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
     double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
     double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
+    TimingScope timing(c);  // the stage times of the whole step
     if (n > 0) {
         VPMX_HIP(c, hipMemcpyAsync(x0, xp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
         VPMX_HIP(c, hipMemcpyAsync(q0, qp, sizeof(double) * nr * n, hipMemcpyDeviceToDevice, c->stream));
     }
-    double acc[ST_COUNT] = {};
     // stage 1 at (x0, q0)
     VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
-    for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
     VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
     // stage 2 at the midpoint
     VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
-    for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
     VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, dt, uinf, x0, q0));
     if (do_remesh) {
         VPMX_TRY(define_from_device(c, ntime));
         VPMX_TRY(run_remesh(c, xp, qp, n, mrem));
-        for (int s = 0; s < ST_COUNT; s++) acc[s] += c->stage_ms[s];
         VPMX_TRY(replace_with_remesh(c));
     }
-    for (int s = 0; s < ST_COUNT; s++) c->stage_ms[s] = acc[s];
     VPMX_HIP(c, hipStreamSynchronize(c->stream));
     return VPMX_OK;
 }

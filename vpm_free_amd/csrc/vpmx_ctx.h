@@ -75,10 +75,13 @@ struct Ctx : vpmx_ctx {
     int64_t rm_count = -1;
     double *rm_xp = nullptr, *rm_qp = nullptr;
 
-    // timing: stages by host wall clock between stream drains; kernels of class K_* by the
-    // dispatch's own start/stop events (hipExtLaunchKernel), read back at the next drain
+    // timing: stages by events recorded on the stream at their boundaries, kernels of class
+    // K_* by the dispatch's own start/stop events (hipExtLaunchKernel); both read back when
+    // the outermost API call ends (TimingScope), so no stage drains the stream
     bool timing = true;
-    double stage_t0[ST_COUNT] = {};
+    int timing_depth = 0;
+    std::vector<hipEvent_t> sev0[ST_COUNT], sev1[ST_COUNT];
+    int sev_used[ST_COUNT] = {};
     double stage_ms[ST_COUNT] = {};
     double last_pairs = 0.0;
     std::vector<hipEvent_t> kev0[K_COUNT], kev1[K_COUNT];
@@ -107,15 +110,52 @@ inline void resolve_kernel_times(Ctx *c) {
         c->kev_pending[k] = 0;
     }
 }
-// Stage timing: drain the stream at both ends so the interval covers exactly the stage's
-// kernels (enqueueing is asynchronous).
+// Stage timing: an event on the stream at the stage's start and end (a stage may run several
+// times in one call; each run gets its own pair).
 inline void mark(Ctx *c, int stage, int end) {
-    if (!c->timing) return;
-    (void)hipStreamSynchronize(c->stream);
-    resolve_kernel_times(c);
-    if (!end) c->stage_t0[stage] = wall_ms();
-    else c->stage_ms[stage] += wall_ms() - c->stage_t0[stage];
+    if (!c->timing || !c->kev_ok || c->timing_depth == 0) return;
+    if (!end) {
+        const int i = c->sev_used[stage];
+        if (i == (int)c->sev0[stage].size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess) return;
+            if (hipEventCreate(&b) != hipSuccess) {
+                (void)hipEventDestroy(a);
+                return;
+            }
+            c->sev0[stage].push_back(a);
+            c->sev1[stage].push_back(b);
+        }
+        if (hipEventRecord(c->sev0[stage][i], c->stream) == hipSuccess) c->sev_used[stage] = i + 1;
+    } else if (c->sev_used[stage] > 0) {
+        (void)hipEventRecord(c->sev1[stage][c->sev_used[stage] - 1], c->stream);
+    }
 }
+// The outermost API call that runs stages: zero the stage times at its start, drain the
+// stream at its end and add up the recorded intervals (and the timed kernels).
+struct TimingScope {
+    Ctx *c;
+    explicit TimingScope(Ctx *cc) : c(cc) {
+        if (c->timing_depth++ == 0) {
+            for (int s = 0; s < ST_COUNT; s++) {
+                c->stage_ms[s] = 0.0;
+                c->sev_used[s] = 0;
+            }
+        }
+    }
+    ~TimingScope() {
+        if (--c->timing_depth != 0) return;
+        (void)hipStreamSynchronize(c->stream);
+        for (int s = 0; s < ST_COUNT; s++) {
+            for (int i = 0; i < c->sev_used[s]; i++) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->sev0[s][i], c->sev1[s][i]) == hipSuccess) c->stage_ms[s] += ms;
+            }
+            c->sev_used[s] = 0;
+        }
+        resolve_kernel_times(c);
+    }
+};
 // Start/stop events for the next timed dispatch of class k (false: time nothing).
 inline bool kernel_events(Ctx *c, int k, hipEvent_t *e0, hipEvent_t *e1) {
     if (!c->kev_ok) return false;
