@@ -47,6 +47,7 @@ struct DstArgs {
     const double *lam[3];  // eigenvalues (z pass)
     double norm;
     const double *sn;      // sin(j pi / N), j <= N/2 (half-length passes)
+    int ncomp;             // components (one-wave kernels: all in one grid)
 };
 
 // Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
@@ -863,8 +864,8 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     const int slot0 = slot_of(tline, tpt0);
     // odd-extension form: the mirror element L - (pt + 1) of point pt, stepping down by SSTEP
     const int mslot0 = 2 * ((tline >> 1) * Np + pad_pos<S>(N - 1 - tpt0)) + (tline & 1);
-    const int comp = blockIdx.y;
-    T *w = (T *)a.w + comp * a.wcs;
+    // (every component's batches in one persistent grid: global batch gb is batch
+    // gb mod nbatch of component gb / nbatch)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     // (S = 0: the plain pointer, so those kernels compile exactly as unpadded code)
     using Buf = typename std::conditional<S == 0, C *, PadBuf<C, S>>::type;
@@ -881,9 +882,12 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     // L2); gridDim.x is a multiple of 16
     const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
     auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
-    int it = 0, b = batch_of(0);
-    if (b < nbatch) wave_prefetch<MODE, T>(a, w, comp, b, tline, tpt0, m, v, vok, fx);
-    for (; b < nbatch; b = batch_of(++it)) {
+    const int nbt = nbatch * a.ncomp;
+    int it = 0, gb = batch_of(0);
+    int comp = gb / nbatch, b = gb - comp * nbatch;
+    if (gb < nbt) wave_prefetch<MODE, T>(a, (T *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
+    for (; gb < nbt; ++it) {
+        T *w = (T *)a.w + comp * a.wcs;
 #pragma unroll
         for (int u = 0; u < DST_PF; u++)
             if (tpt0 + u * WV_PS < m) {
@@ -926,8 +930,9 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
             lamxy[tid] = lv;
         }
         __syncthreads();
-        const int bn = batch_of(it + 1);
-        if (bn < nbatch) wave_prefetch<MODE, T>(a, w, comp, bn, tline, tpt0, m, v, vok, fx);
+        const int gbn = batch_of(it + 1);
+        const int compn = gbn / nbatch, bn = gbn - compn * nbatch;
+        if (gbn < nbt) wave_prefetch<MODE, T>(a, (T *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if constexpr (ODD) {
             // the fill wrote the odd extension; with Dirichlet values the folds change
@@ -987,6 +992,9 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
             else w[ad.dst + u * ad.dst_step] = val;
         }
         __syncthreads();
+        gb = gbn;
+        comp = compn;
+        b = bn;
     }
 }
 
@@ -1272,12 +1280,14 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
         per_cu = 1;
     if (per_cu < 1) per_cu = 1;
-    int64_t resident = (int64_t)c->num_cu * per_cu / (ncomp > 0 ? ncomp : 1);
+    // every component's batches share one grid, so it fills the device whatever ncomp is
+    int64_t resident = (int64_t)c->num_cu * per_cu;
     // multiple of 16 work-groups (batch pairs on one XCD), no more than the batches need
     resident = resident / 16 * 16;
     if (resident < 16) resident = 16;
-    const int64_t need = (nbatch + 15) / 16 * 16;
-    dim3 grid((unsigned)(need < resident ? need : resident), ncomp);
+    const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
+    a.ncomp = ncomp;
+    dim3 grid((unsigned)(need < resident ? need : resident));
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
     constexpr int kc = MODE == DST_Y ? K_DST : K_DSTZ;
