@@ -10,15 +10,19 @@
 // node and the source cell:  |x_t - x_s|^2 = K(p, q) + ((e - 1/2) h_C)^2, with p the
 // target's other coordinate and q the source's.  So each (B, A) pair is a 1-D convolution
 // along C -- real FFTs of length L >= 2 n_C - 2, which leaves the needed outputs free of
-// wrap-around -- and a dense sum over q, which for every frequency w is a complex matrix
-// product
+// wrap-around -- and a dense sum over q, which for every frequency w is a matrix product
 //   O^(p, col; w) = sum_q G^(p, q; w) A^(q, col; w),
-// batched over the L/2+1 frequencies.  Pairs with the same geometry up to a reflection of
-// p or q share one G^: on a cubic block three matrices (perpendicular, same face, opposite
-// face) serve all 36 pairs, whose pairs x components are the columns.  G^ depends only on
-// the block geometry and is kept across calls.
+// batched over the L/2+1 frequencies.  The kernel is even about e = 1/2 (it depends on
+// (e - 1/2)^2), so with the transform X(w) = sum_r x_r exp(-2 pi i w r / L)
+//   G^(p, q; w) = exp(-pi i w / L) R(p, q; w),   R = sum_e g(e) cos(2 pi w (e - 1/2) / L)
+// real: the table holds R, the product is a real one with the real and imaginary parts of
+// A^ as separate columns (half the bytes and flops of a complex product), and the phase
+// is applied when the inverse transform reads O^.  Pairs with the same geometry up to a
+// reflection of p or q share one table: on a cubic block three (perpendicular, same face,
+// opposite face) serve all 36 pairs, whose pairs x components are the columns.  The tables
+// depend only on the block geometry and are kept across calls.
 //
-// The products run in rocBLAS (complex FP64, atomics off, so the results do not depend on
+// The products run in rocBLAS (FP64, atomics off, so the results do not depend on
 // scheduling); the transforms run in LDS with the Stockham FFT of the Poisson passes.  A
 // final pass adds, for every boundary node, the six pair contributions of its owner face
 // in a fixed order (X faces own their edges, Y faces their z-edges).
@@ -77,8 +81,15 @@ __device__ __forceinline__ double2 unpack_half(const double2 *A, int L, int pr, 
     return make_double2(0.5 * (z.y + zc.y), -0.5 * (z.x - zc.x));
 }
 
-// G^[w][q][p] for p in the block's lines, one q per blockIdx.y.
-__global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t, const double2 *tw, double2 *G) {
+// exp(-pi i k / L): the phase of the half-sample shift of the kernel's centre.
+__device__ __forceinline__ double2 half_shift(int k, int L) {
+    double s, c;
+    sincospi((double)k / (double)L, &s, &c);
+    return make_double2(c, -s);
+}
+
+// R[w][q][p] (real, G^ = exp(-pi i w / L) R) for p in the block's lines, one q per blockIdx.y.
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t, const double2 *tw, double *G) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
     const int L = f.L, np = f.npairs, nl = 2 * np;
@@ -109,14 +120,18 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t,
     fft_reg<SPEC_MAXR>(A, twl, f);
     for (int i = tid; i < t.nw * nl; i += DST_THREADS) {
         const int k = i / nl, l = i - k * nl, p = p0 + l;
-        if (p < t.M) G[((int64_t)k * t.K + q) * t.M + p] = unpack_half(A, L, l >> 1, k, l & 1);
+        if (p < t.M) {
+            const double2 z = unpack_half(A, L, l >> 1, k, l & 1), ph = half_shift(k, L);
+            G[((int64_t)k * t.K + q) * t.M + p] = z.x * ph.x + z.y * ph.y;  // Re(z / ph)
+        }
     }
 }
 
-// A^ of every column: Bm[col][w][q], lines q of the block, one column per blockIdx.y.
+// A^ of every column as two real columns: Bm[2 col][w][q] = Re, Bm[2 col + 1][w][q] = Im,
+// lines q of the block, one column per blockIdx.y.
 __global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, const double2 *tw,
                                                          const SpecCol *cols, const double *a, int64_t ns,
-                                                         double2 *Bm) {
+                                                         double *Bm) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
     const int L = f.L, np = f.npairs, nl = 2 * np;
@@ -151,28 +166,39 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, c
     }
     __syncthreads();
     fft_reg<SPEC_MAXR>(A, twl, f);
-    double2 *out = Bm + (int64_t)col * t.nw * t.K;
+    const int64_t cst = (int64_t)t.nw * t.K;
+    double *out = Bm + 2 * col * cst;
     for (int i = tid; i < t.nw * nl; i += DST_THREADS) {
         const int k = i / nl, l = i - k * nl, q = q0 + l;
-        if (q < t.K) out[(int64_t)k * t.K + q] = unpack_half(A, L, l >> 1, k, l & 1);
+        if (q < t.K) {
+            const double2 z = unpack_half(A, L, l >> 1, k, l & 1);
+            out[(int64_t)k * t.K + q] = z.x;
+            out[cst + (int64_t)k * t.K + q] = z.y;
+        }
     }
 }
 
-// Inverse real FFT of O^[col][w][p] over w, first nC outputs -> Ores[col][p][kt].
-__global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, const double2 *tw, const double2 *Cm,
+// Inverse real FFT over w of O^ = exp(-pi i w / L) (Cm[2 col][w][p] + i Cm[2 col + 1][w][p]),
+// first nC outputs -> Ores[col][p][kt].
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, const double2 *tw, const double *Cm,
                                                          double *Ores, int64_t ores_off) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
     const int L = f.L, np = f.npairs, nl = 2 * np;
     double2 *A = lds, *twl = lds + np * L;
     const int tid = threadIdx.x, p0 = blockIdx.x * nl, col = blockIdx.y;
-    const double2 *in = Cm + (int64_t)col * t.nw * t.M;
+    const int64_t cst = (int64_t)t.nw * t.M;
+    const double *in = Cm + 2 * col * cst;
     for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
     // Z_k = X_k + i Y_k with the Hermitian halves of lines p (X) and p+1 (Y)
     for (int i = tid; i < t.nw * np; i += DST_THREADS) {
         const int k = i / np, pr = i - k * np, p = p0 + 2 * pr;
-        const double2 X = p < t.M ? in[(int64_t)k * t.M + p] : make_double2(0.0, 0.0);
-        const double2 Y = p + 1 < t.M ? in[(int64_t)k * t.M + p + 1] : make_double2(0.0, 0.0);
+        const double2 ph = half_shift(k, L);
+        const int64_t o = (int64_t)k * t.M + p;
+        const double xr = p < t.M ? in[o] : 0.0, xi = p < t.M ? in[cst + o] : 0.0;
+        const double yr = p + 1 < t.M ? in[o + 1] : 0.0, yi = p + 1 < t.M ? in[cst + o + 1] : 0.0;
+        const double2 X = make_double2(xr * ph.x - xi * ph.y, xr * ph.y + xi * ph.x);
+        const double2 Y = make_double2(yr * ph.x - yi * ph.y, yr * ph.y + yi * ph.x);
         if (k == 0 || 2 * k == L) {
             A[pr * L + k] = make_double2(X.x, Y.x);
         } else {
@@ -224,10 +250,10 @@ struct SpecTable {
     SpecTab t{};
     SpecFft f{};
     const double2 *tw = nullptr;
-    double2 *G = nullptr;
+    double *G = nullptr;  // R[w][q][p]
     std::vector<SpecCol> cols;
     SpecCol *cols_d = nullptr;
-    int64_t b_off = 0, c_off = 0, o_off = 0;  // column 0 in Bm, Cm (complex) and Ores (real)
+    int64_t b_off = 0, c_off = 0, o_off = 0;  // column 0 in Bm, Cm (real and imaginary columns) and Ores
 };
 
 struct SpecPlan {
@@ -235,7 +261,7 @@ struct SpecPlan {
     double h[3] = {0, 0, 0};
     std::vector<SpecTable> tabs;
     SpecPair *pairs_d = nullptr;
-    double2 *Bm = nullptr, *Cm = nullptr;
+    double *Bm = nullptr, *Cm = nullptr;
     double *Ores = nullptr;
     double flops = 0.0;
 };
@@ -371,8 +397,8 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
         T.b_off = nb;
         T.c_off = ncm;
         T.o_off = no;
-        nb += N * T.t.nw * T.t.K;
-        ncm += N * T.t.nw * T.t.M;
+        nb += 2 * N * T.t.nw * T.t.K;
+        ncm += 2 * N * T.t.nw * T.t.M;
         no += N * T.t.M * T.t.nC;
     }
     int st = VPMX_OK;
@@ -381,12 +407,12 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
         hipError_t e = hipMalloc(p, bytes);
         if (e != hipSuccess) st = fail(c, VPMX_ERR_HIP, std::string("spectral boundary sums: hipMalloc: ") + hipGetErrorString(e));
     };
-    alloc((void **)&P->Bm, sizeof(double2) * nb);
-    alloc((void **)&P->Cm, sizeof(double2) * ncm);
+    alloc((void **)&P->Bm, sizeof(double) * nb);
+    alloc((void **)&P->Cm, sizeof(double) * ncm);
     alloc((void **)&P->Ores, sizeof(double) * no);
     alloc((void **)&P->pairs_d, sizeof(SpecPair) * 36);
     for (auto &T : P->tabs) {
-        alloc((void **)&T.G, sizeof(double2) * (size_t)T.t.nw * T.t.K * T.t.M);
+        alloc((void **)&T.G, sizeof(double) * (size_t)T.t.nw * T.t.K * T.t.M);
         alloc((void **)&T.cols_d, sizeof(SpecCol) * T.cols.size());
     }
     if (st) {
@@ -410,7 +436,7 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
         dim3 grid((unsigned)((T.t.M + 2 * T.f.npairs - 1) / (2 * T.f.npairs)), (unsigned)T.t.K);
         hipLaunchKernelGGL(k_bsx_table, grid, dim3(DST_THREADS), lds, c->stream, T.f, T.t, T.tw, T.G);
         if (e == hipSuccess) e = hipGetLastError();
-        P->flops += 8.0 * T.t.M * T.t.K * (double)T.cols.size() * T.t.nw;
+        P->flops += 4.0 * T.t.M * T.t.K * (double)T.cols.size() * T.t.nw;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host arrays hp / cols go out of scope
     if (!st && e != hipSuccess) st = fail(c, VPMX_ERR_HIP, std::string("spectral boundary sums: ") + hipGetErrorString(e));
@@ -455,15 +481,15 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         hipLaunchKernelGGL(k_bsx_fwd, dim3((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
                            c->stream, T.f, T.t, T.tw, T.cols_d, a, ns, P->Bm + T.b_off);
         VPMX_HIP(c, hipGetLastError());
-        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
-        VPMX_TRY(blas_check(
-            c,
-            rocblas_zgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M, ncol, T.t.K, &one,
-                                          (const rocblas_double_complex *)T.G, T.t.M, (rocblas_stride)T.t.M * T.t.K,
-                                          (const rocblas_double_complex *)(P->Bm + T.b_off), T.t.nw * T.t.K,
-                                          (rocblas_stride)T.t.K, &zero, (rocblas_double_complex *)(P->Cm + T.c_off),
-                                          T.t.nw * T.t.M, (rocblas_stride)T.t.M, T.t.nw),
-            "zgemm_strided_batched"));
+        const double one = 1.0, zero = 0.0;
+        VPMX_TRY(blas_check(c,
+                            rocblas_dgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M,
+                                                          2 * ncol, T.t.K, &one, T.G, T.t.M,
+                                                          (rocblas_stride)T.t.M * T.t.K, P->Bm + T.b_off,
+                                                          T.t.nw * T.t.K, (rocblas_stride)T.t.K, &zero,
+                                                          P->Cm + T.c_off, T.t.nw * T.t.M, (rocblas_stride)T.t.M,
+                                                          T.t.nw),
+                            "dgemm_strided_batched"));
         hipLaunchKernelGGL(k_bsx_inv, dim3((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
                            c->stream, T.f, T.t, T.tw, P->Cm + T.c_off, P->Ores, T.o_off);
         VPMX_HIP(c, hipGetLastError());
@@ -473,7 +499,7 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
     hipLaunchKernelGGL(k_bsx_gather, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, g, nt, P->pairs_d,
                        P->Ores, sol);
     VPMX_HIP(c, hipGetLastError());
-    // FP64 work of the call: the complex products, plus ~5 L log2 L flops per complex FFT
+    // FP64 work of the call: the real products, plus ~5 L log2 L flops per complex FFT
     double fft = 0.0;
     for (auto &T : P->tabs) {
         const double per = 5.0 * T.t.L * std::log2((double)T.t.L);
