@@ -22,16 +22,21 @@
 // opposite face) serve all 36 pairs, whose pairs x components are the columns.  The tables
 // depend only on the block geometry and are kept across calls.
 //
+// For parallel faces p and q run along the same axis and the kernel depends on p - q only,
+// so R(p, q; w) = r_w(p - q) and the sum over q is a convolution too: it runs as FFTs of
+// length Lq >= M + K - 1 along q (the table holds FFT(r_w) / Lq, nw x Lq values, instead of
+// the nw x M x K matrices).  Only the perpendicular family needs a matrix product.
+//
 // The products run in rocBLAS (FP64, atomics off, so the results do not depend on
 // scheduling); the transforms run in LDS with the Stockham FFT of the Poisson passes.  A
 // final pass adds, for every boundary node, the six pair contributions of its owner face
 // in a fixed order (X faces own their edges, Y faces their z-edges).
 #include <rocblas/rocblas.h>
 
+#include <climits>
 #include <cmath>
 #include <vector>
 
-#include "bnodes.h"
 #include "fft_lds.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
@@ -48,6 +53,8 @@ struct SpecTab {  // one Green's-function matrix family
     int type;     // 0: perpendicular faces, 1: parallel faces
     int M, K;     // target coordinates p < M, source coordinates q < K
     int nC, L, nw;
+    int poff;     // parallel faces: the table's line p stands for the offset p - poff (0 for matrices)
+    int Lq;       // parallel faces: convolution length along q (0: matrix product)
     double hT, hS, hC, d;
 };
 
@@ -109,7 +116,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t,
                 const double x = (double)p * t.hT, y = ((double)q + 0.5) * t.hS;
                 r2 = (x * x + y * y) + dc * dc;
             } else {
-                const double x = ((double)(p - q) - 0.5) * t.hT;
+                const double x = ((double)(p - t.poff - q) - 0.5) * t.hT;
                 r2 = (x * x + t.d * t.d) + dc * dc;
             }
             v = 1.0 / sqrt(r2);
@@ -219,14 +226,142 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, c
     }
 }
 
-// SOL(t) += sum over the six source faces (fixed order) of the owner face's pair outputs.
-__global__ void k_bsx_gather(Grid g, int64_t nt, const SpecPair *pairs, const double *Ores, GView sol) {
+// Parallel-face table: FFT over the offset d = p - q of r_w(d), stored at the circular
+// position d mod Lq (d in [-(K-1), M-1], M + K - 1 <= Lq), divided by Lq for the inverse
+// transform of the convolution.  Rl[w][d + K - 1] holds r_w(d); one w per sequence.
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_toep_fft(SpecFft fq, SpecTab t, const double2 *twq,
+                                                              const double *Rl, double2 *T2) {
 #pragma clang fp contract(off)
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    int i, j, k, pl;
-    target_node(g, t, &i, &j, &k, &pl);
-    const int B = pl == 2 ? (i == g.n[0] ? 1 : 0) : (pl == 1 ? 2 + (j == g.n[1] ? 1 : 0) : 4 + (k == g.n[2] ? 1 : 0));
+    extern __shared__ double2 lds[];
+    const int Lq = fq.L, np = fq.npairs, nd = t.M + t.K - 1;
+    double2 *A = lds, *twl = lds + np * Lq;
+    const int tid = threadIdx.x, w0 = blockIdx.x * np;
+    for (int i = tid; i < Lq; i += DST_THREADS) twl[i] = twq[i];
+    for (int i = tid; i < np * Lq; i += DST_THREADS) {
+        const int s = i / Lq, pos = i - s * Lq, w = w0 + s;
+        int d = pos <= t.M - 1 ? pos : (pos >= Lq - (t.K - 1) ? pos - Lq : INT_MIN);
+        double v = 0.0;
+        if (w < t.nw && d != INT_MIN) v = Rl[(int64_t)w * nd + d + t.K - 1];
+        A[i] = make_double2(v, 0.0);
+    }
+    __syncthreads();
+    fft_reg<SPEC_MAXR>(A, twl, fq);
+    const double inv = 1.0 / (double)Lq;
+    for (int i = tid; i < np * Lq; i += DST_THREADS) {
+        const int s = i / Lq, v = i - s * Lq, w = w0 + s;
+        if (w < t.nw) T2[(int64_t)w * Lq + v] = make_double2(A[i].x * inv, A[i].y * inv);
+    }
+}
+
+// Parallel faces: O'(p; w) = sum_q r_w(p - q) A^(q; w) for p < M as a circular convolution
+// of length Lq, with the real and imaginary parts of A^ (Bm[2 col], Bm[2 col + 1]) as one
+// complex sequence (r_w is real); results to Cm[2 col], Cm[2 col + 1] like the matrix
+// product's.  One w per sequence, one column per blockIdx.y.
+__global__ void __launch_bounds__(DST_THREADS) k_bsx_qconv(SpecFft fq, SpecTab t, const double2 *twq,
+                                                           const double2 *T2, const double *Bm, double *Cm) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int Lq = fq.L, np = fq.npairs;
+    double2 *A = lds, *twl = lds + np * Lq;
+    const int tid = threadIdx.x, w0 = blockIdx.x * np, col = blockIdx.y;
+    const int64_t cb = (int64_t)t.nw * t.K, cc = (int64_t)t.nw * t.M;
+    const double *bre = Bm + 2 * col * cb, *bim = bre + cb;
+    for (int i = tid; i < Lq; i += DST_THREADS) twl[i] = twq[i];
+    for (int i = tid; i < np * Lq; i += DST_THREADS) {
+        const int s = i / Lq, q = i - s * Lq, w = w0 + s;
+        double2 z = make_double2(0.0, 0.0);
+        if (w < t.nw && q < t.K) {
+            const int64_t o = (int64_t)w * t.K + q;
+            z = make_double2(bre[o], bim[o]);
+        }
+        A[i] = z;
+    }
+    __syncthreads();
+    fft_reg<SPEC_MAXR>(A, twl, fq);
+    for (int i = tid; i < np * Lq; i += DST_THREADS) {
+        const int s = i / Lq, v = i - s * Lq, w = w0 + s;
+        if (w < t.nw) A[i] = cmul(A[i], T2[(int64_t)w * Lq + v]);
+    }
+    __syncthreads();
+    // inverse transform: FFT again, output n at index (Lq - n) mod Lq (1/Lq is in the table)
+    fft_reg<SPEC_MAXR>(A, twl, fq);
+    double *ore = Cm + 2 * col * cc, *oim = ore + cc;
+    for (int i = tid; i < np * t.M; i += DST_THREADS) {
+        const int s = i / t.M, p = i - s * t.M, w = w0 + s;
+        if (w >= t.nw) continue;
+        const double2 z = A[s * Lq + (p == 0 ? 0 : Lq - p)];
+        ore[(int64_t)w * t.M + p] = z.x;
+        oim[(int64_t)w * t.M + p] = z.y;
+    }
+}
+
+// The same convolution with one wave per (column, w) sequence (Lq <= QC_LMAX): no work-group
+// barriers after the twiddles are staged, QC_WAVES sequences per work-group.
+constexpr int QC_WAVES = 4, QC_LMAX = 576;
+__global__ void __launch_bounds__(64 * QC_WAVES) k_bsx_qconv_w(WavePlan P, SpecTab t, const double2 *twq,
+                                                               const double2 *T2, const double *Bm, double *Cm,
+                                                               int ncol) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    const int Lq = P.L, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double2 *twl = lds + QC_WAVES * Lq;
+    for (int i = threadIdx.x; i < Lq; i += 64 * QC_WAVES) twl[i] = twq[i];
+    __syncthreads();
+    const int seq = blockIdx.x * QC_WAVES + wv;  // col * nw + w
+    if (seq >= ncol * t.nw) return;               // no work-group barrier below
+    const int col = seq / t.nw, w = seq - col * t.nw;
+    const int64_t cb = (int64_t)t.nw * t.K, cc = (int64_t)t.nw * t.M;
+    const double *bre = Bm + 2 * col * cb + (int64_t)w * t.K, *bim = bre + cb;
+    double2 *A = lds + wv * Lq;
+    for (int q = lane; q < Lq; q += 64) A[q] = q < t.K ? make_double2(bre[q], bim[q]) : make_double2(0.0, 0.0);
+    __builtin_amdgcn_wave_barrier();
+    wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
+    const double2 *tr = T2 + (int64_t)w * Lq;
+    for (int v = lane; v < Lq; v += 64) A[v] = cmul(A[v], tr[v]);
+    __builtin_amdgcn_wave_barrier();
+    wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
+    double *ore = Cm + 2 * col * cc + (int64_t)w * t.M, *oim = ore + cc;
+    for (int p = lane; p < t.M; p += 64) {
+        const double2 z = A[p == 0 ? 0 : Lq - p];
+        ore[p] = z.x;
+        oim[p] = z.y;
+    }
+}
+
+// SOL(t) += sum over the six source faces (fixed order) of the owner face's pair outputs.
+// blockIdx.y is the owner face B (X faces whole, Y faces without the x-edges, Z faces without
+// the x- and y-edges); each wave takes an 8 x 8 patch of it, so the pair outputs, contiguous
+// along one face axis or the other, are read in 64-byte runs either way.
+constexpr int GATHER_WAVES = 4;
+__host__ __device__ inline void gather_face(const int n[3], int B, int *na, int *nb) {
+    const int ax = B >> 1;
+    *na = ax == 0 ? n[1] : n[0] - 2;
+    *nb = ax == 2 ? n[1] - 2 : n[2];
+}
+__global__ void __launch_bounds__(64 * GATHER_WAVES) k_bsx_gather(Grid g, const SpecPair *pairs, const double *Ores,
+                                                                  GView sol) {
+#pragma clang fp contract(off)
+    const int B = blockIdx.y, ax = B >> 1, hi = B & 1;
+    int na, nb;
+    gather_face(g.n, B, &na, &nb);
+    const int npa = (na + 7) >> 3, patch = blockIdx.x * GATHER_WAVES + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    if (patch >= npa * ((nb + 7) >> 3)) return;
+    const int a = (patch % npa) * 8 + (l & 7), b = (patch / npa) * 8 + (l >> 3);
+    if (a >= na || b >= nb) return;
+    int i, j, k;  // 1-based
+    if (ax == 0) {
+        i = hi ? g.n[0] : 1;
+        j = a + 1;
+        k = b + 1;
+    } else if (ax == 1) {
+        j = hi ? g.n[1] : 1;
+        i = a + 2;
+        k = b + 1;
+    } else {
+        k = hi ? g.n[2] : 1;
+        i = a + 2;
+        j = b + 2;
+    }
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     for (int A = 0; A < 6; A++) {
         const SpecPair P = pairs[B * 6 + A];
@@ -250,7 +385,11 @@ struct SpecTable {
     SpecTab t{};
     SpecFft f{};
     const double2 *tw = nullptr;
-    double *G = nullptr;  // R[w][q][p]
+    double *G = nullptr;  // perpendicular faces: R[w][q][p]
+    SpecFft fq{};         // parallel faces: the convolution along q
+    WavePlan wq{};        // the same plan for the one-wave-per-sequence kernel (wq.L = 0: not used)
+    const double2 *twq = nullptr;
+    double2 *T2 = nullptr;  // parallel faces: FFT(r_w) / Lq, [w][v]
     std::vector<SpecCol> cols;
     SpecCol *cols_d = nullptr;
     int64_t b_off = 0, c_off = 0, o_off = 0;  // column 0 in Bm, Cm (real and imaginary columns) and Ores
@@ -270,6 +409,7 @@ static void free_plan(SpecPlan *P) {
     if (!P) return;
     for (auto &T : P->tabs) {
         if (T.G) (void)hipFree(T.G);
+        if (T.T2) (void)hipFree(T.T2);
         if (T.cols_d) (void)hipFree(T.cols_d);
     }
     if (P->pairs_d) (void)hipFree(P->pairs_d);
@@ -292,12 +432,14 @@ static bool smooth7(int n) {
     return n == 1;
 }
 
-static int spec_length(int nC) {
-    int L = 2 * nC - 2;
-    if (L < 4) L = 4;
+// Smallest even 7-smooth length >= n (at least 4).
+static int smooth_len(int n) {
+    int L = n < 4 ? 4 : n;
     while (!(L % 2 == 0 && smooth7(L))) L++;
     return L;
 }
+
+static int spec_length(int nC) { return smooth_len(2 * nC - 2); }
 
 static bool same_tab(const SpecTab &a, const SpecTab &b) {
     return a.type == b.type && a.M == b.M && a.K == b.K && a.nC == b.nC && a.L == b.L && a.hT == b.hT &&
@@ -365,13 +507,14 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
                 t.K = n[u] - 1;
                 t.hT = t.hS = h[u];
                 t.d = sA == sB ? 0.0 : (double)(n[At] - 1) * h[At];
+                t.Lq = smooth_len(t.M + t.K - 1);
                 flipp = flipq = 0;
             }
             t.nC = n[cax];
             t.hC = h[cax];
             t.L = spec_length(t.nC);
             t.nw = t.L / 2 + 1;
-            if (t.L > DST_EMAX || 2 * t.L > DST_LDS_MAX) {
+            if (t.L > DST_EMAX || 2 * t.L > DST_LDS_MAX || t.Lq > DST_EMAX || 2 * t.Lq > DST_LDS_MAX) {
                 free_plan(P);
                 return VPMX_ERR_UNSUP;
             }
@@ -411,11 +554,25 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
     alloc((void **)&P->Cm, sizeof(double) * ncm);
     alloc((void **)&P->Ores, sizeof(double) * no);
     alloc((void **)&P->pairs_d, sizeof(SpecPair) * 36);
+    std::vector<double *> lines;  // parallel faces: r_w(d) while the tables are built
     for (auto &T : P->tabs) {
-        alloc((void **)&T.G, sizeof(double) * (size_t)T.t.nw * T.t.K * T.t.M);
+        if (T.t.type == 1) {
+            double *l = nullptr;
+            alloc((void **)&l, sizeof(double) * (size_t)T.t.nw * (T.t.M + T.t.K - 1));
+            lines.push_back(l);
+            alloc((void **)&T.T2, sizeof(double2) * (size_t)T.t.nw * T.t.Lq);
+        } else {
+            alloc((void **)&T.G, sizeof(double) * (size_t)T.t.nw * T.t.K * T.t.M);
+        }
         alloc((void **)&T.cols_d, sizeof(SpecCol) * T.cols.size());
     }
+    auto free_lines = [&]() {
+        for (double *l : lines)
+            if (l) (void)hipFree(l);
+        lines.clear();
+    };
     if (st) {
+        free_lines();
         free_plan(P);
         return st;
     }
@@ -426,20 +583,53 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
         hp[q] = {T.o_off + (int64_t)R.col0 * T.t.M * T.t.nC, T.t.M, T.t.nC, R.tax, R.cax, R.flipp, 0};
     }
     hipError_t e = hipMemcpyAsync(P->pairs_d, hp, sizeof(hp), hipMemcpyHostToDevice, c->stream);
+    size_t li = 0;
     for (auto &T : P->tabs) {
         if (e == hipSuccess)
             e = hipMemcpyAsync(T.cols_d, T.cols.data(), sizeof(SpecCol) * T.cols.size(), hipMemcpyHostToDevice,
                                c->stream);
         if (e == hipSuccess) st = fft_setup(c, T.t.L, &T.f, &T.tw);
+        if (!st && T.t.type == 1) st = fft_setup(c, T.t.Lq, &T.fq, &T.twq);
         if (st) break;
+        if (T.t.type == 1 && T.t.Lq <= QC_LMAX) {
+            int ns = 1;
+            T.wq.L = T.t.Lq;
+            T.wq.nstages = T.fq.nstages;
+            for (int q = 0; q < T.fq.nstages; q++) {
+                T.wq.R[q] = T.fq.radix[q];
+                T.wq.Ns[q] = ns;
+                T.wq.mg[q] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns + 1ull) : 0u;
+                ns *= T.fq.radix[q];
+            }
+        }
         const size_t lds = sizeof(double2) * (size_t)(T.f.npairs + 1) * T.t.L;
-        dim3 grid((unsigned)((T.t.M + 2 * T.f.npairs - 1) / (2 * T.f.npairs)), (unsigned)T.t.K);
-        hipLaunchKernelGGL(k_bsx_table, grid, dim3(DST_THREADS), lds, c->stream, T.f, T.t, T.tw, T.G);
-        if (e == hipSuccess) e = hipGetLastError();
-        P->flops += 4.0 * T.t.M * T.t.K * (double)T.cols.size() * T.t.nw;
+        if (T.t.type == 1) {
+            // r_w(d) for d = -(K-1)..M-1: the table kernel with M + K - 1 lines, one q
+            SpecTab tl = T.t;
+            tl.M = T.t.M + T.t.K - 1;
+            tl.K = 1;
+            tl.poff = T.t.K - 1;
+            double *l = lines[li++];
+            dim3 grid((unsigned)((tl.M + 2 * T.f.npairs - 1) / (2 * T.f.npairs)), 1u);
+            hipLaunchKernelGGL(k_bsx_table, grid, dim3(DST_THREADS), lds, c->stream, T.f, tl, T.tw, l);
+            if (e == hipSuccess) e = hipGetLastError();
+            const size_t ldsq = sizeof(double2) * (size_t)(T.fq.npairs + 1) * T.t.Lq;
+            hipLaunchKernelGGL(k_bsx_toep_fft, dim3((unsigned)((T.t.nw + T.fq.npairs - 1) / T.fq.npairs)),
+                               dim3(DST_THREADS), ldsq, c->stream, T.fq, T.t, T.twq, l, T.T2);
+            if (e == hipSuccess) e = hipGetLastError();
+            // two FFTs of Lq and the spectral product per column and frequency
+            P->flops += (10.0 * T.t.Lq * std::log2((double)T.t.Lq) + 6.0 * T.t.Lq) * (double)T.cols.size() * T.t.nw;
+        } else {
+            dim3 grid((unsigned)((T.t.M + 2 * T.f.npairs - 1) / (2 * T.f.npairs)), (unsigned)T.t.K);
+            hipLaunchKernelGGL(k_bsx_table, grid, dim3(DST_THREADS), lds, c->stream, T.f, T.t, T.tw, T.G);
+            if (e == hipSuccess) e = hipGetLastError();
+            P->flops += 4.0 * T.t.M * T.t.K * (double)T.cols.size() * T.t.nw;
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // host arrays hp / cols go out of scope
     if (!st && e != hipSuccess) st = fail(c, VPMX_ERR_HIP, std::string("spectral boundary sums: ") + hipGetErrorString(e));
+    if (st && e == hipSuccess) (void)hipStreamSynchronize(c->stream);
+    free_lines();
     if (st) {
         free_plan(P);
         return st;
@@ -481,25 +671,45 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         hipLaunchKernelGGL(k_bsx_fwd, dim3((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
                            c->stream, T.f, T.t, T.tw, T.cols_d, a, ns, P->Bm + T.b_off);
         VPMX_HIP(c, hipGetLastError());
-        const double one = 1.0, zero = 0.0;
-        VPMX_TRY(blas_check(c,
-                            rocblas_dgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M,
-                                                          2 * ncol, T.t.K, &one, T.G, T.t.M,
-                                                          (rocblas_stride)T.t.M * T.t.K, P->Bm + T.b_off,
-                                                          T.t.nw * T.t.K, (rocblas_stride)T.t.K, &zero,
-                                                          P->Cm + T.c_off, T.t.nw * T.t.M, (rocblas_stride)T.t.M,
-                                                          T.t.nw),
-                            "dgemm_strided_batched"));
+        if (T.t.type == 1 && T.wq.L > 0) {
+            const size_t ldsq = sizeof(double2) * (size_t)(QC_WAVES + 1) * T.t.Lq;
+            const int nseq = ncol * T.t.nw;
+            hipLaunchKernelGGL(k_bsx_qconv_w, dim3((unsigned)((nseq + QC_WAVES - 1) / QC_WAVES)), dim3(64 * QC_WAVES),
+                               ldsq, c->stream, T.wq, T.t, T.twq, T.T2, P->Bm + T.b_off, P->Cm + T.c_off, ncol);
+            VPMX_HIP(c, hipGetLastError());
+        } else if (T.t.type == 1) {
+            const size_t ldsq = sizeof(double2) * (size_t)(T.fq.npairs + 1) * T.t.Lq;
+            hipLaunchKernelGGL(k_bsx_qconv, dim3((unsigned)((T.t.nw + T.fq.npairs - 1) / T.fq.npairs), (unsigned)ncol),
+                               dim3(DST_THREADS), ldsq, c->stream, T.fq, T.t, T.twq, T.T2, P->Bm + T.b_off,
+                               P->Cm + T.c_off);
+            VPMX_HIP(c, hipGetLastError());
+        } else {
+            const double one = 1.0, zero = 0.0;
+            VPMX_TRY(blas_check(c,
+                                rocblas_dgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M,
+                                                              2 * ncol, T.t.K, &one, T.G, T.t.M,
+                                                              (rocblas_stride)T.t.M * T.t.K, P->Bm + T.b_off,
+                                                              T.t.nw * T.t.K, (rocblas_stride)T.t.K, &zero,
+                                                              P->Cm + T.c_off, T.t.nw * T.t.M, (rocblas_stride)T.t.M,
+                                                              T.t.nw),
+                                "dgemm_strided_batched"));
+        }
         hipLaunchKernelGGL(k_bsx_inv, dim3((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
                            c->stream, T.f, T.t, T.tw, P->Cm + T.c_off, P->Ores, T.o_off);
         VPMX_HIP(c, hipGetLastError());
     }
-    const int64_t nt = 2 * ((int64_t)g.n[1] * g.n[2] + (int64_t)(g.n[0] - 2) * g.n[2] +
-                            (int64_t)(g.n[0] - 2) * (g.n[1] - 2));
-    hipLaunchKernelGGL(k_bsx_gather, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, g, nt, P->pairs_d,
-                       P->Ores, sol);
+    int maxp = 0;
+    for (int B = 0; B < 6; B++) {
+        int na, nb;
+        gather_face(g.n, B, &na, &nb);
+        const int np = ((na + 7) / 8) * ((nb + 7) / 8);
+        maxp = np > maxp ? np : maxp;
+    }
+    hipLaunchKernelGGL(k_bsx_gather, dim3((unsigned)((maxp + GATHER_WAVES - 1) / GATHER_WAVES), 6u),
+                       dim3(64 * GATHER_WAVES), 0, c->stream, g, P->pairs_d, P->Ores, sol);
     VPMX_HIP(c, hipGetLastError());
-    // FP64 work of the call: the real products, plus ~5 L log2 L flops per complex FFT
+    // FP64 work of the call: the real products and q convolutions, plus ~5 L log2 L flops per
+    // complex FFT along C
     double fft = 0.0;
     for (auto &T : P->tabs) {
         const double per = 5.0 * T.t.L * std::log2((double)T.t.L);

@@ -1392,12 +1392,16 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     // x inverse -> output interior
     a.nlines = (int64_t)m[1] * m[2];
     VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0], f32));
-    if (!bc) {
-        int64_t tot = 2 * ((int64_t)wn[0] * wn[1] + (int64_t)wn[0] * wn[2] + (int64_t)wn[1] * wn[2]);
-        hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, out, wn[0], wn[1],
-                           wn[2], ncomp);
-        VPMX_HIP(c, hipGetLastError());
-    }
+    if (!bc) VPMX_TRY(zero_faces(c, wn, out, ncomp));
+    return VPMX_OK;
+}
+
+int zero_faces(Ctx *c, const int n[3], GView v, int ncomp) {
+    const int64_t tot = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
+    if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
+    hipLaunchKernelGGL(k_zero_faces, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, v, n[0], n[1], n[2],
+                       ncomp);
+    VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
 

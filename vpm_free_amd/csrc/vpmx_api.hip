@@ -133,7 +133,8 @@ static int yaps_single(Ctx *c, bool *used) {
     Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
     // 1. fine infinite-domain solve over the whole block
     mark(c, ST_FINE, 0);
-    VPMX_TRY(fill(c, sol_bl, 3 * B, 0.0));
+    // the boundary sums accumulate on the faces; the second solve writes every interior node
+    VPMX_TRY(zero_faces(c, nb, sv));
     c->dst_timed_m = gb.n[1] - 2;  // the block's y lines are the timed sine-transform passes
     // tree levels: ilevmax on the block, ilevmax-1 on the coarse grid (yaps3d.f90:46-48, 286-292)
     const bool tree = P.iyntree == 1;

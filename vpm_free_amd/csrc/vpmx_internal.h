@@ -124,6 +124,7 @@ void free_tree(Ctx *c);
 
 // Grid kernels (k_grid.hip)
 int fill(Ctx *c, double *p, int64_t n, double v);
+int zero_faces(Ctx *c, const int n[3], GView v, int ncomp = 3);  // the six faces of a grid (k_poisson.hip)
 int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel);
 int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out);
 int diffuse_kernel(Ctx *c, const int n[3], const double h[3], double vis, GView rhs, GView out);
