@@ -193,7 +193,7 @@ def _lap7(u, h):
 @pytest.mark.parametrize("n", [289, 273])
 def test_solve_full_size_plans(ctx, n):
     """The block sizes of the 256-cubed workload, where the sine transforms run their
-    compile-time FFT plans (FP64: 288 = 4.4.2.3.3, 272 = 4.4.17; FP32: 576 = 4.4.4.3.3,
+    compile-time FFT plans (FP64: 288 = 4.4.2.9, 272 = 4.4.17; FP32: 576 = 4.4.4.9,
     544 = 17.4.4.2), padded LDS sequences and register-capped kernels — sizes the oracle is
     too slow for.  The FP64 solution satisfies the 7-point equations to rounding, and the
     mixed form agrees with it to FP32 accuracy."""

@@ -36,7 +36,7 @@ constexpr int DST_EMAX = 2304;  // complex line elements held in LDS per work-gr
 constexpr int DST_LDS_MAX = 4096;  // complex elements of LDS per work-group, lines + twiddles (64 KB)
 constexpr int DST_THREADS = 512;
 
-// In-register butterflies of the specialised radices (2, 3, 4).
+// In-register butterflies of the specialised radices (2, 3, 4, 9).
 template <int R, class C>
 __device__ __forceinline__ void dft_reg(C (&v)[R]) {
     using T = typename Cx<C>::R;
@@ -59,6 +59,28 @@ __device__ __forceinline__ void dft_reg(C (&v)[R]) {
         v[1] = cadd(d02, md);
         v[2] = csub(s02, s13);
         v[3] = csub(d02, md);
+    } else if constexpr (R == 9) {
+        // 9 = 3 x 3 with n = 3a + b, k = k1 + 3 k2: a 3-point DFT over a for each b, the
+        // twiddles exp(-2 pi i b k1 / 9), then a 3-point DFT over b for each k1
+        C y0[3] = {v[0], v[3], v[6]}, y1[3] = {v[1], v[4], v[7]}, y2[3] = {v[2], v[5], v[8]};
+        dft_reg<3, C>(y0);
+        dft_reg<3, C>(y1);
+        dft_reg<3, C>(y2);
+        const C w1 = Cx<C>::mk((T)0.7660444431189780352024, (T)-0.6427876096865393263226);
+        const C w2 = Cx<C>::mk((T)0.1736481776669303488517, (T)-0.9848077530122080593667);
+        const C w4 = Cx<C>::mk((T)-0.9396926207859083840541, (T)-0.3420201433256687330441);
+        y1[1] = cmul(y1[1], w1);
+        y1[2] = cmul(y1[2], w2);
+        y2[1] = cmul(y2[1], w2);
+        y2[2] = cmul(y2[2], w4);
+#pragma unroll
+        for (int k1 = 0; k1 < 3; k1++) {
+            C z[3] = {y0[k1], y1[k1], y2[k1]};
+            dft_reg<3, C>(z);
+            v[k1] = z[0];
+            v[k1 + 3] = z[1];
+            v[k1 + 6] = z[2];
+        }
     }
 }
 
@@ -272,8 +294,10 @@ __device__ void wave_fft(Buf buf, const C *tw, const WavePlan &P, int lane) {
 // ------------------------------------------------------------------ compile-time plans
 // The same Stockham stages with the length and radices known at compile time: spans,
 // strides, twiddle steps and divisions by Ns become constants, which removes most of the
-// integer work of the runtime-plan stages.  Same stage order and arithmetic as wave_fft,
-// so results are identical.
+// integer work of the runtime-plan stages.  A plan may also merge two radix-3 stages into
+// one radix-9 stage (half the LDS round trips of that pair; a 576-point sequence has
+// exactly 64 radix-9 butterflies, one per lane), so its rounding differs from the runtime
+// plan's.
 template <int R, int L, int Ns, class C, class Buf>
 __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, int lane) {
     constexpr int nbf = L / R, twstep = L / (Ns * R), BPL = (nbf + 63) / 64;
@@ -292,7 +316,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
 #pragma unroll
                 for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
             }
-            if constexpr (R <= 4) dft_reg<R>(v[i]);
+            if constexpr (R <= 4 || R == 9) dft_reg<R>(v[i]);
             dst[i] = q * Ns * R + k0;
         }
     }
@@ -300,7 +324,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
 #pragma unroll
     for (int i = 0; i < BPL; i++)
         if (dst[i] >= 0) {
-            if constexpr (R <= 4) {
+            if constexpr (R <= 4 || R == 9) {
 #pragma unroll
                 for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
             } else {
@@ -319,16 +343,16 @@ __device__ __forceinline__ void wave_fft_s(Buf buf, const C *tw, int lane) {
 // Lengths with a compile-time plan (radices in the planner's order: 4s, 2s, odd primes).
 // FIX = -1 selects the runtime plan.
 template <int FIX> struct FixedFft;
-template <> struct FixedFft<0> {  // 576 = 4.4.4.3.3 (FP32, 289-node block)
+template <> struct FixedFft<0> {  // 576 = 4.4.4.9 (FP32, 289-node block)
     static constexpr int L = 576;
     template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 576, 1, 4, 4, 4, 3, 3>(b, tw, lane);
+        wave_fft_s<C, Buf, 576, 1, 4, 4, 4, 9>(b, tw, lane);
     }
 };
-template <> struct FixedFft<2> {  // 288 = 4.4.2.3.3 (FP64, 289-node block)
+template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)
     static constexpr int L = 288;
     template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 288, 1, 4, 4, 2, 3, 3>(b, tw, lane);
+        wave_fft_s<C, Buf, 288, 1, 4, 4, 2, 9>(b, tw, lane);
     }
 };
 template <> struct FixedFft<5> {  // 544 = 17.4.4.2 (FP32, 273-node inner box; the prime stage first needs
