@@ -41,71 +41,113 @@ __host__ __device__ inline int stencil_ipf(int itype) { return itype == 2 ? 1 : 
 
 // ------------------------------------------------------------------ scans
 // Three-phase exclusive scan of int32 counts (n entries) into out (n+1 entries).
-__global__ void k_scan_blocks(const int *in, int *out, int *bsum, int64_t n) {
-    __shared__ int s[1024];
-    int64_t base = (int64_t)blockIdx.x * 1024;
-    int t = threadIdx.x;
-    int v[4];
-    int loc = 0;
-    for (int q = 0; q < 4; q++) {
-        int64_t idx = base + t * 4 + q;
-        v[q] = idx < n ? in[idx] : 0;
-        loc += v[q];
+// Exclusive prefix sum out[0..n) of in[0..n), out[n] = total, in three passes over tiles of
+// SCAN_TILE inputs: the tile sums, their scan (one work-group), then every tile's scan plus
+// its offset.  Reads the input twice and writes the output once; in == out is allowed.
+constexpr int SCAN_THREADS = 256, SCAN_PER = 16, SCAN_TILE = SCAN_THREADS * SCAN_PER;
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
     }
-    s[t] = loc;
+    return v;
+}
+// Exclusive scan of one value per thread over the work-group (at most 16 waves); *total gets
+// the sum.  Contains work-group barriers: every thread must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int *total) {
+    __shared__ int ws[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int inc = wave_incl_scan(v, lane);
+    if (lane == 63) ws[w] = inc;
     __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int add = t >= off ? s[t - off] : 0;
-        __syncthreads();
-        s[t] += add;
-        __syncthreads();
+    int off = 0, tot = 0;
+    for (int q = 0; q < nw; q++) {
+        const int x = ws[q];
+        off += q < w ? x : 0;
+        tot += x;
     }
-    int run = s[t] - loc;
-    for (int q = 0; q < 4; q++) {
-        int64_t idx = base + t * 4 + q;
-        if (idx < n) out[idx] = run;
-        run += v[q];
-    }
-    if (t == 255) bsum[blockIdx.x] = s[255];
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
 }
-__global__ void k_scan_sums(int *bsum, int64_t nb, int *total) {
-    __shared__ int s[1024];
-    int t = threadIdx.x;
-    int carry = 0;
-    for (int64_t base = 0; base < nb; base += 1024) {
-        int64_t idx = base + t;
-        int v = idx < nb ? bsum[idx] : 0;
-        s[t] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int add = t >= off ? s[t - off] : 0;
-            __syncthreads();
-            s[t] += add;
-            __syncthreads();
-        }
-        if (idx < nb) bsum[idx] = carry + s[t] - v;
-        int blk = s[1023];
-        __syncthreads();
-        carry += blk;
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int *in, int64_t n, int *bsum) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int64_t i = base + q * SCAN_THREADS + threadIdx.x;
+        s += i < n ? in[i] : 0;
     }
-    if (t == 0) *total = carry;
+    int tot;
+    (void)block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
-__global__ void k_scan_add(int *out, const int *bsum, int64_t n, const int *total) {
-    int64_t idx = (int64_t)blockIdx.x * 1024 + threadIdx.x * 4;
-    int add = bsum[blockIdx.x];
-    for (int q = 0; q < 4; q++)
-        if (idx + q < n) out[idx + q] += add;
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+// bsum[0..nb) -> exclusive scan in place, bsum[nb] = total; one work-group of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_tiles(int *bsum, int64_t nb) {
+    const int64_t per = (nb + 1023) / 1024, lo = threadIdx.x * per, hi = lo + per < nb ? lo + per : nb;
+    int s = 0;
+    for (int64_t i = lo; i < hi; i++) s += bsum[i];
+    int tot;
+    int run = block_excl_scan(s, &tot);
+    for (int64_t i = lo; i < hi; i++) {
+        const int v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0) bsum[nb] = tot;
+}
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_out(const int *in, int *out, int64_t n, const int *bsum,
+                                                           int64_t nb) {
+    __shared__ int tile[SCAN_TILE + SCAN_TILE / 16];  // one pad slot per 16: thread rows are conflict-free
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int i = q * SCAN_THREADS + t;
+        const int64_t g = base + i;
+        tile[i + (i >> 4)] = g < n ? in[g] : 0;
+    }
+    __syncthreads();
+    int v[SCAN_PER], s = 0;
+#pragma unroll
+    for (int r = 0; r < SCAN_PER; r++) {
+        v[r] = tile[t * 17 + r];
+        s += v[r];
+    }
+    int tot;
+    int run = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < SCAN_PER; r++) {
+        tile[t * 17 + r] = run;
+        run += v[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < SCAN_PER; q++) {
+        const int i = q * SCAN_THREADS + t;
+        const int64_t g = base + i;
+        if (g < n) out[g] = tile[i + (i >> 4)];
+    }
+    if (blockIdx.x == 0 && t == 0) out[n] = bsum[nb];
 }
 
+__global__ void k_zero_int(int *p, int64_t n);
+
 int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n) {
+    if (n <= 0) {
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, out, (int64_t)1);
+        VPMX_HIP(c, hipGetLastError());
+        return VPMX_OK;
+    }
     int st = VPMX_OK;
-    int64_t nb = (n + 1023) / 1024;
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     int *bsum = (int *)ensure(c, c->scan_tmp, sizeof(int) * (size_t)(nb + 2), &st);
     if (st) return st;
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(256), 0, c->stream, in, out, bsum, n);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, bsum, nb, bsum + nb);
-    hipLaunchKernelGGL(k_scan_add, dim3((unsigned)nb), dim3(256), 0, c->stream, out, bsum, n, bsum + nb);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, c->stream, bsum, nb);
+    hipLaunchKernelGGL(k_scan_out, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, c->stream, in, out, n, bsum, nb);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

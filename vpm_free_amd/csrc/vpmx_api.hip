@@ -120,6 +120,24 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     return VPMX_OK;
 }
 
+// Coarse-map table on the device: uploaded only when its contents or buffer change (the
+// geometry is usually the same from call to call), from a copy the context keeps, so no
+// call has to wait for the upload before its host vector goes away.
+static int upload_cmap(Ctx *c, const std::vector<int> &cmap, int **out) {
+    int st = VPMX_OK;
+    int *d = (int *)ensure(c, c->cmap, sizeof(int) * cmap.size(), &st);
+    if (st) return st;
+    if (d != c->cmap_dev || cmap != c->cmap_host) {
+        VPMX_HIP(c, hipStreamSynchronize(c->stream));  // an earlier upload may still read cmap_host
+        c->cmap_host = cmap;
+        c->cmap_dev = d;
+        VPMX_HIP(c, hipMemcpyAsync(d, c->cmap_host.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    *out = d;
+    return VPMX_OK;
+}
+
 // ---------------------------------------------------------------- yaps (1 block)
 static int yaps_single(Ctx *c, bool *used) {
     int st = VPMX_OK;
@@ -168,8 +186,8 @@ static int yaps_single(Ctx *c, bool *used) {
         // the reference records NN_coarse_map at i = 1 and i = NNbl; with (NNbl-1) a multiple
         // of ncoarse the last sample is i = NNbl
     }
-    int *cmap_d = (int *)ensure(c, c->cmap, sizeof(int) * cmap.size(), &st); if (st) return st;
-    VPMX_HIP(c, hipMemcpyAsync(cmap_d, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice, c->stream));
+    int *cmap_d = nullptr;
+    VPMX_TRY(upload_cmap(c, cmap, &cmap_d));
     GView smv = view(samp, cn), rcv = view(rhsc, cn), scv = view(solc, cn);
     VPMX_TRY(fill(c, samp, 3 * GC, 0.0));
     VPMX_TRY(coarse_sample(c, 3, ns, ncs, cmap_d, sv, smv, cn));
@@ -177,7 +195,6 @@ static int yaps_single(Ctx *c, bool *used) {
     VPMX_TRY(fill(c, solc, 3 * GC, 0.0));
     Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
     VPMX_TRY(pmesh_inf(c, gc, rcv, scv, c->sol0_c, false, tree ? G.ilevmax - 1 : -1, P.ibctyp));
-    VPMX_HIP(c, hipStreamSynchronize(c->stream));  // cmap host vector goes out of scope
     mark(c, ST_COARSE, 1);
     used[ST_COARSE] = true;
     // 3. TSC correction of the inner-box faces
@@ -279,9 +296,9 @@ static int yaps_multi(Ctx *c, bool *used) {
         M.bj = (b / G.nbi) % G.nbj + 1;
         M.bk = b / (G.nbi * G.nbj) + 1;
     }
-    int *cmap_d = (int *)ensure(c, c->cmap, sizeof(int) * cmap.size(), &st); if (st) return st;
+    int *cmap_d = nullptr;
+    VPMX_TRY(upload_cmap(c, cmap, &cmap_d));
     MbBlock *blk_d = (MbBlock *)ensure(c, c->mb_blocks, sizeof(MbBlock) * (size_t)NB, &st); if (st) return st;
-    VPMX_HIP(c, hipMemcpyAsync(cmap_d, cmap.data(), sizeof(int) * cmap.size(), hipMemcpyHostToDevice, c->stream));
     VPMX_HIP(c, hipMemcpyAsync(blk_d, blk.data(), sizeof(MbBlock) * (size_t)NB, hipMemcpyHostToDevice, c->stream));
     VPMX_TRY(fill(c, samp, 3 * GC * NB, 0.0));
     VPMX_TRY(fill(c, rhsc, 3 * GC, 0.0));

@@ -57,6 +57,8 @@ struct Ctx : vpmx_ctx {
     std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N
     std::map<std::pair<int, double>, void *> eig_cache;  // DST-I eigenvalues per (m, h)
     DBuf samp, rhs_c, sol_c, sol0_c, cmap;
+    std::vector<int> cmap_host;   // contents of cmap as last uploaded (kept alive for the async copy)
+    const void *cmap_dev = nullptr;  // device address they were uploaded to
     DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
