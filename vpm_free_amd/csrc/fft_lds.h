@@ -125,6 +125,67 @@ __device__ __forceinline__ void dft_odd_store(C (&v)[R], Buf buf, int d, int Ns,
     }
 }
 
+// cos and sin of 2 pi q / 17, q = 1..8: the roots of the radix-17 stage of the compile-time
+// plans as constants (the runtime stages read them from the LDS twiddle table).
+__host__ __device__ constexpr double cos17(int q) {
+    switch (q) {
+        case 1: return 0.932472229404355804573;
+        case 2: return 0.739008917220659115925;
+        case 3: return 0.445738355776538267396;
+        case 4: return 0.0922683594633019952397;
+        case 5: return -0.273662990072082863539;
+        case 6: return -0.602634636379256389179;
+        case 7: return -0.850217135729614152134;
+        default: return -0.982973099683901778282;
+    }
+}
+__host__ __device__ constexpr double sin17(int q) {
+    switch (q) {
+        case 1: return 0.361241666187152948745;
+        case 2: return 0.673695643646557211713;
+        case 3: return 0.895163291355062322067;
+        case 4: return 0.995734176295034521871;
+        case 5: return 0.961825643172819070409;
+        case 6: return 0.798017227280239503333;
+        case 7: return 0.526432162877355800245;
+        default: return 0.183749517816570331574;
+    }
+}
+
+// dft_odd_store for R = 17 with the roots as constants and both loops unrolled: the same sums
+// in the same order, without the H * H twiddle reads from LDS.
+template <int R, class C, class Buf>
+__device__ __forceinline__ void dft_odd_store_c(C (&v)[R], Buf buf, int d, int Ns) {
+    static_assert(R == 17, "constant roots are tabulated for radix 17");
+    using T = typename Cx<C>::R;
+    constexpr int H = (R - 1) / 2;
+    C p[H], mm[H];
+    C x0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        p[r - 1] = cadd(v[r], v[R - r]);
+        mm[r - 1] = csub(v[r], v[R - r]);
+        x0 = cadd(x0, p[r - 1]);
+    }
+    buf[d] = x0;
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        T X = v[0].x, Y = v[0].y, U = 0, V = 0;
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            const int q = (r * k) % R;  // exp(-2 pi i q / R) = c - i s
+            const T c = (T)cos17(q <= H ? q : R - q);
+            const T s = q <= H ? (T)sin17(q) : -(T)sin17(R - q);
+            X += c * p[r - 1].x;
+            Y += c * p[r - 1].y;
+            U += s * mm[r - 1].y;
+            V += s * mm[r - 1].x;
+        }
+        buf[d + k * Ns] = Cx<C>::mk(X + U, Y - V);
+        buf[d + (R - k) * Ns] = Cx<C>::mk(X - U, Y + V);
+    }
+}
+
 // One Stockham stage in place: inputs to registers, barrier, outputs to LDS, barrier.
 // Radices above 4 form their outputs after the barrier (v holds the twiddled inputs).
 template <int R, class Args>
@@ -298,7 +359,10 @@ __device__ void wave_fft(Buf buf, const C *tw, const WavePlan &P, int lane) {
 // one radix-9 stage (half the LDS round trips of that pair; a 576-point sequence has
 // exactly 64 radix-9 butterflies, one per lane), so its rounding differs from the runtime
 // plan's.
-template <int R, int L, int Ns, class C, class Buf>
+// CR: a radix-17 stage takes its roots as constants (dft_odd_store_c) instead of from the LDS
+// table; it saves the table reads but needs more registers, which costs some kernels their
+// occupancy, so each kernel chooses.
+template <int R, int L, int Ns, class C, class Buf, bool CR = false>
 __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, int lane) {
     constexpr int nbf = L / R, twstep = L / (Ns * R), BPL = (nbf + 63) / 64;
     C v[BPL][R];
@@ -327,6 +391,8 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
             if constexpr (R <= 4 || R == 9) {
 #pragma unroll
                 for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
+            } else if constexpr (R == 17 && CR) {
+                dft_odd_store_c<R>(v[i], buf, dst[i], Ns);
             } else {
                 dft_odd_store<R>(v[i], buf, dst[i], Ns, tw, L);
             }
@@ -334,10 +400,10 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
     __builtin_amdgcn_wave_barrier();
 }
 
-template <class C, class Buf, int L, int Ns, int R, int... Rest>
+template <class C, class Buf, bool CR, int L, int Ns, int R, int... Rest>
 __device__ __forceinline__ void wave_fft_s(Buf buf, const C *tw, int lane) {
-    wave_stage_s<R, L, Ns, C, Buf>(buf, tw, lane);
-    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, Buf, L, Ns * R, Rest...>(buf, tw, lane);
+    wave_stage_s<R, L, Ns, C, Buf, CR>(buf, tw, lane);
+    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, Buf, CR, L, Ns * R, Rest...>(buf, tw, lane);
 }
 
 // Lengths with a compile-time plan (radices in the planner's order: 4s, 2s, odd primes).
@@ -345,27 +411,31 @@ __device__ __forceinline__ void wave_fft_s(Buf buf, const C *tw, int lane) {
 template <int FIX> struct FixedFft;
 template <> struct FixedFft<0> {  // 576 = 4.4.4.9 (FP32, 289-node block)
     static constexpr int L = 576;
-    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 576, 1, 4, 4, 4, 9>(b, tw, lane);
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 576, 1, 4, 4, 4, 9>(b, tw, lane);
     }
 };
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)
     static constexpr int L = 288;
-    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 288, 1, 4, 4, 2, 9>(b, tw, lane);
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 288, 1, 4, 4, 2, 9>(b, tw, lane);
     }
 };
 template <> struct FixedFft<5> {  // 544 = 17.4.4.2 (FP32, 273-node inner box; the prime stage first needs
                                   // the fewest registers of the orders tried)
     static constexpr int L = 544;
-    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 544, 1, 17, 4, 4, 2>(b, tw, lane);
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 544, 1, 17, 4, 4, 2>(b, tw, lane);
     }
 };
 template <> struct FixedFft<3> {  // 272 = 4.4.17 (FP64, 273-node inner box)
     static constexpr int L = 272;
-    template <class C, class Buf> static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
-        wave_fft_s<C, Buf, 272, 1, 4, 4, 17>(b, tw, lane);
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 272, 1, 4, 4, 17>(b, tw, lane);
     }
 };
 

@@ -828,6 +828,11 @@ __host__ __device__ constexpr int fix_len() {
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && dbl)) ? (dbl ? 3 : 4) : 0;
 }
+// Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
+// the extra registers cost no occupancy).
+__host__ __device__ constexpr bool const_roots(bool dbl, int mode, int fix) {
+    return dbl ? (fix == 3 && mode != DST_Z) : (fix == 5 && (mode == DST_Z || mode == DST_XI));
+}
 
 // T: arithmetic and work-array type (double: plain FP64 form, half-length transforms;
 // float: the mixed form, odd-extension transforms, whose RHS, Dirichlet values and output
@@ -850,6 +855,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     const int nbatch = (int)((a.nlines + nl - 1) / nl);
     // eight padded sequences (one per wave), then the twiddles
     constexpr int S = wave_pad_shift(sizeof(T) == 8, MODE, FIX);
+    constexpr bool CROOTS = const_roots(sizeof(T) == 8, MODE, FIX);
     const int Np = pad_len<S>(N);
     C *A = lds;
     C *twl = lds + 8 * Np;  // exp(-2 pi i t / N)
@@ -946,7 +952,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
         } else {
             wave_pre<C>(B, a.sn, N, lane, fold, a.ih2[0]);
         }
-        if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
+        if constexpr (FIX >= 0) FixedFft<FIX>::template run<C, decltype(B), CROOTS>(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
         if (!ODD) wave_post<C>(B, N, lane);
         if (MODE == DST_Z) {
@@ -975,7 +981,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
                 __builtin_amdgcn_wave_barrier();
                 wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
             }
-            if constexpr (FIX >= 0) FixedFft<FIX>::run(B, twl, lane);
+            if constexpr (FIX >= 0) FixedFft<FIX>::template run<C, decltype(B), CROOTS>(B, twl, lane);
             else wave_fft<MAXR>(B, twl, P, lane);
             if (!ODD) wave_post<C>(B, N, lane);
         }
