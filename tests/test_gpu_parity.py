@@ -446,3 +446,23 @@ def test_vpm_modes_4_and_5(vpmlib, oracle):
     for k, v in errs.items():
         assert v <= TOL, (k, v)
     assert np.all(up5 == 0.0)
+
+
+def test_boundary_sums_spectral_long_axis(vpmlib, monkeypatch):
+    """Spectral boundary sums on a block with one 289-node axis, where the transforms along
+    that axis run the 576-point compile-time plan (one wave per sequence) and the others the
+    runtime wave plans, against the direct LDS-tile sums (pinned to the oracle on smaller
+    blocks); the oracle's direct sum is too slow at this size."""
+    V = vpmlib
+    nn = np.array([289, 33, 41], np.int32)
+    dpm = np.array([1 / 64, 1 / 60, 1 / 56])
+    xb = np.array([-2.25, -0.27, -0.36, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.1, -0.01, 0.02), s=0.08)
+    sol = np.zeros_like(rhs)
+    say("case: spectral boundary sums, 289-node axis", nn.tolist())
+    out = {}
+    for mode in ("spectral", "direct"):
+        monkeypatch.setenv("VPMX_BOUNDSUM", mode)
+        with V.VPM(V.Params.make(0.125)) as c:
+            out[mode] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
+    assert rel_l2(out["spectral"], out["direct"]) <= TOL

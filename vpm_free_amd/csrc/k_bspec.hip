@@ -134,24 +134,46 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_table(SpecFft f, SpecTab t,
     }
 }
 
+// Transforms of the forward and inverse kernels.  FFT = -2: the work-group Stockham stages
+// (f.npairs sequences, DST_THREADS threads); -1: one wave per sequence with the runtime plan P;
+// 0: one wave per sequence with the 576-point compile-time plan.  The wave forms run
+// BW_WAVES sequences per work-group and need no work-group barrier between their stages.
+constexpr int BW_WAVES = 4, BW_LMAX = 576;
+template <int FFT>
+constexpr int bsx_threads() { return FFT == -2 ? DST_THREADS : 64 * BW_WAVES; }
+template <int FFT>
+__device__ __forceinline__ void bsx_fft(double2 *A, const double2 *twl, const SpecFft &f, const WavePlan &P) {
+    if constexpr (FFT == -2) {
+        fft_reg<SPEC_MAXR>(A, twl, f);  // ends with a barrier
+    } else {
+        const int lane = threadIdx.x & 63;
+        double2 *S = A + (threadIdx.x >> 6) * P.L;
+        if constexpr (FFT == 0) FixedFft<0>::template run<double2, double2 *, false>(S, twl, lane);
+        else wave_fft<SPEC_MAXR, double2, double2 *, BW_LMAX>(S, twl, P, lane);
+        __syncthreads();
+    }
+}
+
 // A^ of every column as two real columns: Bm[2 col][w][q] = Re, Bm[2 col + 1][w][q] = Im,
 // lines q of the block, one column per blockIdx.y.
-__global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, const double2 *tw,
-                                                         const SpecCol *cols, const double *a, int64_t ns,
-                                                         double *Bm) {
+template <int FFT>
+__global__ void __launch_bounds__(bsx_threads<FFT>()) k_bsx_fwd(SpecFft f, WavePlan P, SpecTab t, const double2 *tw,
+                                                                 const SpecCol *cols, const double *a, int64_t ns,
+                                                                 double *Bm) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
-    const int L = f.L, np = f.npairs, nl = 2 * np;
+    constexpr int NT = bsx_threads<FFT>();
+    const int L = f.L, np = FFT == -2 ? f.npairs : BW_WAVES, nl = 2 * np;
     double2 *A = lds, *twl = lds + np * L;
     double *Ad = (double *)A;
     const int tid = threadIdx.x, q0 = blockIdx.x * nl, col = blockIdx.y;
     const SpecCol cd = cols[col];
     const double *src = a + (int64_t)cd.comp * ns + cd.src_off;
     const int ncell = t.nC - 1;  // source cells along C
-    for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
+    for (int i = tid; i < L; i += NT) twl[i] = tw[i];
     if (cd.s_is_u) {
         // q along the face's contiguous axis: neighbouring threads take neighbouring lines
-        for (int i = tid; i < nl * L; i += DST_THREADS) {
+        for (int i = tid; i < nl * L; i += NT) {
             const int r = i / nl, l = i - r * nl, q = q0 + l;
             double v = 0.0;
             if (r < ncell && q < t.K) {
@@ -161,7 +183,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, c
             Ad[2 * ((l >> 1) * L + r) + (l & 1)] = v;
         }
     } else {
-        for (int i = tid; i < nl * L; i += DST_THREADS) {
+        for (int i = tid; i < nl * L; i += NT) {
             const int l = i / L, r = i - l * L, q = q0 + l;
             double v = 0.0;
             if (r < ncell && q < t.K) {
@@ -172,10 +194,10 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, c
         }
     }
     __syncthreads();
-    fft_reg<SPEC_MAXR>(A, twl, f);
+    bsx_fft<FFT>(A, twl, f, P);
     const int64_t cst = (int64_t)t.nw * t.K;
     double *out = Bm + 2 * col * cst;
-    for (int i = tid; i < t.nw * nl; i += DST_THREADS) {
+    for (int i = tid; i < t.nw * nl; i += NT) {
         const int k = i / nl, l = i - k * nl, q = q0 + l;
         if (q < t.K) {
             const double2 z = unpack_half(A, L, l >> 1, k, l & 1);
@@ -187,18 +209,20 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_fwd(SpecFft f, SpecTab t, c
 
 // Inverse real FFT over w of O^ = exp(-pi i w / L) (Cm[2 col][w][p] + i Cm[2 col + 1][w][p]),
 // first nC outputs -> Ores[col][p][kt].
-__global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, const double2 *tw, const double *Cm,
-                                                         double *Ores, int64_t ores_off) {
+template <int FFT>
+__global__ void __launch_bounds__(bsx_threads<FFT>()) k_bsx_inv(SpecFft f, WavePlan P, SpecTab t, const double2 *tw,
+                                                                 const double *Cm, double *Ores, int64_t ores_off) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
-    const int L = f.L, np = f.npairs, nl = 2 * np;
+    constexpr int NT = bsx_threads<FFT>();
+    const int L = f.L, np = FFT == -2 ? f.npairs : BW_WAVES, nl = 2 * np;
     double2 *A = lds, *twl = lds + np * L;
     const int tid = threadIdx.x, p0 = blockIdx.x * nl, col = blockIdx.y;
     const int64_t cst = (int64_t)t.nw * t.M;
     const double *in = Cm + 2 * col * cst;
-    for (int i = tid; i < L; i += DST_THREADS) twl[i] = tw[i];
+    for (int i = tid; i < L; i += NT) twl[i] = tw[i];
     // Z_k = X_k + i Y_k with the Hermitian halves of lines p (X) and p+1 (Y)
-    for (int i = tid; i < t.nw * np; i += DST_THREADS) {
+    for (int i = tid; i < t.nw * np; i += NT) {
         const int k = i / np, pr = i - k * np, p = p0 + 2 * pr;
         const double2 ph = half_shift(k, L);
         const int64_t o = (int64_t)k * t.M + p;
@@ -214,11 +238,11 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_inv(SpecFft f, SpecTab t, c
         }
     }
     __syncthreads();
-    fft_reg<SPEC_MAXR>(A, twl, f);
+    bsx_fft<FFT>(A, twl, f, P);
     // z_n = FFT(Z)[(L - n) mod L] / L
     const double invL = 1.0 / (double)L;
     double *out = Ores + ores_off + (int64_t)col * t.M * t.nC;
-    for (int i = tid; i < nl * t.nC; i += DST_THREADS) {
+    for (int i = tid; i < nl * t.nC; i += NT) {
         const int l = i / t.nC, n = i - l * t.nC, p = p0 + l;
         if (p >= t.M) continue;
         const double2 w = A[(l >> 1) * L + (n == 0 ? 0 : L - n)];
@@ -298,6 +322,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_qconv(SpecFft fq, SpecTab t
 // The same convolution with one wave per (column, w) sequence (Lq <= QC_LMAX): no work-group
 // barriers after the twiddles are staged, QC_WAVES sequences per work-group.
 constexpr int QC_WAVES = 4, QC_LMAX = 576;
+template <int FFT>  // 0: the 576-point compile-time plan, -1: the runtime plan P
 __global__ void __launch_bounds__(64 * QC_WAVES) k_bsx_qconv_w(WavePlan P, SpecTab t, const double2 *twq,
                                                                const double2 *T2, const double *Bm, double *Cm,
                                                                int ncol) {
@@ -315,11 +340,13 @@ __global__ void __launch_bounds__(64 * QC_WAVES) k_bsx_qconv_w(WavePlan P, SpecT
     double2 *A = lds + wv * Lq;
     for (int q = lane; q < Lq; q += 64) A[q] = q < t.K ? make_double2(bre[q], bim[q]) : make_double2(0.0, 0.0);
     __builtin_amdgcn_wave_barrier();
-    wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
+    if constexpr (FFT == 0) FixedFft<0>::template run<double2, double2 *, false>(A, twl, lane);
+    else wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
     const double2 *tr = T2 + (int64_t)w * Lq;
     for (int v = lane; v < Lq; v += 64) A[v] = cmul(A[v], tr[v]);
     __builtin_amdgcn_wave_barrier();
-    wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
+    if constexpr (FFT == 0) FixedFft<0>::template run<double2, double2 *, false>(A, twl, lane);
+    else wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
     double *ore = Cm + 2 * col * cc + (int64_t)w * t.M, *oim = ore + cc;
     for (int p = lane; p < t.M; p += 64) {
         const double2 z = A[p == 0 ? 0 : Lq - p];
@@ -388,6 +415,8 @@ struct SpecTable {
     double *G = nullptr;  // perpendicular faces: R[w][q][p]
     SpecFft fq{};         // parallel faces: the convolution along q
     WavePlan wq{};        // the same plan for the one-wave-per-sequence kernel (wq.L = 0: not used)
+    WavePlan wf{};        // the transform along C for the one-wave-per-sequence forms
+    int form = -2;        // bsx_form of the transforms along C
     const double2 *twq = nullptr;
     double2 *T2 = nullptr;  // parallel faces: FFT(r_w) / Lq, [w][v]
     std::vector<SpecCol> cols;
@@ -461,6 +490,25 @@ static int fft_setup(Ctx *c, int L, SpecFft *f, const double2 **tw) {
     *tw = fs.tw;
     return VPMX_OK;
 }
+
+// The same plan for the one-wave-per-sequence stages.
+static WavePlan wave_plan_of(const SpecFft &f) {
+    WavePlan P{};
+    int ns = 1;
+    P.L = f.L;
+    P.nstages = f.nstages;
+    for (int q = 0; q < f.nstages; q++) {
+        P.R[q] = f.radix[q];
+        P.Ns[q] = ns;
+        P.mg[q] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns + 1ull) : 0u;
+        ns *= f.radix[q];
+    }
+    return P;
+}
+
+// Transform form of the forward and inverse kernels (bsx_fft): the compile-time plan at 576
+// points, one wave per sequence up to BW_LMAX, else the work-group stages.
+static int bsx_form(int L) { return L == 576 ? 0 : (L <= BW_LMAX ? -1 : -2); }
 
 static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
     SpecPlan *P = new SpecPlan();
@@ -591,17 +639,9 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
         if (e == hipSuccess) st = fft_setup(c, T.t.L, &T.f, &T.tw);
         if (!st && T.t.type == 1) st = fft_setup(c, T.t.Lq, &T.fq, &T.twq);
         if (st) break;
-        if (T.t.type == 1 && T.t.Lq <= QC_LMAX) {
-            int ns = 1;
-            T.wq.L = T.t.Lq;
-            T.wq.nstages = T.fq.nstages;
-            for (int q = 0; q < T.fq.nstages; q++) {
-                T.wq.R[q] = T.fq.radix[q];
-                T.wq.Ns[q] = ns;
-                T.wq.mg[q] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns + 1ull) : 0u;
-                ns *= T.fq.radix[q];
-            }
-        }
+        if (T.t.type == 1 && T.t.Lq <= QC_LMAX) T.wq = wave_plan_of(T.fq);
+        T.wf = wave_plan_of(T.f);
+        T.form = bsx_form(T.t.L);
         const size_t lds = sizeof(double2) * (size_t)(T.f.npairs + 1) * T.t.L;
         if (T.t.type == 1) {
             // r_w(d) for d = -(K-1)..M-1: the table kernel with M + K - 1 lines, one q
@@ -666,16 +706,29 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
     rocblas_handle hb = (rocblas_handle)c->blas;
     VPMX_TRY(blas_check(c, rocblas_set_stream(hb, c->stream), "set_stream"));
     for (auto &T : P->tabs) {
-        const size_t lds = sizeof(double2) * (size_t)(T.f.npairs + 1) * T.t.L;
-        const int nl = 2 * T.f.npairs, ncol = (int)T.cols.size();
-        hipLaunchKernelGGL(k_bsx_fwd, dim3((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
-                           c->stream, T.f, T.t, T.tw, T.cols_d, a, ns, P->Bm + T.b_off);
+        const int np = T.form == -2 ? T.f.npairs : BW_WAVES, nl = 2 * np, ncol = (int)T.cols.size();
+        const size_t lds = sizeof(double2) * (size_t)(np + 1) * T.t.L;
+        const dim3 gf((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol), gi((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol);
+        if (T.form == 0)
+            hipLaunchKernelGGL(k_bsx_fwd<0>, gf, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw, T.cols_d,
+                               a, ns, P->Bm + T.b_off);
+        else if (T.form == -1)
+            hipLaunchKernelGGL(k_bsx_fwd<-1>, gf, dim3(bsx_threads<-1>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, a, ns, P->Bm + T.b_off);
+        else
+            hipLaunchKernelGGL(k_bsx_fwd<-2>, gf, dim3(bsx_threads<-2>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, a, ns, P->Bm + T.b_off);
         VPMX_HIP(c, hipGetLastError());
         if (T.t.type == 1 && T.wq.L > 0) {
             const size_t ldsq = sizeof(double2) * (size_t)(QC_WAVES + 1) * T.t.Lq;
             const int nseq = ncol * T.t.nw;
-            hipLaunchKernelGGL(k_bsx_qconv_w, dim3((unsigned)((nseq + QC_WAVES - 1) / QC_WAVES)), dim3(64 * QC_WAVES),
-                               ldsq, c->stream, T.wq, T.t, T.twq, T.T2, P->Bm + T.b_off, P->Cm + T.c_off, ncol);
+            const dim3 gq((unsigned)((nseq + QC_WAVES - 1) / QC_WAVES));
+            if (T.t.Lq == 576)
+                hipLaunchKernelGGL(k_bsx_qconv_w<0>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, T.t, T.twq, T.T2,
+                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol);
+            else
+                hipLaunchKernelGGL(k_bsx_qconv_w<-1>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, T.t, T.twq, T.T2,
+                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol);
             VPMX_HIP(c, hipGetLastError());
         } else if (T.t.type == 1) {
             const size_t ldsq = sizeof(double2) * (size_t)(T.fq.npairs + 1) * T.t.Lq;
@@ -694,8 +747,15 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
                                                               T.t.nw),
                                 "dgemm_strided_batched"));
         }
-        hipLaunchKernelGGL(k_bsx_inv, dim3((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol), dim3(DST_THREADS), lds,
-                           c->stream, T.f, T.t, T.tw, P->Cm + T.c_off, P->Ores, T.o_off);
+        if (T.form == 0)
+            hipLaunchKernelGGL(k_bsx_inv<0>, gi, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               P->Cm + T.c_off, P->Ores, T.o_off);
+        else if (T.form == -1)
+            hipLaunchKernelGGL(k_bsx_inv<-1>, gi, dim3(bsx_threads<-1>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               P->Cm + T.c_off, P->Ores, T.o_off);
+        else
+            hipLaunchKernelGGL(k_bsx_inv<-2>, gi, dim3(bsx_threads<-2>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               P->Cm + T.c_off, P->Ores, T.o_off);
         VPMX_HIP(c, hipGetLastError());
     }
     int maxp = 0;
