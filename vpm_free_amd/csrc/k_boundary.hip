@@ -529,7 +529,7 @@ int point_sums(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, const doub
         hipEvent_t e0, e1;
         const bool timed = kernel_events(c, K_BOUNDSUM, &e0, &e1);
         if (timed) VPMX_HIP(c, hipEventRecord(e0, c->stream));
-        const int st = boundary_sums_spectral(c, g, ns, a, sol);
+        const int st = boundary_sums_spectral(c, g, ns, a, sol, 0);
         if (st == VPMX_OK) {
             if (timed) VPMX_HIP(c, hipEventRecord(e1, c->stream));
             mark(c, ST_BOUNDS, 1);
@@ -539,6 +539,10 @@ int point_sums(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, const doub
         if (st != VPMX_ERR_UNSUP) return st;
         if (timed) c->kev_pending[K_BOUNDSUM] -= 1;  // nothing ran: drop the interval
         mark(c, ST_BOUNDS, 1);  // faces too long for the in-LDS transforms: lattice kernel
+    } else if (c->bs_mode == 2) {  // the coarse grid: its own plan
+        const int st = boundary_sums_spectral(c, g, ns, a, sol, 1);
+        if (st == VPMX_OK) return VPMX_OK;
+        if (st != VPMX_ERR_UNSUP) return st;
     }
     if (c->bs_mode != 1) return boundary_sums_lattice(c, g, ns, a, sol, fine);
     int64_t tblocks = (nt + BS_THREADS * BS_TPT - 1) / (BS_THREADS * BS_TPT);

@@ -449,8 +449,10 @@ static void free_plan(SpecPlan *P) {
 }
 
 void free_bspec(Ctx *c) {
-    free_plan((SpecPlan *)c->bsx);
-    c->bsx = nullptr;
+    for (int q = 0; q < 2; q++) {
+        free_plan((SpecPlan *)c->bsx[q]);
+        c->bsx[q] = nullptr;
+    }
     if (c->blas) (void)rocblas_destroy_handle((rocblas_handle)c->blas);
     c->blas = nullptr;
 }
@@ -683,18 +685,18 @@ static int blas_check(Ctx *c, rocblas_status s, const char *what) {
     return fail(c, VPMX_ERR_HIP, std::string("rocBLAS ") + what + ": " + rocblas_status_to_string(s));
 }
 
-int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol) {
-    SpecPlan *P = (SpecPlan *)c->bsx;
+int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, int slot) {
+    SpecPlan *P = (SpecPlan *)c->bsx[slot];
     if (P && (P->n[0] != g.n[0] || P->n[1] != g.n[1] || P->n[2] != g.n[2] || P->h[0] != g.h[0] ||
               P->h[1] != g.h[1] || P->h[2] != g.h[2])) {
         VPMX_HIP(c, hipStreamSynchronize(c->stream));
         free_plan(P);
-        c->bsx = P = nullptr;
+        c->bsx[slot] = P = nullptr;
     }
     if (!P) {
         int st = build_plan(c, g, &P);
         if (st) return st;
-        c->bsx = P;
+        c->bsx[slot] = P;
     }
     if (!c->blas) {
         rocblas_handle hb;
@@ -775,7 +777,7 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         const double per = 5.0 * T.t.L * std::log2((double)T.t.L);
         fft += per * (double)T.cols.size() * (T.t.K + T.t.M) / 2.0;
     }
-    c->k_work[K_BOUNDSUM] = P->flops + fft;
+    if (slot == 0) c->k_work[K_BOUNDSUM] = P->flops + fft;
     return VPMX_OK;
 }
 

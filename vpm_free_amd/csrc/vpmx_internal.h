@@ -113,7 +113,8 @@ int boundary_sums_panels(Ctx *c, const Grid &g, int64_t nb, const double *sig, c
                          GView sol, bool fine, int levmax, GView scratch);
 // Spectral evaluation of the same sums (k_bspec.hip); returns VPMX_ERR_UNSUP when the
 // block's faces are too long for its in-LDS FFTs, so the caller can use the lattice kernel.
-int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol);
+// slot: 0 the fine block, 1 the coarse grid (each keeps its own plan across calls).
+int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol, int slot);
 void free_bspec(Ctx *c);
 // Tree form (k_tree.hip): sig (3 x nbound), cxyz (3 x nbound), ds (nbound) of normal_derivs.
 // panels: tree_calc_3d_s (ibctyp = 3) instead of tree_calc_3d.
