@@ -725,7 +725,9 @@ __global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const 
                                 int *__restrict__ list);
 
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
-               int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32) {
+               int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32,
+               bool zero_empty) {
+    c->tile_map = false;
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
@@ -781,8 +783,15 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
                                vn[2], g, s.cn[1], rowcnt, wlo[1], wlo[2], flag);
             VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
             hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
-            hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx, nty,
-                               vn[0], vn[1], vn[2], flag, out);
+            if (zero_empty) {
+                hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx,
+                                   nty, vn[0], vn[1], vn[2], flag, out);
+            } else {
+                c->tile_map = true;
+                for (int d = 0; d < 3; d++) c->tile_map_vn[d] = vn[d];
+                c->tile_map_ntx = ntx;
+                c->tile_map_nty = nty;
+            }
             const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
             int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
             const int *lst = list, *nl = pos + tiles;
@@ -1038,6 +1047,7 @@ __global__ void k_m2p_outside(const int *__restrict__ keys, int64_t n, double *u
 
 int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int itype,
                GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb) {
+    c->tile_map = false;  // the interpolation tiles reuse tile_flag
     if (n <= 0) return VPMX_OK;
     hipEvent_t e0, e1;
     const dim3 grid((unsigned)((n + 255) / 256));
@@ -1195,13 +1205,21 @@ int diagnostics(Ctx *c, const double *xp, const double *qp, int qs, int64_t n, d
 // ------------------------------------------------------------------ remesh
 // remesh_particles_3d (vpm_remesh.f90:137-195), ncell_rem=1: nodes k->j->i over [lo,hi],
 // kept when sqrt(w1^2+w2^2+w3^2) >= thr; the kept nodes keep that order (prefix sum).
-__global__ void k_remesh_flag(GView w, int lx, int ly, int lz, int nx, int ny, int nz, double thr, int *flag) {
+// tmap (optional): node tiles of the projection holding values (ntx x nty per plane); nodes
+// of the other tiles were not written and are zero.
+__global__ void k_remesh_flag(GView w, int lx, int ly, int lz, int nx, int ny, int nz, double thr, int *flag,
+                              const int *__restrict__ tmap, int ntx, int nty) {
 #pragma clang fp contract(off)
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t nt = (int64_t)nx * ny * nz;
     if (t >= nt) return;
     int a = (int)(t % nx), b = (int)((t / nx) % ny), cc = (int)(t / ((int64_t)nx * ny));
-    int64_t o = w.at(lx - 1 + a, ly - 1 + b, lz - 1 + cc);
+    const int i = lx - 1 + a, j = ly - 1 + b, k = lz - 1 + cc;
+    if (tmap && !tmap[i / PT_X + ntx * (j / PT_Y + nty * (k / PT_Z))]) {
+        flag[t] = 0;
+        return;
+    }
+    int64_t o = w.at(i, j, k);
     double r0 = w.p[o], r1 = w.p[w.cs + o], r2 = w.p[2 * w.cs + o];
     double wmag = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
     flag[t] = (wmag < thr) ? 0 : 1;
@@ -1228,7 +1246,6 @@ __global__ void k_remesh_write(GView w, int lx, int ly, int lz, int nx, int ny, 
 int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr, const double x0[3],
                    const double h[3], double dv, int neqpm, double **xp_out, double **qp_out, DBuf &bx, DBuf &bq,
                    int64_t *count) {
-    (void)vn;
     int st = VPMX_OK;
     int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
     if (nx <= 0 || ny <= 0 || nz <= 0) { *count = 0; return VPMX_OK; }
@@ -1236,7 +1253,11 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
     int *flag = (int *)ensure(c, c->rm_flag, sizeof(int) * (size_t)nt, &st); if (st) return st;
     int *pos = (int *)ensure(c, c->rm_pos, sizeof(int) * (size_t)(nt + 1), &st); if (st) return st;
     unsigned gr = (unsigned)((nt + 255) / 256);
-    hipLaunchKernelGGL(k_remesh_flag, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, thr, flag);
+    // the tile map of the projection that filled w, if it left empty tiles unwritten
+    const bool tm = c->tile_map && c->tile_map_vn[0] == vn[0] && c->tile_map_vn[1] == vn[1] && c->tile_map_vn[2] == vn[2];
+    hipLaunchKernelGGL(k_remesh_flag, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, thr, flag,
+                       tm ? (const int *)c->tile_flag.p : nullptr, c->tile_map_ntx, c->tile_map_nty);
+    c->tile_map = false;
     VPMX_TRY(exclusive_scan(c, flag, pos, nt));
     int total = 0;
     VPMX_HIP(c, hipMemcpyAsync(&total, pos + nt, sizeof(int), hipMemcpyDeviceToHost, c->stream));

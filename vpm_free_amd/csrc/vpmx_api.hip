@@ -522,7 +522,10 @@ static int run_remesh(Ctx *c, const double *xp, const double *qp, int64_t n, int
     Grid gr = make_grid(NN, G.xbound, D);
     VPMX_TRY(sort_particles(c, gr, xp, n, c->sort));
     int wlo[3] = {0, 0, 0};
-    VPMX_TRY(p2m_gather(c, gr, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(grid, NN), NN, wlo));
+    // FP64 in both forms (the flags and positions are exact); nodes of tiles no particle
+    // reaches stay unwritten, and the flag pass reads the tile map instead
+    VPMX_TRY(p2m_gather(c, gr, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(grid, NN), NN, wlo, -1,
+                        false, false));
     int half = P.interf_iproj / 2;
     int lo[3] = {NNb[0] + half, NNb[1] + half, NNb[2] + half}, hi[3] = {NNb[3] - half, NNb[4] - half, NNb[5] - half};
     const double thr = (double)1e-09f;  // default-REAL literal (vpm_remesh.f90:184)

@@ -63,6 +63,10 @@ struct Ctx : vpmx_ctx {
     DBuf dst_work, lam[3];
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
     DBuf tile_flag, tile_pos, tile_list;  // interpolation: cell tiles holding particles
+    // projection without zeros for empty tiles: tile_flag maps the node tiles of a view of
+    // tile_map_vn nodes (ntx x nty tiles per plane) that hold values; valid until the next use
+    bool tile_map = false;
+    int tile_map_vn[3] = {0, 0, 0}, tile_map_ntx = 0, tile_map_nty = 0;
     DBuf stage_a, stage_b, stage_c;
     SortState sort;
     int64_t sort_n = 0;     // particles of the last sort
