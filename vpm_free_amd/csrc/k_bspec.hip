@@ -685,6 +685,33 @@ static int blas_check(Ctx *c, rocblas_status s, const char *what) {
     return fail(c, VPMX_ERR_HIP, std::string("rocBLAS ") + what + ": " + rocblas_status_to_string(s));
 }
 
+// The context's rocBLAS handle on its stream (atomics off: results independent of scheduling).
+static int blas_handle(Ctx *c, rocblas_handle *out) {
+    if (!c->blas) {
+        rocblas_handle hb;
+        VPMX_TRY(blas_check(c, rocblas_create_handle(&hb), "create_handle"));
+        c->blas = hb;
+        VPMX_TRY(blas_check(c, rocblas_set_atomics_mode(hb, rocblas_atomics_not_allowed), "set_atomics_mode"));
+        VPMX_TRY(blas_check(c, rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host), "set_pointer_mode"));
+    }
+    rocblas_handle hb = (rocblas_handle)c->blas;
+    VPMX_TRY(blas_check(c, rocblas_set_stream(hb, c->stream), "set_stream"));
+    *out = hb;
+    return VPMX_OK;
+}
+
+int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
+                  int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch) {
+    rocblas_handle hb = nullptr;
+    VPMX_TRY(blas_handle(c, &hb));
+    const double one = 1.0, zero = 0.0;
+    return blas_check(c,
+                      rocblas_dgemm_strided_batched(hb, rocblas_operation_none,
+                                                    transb ? rocblas_operation_transpose : rocblas_operation_none, M, N,
+                                                    K, &one, A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, batch),
+                      "dgemm_strided_batched");
+}
+
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, int slot) {
     SpecPlan *P = (SpecPlan *)c->bsx[slot];
     if (P && (P->n[0] != g.n[0] || P->n[1] != g.n[1] || P->n[2] != g.n[2] || P->h[0] != g.h[0] ||
@@ -698,15 +725,8 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         if (st) return st;
         c->bsx[slot] = P;
     }
-    if (!c->blas) {
-        rocblas_handle hb;
-        VPMX_TRY(blas_check(c, rocblas_create_handle(&hb), "create_handle"));
-        c->blas = hb;
-        VPMX_TRY(blas_check(c, rocblas_set_atomics_mode(hb, rocblas_atomics_not_allowed), "set_atomics_mode"));
-        VPMX_TRY(blas_check(c, rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host), "set_pointer_mode"));
-    }
-    rocblas_handle hb = (rocblas_handle)c->blas;
-    VPMX_TRY(blas_check(c, rocblas_set_stream(hb, c->stream), "set_stream"));
+    rocblas_handle hb = nullptr;
+    VPMX_TRY(blas_handle(c, &hb));
     for (auto &T : P->tabs) {
         const int np = T.form == -2 ? T.f.npairs : BW_WAVES, nl = 2 * np, ncol = (int)T.cols.size();
         const size_t lds = sizeof(double2) * (size_t)(np + 1) * T.t.L;

@@ -13,6 +13,7 @@
 // autosort (any factorisation of L; radices 2, 3, 4 specialised, others generic).
 // The z pass does forward transform, diagonal scaling and inverse transform without
 // leaving LDS, so one solve costs 5 sweeps over the interior.
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <type_traits>
@@ -1099,6 +1100,8 @@ void free_fft_plans(Ctx *c) {
     c->eig_cache.clear();
     for (auto &kv : c->sin_cache) (void)hipFree(kv.second);
     c->sin_cache.clear();
+    for (auto &kv : c->sinm_cache) (void)hipFree(kv.second);
+    c->sinm_cache.clear();
 }
 
 // Eigenvalues 4 sin^2(pi k / (2(m+1))) / h^2 of the 1-D Dirichlet second difference,
@@ -1171,6 +1174,29 @@ static void setup_axis(DstArgs &a, FftPlan *p, int m) {
     for (int s = 0; s < p->nstages; s++) a.radix[s] = p->radix[s];
     a.tw = p->tw;
     a.npairs = 1;
+}
+
+// The DST-I matrix sin(pi (t+1)(k+1) / (m+1)), t, k < m (row t), uploaded once per m.
+static int sine_matrix(Ctx *c, int m, const double **out) {
+    auto it = c->sinm_cache.find(m);
+    if (it != c->sinm_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    std::vector<double> v((size_t)m * m);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int t = 0; t < m; t++)
+        for (int k = 0; k < m; k++) {
+            // reduce (t+1)(k+1) mod 2(m+1) first so the argument stays below 2 pi
+            const long long q = ((long long)(t + 1) * (k + 1)) % (2LL * (m + 1));
+            v[(size_t)t * m + k] = (double)sinl(PI * (long double)q / (long double)(m + 1));
+        }
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * v.size()));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    c->sinm_cache[m] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
 }
 
 // sin(j pi / N), j = 0..N/2, uploaded once per N.
@@ -1341,7 +1367,132 @@ static int run_pass(Ctx *c, DstArgs a, int ncomp, const AxisPlan &P, bool f32) {
     return launch_pass<MODE>(c, a, ncomp);
 }
 
-int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out) {
+// ---------------------------------------------------------------- forward-pass reuse
+// The x and y forward passes are linear, and the second solve of pmesh_inf differs from the
+// first only by the folded Dirichlet terms F (on the layers next to the six faces), so after
+// those passes its work array is the first solve's minus
+//   C(kx,ky,k) = sum_{i,j} F(i,j,k) S_x(i,kx) S_y(j,ky),   S(i,k) = sin(pi i k / (m+1)),
+// which reduces to 1-D sine transforms of the x and y faces' values and a 2-D one of the z
+// faces' values:
+//   C = ih2x (S_x(1,kx) X0(ky,k) + S_x(mx,kx) X1(ky,k)) + ih2y (S_y(1,ky) Y0(kx,k) + S_y(my,ky) Y1(kx,k))
+//     + ih2z ([k = 1] Z0(kx,ky) + [k = mz] Z1(kx,ky)).
+// FP64 tables; one pass forms W2 = W1 - C in the work array's precision.
+// Face values into dense matrices, job q = 6 comp + face: x faces (face 0, 1) [k][j],
+// y faces (2, 3) [k][i], z faces (4, 5) [i][j] (the first index is the row).
+__global__ void k_fold_gather(GView bc, int mx, int my, int mz, double *ax, double *ay, double *az) {
+    const int q = blockIdx.y, comp = q / 6, f = q - 6 * comp;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double *b = bc.p + comp * bc.cs;
+    const int side = f & 1;
+    if (f < 2) {
+        if (e >= (int64_t)mz * my) return;
+        const int k = (int)(e / my), j = (int)(e - (int64_t)k * my);
+        ax[(int64_t)(2 * comp + side) * mz * my + e] = b[bc.at(side ? mx + 1 : 0, j + 1, k + 1)];
+    } else if (f < 4) {
+        if (e >= (int64_t)mz * mx) return;
+        const int k = (int)(e / mx), i = (int)(e - (int64_t)k * mx);
+        ay[(int64_t)(2 * comp + side) * mz * mx + e] = b[bc.at(i + 1, side ? my + 1 : 0, k + 1)];
+    } else {
+        if (e >= (int64_t)mx * my) return;
+        const int i = (int)(e / my), j = (int)(e - (int64_t)i * my);
+        az[(int64_t)(2 * comp + side) * mx * my + e] = b[bc.at(i + 1, j + 1, side ? mz + 1 : 0)];
+    }
+}
+
+// Tables per (comp, side) pair p = 2 comp + side: X[p] [k][ky] (mz x my), Y[p] [k][kx]
+// (mz x mx), Z[p] [ky][kx] (my x mx).  A work-group takes FC_SPAN rows ky of one plane
+// (k, comp), a thread one kx: the kx-dependent factors are loaded once, the ky-dependent ones
+// are uniform (scalar loads), and FC_UNROLL row loads are in flight per thread.
+constexpr int FC_SPAN = 48, FC_UNROLL = 8;
+template <class T>
+__global__ void __launch_bounds__(320) k_fold_correct(const T *__restrict__ w1, T *__restrict__ w2, int px, int mx,
+                                                      int my, int mz, const double *__restrict__ X,
+                                                      const double *__restrict__ Y, const double *__restrict__ Z,
+                                                      double ihx, double ihy, double ihz, const double *__restrict__ Sx,
+                                                      const double *__restrict__ Sy) {
+    const int kx = threadIdx.x, k = blockIdx.y % mz, comp = blockIdx.y / mz;
+    const int ky0 = blockIdx.x * FC_SPAN, ky1 = ky0 + FC_SPAN < my ? ky0 + FC_SPAN : my;
+    const double *X0 = X + (int64_t)(2 * comp) * mz * my + (int64_t)k * my, *X1 = X0 + (int64_t)mz * my;
+    const double *Y0 = Y + (int64_t)(2 * comp) * mz * mx + (int64_t)k * mx, *Y1 = Y0 + (int64_t)mz * mx;
+    const double *Z0 = Z + (int64_t)(2 * comp) * my * mx, *Z1 = Z0 + (int64_t)my * mx;
+    const double *Sy1 = Sy + (int64_t)(my - 1) * my;
+    // the rows' factors, staged once per work-group
+    __shared__ double rf[4][FC_SPAN];
+    for (int t = threadIdx.x; t < 4 * FC_SPAN; t += blockDim.x) {
+        const int q = t / FC_SPAN, y = ky0 + (t - q * FC_SPAN);
+        double v = 0.0;
+        if (y < ky1) v = q == 0 ? X0[y] : (q == 1 ? X1[y] : (q == 2 ? Sy[y] : Sy1[y]));
+        rf[q][t - q * FC_SPAN] = v;
+    }
+    __syncthreads();
+    if (kx >= px) return;
+    // the row padding (kx >= mx) is copied along, so every row is written whole
+    const bool in = kx < mx;
+    const double ax0 = in ? ihx * Sx[kx] : 0.0, ax1 = in ? ihx * Sx[(int64_t)(mx - 1) * mx + kx] : 0.0;
+    const double by0 = in ? ihy * Y0[kx] : 0.0, by1 = in ? ihy * Y1[kx] : 0.0;
+    const int64_t base = ((int64_t)comp * mz + k) * my * px + kx;
+    for (int ky = ky0; ky < ky1; ky += FC_UNROLL) {
+        T v[FC_UNROLL];
+#pragma unroll
+        for (int r = 0; r < FC_UNROLL; r++) {
+            // every load issued (a row past the span reads the last row again): a conditional
+            // load would wait before the next one is issued
+            const int y = ky + r < ky1 ? ky + r : ky1 - 1;
+            v[r] = w1[base + (int64_t)y * px];
+        }
+#pragma unroll
+        for (int r = 0; r < FC_UNROLL; r++) {
+            const int y = ky + r, yl = y - ky0;
+            if (y >= ky1) break;
+            double cr = (ax0 * rf[0][yl] + ax1 * rf[1][yl]) + (by0 * rf[2][yl] + by1 * rf[3][yl]);
+            if (in && k == 0) cr += ihz * Z0[(int64_t)y * mx + kx];
+            if (in && k == mz - 1) cr += ihz * Z1[(int64_t)y * mx + kx];
+            w2[base + (int64_t)y * px] = (T)((double)v[r] - cr);
+        }
+    }
+}
+
+// W (the work array) = kept array - C from the Dirichlet values in bc.  The face transforms
+// are products with the DST-I matrices (row-major out[a][k] = sum_t in[a][t] S[t][k], i.e.
+// column-major out^T = S^T in^T).
+static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GView &bc) {
+    int st = VPMX_OK;
+    const int mx = a.mx, my = a.my, mz = a.mz, np = 2 * ncomp;
+    const double *Sx = nullptr, *Sy = nullptr;
+    VPMX_TRY(sine_matrix(c, mx, &Sx));
+    VPMX_TRY(sine_matrix(c, my, &Sy));
+    const int64_t fx = (int64_t)mz * my, fy = (int64_t)mz * mx, fz = (int64_t)mx * my;
+    // gathered faces (fx + fy + fz per pair), then X, Y, T, Z (fx + fy + 2 fz per pair)
+    double *tab = (double *)ensure(c, c->fold_tab, sizeof(double) * (size_t)(np * (2 * fx + 2 * fy + 3 * fz)), &st);
+    if (st) return st;
+    double *ax = tab, *ay = ax + np * fx, *az = ay + np * fy;         // gathered faces
+    double *X = az + np * fz, *Y = X + np * fx, *T = Y + np * fy, *Z = T + np * fz;  // transforms
+    const int64_t fmax = std::max(fx, std::max(fy, fz));
+    hipLaunchKernelGGL(k_fold_gather, dim3((unsigned)((fmax + 255) / 256), (unsigned)(6 * ncomp)), dim3(256), 0,
+                       c->stream, bc, mx, my, mz, ax, ay, az);
+    VPMX_HIP(c, hipGetLastError());
+    // X[p][k][ky] = sum_j ax[p][k][j] Sy[j][ky];  Y[p][k][kx] = sum_i ay[p][k][i] Sx[i][kx]
+    VPMX_TRY(dgemm_batched(c, false, my, np * mz, my, Sy, my, 0, ax, my, 0, X, my, 0, 1));
+    VPMX_TRY(dgemm_batched(c, false, mx, np * mz, mx, Sx, mx, 0, ay, mx, 0, Y, mx, 0, 1));
+    // T[p][i][ky] = sum_j az[p][i][j] Sy[j][ky];  Z[p][ky][kx] = sum_i T[p][i][ky] Sx[i][kx]
+    VPMX_TRY(dgemm_batched(c, false, my, np * mx, my, Sy, my, 0, az, my, 0, T, my, 0, 1));
+    VPMX_TRY(dgemm_batched(c, true, mx, my, mx, Sx, mx, 0, T, my, fz, Z, mx, fz, np));
+    // one thread per kx: rows of up to 320 (the one-wave kernels' limit is below that)
+    if (a.px > 320) return fail(c, VPMX_ERR_UNSUP, "Poisson: forward-pass reuse needs rows of at most 320 points");
+    const dim3 g((unsigned)((my + FC_SPAN - 1) / FC_SPAN), (unsigned)(mz * ncomp));
+    const dim3 blk((unsigned)((a.px + 63) / 64 * 64));
+    if (f32)
+        hipLaunchKernelGGL(k_fold_correct<float>, g, blk, 0, c->stream, (const float *)c->dst_save.p, (float *)a.w, a.px,
+                           mx, my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
+    else
+        hipLaunchKernelGGL(k_fold_correct<double>, g, blk, 0, c->stream, (const double *)c->dst_save.p, a.w, a.px, mx,
+                           my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out,
+                      int reuse) {
     int st = VPMX_OK;
     int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
     if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(c, VPMX_ERR_ARG, "Poisson window needs >= 3 nodes per axis");
@@ -1383,12 +1534,26 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)a.wcs * ncomp, &st);
     if (st) return st;
     a.w = w;
-    // x forward (with boundary folding)
-    a.nlines = (int64_t)m[1] * m[2];
-    VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0], f32));
-    // y forward
-    a.nlines = (int64_t)a.px * m[2];
-    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
+    const int key[6] = {m[0], m[1], m[2], a.px, ncomp, f32 ? 1 : 0};
+    const size_t wbytes = (f32 ? sizeof(float) : sizeof(double)) * (size_t)a.wcs * ncomp;
+    if (reuse == 2 && bc && std::equal(key, key + 6, c->save_key)) {
+        // the kept x/y forward transforms of the RHS, less those of the folded Dirichlet terms
+        VPMX_TRY(fold_correct(c, a, ncomp, f32, *bc));
+    } else {
+        // x forward (with boundary folding)
+        a.nlines = (int64_t)m[1] * m[2];
+        VPMX_TRY(run_pass<DST_XF>(c, a, ncomp, ax[0], f32));
+        // y forward
+        a.nlines = (int64_t)a.px * m[2];
+        VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
+        c->save_key[5] = -1;
+        if (reuse == 1 && !bc && a.px <= 320) {  // (the correction pass takes a row per work-group)
+            void *sv = ensure(c, c->dst_save, wbytes, &st);
+            if (st) return st;
+            VPMX_HIP(c, hipMemcpyAsync(sv, w, wbytes, hipMemcpyDeviceToDevice, c->stream));
+            std::copy(key, key + 6, c->save_key);
+        }
+    }
     // z forward + diagonal + inverse
     a.nlines = (int64_t)a.px * m[1];
     VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));

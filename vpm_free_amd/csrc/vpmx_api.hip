@@ -107,7 +107,7 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     int32_t win[6] = {1, 1, 1, n[0], n[1], n[2]};
     double h[3];
     window_h(g.x0, g.h, win, h);
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v));
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, 1));  // keeps its forward transforms
     int64_t nbmax = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
     double *sig = (double *)ensure(c, c->src_sig, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
     double *cxyz = (double *)ensure(c, c->src_cxyz, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
@@ -116,7 +116,7 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
     // SOL_0 is dead once the sources exist: scratch of the panel sums
     VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax, ibctyp, &s0v));
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol));
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2));  // same RHS: reuses them
     return VPMX_OK;
 }
 

@@ -61,6 +61,9 @@ struct Ctx : vpmx_ctx {
     const void *cmap_dev = nullptr;  // device address they were uploaded to
     DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
     DBuf dst_work, lam[3];
+    DBuf dst_save, fold_tab;  // forward-pass reuse: the kept work array, the face transforms
+    int save_key[6] = {0, 0, 0, 0, 0, -1};  // m[3], px, ncomp, f32 of the kept array (f32 -1: none)
+    std::map<int, void *> sinm_cache;  // sin(pi (t+1)(k+1) / (m+1)), t, k < m, per m
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
     DBuf tile_flag, tile_pos, tile_list;  // interpolation: cell tiles holding particles
     // projection without zeros for empty tiles: tile_flag maps the node tiles of a view of

@@ -87,8 +87,12 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
 // extent described by wn).  rhs: RHS view; bc: view holding Dirichlet values on the
 // window faces (nullptr for zero BCs); out: view receiving the solution (interior; the
 // faces are written with the BC values or zero).  h: node spacings.
+// reuse (two solves of one window with the same RHS, the second with Dirichlet values):
+// 1 keeps the work array after the x and y forward passes, 2 starts from the kept array with
+// the transform of the folded Dirichlet terms taken off instead of running those passes
+// (falls back to the full solve when the kept array does not match the window).
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
-                      GView out);
+                      GView out, int reuse = 0);
 
 // Complex FFT plan of length L (Stockham radices, device twiddle table exp(-2 pi i t / L)),
 // cached per context.
@@ -118,6 +122,9 @@ int boundary_sums_panels(Ctx *c, const Grid &g, int64_t nb, const double *sig, c
 // slot: 0 the fine block, 1 the coarse grid (each keeps its own plan across calls).
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol, int slot);
 void free_bspec(Ctx *c);
+// Column-major C_b = A_b op(B_b) for b < batch (FP64, rocBLAS on the context's stream).
+int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
+                  int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch);
 // Tree form (k_tree.hip): sig (3 x nbound), cxyz (3 x nbound), ds (nbound) of normal_derivs.
 // panels: tree_calc_3d_s (ibctyp = 3) instead of tree_calc_3d.
 int boundary_sums_tree(Ctx *c, const Grid &g, int64_t nbound, const double *sig, const double *cxyz, const double *ds,
