@@ -39,6 +39,7 @@ struct DstArgs {
     int px;             // row pitch of the work array (elements, >= mx)
     int64_t wcs;        // component stride of the work array
     double *w;          // work array (mx,my,mz) per component
+    double *wo;         // where the one-wave kernels store their results (normally w)
     const double2 *tw;  // exp(-2 pi i t / L), t < L
     // fold / store
     GView rhs, bc, out;
@@ -894,7 +895,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     int comp = gb / nbatch, b = gb - comp * nbatch;
     if (gb < nbt) wave_prefetch<MODE, T>(a, (T *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
     for (; gb < nbt; ++it) {
-        T *w = (T *)a.w + comp * a.wcs;
+        T *w = (T *)a.wo + comp * a.wcs;
 #pragma unroll
         for (int u = 0; u < DST_PF; u++)
             if (tpt0 + u * WV_PS < m) {
@@ -1405,7 +1406,7 @@ __global__ void k_fold_gather(GView bc, int mx, int my, int mz, double *ax, doub
 // are uniform (scalar loads), and FC_UNROLL row loads are in flight per thread.
 constexpr int FC_SPAN = 48, FC_UNROLL = 8;
 template <class T>
-__global__ void __launch_bounds__(320) k_fold_correct(const T *__restrict__ w1, T *__restrict__ w2, int px, int mx,
+__global__ void __launch_bounds__(320) k_fold_correct(const T *w1, T *w2, int px, int mx,
                                                       int my, int mz, const double *__restrict__ X,
                                                       const double *__restrict__ Y, const double *__restrict__ Z,
                                                       double ihx, double ihy, double ihz, const double *__restrict__ Sx,
@@ -1452,7 +1453,7 @@ __global__ void __launch_bounds__(320) k_fold_correct(const T *__restrict__ w1, 
     }
 }
 
-// W (the work array) = kept array - C from the Dirichlet values in bc.  The face transforms
+// W (the work array, holding the kept transforms) -= C from the Dirichlet values in bc.  The face transforms
 // are products with the DST-I matrices (row-major out[a][k] = sum_t in[a][t] S[t][k], i.e.
 // column-major out^T = S^T in^T).
 static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GView &bc) {
@@ -1482,10 +1483,10 @@ static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GVi
     const dim3 g((unsigned)((my + FC_SPAN - 1) / FC_SPAN), (unsigned)(mz * ncomp));
     const dim3 blk((unsigned)((a.px + 63) / 64 * 64));
     if (f32)
-        hipLaunchKernelGGL(k_fold_correct<float>, g, blk, 0, c->stream, (const float *)c->dst_save.p, (float *)a.w, a.px,
+        hipLaunchKernelGGL(k_fold_correct<float>, g, blk, 0, c->stream, (const float *)a.w, (float *)a.w, a.px,
                            mx, my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
     else
-        hipLaunchKernelGGL(k_fold_correct<double>, g, blk, 0, c->stream, (const double *)c->dst_save.p, a.w, a.px, mx,
+        hipLaunchKernelGGL(k_fold_correct<double>, g, blk, 0, c->stream, (const double *)a.w, a.w, a.px, mx,
                            my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
@@ -1533,12 +1534,14 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     a.wcs = (int64_t)a.px * m[1] * m[2];
     w = (double *)ensure(c, c->dst_work, sizeof(double) * (size_t)a.wcs * ncomp, &st);
     if (st) return st;
-    a.w = w;
+    a.w = a.wo = w;
     const int key[6] = {m[0], m[1], m[2], a.px, ncomp, f32 ? 1 : 0};
     const size_t wbytes = (f32 ? sizeof(float) : sizeof(double)) * (size_t)a.wcs * ncomp;
     if (reuse == 2 && bc && std::equal(key, key + 6, c->save_key)) {
-        // the kept x/y forward transforms of the RHS, less those of the folded Dirichlet terms
+        // the kept x/y forward transforms of the RHS (still in the work array), less those of
+        // the folded Dirichlet terms
         VPMX_TRY(fold_correct(c, a, ncomp, f32, *bc));
+        c->save_key[5] = -1;
     } else {
         // x forward (with boundary folding)
         a.nlines = (int64_t)m[1] * m[2];
@@ -1547,16 +1550,19 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
         a.nlines = (int64_t)a.px * m[2];
         VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
         c->save_key[5] = -1;
-        if (reuse == 1 && !bc && a.px <= 320) {  // (the correction pass takes a row per work-group)
-            void *sv = ensure(c, c->dst_save, wbytes, &st);
+        // keep them: the rest of this solve runs in a second array (one-wave kernels only; the
+        // correction pass takes a row per work-group)
+        if (reuse == 1 && !bc && wave_all && a.px <= 320) {
+            double *w2 = (double *)ensure(c, c->dst_save, wbytes, &st);
             if (st) return st;
-            VPMX_HIP(c, hipMemcpyAsync(sv, w, wbytes, hipMemcpyDeviceToDevice, c->stream));
+            a.wo = w2;
             std::copy(key, key + 6, c->save_key);
         }
     }
-    // z forward + diagonal + inverse
+    // z forward + diagonal + inverse (into wo), then the inverse passes in place there
     a.nlines = (int64_t)a.px * m[1];
     VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
+    a.w = a.wo;
     // y inverse
     a.nlines = (int64_t)a.px * m[2];
     VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
