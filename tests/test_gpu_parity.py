@@ -466,3 +466,24 @@ def test_boundary_sums_spectral_long_axis(vpmlib, monkeypatch):
         with V.VPM(V.Params.make(0.125)) as c:
             out[mode] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(out["spectral"], out["direct"]) <= TOL
+
+
+@pytest.mark.parametrize("prec,tol", [(0, TOL), (1, 1e-6)])
+def test_pmesh_inf_reuse_full_size(vpmlib, monkeypatch, prec, tol):
+    """The second fine solve of pmesh_inf starts from the first one's x/y forward transforms
+    less those of the folded Dirichlet values; at the 289-node block (compile-time plans, the
+    spectral boundary sums' 576-point path) it must agree with the two full solves."""
+    V = vpmlib
+    nn = np.array([289, 289, 289], np.int32)
+    dpm = np.array([1 / 64] * 3)
+    xb = np.array([-2.25, -2.25, -2.25, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.1, -0.05, 0.02), s=0.3)
+    sol = np.zeros_like(rhs)
+    say("case: pmesh_inf forward-pass reuse, 289^3, precision", prec)
+    out = {}
+    for reuse in ("1", "0"):
+        monkeypatch.setenv("VPMX_DST_REUSE", reuse)
+        with V.VPM(V.Params.make(0.125)) as c:
+            c.set_precision(prec)
+            out[reuse] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
+    assert rel_l2(out["1"], out["0"]) <= tol

@@ -107,7 +107,8 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     int32_t win[6] = {1, 1, 1, n[0], n[1], n[2]};
     double h[3];
     window_h(g.x0, g.h, win, h);
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, 1));  // keeps its forward transforms
+    const int ru = c->dst_reuse ? 1 : 0;
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, ru));  // keeps its forward transforms
     int64_t nbmax = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
     double *sig = (double *)ensure(c, c->src_sig, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
     double *cxyz = (double *)ensure(c, c->src_cxyz, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
@@ -116,7 +117,7 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
     // SOL_0 is dead once the sources exist: scratch of the panel sums
     VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax, ibctyp, &s0v));
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2));  // same RHS: reuses them
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2 * ru));  // same RHS: reuses them
     return VPMX_OK;
 }
 
@@ -618,6 +619,8 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
     if (dm && std::string(dm) == "v2") c->dst_kernel = 2;
     if (dm && std::string(dm) == "v3") c->dst_kernel = 3;
+    const char *dr = getenv("VPMX_DST_REUSE");
+    if (dr && std::string(dr) == "0") c->dst_reuse = false;
 
     *out = c;
     return VPMX_OK;

@@ -52,6 +52,7 @@ struct Ctx : vpmx_ctx {
     void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
     int num_cu = 256;       // compute units of the device
     int dst_wg_per_cu = 0;  // resident work-groups per CU of the persistent DST passes (0: occupancy query)
+    bool dst_reuse = true;  // second fine solve from the first one's forward transforms (VPMX_DST_REUSE=0: off)
     int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length
                           // work-group FFTs, 2 odd extension, 1 two-buffer
     std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N
