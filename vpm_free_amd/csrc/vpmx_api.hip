@@ -636,7 +636,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
-                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt};
+                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab};
     for (DBuf *b : bufs) release(*b);
     free_bspec(c);
     free_tree(c);
