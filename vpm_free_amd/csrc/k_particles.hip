@@ -6,6 +6,7 @@
 // kept in particle order), and every node sums the particles of the 4x4x4 cells whose
 // stencil covers it, so no floating-point atomics are used and the result does not
 // depend on scheduling.
+#include <algorithm>
 #include <hip/hip_ext.h>
 
 #include "vpmx_internal.h"
@@ -581,6 +582,28 @@ __global__ void k_p2m_zero_tiles(int ntx, int nty, int vnx, int vny, int vnz, co
     for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = 0.0;
 }
 
+// The same for the tiles that held particles in the previous projection and hold none now
+// (the other empty tiles are still zero): a work-group walks ZT_TILES tiles, one thread per
+// node of a tile (PT_X x PT_Y x PT_Z = 256), and skips the tiles with nothing to clear.
+constexpr int ZT_TILES = 16;
+static_assert(PT_X * PT_Y * PT_Z == 256, "one thread per node of a tile");
+__global__ void __launch_bounds__(256) k_p2m_zero_changed(int ntiles, int ntx, int nty, int vnx, int vny, int vnz,
+                                                          const int *__restrict__ flag, const int *__restrict__ prev,
+                                                          GView out) {
+    const int xa = threadIdx.x % PT_X, ya = (threadIdx.x / PT_X) % PT_Y, za = threadIdx.x / (PT_X * PT_Y);
+    for (int u = 0; u < ZT_TILES; u++) {
+        const int tile = blockIdx.x * ZT_TILES + u;
+        if (tile >= ntiles) return;
+        if (flag[tile] || !prev[tile]) continue;
+        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+        const int va = tx * PT_X + xa, vb = ty * PT_Y + ya, vc = tz * PT_Z + za;
+        if (va >= vnx || vb >= vny || vc >= vnz) continue;
+        const int64_t o = out.at(va, vb, vc);
+#pragma unroll
+        for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = 0.0;
+    }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict__ start,
                                                   const int *__restrict__ rowcnt, int cnx, int cny,
@@ -726,8 +749,10 @@ __global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const 
 
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride, int ncomp,
                int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32,
-               bool zero_empty) {
+               bool zero_empty, bool keep_zeros) {
     c->tile_map = false;
+    if ((!keep_zeros && out.p == c->zt_ptr) || (keep_zeros && !(ncomp == 3 && itype == 4 && c->p2m_block && c->p2m_tile)))
+        c->zt_valid = false;  // another writer of that buffer, or a path that writes every node
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
@@ -760,6 +785,7 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
             if (!ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles0, &st)) return st;
             if (!ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles0 + 1), &st)) return st;
             if (!ensure(c, c->tile_list, sizeof(int) * (size_t)tiles0, &st)) return st;
+            if (keep_zeros && !ensure(c, c->zt_prev, sizeof(int) * (size_t)tiles0, &st)) return st;
             const bool timed_all = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
             if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
             if (n > 0) {
@@ -784,8 +810,25 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
             VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
             hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
             if (zero_empty) {
-                hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx,
-                                   nty, vn[0], vn[1], vn[2], flag, out);
+                // buffers are all in place here, so alloc_gen is final for this call
+                const bool same = keep_zeros && c->zt_valid && out.p == c->zt_ptr && c->zt_gen == c->alloc_gen &&
+                                  std::equal(vn, vn + 3, c->zt_vn) && std::equal(wlo, wlo + 3, c->zt_wlo);
+                if (same)
+                    hipLaunchKernelGGL(k_p2m_zero_changed, dim3((unsigned)((tiles + ZT_TILES - 1) / ZT_TILES)), dim3(256),
+                                       0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1], vn[2], flag,
+                                       (const int *)c->zt_prev.p, out);
+                else
+                    hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx,
+                                       nty, vn[0], vn[1], vn[2], flag, out);
+                if (keep_zeros) {
+                    VPMX_HIP(c, hipMemcpyAsync(c->zt_prev.p, flag, sizeof(int) * (size_t)tiles, hipMemcpyDeviceToDevice,
+                                               c->stream));
+                    c->zt_valid = true;
+                    c->zt_ptr = out.p;
+                    std::copy(vn, vn + 3, c->zt_vn);
+                    std::copy(wlo, wlo + 3, c->zt_wlo);
+                    c->zt_gen = c->alloc_gen;
+                }
             } else {
                 c->tile_map = true;
                 for (int d = 0; d < 3; d++) c->tile_map_vn[d] = vn[d];

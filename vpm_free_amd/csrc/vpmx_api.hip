@@ -36,6 +36,7 @@ void *ensure(Ctx *c, DBuf &b, size_t bytes, int *st) {
         b.p = nullptr;
         b.bytes = 0;
     }
+    c->alloc_gen++;
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         *st = fail(c, VPMX_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
@@ -426,13 +427,14 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
         // project_particles (vpm.f90:408-444) straight into the block layout of rhsscat_3d
         int wlo[3] = {G.nnbl_bl[0] - G.nn_bl[0], G.nnbl_bl[1] - G.nn_bl[1], G.nnbl_bl[2] - G.nn_bl[2]};
         VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_bl, nb), nb, wlo,
-                            K_P2M, c->precision == 1));
+                            K_P2M, c->precision == 1, true, true));
     } else {
         // the global grid, then rhsscat_3d into every block (vpm_mpi.f90:199-238)
         double *rhs_g = (double *)ensure(c, c->rhs_g, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
         const int wlo[3] = {0, 0, 0};
         VPMX_TRY(p2m_gather(c, gg, c->sort, xp, qp, c->neqpm + 1, 3, P.interf_iproj, DV, view(rhs_g, gn), gn, wlo,
                             K_P2M, c->precision == 1));
+        c->zt_valid = false;  // the blocks' RHS come from the global grid
         for (int b = 0; b < G.nblocks; b++) {
             int g0[3];
             block_origin(G, b, g0);
@@ -636,7 +638,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
-                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab};
+                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev};
     for (DBuf *b : bufs) release(*b);
     free_bspec(c);
     free_tree(c);

@@ -74,6 +74,15 @@ struct Ctx : vpmx_ctx {
     DBuf stage_a, stage_b, stage_c;
     SortState sort;
     int64_t sort_n = 0;     // particles of the last sort
+    int64_t alloc_gen = 0;  // counts (re)allocations by ensure()
+    // projection into the block RHS with the zeros kept: the empty tiles of the last such
+    // projection into zt_ptr (view zt_vn, offsets zt_wlo) are still zero, so the next one
+    // zeroes only the tiles that were occupied then (zt_prev) and are empty now
+    bool zt_valid = false;
+    const void *zt_ptr = nullptr;
+    int zt_vn[3] = {0, 0, 0}, zt_wlo[3] = {0, 0, 0};
+    int64_t zt_gen = -1;
+    DBuf zt_prev;
     bool p2m_block = true;  // projection: node-run / node-block kernels (else one thread per node)
     bool p2m_rows = true;   // projection: 4 x-nodes per thread, cell-row walks (else 4x2x2 node blocks)
     bool p2m_tile = true;   // projection: node tiles with the particles staged in LDS (else the above)

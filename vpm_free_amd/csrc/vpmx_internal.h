@@ -68,9 +68,11 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
 // f32: weights and sums in FP32 (mixed form), the grid stays FP64.
 // zero_empty false: the node-tile path leaves the nodes of tiles that no particle reaches
 // unwritten and keeps the tile map (Ctx::tile_map) for a caller that reads only the rest.
+// keep_zeros: nothing but this call writes `out` between such calls (the block RHS), so the
+// tiles left empty by the last one are still zero and only newly emptied tiles are cleared.
 int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
                int ncomp, int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass = -1,
-               bool f32 = false, bool zero_empty = true);
+               bool f32 = false, bool zero_empty = true, bool keep_zeros = false);
 // itypeb 1: UP = sum W vel, GP = (sum W sol) QP(4); itypeb 2 (diffusion): UP = 0, GP as above
 int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, const double *qp, int qstride,
                int itype, GView vel, GView sol, double *up, double *gp, int64_t n, int itypeb = 1);
