@@ -823,12 +823,13 @@ __host__ __device__ constexpr int fix_len() {
     else return 0;
 }
 
-// Padding of the wave kernels' LDS sequences (PadBuf shift, 0: none).  Measured on the 289^3
-// block: the y passes with compile-time plans gain 10-16% (their Stockham writes conflicted
-// most); the x passes, the FP32 z pass and the runtime plans lose more to the extra
-// registers and address arithmetic than the conflicts cost them.
+// Padding of the wave kernels' LDS sequences (PadBuf shift, 0: none).  Measured: the y passes
+// with compile-time plans gain 10-16% (their Stockham writes conflicted most), the z passes
+// of the 289^3 block 5% (FP32, 576 = 4.4.4.9); the x passes, the 544-point FP32 z pass (-47%)
+// and the runtime plans lose more to the extra registers and address arithmetic than the
+// conflicts cost them.
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
-    return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && dbl)) ? (dbl ? 3 : 4) : 0;
+    return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0))) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
 // the extra registers cost no occupancy).
@@ -1024,6 +1025,7 @@ k_dst_wave_cap(DstArgs a, WavePlan P) {
 __host__ __device__ constexpr int wave_cap(bool dbl, int mode, int fix) {
     if (dbl) return ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI)) ? 4 : 0;
     if (fix == 0 && mode == DST_XI) return 6;
+    if (fix == 0 && mode == DST_Z) return 4;
     return (fix == 5 && mode == DST_Z) ? 4 : 0;
 }
 
