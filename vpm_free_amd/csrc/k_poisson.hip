@@ -1025,7 +1025,6 @@ k_dst_wave_cap(DstArgs a, WavePlan P) {
 __host__ __device__ constexpr int wave_cap(bool dbl, int mode, int fix) {
     if (dbl) return ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI)) ? 4 : 0;
     if (fix == 0 && mode == DST_XI) return 6;
-    if (fix == 0 && mode == DST_Z) return 4;
     return (fix == 5 && mode == DST_Z) ? 4 : 0;
 }
 
