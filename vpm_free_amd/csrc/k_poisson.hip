@@ -1402,7 +1402,7 @@ __global__ void k_fold_gather(GView bc, int mx, int my, int mz, double *ax, doub
 }
 
 // Tables per (comp, side) pair p = 2 comp + side: X[p] [k][ky] (mz x my), Y[p] [k][kx]
-// (mz x mx), Z[p] [ky][kx] (my x mx).  A work-group takes FC_SPAN rows ky of one plane
+// (mz x mx), Z[p] [ky][kx] (my x mx).  A work-group takes `span` (<= FC_SPAN) rows ky of one plane
 // (k, comp), a thread one kx: the kx-dependent factors are loaded once, the ky-dependent ones
 // are uniform (scalar loads), and FC_UNROLL row loads are in flight per thread.
 constexpr int FC_SPAN = 48, FC_UNROLL = 8;
@@ -1411,9 +1411,9 @@ __global__ void __launch_bounds__(320) k_fold_correct(const T *w1, T *w2, int px
                                                       int my, int mz, const double *__restrict__ X,
                                                       const double *__restrict__ Y, const double *__restrict__ Z,
                                                       double ihx, double ihy, double ihz, const double *__restrict__ Sx,
-                                                      const double *__restrict__ Sy) {
+                                                      const double *__restrict__ Sy, int span) {
     const int kx = threadIdx.x, k = blockIdx.y % mz, comp = blockIdx.y / mz;
-    const int ky0 = blockIdx.x * FC_SPAN, ky1 = ky0 + FC_SPAN < my ? ky0 + FC_SPAN : my;
+    const int ky0 = blockIdx.x * span, ky1 = ky0 + span < my ? ky0 + span : my;
     const double *X0 = X + (int64_t)(2 * comp) * mz * my + (int64_t)k * my, *X1 = X0 + (int64_t)mz * my;
     const double *Y0 = Y + (int64_t)(2 * comp) * mz * mx + (int64_t)k * mx, *Y1 = Y0 + (int64_t)mz * mx;
     const double *Z0 = Z + (int64_t)(2 * comp) * my * mx, *Z1 = Z0 + (int64_t)my * mx;
@@ -1421,7 +1421,7 @@ __global__ void __launch_bounds__(320) k_fold_correct(const T *w1, T *w2, int px
     // the rows' factors, staged once per work-group
     __shared__ double rf[4][FC_SPAN];
     for (int t = threadIdx.x; t < 4 * FC_SPAN; t += blockDim.x) {
-        const int q = t / FC_SPAN, y = ky0 + (t - q * FC_SPAN);
+        const int q = t / FC_SPAN, y = ky0 + (t - q * FC_SPAN);  // rows past the span are not read
         double v = 0.0;
         if (y < ky1) v = q == 0 ? X0[y] : (q == 1 ? X1[y] : (q == 2 ? Sy[y] : Sy1[y]));
         rf[q][t - q * FC_SPAN] = v;
@@ -1481,14 +1481,16 @@ static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GVi
     VPMX_TRY(dgemm_batched(c, true, mx, my, mx, Sx, mx, 0, T, my, fz, Z, mx, fz, np));
     // one thread per kx: rows of up to 320 (the one-wave kernels' limit is below that)
     if (a.px > 320) return fail(c, VPMX_ERR_UNSUP, "Poisson: forward-pass reuse needs rows of at most 320 points");
-    const dim3 g((unsigned)((my + FC_SPAN - 1) / FC_SPAN), (unsigned)(mz * ncomp));
+    // small grids (the coarse problem): short spans, so the grid still fills the device
+    const int span = my <= 96 ? 8 : FC_SPAN;
+    const dim3 g((unsigned)((my + span - 1) / span), (unsigned)(mz * ncomp));
     const dim3 blk((unsigned)((a.px + 63) / 64 * 64));
     if (f32)
         hipLaunchKernelGGL(k_fold_correct<float>, g, blk, 0, c->stream, (const float *)a.w, (float *)a.w, a.px,
-                           mx, my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
+                           mx, my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy, span);
     else
         hipLaunchKernelGGL(k_fold_correct<double>, g, blk, 0, c->stream, (const double *)a.w, a.w, a.px, mx,
-                           my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy);
+                           my, mz, X, Y, Z, a.ih2[0], a.ih2[1], a.ih2[2], Sx, Sy, span);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
