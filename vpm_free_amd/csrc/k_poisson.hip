@@ -824,12 +824,13 @@ __host__ __device__ constexpr int fix_len() {
 }
 
 // Padding of the wave kernels' LDS sequences (PadBuf shift, 0: none).  Measured: the y passes
-// with compile-time plans gain 10-16% (their Stockham writes conflicted most), the z passes
-// of the 289^3 block 5% (FP32, 576 = 4.4.4.9); the x passes, the 544-point FP32 z pass (-47%)
-// and the runtime plans lose more to the extra registers and address arithmetic than the
-// conflicts cost them.
+// with compile-time plans gain 10-16% (their Stockham writes conflicted most), the FP32 z and
+// x forward passes of the 289^3 block 5-8% (576 = 4.4.4.9); the FP64 x passes, the FP32 x
+// inverse (register-capped: it spills, -66%), the 544-point FP32 z pass (-47%) and the
+// runtime plans lose more to the extra registers and address arithmetic than the conflicts
+// cost them.
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
-    return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0))) ? (dbl ? 3 : 4) : 0;
+    return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
 // the extra registers cost no occupancy).
