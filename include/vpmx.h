@@ -78,16 +78,32 @@ int32_t vpmx_read_pm_inp(const char *path, vpmx_params *out);
  * recomputed only when ntime==0, or every call when idefine==0).  No device needed. */
 int32_t vpmx_define_sizes(const vpmx_params *p, vpmx_geom *g, const double *xp, int64_t nvr, int32_t ntime);
 
+/* define_sizes with yaps blocks of uneven extents (no counterpart in the reference, whose
+ * split is even, NXbl = NXB/NBI, vpm.f90:556-596): along each axis the block extents are in
+ * proportion to the weights wx[NBI], wy[NBJ], wz[NBK] (positive), in whole coarse cells
+ * (ncoarse fine cells each, at least one per block); equal weights give the reference's even
+ * split bit for bit, and null weight arrays mean equal weights.  The box, the coarse grid and
+ * every other rule are those of vpmx_define_sizes. */
+int32_t vpmx_define_sizes_weighted(const vpmx_params *p, const int32_t *wx, const int32_t *wy, const int32_t *wz,
+                                   vpmx_geom *g, const double *xp, int64_t nvr, int32_t ntime);
+
 /* ------------------------------------------------------------------ context */
 int32_t vpmx_create(vpmx_ctx **out, int32_t device);
 void vpmx_destroy(vpmx_ctx *ctx);
 const char *vpmx_last_error(const vpmx_ctx *ctx);
 int32_t vpmx_set_params(vpmx_ctx *ctx, const vpmx_params *p);
+/* Uneven yaps blocks for the context's later define_sizes (vpm mode 0): weights as in
+ * vpmx_define_sizes_weighted, NBI/NBJ/NBK entries of the current parameters; null arrays
+ * restore the even split.  No counterpart in the reference (its split is even). */
+int32_t vpmx_set_block_weights(vpmx_ctx *ctx, const int32_t *wx, const int32_t *wy, const int32_t *wz);
 int32_t vpmx_get_geometry(const vpmx_ctx *ctx, vpmx_geom *out);
 /* Arithmetic form (no reference counterpart: the reference is FP64 throughout).
- * 0 = plain FP64 form (default); 1 = mixed form: the Poisson transform passes in FP32, the
- * Green's-function boundary sums, projection, interpolation, remesh and diagnostics in FP64.
- * VPMX_PRECISION=mixed in the environment selects 1 at vpmx_create. */
+ * 0 = plain FP64 form (default).  1 = mixed form: in FP32 the projection's weights and sums,
+ * the sine-transform passes of the fine-block and coarse infinite-domain solves, the forward
+ * passes and diagonal of the inner-box solve, and the interpolation's node values and sums; in
+ * FP64 the inner-box solve's inverse passes, the Green's-function boundary sums, the normal
+ * derivatives, curl and stretching, remesh and diagnostics, and every particle array and grid
+ * in HBM.  VPMX_PRECISION=mixed in the environment selects 1 at vpmx_create. */
 int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
 
 /* vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...) (vpm.f90:54-475) on host arrays.

@@ -94,6 +94,8 @@ def lib():
             "orc_set_boundsum_stride": (None, [C.c_int]),
             "orc_diffuse": (None, [C.POINTER(Geom), C.c_int, C.c_double, D, D]),
             "orc_set_tree": (None, [C.c_int]),
+            "orc_set_yaps_plant": (None, [C.c_int]),
+            "orc_set_block_weights": (None, [I32P, I32P, I32P, C.c_int, C.c_int, C.c_int]),
             "orc_set_tree_factor": (None, [C.c_double]),
             "orc_boundsum_tree": (None, [D, D, I32P, C.c_int, i64, D, D, D, D, C.c_int]),
             "orc_vpm45": (i32, [C.POINTER(Params), C.POINTER(Geom), C.c_int, C.c_int, C.c_double, D, D, D, D, i64,
@@ -239,6 +241,23 @@ def vpm2(params, geom, xp, qp, neqpm=3, fields=False):
     if fields:
         return up, gp, {"rhs": rhs, "psi": psi, "sol": sol, "vel": vel}
     return up, gp
+
+
+def set_yaps_plant(mode):
+    """Plant a mistake in the yaps composition (tests of the tests): 0 none, 1 no TSC
+    correction, 2 its sign flipped, 3 no neighbour fine values, 4 nnb = 0 everywhere, 5 the
+    coarse Laplacian over the footprint boundary too."""
+    lib().orc_set_yaps_plant(int(mode))
+
+
+def set_block_weights(wx=None, wy=None, wz=None):
+    """Uneven yaps blocks for later define_sizes calls: extents in proportion to the weights
+    (None: the reference's even split)."""
+    if wx is None:
+        lib().orc_set_block_weights(None, None, None, 0, 0, 0)
+        return
+    a = [np.ascontiguousarray(w, dtype=np.int32) for w in (wx, wy, wz)]
+    lib().orc_set_block_weights(_i(a[0]), _i(a[1]), _i(a[2]), a[0].size, a[1].size, a[2].size)
 
 
 def set_tree(levmax):

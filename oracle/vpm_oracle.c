@@ -188,6 +188,27 @@ void orc_definepm(int itype, double xb[6], double dpm[3], int ndum, const int32_
 
 /* ---------------------------------------------------------------- define_sizes */
 /* vpm.f90:477-660 with BLOCKS = NBI*NBJ*NBK (the reference sets BLOCKS = MPI ranks). */
+/* Uneven yaps blocks (an extension of the build; the reference's split is even): block
+ * extents along each axis in proportion to these weights, in whole coarse cells,
+ * cut_i = ncoarse * floor(units * (w_1 + .. + w_i) / W), units = cells / ncoarse.  No weights
+ * (the default) is the reference's even split. */
+static int g_bw_on = 0, g_bw[3][ORC_MAXB];
+void orc_set_block_weights(const int *wx, const int *wy, const int *wz, int nbi, int nbj, int nbk) {
+    g_bw_on = wx != NULL;
+    if (!g_bw_on) return;
+    for (int i = 0; i < nbi; i++) g_bw[0][i] = wx[i];
+    for (int i = 0; i < nbj; i++) g_bw[1][i] = wy[i];
+    for (int i = 0; i < nbk; i++) g_bw[2][i] = wz[i];
+}
+static void block_cuts(int cells, int nb, int ncoarse, const int *w, int *cut) {
+    long long units = cells / ncoarse, W = 0, acc = 0;
+    for (int i = 0; i < nb; i++) W += w ? w[i] : 1;
+    for (int i = 0; i <= nb; i++) {
+        cut[i] = (int)(ncoarse * ((units * acc) / W));
+        if (i < nb) acc += w ? w[i] : 1;
+    }
+}
+
 int orc_define_sizes(const orc_params *p, orc_geom *g, const double *xp, int64_t nvr, int ntime) {
     int nb = p->nbi * p->nbj * p->nbk;
     if (nb < 1 || nb > ORC_MAXB) return -1;
@@ -220,19 +241,22 @@ int orc_define_sizes(const orc_params *p, orc_geom *g, const double *xp, int64_t
     int nxb = (int)lround((xb[3] - xb[0]) / dpm[0]);
     int nyb = (int)lround((xb[4] - xb[1]) / dpm[1]);
     int nzb = (int)lround((xb[5] - xb[2]) / dpm[2]);
-    int nxbl = nxb / p->nbi, nybl = nyb / p->nbj, nzbl = nzb / p->nbk;
+    int cx[ORC_MAXB + 1], cy[ORC_MAXB + 1], cz[ORC_MAXB + 1];
+    block_cuts(nxb, p->nbi, p->ncoarse, g_bw_on ? g_bw[0] : NULL, cx);
+    block_cuts(nyb, p->nbj, p->ncoarse, g_bw_on ? g_bw[1] : NULL, cy);
+    block_cuts(nzb, p->nbk, p->ncoarse, g_bw_on ? g_bw[2] : NULL, cz);
     int32_t nsiz_bl[3] = {p->ncoarse, p->ncoarse, p->ncoarse};
     for (int k = 1; k <= p->nbk; k++)
         for (int j = 1; j <= p->nbj; j++)
             for (int i = 1; i <= p->nbi; i++) {
                 int b = (k - 1) * p->nbj * p->nbi + (j - 1) * p->nbi + i - 1;
                 double t[6];
-                t[0] = xb[0] + (double)((i - 1) * nxbl) * dpm[0];
-                t[3] = xb[0] + (double)(i * nxbl) * dpm[0];
-                t[1] = xb[1] + (double)((j - 1) * nybl) * dpm[1];
-                t[4] = xb[1] + (double)(j * nybl) * dpm[1];
-                t[2] = xb[2] + (double)((k - 1) * nzbl) * dpm[2];
-                t[5] = xb[2] + (double)(k * nzbl) * dpm[2];
+                t[0] = xb[0] + (double)cx[i - 1] * dpm[0];
+                t[3] = xb[0] + (double)cx[i] * dpm[0];
+                t[1] = xb[1] + (double)cy[j - 1] * dpm[1];
+                t[4] = xb[1] + (double)cy[j] * dpm[1];
+                t[2] = xb[2] + (double)cz[k - 1] * dpm[2];
+                t[5] = xb[2] + (double)cz[k] * dpm[2];
                 double d2[3] = {dpm[0], dpm[1], dpm[2]};
                 orc_definepm(1, t, d2, p->ncoarse, nsiz_bl, &g->nnbl[3 * b], &g->nnbl_bl[6 * b]);
                 memcpy(&g->xbound_bl[6 * b], t, sizeof t);
@@ -243,10 +267,14 @@ int orc_define_sizes(const orc_params *p, orc_geom *g, const double *xp, int64_t
     for (int d = 0; d < 3; d++) nsc[d] = 1 << p->ilevmax;
     orc_definepm(1, g->xbound_coarse, g->dpm_coarse, 4, nsc, g->nn_coarse, g->nn_bl_coarse);
     g->ilevmax = p->ilevmax;
+    /* vpm.f90:628-645 tests block 1; with uneven blocks the smallest inner box decides */
+    int ex[3] = {1 << 30, 1 << 30, 1 << 30};
+    for (int b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++)
+            if (g->nnbl_bl[6 * b + d + 3] - g->nnbl_bl[6 * b + d] < ex[d]) ex[d] = g->nnbl_bl[6 * b + d + 3] - g->nnbl_bl[6 * b + d];
     for (int lev = 0; lev <= p->ilevmax; lev++) {
         int istep = 1 << lev;
-        if ((g->nnbl_bl[3] - g->nnbl_bl[0]) / istep == 0 || (g->nnbl_bl[4] - g->nnbl_bl[1]) / istep == 0 ||
-            (g->nnbl_bl[5] - g->nnbl_bl[2]) / istep == 0) {
+        if (ex[0] / istep == 0 || ex[1] / istep == 0 || ex[2] / istep == 0) {
             g->ilevmax = lev - 1;
             break;
         }
@@ -1033,6 +1061,13 @@ void orc_pmesh_inf(const double xb[6], const double dpm[3], const int32_t nn[3],
  *      :457-1912: a neighbour sends every node of the receiver's inner faces that lies in
  *      its own block grid; nodes it does not reach stay 0);
  *   6. per block, the Dirichlet solve of its inner box (:397-418). */
+/* Planted mistakes for the tests that pin this composition (0: none): 1 drops the TSC
+ * correction term, 2 flips its sign, 3 drops the neighbours' fine values (BBound), 4 takes
+ * every block as covering the stencil (nnb = 0), 5 samples the coarse Laplacian one coarse
+ * node too far (footprint boundary included). */
+static int g_yaps_plant = 0;
+void orc_set_yaps_plant(int mode) { g_yaps_plant = mode; }
+
 int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs_bl, double *sol_bl) {
     const int NBL = g->nblocks;
     if (NBL < 1 || NBL > ORC_MAXB || g->nbi * g->nbj * g->nbk != NBL) return -1;
@@ -1087,9 +1122,10 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
         const double *sb = samp + GC * nc * b;
         const int *cm = &cmap[6 * b];
 #define SC(c, i, j, k) sb[IX(c, (i)-1, (j)-1, (k)-1, nc, ncx, ncy)]
-        for (int k = cm[2] + 1; k <= cm[5] - 1; k++)
-            for (int j = cm[1] + 1; j <= cm[4] - 1; j++)
-                for (int i = cm[0] + 1; i <= cm[3] - 1; i++)
+        const int e = g_yaps_plant == 5 ? 0 : 1;
+        for (int k = cm[2] + e; k <= cm[5] - e; k++)
+            for (int j = cm[1] + e; j <= cm[4] - e; j++)
+                for (int i = cm[0] + e; i <= cm[3] - e; i++)
                     for (int c = 0; c < nc; c++) {
                         double v = ((SC(c, i + 1, j, k) - 2 * SC(c, i, j, k)) + SC(c, i - 1, j, k)) / (dc[0] * dc[0]) +
                                    ((SC(c, i, j + 1, k) - 2 * SC(c, i, j, k)) + SC(c, i, j - 1, k)) / (dc[1] * dc[1]) +
@@ -1143,6 +1179,7 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
                 const unsigned char *mb = map + GC * bc;
                 nnb[bc] = (!mb[(size_t)(in - 2) + (size_t)ncx * ((jn - 2) + (size_t)ncy * (kn - 2))] ||
                            !mb[(size_t)(in) + (size_t)ncx * ((jn) + (size_t)ncy * (kn))]) ? 1 : 0;
+                if (g_yaps_plant == 4) nnb[bc] = 0;
             }
             size_t o = IX(0, i - 1, j - 1, k - 1, nc, nnbl[0], nnbl[1]);
             for (int kc = kn - 1; kc <= kn + 1; kc++)
@@ -1160,15 +1197,17 @@ int orc_yaps3d(const orc_params *p, const orc_geom *g, int nc, const double *rhs
                         double yc = (X1 - g->xbound_coarse[1] - (double)(jc - 1) * dc[1]) / dc[1];
                         double zc = (X2 - g->xbound_coarse[2] - (double)(kc - 1) * dc[2]) / dc[2];
                         double fw = orc_projection_fun(3, xc) * orc_projection_fun(3, yc) * orc_projection_fun(3, zc);
-                        for (int c = 0; c < nc; c++)
-                            sb[o + c] = sb[o + c] + fw * ((sol_c[oc + c] - add_sample[c]) + add[c]);
+                        const double sgn = g_yaps_plant == 2 ? -1.0 : 1.0;
+                        if (g_yaps_plant != 1)
+                            for (int c = 0; c < nc; c++)
+                                sb[o + c] = sb[o + c] + sgn * (fw * ((sol_c[oc + c] - add_sample[c]) + add[c]));
                     }
             for (int k_nb = nb_k - 1; k_nb <= nb_k + 1; k_nb++)
                 for (int j_nb = nb_j - 1; j_nb <= nb_j + 1; j_nb++)
                     for (int i_nb = nb_i - 1; i_nb <= nb_i + 1; i_nb++) {
                         if (i_nb < 1 || i_nb > NBI || j_nb < 1 || j_nb > NBJ || k_nb < 1 || k_nb > NBK) continue;
                         int bc = (k_nb - 1) * NBI * NBJ + (j_nb - 1) * NBI + i_nb - 1;
-                        if (nnb[bc] || bc == b) continue;
+                        if (nnb[bc] || bc == b || g_yaps_plant == 3) continue;
                         /* the node in block bc's grid (all blocks share the lattice) */
                         const double *xc = &g->xbound_bl[6 * bc];
                         const int32_t *nc3 = &g->nnbl[3 * bc];
@@ -1345,6 +1384,26 @@ int orc_project(const double xb[6], const double dpm[3], const int32_t nn[3], co
 /* rhsscat_3d (vpm_mpi.f90:199-238) for every block: the block's inner box takes the global
  * window ixs = (nb_i-1)(NXf-NXs) + NN_bl(1) ..., the rest of the block is 0, and a block that
  * is not first along an axis zeroes its min face on that axis. */
+/* First global node of block b's inner box: vpm_mpi.f90:199-238 has (nb_i-1)*(NXf-NXs) +
+ * NN_bl(1) for its even blocks; with uneven blocks, the inner extents of the blocks before
+ * b along each axis summed. */
+static void inner_origin(const orc_geom *g, int b, int *ixs, int *jxs, int *kxs) {
+    int nb_i = b % g->nbi + 1, nb_j = (b / g->nbi) % g->nbj + 1, nb_k = b / (g->nbi * g->nbj) + 1;
+    *ixs = g->nn_bl[0]; *jxs = g->nn_bl[1]; *kxs = g->nn_bl[2];
+    for (int i = 1; i < nb_i; i++) {
+        const int32_t *o = &g->nnbl_bl[6 * ((nb_k - 1) * g->nbi * g->nbj + (nb_j - 1) * g->nbi + i - 1)];
+        *ixs += o[3] - o[0];
+    }
+    for (int j = 1; j < nb_j; j++) {
+        const int32_t *o = &g->nnbl_bl[6 * ((nb_k - 1) * g->nbi * g->nbj + (j - 1) * g->nbi + nb_i - 1)];
+        *jxs += o[4] - o[1];
+    }
+    for (int k = 1; k < nb_k; k++) {
+        const int32_t *o = &g->nnbl_bl[6 * ((k - 1) * g->nbi * g->nbj + (nb_j - 1) * g->nbi + nb_i - 1)];
+        *kxs += o[5] - o[2];
+    }
+}
+
 static void rhsscat(const orc_geom *g, int neqpm, const double *rhs, int nr, double *rhs_bl, size_t *boff) {
     int NX = g->nn[0], NY = g->nn[1];
     boff[0] = 0;
@@ -1354,9 +1413,8 @@ static void rhsscat(const orc_geom *g, int neqpm, const double *rhs, int nr, dou
     for (int b = 0; b < g->nblocks; b++) {
         const int32_t *nnbl = &g->nnbl[3 * b], *bl = &g->nnbl_bl[6 * b];
         int nb_i = b % g->nbi + 1, nb_j = (b / g->nbi) % g->nbj + 1, nb_k = b / (g->nbi * g->nbj) + 1;
-        int ixs = (nb_i - 1) * (bl[3] - bl[0]) + g->nn_bl[0];
-        int jxs = (nb_j - 1) * (bl[4] - bl[1]) + g->nn_bl[1];
-        int kxs = (nb_k - 1) * (bl[5] - bl[2]) + g->nn_bl[2];
+        int ixs, jxs, kxs;
+        inner_origin(g, b, &ixs, &jxs, &kxs);
         double *rb = rhs_bl + boff[b];
         for (int k = bl[2]; k <= bl[5]; k++)
             for (int j = bl[1]; j <= bl[4]; j++)
@@ -1375,10 +1433,8 @@ static void solget(const orc_geom *g, int neqpm, const double *sol_bl, const siz
     int NX = g->nn[0], NY = g->nn[1];
     for (int b = 0; b < g->nblocks; b++) {
         const int32_t *nnbl = &g->nnbl[3 * b], *bl = &g->nnbl_bl[6 * b];
-        int nb_i = b % g->nbi + 1, nb_j = (b / g->nbi) % g->nbj + 1, nb_k = b / (g->nbi * g->nbj) + 1;
-        int ixs = (nb_i - 1) * (bl[3] - bl[0]) + g->nn_bl[0];
-        int jxs = (nb_j - 1) * (bl[4] - bl[1]) + g->nn_bl[1];
-        int kxs = (nb_k - 1) * (bl[5] - bl[2]) + g->nn_bl[2];
+        int ixs, jxs, kxs;
+        inner_origin(g, b, &ixs, &jxs, &kxs);
         const double *sb = sol_bl + boff[b];
         for (int k = bl[2]; k <= bl[5]; k++)
             for (int j = bl[1]; j <= bl[4]; j++)

@@ -103,6 +103,8 @@ void orc_set_boundsum_stride(int s);
 /* Boundary sums of orc_pmesh_inf by the tree (build_level_nbound_3d + tree_calc_3d) with this
  * levmax (>= 0), or direct (-1). */
 void orc_set_tree(int levmax);
+void orc_set_yaps_plant(int mode);
+void orc_set_block_weights(const int *wx, const int *wy, const int *wz, int nbi, int nbj, int nbk);
 void orc_set_tree_factor(double f);
 void orc_boundsum_tree(const double xb[6], const double dpm[3], const int32_t nn[3], int nc, int64_t nbound,
                        const double *src, const double *cxyz, const double *ds, double *sol, int levmax);

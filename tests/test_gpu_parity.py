@@ -80,7 +80,11 @@ def test_pmesh_inf(ctx, oracle):
     assert rel_l2(got, ref) <= TOL
 
 
-def test_project(ctx, oracle):
+@pytest.mark.parametrize("itype", [4, 2, 3, 5, 6])
+def test_project(ctx, oracle, itype):
+    """project_particles_3D + project_vol3d with every projection function of
+    projection_fun (pmproject.f90:391-431): itype 4 takes the node-tile kernel, the others
+    the one-thread-per-node gather."""
     rng = np.random.default_rng(3)
     nn = np.array([24, 20, 22], np.int32)
     dpm = np.array([0.1, 0.1, 0.1])
@@ -89,9 +93,10 @@ def test_project(ctx, oracle):
     xp = np.stack([rng.uniform(-0.8, 1.2, n), rng.uniform(-0.8, 0.8, n), rng.uniform(-0.8, 1.0, n)], axis=1)
     qp = np.concatenate([rng.standard_normal((n, 3)), np.full((n, 1), 1e-3)], axis=1)
     nnb = np.array([1, 1, 1, nn[0], nn[1], nn[2]], np.int32)
-    say("case: project", n)
-    ref = oracle.project(xb, dpm, nn, nnb, xp, qp)
-    got = ctx.stage_project(xb, dpm, nn, nnb, xp, qp)
+    say("case: project", n, "itype", itype)
+    ref = oracle.project(xb, dpm, nn, nnb, xp, qp, itype=itype)
+    got = ctx.stage_project(xb, dpm, nn, nnb, xp, qp, itype=itype)
+    assert np.linalg.norm(ref) > 0
     assert rel_l2(got, ref) <= TOL
 
 
@@ -487,3 +492,47 @@ def test_pmesh_inf_reuse_full_size(vpmlib, monkeypatch, prec, tol):
             c.set_precision(prec)
             out[reuse] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(out["1"], out["0"]) <= tol
+
+
+def test_vpm_mode2_call_sequence_kept_zeros(vpmlib, oracle):
+    """One single-block context, several vpm(2) calls at the geometry of the first one
+    (idefine = 1): ring + blob; the same again (the block RHS keeps its zeros between calls
+    and only tiles emptied since the last call are cleared); the ring alone (the blob's tiles
+    empty and must be cleared); a mixed-form call; the ring + blob in FP64 again; the ring
+    alone twice.  RHS grid, UP and GP against the oracle after every call."""
+    V = vpmlib
+    h = 1 / 16
+    xr, qr = cases.single_ring(h=h, sigma=0.1)
+    rng = np.random.default_rng(21)
+    nb = 1500
+    xb = rng.uniform(-0.3, 0.3, (nb, 3)) + np.array([0.3, -0.2, 1.1])
+    qb = np.concatenate([rng.standard_normal((nb, 3)) * h ** 3, np.full((nb, 1), h ** 3)], axis=1)
+    xa, qa = np.concatenate([xr, xb]), np.concatenate([qr, qb])
+    po = oracle.Params.make(h)
+    go = oracle.define_sizes(po, xa)
+    ref = {}
+    for name, (x, q) in {"a": (xa, qa), "r": (xr, qr)}.items():
+        up_r, gp_r, f = oracle.vpm2(po, go, x, q, fields=True)
+        ref[name] = (up_r, gp_r, f["rhs"])
+    say("case: vpm(2) call sequence, ring %d + blob %d particles" % (xr.shape[0], nb))
+
+    def check(c, name, x, q, tol=TOL):
+        up, gp = np.zeros_like(x), np.zeros_like(x)
+        c.vpm(x, q, up, gp, x.shape[0], 3, 2, 1)
+        up_r, gp_r, rhs_r = ref[name]
+        errs = {"rhs": rel_l2(c.get_grid(0), rhs_r), "up": rel_l2(up, up_r), "gp": rel_l2(gp, gp_r)}
+        say("  call", name, errs)
+        for k, v in errs.items():
+            assert v <= tol, (name, k, v)
+
+    with V.VPM(V.Params.make(h)) as c:
+        c.vpm(xa, qa, None, None, xa.shape[0], 3, 0, 0)
+        check(c, "a", xa, qa)
+        check(c, "a", xa, qa)
+        check(c, "r", xr, qr)
+        c.set_precision(1)
+        check(c, "a", xa, qa, tol=1e-5)
+        c.set_precision(0)
+        check(c, "a", xa, qa)
+        check(c, "r", xr, qr)
+        check(c, "r", xr, qr)

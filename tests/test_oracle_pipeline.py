@@ -83,6 +83,52 @@ def test_define_sizes_random_clouds_bit_identical(oracle, vpmlib):
             assert g_o.xbound[d + 3] >= xp[:, d].max() + p_o.interf_iproj * h * (1 - 1e-12)
 
 
+def test_define_sizes_uneven_blocks(oracle, vpmlib):
+    """Blocks of uneven extents: along each axis the inner boxes tile the global grid, each
+    extent is a whole number of coarse cells, cut i is ncoarse * floor(units * W_<i / W), and
+    equal weights give the even split bit for bit; the library's host geometry is the
+    oracle's."""
+    rng = np.random.default_rng(5)
+    for trial in range(30):
+        h = float(rng.choice([0.1, 1 / 16, 0.05]))
+        blocks = tuple(int(v) for v in rng.integers(1, 4, 3))
+        nc = int(rng.choice([4, 8]))
+        xp = rng.normal(0.0, rng.uniform(0.5, 2, 3), (40, 3))
+        w = [rng.integers(1, 6, blocks[d]).astype(np.int32) for d in range(3)]
+        p_o = oracle.Params.make(h, blocks=blocks, ncoarse=nc)
+        p_v = vpmlib.Params.make(h, blocks=blocks, ncoarse=nc)
+        oracle.set_block_weights(*w)
+        try:
+            g_o = oracle.define_sizes(p_o, xp)
+        finally:
+            oracle.set_block_weights(None)
+        g_v = vpmlib.define_sizes(p_v, xp, weights=w)
+        assert bytes(g_o) == bytes(g_v), trial
+        nb = blocks[0] * blocks[1] * blocks[2]
+        for d in range(3):
+            units = (g_o.nn[d] - 1) // nc
+            W = int(w[d].sum())
+            stride = [1, blocks[0], blocks[0] * blocks[1]][d]
+            pos = g_o.xbound[d]
+            for i in range(blocks[d]):
+                b = i * stride
+                lo, hi = g_o.nnbl_bl[6 * b + d], g_o.nnbl_bl[6 * b + d + 3]
+                ext = hi - lo
+                cut0 = nc * ((units * int(w[d][:i].sum())) // W)
+                cut1 = nc * ((units * int(w[d][:i + 1].sum())) // W)
+                assert ext == cut1 - cut0 and ext % nc == 0 and ext >= nc, (trial, d, i)
+                # the inner box starts where the previous one ended
+                x_in = g_o.xbound_bl[6 * b + d] + (lo - 1) * h
+                assert abs(x_in - pos) < 1e-9 * max(1.0, abs(pos)), (trial, d, i)
+                pos = g_o.xbound_bl[6 * b + d] + (hi - 1) * h
+            last = g_o.xbound[d] + (g_o.nn[d] - 1) * h  # the global grid's last node
+            assert abs(pos - last) < 1e-9 * max(1.0, abs(pos))
+        assert g_o.nblocks == nb
+        # equal weights: the reference's even split
+        eq = [np.full(blocks[d], 3, np.int32) for d in range(3)]
+        assert bytes(vpmlib.define_sizes(p_v, xp, weights=eq)) == bytes(vpmlib.define_sizes(p_v, xp))
+
+
 def test_define_sizes_box_fixed_after_first_call(oracle, vpmlib):
     """idefine=1: the box is computed at NTIME=0 only (vpm.f90:500)."""
     xp = np.array([[-1.0, -1, -1], [1, 1, 1]])

@@ -7,6 +7,7 @@
 // These are FP64 operation sequences whose results (node counts, box corners) must be
 // bit-identical to the reference; they run on the host in the same order.  Fortran NINT
 // is lround (half away from zero), INT is truncation.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -82,8 +83,37 @@ extern "C" int32_t vpmx_read_pm_inp(const char *path, vpmx_params *p) {
     return ok ? VPMX_OK : VPMX_ERR_IO;
 }
 
+// Cell offsets of the block boundaries along one axis: ncells cells (a multiple of
+// nb * ncoarse, as definepm pads the box) in nb blocks; with weights the block extents are in
+// proportion to them, in whole coarse cells (cut_i = ncoarse * floor(units * W_<i / W)),
+// which for equal weights is the reference's even split (ncells / nb cells each,
+// vpm.f90:556-596).  False if a block would get no coarse cell.
+static bool block_cuts(int ncells, int nb, int ncoarse, const int32_t *w, int cut[]) {
+    const int64_t units = ncells / ncoarse;
+    int64_t W = 0;
+    for (int i = 0; i < nb; i++) {
+        const int64_t wi = w ? w[i] : 1;
+        if (wi < 1) return false;
+        W += wi;
+    }
+    int64_t acc = 0;
+    for (int i = 0; i <= nb; i++) {
+        cut[i] = (int)(ncoarse * ((units * acc) / W));
+        if (i < nb) acc += w ? w[i] : 1;
+    }
+    for (int i = 0; i < nb; i++)
+        if (cut[i + 1] - cut[i] < ncoarse) return false;
+    return true;
+}
+
 extern "C" int32_t vpmx_define_sizes(const vpmx_params *p, vpmx_geom *g, const double *xp, int64_t nvr,
                                      int32_t ntime) {
+    return vpmx_define_sizes_weighted(p, nullptr, nullptr, nullptr, g, xp, nvr, ntime);
+}
+
+extern "C" int32_t vpmx_define_sizes_weighted(const vpmx_params *p, const int32_t *wx, const int32_t *wy,
+                                              const int32_t *wz, vpmx_geom *g, const double *xp, int64_t nvr,
+                                              int32_t ntime) {
     if (!p || !g) return VPMX_ERR_ARG;
     int nb = p->nbi * p->nbj * p->nbk;
     if (p->nbi < 1 || p->nbj < 1 || p->nbk < 1 || nb > VPMX_MAXB || p->ncoarse < 1) return VPMX_ERR_ARG;
@@ -120,19 +150,24 @@ extern "C" int32_t vpmx_define_sizes(const vpmx_params *p, vpmx_geom *g, const d
     int nxb = (int)std::lround((xb[3] - xb[0]) / dpm[0]);
     int nyb = (int)std::lround((xb[4] - xb[1]) / dpm[1]);
     int nzb = (int)std::lround((xb[5] - xb[2]) / dpm[2]);
-    int nxbl = nxb / p->nbi, nybl = nyb / p->nbj, nzbl = nzb / p->nbk;
+    // block boundaries in cells: the reference's even split (nxb / NBI cells per block), or
+    // extents in proportion to the given weights
+    int cx[VPMX_MAXB + 1], cy[VPMX_MAXB + 1], cz[VPMX_MAXB + 1];
+    if (!block_cuts(nxb, p->nbi, p->ncoarse, wx, cx) || !block_cuts(nyb, p->nbj, p->ncoarse, wy, cy) ||
+        !block_cuts(nzb, p->nbk, p->ncoarse, wz, cz))
+        return VPMX_ERR_ARG;
     int32_t nsiz_bl[3] = {p->ncoarse, p->ncoarse, p->ncoarse};
     for (int k = 1; k <= p->nbk; k++)
         for (int j = 1; j <= p->nbj; j++)
             for (int i = 1; i <= p->nbi; i++) {
                 int b = (k - 1) * p->nbj * p->nbi + (j - 1) * p->nbi + (i - 1);
                 double t[6];
-                t[0] = xb[0] + (double)((i - 1) * nxbl) * dpm[0];
-                t[3] = xb[0] + (double)(i * nxbl) * dpm[0];
-                t[1] = xb[1] + (double)((j - 1) * nybl) * dpm[1];
-                t[4] = xb[1] + (double)(j * nybl) * dpm[1];
-                t[2] = xb[2] + (double)((k - 1) * nzbl) * dpm[2];
-                t[5] = xb[2] + (double)(k * nzbl) * dpm[2];
+                t[0] = xb[0] + (double)cx[i - 1] * dpm[0];
+                t[3] = xb[0] + (double)cx[i] * dpm[0];
+                t[1] = xb[1] + (double)cy[j - 1] * dpm[1];
+                t[4] = xb[1] + (double)cy[j] * dpm[1];
+                t[2] = xb[2] + (double)cz[k - 1] * dpm[2];
+                t[5] = xb[2] + (double)cz[k] * dpm[2];
                 double d2[3] = {dpm[0], dpm[1], dpm[2]};
                 definepm(1, t, d2, p->ncoarse, nsiz_bl, &g->nnbl[3 * b], &g->nnbl_bl[6 * b]);
                 std::memcpy(&g->xbound_bl[6 * b], t, sizeof t);
@@ -142,10 +177,14 @@ extern "C" int32_t vpmx_define_sizes(const vpmx_params *p, vpmx_geom *g, const d
     int32_t nsc[3] = {1 << p->ilevmax, 1 << p->ilevmax, 1 << p->ilevmax};
     definepm(1, g->xbound_coarse, g->dpm_coarse, 4, nsc, g->nn_coarse, g->nn_bl_coarse);
     g->ilevmax = p->ilevmax;
+    // (every block's inner box: with uneven blocks the smallest decides; the even blocks of
+    // the reference are all the first one's size)
+    int ext[3] = {1 << 30, 1 << 30, 1 << 30};
+    for (int b = 0; b < nb; b++)
+        for (int d = 0; d < 3; d++) ext[d] = std::min(ext[d], g->nnbl_bl[6 * b + d + 3] - g->nnbl_bl[6 * b + d]);
     for (int lev = 0; lev <= p->ilevmax; lev++) {
         int istep = 1 << lev;
-        if ((g->nnbl_bl[3] - g->nnbl_bl[0]) / istep == 0 || (g->nnbl_bl[4] - g->nnbl_bl[1]) / istep == 0 ||
-            (g->nnbl_bl[5] - g->nnbl_bl[2]) / istep == 0) {
+        if (ext[0] / istep == 0 || ext[1] / istep == 0 || ext[2] / istep == 0) {
             g->ilevmax = lev - 1;
             break;
         }

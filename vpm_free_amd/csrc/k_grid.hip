@@ -276,8 +276,15 @@ __device__ __forceinline__ void inner_face_node(const int nbl[6], int64_t t, int
     }
 }
 
+// The block of face node t of the list over all blocks (blocks in order, f0 ascending).
+__device__ __forceinline__ int face_block(const MbBlock *__restrict__ blk, int nblocks, int64_t t) {
+    int b = nblocks - 1;
+    while (b > 0 && t < blk[b].f0) b--;
+    return b;
+}
+
 // interp_stencil (yaps3d.f90:1914-1983) for the face nodes of every block in one launch.
-// Block b's arrays: fine + b * bstride (3 components of B nodes, SoA), samp + b * sstride
+// Block b's arrays: fine + blk[b].off (3 components of its nodes, SoA), samp + b * sstride
 // (3 x coarse grid).  fine holds every block's infinite-domain solution as it is before
 // this step; the new face values go to out (read nothing that this launch writes).
 __global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock *__restrict__ blk, GView csol,
@@ -285,10 +292,10 @@ __global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock 
                                                          const double *__restrict__ fine, double *__restrict__ out) {
 #pragma clang fp contract(off)
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nface * a.nblocks) return;
-    const int b = (int)(t / a.nface);
-    const int64_t tl = t % a.nface;
+    if (t >= a.ntot) return;
+    const int b = face_block(blk, a.nblocks, t);
     const MbBlock &B = blk[b];
+    const int64_t tl = t - B.f0;
     int i, j, k;
     inner_face_node(B.nbl, tl, &i, &j, &k);
     const double X0 = B.xbl[0] + (double)(i - 1) * a.h[0];
@@ -309,7 +316,7 @@ __global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock 
     }
     const int64_t B3 = (int64_t)B.n[0] * B.n[1] * B.n[2];
     const int64_t o = ((int64_t)(k - 1) * B.n[1] + (j - 1)) * B.n[0] + (i - 1);
-    const double *own = fine + (int64_t)b * a.bstride;
+    const double *own = fine + B.off;
     double acc[3] = {own[o], own[B3 + o], own[2 * B3 + o]};
     for (int kc = kn - 1; kc <= kn + 1; kc++)
         for (int jc = jn - 1; jc <= jn + 1; jc++)
@@ -343,7 +350,7 @@ __global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock 
                 const int jj = j + (int)round((B.xbl[1] - C.xbl[1]) / a.h[1]);
                 const int kk = k + (int)round((B.xbl[2] - C.xbl[2]) / a.h[2]);
                 if (ii < 1 || ii > C.n[0] || jj < 1 || jj > C.n[1] || kk < 1 || kk > C.n[2]) continue;
-                const double *fc = fine + (int64_t)bc * a.bstride;
+                const double *fc = fine + C.off;
                 const int64_t C3 = (int64_t)C.n[0] * C.n[1] * C.n[2];
                 const int64_t on = ((int64_t)(kk - 1) * C.n[1] + (jj - 1)) * C.n[0] + (ii - 1);
                 for (int q = 0; q < 3; q++) acc[q] = acc[q] + fc[q * C3 + on];
@@ -354,20 +361,20 @@ __global__ void __launch_bounds__(256) k_face_interp_mb(MbArgs a, const MbBlock 
 __global__ void k_face_store_mb(MbArgs a, const MbBlock *__restrict__ blk, const double *__restrict__ in_vals,
                                 double *__restrict__ sol) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.nface * a.nblocks) return;
-    const int b = (int)(t / a.nface);
-    const MbBlock &B = blk[b];
+    if (t >= a.ntot) return;
+    const MbBlock &B = blk[face_block(blk, a.nblocks, t)];
     int i, j, k;
-    inner_face_node(B.nbl, t % a.nface, &i, &j, &k);
+    inner_face_node(B.nbl, t - B.f0, &i, &j, &k);
     const int64_t B3 = (int64_t)B.n[0] * B.n[1] * B.n[2];
     const int64_t o = ((int64_t)(k - 1) * B.n[1] + (j - 1)) * B.n[0] + (i - 1);
-    double *sb = sol + (int64_t)b * a.bstride;
+    double *sb = sol + B.off;
     for (int q = 0; q < 3; q++) sb[q * B3 + o] = in_vals[3 * t + q];
 }
 
 int face_interp_blocks(Ctx *c, const MbArgs &a, const MbBlock *blk_dev, GView csol, const double *samp, double *sol,
                        double *tmp) {
-    const int64_t nt = a.nface * a.nblocks;
+    const int64_t nt = a.ntot;
+    if (nt <= 0) return VPMX_OK;
     hipLaunchKernelGGL(k_face_interp_mb, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, a, blk_dev, csol,
                        samp, sol, tmp);
     hipLaunchKernelGGL(k_face_store_mb, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, c->stream, a, blk_dev, tmp,

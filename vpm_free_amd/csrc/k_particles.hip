@@ -1198,19 +1198,35 @@ __global__ void __launch_bounds__(256) k_reduce_particles(const double *xp, cons
     if (t == 0)
         for (int q = 0; q < NV; q++) part[(int64_t)blockIdx.x * NV + q] = s[q][0];
 }
+// The partials in a fixed order: thread t folds partials t, t + 256, ... in turn, then a
+// fixed tree over the 256 threads (the same bits for the same partials, whatever the launch).
 template <int NV, int MODE>
-__global__ void k_reduce_final(const double *part, int nb, double *out) {
+__global__ void __launch_bounds__(256) k_reduce_final(const double *part, int nb, double *out, double hs = 1.0) {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0) return;
-    for (int q = 0; q < NV; q++) {
-        double v = part[q];
-        for (int b = 1; b < nb; b++) {
-            double x = part[(int64_t)b * NV + q];
-            if (MODE == 1) v = q < 3 ? fmin(v, x) : fmax(v, x);
-            else v = v + x;
+    __shared__ double s[NV][256];
+    const int t = threadIdx.x;
+    double v[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) v[q] = MODE == 1 ? (q < 3 ? 1e308 : -1e308) : 0.0;
+    for (int b = t; b < nb; b += 256)
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            const double x = part[(int64_t)b * NV + q];
+            if (MODE == 1) v[q] = q < 3 ? fmin(v[q], x) : fmax(v[q], x);
+            else v[q] = v[q] + x;
         }
-        out[q] = v;
+#pragma unroll
+    for (int q = 0; q < NV; q++) s[q][t] = v[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int q = 0; q < NV; q++) {
+                if (MODE == 1) s[q][t] = q < 3 ? fmin(s[q][t], s[q][t + off]) : fmax(s[q][t], s[q][t + off]);
+                else s[q][t] = s[q][t] + s[q][t + off];
+            }
+        __syncthreads();
     }
+    if (t < NV) out[t] = (MODE == 0 && t >= 3) ? hs * s[t][0] : s[t][0];
 }
 
 static int reduce6(Ctx *c, int mode, const double *xp, const double *qp, int qs, int64_t n, double out[6]) {
@@ -1222,10 +1238,10 @@ static int reduce6(Ctx *c, int mode, const double *xp, const double *qp, int qs,
     if (st) return st;
     if (mode == 1) {
         hipLaunchKernelGGL((k_reduce_particles<6, 1>), dim3(nb), dim3(256), 0, c->stream, xp, qp, qs, n, part);
-        hipLaunchKernelGGL((k_reduce_final<6, 1>), dim3(1), dim3(64), 0, c->stream, part, nb, part + 6 * nb);
+        hipLaunchKernelGGL((k_reduce_final<6, 1>), dim3(1), dim3(256), 0, c->stream, part, nb, part + 6 * nb);
     } else {
         hipLaunchKernelGGL((k_reduce_particles<6, 0>), dim3(nb), dim3(256), 0, c->stream, xp, qp, qs, n, part);
-        hipLaunchKernelGGL((k_reduce_final<6, 0>), dim3(1), dim3(64), 0, c->stream, part, nb, part + 6 * nb);
+        hipLaunchKernelGGL((k_reduce_final<6, 0>), dim3(1), dim3(256), 0, c->stream, part, nb, part + 6 * nb);
     }
     VPMX_HIP(c, hipGetLastError());
     VPMX_HIP(c, hipMemcpyAsync(out, part + 6 * nb, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));

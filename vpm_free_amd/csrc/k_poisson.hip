@@ -1029,6 +1029,134 @@ __host__ __device__ constexpr int wave_cap(bool dbl, int mode, int fix) {
     return (fix == 5 && mode == DST_Z) ? 4 : 0;
 }
 
+// ---------------------------------------------------------------- mixed z pass
+// The z pass of the mixed form's accurate solves: forward transform and diagonal from the
+// FP32 work array in FP32 (odd-extension form), inverse transform in FP64 (half-length
+// form), result stored FP64 into a.wo (same pitch and component stride).  A rounding error
+// made before the diagonal is divided by the eigenvalue of its mode, so FP32 forward passes
+// leave the solution's high modes accurate; one made after it is white noise on the solution,
+// which the second differences of the curl and stretching stencils amplify by 1/h^2 -- so
+// everything after the diagonal runs, and is stored, in FP64.  Each wave's LDS region holds
+// either form's sequence (the FP32 one of 2(m+1) points and the FP64 one of m+1 points take
+// the same bytes); between the two transforms a wave holds the diagonal's outputs in
+// registers.
+template <int MAXR, int FIX32, int FIX64>
+__device__ __forceinline__ void dst_zmix_body(const DstArgs &a, const WavePlan &P32, const WavePlan &P64,
+                                              const double2 *__restrict__ tw64) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds_raw[];
+    __shared__ double lamxy[16];
+    __shared__ double lzs[WAVE_LMAX];
+    constexpr int nl = WV_NL;
+    const int N32 = FIX32 >= 0 ? fix_len<FIX32>() : P32.L;  // 2(m+1)
+    const int N64 = FIX64 >= 0 ? fix_len<FIX64>() : P64.L;  // m+1
+    const int m = N64 - 1;
+    const int nbatch = (int)((a.nlines + nl - 1) / nl);
+    constexpr int S32 = wave_pad_shift(false, DST_Z, FIX32), S64 = wave_pad_shift(true, DST_Z, FIX64);
+    constexpr bool CR32 = const_roots(false, DST_Z, FIX32), CR64 = const_roots(true, DST_Z, FIX64);
+    const int Np32 = pad_len<S32>(N32), Np64 = pad_len<S64>(N64);
+    const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;  // double2 slots per wave
+    float *Af = (float *)lds_raw;
+    double *Ad = (double *)lds_raw;
+    float2 *tw32l = (float2 *)(lds_raw + 8 * Wq);
+    double2 *tw64l = lds_raw + 8 * Wq + N32 / 2;
+    // scalar slot of element e of line `line` in a batch (lines 2p and 2p+1: real and
+    // imaginary parts of wave p's sequence)
+    auto slot32 = [&](int line, int e) { return 2 * ((line >> 1) * 2 * Wq + pad_pos<S32>(e)) + (line & 1); };
+    auto slot64 = [&](int line, int e) { return 2 * ((line >> 1) * Wq + pad_pos<S64>(e)) + (line & 1); };
+    const int tline = (int)threadIdx.x & (WV_NL - 1), tpt0 = (int)threadIdx.x / WV_NL;
+    constexpr int SSTEP32 = 2 * (WV_PS + (S32 ? WV_PS >> S32 : 0));
+    constexpr int SSTEP64 = 2 * (WV_PS + (S64 ? WV_PS >> S64 : 0));
+    const int s32 = slot32(tline, tpt0 + 1), ms32 = slot32(tline, N32 - 1 - tpt0), s64 = slot64(tline, tpt0 + 1);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    using Buf32 = typename std::conditional<S32 == 0, float2 *, PadBuf<float2, S32>>::type;
+    using Buf64 = typename std::conditional<S64 == 0, double2 *, PadBuf<double2, S64>>::type;
+    const Buf32 B32{(float2 *)(lds_raw + wave * Wq)};
+    const Buf64 B64{lds_raw + wave * Wq};
+    for (int t = tid; t < N32; t += DST_THREADS) tw32l[t] = make_float2((float)a.tw[t].x, (float)a.tw[t].y);
+    for (int t = tid; t < N64; t += DST_THREADS) tw64l[t] = tw64[t];
+    for (int t = tid; t < m; t += DST_THREADS) lzs[t] = a.lam[2][t];
+    float v[DST_PF];
+    bool vok = false;
+    double fx[2];
+    const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
+    auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
+    const int nbt = nbatch * a.ncomp;
+    int it = 0, gb = batch_of(0);
+    int comp = gb / nbatch, b = gb - comp * nbatch;
+    if (gb < nbt) wave_prefetch<DST_Z, float>(a, (const float *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
+    for (; gb < nbt; ++it) {
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++)
+            if (tpt0 + u * WV_PS < m) {
+                const float x = vok ? v[u] : 0.f;
+                Af[s32 + u * SSTEP32] = x;
+                Af[ms32 - u * SSTEP32] = -x;  // odd extension
+            }
+        if (tid < nl) {
+            const int gl = b * nl + tid;
+            double lv = 0.0;
+            if (dst_line_ok<DST_Z>(a, gl)) lv = a.lam[0][gl % a.px] + a.lam[1][gl / a.px];
+            lamxy[tid] = lv;
+        }
+        __syncthreads();
+        const int gbn = batch_of(it + 1);
+        const int compn = gbn / nbatch, bn = gbn - compn * nbatch;
+        if (gbn < nbt)
+            wave_prefetch<DST_Z, float>(a, (const float *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
+        // forward transform in FP32
+        if (lane == 0) B32[0] = B32[m + 1] = make_float2(0.f, 0.f);
+        __builtin_amdgcn_wave_barrier();
+        if constexpr (FIX32 >= 0) FixedFft<FIX32>::template run<float2, Buf32, CR32>(B32, tw32l, lane);
+        else wave_fft<MAXR>(B32, tw32l, P32, lane);
+        // F = (-Im Z/2, Re Z/2) times -norm / lambda in FP64 (the FP32 values convert exactly),
+        // held in registers while the region changes form
+        const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
+        constexpr int GPL = (WAVE_LMAX + 63) / 64;
+        double2 g[GPL];
+#pragma unroll
+        for (int i = 0; i < GPL; i++) {
+            const int j = 1 + lane + 64 * i;
+            g[i] = make_double2(0.0, 0.0);
+            if (j <= m) {
+                const float2 z = B32[j];
+                const double lz = lzs[j - 1];
+                g[i] = make_double2((double)(-0.5f * z.y) * neg_quot(a.norm, la + lz),
+                                    (double)(0.5f * z.x) * neg_quot(a.norm, lb + lz));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int i = 0; i < GPL; i++) {
+            const int j = 1 + lane + 64 * i;
+            if (j <= m) B64[j] = g[i];
+        }
+        __builtin_amdgcn_wave_barrier();
+        // inverse transform in FP64 (half-length form)
+        wave_pre<double2>(B64, a.sn, N64, lane, nullptr, 0.0);
+        if constexpr (FIX64 >= 0) FixedFft<FIX64>::template run<double2, Buf64, CR64>(B64, tw64l, lane);
+        else wave_fft<MAXR>(B64, tw64l, P64, lane);
+        wave_post<double2>(B64, N64, lane);
+        __syncthreads();
+        const WaveAddr ad = wave_addr<DST_Z>(a, comp, b * nl + tline, tpt0);
+        double *wo = a.wo + comp * a.wcs;
+#pragma unroll
+        for (int u = 0; u < DST_PF; u++) {
+            if (!ad.ok || tpt0 + u * WV_PS >= m) continue;
+            wo[ad.dst + u * ad.dst_step] = Ad[s64 + u * SSTEP64] * a.scale;
+        }
+        __syncthreads();
+        gb = gbn;
+        comp = compn;
+        b = bn;
+    }
+}
+
+template <int MAXR, int FIX32, int FIX64>
+__global__ void __launch_bounds__(DST_THREADS) k_dst_zmix(DstArgs a, WavePlan P32, WavePlan P64, const double2 *tw64) {
+    dst_zmix_body<MAXR, FIX32, FIX64>(a, P32, P64, tw64);
+}
+
 // Zero the six faces of a window (zero Dirichlet values of solve_eq_0_3d).
 __global__ void k_zero_faces(GView out, int nx, int ny, int nz, int ncomp) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1338,6 +1466,86 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     return VPMX_OK;
 }
 
+static WavePlan wave_plan(const FftPlan *fp) {
+    WavePlan P{};
+    P.L = fp->L;
+    P.nstages = fp->nstages;
+    int ns = 1;
+    for (int s = 0; s < fp->nstages; s++) {
+        P.R[s] = fp->radix[s];
+        P.Ns[s] = ns;
+        P.mg[s] = ns > 1 ? (unsigned)((1ull << 32) / (unsigned long long)ns + 1ull) : 0u;
+        ns *= fp->radix[s];
+    }
+    return P;
+}
+
+// The mixed z pass (k_dst_zmix) on the FP32 work array a.w, FP64 results into wo.
+static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *wo) {
+    const FftPlan *f32 = X.full, *f64 = X.half;
+    const WavePlan P32 = wave_plan(f32), P64 = wave_plan(f64);
+    setup_axis(a, (FftPlan *)f32, X.m);
+    const int rc32 = radix_class(a);
+    DstArgs ah = a;
+    setup_axis(ah, (FftPlan *)f64, X.m);
+    const int rc64 = radix_class(ah);
+    const int rc = rc32 > rc64 ? rc32 : rc64;
+    a.sn = X.sn;
+    a.wo = wo;
+    a.npairs = 8;
+    a.ncomp = ncomp;
+    auto same = [](const FftPlan *fp, const int *rr, int n) {
+        if (fp->nstages != n) return false;
+        for (int q = 0; q < n; q++)
+            if (fp->radix[q] != rr[q]) return false;
+        return true;
+    };
+    static const int r576[] = {4, 4, 4, 3, 3}, r288[] = {4, 4, 2, 3, 3}, r272[] = {4, 4, 17}, r544[] = {4, 4, 2, 17};
+    int fix32 = -1, fix64 = -1;
+    const void *fn = nullptr;
+    if (f32->L == 576 && same(f32, r576, 5) && f64->L == 288 && same(f64, r288, 5)) {
+        fix32 = 0;
+        fix64 = 2;
+        fn = (const void *)k_dst_zmix<4, 0, 2>;
+    } else if (f32->L == 544 && same(f32, r544, 4) && f64->L == 272 && same(f64, r272, 3)) {
+        fix32 = 5;
+        fix64 = 3;
+        fn = (const void *)k_dst_zmix<17, 5, 3>;
+    } else {
+        fn = rc == 4 ? (const void *)k_dst_zmix<4, -1, -1>
+           : rc == 7 ? (const void *)k_dst_zmix<7, -1, -1>
+           : rc == 17 ? (const void *)k_dst_zmix<17, -1, -1> : (const void *)k_dst_zmix<37, -1, -1>;
+    }
+    // the kernel's LDS layout: eight regions of Wq double2 slots, then both twiddle tables
+    const int S32 = wave_pad_shift(false, DST_Z, fix32), S64 = wave_pad_shift(true, DST_Z, fix64);
+    const int Np32 = S32 == 0 ? pad_len<0>(f32->L) : S32 == 3 ? pad_len<3>(f32->L) : pad_len<4>(f32->L);
+    const int Np64 = S64 == 0 ? pad_len<0>(f64->L) : S64 == 3 ? pad_len<3>(f64->L) : pad_len<4>(f64->L);
+    const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
+    const size_t lds = sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
+    int per_cu = c->dst_wg_per_cu;
+    if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
+        per_cu = 1;
+    if (per_cu < 1) per_cu = 1;
+    int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
+    if (resident < 16) resident = 16;
+    const int64_t nbatch = (a.nlines + 15) / 16;
+    const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
+    dim3 grid((unsigned)(need < resident ? need : resident));
+    const double2 *tw64 = f64->tw;
+    WavePlan p32 = P32, p64 = P64;
+    void *args[] = {&a, &p32, &p64, &tw64};
+    hipEvent_t e0, e1;
+    if (a.m == c->dst_timed_m && kernel_events(c, K_DSTZ, &e0, &e1)) {
+        // FP32 read + FP64 write of every interior point
+        c->k_work[K_DSTZ] = (4.0 + 8.0) * (double)a.mx * a.my * a.m * ncomp;
+        VPMX_HIP(c, hipExtLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream, e0, e1, 0));
+    } else {
+        VPMX_HIP(c, hipLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream));
+    }
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
 // The one-wave-per-pair kernel: FP64 lines of up to WAVE_LMAX - 1 points (FFT of m+1),
 // FP32 lines of up to WAVE_LMAX_F / 2 - 1 points (FFT of 2(m+1)), radices up to 37.
 static int wave_class(const AxisPlan &P, bool f32) {
@@ -1497,7 +1705,7 @@ static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GVi
 }
 
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out,
-                      int reuse) {
+                      int reuse, int form) {
     int st = VPMX_OK;
     int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
     if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(c, VPMX_ERR_ARG, "Poisson window needs >= 3 nodes per axis");
@@ -1526,8 +1734,14 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     VPMX_TRY(eigen(c, m[2], h[2], c->lam[2], &a.lam[2]));
     a.norm = (2.0 / (m[0] + 1)) * (2.0 / (m[1] + 1)) * (2.0 / (m[2] + 1));
     a.scale = 1.0;
-    // mixed form: the five passes in FP32 when every axis takes the one-wave kernel
-    const bool f32 = c->precision == 1 && wave_class(ax[0], true) && wave_class(ax[1], true) && wave_class(ax[2], true);
+    // the FP32 passes run when every axis takes the one-wave kernel in both forms; SOLVE_CTX
+    // follows the context's precision (FP32 passes in the mixed form)
+    if (form == SOLVE_CTX) form = c->precision == 1 ? SOLVE_FP32 : SOLVE_FP64;
+    const bool wave32 = wave_class(ax[0], true) && wave_class(ax[1], true) && wave_class(ax[2], true);
+    const bool wave64 = wave_class(ax[0], false) && wave_class(ax[1], false) && wave_class(ax[2], false);
+    const bool inv64 = form == SOLVE_INV64 && wave32 && wave64 && c->dst_kernel == 4;
+    const bool f32 = (form == SOLVE_FP32 && wave32) || inv64;
+    if (inv64) reuse = 0;  // (the fine solves' reuse keeps FP32 arrays)
     // one-wave kernels on every axis: rows of the work array padded to 128 bytes
     const bool wave_all = f32 || (c->dst_kernel == 4 && wave_class(ax[0], false) && wave_class(ax[1], false) &&
                                   wave_class(ax[2], false));
@@ -1565,14 +1779,23 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     }
     // z forward + diagonal + inverse (into wo), then the inverse passes in place there
     a.nlines = (int64_t)a.px * m[1];
-    VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
-    a.w = a.wo;
+    if (inv64) {
+        // FP32 forward transforms and diagonal; the inverse passes in FP64
+        double *w64 = (double *)ensure(c, c->dst_work64, sizeof(double) * (size_t)a.wcs * ncomp, &st);
+        if (st) return st;
+        VPMX_TRY(launch_zmix(c, a, ncomp, ax[2], w64));
+        a.w = a.wo = w64;
+    } else {
+        VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
+        a.w = a.wo;
+    }
+    const bool fi = f32 && !inv64;
     // y inverse
     a.nlines = (int64_t)a.px * m[2];
-    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], f32));
+    VPMX_TRY(run_pass<DST_Y>(c, a, ncomp, ax[1], fi));
     // x inverse -> output interior
     a.nlines = (int64_t)m[1] * m[2];
-    VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0], f32));
+    VPMX_TRY(run_pass<DST_XI>(c, a, ncomp, ax[0], fi));
     if (!bc) VPMX_TRY(zero_faces(c, wn, out, ncomp));
     return VPMX_OK;
 }

@@ -140,6 +140,12 @@ static int upload_cmap(Ctx *c, const std::vector<int> &cmap, int **out) {
     return VPMX_OK;
 }
 
+// The inner-box solve gives the stream function whose second differences are the stretching
+// term: in the mixed form its passes after the diagonal run in FP64 (an FP32 inverse transform
+// leaves white noise that the stencils amplify by 1/h^2); its forward passes, and the fine and
+// coarse infinite-domain solves, run in FP32.
+static int final_form(const Ctx *c) { return c->precision == 1 ? SOLVE_INV64 : SOLVE_FP64; }
+
 // ---------------------------------------------------------------- yaps (1 block)
 static int yaps_single(Ctx *c, bool *used) {
     int st = VPMX_OK;
@@ -223,13 +229,25 @@ static int yaps_single(Ctx *c, bool *used) {
     window_h(&G.xbound_bl[0], G.dpm, w, h);
     GView rin = view(rhs_bl, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
     GView sin = view(sol_bl, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
-    VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin));
+    VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin, 0, final_form(c)));
     mark(c, ST_FINAL, 1);
     used[ST_FINAL] = true;
     return VPMX_OK;
 }
 
 // ---------------------------------------------------------------- yaps (several blocks)
+// Offsets of the blocks' arrays (3 components of each block's NNbl nodes, back to back) in
+// the blocks' RHS / SOL buffers; returns the total.  Blocks may differ in size (uneven
+// splits); the reference's even blocks are all the first one's size.
+static int64_t block_offsets(const vpmx_geom &G, std::vector<int64_t> &off) {
+    off.resize((size_t)G.nblocks + 1);
+    off[0] = 0;
+    for (int b = 0; b < G.nblocks; b++)
+        off[b + 1] = off[b] + 3 * (int64_t)G.nnbl[3 * b] * G.nnbl[3 * b + 1] * G.nnbl[3 * b + 2];
+    return off[G.nblocks];
+}
+static const int32_t *block_n(const vpmx_geom &G, int b) { return &G.nnbl[3 * b]; }
+
 // yaps3d.f90:1-453 with BLOCKS = NBI*NBJ*NBK blocks on this device, the work of every rank
 // in turn: all fine infinite-domain solves; the coarse problem from every block's samples
 // (Laplacians summed in block order); the coarse solve; interp_stencil on every block's
@@ -241,23 +259,20 @@ static int yaps_multi(Ctx *c, bool *used) {
     const vpmx_geom &G = c->geom;
     const vpmx_params &P = c->prm;
     const int NB = G.nblocks;
-    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
-    for (int b = 1; b < NB; b++)
-        for (int d = 0; d < 3; d++)
-            if (G.nnbl[3 * b + d] != nb[d]) return fail(c, VPMX_ERR_UNSUP, "yaps blocks of different sizes");
-    const int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
+    std::vector<int64_t> boff;
+    const int64_t tot = block_offsets(G, boff);
     double *rhs_bl = (double *)c->rhs_bl.p;
-    double *sol_bl = (double *)ensure(c, c->sol_bl, sizeof(double) * 3 * (size_t)B * NB, &st); if (st) return st;
-    auto bview = [&](double *base, int b) { return view(base + (int64_t)b * 3 * B, nb); };
+    double *sol_bl = (double *)ensure(c, c->sol_bl, sizeof(double) * (size_t)tot, &st); if (st) return st;
+    auto bview = [&](double *base, int b) { return view(base + boff[b], block_n(G, b)); };
     const bool tree = P.iyntree == 1;
     if (tree && G.ilevmax < 1)
         return fail(c, VPMX_ERR_UNSUP, "tree = 1 needs at least one level on the coarse grid (ilevmax >= 1)");
     // 1. fine infinite-domain solves
     mark(c, ST_FINE, 0);
-    VPMX_TRY(fill(c, sol_bl, 3 * B * NB, 0.0));
-    c->dst_timed_m = nb[1] - 2;
+    VPMX_TRY(fill(c, sol_bl, tot, 0.0));
+    c->dst_timed_m = G.nnbl[1] - 2;
     for (int b = 0; b < NB; b++) {
-        Grid gb = make_grid(&G.nnbl[3 * b], &G.xbound_bl[6 * b], G.dpm);
+        Grid gb = make_grid(block_n(G, b), &G.xbound_bl[6 * b], G.dpm);
         VPMX_TRY(pmesh_inf(c, gb, bview(rhs_bl, b), bview(sol_bl, b), c->sol0, true, tree ? G.ilevmax : -1, P.ibctyp));
     }
     mark(c, ST_FINE, 1);
@@ -270,33 +285,42 @@ static int yaps_multi(Ctx *c, bool *used) {
     double *rhsc = (double *)ensure(c, c->rhs_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
     double *solc = (double *)ensure(c, c->sol_c, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
     const int ncs = P.ncoarse;
-    int ns[3];
-    for (int d = 0; d < 3; d++) ns[d] = (nb[d] - 1) / ncs + 1;
-    std::vector<int> cmap((size_t)(ns[0] + ns[1] + ns[2]) * NB);
+    // per block: samples every ncoarse nodes along each axis (their coarse nodes in cmap)
+    std::vector<int> ns(3 * (size_t)NB), cmoff((size_t)NB + 1, 0);
+    for (int b = 0; b < NB; b++) {
+        for (int d = 0; d < 3; d++) ns[3 * b + d] = (block_n(G, b)[d] - 1) / ncs + 1;
+        cmoff[b + 1] = cmoff[b] + ns[3 * b] + ns[3 * b + 1] + ns[3 * b + 2];
+    }
+    std::vector<int> cmap((size_t)cmoff[NB]);
     std::vector<MbBlock> blk(NB);
+    int64_t nface_tot = 0;
     for (int b = 0; b < NB; b++) {
         MbBlock &M = blk[b];
+        M.off = boff[b];
         for (int d = 0; d < 6; d++) M.nbl[d] = G.nnbl_bl[6 * b + d];
-        int off = 0;
+        int off = cmoff[b];
         for (int d = 0; d < 3; d++) {
-            M.n[d] = nb[d];
+            M.n[d] = block_n(G, b)[d];
             M.xbl[d] = G.xbound_bl[6 * b + d];
-            for (int a = 0; a < ns[d]; a++) {
+            for (int a = 0; a < ns[3 * b + d]; a++) {
                 const int i = 1 + a * ncs;
                 const double X = G.xbound_bl[6 * b + d] + (double)(i - 1) * G.dpm[d];
                 const int node = (int)std::lround((X - G.xbound_coarse[d]) / G.dpm_coarse[d]) + 1;
                 if (node < 1 || node > cn[d]) return fail(c, VPMX_ERR_STATE, "coarse sample outside the coarse grid");
-                if (a > 0 && node != cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + a - 1] + 1)
+                if (a > 0 && node != cmap[(size_t)off + a - 1] + 1)
                     return fail(c, VPMX_ERR_STATE, "block samples are not consecutive coarse nodes");
-                cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + a] = node;
+                cmap[(size_t)off + a] = node;
             }
-            M.fp_lo[d] = cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off];
-            M.fp_hi[d] = cmap[(size_t)(ns[0] + ns[1] + ns[2]) * b + off + ns[d] - 1];
-            off += ns[d];
+            M.fp_lo[d] = cmap[(size_t)off];
+            M.fp_hi[d] = cmap[(size_t)off + ns[3 * b + d] - 1];
+            off += ns[3 * b + d];
         }
         M.bi = b % G.nbi + 1;
         M.bj = (b / G.nbi) % G.nbj + 1;
         M.bk = b / (G.nbi * G.nbj) + 1;
+        const int64_t fx = M.nbl[3] - M.nbl[0] + 1, fy = M.nbl[4] - M.nbl[1] + 1, fz = M.nbl[5] - M.nbl[2] + 1;
+        M.f0 = nface_tot;
+        nface_tot += 2 * (fy * fz + (fx - 2) * fz + (fx - 2) * (fy - 2));
     }
     int *cmap_d = nullptr;
     VPMX_TRY(upload_cmap(c, cmap, &cmap_d));
@@ -306,8 +330,8 @@ static int yaps_multi(Ctx *c, bool *used) {
     VPMX_TRY(fill(c, rhsc, 3 * GC, 0.0));
     auto cview = [&](double *base, int b) { return view(base + (int64_t)b * 3 * GC, cn); };
     for (int b = 0; b < NB; b++) {
-        const int *cm = cmap_d + (size_t)(ns[0] + ns[1] + ns[2]) * b;
-        VPMX_TRY(coarse_sample(c, 3, ns, ncs, cm, bview(sol_bl, b), cview(samp, b), cn));
+        const int *cm = cmap_d + cmoff[b];
+        VPMX_TRY(coarse_sample(c, 3, &ns[3 * b], ncs, cm, bview(sol_bl, b), cview(samp, b), cn));
         VPMX_TRY(coarse_laplacian(c, 3, cn, blk[b].fp_lo, blk[b].fp_hi, G.dpm_coarse, cview(samp, b), view(rhsc, cn),
                                   true));
     }
@@ -329,12 +353,9 @@ static int yaps_multi(Ctx *c, bool *used) {
     a.nbj = G.nbj;
     a.nbk = G.nbk;
     a.nblocks = NB;
-    a.bstride = 3 * B;
     a.sstride = 3 * GC;
-    const int *w0 = G.nnbl_bl;
-    const int64_t fx = w0[3] - w0[0] + 1, fy = w0[4] - w0[1] + 1, fz = w0[5] - w0[2] + 1;
-    a.nface = 2 * (fy * fz + (fx - 2) * fz + (fx - 2) * (fy - 2));
-    double *tmp = (double *)ensure(c, c->mb_tmp, sizeof(double) * 3 * (size_t)(a.nface * NB), &st); if (st) return st;
+    a.ntot = nface_tot;
+    double *tmp = (double *)ensure(c, c->mb_tmp, sizeof(double) * 3 * (size_t)nface_tot, &st); if (st) return st;
     VPMX_TRY(face_interp_blocks(c, a, blk_d, view(solc, cn), samp, sol_bl, tmp));
     mark(c, ST_BC, 1);
     used[ST_BC] = true;
@@ -345,9 +366,9 @@ static int yaps_multi(Ctx *c, bool *used) {
         int wn[3] = {w[3] - w[0] + 1, w[4] - w[1] + 1, w[5] - w[2] + 1};
         double h[3];
         window_h(&G.xbound_bl[6 * b], G.dpm, w, h);
-        GView rin = view(rhs_bl + (int64_t)b * 3 * B, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
-        GView sin = view(sol_bl + (int64_t)b * 3 * B, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
-        VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin));
+        GView rin = view(rhs_bl + boff[b], wn, w[0] - 1, w[1] - 1, w[2] - 1, block_n(G, b));
+        GView sin = view(sol_bl + boff[b], wn, w[0] - 1, w[1] - 1, w[2] - 1, block_n(G, b));
+        VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin, 0, final_form(c)));
     }
     VPMX_HIP(c, hipStreamSynchronize(c->stream));  // host tables go out of scope
     mark(c, ST_FINAL, 1);
@@ -355,13 +376,15 @@ static int yaps_multi(Ctx *c, bool *used) {
     return VPMX_OK;
 }
 
-// Global origin (1-based global node) of block b's inner box (rhsscat_3d / solget_3d).
+// Global origin (1-based global node) of block b's inner box (rhsscat_3d / solget_3d): its
+// first inner node's position on the global lattice (the reference's
+// (nb_i - 1)(NXf - NXs) + NN_bl(1) for its even blocks, vpm_mpi.f90:199-238).
 static void block_origin(const vpmx_geom &G, int b, int g0[3]) {
-    const int bi = b % G.nbi, bj = (b / G.nbi) % G.nbj, bk = b / (G.nbi * G.nbj);
     const int32_t *w = &G.nnbl_bl[6 * b];
-    g0[0] = bi * (w[3] - w[0]) + G.nn_bl[0];
-    g0[1] = bj * (w[4] - w[1]) + G.nn_bl[1];
-    g0[2] = bk * (w[5] - w[2]) + G.nn_bl[2];
+    for (int d = 0; d < 3; d++) {
+        const double x = G.xbound_bl[6 * b + d] + (double)(w[d] - 1) * G.dpm[d];
+        g0[d] = (int)std::lround((x - G.xbound[d]) / G.dpm[d]) + 1;
+    }
 }
 
 // The global RHS / SOL grids: the block arrays' inner box for one block, own grids otherwise.
@@ -414,11 +437,12 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
     bool used[ST_COUNT] = {};
     c->last_pairs = 0.0;
     const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
-    int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
     const bool multi = G.nblocks > 1;
     const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
     const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
-    double *rhs_bl = (double *)ensure(c, c->rhs_bl, sizeof(double) * 3 * (size_t)B * G.nblocks, &st); if (st) return st;
+    std::vector<int64_t> boff;
+    const int64_t btot = block_offsets(G, boff);
+    double *rhs_bl = (double *)ensure(c, c->rhs_bl, sizeof(double) * (size_t)btot, &st); if (st) return st;
     mark(c, ST_PROJ, 0);
     Grid gg = make_grid(G.nn, G.xbound, G.dpm);
     VPMX_TRY(sort_particles(c, gg, xp, n, c->sort));
@@ -439,8 +463,8 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
             int g0[3];
             block_origin(G, b, g0);
             const int zmin[3] = {b % G.nbi > 0, (b / G.nbi) % G.nbj > 0, b / (G.nbi * G.nbj) > 0};
-            VPMX_TRY(rhsscat_block(c, nb, &G.nnbl_bl[6 * b], g0, zmin, view(rhs_g, gn),
-                                   view(rhs_bl + (int64_t)b * 3 * B, nb)));
+            VPMX_TRY(rhsscat_block(c, block_n(G, b), &G.nnbl_bl[6 * b], g0, zmin, view(rhs_g, gn),
+                                   view(rhs_bl + boff[b], block_n(G, b))));
         }
     }
     mark(c, ST_PROJ, 1);
@@ -476,7 +500,7 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
         for (int b = 0; b < G.nblocks; b++) {
             int g0[3];
             block_origin(G, b, g0);
-            VPMX_TRY(solget_block(c, &G.nnbl_bl[6 * b], g0, view((double *)c->sol_bl.p + (int64_t)b * 3 * B, nb),
+            VPMX_TRY(solget_block(c, &G.nnbl_bl[6 * b], g0, view((double *)c->sol_bl.p + boff[b], block_n(G, b)),
                                   view(sol_g, gn)));
         }
     }
@@ -562,21 +586,25 @@ static int upload_particles(Ctx *c, const double *xp, const double *qp, int64_t 
     return VPMX_OK;
 }
 
+// define_sizes with the context's block weights (even split when none are set).
+static int define_sizes_ctx(Ctx *c, const double *xp, int64_t n, int ntime) {
+    const int32_t *w[3];
+    for (int d = 0; d < 3; d++) w[d] = c->block_w[d].empty() ? nullptr : c->block_w[d].data();
+    const int r = vpmx_define_sizes_weighted(&c->prm, w[0], w[1], w[2], &c->geom, xp, n, ntime);
+    return r == VPMX_OK ? r : fail(c, r, "define_sizes failed (box, blocks or block weights)");
+}
+
 static int define_from_device(Ctx *c, int ntime) {
     // define_sizes needs the particle bounding box only when it redefines the box
     const vpmx_params &P = c->prm;
     bool redefine = (ntime == 0 && P.idefine == 1) || P.idefine == 0 || !c->geom.defined;
-    if (!redefine) {
-        int r = vpmx_define_sizes(&c->prm, &c->geom, nullptr, 0, ntime);
-        return r == VPMX_OK ? r : fail(c, r, "define_sizes failed");
-    }
+    if (!redefine) return define_sizes_ctx(c, nullptr, 0, ntime);
     if (c->np <= 0) return fail(c, VPMX_ERR_ARG, "define_sizes: no particles");
     double mn[3], mx[3];
     VPMX_TRY(particle_bbox(c, (const double *)c->xp.p, c->np, mn, mx));
     // the bbox is all define_sizes reads from the particles: two corner points suffice
     double pts[6] = {mn[0], mn[1], mn[2], mx[0], mx[1], mx[2]};
-    int r = vpmx_define_sizes(&c->prm, &c->geom, pts, 2, ntime);
-    return r == VPMX_OK ? r : fail(c, r, "define_sizes failed");
+    return define_sizes_ctx(c, pts, 2, ntime);
 }
 
 }  // namespace vpmx
@@ -638,7 +666,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
-                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev};
+                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev, &c->dst_work64};
     for (DBuf *b : bufs) release(*b);
     free_bspec(c);
     free_tree(c);
@@ -661,8 +689,26 @@ int32_t vpmx_set_params(vpmx_ctx *p, const vpmx_params *prm) {
     Ctx *c = C(p);
     if (!c || !prm) return VPMX_ERR_ARG;
     if (prm->ipmwrite > 10) return fail(c, VPMX_ERR_ARG, "IPMWRITE > 10");
+    if (c->have_prm && (prm->nbi != c->prm.nbi || prm->nbj != c->prm.nbj || prm->nbk != c->prm.nbk))
+        for (int d = 0; d < 3; d++) c->block_w[d].clear();  // the weights were for other block counts
     c->prm = *prm;
     c->have_prm = true;
+    return VPMX_OK;
+}
+
+int32_t vpmx_set_block_weights(vpmx_ctx *p, const int32_t *wx, const int32_t *wy, const int32_t *wz) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "block weights: set the parameters first");
+    const int32_t *w[3] = {wx, wy, wz};
+    const int n[3] = {c->prm.nbi, c->prm.nbj, c->prm.nbk};
+    for (int d = 0; d < 3; d++) {
+        c->block_w[d].clear();
+        if (!w[d]) continue;
+        for (int i = 0; i < n[d]; i++)
+            if (w[d][i] < 1) return fail(c, VPMX_ERR_ARG, "block weights must be positive");
+        c->block_w[d].assign(w[d], w[d] + n[d]);
+    }
     return VPMX_OK;
 }
 
@@ -688,8 +734,7 @@ int32_t vpmx_vpm(vpmx_ctx *p, double *xp, double *qp, double *up, double *gp, in
     if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
     c->ni = ni;
     if (whattodo == 0) {
-        int r = vpmx_define_sizes(&c->prm, &c->geom, xp, nvr, ntime);
-        if (r) return fail(c, r, "define_sizes failed");
+        VPMX_TRY(define_sizes_ctx(c, xp, nvr, ntime));
         c->have_grids = false;
         // velvr*_pm = 0 after WhatToDo = 0 (vpm.f90:154-166)
         if (c->vel.p) VPMX_TRY(fill(c, (double *)c->vel.p, (int64_t)(c->vel.bytes / sizeof(double)), 0.0));

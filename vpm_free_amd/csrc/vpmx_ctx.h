@@ -34,6 +34,7 @@ struct Ctx : vpmx_ctx {
     std::string err;
     vpmx_params prm{};
     bool have_prm = false;
+    std::vector<int32_t> block_w[3];  // uneven yaps blocks: extents in proportion (empty: even split)
     vpmx_geom geom{};
     int precision = 0;
 
@@ -62,6 +63,7 @@ struct Ctx : vpmx_ctx {
     const void *cmap_dev = nullptr;  // device address they were uploaded to
     DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
     DBuf dst_work, lam[3];
+    DBuf dst_work64;  // FP64 work array of the mixed form's accurate solves (after the diagonal)
     DBuf dst_save, fold_tab;  // forward-pass reuse: the kept work array, the face transforms
     int save_key[6] = {0, 0, 0, 0, 0, -1};  // m[3], px, ncomp, f32 of the kept array (f32 -1: none)
     std::map<int, void *> sinm_cache;  // sin(pi (t+1)(k+1) / (m+1)), t, k < m, per m

@@ -93,8 +93,12 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
 // 1 keeps the work array after the x and y forward passes, 2 starts from the kept array with
 // the transform of the folded Dirichlet terms taken off instead of running those passes
 // (falls back to the full solve when the kept array does not match the window).
+// form: the arithmetic of the passes -- SOLVE_CTX the context's precision (FP32 passes in the
+// mixed form), SOLVE_FP64, SOLVE_FP32, SOLVE_INV64 (forward passes and diagonal in FP32, the
+// inverse passes in FP64: the accuracy of an FP64 solve where second differences are taken).
+enum SolveForm { SOLVE_CTX = -1, SOLVE_FP64 = 0, SOLVE_FP32 = 1, SOLVE_INV64 = 2 };
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
-                      GView out, int reuse = 0);
+                      GView out, int reuse = 0, int form = SOLVE_CTX);
 
 // Complex FFT plan of length L (Stockham radices, device twiddle table exp(-2 pi i t / L)),
 // cached per context.
@@ -152,8 +156,11 @@ struct InterpArgs {
     int fp_lo[3], fp_hi[3];  // footprint of the block on the coarse grid (1-based coarse index)
 };
 int face_interp(Ctx *c, int ncomp, const InterpArgs &a, GView coarse_sol, GView samp, GView sol_bl);
-// Several yaps blocks (all blocks' arrays back to back, `bstride` doubles apart).
+// Several yaps blocks, of equal or uneven sizes (all blocks' arrays back to back, block b's
+// from MbBlock::off).
 struct MbBlock {
+    int64_t off;             // first double of the block's arrays in the blocks' buffers (3 x its nodes)
+    int64_t f0;              // first of its inner-face nodes in the list over all blocks
     int nbl[6];              // inner box NNbl_bl (1-based)
     int n[3];                // block grid NNbl
     int fp_lo[3], fp_hi[3];  // sampled footprint on the coarse grid (1-based, inclusive)
@@ -164,9 +171,8 @@ struct MbArgs {
     double h[3], xc[3], hc[3];
     int cn[3];
     int nbi, nbj, nbk, nblocks;
-    int64_t bstride;  // doubles per block in the fine arrays (3 x B)
     int64_t sstride;  // doubles per block in the sample array (3 x coarse nodes)
-    int64_t nface;    // inner-face nodes per block (all blocks equal)
+    int64_t ntot;     // inner-face nodes of all blocks
 };
 int face_interp_blocks(Ctx *c, const MbArgs &a, const MbBlock *blk_dev, GView csol, const double *samp, double *sol,
                        double *tmp);

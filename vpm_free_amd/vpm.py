@@ -81,6 +81,9 @@ def load_library():
     sig = {
         "vpmx_read_pm_inp": [C.c_char_p, C.POINTER(Params)],
         "vpmx_define_sizes": [C.POINTER(Params), C.POINTER(Geom), d, i64, i32],
+        "vpmx_define_sizes_weighted": [C.POINTER(Params), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                       C.POINTER(Geom), d, i64, i32],
+        "vpmx_set_block_weights": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
         "vpmx_create": [C.POINTER(vp), i32],
         "vpmx_set_params": [vp, C.POINTER(Params)],
         "vpmx_get_geometry": [vp, C.POINTER(Geom)],
@@ -142,12 +145,23 @@ def read_pm_inp(path):
     return p
 
 
-def define_sizes(params, xp, ntime=0, geom=None):
-    """define_sizes (vpm.f90:477-660) on the host; returns the (updated) Geom."""
+def _weights(w):
+    return None if w is None else np.ascontiguousarray(w, dtype=np.int32)
+
+
+def _wp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def define_sizes(params, xp, ntime=0, geom=None, weights=None):
+    """define_sizes (vpm.f90:477-660) on the host; returns the (updated) Geom.  weights
+    (wx, wy, wz): yaps blocks of uneven extents in proportion to them (None: even split)."""
     lib = load_library()
     g = geom if geom is not None else Geom()
     xp = _f64(xp)
-    r = lib.vpmx_define_sizes(C.byref(params), C.byref(g), _dp(xp), xp.shape[0] if xp.ndim == 2 else 0, ntime)
+    w = [None, None, None] if weights is None else [_weights(v) for v in weights]
+    r = lib.vpmx_define_sizes_weighted(C.byref(params), _wp(w[0]), _wp(w[1]), _wp(w[2]), C.byref(g), _dp(xp),
+                                       xp.shape[0] if xp.ndim == 2 else 0, ntime)
     if r != OK:
         raise VpmError(r, "define_sizes")
     return g
@@ -179,6 +193,11 @@ class VPM:
         self.params = params
         self._check(self._lib.vpmx_set_params(self._h, C.byref(params)))
         self.neqpm = 3
+
+    def set_block_weights(self, wx=None, wy=None, wz=None):
+        """Uneven yaps blocks for the next vpm mode 0 (None: the even split)."""
+        self._w = [_weights(wx), _weights(wy), _weights(wz)]  # kept alive for the call
+        self._check(self._lib.vpmx_set_block_weights(self._h, _wp(self._w[0]), _wp(self._w[1]), _wp(self._w[2])))
 
     def set_precision(self, mode):
         """0: plain FP64 form; 1: mixed form (FP32 Poisson passes, FP64 boundary sums)."""
