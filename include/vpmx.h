@@ -105,6 +105,17 @@ int32_t vpmx_get_geometry(const vpmx_ctx *ctx, vpmx_geom *out);
  * derivatives, curl and stretching, remesh and diagnostics, and every particle array and grid
  * in HBM.  VPMX_PRECISION=mixed in the environment selects 1 at vpmx_create. */
 int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
+/* Evaluation choices that leave the results unchanged beyond rounding (no counterpart in the
+ * reference): "boundsum" -- the Green's-function boundary sums by 2 spectral convolution
+ * (default), 0 direct sums over lattice tables, 1 direct sums over LDS source tiles;
+ * "dst_reuse" -- 1 (default): the second fine solve of pmesh starts from the first one's
+ * forward transforms, 0: both solves in full; "tuning" -- 1 (default): the autotuner picks
+ * kernel variants and persistent grid sizes by timing them once per launch key, 0: fixed
+ * defaults (clears the choices made so far). */
+int32_t vpmx_set_option(vpmx_ctx *ctx, const char *name, int32_t value);
+/* The autotuner's choices so far, one line per launch key ("key -> choice (time)"), into buf
+ * (NUL-terminated, cut to capacity). */
+int32_t vpmx_tuning_report(const vpmx_ctx *ctx, char *buf, int64_t capacity);
 
 /* vpm(XP,QP,UP,GP,NVR,neqpm,WhatToDo,...) (vpm.f90:54-475) on host arrays.
  * WhatToDo 0: geometry (define_sizes), velocity grids zeroed; 2: project, yaps solve,
@@ -146,6 +157,31 @@ int32_t vpmx_step_rk2(vpmx_ctx *ctx, double dt, const double uinf[3], int32_t nt
 /* FP64 diagnostics of the resident particles: circ = -sum QP(1:3),
  * imp = 1/2 sum x x (-QP(1:3)). */
 int32_t vpmx_diagnostics(vpmx_ctx *ctx, double circ[3], double imp[3]);
+
+/* ------------------------------------------------------------- batched clouds
+ * No counterpart in the reference, which runs independent clouds as one vpm() call after
+ * another (vpm.f90:54-475; per cloud the application's RK2 step with remesh_particles_3d,
+ * test.f90:83-150).  A batch owns one context per cloud; set each cloud up with the calls above
+ * (vpmx_set_params, vpmx_particles_upload, vpmx_vpm_resident(ctx, 0, 0) for its geometry with
+ * idefine = 1, vpmx_set_precision), then step all clouds together.  The first step prepares the
+ * batch: every cloud's particle arrays get the capacity of its remesh window and the count
+ * moves to the device, one step of every cloud runs and is undone (all buffers and plans reach
+ * their sizes); after that a cloud's particles change only through the batch (its upload and
+ * remesh calls return VPMX_ERR_STATE) and can be read with vpmx_particles_download. */
+typedef struct vpmx_batch vpmx_batch;
+int32_t vpmx_batch_create(vpmx_batch **out, int32_t device, int32_t nclouds);
+void vpmx_batch_destroy(vpmx_batch *b);
+const char *vpmx_batch_last_error(const vpmx_batch *b);
+int32_t vpmx_batch_cloud(vpmx_batch *b, int32_t i, vpmx_ctx **ctx);
+/* One RK2 step of every cloud (vpmx_step_rk2's step, remesh_particles_3d(mrem) when
+ * do_remesh).  use_graph = 0: each cloud's launches queued on its own stream, forked from and
+ * joined to the batch's stream; use_graph = 1: that whole step captured once as a hipGraph
+ * (again when dt, uinf, mrem or do_remesh change) and replayed.  Synchronous at return. */
+int32_t vpmx_batch_step_rk2(vpmx_batch *b, double dt, const double uinf[3], int32_t mrem, int32_t do_remesh,
+                            int32_t use_graph);
+/* Device time of the last batch step (events on the batch's stream around it) and the host
+ * time spent queueing it. */
+int32_t vpmx_batch_times(const vpmx_batch *b, double *step_ms, double *enqueue_ms);
 
 /* ----------------------------------------------------- stage entry points
  * Single routines of the path on host arrays in the reference layout, for parity

@@ -9,6 +9,7 @@ from vpm_free_amd import cases
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+BOUNDSUM = {"spectral": 2, "lattice": 0, "direct": 1}
 
 
 def rel_l2(a, b):
@@ -224,12 +225,11 @@ def test_solve_full_size_plans(ctx, n):
 
 
 @pytest.mark.parametrize("mode", ["spectral", "lattice", "direct"])
-def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
+def test_boundary_sum_forms(vpmlib, oracle, mode):
     """The three evaluations of the fine boundary sums (spectral convolution + batched
     products, lattice 1/r tables, direct LDS source tiles) against the oracle's direct sum,
     on a non-cubic block (every face pair has its own geometry)."""
     V = vpmlib
-    monkeypatch.setenv("VPMX_BOUNDSUM", mode)
     nn = np.array([65, 57, 49], np.int32)
     dpm = np.array([1 / 32, 1 / 30, 1 / 28])
     xb = np.array([-1.0, -0.95, -0.85, 0, 0, 0])
@@ -238,6 +238,7 @@ def test_boundary_sum_forms(vpmlib, oracle, monkeypatch, mode):
     say("case: boundary sums", mode, nn.tolist())
     ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
     with V.VPM(V.Params.make(0.125)) as c:
+        c.set_option("boundsum", BOUNDSUM[mode])
         got = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(got, ref) <= TOL
 
@@ -453,7 +454,7 @@ def test_vpm_modes_4_and_5(vpmlib, oracle):
     assert np.all(up5 == 0.0)
 
 
-def test_boundary_sums_spectral_long_axis(vpmlib, monkeypatch):
+def test_boundary_sums_spectral_long_axis(vpmlib):
     """Spectral boundary sums on a block with one 289-node axis, where the transforms along
     that axis run the 576-point compile-time plan (one wave per sequence) and the others the
     runtime wave plans, against the direct LDS-tile sums (pinned to the oracle on smaller
@@ -467,14 +468,14 @@ def test_boundary_sums_spectral_long_axis(vpmlib, monkeypatch):
     say("case: spectral boundary sums, 289-node axis", nn.tolist())
     out = {}
     for mode in ("spectral", "direct"):
-        monkeypatch.setenv("VPMX_BOUNDSUM", mode)
         with V.VPM(V.Params.make(0.125)) as c:
+            c.set_option("boundsum", BOUNDSUM[mode])
             out[mode] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(out["spectral"], out["direct"]) <= TOL
 
 
 @pytest.mark.parametrize("prec,tol", [(0, TOL), (1, 1e-6)])
-def test_pmesh_inf_reuse_full_size(vpmlib, monkeypatch, prec, tol):
+def test_pmesh_inf_reuse_full_size(vpmlib, prec, tol):
     """The second fine solve of pmesh_inf starts from the first one's x/y forward transforms
     less those of the folded Dirichlet values; at the 289-node block (compile-time plans, the
     spectral boundary sums' 576-point path) it must agree with the two full solves."""
@@ -487,8 +488,8 @@ def test_pmesh_inf_reuse_full_size(vpmlib, monkeypatch, prec, tol):
     say("case: pmesh_inf forward-pass reuse, 289^3, precision", prec)
     out = {}
     for reuse in ("1", "0"):
-        monkeypatch.setenv("VPMX_DST_REUSE", reuse)
         with V.VPM(V.Params.make(0.125)) as c:
+            c.set_option("dst_reuse", int(reuse))
             c.set_precision(prec)
             out[reuse] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
     assert rel_l2(out["1"], out["0"]) <= tol
@@ -536,3 +537,30 @@ def test_vpm_mode2_call_sequence_kept_zeros(vpmlib, oracle):
         check(c, "a", xa, qa)
         check(c, "r", xr, qr)
         check(c, "r", xr, qr)
+
+
+def test_tuner_choices_keep_results(vpmlib):
+    """The autotuner picks kernel variants and persistent grid sizes by timing them; the
+    results are those of the fixed defaults bit for bit (the variants run the same
+    arithmetic, and every work item's sums do not depend on which work-group takes it)."""
+    V = vpmlib
+    h = 1 / 16
+    xp, qp = cases.single_ring(h=h, sigma=0.1)
+    out = {}
+    for tuning in (1, 0):
+        with V.VPM(V.Params.make(h)) as c:
+            c.set_option("tuning", tuning)
+            for prec in (0, 1):
+                c.set_precision(prec)
+                up, gp = np.zeros_like(xp), np.zeros_like(xp)
+                c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+                c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+                out[(tuning, prec)] = (up, gp)
+            rep = c.tuning_report()
+        say("tuning", tuning, "report:\n" + rep)
+        assert "sine transforms" in rep
+        if tuning:
+            assert "ms)" in rep
+    for prec in (0, 1):
+        assert np.array_equal(out[(1, prec)][0], out[(0, prec)][0])
+        assert np.array_equal(out[(1, prec)][1], out[(0, prec)][1])

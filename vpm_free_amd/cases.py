@@ -73,3 +73,46 @@ def random_blobs(n, h, seed, lo=-1.0, hi=1.0):
     qp[:, :3] = rng.standard_normal((n, 3)) * h ** 3
     qp[:, 3] = h ** 3
     return xp, qp
+
+
+def batch_rings(seed=3, count=64, grids=(32, 48, 64, 96, 128), threshold=1e-6):
+    """Independent single-ring clouds: radius U[0.5, 1.5], core sigma U[0.05, 0.15], axis
+    uniform on the sphere, each in a box [-L, L]^3 with L = R + 4 sigma, cut into N cells per
+    axis with N drawn from `grids` with probability proportional to 1/N^3 (every size class
+    holds a similar share of the cells: about 6-7M cells for 64 clouds), h = 2L/N.  Returns a
+    list of (h, xp, qp)."""
+    rng = np.random.default_rng(seed)
+    g = np.asarray(grids, dtype=np.float64)
+    prob = (1.0 / g ** 3) / np.sum(1.0 / g ** 3)
+    out = []
+    for _ in range(count):
+        R = rng.uniform(0.5, 1.5)
+        sigma = rng.uniform(0.05, 0.15)
+        a = rng.standard_normal(3)
+        a /= np.linalg.norm(a)
+        N = int(rng.choice(grids, p=prob))
+        L = R + 4.0 * sigma
+        h = 2.0 * L / N
+        pts = cell_centres(-L, L, h)
+        w = ring_vorticity(pts, R=R, sigma=sigma, axis=a)
+        xp, qp = particles_from_vorticity(pts, w, h ** 3, threshold, box_corners=(-L, L))
+        out.append((h, xp, qp))
+    return out
+
+
+def blob_cloud(n=10_000_000, h=1.0 / 64, half=4.0, seed=5):
+    """Random vortex blobs: positions uniform in the box [-half, half]^3 (its inner part: half
+    minus the projection margin of 2h), strengths N(0, 1) per component, QP(1:3) = -w h^3,
+    QP(4) = h^3; 8 zero-strength corner particles fix the box at [-half, half]^3."""
+    rng = np.random.default_rng(seed)
+    lo, hi = -half + 2 * h, half - 2 * h
+    xp = rng.uniform(lo, hi, size=(n, 3))
+    w = rng.standard_normal((n, 3))
+    vol = h ** 3
+    qp = np.empty((n, 4))
+    qp[:, :3] = -w * vol
+    qp[:, 3] = vol
+    cs = np.array([[x, y, z] for z in (-half, half) for y in (-half, half) for x in (-half, half)], dtype=np.float64)
+    cq = np.zeros((8, 4))
+    cq[:, 3] = vol
+    return np.concatenate([cs, xp]), np.concatenate([cq, qp])

@@ -7,8 +7,11 @@
 // stencil covers it, so no floating-point atomics are used and the result does not
 // depend on scheduling.
 #include <algorithm>
+#include <cstdio>
+
 #include <hip/hip_ext.h>
 
+#include "tune.h"
 #include "vpmx_internal.h"
 #include "vpmx_ctx.h"
 
@@ -136,6 +139,14 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_out(const int *in, int *o
 
 __global__ void k_zero_int(int *p, int64_t n);
 
+// Particle count of a launch: n, or the count the device holds at *dn (at most n, the
+// capacity the grid was sized for) in the device-count mode of captured steps.
+__device__ __forceinline__ int64_t pcount(int64_t n, const int *dn) {
+    if (!dn) return n;
+    const int64_t d = *dn;
+    return d < n ? d : n;
+}
+
 int exclusive_scan(Ctx *c, const int *in, int *out, int64_t n) {
     if (n <= 0) {
         hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, c->stream, out, (int64_t)1);
@@ -161,9 +172,10 @@ __global__ void k_zero_int(int *p, int64_t n) {
 
 // Cell of a particle: inode = int((x - X0)/DX) + 1 (pmproject.f90:59-61), kept when
 // inode is in [-1, n+1] in every direction (any node of its stencil can be on the grid).
-__global__ void k_cell_keys(Grid g, const double *xp, int64_t n, int cnx, int cny, int cnz, int *keys, int *count) {
+__global__ void k_cell_keys(Grid g, const double *xp, int64_t n, const int *dn, int cnx, int cny, int cnz, int *keys,
+                            int *count) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= pcount(n, dn)) return;
     int ci[3];
     bool ok = true;
     for (int d = 0; d < 3; d++) {
@@ -180,9 +192,9 @@ __global__ void k_cell_keys(Grid g, const double *xp, int64_t n, int cnx, int cn
     }
     keys[p] = key;
 }
-__global__ void k_cell_scatter(const int *keys, int64_t n, const int *start, int *fill, int *perm) {
+__global__ void k_cell_scatter(const int *keys, int64_t n, const int *dn, const int *start, int *fill, int *perm) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= pcount(n, dn)) return;
     int key = keys[p];
     if (key < 0) return;
     int pos = start[key] + atomicAdd(&fill[key], 1);
@@ -220,10 +232,10 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
     hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, count, s.ncells);
     hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, fill, s.ncells);
     if (n > 0)
-        hipLaunchKernelGGL(k_cell_keys, dim3(gp), dim3(256), 0, c->stream, g, xp, n, s.cn[0], s.cn[1], s.cn[2], keys,
-                           count);
+        hipLaunchKernelGGL(k_cell_keys, dim3(gp), dim3(256), 0, c->stream, g, xp, n, c->dn, s.cn[0], s.cn[1], s.cn[2],
+                           keys, count);
     VPMX_TRY(exclusive_scan(c, count, start, s.ncells));
-    if (n > 0) hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(256), 0, c->stream, keys, n, start, fill, perm);
+    if (n > 0) hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(256), 0, c->stream, keys, n, c->dn, start, fill, perm);
     hipLaunchKernelGGL(k_cell_order, dim3(gc), dim3(256), 0, c->stream, start, count, s.ncells, perm);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
@@ -289,112 +301,6 @@ __device__ __forceinline__ double m4p(double x) {
     return r;
 }
 
-// Particle data in cell order (positions, NC strengths), for coalesced reads in k_p2m_block.
-template <int NC>
-__global__ void k_sorted_copy(const int *perm, const int *total, int64_t n, const double *xp, const double *qp,
-                              int qstride, double *xs, double *qs) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || s >= (int64_t)*total) return;
-    const int64_t p = perm[s];
-#pragma unroll
-    for (int d = 0; d < 3; d++) xs[3 * s + d] = xp[3 * p + d];
-#pragma unroll
-    for (int r = 0; r < NC; r++) qs[NC * s + r] = qp[(int64_t)qstride * p + r];
-}
-
-// M4' projection, one thread per block of BX x BY x BZ view nodes.  A particle in cell ci
-// (inode) reaches nodes ci-1..ci+2, so the block's nodes i0..i0+BX-1 collect the cells
-// i0-2..i0+BX; cells are visited in (k, j, i) order and the particles of a cell in their
-// sorted order, so every node sums its particles in the same order as k_p2m_gather (and
-// gets the same bits).  The x extent of each cell's reach is resolved at compile time;
-// in y and z both block nodes are evaluated (M4' is exactly 0 beyond its support).
-template <int NC, int BX, int BY, int BZ>
-__global__ void __launch_bounds__(256) k_p2m_block(Grid g, const int *start, int cnx, int cny, const double *xs,
-                                                   const double *qs, double div, GView out, int vnx, int vny, int vnz,
-                                                   int wlo0, int wlo1, int wlo2) {
-#pragma clang fp contract(off)
-    const int nbx = (vnx + BX - 1) / BX, nby = (vny + BY - 1) / BY, nbz = (vnz + BZ - 1) / BZ;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nbx * nby * nbz) return;
-    const int bx = (int)(t % nbx), by = (int)((t / nbx) % nby), bz = (int)(t / ((int64_t)nbx * nby));
-    const int a0 = bx * BX, b0 = by * BY, c0 = bz * BZ;
-    const int i0 = a0 - wlo0 + 1, j0 = b0 - wlo1 + 1, k0 = c0 - wlo2 + 1;
-    double acc[BZ][BY][BX][NC];
-#pragma unroll
-    for (int c = 0; c < BZ; c++)
-#pragma unroll
-        for (int b = 0; b < BY; b++)
-#pragma unroll
-            for (int a = 0; a < BX; a++)
-#pragma unroll
-                for (int r = 0; r < NC; r++) acc[c][b][a][r] = 0.0;
-    const double X = g.x0[0], Y = g.x0[1], Z = g.x0[2], hx = g.h[0], hy = g.h[1], hz = g.h[2];
-    double ox[BX], oy[BY], oz[BZ];
-#pragma unroll
-    for (int a = 0; a < BX; a++) ox[a] = (double)(i0 + a - 1) * hx;
-#pragma unroll
-    for (int b = 0; b < BY; b++) oy[b] = (double)(j0 + b - 1) * hy;
-#pragma unroll
-    for (int c = 0; c < BZ; c++) oz[c] = (double)(k0 + c - 1) * hz;
-    for (int cz = 0; cz < BZ + 3; cz++) {
-        const int ck = k0 - 2 + cz;
-        if (ck < -1 || ck > g.n[2] + 1) continue;
-        for (int cy = 0; cy < BY + 3; cy++) {
-            const int cj = j0 - 2 + cy;
-            if (cj < -1 || cj > g.n[1] + 1) continue;
-            const int64_t row = (int64_t)cnx * ((cj + 1) + (int64_t)cny * (ck + 1));
-#pragma unroll
-            for (int cx = 0; cx < BX + 3; cx++) {
-                const int ci = i0 - 2 + cx;
-                if (ci < -1 || ci > g.n[0] + 1) continue;
-                constexpr int dummy = 0;
-                (void)dummy;
-                const int xlo = cx - 3 > 0 ? cx - 3 : 0, xhi = cx < BX - 1 ? cx : BX - 1;
-                const int p0 = start[row + ci + 1], p1 = start[row + ci + 2];
-                for (int sidx = p0; sidx < p1; sidx++) {
-                    const double px = xs[3 * (int64_t)sidx], py = xs[3 * (int64_t)sidx + 1],
-                                 pz = xs[3 * (int64_t)sidx + 2];
-                    double q[NC];
-#pragma unroll
-                    for (int r = 0; r < NC; r++) q[r] = qs[NC * (int64_t)sidx + r];
-                    double fx[BX], fy[BY], fz[BZ];
-#pragma unroll
-                    for (int a = 0; a < BX; a++) fx[a] = (a >= xlo && a <= xhi) ? m4p((px - X - ox[a]) / hx) : 0.0;
-#pragma unroll
-                    for (int b = 0; b < BY; b++) fy[b] = m4p((py - Y - oy[b]) / hy);
-#pragma unroll
-                    for (int c = 0; c < BZ; c++) fz[c] = m4p((pz - Z - oz[c]) / hz);
-#pragma unroll
-                    for (int c = 0; c < BZ; c++)
-#pragma unroll
-                        for (int b = 0; b < BY; b++)
-#pragma unroll
-                            for (int a = 0; a < BX; a++) {
-                                if (a < xlo || a > xhi) continue;
-                                const double f = fx[a] * fy[b] * fz[c];
-#pragma unroll
-                                for (int r = 0; r < NC; r++) acc[c][b][a][r] = acc[c][b][a][r] + f * q[r];
-                            }
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < BZ; c++)
-#pragma unroll
-        for (int b = 0; b < BY; b++)
-#pragma unroll
-            for (int a = 0; a < BX; a++) {
-                const int va = a0 + a, vb = b0 + b, vc = c0 + c;
-                if (va >= vnx || vb >= vny || vc >= vnz) continue;
-                const int i = i0 + a, j = j0 + b, k = k0 + c;
-                const bool in = i >= 1 && i <= g.n[0] && j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
-                const int64_t o = out.at(va, vb, vc);
-#pragma unroll
-                for (int r = 0; r < NC; r++) out.p[r * out.cs + o] = in ? acc[c][b][a][r] / div : 0.0;
-            }
-}
-
 template <typename T>
 __device__ __forceinline__ T m4p_t(T x) {
 #pragma clang fp contract(off)
@@ -405,17 +311,6 @@ __device__ __forceinline__ T m4p_t(T x) {
     return r;
 }
 
-// M4' projection, one thread per 4 consecutive x-nodes of the view.  Node i collects the
-// particles of cells i-2..i+1 (a particle in cell ci reaches nodes ci-1..ci+2), so the nodes
-// i0..i0+3 collect cells i0-2..i0+4 of each of the 16 (y, z) cell rows around them, and in
-// cell order the particles of such a run of cells are one contiguous range.  The 32 range
-// loads are issued together, and particles are fetched four at a time, so a thread waits on
-// memory a few times per row instead of once per cell and once per particle.  Per particle
-// t = (x - x0)/h once per axis and the node weights W(t - (i-1)) (M4' vanishes outside a
-// node's four cells, so no masks).  Every node sums its particles in row order (z outer,
-// y inner), cell and particle order within the row: deterministic, no atomics.
-// T: arithmetic of the weights and the sums (double: FP64 form; float: mixed form, the
-// offsets still formed in FP64).
 // Particles per cell row (the start table is too large for the caches; the row counts stay
 // in L2 and let the projection skip the start loads of empty rows).
 __global__ void k_row_counts(const int *__restrict__ start, int cnx, int64_t nrows, int *__restrict__ rowcnt) {
@@ -423,86 +318,13 @@ __global__ void k_row_counts(const int *__restrict__ start, int cnx, int64_t nro
     if (r < nrows) rowcnt[r] = start[(r + 1) * cnx] - start[r * cnx];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_p2m_rows(Grid g, const int *__restrict__ start,
-                                                  const int *__restrict__ rowcnt, int cnx, int cny,
-                                                  const double *__restrict__ xs, const double *__restrict__ qs, double div,
-                                                  GView out, int vnx, int vny, int vnz, int wlo0, int wlo1, int wlo2) {
-#pragma clang fp contract(off)
-    const int nq = (vnx + 3) >> 2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nq * vny * vnz) return;
-    const int qa = (int)(t % nq), vb = (int)((t / nq) % vny), vc = (int)(t / ((int64_t)nq * vny));
-    const int a0 = 4 * qa;
-    const int i0 = a0 - wlo0 + 1, j = vb - wlo1 + 1, k = vc - wlo2 + 1;
-    const double ihx = 1.0 / g.h[0], ihy = 1.0 / g.h[1], ihz = 1.0 / g.h[2];
-    const double X = g.x0[0], Y = g.x0[1], Z = g.x0[2];
-    T acc[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int r = 0; r < 3; r++) acc[a][r] = T(0);
-    const int clo = i0 - 2 > -1 ? i0 - 2 : -1;
-    const int chi = i0 + 4 < g.n[0] + 1 ? i0 + 4 : g.n[0] + 1;
-    int p0[16], p1[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int cj = j - 2 + (r & 3), ck = k - 2 + (r >> 2);
-        bool ok = clo <= chi && cj >= -1 && cj <= g.n[1] + 1 && ck >= -1 && ck <= g.n[2] + 1;
-        const int64_t rid = ok ? (cj + 1) + (int64_t)cny * (ck + 1) : 0;
-        ok = ok && rowcnt[rid] > 0;
-        const int64_t row = ok ? (int64_t)cnx * rid : 0;
-        p0[r] = ok ? start[row + clo + 1] : 0;
-        p1[r] = ok ? start[row + chi + 2] : 0;
-    }
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int jr = j - 1, kr = k - 1;
-        for (int s0 = p0[r]; s0 < p1[r]; s0 += 4) {
-            double px[4], py[4], pz[4], q0[4], q1[4], q2[4];
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int64_t sl = s0 + u < p1[r] ? s0 + u : s0;
-                px[u] = xs[3 * sl]; py[u] = xs[3 * sl + 1]; pz[u] = xs[3 * sl + 2];
-                q0[u] = qs[3 * sl]; q1[u] = qs[3 * sl + 1]; q2[u] = qs[3 * sl + 2];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-                if (s0 + u >= p1[r]) break;
-                const T fy = m4p_t<T>((T)((py[u] - Y) * ihy - (double)jr));
-                const T fz = m4p_t<T>((T)((pz[u] - Z) * ihz - (double)kr));
-                const double tx = (px[u] - X) * ihx - (double)(i0 - 1);
-#pragma unroll
-                for (int a = 0; a < 4; a++) {
-                    const T f = (m4p_t<T>((T)(tx - (double)a)) * fy) * fz;
-                    acc[a][0] = acc[a][0] + f * (T)q0[u];
-                    acc[a][1] = acc[a][1] + f * (T)q1[u];
-                    acc[a][2] = acc[a][2] + f * (T)q2[u];
-                }
-            }
-        }
-    }
-    const bool jk_in = j >= 1 && j <= g.n[1] && k >= 1 && k <= g.n[2];
-#pragma unroll
-    for (int a = 0; a < 4; a++) {
-        const int va = a0 + a;
-        if (va >= vnx) continue;
-        const int i = i0 + a;
-        const bool in = jk_in && i >= 1 && i <= g.n[0];
-        const int64_t o = out.at(va, vb, vc);
-#pragma unroll
-        for (int r = 0; r < 3; r++) out.p[r * out.cs + o] = in ? (double)acc[a][r] / div : 0.0;
-    }
-}
-
 // M4' projection on node tiles.  A work-group takes 16 x 4 x 4 nodes (one per thread); the
 // particles that reach them lie in 19 x 7 x 7 cells (the particles of cell c are visited by
 // nodes c-1..c+2), 49 cell rows whose particles are contiguous in cell order.  The rows'
 // particles are staged in LDS, each with the M4' weights of the 4 x 4 x 4 nodes that visit
 // it (W((x - x0)/h - (i - 1)) for node i), formed once per tile, in chunks of whole rows that
-// fit; every thread then sums its node's 16 rows, the four cells of each, in the order of
-// k_p2m_rows (rows z-outer, y-inner, cells and particles in cell order): deterministic, no
-// atomics.  A row with more particles than a chunk holds is read straight from memory in the
+// fit; every thread then sums its node's 16 rows, the four cells of each, rows z-outer,
+// y-inner, cells and particles in cell order: deterministic, no atomics.  A row with more particles than a chunk holds is read straight from memory in the
 // same order.  Tiles without particles are written by k_p2m_zero_tiles.
 constexpr int PT_X = 16, PT_Y = 4, PT_Z = 4;
 constexpr int PC_X = PT_X + 3, PC_Y = PT_Y + 3, PC_Z = PT_Z + 3;
@@ -698,7 +520,7 @@ __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict_
                 }
             }
             __syncthreads();
-            // this node's rows of the chunk: z-outer, y-inner, as in k_p2m_rows; in row (dy, dz)
+            // this node's rows of the chunk: z-outer, y-inner; in row (dy, dz)
             // and cell dc of its 4 x 4 x 4 neighbourhood the node is the (3 - dc, 3 - dy, 3 - dz)-th
             // visitor of the particles
 #pragma unroll
@@ -751,136 +573,88 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
                int itype, double div, GView out, const int vn[3], const int wlo[3], int kclass, bool f32,
                bool zero_empty, bool keep_zeros) {
     c->tile_map = false;
-    if ((!keep_zeros && out.p == c->zt_ptr) || (keep_zeros && !(ncomp == 3 && itype == 4 && c->p2m_block && c->p2m_tile)))
+    if ((!keep_zeros && out.p == c->zt_ptr) || (keep_zeros && !(ncomp == 3 && itype == 4)))
         c->zt_valid = false;  // another writer of that buffer, or a path that writes every node
     int64_t nv = (int64_t)vn[0] * vn[1] * vn[2];
     unsigned grid = (unsigned)((nv + 255) / 256);
     const int *start = (const int *)s.start.p, *perm = (const int *)s.perm.p;
     hipEvent_t e0, e1;
-    if (ncomp == 3 && itype == 4 && c->p2m_block) {
-        // cell-ordered copy of the particles, then the node-run kernel (or one thread per
-        // 4x2x2 node block)
-        constexpr int BX = 4, BY = 2, BZ = 2;
+    if (ncomp == 3 && itype == 4) {
+        // node tiles with the particles' records staged in LDS
         int st = VPMX_OK;
         const int64_t n = c->sort_n;
-        double *xs = (double *)ensure(c, c->sort.xs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
+        // buffers first (a growing one drains the stream), then the path, timed as a whole
+        // (records, row counts, tile list, zeros, tiles); the cell-ordered particle records
+        // use the space of the cell-ordered copy, which this path does not use
+        const size_t ws = f32 ? sizeof(float) : sizeof(double);
+        void *Rv = ensure(c, c->sort.xs, ws * 16 * (size_t)(n > 0 ? n : 1), &st);
         if (st) return st;
-        double *qs = (double *)ensure(c, c->sort.qs, sizeof(double) * 3 * (size_t)(n > 0 ? n : 1), &st);
+        const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
+        int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
         if (st) return st;
-        if (n > 0 && !c->p2m_tile)
-            hipLaunchKernelGGL(k_sorted_copy<3>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, perm,
-                               start + s.ncells, n, xp, qp, qstride, xs, qs);
-        if (c->p2m_tile) {
-            // buffers first (a growing one drains the stream), then the path, timed as a whole
-            // (records, row counts, tile list, zeros, tiles); the cell-ordered particle records
-            // use the space of the cell-ordered copy, which this path does not use
-            const size_t ws = f32 ? sizeof(float) : sizeof(double);
-            void *Rv = ensure(c, c->sort.xs, ws * 16 * (size_t)(n > 0 ? n : 1), &st);
-            if (st) return st;
-            const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
-            int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
-            if (st) return st;
-            const int ntx0 = (vn[0] + PT_X - 1) / PT_X, nty0 = (vn[1] + PT_Y - 1) / PT_Y;
-            const int64_t tiles0 = (int64_t)ntx0 * nty0 * ((vn[2] + PT_Z - 1) / PT_Z);
-            if (!ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles0, &st)) return st;
-            if (!ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles0 + 1), &st)) return st;
-            if (!ensure(c, c->tile_list, sizeof(int) * (size_t)tiles0, &st)) return st;
-            if (keep_zeros && !ensure(c, c->zt_prev, sizeof(int) * (size_t)tiles0, &st)) return st;
-            const bool timed_all = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
-            if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
-            if (n > 0) {
-                const dim3 gs((unsigned)((n + 255) / 256));
-                if (f32)
-                    hipLaunchKernelGGL(k_p2m_sorted<float>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n, g,
-                                       xp, qp, qstride, (float *)Rv);
-                else
-                    hipLaunchKernelGGL(k_p2m_sorted<double>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n,
-                                       g, xp, qp, qstride, (double *)Rv);
-            }
-            hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
-                               s.cn[0], nrows, rowcnt);
-            const int ntx = (vn[0] + PT_X - 1) / PT_X, nty = (vn[1] + PT_Y - 1) / PT_Y;
-            const int64_t tiles = (int64_t)ntx * nty * ((vn[2] + PT_Z - 1) / PT_Z);
-            int *flag = (int *)ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles, &st); if (st) return st;
-            int *pos = (int *)ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles + 1), &st); if (st) return st;
-            int *list = (int *)ensure(c, c->tile_list, sizeof(int) * (size_t)tiles, &st); if (st) return st;
-            const unsigned gt = (unsigned)((tiles + 255) / 256);
-            hipLaunchKernelGGL(k_p2m_tile_flags, dim3(gt), dim3(256), 0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1],
-                               vn[2], g, s.cn[1], rowcnt, wlo[1], wlo[2], flag);
-            VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
-            hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
-            if (zero_empty) {
-                // buffers are all in place here, so alloc_gen is final for this call
-                const bool same = keep_zeros && c->zt_valid && out.p == c->zt_ptr && c->zt_gen == c->alloc_gen &&
-                                  std::equal(vn, vn + 3, c->zt_vn) && std::equal(wlo, wlo + 3, c->zt_wlo);
-                if (same)
-                    hipLaunchKernelGGL(k_p2m_zero_changed, dim3((unsigned)((tiles + ZT_TILES - 1) / ZT_TILES)), dim3(256),
-                                       0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1], vn[2], flag,
-                                       (const int *)c->zt_prev.p, out);
-                else
-                    hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx,
-                                       nty, vn[0], vn[1], vn[2], flag, out);
-                if (keep_zeros) {
-                    VPMX_HIP(c, hipMemcpyAsync(c->zt_prev.p, flag, sizeof(int) * (size_t)tiles, hipMemcpyDeviceToDevice,
-                                               c->stream));
-                    c->zt_valid = true;
-                    c->zt_ptr = out.p;
-                    std::copy(vn, vn + 3, c->zt_vn);
-                    std::copy(wlo, wlo + 3, c->zt_wlo);
-                    c->zt_gen = c->alloc_gen;
-                }
-            } else {
-                c->tile_map = true;
-                for (int d = 0; d < 3; d++) c->tile_map_vn[d] = vn[d];
-                c->tile_map_ntx = ntx;
-                c->tile_map_nty = nty;
-            }
-            const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
-            int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
-            const int *lst = list, *nl = pos + tiles;
-            void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, &Rv, &div,
-                            &out, &vx, &vy, &vz, &w0, &w1, &w2, (void *)&lst, (void *)&nl};
-            VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
-            VPMX_HIP(c, hipGetLastError());
-            if (timed_all) VPMX_HIP(c, hipEventRecord(e1, c->stream));
-            return VPMX_OK;
+        const int ntx0 = (vn[0] + PT_X - 1) / PT_X, nty0 = (vn[1] + PT_Y - 1) / PT_Y;
+        const int64_t tiles0 = (int64_t)ntx0 * nty0 * ((vn[2] + PT_Z - 1) / PT_Z);
+        if (!ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles0, &st)) return st;
+        if (!ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles0 + 1), &st)) return st;
+        if (!ensure(c, c->tile_list, sizeof(int) * (size_t)tiles0, &st)) return st;
+        if (keep_zeros && !ensure(c, c->zt_prev, sizeof(int) * (size_t)tiles0, &st)) return st;
+        const bool timed_all = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
+        if (timed_all) VPMX_HIP(c, hipEventRecord(e0, c->stream));
+        if (n > 0) {
+            const dim3 gs((unsigned)((n + 255) / 256));
+            if (f32)
+                hipLaunchKernelGGL(k_p2m_sorted<float>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n, g,
+                                   xp, qp, qstride, (float *)Rv);
+            else
+                hipLaunchKernelGGL(k_p2m_sorted<double>, gs, dim3(256), 0, c->stream, perm, start + s.ncells, n,
+                                   g, xp, qp, qstride, (double *)Rv);
         }
-        if (c->p2m_rows) {
-            const int64_t nth = (int64_t)((vn[0] + 3) / 4) * vn[1] * vn[2];
-            const unsigned gr = (unsigned)((nth + 255) / 256);
-            const int64_t nrows = (int64_t)s.cn[1] * s.cn[2];
-            int *rowcnt = (int *)ensure(c, c->sort.rowcnt, sizeof(int) * (size_t)nrows, &st);
-            if (st) return st;
-            hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
-                               s.cn[0], nrows, rowcnt);
-            const bool timed = kclass >= 0 && kernel_events(c, kclass, &e0, &e1);
-            if (f32) {
-                if (timed)
-                    hipExtLaunchKernelGGL(k_p2m_rows<float>, dim3(gr), dim3(256), 0, c->stream, e0, e1, 0, g, start,
-                                          rowcnt, s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
-                else
-                    hipLaunchKernelGGL(k_p2m_rows<float>, dim3(gr), dim3(256), 0, c->stream, g, start, rowcnt, s.cn[0], s.cn[1],
-                                       xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
-            } else {
-                if (timed)
-                    hipExtLaunchKernelGGL(k_p2m_rows<double>, dim3(gr), dim3(256), 0, c->stream, e0, e1, 0, g, start,
-                                          rowcnt, s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
-                else
-                    hipLaunchKernelGGL(k_p2m_rows<double>, dim3(gr), dim3(256), 0, c->stream, g, start, rowcnt,
-                                       s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+        hipLaunchKernelGGL(k_row_counts, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, c->stream, start,
+                           s.cn[0], nrows, rowcnt);
+        const int ntx = (vn[0] + PT_X - 1) / PT_X, nty = (vn[1] + PT_Y - 1) / PT_Y;
+        const int64_t tiles = (int64_t)ntx * nty * ((vn[2] + PT_Z - 1) / PT_Z);
+        int *flag = (int *)ensure(c, c->tile_flag, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+        int *pos = (int *)ensure(c, c->tile_pos, sizeof(int) * (size_t)(tiles + 1), &st); if (st) return st;
+        int *list = (int *)ensure(c, c->tile_list, sizeof(int) * (size_t)tiles, &st); if (st) return st;
+        const unsigned gt = (unsigned)((tiles + 255) / 256);
+        hipLaunchKernelGGL(k_p2m_tile_flags, dim3(gt), dim3(256), 0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1],
+                           vn[2], g, s.cn[1], rowcnt, wlo[1], wlo[2], flag);
+        VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
+        hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, (int)tiles, flag, pos, list);
+        if (zero_empty) {
+            // buffers are all in place here, so alloc_gen is final for this call
+            const bool same = keep_zeros && c->zt_valid && out.p == c->zt_ptr && c->zt_gen == c->alloc_gen &&
+                              std::equal(vn, vn + 3, c->zt_vn) && std::equal(wlo, wlo + 3, c->zt_wlo);
+            if (same)
+                hipLaunchKernelGGL(k_p2m_zero_changed, dim3((unsigned)((tiles + ZT_TILES - 1) / ZT_TILES)), dim3(256),
+                                   0, c->stream, (int)tiles, ntx, nty, vn[0], vn[1], vn[2], flag,
+                                   (const int *)c->zt_prev.p, out);
+            else
+                hipLaunchKernelGGL(k_p2m_zero_tiles, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, c->stream, ntx,
+                                   nty, vn[0], vn[1], vn[2], flag, out);
+            if (keep_zeros) {
+                VPMX_HIP(c, hipMemcpyAsync(c->zt_prev.p, flag, sizeof(int) * (size_t)tiles, hipMemcpyDeviceToDevice,
+                                           c->stream));
+                c->zt_valid = true;
+                c->zt_ptr = out.p;
+                std::copy(vn, vn + 3, c->zt_vn);
+                std::copy(wlo, wlo + 3, c->zt_wlo);
+                c->zt_gen = c->alloc_gen;
             }
-            VPMX_HIP(c, hipGetLastError());
-            return VPMX_OK;
+        } else {
+            c->tile_map = true;
+            for (int d = 0; d < 3; d++) c->tile_map_vn[d] = vn[d];
+            c->tile_map_ntx = ntx;
+            c->tile_map_nty = nty;
         }
-        const int64_t nblk = (int64_t)((vn[0] + BX - 1) / BX) * ((vn[1] + BY - 1) / BY) * ((vn[2] + BZ - 1) / BZ);
-        const unsigned gb = (unsigned)((nblk + 255) / 256);
-        if (kclass >= 0 && kernel_events(c, kclass, &e0, &e1))
-            hipExtLaunchKernelGGL((k_p2m_block<3, BX, BY, BZ>), dim3(gb), dim3(256), 0, c->stream, e0, e1, 0, g, start,
-                                  s.cn[0], s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
-        else
-            hipLaunchKernelGGL((k_p2m_block<3, BX, BY, BZ>), dim3(gb), dim3(256), 0, c->stream, g, start, s.cn[0],
-                               s.cn[1], xs, qs, div, out, vn[0], vn[1], vn[2], wlo[0], wlo[1], wlo[2]);
+        const void *fn = f32 ? (const void *)k_p2m_tile<float> : (const void *)k_p2m_tile<double>;
+        int cnx = s.cn[0], cny = s.cn[1], vx = vn[0], vy = vn[1], vz = vn[2], w0 = wlo[0], w1 = wlo[1], w2 = wlo[2];
+        const int *lst = list, *nl = pos + tiles;
+        void *args[] = {(void *)&g, (void *)&start, (void *)&rowcnt, &cnx, &cny, &Rv, &div,
+                        &out, &vx, &vy, &vz, &w0, &w1, &w2, (void *)&lst, (void *)&nl};
+        VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)tiles), dim3(256), args, 0, c->stream));
         VPMX_HIP(c, hipGetLastError());
+        if (timed_all) VPMX_HIP(c, hipEventRecord(e1, c->stream));
         return VPMX_OK;
     }
     if (ncomp == 3 && kclass >= 0 && kernel_events(c, kclass, &e0, &e1)) {
@@ -902,10 +676,10 @@ int p2m_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
 // back_to_particles_3D itype 1 (vpm_time.f90:30-107): UP = sum W vel, GP = (sum W SOL)*QP(neqpm+1).
 template <int ITYPEB>
 __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const double *qp, int qstride, int itype,
-                                              GView vel, GView sol, double *up, double *gp, int64_t n) {
+                                              GView vel, GView sol, double *up, double *gp, int64_t n, const int *dn) {
 #pragma clang fp contract(off)
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= pcount(n, dn)) return;
     const int ips = stencil_ips(itype), ipf = stencil_ipf(itype);
     double px = xp[3 * p], py = xp[3 * p + 1], pz = xp[3 * p + 2];
     double u0 = 0, u1 = 0, u2 = 0, g0 = 0, g1 = 0, g2 = 0;
@@ -1082,9 +856,9 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
 }
 
 // Particles outside the cell grid reach no node: UP = GP = 0.
-__global__ void k_m2p_outside(const int *__restrict__ keys, int64_t n, double *up, double *gp) {
+__global__ void k_m2p_outside(const int *__restrict__ keys, int64_t n, const int *dn, double *up, double *gp) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n || keys[p] >= 0) return;
+    if (p >= pcount(n, dn) || keys[p] >= 0) return;
     for (int q = 0; q < 3; q++) up[3 * p + q] = 0.0, gp[3 * p + q] = 0.0;
 }
 
@@ -1094,7 +868,7 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
     if (n <= 0) return VPMX_OK;
     hipEvent_t e0, e1;
     const dim3 grid((unsigned)((n + 255) / 256));
-    if (itypeb == 1 && c->m2p_tile && c->sort_n == n && s.cn[0] == g.n[0] + 3 && s.cn[1] == g.n[1] + 3 &&
+    if (itypeb == 1 && c->sort_n == n && s.cn[0] == g.n[0] + 3 && s.cn[1] == g.n[1] + 3 &&
         s.cn[2] == g.n[2] + 3) {
         const int ntx = (s.cn[0] + M2P_TX - 1) / M2P_TX, nty = (s.cn[1] + M2P_TY - 1) / M2P_TY,
                   ntz = (s.cn[2] + M2P_TZ - 1) / M2P_TZ;
@@ -1108,43 +882,55 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         // timed as a whole (tile list and the tiles)
         const bool timed = kernel_events(c, K_M2P, &e0, &e1);
         if (timed) VPMX_HIP(c, hipEventRecord(e0, c->stream));
-        hipLaunchKernelGGL(k_m2p_outside, grid, dim3(256), 0, c->stream, keys, n, up, gp);
+        hipLaunchKernelGGL(k_m2p_outside, grid, dim3(256), 0, c->stream, keys, n, c->dn, up, gp);
         hipLaunchKernelGGL(k_m2p_tile_flags, dim3((unsigned)(((int64_t)tiles * 32 + 255) / 256)), dim3(256), 0,
                            c->stream, tiles, start, s.cn[0], s.cn[1], s.cn[2], flag);
         VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
         hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, tiles, flag, pos, list);
-        // persistent: the work-groups one CU holds (LDS-bound: 40 KB of FP64 node values, 20 KB
-        // of FP32 in the mixed form)
+        // persistent: the work-groups per CU the tuner picks, at most what one CU holds (LDS-bound:
+        // 40 KB of FP64 node values, 20 KB of FP32 in the mixed form); the kernel's results do
+        // not depend on the grid, so the tuner's trials are real launches
         const bool f32 = c->precision == 1;
         const void *fn = f32 ? (const void *)k_m2p_tile<float> : (const void *)k_m2p_tile<double>;
-        const unsigned wg = (unsigned)((f32 ? 8 : 4) * c->num_cu);
         int gn = (int)s.cn[0], gny = (int)s.cn[1], gnz = (int)s.cn[2];
         const int *nl = pos + tiles;
         void *args[] = {(void *)&g, (void *)&start, (void *)&perm, &gn, &gny, &gnz, (void *)&list, (void *)&nl,
                         (void *)&xp, (void *)&qp, &qstride, &itype, &vel, &sol, &up, &gp};
+        char key[128];
+        std::snprintf(key, sizeof key, "interpolation tiles %s cells=%dx%dx%d", f32 ? "FP32" : "FP64", gn, gny, gnz);
+        const char *names[1] = {"node tiles in LDS"};
+        Ctx::Tuned ch;
+        auto trial = [&](int, int pc) -> int {
+            VPMX_HIP(c, hipLaunchKernel(fn, dim3((unsigned)(pc * c->num_cu)), dim3(256), args, 0, c->stream));
+            return VPMX_OK;
+        };
+        VPMX_TRY(tune(c, key, 1, &fn, names, 0, 256, trial, &ch));
+        const unsigned wg = (unsigned)(ch.per_cu * c->num_cu);
         VPMX_HIP(c, hipLaunchKernel(fn, dim3(wg), dim3(256), args, 0, c->stream));
         VPMX_HIP(c, hipGetLastError());
         if (timed) VPMX_HIP(c, hipEventRecord(e1, c->stream));
         return VPMX_OK;
     }
     if (itypeb == 2)
-        hipLaunchKernelGGL(k_m2p<2>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
+        hipLaunchKernelGGL(k_m2p<2>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n,
+                           c->dn);
     else if (kernel_events(c, K_M2P, &e0, &e1))
         hipExtLaunchKernelGGL(k_m2p<1>, grid, dim3(256), 0, c->stream, e0, e1, 0, g, xp, qp, qstride, itype, vel, sol,
-                              up, gp, n);
+                              up, gp, n, c->dn);
     else
-        hipLaunchKernelGGL(k_m2p<1>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n);
+        hipLaunchKernelGGL(k_m2p<1>, grid, dim3(256), 0, c->stream, g, xp, qp, qstride, itype, vel, sol, up, gp, n,
+                           c->dn);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
 
 // ------------------------------------------------------------------ updates
 // x = x0 + (u + uinf)*dt ; q(1:3) = q0(1:3) - g*dt  (test.f90:104-117 with FACDEF = 1)
-__global__ void k_euler(double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, double dt,
-                        double ux, double uy, double uz, const double *x0, const double *q0) {
+__global__ void k_euler(double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, const int *dn,
+                        double dt, double ux, double uy, double uz, const double *x0, const double *q0) {
 #pragma clang fp contract(off)
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    if (p >= pcount(n, dn)) return;
     double ui[3] = {ux, uy, uz};
     for (int d = 0; d < 3; d++) xp[3 * p + d] = x0[3 * p + d] + (up[3 * p + d] + ui[d]) * dt;
     for (int d = 0; d < 3; d++) qp[(int64_t)qs * p + d] = q0[(int64_t)qs * p + d] - gp[3 * p + d] * dt;
@@ -1153,8 +939,8 @@ __global__ void k_euler(double *xp, double *qp, int qs, const double *up, const 
 int euler_update(Ctx *c, double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, double dt,
                  const double uinf[3], const double *x0, const double *q0) {
     if (n <= 0) return VPMX_OK;
-    hipLaunchKernelGGL(k_euler, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xp, qp, qs, up, gp, n, dt,
-                       uinf[0], uinf[1], uinf[2], x0, q0);
+    hipLaunchKernelGGL(k_euler, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xp, qp, qs, up, gp, n, c->dn,
+                       dt, uinf[0], uinf[1], uinf[2], x0, q0);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
@@ -1302,6 +1088,10 @@ __global__ void k_remesh_write(GView w, int lx, int ly, int lz, int nx, int ny, 
     qo[(int64_t)nr * p + neqpm] = dv;
 }
 
+__global__ void k_copy_count(const int *src, int *dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr, const double x0[3],
                    const double h[3], double dv, int neqpm, double **xp_out, double **qp_out, DBuf &bx, DBuf &bq,
                    int64_t *count) {
@@ -1318,10 +1108,26 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
                        tm ? (const int *)c->tile_flag.p : nullptr, c->tile_map_ntx, c->tile_map_nty);
     c->tile_map = false;
     VPMX_TRY(exclusive_scan(c, flag, pos, nt));
+    int nr = neqpm + 1;
+    if (c->dn) {
+        // device-count mode: the kept nodes go straight into the resident arrays (their
+        // capacity holds every node of the window: checked when the mode was set up) and the
+        // count to the device, with no drain; the projection that read the particles precedes
+        // this on the stream
+        if (nt > c->cap) return fail(c, VPMX_ERR_STATE, "remesh window exceeds the particle capacity");
+        double *xo = (double *)c->xp.p, *qo = (double *)c->qp.p;
+        hipLaunchKernelGGL(k_remesh_write, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, flag,
+                           pos, x0[0], x0[1], x0[2], h[0], h[1], h[2], dv, neqpm, xo, qo);
+        hipLaunchKernelGGL(k_copy_count, dim3(1), dim3(64), 0, c->stream, pos + nt, c->dn);
+        VPMX_HIP(c, hipGetLastError());
+        *xp_out = xo;
+        *qp_out = qo;
+        *count = -1;
+        return VPMX_OK;
+    }
     int total = 0;
     VPMX_HIP(c, hipMemcpyAsync(&total, pos + nt, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));
-    int nr = neqpm + 1;
     double *xo = (double *)ensure(c, bx, sizeof(double) * 3 * (size_t)(total > 0 ? total : 1), &st); if (st) return st;
     double *qo = (double *)ensure(c, bq, sizeof(double) * nr * (size_t)(total > 0 ? total : 1), &st); if (st) return st;
     hipLaunchKernelGGL(k_remesh_write, dim3(gr), dim3(256), 0, c->stream, w, lo[0], lo[1], lo[2], nx, ny, nz, flag, pos,

@@ -15,12 +15,14 @@
 // leaving LDS, so one solve costs 5 sweeps over the interior.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <map>
 #include <type_traits>
 
 #include <hip/hip_ext.h>
 
 #include "fft_lds.h"
+#include "tune.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -1356,7 +1358,7 @@ static int launch_pass3(Ctx *c, DstArgs a, int ncomp, int rc) {
     const size_t lds = sizeof(double2) * (size_t)(np + 1) * a.L;
     // persistent grid: the work-groups the device holds at once, batches looped inside
     const int64_t nbatch = (a.nlines + 2 * np - 1) / (2 * np);
-    const int per_cu = c->dst_wg_per_cu > 0 ? c->dst_wg_per_cu : 1;
+    const int per_cu = 1;
     int64_t resident = (int64_t)c->num_cu * per_cu / (ncomp > 0 ? ncomp : 1);
     if (resident < 1) resident = 1;
     dim3 grid((unsigned)(nbatch < resident ? nbatch : resident), ncomp);
@@ -1405,52 +1407,85 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (sizeof(T) == 8 && fp->L == 288 && same(r288, 5)) fix = 2;
     if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
     if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
-    const void *fn = nullptr;
+    // kernel variants: with a compile-time plan the registers capped for four waves per SIMD
+    // (six for the FP32 x inverse of the 576-point plan) or not -- the tuner times both
+    const void *fns[2] = {nullptr, nullptr};
+    const char *names[2] = {"uncapped", "capped"};
+    int nf = 1;
     constexpr bool dbl = sizeof(T) == 8;
     // (each length's plan is instantiated only for the form that uses it)
     constexpr int c2 = wave_cap(dbl, MODE, 2), c3 = wave_cap(dbl, MODE, 3), c0 = wave_cap(dbl, MODE, 0),
                   c5 = wave_cap(dbl, MODE, 5);
+    constexpr int k2 = c2 ? c2 : 4, k3 = c3 ? c3 : 4, k0 = c0 ? c0 : 4, k5 = c5 ? c5 : 4;
     if constexpr (dbl) {
         if (fix == 2) {
-            if constexpr (c2) fn = (const void *)k_dst_wave_cap<c2, MODE, 4, T, 2>;
-            else fn = (const void *)k_dst_wave<MODE, 4, T, 2>;
+            fns[0] = (const void *)k_dst_wave<MODE, 4, T, 2>;
+            fns[1] = (const void *)k_dst_wave_cap<k2, MODE, 4, T, 2>;
+            nf = 2;
         } else if (fix == 3) {
-            if constexpr (c3) fn = (const void *)k_dst_wave_cap<c3, MODE, 17, T, 3>;
-            else fn = (const void *)k_dst_wave<MODE, 17, T, 3>;
+            fns[0] = (const void *)k_dst_wave<MODE, 17, T, 3>;
+            fns[1] = (const void *)k_dst_wave_cap<k3, MODE, 17, T, 3>;
+            nf = 2;
         }
     } else {
         if (fix == 0) {
-            if constexpr (c0) fn = (const void *)k_dst_wave_cap<c0, MODE, 4, T, 0>;
-            else fn = (const void *)k_dst_wave<MODE, 4, T, 0>;
+            fns[0] = (const void *)k_dst_wave<MODE, 4, T, 0>;
+            fns[1] = (const void *)k_dst_wave_cap<k0, MODE, 4, T, 0>;
+            nf = 2;
         } else if (fix == 5) {
-            if constexpr (c5) fn = (const void *)k_dst_wave_cap<c5, MODE, 17, T, 5>;
-            else fn = (const void *)k_dst_wave<MODE, 17, T, 5>;
+            fns[0] = (const void *)k_dst_wave<MODE, 17, T, 5>;
+            fns[1] = (const void *)k_dst_wave_cap<k5, MODE, 17, T, 5>;
+            nf = 2;
         }
     }
-    if (!fn) {
+    if (!fns[0]) {
         fix = -1;
-        fn = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
-           : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
-           : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
+        nf = 1;
+        fns[0] = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
+               : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
+               : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
     }
     // eight padded sequences and the twiddles (the kernel's LDS layout)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const int S = wave_pad_shift(sizeof(T) == 8, MODE, fix);
     const int Np = S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
     const size_t lds = sizeof(C) * (size_t)(8 * Np + a.L);
-    // persistent grid: as many work-groups as the device holds at once
-    int per_cu = c->dst_wg_per_cu;
-    if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
-        per_cu = 1;
-    if (per_cu < 1) per_cu = 1;
-    // every component's batches share one grid, so it fills the device whatever ncomp is
-    int64_t resident = (int64_t)c->num_cu * per_cu;
-    // multiple of 16 work-groups (batch pairs on one XCD), no more than the batches need
-    resident = resident / 16 * 16;
-    if (resident < 16) resident = 16;
-    const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
     a.ncomp = ncomp;
-    dim3 grid((unsigned)(need < resident ? need : resident));
+    // persistent grid: every component's batches in one grid, per_cu resident work-groups per
+    // CU, a multiple of 16 (batch pairs on one XCD), no more than the batches need
+    auto grid_of = [&](int per_cu) {
+        int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
+        if (resident < 16) resident = 16;
+        const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
+        return dim3((unsigned)(need < resident ? need : resident));
+    };
+    // the tuner's trials run on scratch arrays (the x inverse pass writes the output, which its
+    // real launch then overwrites)
+    static const char *mname[4] = {"x forward", "y", "z", "x inverse"};
+    char key[160];
+    std::snprintf(key, sizeof key, "sine transforms %s pass %s L=%d m=%d lines=%lld comps=%d", mname[MODE],
+                  dbl ? "FP64" : "FP32", a.L, a.m, (long long)a.nlines, ncomp);
+    Ctx::Tuned ch;
+    {
+        int st = VPMX_OK;
+        const size_t wb = sizeof(T) * (size_t)a.wcs * ncomp;
+        auto trial = [&](int fi, int pc) -> int {
+            if (!c->tune_scratch.p || c->tune_scratch.bytes < wb) {
+                void *p = ensure(c, c->tune_scratch, wb, &st);
+                if (!p) return st;
+                VPMX_TRY(fill(c, (double *)p, (int64_t)(c->tune_scratch.bytes / sizeof(double)), 0.0));
+            }
+            DstArgs t = a;
+            t.w = t.wo = (double *)c->tune_scratch.p;
+            WavePlan Pt = P;
+            void *targs[] = {&t, &Pt};
+            VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
+            return VPMX_OK;
+        };
+        VPMX_TRY(tune(c, key, nf, fns, names, lds, DST_THREADS, trial, &ch));
+    }
+    const void *fn = fns[ch.fi];
+    dim3 grid = grid_of(ch.per_cu);
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
     constexpr int kc = MODE == DST_Y ? K_DST : K_DSTZ;
@@ -1522,15 +1557,40 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
     const int Np64 = S64 == 0 ? pad_len<0>(f64->L) : S64 == 3 ? pad_len<3>(f64->L) : pad_len<4>(f64->L);
     const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
     const size_t lds = sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
-    int per_cu = c->dst_wg_per_cu;
-    if (per_cu <= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, DST_THREADS, lds) != hipSuccess)
-        per_cu = 1;
-    if (per_cu < 1) per_cu = 1;
-    int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
-    if (resident < 16) resident = 16;
     const int64_t nbatch = (a.nlines + 15) / 16;
-    const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
-    dim3 grid((unsigned)(need < resident ? need : resident));
+    auto grid_of = [&](int per_cu) {
+        int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
+        if (resident < 16) resident = 16;
+        const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
+        return dim3((unsigned)(need < resident ? need : resident));
+    };
+    char key[160];
+    std::snprintf(key, sizeof key, "sine transforms z pass FP32/FP64 L=%d/%d m=%d lines=%lld comps=%d", f32->L, f64->L,
+                  a.m, (long long)a.nlines, ncomp);
+    const char *names[1] = {"FP32 forward, FP64 inverse"};
+    Ctx::Tuned ch;
+    {
+        int st = VPMX_OK;
+        // trials: FP32 input and FP64 output in scratch
+        const size_t wb = (sizeof(float) + sizeof(double)) * (size_t)a.wcs * ncomp;
+        auto trial = [&](int, int pc) -> int {
+            if (!c->tune_scratch.p || c->tune_scratch.bytes < wb) {
+                void *p = ensure(c, c->tune_scratch, wb, &st);
+                if (!p) return st;
+                VPMX_TRY(fill(c, (double *)p, (int64_t)(c->tune_scratch.bytes / sizeof(double)), 0.0));
+            }
+            DstArgs t = a;
+            t.w = (double *)c->tune_scratch.p;
+            t.wo = (double *)((char *)c->tune_scratch.p + sizeof(float) * (size_t)a.wcs * ncomp);
+            WavePlan p32 = P32, p64 = P64;
+            const double2 *t64 = f64->tw;
+            void *targs[] = {&t, &p32, &p64, &t64};
+            VPMX_HIP(c, hipLaunchKernel(fn, grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
+            return VPMX_OK;
+        };
+        VPMX_TRY(tune(c, key, 1, &fn, names, lds, DST_THREADS, trial, &ch));
+    }
+    dim3 grid = grid_of(ch.per_cu);
     const double2 *tw64 = f64->tw;
     WavePlan p32 = P32, p64 = P64;
     void *args[] = {&a, &p32, &p64, &tw64};

@@ -325,7 +325,15 @@ static int yaps_multi(Ctx *c, bool *used) {
     int *cmap_d = nullptr;
     VPMX_TRY(upload_cmap(c, cmap, &cmap_d));
     MbBlock *blk_d = (MbBlock *)ensure(c, c->mb_blocks, sizeof(MbBlock) * (size_t)NB, &st); if (st) return st;
-    VPMX_HIP(c, hipMemcpyAsync(blk_d, blk.data(), sizeof(MbBlock) * (size_t)NB, hipMemcpyHostToDevice, c->stream));
+    // the block table, uploaded when it or its buffer changes (from a copy the context keeps,
+    // so no call waits for the upload)
+    const size_t tb = sizeof(MbBlock) * (size_t)NB;
+    if (blk_d != c->mb_dev || c->mb_host.size() != tb || std::memcmp(c->mb_host.data(), blk.data(), tb) != 0) {
+        VPMX_HIP(c, hipStreamSynchronize(c->stream));  // an earlier upload may still read mb_host
+        c->mb_host.assign((const char *)blk.data(), (const char *)blk.data() + tb);
+        c->mb_dev = blk_d;
+        VPMX_HIP(c, hipMemcpyAsync(blk_d, c->mb_host.data(), tb, hipMemcpyHostToDevice, c->stream));
+    }
     VPMX_TRY(fill(c, samp, 3 * GC * NB, 0.0));
     VPMX_TRY(fill(c, rhsc, 3 * GC, 0.0));
     auto cview = [&](double *base, int b) { return view(base + (int64_t)b * 3 * GC, cn); };
@@ -370,7 +378,6 @@ static int yaps_multi(Ctx *c, bool *used) {
         GView sin = view(sol_bl + boff[b], wn, w[0] - 1, w[1] - 1, w[2] - 1, block_n(G, b));
         VPMX_TRY(poisson_dirichlet(c, 3, wn, h, rin, &sin, sin, 0, final_form(c)));
     }
-    VPMX_HIP(c, hipStreamSynchronize(c->stream));  // host tables go out of scope
     mark(c, ST_FINAL, 1);
     used[ST_FINAL] = true;
     return VPMX_OK;
@@ -571,6 +578,7 @@ static int run_remesh(Ctx *c, const double *xp, const double *qp, int64_t n, int
 
 static int upload_particles(Ctx *c, const double *xp, const double *qp, int64_t n, int neqpm) {
     int st = VPMX_OK;
+    if (c->dn) return fail(c, VPMX_ERR_STATE, "a prepared batch's cloud: its particles change only through the batch");
     if (n < 0 || n > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "particle count out of range");
     int nr = neqpm + 1;
     double *dx = (double *)ensure(c, c->xp, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
@@ -607,6 +615,63 @@ static int define_from_device(Ctx *c, int ntime) {
     return define_sizes_ctx(c, pts, 2, ntime);
 }
 
+static int replace_with_remesh(Ctx *c) {
+    // swap the compacted particles in as the resident set
+    std::swap(c->xp, c->rm_x);
+    std::swap(c->qp, c->rm_q);
+    c->np = c->rm_count;
+    int st = VPMX_OK;
+    ensure(c, c->up, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
+    ensure(c, c->gp, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
+    c->rm_count = -1;
+    c->rm_xp = c->rm_qp = nullptr;
+    return VPMX_OK;
+}
+
+// One RK2 (midpoint) step of the resident particles, queued on the context's stream: two
+// vpm(2) evaluations with the Euler updates of test.f90:104-117, then vpm(0) and
+// remesh_particles_3d(1) when do_remesh.  In the device-count mode the launches are sized for
+// the capacity, the count stays on the device and the remesh writes the resident arrays in
+// place; the ordinary mode swaps the remeshed particles in (their count is read back).
+int step_rk2_enqueue(Ctx *c, double dt, const double uinf[3], int ntime, int mrem, int do_remesh) {
+    int st = VPMX_OK;
+    const int64_t n = c->dn ? c->cap : c->np;
+    int nr = c->neqpm + 1;
+    double *x0 = (double *)ensure(c, c->x0, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
+    TimingScope timing(c);  // the stage times of the whole step
+    if (n > 0) {
+        VPMX_HIP(c, hipMemcpyAsync(x0, xp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(q0, qp, sizeof(double) * nr * n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    // stage 1 at (x0, q0)
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
+    // stage 2 at the midpoint
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, dt, uinf, x0, q0));
+    if (do_remesh) {
+        VPMX_TRY(define_from_device(c, ntime));
+        VPMX_TRY(run_remesh(c, xp, qp, n, mrem));
+        if (!c->dn) VPMX_TRY(replace_with_remesh(c));
+    }
+    return VPMX_OK;
+}
+
+// The particles of the remesh window of the context's geometry (mrem-fold grid): the most a
+// remesh can keep.
+int64_t remesh_window_nodes(const Ctx *c, int mrem) {
+    const vpmx_geom &G = c->geom;
+    const int half = c->prm.interf_iproj / 2;
+    int64_t n = 1;
+    for (int d = 0; d < 3; d++) {
+        const int lo = G.nn_bl[d] * mrem + half, hi = G.nn_bl[d + 3] * mrem - half;
+        n *= hi >= lo ? (int64_t)(hi - lo + 1) : 0;
+    }
+    return n;
+}
+
 }  // namespace vpmx
 
 using namespace vpmx;
@@ -629,28 +694,11 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
         return VPMX_ERR_HIP;
     }
     c->kev_ok = true;
-    const char *bm = getenv("VPMX_BOUNDSUM");
-    if (bm && std::string(bm) == "direct") c->bs_mode = 1;
-    if (bm && std::string(bm) == "lattice") c->bs_mode = 0;
-    const char *dm = getenv("VPMX_DST");
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         c->num_cu = ncu;
-    const char *wpc = getenv("VPMX_DST_WG_PER_CU");
-    if (wpc && atoi(wpc) > 0) c->dst_wg_per_cu = atoi(wpc);
-    const char *pb = getenv("VPMX_P2M");
-    if (pb && std::string(pb) == "node") c->p2m_block = false;
-    if (pb && std::string(pb) == "block") c->p2m_rows = false;
-    if (pb && (std::string(pb) == "rows" || std::string(pb) == "block" || std::string(pb) == "node")) c->p2m_tile = false;
-    const char *mb = getenv("VPMX_M2P");
-    if (mb && std::string(mb) == "particle") c->m2p_tile = false;
     const char *pm = getenv("VPMX_PRECISION");
     if (pm && std::string(pm) == "mixed") c->precision = 1;
-    if (dm && std::string(dm) == "v1") c->dst_kernel = 1;
-    if (dm && std::string(dm) == "v2") c->dst_kernel = 2;
-    if (dm && std::string(dm) == "v3") c->dst_kernel = 3;
-    const char *dr = getenv("VPMX_DST_REUSE");
-    if (dr && std::string(dr) == "0") c->dst_reuse = false;
 
     *out = c;
     return VPMX_OK;
@@ -715,6 +763,35 @@ int32_t vpmx_set_block_weights(vpmx_ctx *p, const int32_t *wx, const int32_t *wy
 int32_t vpmx_get_geometry(const vpmx_ctx *p, vpmx_geom *out) {
     if (!p || !out) return VPMX_ERR_ARG;
     *out = C(p)->geom;
+    return VPMX_OK;
+}
+
+int32_t vpmx_set_option(vpmx_ctx *p, const char *name, int32_t value) {
+    Ctx *c = C(p);
+    if (!c || !name) return VPMX_ERR_ARG;
+    const std::string k(name);
+    if (k == "boundsum") {
+        if (value < 0 || value > 2) return fail(c, VPMX_ERR_ARG, "boundsum: 0 lattice, 1 direct, 2 spectral");
+        c->bs_mode = value;
+    } else if (k == "dst_reuse") {
+        c->dst_reuse = value != 0;
+    } else if (k == "tuning") {
+        c->tuning = value != 0;
+        c->tuned.clear();
+    } else {
+        return fail(c, VPMX_ERR_ARG, "unknown option " + k);
+    }
+    return VPMX_OK;
+}
+
+int32_t vpmx_tuning_report(const vpmx_ctx *p, char *buf, int64_t capacity) {
+    if (!p || !buf || capacity < 1) return VPMX_ERR_ARG;
+    const Ctx *c = C(p);
+    std::string out;
+    for (const auto &kv : c->tuned) out += kv.first + " -> " + kv.second.what + "\n";
+    const size_t n = std::min((size_t)(capacity - 1), out.size());
+    std::memcpy(buf, out.data(), n);
+    buf[n] = 0;
     return VPMX_OK;
 }
 
@@ -864,22 +941,11 @@ int32_t vpmx_vpm_resident(vpmx_ctx *p, int32_t whattodo, int32_t ntime) {
     return VPMX_OK;
 }
 
-static int replace_with_remesh(Ctx *c) {
-    // swap the compacted particles in as the resident set
-    std::swap(c->xp, c->rm_x);
-    std::swap(c->qp, c->rm_q);
-    c->np = c->rm_count;
-    int st = VPMX_OK;
-    ensure(c, c->up, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
-    ensure(c, c->gp, sizeof(double) * 3 * (size_t)(c->np ? c->np : 1), &st); if (st) return st;
-    c->rm_count = -1;
-    c->rm_xp = c->rm_qp = nullptr;
-    return VPMX_OK;
-}
 
 int32_t vpmx_remesh_resident(vpmx_ctx *p, int32_t mrem) {
     Ctx *c = C(p);
     if (!c) return VPMX_ERR_ARG;
+    if (c->dn) return fail(c, VPMX_ERR_STATE, "a prepared batch's cloud: remesh through vpmx_batch_step_rk2");
     VPMX_HIP(c, hipSetDevice(c->device));
     VPMX_TRY(run_remesh(c, (const double *)c->xp.p, (const double *)c->qp.p, c->np, mrem));
     VPMX_TRY(replace_with_remesh(c));
@@ -891,28 +957,8 @@ int32_t vpmx_step_rk2(vpmx_ctx *p, double dt, const double uinf[3], int32_t ntim
     Ctx *c = C(p);
     if (!c || !uinf) return VPMX_ERR_ARG;
     VPMX_HIP(c, hipSetDevice(c->device));
-    int st = VPMX_OK;
-    int64_t n = c->np;
-    int nr = c->neqpm + 1;
-    double *x0 = (double *)ensure(c, c->x0, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
-    double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
-    double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
-    TimingScope timing(c);  // the stage times of the whole step
-    if (n > 0) {
-        VPMX_HIP(c, hipMemcpyAsync(x0, xp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
-        VPMX_HIP(c, hipMemcpyAsync(q0, qp, sizeof(double) * nr * n, hipMemcpyDeviceToDevice, c->stream));
-    }
-    // stage 1 at (x0, q0)
-    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
-    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
-    // stage 2 at the midpoint
-    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
-    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, dt, uinf, x0, q0));
-    if (do_remesh) {
-        VPMX_TRY(define_from_device(c, ntime));
-        VPMX_TRY(run_remesh(c, xp, qp, n, mrem));
-        VPMX_TRY(replace_with_remesh(c));
-    }
+    if (c->dn) return fail(c, VPMX_ERR_STATE, "this context belongs to a batch: step it with vpmx_batch_step_rk2");
+    VPMX_TRY(step_rk2_enqueue(c, dt, uinf, ntime, mrem, do_remesh));
     VPMX_HIP(c, hipStreamSynchronize(c->stream));
     return VPMX_OK;
 }

@@ -41,19 +41,36 @@ struct Ctx : vpmx_ctx {
     // resident particles
     DBuf xp, qp, up, gp, x0, q0;
     int64_t np = 0;
+    // device-count mode (steps captured in a graph): the particle count lives at dn on the
+    // device (remesh writes it there), every particle launch is sized for np = the capacity
+    // and reads the count; null in the ordinary mode, where np is the count
+    int *dn = nullptr;
+    int64_t cap = 0;  // particle capacity of the resident arrays in that mode
+    bool nodrain = false;  // queued as part of a batch (or captured): no stream drains, no timing events
+    bool capturing = false;  // a graph is being captured: nothing may wait on the stream
+
+    // autotuner: per launch key (pass, precision, lengths, sizes) the kernel variant and the
+    // resident work-groups per CU chosen by timing the candidates once
+    bool tuning = true;
+    struct Tuned {
+        int fi = 0, per_cu = 1;
+        double ms = 0.0;
+        std::string what;
+    };
+    std::map<std::string, Tuned> tuned;
+    DBuf tune_scratch;
     int neqpm = 3;
     double ni = 0.0;  // kinematic viscosity NI of the diffusion mode (resident calls)
 
     // grids
     DBuf rhs_bl, sol_bl, sol0, vel, str;
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c, panel_a;
-    int bs_mode = 2;  // fine boundary sums: 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
+    int bs_mode = 2;  // boundary sums (option boundsum): 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx[2] = {nullptr, nullptr};  // spectral boundary-sum plans: fine block, coarse grid
     void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
     void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
     int num_cu = 256;       // compute units of the device
-    int dst_wg_per_cu = 0;  // resident work-groups per CU of the persistent DST passes (0: occupancy query)
-    bool dst_reuse = true;  // second fine solve from the first one's forward transforms (VPMX_DST_REUSE=0: off)
+    bool dst_reuse = true;  // second fine solve from the first one's forward transforms (option dst_reuse)
     int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length
                           // work-group FFTs, 2 odd extension, 1 two-buffer
     std::map<int, void *> sin_cache;  // sin(j pi / N) per half-length FFT size N
@@ -62,6 +79,8 @@ struct Ctx : vpmx_ctx {
     std::vector<int> cmap_host;   // contents of cmap as last uploaded (kept alive for the async copy)
     const void *cmap_dev = nullptr;  // device address they were uploaded to
     DBuf rhs_g, sol_g, mb_blocks, mb_tmp;  // several yaps blocks: global RHS / SOL grids, block table
+    std::vector<char> mb_host;             // the block table as last uploaded
+    const void *mb_dev = nullptr;          // where it was uploaded to
     DBuf dst_work, lam[3];
     DBuf dst_work64;  // FP64 work array of the mixed form's accurate solves (after the diagonal)
     DBuf dst_save, fold_tab;  // forward-pass reuse: the kept work array, the face transforms
@@ -85,10 +104,7 @@ struct Ctx : vpmx_ctx {
     int zt_vn[3] = {0, 0, 0}, zt_wlo[3] = {0, 0, 0};
     int64_t zt_gen = -1;
     DBuf zt_prev;
-    bool p2m_block = true;  // projection: node-run / node-block kernels (else one thread per node)
-    bool p2m_rows = true;   // projection: 4 x-nodes per thread, cell-row walks (else 4x2x2 node blocks)
-    bool p2m_tile = true;   // projection: node tiles with the particles staged in LDS (else the above)
-    bool m2p_tile = true;   // interpolation: cell tiles with the nodes staged in LDS
+
     std::map<int, void *> fft_plans;
     bool have_grids = false;  // RHS/SOL/vel of the last mode 1-3 call are valid
 
@@ -134,7 +150,7 @@ inline void resolve_kernel_times(Ctx *c) {
 // Stage timing: an event on the stream at the stage's start and end (a stage may run several
 // times in one call; each run gets its own pair).
 inline void mark(Ctx *c, int stage, int end) {
-    if (!c->timing || !c->kev_ok || c->timing_depth == 0) return;
+    if (!c->timing || !c->kev_ok || c->timing_depth == 0 || c->nodrain) return;
     if (!end) {
         const int i = c->sev_used[stage];
         if (i == (int)c->sev0[stage].size()) {
@@ -166,6 +182,7 @@ struct TimingScope {
     }
     ~TimingScope() {
         if (--c->timing_depth != 0) return;
+        if (c->nodrain) return;  // (nothing was recorded; a drain is not allowed under capture)
         (void)hipStreamSynchronize(c->stream);
         for (int s = 0; s < ST_COUNT; s++) {
             for (int i = 0; i < c->sev_used[s]; i++) {
@@ -179,7 +196,7 @@ struct TimingScope {
 };
 // Start/stop events for the next timed dispatch of class k (false: time nothing).
 inline bool kernel_events(Ctx *c, int k, hipEvent_t *e0, hipEvent_t *e1) {
-    if (!c->kev_ok) return false;
+    if (!c->kev_ok || c->nodrain) return false;
     const int i = c->kev_pending[k];
     if (i >= 64) return false;  // more than 64 undrained dispatches: leave the rest untimed
     if (i == (int)c->kev0[k].size()) {

@@ -79,6 +79,11 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
 int euler_update(Ctx *c, double *xp, double *qp, int qstride, const double *up, const double *gp, int64_t n,
                  double dt, const double uinf[3], const double *x0, const double *q0);
 int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3]);
+// One RK2 step of the context's resident particles queued on its stream (no drain at the end).
+int step_rk2_enqueue(Ctx *c, double dt, const double uinf[3], int ntime, int mrem, int do_remesh);
+int64_t remesh_window_nodes(const Ctx *c, int mrem);
+
+
 int diagnostics(Ctx *c, const double *xp, const double *qp, int qstride, int64_t n, double circ[3], double imp[3]);
 int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr,
                    const double x0[3], const double h[3], double dv, int neqpm, double **xp_out, double **qp_out,

@@ -88,6 +88,8 @@ def load_library():
         "vpmx_set_params": [vp, C.POINTER(Params)],
         "vpmx_get_geometry": [vp, C.POINTER(Geom)],
         "vpmx_set_precision": [vp, i32],
+        "vpmx_set_option": [vp, C.c_char_p, i32],
+        "vpmx_tuning_report": [vp, C.c_char_p, i64],
         "vpmx_vpm": [vp, d, d, d, d, i64, i32, i32, i32, C.c_double],
         "vpmx_remesh_count": [vp, d, d, i64, i32, i32, C.POINTER(i64)],
         "vpmx_remesh_fetch": [vp, d, d],
@@ -99,6 +101,11 @@ def load_library():
         "vpmx_remesh_resident": [vp, i32],
         "vpmx_step_rk2": [vp, C.c_double, d, i32, i32, i32],
         "vpmx_diagnostics": [vp, d, d],
+        "vpmx_batch_create": [C.POINTER(vp), i32, i32],
+        "vpmx_batch_destroy": [vp],
+        "vpmx_batch_cloud": [vp, i32, C.POINTER(vp)],
+        "vpmx_batch_step_rk2": [vp, C.c_double, d, i32, i32, i32],
+        "vpmx_batch_times": [vp, d, d],
         "vpmx_stage_solve": [vp, d, d, C.POINTER(i32), i32, C.POINTER(i32), d, d, i32, d],
         "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
         "vpmx_stage_pmesh_inf_lev": [vp, d, d, C.POINTER(i32), i32, d, d, i32, i32, i32],
@@ -113,8 +120,11 @@ def load_library():
         f.restype = i32
     lib.vpmx_destroy.argtypes = [vp]
     lib.vpmx_destroy.restype = None
+    lib.vpmx_batch_destroy.restype = None
     lib.vpmx_last_error.argtypes = [vp]
     lib.vpmx_last_error.restype = C.c_char_p
+    lib.vpmx_batch_last_error.restype = C.c_char_p
+    lib.vpmx_batch_last_error.argtypes = [vp]
     _lib = lib
     return lib
 
@@ -183,6 +193,15 @@ def geom_dict(g):
 class VPM:
     """One device context: vpm() modes, remesh_particles_3d(), device-resident stepping."""
 
+    @classmethod
+    def _borrow(cls, lib, handle, params):
+        """A view of a context owned elsewhere (a batch's cloud); closing it does nothing."""
+        c = cls.__new__(cls)
+        c._lib, c._h, c._owned, c.neqpm = lib, handle, False, 3
+        c.params = params
+        c._check(c._lib.vpmx_set_params(c._h, C.byref(params)))
+        return c
+
     def __init__(self, params, device=0):
         self._lib = load_library()
         h = C.c_void_p()
@@ -199,13 +218,23 @@ class VPM:
         self._w = [_weights(wx), _weights(wy), _weights(wz)]  # kept alive for the call
         self._check(self._lib.vpmx_set_block_weights(self._h, _wp(self._w[0]), _wp(self._w[1]), _wp(self._w[2])))
 
+    def set_option(self, name, value):
+        """boundsum (0 lattice, 1 direct, 2 spectral), dst_reuse (0/1), tuning (0/1)."""
+        self._check(self._lib.vpmx_set_option(self._h, name.encode(), int(value)))
+
+    def tuning_report(self):
+        buf = C.create_string_buffer(1 << 16)
+        self._check(self._lib.vpmx_tuning_report(self._h, buf, len(buf)))
+        return buf.value.decode()
+
     def set_precision(self, mode):
         """0: plain FP64 form; 1: mixed form (FP32 Poisson passes, FP64 boundary sums)."""
         self._check(self._lib.vpmx_set_precision(self._h, int(mode)))
 
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.vpmx_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self._lib.vpmx_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -349,3 +378,59 @@ class VPM:
         self._check(self._lib.vpmx_stage_project(self._h, _dp(xb), _dp(dp), nn_p, nb_p, _dp(XP), _dp(QP),
                                                  XP.shape[0], QP.shape[1], itype, idvpm, _dp(out)))
         return out
+
+
+class Batch:
+    """Many independent clouds stepped together (vpmx_batch_*); cloud(i) is a VPM view of
+    cloud i's context for its set-up (parameters, particles, geometry, precision)."""
+
+    def __init__(self, nclouds, device=0):
+        self._lib = load_library()
+        h = C.c_void_p()
+        r = self._lib.vpmx_batch_create(C.byref(h), device, int(nclouds))
+        if r != OK:
+            raise VpmError(r, "vpmx_batch_create failed")
+        self._h = h
+        self.n = int(nclouds)
+        self._views = {}
+
+    def _check(self, r):
+        if r != OK:
+            raise VpmError(r, self._lib.vpmx_batch_last_error(self._h).decode(errors="replace"))
+
+    def cloud(self, i, params=None):
+        if i not in self._views:
+            h = C.c_void_p()
+            self._check(self._lib.vpmx_batch_cloud(self._h, int(i), C.byref(h)))
+            self._views[i] = VPM._borrow(self._lib, h, params)
+        elif params is not None:
+            self._views[i].params = params
+            self._views[i]._check(self._lib.vpmx_set_params(self._views[i]._h, C.byref(params)))
+        return self._views[i]
+
+    def step_rk2(self, dt, uinf=(0.0, 0.0, 0.0), mrem=1, remesh=True, graph=False):
+        u = np.asarray(uinf, dtype=np.float64)
+        self._check(self._lib.vpmx_batch_step_rk2(self._h, float(dt), _dp(u), int(mrem), 1 if remesh else 0,
+                                                  1 if graph else 0))
+
+    def times(self):
+        """(device ms of the last step, host ms spent queueing it)."""
+        a, b = np.zeros(1), np.zeros(1)
+        self._check(self._lib.vpmx_batch_times(self._h, _dp(a), _dp(b)))
+        return float(a[0]), float(b[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for v in self._views.values():
+                v._h = None
+            self._lib.vpmx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
