@@ -157,6 +157,18 @@ int32_t vpmx_step_rk2(vpmx_ctx *ctx, double dt, const double uinf[3], int32_t nt
 /* FP64 diagnostics of the resident particles: circ = -sum QP(1:3),
  * imp = 1/2 sum x x (-QP(1:3)). */
 int32_t vpmx_diagnostics(vpmx_ctx *ctx, double circ[3], double imp[3]);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the diagnostics of
+ * vpmx_diagnostics on caller-provided DEVICE arrays xp (3 x n), qp (qstride x n, qstride >= 3)
+ * into the device array out[6] = (circ[0..2], imp[0..2]), with the partial sums in a fixed
+ * device array of the library (48 KiB per GPU, held until the process ends) instead of a
+ * buffer it allocates.  n must be 0..2^31-1.  The call allocates nothing and does not wait:
+ * its two kernels are queued on the context's stream, and out is ready after the next
+ * synchronizing call on the context or a device synchronization.  Same summation order and
+ * bits as vpmx_diagnostics.  Every call on one GPU (any context) uses that GPU's one array:
+ * calls that overlap on one GPU race. */
+int32_t vpmx_diagnostics_noalloc(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t n, int32_t qstride,
+                                 double *out);
 
 /* ------------------------------------------------------------- batched clouds
  * No counterpart in the reference, which runs independent clouds as one vpm() call after

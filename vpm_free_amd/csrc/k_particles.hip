@@ -8,6 +8,7 @@
 // depend on scheduling.
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 
 #include <hip/hip_ext.h>
 
@@ -1034,6 +1035,49 @@ static int reduce6(Ctx *c, int mode, const double *xp, const double *qp, int qs,
     VPMX_HIP(c, hipStreamSynchronize(c->stream));
     return VPMX_OK;
 }
+// ------------------------------------------------------------------ allocation-free diagnostics
+// The partial sums of diagnostics_noalloc live in this fixed device array instead of a buffer
+// the library allocates: RED_MAXB blocks of 6 sums (48 KiB + 64 B).  Each GPU holds its own
+// copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for the device (scratch_init), never through the array's name.
+constexpr int RED_MAXB = 1024;
+static __device__ __align__(256) double scratch[6 * RED_MAXB + 8];
+constexpr int SCRATCH_MAXDEV = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[SCRATCH_MAXDEV] = {};
+
+int scratch_init(Ctx *c) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device < 0 || c->device >= SCRATCH_MAXDEV)
+        return fail(c, VPMX_ERR_UNSUP, "allocation-free entry points: device index above 63");
+    if (!g_scratch_ptr[c->device]) {
+        // (the context's device is current: vpmx_create set it)
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, VPMX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[c->device] = (double *)p;
+    }
+    c->scratch = g_scratch_ptr[c->device];
+    return VPMX_OK;
+}
+
+int diagnostics_noalloc(Ctx *c, const double *xp, const double *qp, int qs, int64_t n, double *out) {
+    if (!c->scratch) return fail(c, VPMX_ERR_STATE, "allocation-free diagnostics: the device array is not set up");
+    if (n < 0 || n > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "allocation-free diagnostics: n must be 0..2^31-1");
+    if (qs < 3) return fail(c, VPMX_ERR_ARG, "allocation-free diagnostics: qstride must be at least 3");
+    if (!out || (n > 0 && (!xp || !qp))) return fail(c, VPMX_ERR_ARG, "allocation-free diagnostics: null pointer");
+    // the block count of reduce6, so both entry points sum in the same order
+    int nb = (int)((n + 255) / 256);
+    if (nb > RED_MAXB) nb = RED_MAXB;
+    if (nb < 1) nb = 1;
+    double *part = c->scratch;  // every block writes its partials: nothing to clear
+    hipLaunchKernelGGL((k_reduce_particles<6, 0>), dim3(nb), dim3(256), 0, c->stream, xp, qp, qs, n, part);
+    hipLaunchKernelGGL((k_reduce_final<6, 0>), dim3(1), dim3(256), 0, c->stream, part, nb, out, 0.5);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
 int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3]) {
     double o[6];
     VPMX_TRY(reduce6(c, 1, xp, nullptr, 0, n, o));

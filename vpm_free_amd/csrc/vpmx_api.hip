@@ -694,6 +694,7 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
         return VPMX_ERR_HIP;
     }
     c->kev_ok = true;
+    if (scratch_init(c) != VPMX_OK) c->scratch_err = c->err;  // only the allocation-free calls need it
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         c->num_cu = ncu;
@@ -972,6 +973,15 @@ int32_t vpmx_diagnostics(vpmx_ctx *p, double circ[3], double imp[3]) {
         return VPMX_OK;
     }
     return diagnostics(c, (const double *)c->xp.p, (const double *)c->qp.p, c->neqpm + 1, c->np, circ, imp);
+}
+
+int32_t vpmx_diagnostics_noalloc(vpmx_ctx *p, const double *xp, const double *qp, int64_t nvr, int32_t qstride,
+                                 double *out) {
+    Ctx *c = C(p);
+    if (!c) return VPMX_ERR_ARG;
+    if (!c->scratch) return fail(c, VPMX_ERR_STATE, "allocation-free diagnostics unavailable: " + c->scratch_err);
+    VPMX_HIP(c, hipSetDevice(c->device));
+    return diagnostics_noalloc(c, xp, qp, qstride, nvr, out);
 }
 
 int32_t vpmx_kernel_times(vpmx_ctx *p, double *ms_total, int64_t *launches, int32_t n, int32_t reset) {

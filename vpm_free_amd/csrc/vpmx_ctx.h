@@ -106,6 +106,8 @@ struct Ctx : vpmx_ctx {
     DBuf zt_prev;
 
     std::map<int, void *> fft_plans;
+    double *scratch = nullptr;  // the device's fixed array of the allocation-free entry points
+    std::string scratch_err;    // why it could not be set up
     bool have_grids = false;  // RHS/SOL/vel of the last mode 1-3 call are valid
 
     // remesh result waiting for vpmx_remesh_fetch

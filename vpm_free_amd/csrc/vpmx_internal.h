@@ -83,7 +83,10 @@ int particle_bbox(Ctx *c, const double *xp, int64_t n, double mn[3], double mx[3
 int step_rk2_enqueue(Ctx *c, double dt, const double uinf[3], int ntime, int mrem, int do_remesh);
 int64_t remesh_window_nodes(const Ctx *c, int mrem);
 
-
+// Address of the fixed device array of the allocation-free entry points on the context's
+// device (once per device), and diagnostics with its partials there (device output out[6]).
+int scratch_init(Ctx *c);
+int diagnostics_noalloc(Ctx *c, const double *xp, const double *qp, int qs, int64_t n, double *out);
 int diagnostics(Ctx *c, const double *xp, const double *qp, int qstride, int64_t n, double circ[3], double imp[3]);
 int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int hi[3], double thr,
                    const double x0[3], const double h[3], double dv, int neqpm, double **xp_out, double **qp_out,

@@ -106,6 +106,7 @@ def load_library():
         "vpmx_batch_cloud": [vp, i32, C.POINTER(vp)],
         "vpmx_batch_step_rk2": [vp, C.c_double, d, i32, i32, i32],
         "vpmx_batch_times": [vp, d, d],
+        "vpmx_diagnostics_noalloc": [vp, vp, vp, i64, i32, vp],
         "vpmx_stage_solve": [vp, d, d, C.POINTER(i32), i32, C.POINTER(i32), d, d, i32, d],
         "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
         "vpmx_stage_pmesh_inf_lev": [vp, d, d, C.POINTER(i32), i32, d, d, i32, i32, i32],
@@ -342,6 +343,12 @@ class VPM:
         circ, imp = np.zeros(3), np.zeros(3)
         self._check(self._lib.vpmx_diagnostics(self._h, _dp(circ), _dp(imp)))
         return circ, imp
+
+    def diagnostics_noalloc(self, xp_dev, qp_dev, n, qstride, out_dev):
+        """vpmx_diagnostics_noalloc on device addresses (integers); out_dev receives 6 doubles
+        once the context's stream has run the call."""
+        self._check(self._lib.vpmx_diagnostics_noalloc(self._h, C.c_void_p(xp_dev), C.c_void_p(qp_dev), int(n),
+                                                        int(qstride), C.c_void_p(out_dev)))
 
     # -- single routines of the path -------------------------------------------------
     def stage_solve(self, xbound, dpm, nn, nc, win, rhs, sol=None, zero_bc=False):
