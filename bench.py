@@ -1,6 +1,7 @@
 """Benchmark: VPM steps/s on one MI355X, two coaxial leapfrogging rings on the 256^3 grid,
-mixed form (FP32 sine-transform Poisson passes, FP64 boundary sums and diagnostics) beside
-the plain FP64 form.
+mixed form (FP32 projection, interpolation and sine-transform passes with the inner-box solve's
+inverse passes in FP64; FP64 boundary sums, stencils and diagnostics) beside the plain FP64
+form, plus a leg of 64 independent rings stepped as one batch (queued and graph-replayed).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
@@ -39,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=256,
                     help="boundary-sum target stride of the bounded CPU sample")
     ap.add_argument("--forms", default="fp64,mixed", help="forms to time: fp64, mixed or both")
+    ap.add_argument("--batch-steps", type=int, default=5,
+                    help="steps of the 64-ring batch leg, queued and graph-replayed (0: skip the leg)")
     return ap.parse_args()
 
 
@@ -183,6 +186,39 @@ def time_form(ctx, precision, xp0, qp0, args, g):
     return res
 
 
+def batch_leg(V, steps):
+    """64 independent rings (radius U[0.5,1.5], sigma U[0.05,0.15], random axis, grids 32^3..128^3
+    drawn with probability 1/N^3, seed 3), mixed form: one RK2 step with remesh of every ring,
+    queued on the rings' streams vs one captured hipGraph replayed."""
+    from vpm_free_amd import cases
+    clouds = cases.batch_rings(seed=3, count=64)
+    cells = int(sum(round(2 * np.abs(xp).max() / h) ** 3 for h, xp, _ in clouds))
+    out = {"clouds": len(clouds), "cells": cells, "particles": int(sum(xp.shape[0] for _, xp, _ in clouds))}
+    for graph in (False, True):
+        with V.Batch(len(clouds)) as b:
+            for i, (h, xp, qp) in enumerate(clouds):
+                c = b.cloud(i, V.Params.make(h))
+                c.set_precision(1)
+                c.upload(xp, qp)
+                c.vpm_resident(0, 0)
+            b.step_rk2(0.01, graph=graph)  # prepares the batch (and captures the graph)
+            dev, host = [], []
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.step_rk2(0.01, graph=graph)
+                d, e = b.times()
+                dev.append(d)
+                host.append(e)
+            wall = (time.perf_counter() - t0) / steps
+        key = "graph" if graph else "queued"
+        out[key] = {"ms_per_step": round(1000 * wall, 3), "device_ms": round(float(np.median(dev)), 3),
+                    "host_enqueue_ms": round(float(np.median(host)), 3),
+                    "clouds_per_s": round(len(clouds) / wall, 1)}
+        log(f"batch {key}: {1000 * wall:.2f} ms per step of {len(clouds)} clouds "
+            f"(device {np.median(dev):.2f} ms, host queueing {np.median(host):.2f} ms)")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -217,12 +253,14 @@ def main():
     forms = {}
     for name in args.forms.split(","):
         forms[name] = time_form(ctx, 1 if name == "mixed" else 0, xp0, qp0, args, g)
+        forms[name]["tuning"] = ctx.tuning_report().strip().splitlines()
         f = forms[name]
         log(f"{name}: {f['ms_per_step']:.2f} ms/step over {args.steps} steps; particles "
             f"{f['particles_range'][0]}..{f['particles_range'][1]}")
         log("  stage ms/step: " + ", ".join(f"{k}={v:.2f}" for k, v in f["stage_ms_per_step"].items()))
         log("  kernels: " + json.dumps(f["kernels"]))
     ctx.close()
+    batch = batch_leg(V, args.batch_steps) if args.batch_steps > 0 else None
 
     mixed = "mixed" in forms
     head = forms["mixed"] if mixed else forms["fp64"]
@@ -265,8 +303,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (mixed form: FP32 projection weights and sums, Poisson passes and interpolation; FP64 "
-                 "particles, grids in HBM, boundary sums, curl/stretching, diagnostics)"
+        "dtype": "f32 (mixed form: FP32 projection weights and sums, fine and coarse Poisson passes, the inner-box "
+                 "solve's forward passes, interpolation; FP64 inner-box inverse passes, particles, grids in HBM, "
+                 "boundary sums, curl/stretching, remesh, diagnostics)"
                  if mixed else "f64",
         "data": "synthetic",
         "config": {
@@ -284,6 +323,7 @@ def main():
             "accuracy_one_vpm2": acc,
         },
         "forms": forms,
+        "batched_64_rings": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

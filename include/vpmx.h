@@ -170,6 +170,32 @@ int32_t vpmx_diagnostics(vpmx_ctx *ctx, double circ[3], double imp[3]);
 int32_t vpmx_diagnostics_noalloc(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t n, int32_t qstride,
                                  double *out);
 
+/* ------------------------------------------------------------- adjoint
+ * No counterpart in the reference (it has no derivative of vpm()).  For the particles (xp, qp as
+ * vpmx_vpm, host arrays) and nprobe probe points (3 doubles each), evaluates
+ *   J = 1/2 sum_k |u(p_k)|^2,  u = the velocity grid of vpm(WhatToDo=2) (vpm.f90:230-313)
+ * interpolated at the probes with the M4' weights of the projection, in the plain FP64 form, and
+ * the gradient of J with respect to QP(1:3) (dq, nvr x 3) and XP (dx, nvr x 3) by the transposed
+ * stages of the solve.  Geometry from an earlier vpm(WhatToDo=0).  One yaps block (NBI = NBJ =
+ * NBK = 1), ibctyp 2 without the tree, interf_iproj 4: other set-ups return VPMX_ERR_UNSUP.
+ * Probes closer than two cells to the edge of the grid see the truncated stencil. */
+int32_t vpmx_adjoint_probe_energy(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
+                                  const double *probes, int64_t nprobe, double *J, double *dq, double *dx);
+
+/* The same J after nsteps RK2 steps of the particles (the step of vpmx_step_rk2 with mrem = 1:
+ * two vpm(2) evaluations, the Euler-form updates of test.f90:104-117 with velocity dt*(UP+uinf)
+ * and strength -dt*GP, remesh_particles_3d(1) after each step), in the plain FP64 form, and its
+ * gradient with respect to the starting QP(1:3) and XP (dq, dx: nvr x 3).  The remesh's
+ * threshold mask is held fixed (it is piecewise constant in the particles), so the remeshed
+ * particles' positions (grid nodes) carry no gradient.  The box must stay fixed (idefine = 1,
+ * geometry from vpm(WhatToDo=0) at NTIME = 0); steps run with NTIME = 1..nsteps.  The particles
+ * are kept every ckpt steps; the backward pass recomputes ckpt steps at a time and keeps their
+ * grids (about 6 grids and 7 particle arrays per step).  The context's resident particles are
+ * left in an unspecified state.  Same set-up limits as vpmx_adjoint_probe_energy. */
+int32_t vpmx_adjoint_rollout(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
+                             int32_t nsteps, double dt, const double *uinf, int32_t ckpt, const double *probes,
+                             int64_t nprobe, double *J, double *dq, double *dx);
+
 /* ------------------------------------------------------------- batched clouds
  * No counterpart in the reference, which runs independent clouds as one vpm() call after
  * another (vpm.f90:54-475; per cloud the application's RK2 step with remesh_particles_3d,

@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["vpmx_api.hip", "vpmx_batch.hip", "tune.hip", "k_particles.hip", "k_poisson.hip", "k_boundary.hip", "k_bspec.hip", "k_grid.hip",
+HIP_SOURCES = ["vpmx_api.hip", "vpmx_batch.hip", "tune.hip", "k_adjoint.hip", "k_particles.hip", "k_poisson.hip", "k_boundary.hip", "k_bspec.hip", "k_grid.hip",
                "k_tree.hip", "k_panel.hip"]
 CXX_SOURCES = ["geometry.cpp"]
 HIP_FLAGS = [

@@ -8,6 +8,7 @@
 //   calc_velocity_serial_3d(1), back_to_particles_3D(itype=1)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -628,6 +629,382 @@ static int replace_with_remesh(Ctx *c) {
     return VPMX_OK;
 }
 
+// ---------------------------------------------------------------- adjoint
+// pmesh(iynbc=1)^T on grid g: the adjoint pb of the infinite-domain solution (on every node)
+// to the adjoint rb of its right-hand side (interior nodes; faces 0):
+//   rb = D0(pb_in) + D0(A^T BS^T (pb_faces - F^T D0(pb_in)))
+// (D0: zero-BC solve, symmetric; F: fold of face values; BS: boundary sums; A: normal
+// derivatives times -area/4pi).  t1, t2: scratch grids of g's size.
+static int pmesh_inf_adj(Ctx *c, const Grid &g, GView pb, GView rb, DBuf &t1, DBuf &t2) {
+    int st = VPMX_OK;
+    const int n[3] = {g.n[0], g.n[1], g.n[2]};
+    const int64_t G = (int64_t)n[0] * n[1] * n[2];
+    double *y1 = (double *)ensure(c, t1, sizeof(double) * 3 * (size_t)G, &st); if (st) return st;
+    double *w2 = (double *)ensure(c, t2, sizeof(double) * 3 * (size_t)G, &st); if (st) return st;
+    int32_t win[6] = {1, 1, 1, n[0], n[1], n[2]};
+    double h[3];
+    window_h(g.x0, g.h, win, h);
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, pb, nullptr, view(y1, n), 0, SOLVE_FP64));
+    VPMX_TRY(fold_adj(c, n, h, pb, view(y1, n), view(w2, n)));  // gbar on the faces of w2
+    const int64_t nbmax = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
+    double *sig = (double *)ensure(c, c->src_sig, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
+    double *cxyz = (double *)ensure(c, c->src_cxyz, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
+    double *wgt = (double *)ensure(c, c->src_w, sizeof(double) * 4 * (size_t)nbmax, &st); if (st) return st;
+    double *abar = (double *)ensure(c, c->bs_part, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
+    int64_t nb = 0;
+    // (the face-cell geometry: centroids and areas; the sources it also forms are not used)
+    VPMX_TRY(normal_derivs(c, 3, g, view(y1, n), sig, cxyz, wgt, &nb));
+    VPMX_TRY(bsum_adj(c, g, nb, cxyz, view(w2, n), abar));
+    VPMX_TRY(fill(c, w2, 3 * G, 0.0));
+    VPMX_TRY(nderiv_adj(c, g, nb, abar, wgt + 3 * nb, view(w2, n)));
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, view(w2, n), nullptr, rb, 0, SOLVE_FP64));
+    VPMX_TRY(axpy_view(c, n, view(y1, n), rb));
+    return VPMX_OK;
+}
+
+// The transposed stages of one yaps block after the curl: psibar, the adjoint of the stream
+// function on the global grid (the block's inner window), to the adjoint of the block's RHS on
+// every block node (returned in c->adj_b).  Transposes, last stage first, of: the inner-box
+// solve with the corrected faces, the TSC face correction, the coarse infinite-domain solve,
+// the coarse Laplacian and sampling, and the block's infinite-domain solve (yaps3d.f90:1-453).
+static int yaps_adj(Ctx *c, GView pbg, double **wb_out) {
+    int st = VPMX_OK;
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    const int64_t B = (int64_t)nb[0] * nb[1] * nb[2];
+    const int32_t *w = G.nnbl_bl;
+    const int wn[3] = {w[3] - w[0] + 1, w[4] - w[1] + 1, w[5] - w[2] + 1};
+    if (wn[0] != gn[0] || wn[1] != gn[1] || wn[2] != gn[2])
+        return fail(c, VPMX_ERR_STATE, "adjoint: the block's inner window is not the global grid");
+    double *PB = (double *)ensure(c, c->adj_a, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    double *WB = (double *)ensure(c, c->adj_b, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    double *YB = (double *)ensure(c, c->adj_c, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    double *FB = (double *)ensure(c, c->adj_d, sizeof(double) * 3 * (size_t)B, &st); if (st) return st;
+    auto inner = [&](double *base) { return view(base, wn, w[0] - 1, w[1] - 1, w[2] - 1, nb); };
+    VPMX_TRY(fill(c, PB, 3 * B, 0.0));
+    VPMX_TRY(fill(c, WB, 3 * B, 0.0));
+    VPMX_TRY(fill(c, YB, 3 * B, 0.0));
+    VPMX_TRY(fill(c, FB, 3 * B, 0.0));
+    VPMX_TRY(axpy_view(c, wn, pbg, inner(PB)));
+    // final inner-box solve^T: wbar += D0(psibar_in); fbar = psibar_faces - F^T D0(psibar_in)
+    double hI[3];
+    window_h(&G.xbound_bl[0], G.dpm, w, hI);
+    VPMX_TRY(poisson_dirichlet(c, 3, wn, hI, inner(PB), nullptr, inner(YB), 0, SOLVE_FP64));
+    VPMX_TRY(axpy_view(c, wn, inner(YB), inner(WB)));
+    VPMX_TRY(fold_adj(c, wn, hI, inner(PB), inner(YB), inner(FB)));
+    // face correction^T: phibar (PB reused) = fbar on the inner faces; psibar_c, smpbar by TSC^T
+    VPMX_TRY(fill(c, PB, 3 * B, 0.0));
+    const int cn[3] = {G.nn_coarse[0], G.nn_coarse[1], G.nn_coarse[2]};
+    const int64_t GC = (int64_t)cn[0] * cn[1] * cn[2];
+    double *cb = (double *)ensure(c, c->adj_e, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    double *sb = (double *)ensure(c, c->adj_f, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    double *rc = (double *)ensure(c, c->adj_g, sizeof(double) * 3 * (size_t)GC, &st); if (st) return st;
+    VPMX_TRY(fill(c, cb, 3 * GC, 0.0));
+    VPMX_TRY(fill(c, sb, 3 * GC, 0.0));
+    // the forward's coarse map and footprint (yaps_single)
+    const int ncs = P.ncoarse;
+    int ns[3], fp_lo[3], fp_hi[3];
+    std::vector<int> cmap;
+    for (int d = 0; d < 3; d++) {
+        ns[d] = (nb[d] - 1) / ncs + 1;
+        for (int a = 0; a < ns[d]; a++) {
+            const double X = G.xbound_bl[d] + (double)(a * ncs) * G.dpm[d];
+            cmap.push_back((int)std::lround((X - G.xbound_coarse[d]) / G.dpm_coarse[d]) + 1);
+        }
+        fp_lo[d] = cmap[cmap.size() - ns[d]];
+        fp_hi[d] = cmap.back();
+    }
+    int *cmap_d = nullptr;
+    VPMX_TRY(upload_cmap(c, cmap, &cmap_d));
+    InterpArgs ia;
+    for (int d = 0; d < 6; d++) ia.nbl[d] = w[d];
+    for (int d = 0; d < 3; d++) {
+        ia.xbl[d] = G.xbound_bl[d];
+        ia.h[d] = G.dpm[d];
+        ia.xc[d] = G.xbound_coarse[d];
+        ia.hc[d] = G.dpm_coarse[d];
+        ia.cn[d] = cn[d];
+        ia.fp_lo[d] = fp_lo[d];
+        ia.fp_hi[d] = fp_hi[d];
+    }
+    VPMX_TRY(face_interp_adj(c, ia, view(FB, nb), view(PB, nb), view(cb, cn), view(sb, cn)));
+    // coarse solve^T, coarse Laplacian^T, sampling^T
+    const Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
+    VPMX_TRY(pmesh_inf_adj(c, gc, view(cb, cn), view(rc, cn), c->adj_h, c->adj_i));
+    VPMX_TRY(coarse_laplacian_adj(c, cn, fp_lo, fp_hi, G.dpm_coarse, view(rc, cn), view(sb, cn)));
+    VPMX_TRY(coarse_sample_adj(c, ns, ncs, cmap_d, view(sb, cn), view(PB, nb)));
+    // fine block solve^T
+    const Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
+    VPMX_TRY(pmesh_inf_adj(c, gb, view(PB, nb), view(YB, nb), c->adj_h, c->adj_i));
+    VPMX_TRY(axpy_view(c, nb, view(YB, nb), view(WB, nb)));
+    *wb_out = WB;
+    return VPMX_OK;
+}
+
+static int adjoint_supported(Ctx *c) {
+    const vpmx_geom &G = c->geom;
+    const vpmx_params &P = c->prm;
+    if (G.nblocks != 1) return fail(c, VPMX_ERR_UNSUP, "adjoint: one yaps block only");
+    if (P.iyntree != 0 || P.ibctyp != 2 || P.interf_iproj != 4)
+        return fail(c, VPMX_ERR_UNSUP, "adjoint: direct point-source boundary sums and the M4' projection only");
+    return VPMX_OK;
+}
+
+// J = 1/2 sum_k |u(p_k)|^2 for the velocity grid of vpm(2) on the particles (xp, qp: device),
+// and its gradient with respect to QP(1:3) (dq) and XP (dx), device arrays of 3 per particle.
+// Runs the forward (projection, yaps solve, curl) itself, in the plain FP64 form.
+static int adjoint_probe_energy(Ctx *c, const double *xp, const double *qp, int64_t n, const double *pts, int64_t np,
+                                double *J, double *dq, double *dx) {
+    int st = VPMX_OK;
+    VPMX_TRY(check_ready(c, c->neqpm));
+    VPMX_TRY(adjoint_supported(c));
+    const vpmx_geom &G = c->geom;
+    const int saved_prec = c->precision;
+    c->precision = 0;
+    int r = run_vpm(c, 1, xp, qp, nullptr, nullptr, n);  // projection, yaps solve, velocity grid
+    c->precision = saved_prec;
+    VPMX_TRY(r);
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    const Grid gg = make_grid(G.nn, G.xbound, G.dpm);
+    // velocity at the probes and J
+    double *u = (double *)ensure(c, c->stage_c, sizeof(double) * 3 * (size_t)(np > 0 ? np : 1), &st); if (st) return st;
+    VPMX_TRY(probe_interp(c, gg, pts, np, view((double *)c->vel.p, gn), u));
+    std::vector<double> uh((size_t)(3 * np));
+    if (np) VPMX_HIP(c, hipMemcpyAsync(uh.data(), u, sizeof(double) * 3 * np, hipMemcpyDeviceToHost, c->stream));
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    double j = 0.0;
+    for (double v : uh) j += 0.5 * v * v;
+    *J = j;
+    // ubar = M^T u on the global grid, psibar = curl^T ubar
+    double *ub = (double *)ensure(c, c->stage_a, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    double *pbg = (double *)ensure(c, c->stage_b, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    VPMX_TRY(fill(c, ub, 3 * GG, 0.0));
+    VPMX_TRY(probe_scatter(c, gg, pts, np, u, view(ub, gn)));
+    VPMX_TRY(curl_adj(c, gn, G.dpm, view(ub, gn), view(pbg, gn)));
+    double *WB = nullptr;
+    VPMX_TRY(yaps_adj(c, view(pbg, gn), &WB));
+    // the projection^T over the whole block, as the projection writes it
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    const Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
+    const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
+    VPMX_TRY(grad_particles(c, gb, xp, qp, c->neqpm + 1, n, view(WB, nb), 1.0 / DV, dq, dx));
+    return VPMX_OK;
+}
+
+// One vpm(WhatToDo=2) evaluation as the adjoint needs it: the particles and the grids the
+// forward formed (the block RHS, the velocity and stretching grids on the global grid).
+struct StageRef {
+    const double *x, *q;
+    int64_t n;
+    const double *R, *u, *s;
+};
+
+// vpm(2)^T at one evaluation: upstream adjoints ubar_p = au*us_p of UP and gbar_p = ag*gs_p of
+// GP (3 per particle) to the particles' adjoints, added to xacc and qacc (3 per particle):
+// M2P^T, stretching^T, curl^T, yaps^T, projection^T.
+static int vpm2_adj(Ctx *c, const StageRef &S, const double *us, double au, const double *gs, double ag, double *xacc,
+                    double *qacc) {
+    int st = VPMX_OK;
+    const vpmx_geom &G = c->geom;
+    const int gn[3] = {G.nn[0], G.nn[1], G.nn[2]};
+    const int64_t GG = (int64_t)gn[0] * gn[1] * gn[2];
+    const int nb[3] = {G.nnbl[0], G.nnbl[1], G.nnbl[2]};
+    const int32_t *w = G.nnbl_bl;
+    const Grid gg = make_grid(G.nn, G.xbound, G.dpm);
+    const int qs = c->neqpm + 1;
+    double *ubg = (double *)ensure(c, c->adj_u, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    double *sbg = (double *)ensure(c, c->adj_s, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    double *rs = (double *)ensure(c, c->adj_r, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    double *pbg = (double *)ensure(c, c->stage_b, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    VPMX_TRY(fill(c, ubg, 3 * GG, 0.0));
+    VPMX_TRY(fill(c, sbg, 3 * GG, 0.0));
+    VPMX_TRY(m2p_adj(c, gg, S.x, S.q, qs, S.n, us, au, gs, ag, view((double *)S.u, gn), view((double *)S.s, gn),
+                     view(ubg, gn), view(sbg, gn), xacc));
+    const GView Rg = view((double *)S.R, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
+    VPMX_TRY(stretch_adj(c, gn, G.dpm, Rg, view((double *)S.u, gn), view(sbg, gn), view(rs, gn), view(ubg, gn)));
+    VPMX_TRY(curl_adj(c, gn, G.dpm, view(ubg, gn), view(pbg, gn)));
+    double *WB = nullptr;
+    VPMX_TRY(yaps_adj(c, view(pbg, gn), &WB));
+    VPMX_TRY(axpy_view(c, gn, view(rs, gn), view(WB, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb)));
+    const Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
+    const double DV = G.dpm[0] * G.dpm[1] * G.dpm[2];
+    VPMX_TRY(grad_particles(c, gb, S.x, S.q, qs, S.n, view(WB, nb), 1.0 / DV, qacc, xacc, true));
+    return VPMX_OK;
+}
+
+static int save_buf(Ctx *c, DBuf &dst, const void *src, size_t bytes) {
+    int st = VPMX_OK;
+    void *d = ensure(c, dst, bytes, &st);
+    if (st) return st;
+    if (bytes) VPMX_HIP(c, hipMemcpyAsync(d, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return VPMX_OK;
+}
+
+// One RK2 step of the resident particles in the plain FP64 form (the sequence of
+// step_rk2_enqueue with mrem = 1), keeping what the step's adjoint needs when rec is given:
+// b[0..5] particles at the start, the midpoint and the end (before the remesh), b[6..8] and
+// b[9..11] the grids of the two evaluations, b[12] the remeshed positions.
+static int rk2_step_rec(Ctx *c, double dt, const double uinf[3], int ntime, Ctx::AdjStep *rec) {
+    int st = VPMX_OK;
+    const int64_t n = c->np;
+    const int nr = c->neqpm + 1;
+    const vpmx_geom &G = c->geom;
+    const size_t xb = sizeof(double) * 3 * (size_t)n, qb = sizeof(double) * nr * (size_t)n;
+    const size_t gb = sizeof(double) * 3 * (size_t)G.nn[0] * G.nn[1] * G.nn[2];
+    const size_t bb = sizeof(double) * 3 * (size_t)G.nnbl[0] * G.nnbl[1] * G.nnbl[2];
+    double *x0 = (double *)ensure(c, c->x0, sizeof(double) * 3 * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
+    double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
+    if (n > 0) {
+        VPMX_HIP(c, hipMemcpyAsync(x0, xp, xb, hipMemcpyDeviceToDevice, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(q0, qp, qb, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (rec) {
+        rec->n0 = n;
+        VPMX_TRY(save_buf(c, rec->b[0], xp, xb));
+        VPMX_TRY(save_buf(c, rec->b[1], qp, qb));
+    }
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    if (rec) {
+        VPMX_TRY(save_buf(c, rec->b[6], c->rhs_bl.p, bb));
+        VPMX_TRY(save_buf(c, rec->b[7], c->vel.p, gb));
+        VPMX_TRY(save_buf(c, rec->b[8], c->str.p, gb));
+    }
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
+    if (rec) {
+        VPMX_TRY(save_buf(c, rec->b[2], xp, xb));
+        VPMX_TRY(save_buf(c, rec->b[3], qp, qb));
+    }
+    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    if (rec) {
+        VPMX_TRY(save_buf(c, rec->b[9], c->rhs_bl.p, bb));
+        VPMX_TRY(save_buf(c, rec->b[10], c->vel.p, gb));
+        VPMX_TRY(save_buf(c, rec->b[11], c->str.p, gb));
+    }
+    VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, dt, uinf, x0, q0));
+    if (rec) {
+        VPMX_TRY(save_buf(c, rec->b[4], xp, xb));
+        VPMX_TRY(save_buf(c, rec->b[5], qp, qb));
+    }
+    const vpmx_geom g0 = c->geom;
+    VPMX_TRY(define_from_device(c, ntime));
+    const vpmx_geom &g1 = c->geom;
+    auto same = [](const void *a, const void *b, size_t n) { return std::memcmp(a, b, n) == 0; };
+    if (!same(g0.xbound, g1.xbound, sizeof g0.xbound) || !same(g0.dpm, g1.dpm, sizeof g0.dpm) ||
+        !same(g0.nn, g1.nn, sizeof g0.nn) || !same(g0.nnbl, g1.nnbl, sizeof g0.nnbl) ||
+        !same(g0.xbound_bl, g1.xbound_bl, sizeof g0.xbound_bl) || !same(g0.nnbl_bl, g1.nnbl_bl, sizeof g0.nnbl_bl) ||
+        !same(g0.xbound_coarse, g1.xbound_coarse, sizeof g0.xbound_coarse) ||
+        !same(g0.nn_coarse, g1.nn_coarse, sizeof g0.nn_coarse))
+        return fail(c, VPMX_ERR_UNSUP, "adjoint roll-out: the box changed during the roll-out (use idefine = 1)");
+    VPMX_TRY(run_remesh(c, (const double *)c->xp.p, (const double *)c->qp.p, n, 1));
+    VPMX_TRY(replace_with_remesh(c));
+    if (rec) {
+        rec->n1 = c->np;
+        VPMX_TRY(save_buf(c, rec->b[12], c->xp.p, sizeof(double) * 3 * (size_t)c->np));
+    }
+    return VPMX_OK;
+}
+
+// The adjoint of one recorded step: (xbar, qbar) of the remeshed particles (rec.n1, 3 per
+// particle) in adj_xb / adj_qb become those of the step's starting particles (rec.n0).
+// remesh^T with its threshold mask held fixed (the remeshed particles sit on the kept nodes;
+// their positions carry no adjoint), then the two stages' vpm(2)^T.
+static int rk2_step_adj(Ctx *c, const Ctx::AdjStep &rec, double dt) {
+    int st = VPMX_OK;
+    const vpmx_geom &G = c->geom;
+    const int64_t n = rec.n0;
+    const int qs = c->neqpm + 1;
+    const size_t pb = sizeof(double) * 3 * (size_t)(n > 0 ? n : 1);
+    // remesh^T: q' = sum_p W(x_p - X) q_p on the remesh grid (vpm_remesh.f90:137-195, ncell_rem 1)
+    int32_t NN[3];
+    double D[3];
+    for (int d = 0; d < 3; d++) {
+        NN[d] = G.nn[d];
+        D[d] = (G.xbound[d + 3] - G.xbound[d]) / (double)(NN[d] - 1);
+    }
+    const Grid gr = make_grid(NN, G.xbound, D);
+    const int64_t GG = (int64_t)NN[0] * NN[1] * NN[2];
+    double *grid = (double *)ensure(c, c->adj_r, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
+    VPMX_TRY(fill(c, grid, 3 * GG, 0.0));
+    VPMX_TRY(scatter_nodes(c, gr, (const double *)rec.b[12].p, (const double *)c->adj_qb.p, rec.n1, view(grid, NN)));
+    double *XB = (double *)ensure(c, c->adj_xb2, pb, &st); if (st) return st;
+    double *QB = (double *)ensure(c, c->adj_qb2, pb, &st); if (st) return st;
+    VPMX_TRY(grad_particles(c, gr, (const double *)rec.b[4].p, (const double *)rec.b[5].p, qs, n, view(grid, NN), 1.0, QB,
+                            XB, false));
+    std::swap(c->adj_xb, c->adj_xb2);
+    std::swap(c->adj_qb, c->adj_qb2);
+    // x1 = x0 + dt (u_m + uinf), q1 = q0 - dt g_m at the midpoint (x_m, q_m) = (x0, q0) + dt/2 (u_0 + uinf, -g_0)
+    double *XM = (double *)ensure(c, c->adj_xm, pb, &st); if (st) return st;
+    double *QM = (double *)ensure(c, c->adj_qm, pb, &st); if (st) return st;
+    XB = (double *)c->adj_xb.p;
+    QB = (double *)c->adj_qb.p;
+    VPMX_TRY(fill(c, XM, 3 * n, 0.0));
+    VPMX_TRY(fill(c, QM, 3 * n, 0.0));
+    const StageRef Sm{(const double *)rec.b[2].p, (const double *)rec.b[3].p, n, (const double *)rec.b[9].p,
+                      (const double *)rec.b[10].p, (const double *)rec.b[11].p};
+    VPMX_TRY(vpm2_adj(c, Sm, XB, dt, QB, -dt, XM, QM));
+    VPMX_TRY(axpy(c, 3 * n, 1.0, XM, XB));
+    VPMX_TRY(axpy(c, 3 * n, 1.0, QM, QB));
+    const StageRef S0{(const double *)rec.b[0].p, (const double *)rec.b[1].p, n, (const double *)rec.b[6].p,
+                      (const double *)rec.b[7].p, (const double *)rec.b[8].p};
+    VPMX_TRY(vpm2_adj(c, S0, XM, 0.5 * dt, QM, -0.5 * dt, XB, QB));
+    return VPMX_OK;
+}
+
+// J of the probes after nsteps RK2 steps (each with its remesh) of the resident particles, and
+// its gradient with respect to the starting QP(1:3) and XP (in adj_qb / adj_xb, 3 per particle).
+// Checkpoints every ckpt steps; the backward pass recomputes one segment of steps at a time,
+// keeping its records.
+static int adjoint_rollout(Ctx *c, int nsteps, double dt, const double uinf[3], int ckpt, const double *pts,
+                           int64_t np, double *J) {
+    int st = VPMX_OK;
+    VPMX_TRY(check_ready(c, c->neqpm));
+    VPMX_TRY(adjoint_supported(c));
+    if (c->prm.ncell_rem != 1) return fail(c, VPMX_ERR_UNSUP, "ncell_rem must be 1");
+    const int nr = c->neqpm + 1;
+    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    if ((int)c->adj_ck.size() < nseg) c->adj_ck.resize(nseg);
+    if ((int)c->adj_steps.size() < ckpt) c->adj_steps.resize(ckpt);
+    std::vector<int64_t> ckn(nseg), fwd_n(nsteps);
+    for (int k = 0; k < nsteps; k++) {
+        if (k % ckpt == 0) {
+            ckn[k / ckpt] = c->np;
+            VPMX_TRY(save_buf(c, c->adj_ck[k / ckpt][0], c->xp.p, sizeof(double) * 3 * (size_t)c->np));
+            VPMX_TRY(save_buf(c, c->adj_ck[k / ckpt][1], c->qp.p, sizeof(double) * nr * (size_t)c->np));
+        }
+        VPMX_TRY(rk2_step_rec(c, dt, uinf, k + 1, nullptr));
+        fwd_n[k] = c->np;
+    }
+    const int64_t nK = c->np;
+    const size_t pbK = sizeof(double) * 3 * (size_t)(nK > 0 ? nK : 1);
+    double *XB = (double *)ensure(c, c->adj_xb, pbK, &st); if (st) return st;
+    double *QB = (double *)ensure(c, c->adj_qb, pbK, &st); if (st) return st;
+    VPMX_TRY(adjoint_probe_energy(c, (const double *)c->xp.p, (const double *)c->qp.p, nK, pts, np, J, QB, XB));
+    for (int s = nseg - 1; s >= 0; s--) {
+        const int k0 = s * ckpt, k1 = std::min(nsteps, k0 + ckpt);
+        const int64_t n0 = ckn[s];
+        double *dx = (double *)ensure(c, c->xp, sizeof(double) * 3 * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
+        double *dq = (double *)ensure(c, c->qp, sizeof(double) * nr * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
+        if (n0) {
+            VPMX_HIP(c, hipMemcpyAsync(dx, c->adj_ck[s][0].p, sizeof(double) * 3 * n0, hipMemcpyDeviceToDevice, c->stream));
+            VPMX_HIP(c, hipMemcpyAsync(dq, c->adj_ck[s][1].p, sizeof(double) * nr * n0, hipMemcpyDeviceToDevice,
+                                       c->stream));
+        }
+        c->np = n0;
+        ensure(c, c->up, sizeof(double) * 3 * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
+        ensure(c, c->gp, sizeof(double) * 3 * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
+        for (int k = k0; k < k1; k++) {
+            VPMX_TRY(rk2_step_rec(c, dt, uinf, k + 1, &c->adj_steps[k - k0]));
+            if (c->np != fwd_n[k]) return fail(c, VPMX_ERR_STATE, "adjoint roll-out: the recomputed steps differ");
+        }
+        for (int k = k1 - 1; k >= k0; k--) VPMX_TRY(rk2_step_adj(c, c->adj_steps[k - k0], dt));
+    }
+    return VPMX_OK;
+}
+
 // One RK2 (midpoint) step of the resident particles, queued on the context's stream: two
 // vpm(2) evaluations with the Euler updates of test.f90:104-117, then vpm(0) and
 // remesh_particles_3d(1) when do_remesh.  In the device-count mode the launches are sized for
@@ -715,8 +1092,15 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sol0_c, &c->cmap, &c->dst_work, &c->lam[0], &c->lam[1], &c->lam[2], &c->scan_tmp,
                     &c->red_tmp, &c->rm_flag, &c->rm_pos, &c->rm_x, &c->rm_q, &c->rm_grid, &c->stage_a, &c->stage_b,
                     &c->stage_c, &c->sort.keys, &c->sort.perm, &c->sort.count, &c->sort.start, &c->sort.fill,
-                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev, &c->dst_work64};
+                    &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev, &c->dst_work64,
+                    &c->tune_scratch, &c->adj_a, &c->adj_b, &c->adj_c, &c->adj_d, &c->adj_e, &c->adj_f,
+                    &c->adj_g, &c->adj_h, &c->adj_i, &c->adj_p, &c->adj_q, &c->adj_u, &c->adj_s, &c->adj_r,
+                    &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm};
     for (DBuf *b : bufs) release(*b);
+    for (auto &r : c->adj_steps)
+        for (DBuf &b : r.b) release(b);
+    for (auto &ck : c->adj_ck)
+        for (DBuf &b : ck) release(b);
     free_bspec(c);
     free_tree(c);
     free_fft_plans(c);
@@ -982,6 +1366,57 @@ int32_t vpmx_diagnostics_noalloc(vpmx_ctx *p, const double *xp, const double *qp
     if (!c->scratch) return fail(c, VPMX_ERR_STATE, "allocation-free diagnostics unavailable: " + c->scratch_err);
     VPMX_HIP(c, hipSetDevice(c->device));
     return diagnostics_noalloc(c, xp, qp, qstride, nvr, out);
+}
+
+int32_t vpmx_adjoint_probe_energy(vpmx_ctx *p, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
+                                  const double *probes, int64_t nprobe, double *J, double *dq, double *dx) {
+    Ctx *c = C(p);
+    if (!c || !J || !dq || !dx || (nprobe > 0 && !probes)) return VPMX_ERR_ARG;
+    if (nprobe < 0 || nprobe > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "probe count out of range");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
+    int st = VPMX_OK;
+    double *pd = (double *)ensure(c, c->adj_p, sizeof(double) * 3 * (size_t)(nprobe > 0 ? nprobe : 1), &st);
+    if (st) return st;
+    double *g = (double *)ensure(c, c->adj_q, sizeof(double) * 6 * (size_t)(nvr > 0 ? nvr : 1), &st);
+    if (st) return st;
+    if (nprobe) VPMX_HIP(c, hipMemcpyAsync(pd, probes, sizeof(double) * 3 * nprobe, hipMemcpyHostToDevice, c->stream));
+    VPMX_TRY(adjoint_probe_energy(c, (const double *)c->xp.p, (const double *)c->qp.p, nvr, pd, nprobe, J, g,
+                                  g + 3 * nvr));
+    if (nvr) {
+        VPMX_HIP(c, hipMemcpyAsync(dq, g, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(dx, g + 3 * nvr, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
+}
+
+int32_t vpmx_adjoint_rollout(vpmx_ctx *p, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
+                             int32_t nsteps, double dt, const double *uinf, int32_t ckpt, const double *probes,
+                             int64_t nprobe, double *J, double *dq, double *dx) {
+    Ctx *c = C(p);
+    if (!c || !J || !dq || !dx || !uinf || (nprobe > 0 && !probes)) return VPMX_ERR_ARG;
+    if (nprobe < 0 || nprobe > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "probe count out of range");
+    if (nsteps < 0 || ckpt < 1) return fail(c, VPMX_ERR_ARG, "nsteps >= 0 and ckpt >= 1");
+    VPMX_HIP(c, hipSetDevice(c->device));
+    if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
+    VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));
+    int st = VPMX_OK;
+    double *pd = (double *)ensure(c, c->adj_p, sizeof(double) * 3 * (size_t)(nprobe > 0 ? nprobe : 1), &st);
+    if (st) return st;
+    if (nprobe) VPMX_HIP(c, hipMemcpyAsync(pd, probes, sizeof(double) * 3 * nprobe, hipMemcpyHostToDevice, c->stream));
+    const int saved_prec = c->precision;
+    c->precision = 0;
+    const int r = adjoint_rollout(c, nsteps, dt, uinf, ckpt, pd, nprobe, J);
+    c->precision = saved_prec;
+    VPMX_TRY(r);
+    if (nvr) {
+        VPMX_HIP(c, hipMemcpyAsync(dq, c->adj_qb.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+        VPMX_HIP(c, hipMemcpyAsync(dx, c->adj_xb.p, sizeof(double) * 3 * nvr, hipMemcpyDeviceToHost, c->stream));
+    }
+    VPMX_HIP(c, hipStreamSynchronize(c->stream));
+    return VPMX_OK;
 }
 
 int32_t vpmx_kernel_times(vpmx_ctx *p, double *ms_total, int64_t *launches, int32_t n, int32_t reset) {

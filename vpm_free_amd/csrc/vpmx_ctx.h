@@ -2,6 +2,7 @@
 #pragma once
 #include <map>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "vpmx_internal.h"
@@ -59,6 +60,16 @@ struct Ctx : vpmx_ctx {
     };
     std::map<std::string, Tuned> tuned;
     DBuf tune_scratch;
+    DBuf adj_a, adj_b, adj_c, adj_d, adj_e, adj_f, adj_g, adj_h, adj_i, adj_p, adj_q;  // adjoint work arrays
+    DBuf adj_u, adj_s, adj_r, adj_xb, adj_qb, adj_xb2, adj_qb2, adj_xm, adj_qm;
+    // roll-out adjoint: one recorded RK2 step (particles at the start, midpoint and end, the two
+    // evaluations' grids, the remeshed positions) and the checkpoints (XP, QP)
+    struct AdjStep {
+        DBuf b[13];
+        int64_t n0 = 0, n1 = 0;
+    };
+    std::vector<AdjStep> adj_steps;
+    std::vector<std::array<DBuf, 2>> adj_ck;
     int neqpm = 3;
     double ni = 0.0;  // kinematic viscosity NI of the diffusion mode (resident calls)
 

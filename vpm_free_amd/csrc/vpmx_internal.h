@@ -195,4 +195,30 @@ int solget_block(Ctx *c, const int nbl[6], const int g0[3], GView blk, GView glo
 int soa_to_aos(Ctx *c, int ncomp_src, int ncomp_dst, const int n[3], GView src, double fill_last, double *dst);
 int aos_to_soa(Ctx *c, int ncomp, const int n[3], const double *src, GView dst);
 
+// Adjoint stages (k_adjoint.hip): transposes of the curl, the fold of Dirichlet values, the
+// boundary sums, the normal derivatives, the TSC face correction, the coarse Laplacian and
+// sampling; probe interpolation and its transpose; the particles' gradients.
+int curl_adj(Ctx *c, const int n[3], const double h[3], GView ub, GView pb);
+int fold_adj(Ctx *c, const int n[3], const double h[3], GView pb, GView y, GView out);
+int bsum_adj(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, GView gb, double *abar);
+int nderiv_adj(Ctx *c, const Grid &g, int64_t nb, const double *abar, const double *ds, GView s0b);
+int face_interp_adj(Ctx *c, const InterpArgs &a, GView fb, GView phib, GView cb, GView sb);
+int coarse_laplacian_adj(Ctx *c, const int cn[3], const int lo[3], const int hi[3], const double hc[3], GView rb,
+                         GView sb);
+int coarse_sample_adj(Ctx *c, const int ns[3], int nc, const int *cmap, GView coarse, GView fine);
+int axpy_view(Ctx *c, const int n[3], GView in, GView out);
+// dq = (1/DV) P^T wb, dx = its position derivative (acc: added to dq, dx instead)
+int grad_particles(Ctx *c, const Grid &g, const double *xp, const double *qp, int qs, int64_t n, GView wb, double idv,
+                   double *dq, double *dx, bool acc = false);
+// M2P^T: ubar_p = au*us_p, gbar_p = ag*gs_p scattered to ubg / sbg (vol_p GP weight), xacc += dx
+int m2p_adj(Ctx *c, const Grid &g, const double *xp, const double *qp, int qs, int64_t n, const double *us, double au,
+            const double *gs, double ag, GView vel, GView str, GView ubg, GView sbg, double *xacc);
+// stretching^T: rb = S_R^T sb (written), ub += S_u^T sb
+int stretch_adj(Ctx *c, const int n[3], const double h[3], GView R, GView u, GView sb, GView rb, GView ub);
+// remesh^T: node particles' strength adjoints added to their nodes
+int scatter_nodes(Ctx *c, const Grid &g, const double *xp, const double *qb, int64_t n, GView out);
+int axpy(Ctx *c, int64_t n, double a, const double *x, double *y);
+int probe_interp(Ctx *c, const Grid &g, const double *pts, int64_t np, GView v, double *u);
+int probe_scatter(Ctx *c, const Grid &g, const double *pts, int64_t np, const double *ub, GView out);
+
 }  // namespace vpmx

@@ -107,6 +107,8 @@ def load_library():
         "vpmx_batch_step_rk2": [vp, C.c_double, d, i32, i32, i32],
         "vpmx_batch_times": [vp, d, d],
         "vpmx_diagnostics_noalloc": [vp, vp, vp, i64, i32, vp],
+        "vpmx_adjoint_probe_energy": [vp, d, d, i64, i32, d, i64, d, d, d],
+        "vpmx_adjoint_rollout": [vp, d, d, i64, i32, i32, C.c_double, d, i32, d, i64, d, d, d],
         "vpmx_stage_solve": [vp, d, d, C.POINTER(i32), i32, C.POINTER(i32), d, d, i32, d],
         "vpmx_stage_pmesh_inf": [vp, d, d, C.POINTER(i32), i32, d, d],
         "vpmx_stage_pmesh_inf_lev": [vp, d, d, C.POINTER(i32), i32, d, d, i32, i32, i32],
@@ -343,6 +345,30 @@ class VPM:
         circ, imp = np.zeros(3), np.zeros(3)
         self._check(self._lib.vpmx_diagnostics(self._h, _dp(circ), _dp(imp)))
         return circ, imp
+
+    def adjoint_probe_energy(self, XP, QP, probes):
+        """J = 1/2 sum |u(probe)|^2 of the vpm(2) velocity grid (plain FP64 form) and its gradient
+        with respect to QP(:, 0:3) and XP: returns (J, dJ/dQP, dJ/dXP)."""
+        XP, QP, pr = _f64(XP), _f64(QP), _f64(np.asarray(probes).reshape(-1, 3))
+        n = XP.shape[0]
+        dq, dx = np.zeros((n, 3)), np.zeros((n, 3))
+        J = np.zeros(1)
+        self._check(self._lib.vpmx_adjoint_probe_energy(self._h, _dp(XP), _dp(QP), n, QP.shape[1] - 1, _dp(pr),
+                                                        pr.shape[0], _dp(J), _dp(dq), _dp(dx)))
+        return float(J[0]), dq, dx
+
+    def adjoint_rollout(self, XP, QP, probes, nsteps, dt, uinf=(0.0, 0.0, 0.0), ckpt=2):
+        """J = 1/2 sum |u(probe)|^2 after nsteps RK2 steps with remesh (plain FP64 form) and its
+        gradient with respect to the starting QP(:, 0:3) and XP: returns (J, dJ/dQP, dJ/dXP)."""
+        XP, QP, pr = _f64(XP), _f64(QP), _f64(np.asarray(probes).reshape(-1, 3))
+        n = XP.shape[0]
+        dq, dx = np.zeros((n, 3)), np.zeros((n, 3))
+        J = np.zeros(1)
+        u = _f64(np.asarray(uinf, dtype=np.float64).reshape(3))
+        self._check(self._lib.vpmx_adjoint_rollout(self._h, _dp(XP), _dp(QP), n, QP.shape[1] - 1, int(nsteps),
+                                                   float(dt), _dp(u), int(ckpt), _dp(pr), pr.shape[0], _dp(J),
+                                                   _dp(dq), _dp(dx)))
+        return float(J[0]), dq, dx
 
     def diagnostics_noalloc(self, xp_dev, qp_dev, n, qstride, out_dev):
         """vpmx_diagnostics_noalloc on device addresses (integers); out_dev receives 6 doubles
