@@ -124,9 +124,11 @@ def test_adjoint_autograd_function(ctx):
     J = probe_energy(c, X, Q, torch.tensor(probes))
     J.backward()
     J2, dq, dx = c.adjoint_probe_energy(xp, qp, probes)
-    assert float(J) == J2
-    assert np.array_equal(X.grad.numpy(), dx)
-    assert np.array_equal(Q.grad.numpy()[:, :3], dq) and not Q.grad.numpy()[:, 3].any()
+    assert float(J.detach()) == J2
+    # the scatters' FP64 atomics leave the gradients' last bits to the order of the adds
+    assert np.linalg.norm(X.grad.numpy() - dx) <= 1e-12 * np.linalg.norm(dx)
+    assert np.linalg.norm(Q.grad.numpy()[:, :3] - dq) <= 1e-12 * np.linalg.norm(dq)
+    assert not Q.grad.numpy()[:, 3].any()
 
 
 def test_adjoint_rejects_several_blocks(vpmlib):
@@ -184,13 +186,16 @@ def test_rollout_checkpoint_spacing(ctx):
 
 
 def test_rollout_gradient_fd(ctx):
-    """Central differences in FP64: strengths along a random direction (relative step 1e-4),
-    positions along the gradient (step 1e-7); the remesh mask does not change at these steps."""
+    """Central differences in FP64: strengths along a random direction, each particle's
+    perturbation 1e-4 of its own strength; positions along the gradient (step 1e-7).  The
+    remesh threshold mask is piecewise constant and the gradient holds it fixed, so the steps
+    must not switch nodes on: a perturbation of fixed size would give the zero-strength
+    particles (the box corners) strengths that the remesh then keeps."""
     c, xp, qp, probes = ctx
     print("case: roll-out gradient vs central differences", flush=True)
     J, dq, dx = c.adjoint_rollout(xp, qp, probes, NSTEPS, DT)
     rng = np.random.default_rng(9)
-    v = rng.standard_normal((xp.shape[0], 3)) * np.abs(qp[:, :3]).max() * 1e-4
+    v = rng.standard_normal((xp.shape[0], 3)) * qp[:, :3] * 1e-4
     qa, qb = qp.copy(), qp.copy()
     qa[:, :3] += v
     qb[:, :3] -= v
