@@ -1,7 +1,8 @@
 """Benchmark: VPM steps/s on one MI355X, two coaxial leapfrogging rings on the 256^3 grid,
 mixed form (FP32 projection, interpolation and sine-transform passes with the inner-box solve's
 inverse passes in FP64; FP64 boundary sums, stencils and diagnostics) beside the plain FP64
-form, plus a leg of 64 independent rings stepped as one batch (queued and graph-replayed).
+form, plus a leg of 64 independent rings stepped as one batch (queued and graph-replayed) and
+a leg timing the gradient of a probe energy through 10 checkpointed RK2 steps (adjoint).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=256,
                     help="boundary-sum target stride of the bounded CPU sample")
     ap.add_argument("--forms", default="fp64,mixed", help="forms to time: fp64, mixed or both")
+    ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the adjoint leg (0: skip)")
+    ap.add_argument("--adjoint-ckpt", type=int, default=2, help="checkpoint spacing of the adjoint leg")
     ap.add_argument("--batch-steps", type=int, default=5,
                     help="steps of the 64-ring batch leg, queued and graph-replayed (0: skip the leg)")
     return ap.parse_args()
@@ -219,6 +222,43 @@ def batch_leg(V, steps):
     return out
 
 
+def adjoint_leg(V, nsteps, ckpt, dt=0.01):
+    """Gradient of the probe energy J = 1/2 sum |u(p)|^2 at 4096 random probes (seed 4, inside
+    [-1.5,1.5]^3) after `nsteps` RK2 steps with remesh, with respect to the starting particle
+    strengths and positions: one ring (R=1, sigma=0.22) on [-2,2]^3 at h=1/32 (128^3 class:
+    N=129 global nodes), plain FP64 form, particles kept every `ckpt` steps.  Timed against the
+    same steps run forward alone."""
+    from vpm_free_amd import cases
+    h = 1.0 / 32
+    xp, qp = cases.single_ring(h=h, sigma=0.22)
+    probes = np.random.default_rng(4).uniform(-1.5, 1.5, size=(4096, 3))
+    with V.VPM(V.Params.make(h)) as c:
+        c.set_precision(0)
+        c.upload(xp, qp)
+        c.vpm_resident(0, 0)
+        g = V.geom_dict(c.geometry)
+        c.step_rk2(dt, ntime=1)  # warm-up: buffers and tuned launches
+        c.upload(xp, qp)
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            c.step_rk2(dt, ntime=k + 1)
+        n_end = c.count()
+        fwd = time.perf_counter() - t0
+        c.adjoint_rollout(xp, qp, probes, min(nsteps, 2), dt, ckpt=ckpt)  # warm-up: plans, records
+        t0 = time.perf_counter()
+        J, dq, dx = c.adjoint_rollout(xp, qp, probes, nsteps, dt, ckpt=ckpt)
+        adj = time.perf_counter() - t0
+    out = {"workload": "one ring R=1 sigma=0.22, h=1/32, box [-2,2]^3, %d RK2 steps dt=%g + remesh, 4096 probes"
+                       % (nsteps, dt),
+           "grid_nodes": g["nn"], "block_nodes": g["nnbl"][0], "particles": [int(xp.shape[0]), int(n_end)],
+           "ckpt": ckpt, "forward_ms": round(1000 * fwd, 2), "adjoint_ms": round(1000 * adj, 2),
+           "adjoint_over_forward": round(adj / fwd, 3), "J": J,
+           "grad_norms": {"dq": float(np.linalg.norm(dq)), "dx": float(np.linalg.norm(dx))}}
+    log(f"adjoint: {nsteps} steps forward {1000 * fwd:.1f} ms, gradient (forward + backward) {1000 * adj:.1f} ms "
+        f"= {adj / fwd:.2f}x; particles {xp.shape[0]} -> {n_end}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -261,6 +301,7 @@ def main():
         log("  kernels: " + json.dumps(f["kernels"]))
     ctx.close()
     batch = batch_leg(V, args.batch_steps) if args.batch_steps > 0 else None
+    adjoint = adjoint_leg(V, args.adjoint_steps, args.adjoint_ckpt) if args.adjoint_steps > 0 else None
 
     mixed = "mixed" in forms
     head = forms["mixed"] if mixed else forms["fp64"]
@@ -324,6 +365,7 @@ def main():
         },
         "forms": forms,
         "batched_64_rings": batch,
+        "adjoint_ring_128": adjoint,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -111,7 +111,10 @@ int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
  * "dst_reuse" -- 1 (default): the second fine solve of pmesh starts from the first one's
  * forward transforms, 0: both solves in full; "tuning" -- 1 (default): the autotuner picks
  * kernel variants and persistent grid sizes by timing them once per launch key, 0: fixed
- * defaults (clears the choices made so far). */
+ * defaults (clears the choices made so far); "adjoint_boundsum" -- the adjoint's transposed
+ * boundary sums by 2 spectral correlation on the forward's plan (default), 1 direct sums;
+ * "adjoint_m2p_scatter" -- 1: the adjoint's transposed interpolation by FP64 atomic scatters,
+ * 0 (default): by the sorted gather projection. */
 int32_t vpmx_set_option(vpmx_ctx *ctx, const char *name, int32_t value);
 /* The autotuner's choices so far, one line per launch key ("key -> choice (time)"), into buf
  * (NUL-terminated, cut to capacity). */
@@ -190,7 +193,8 @@ int32_t vpmx_adjoint_probe_energy(vpmx_ctx *ctx, const double *xp, const double 
  * particles' positions (grid nodes) carry no gradient.  The box must stay fixed (idefine = 1,
  * geometry from vpm(WhatToDo=0) at NTIME = 0); steps run with NTIME = 1..nsteps.  The particles
  * are kept every ckpt steps; the backward pass recomputes ckpt steps at a time and keeps their
- * grids (about 6 grids and 7 particle arrays per step).  The context's resident particles are
+ * grids (about 6 grids and 7 particle arrays per step).  ckpt = 0 keeps every step's grids from
+ * the forward pass instead (no recomputation; HBM for all nsteps records).  The context's resident particles are
  * left in an unspecified state.  Same set-up limits as vpmx_adjoint_probe_energy. */
 int32_t vpmx_adjoint_rollout(vpmx_ctx *ctx, const double *xp, const double *qp, int64_t nvr, int32_t neqpm,
                              int32_t nsteps, double dt, const double *uinf, int32_t ckpt, const double *probes,

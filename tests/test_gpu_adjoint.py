@@ -114,6 +114,40 @@ def test_adjoint_position_gradient_fd(ctx):
             assert abs(fd - dx[p, d]) <= 1e-5 * np.abs(dx[p]).max(), (p, d, fd, dx[p, d])
 
 
+def test_adjoint_spectral_vs_direct_transposed_sums(ctx):
+    """The transposed boundary sums by spectral correlation on the forward's plan give the
+    gradient of the direct O(faces^2) sums to rounding."""
+    c, xp, qp, probes = ctx
+    print("case: adjoint, spectral vs direct transposed boundary sums", flush=True)
+    J1, dq1, dx1 = c.adjoint_probe_energy(xp, qp, probes)
+    c.set_option("adjoint_boundsum", 1)
+    try:
+        J2, dq2, dx2 = c.adjoint_probe_energy(xp, qp, probes)
+    finally:
+        c.set_option("adjoint_boundsum", 2)
+    assert J1 == J2
+    e = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in ((dq1, dq2), (dx1, dx2))]
+    print("  rel L2 dq %.3e dx %.3e" % tuple(e), flush=True)
+    assert max(e) <= 1e-12, e
+
+
+def test_adjoint_m2p_gather_vs_scatter(ctx):
+    """The transposed interpolation as a sorted gather projection of the particles' adjoints
+    equals its FP64-atomic scatter form to rounding (one roll-out step exercises it)."""
+    c, xp, qp, probes = ctx
+    print("case: roll-out, M2P^T gather vs scatter", flush=True)
+    J1, dq1, dx1 = c.adjoint_rollout(xp, qp, probes, 1, 0.05)
+    c.set_option("adjoint_m2p_scatter", 1)
+    try:
+        J2, dq2, dx2 = c.adjoint_rollout(xp, qp, probes, 1, 0.05)
+    finally:
+        c.set_option("adjoint_m2p_scatter", 0)
+    assert J1 == J2
+    e = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in ((dq1, dq2), (dx1, dx2))]
+    print("  rel L2 dq %.3e dx %.3e" % tuple(e), flush=True)
+    assert max(e) <= 1e-12, e
+
+
 def test_adjoint_autograd_function(ctx):
     import torch
     from vpm_free_amd.autograd import probe_energy
@@ -177,8 +211,8 @@ def test_rollout_zero_steps_is_probe_energy(ctx):
 
 def test_rollout_checkpoint_spacing(ctx):
     c, xp, qp, probes = ctx
-    print("case: roll-out checkpoints every 1, 2, 3 steps", flush=True)
-    res = [c.adjoint_rollout(xp, qp, probes, NSTEPS, DT, ckpt=k) for k in (1, 2, 3)]
+    print("case: roll-out checkpoints every 1, 2, 3 steps and every step kept", flush=True)
+    res = [c.adjoint_rollout(xp, qp, probes, NSTEPS, DT, ckpt=k) for k in (1, 2, 3, 0)]
     for J, dq, dx in res[1:]:
         assert J == res[0][0]
         assert np.linalg.norm(dq - res[0][1]) <= 1e-12 * np.linalg.norm(res[0][1])

@@ -184,54 +184,91 @@ __device__ __forceinline__ double tsc_w(double x) {
     if (a >= 0.5) return 0.5 * ((-a + 1.5) * (-a + 1.5));
     return 0.5 * ((a + 1.5) * (a + 1.5)) - 1.5 * ((a + 0.5) * (a + 0.5));
 }
-__global__ void k_face_interp_adj(InterpArgs a, int64_t nface, GView fb, GView phib, GView cb, GView sb) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nface) return;
-    const int XS = a.nbl[0], YS = a.nbl[1], ZS = a.nbl[2], XF = a.nbl[3], YF = a.nbl[4], ZF = a.nbl[5];
-    const int64_t f12 = (int64_t)(YF - YS + 1) * (ZF - ZS + 1), f34 = (int64_t)(XF - XS - 1) * (ZF - ZS + 1);
-    int i, j, k;
-    if (t < 2 * f12) {
-        int f = (int)(t / f12); int64_t r = t % f12;
-        i = f ? XF : XS; j = YS + (int)(r % (YF - YS + 1)); k = ZS + (int)(r / (YF - YS + 1));
-    } else if (t < 2 * (f12 + f34)) {
-        int64_t u = t - 2 * f12; int f = (int)(u / f34); int64_t r = u % f34;
-        j = f ? YF : YS; i = XS + 1 + (int)(r % (XF - XS - 1)); k = ZS + (int)(r / (XF - XS - 1));
-    } else {
-        int64_t u = t - 2 * (f12 + f34); int64_t f56 = (int64_t)(XF - XS - 1) * (YF - YS - 1);
-        int f = (int)(u / f56); int64_t r = u % f56;
-        k = f ? ZF : ZS; i = XS + 1 + (int)(r % (XF - XS - 1)); j = YS + 1 + (int)(r / (XF - XS - 1));
-    }
-    const double X0 = a.xbl[0] + (double)(i - 1) * a.h[0];
-    const double X1 = a.xbl[1] + (double)(j - 1) * a.h[1];
-    const double X2 = a.xbl[2] + (double)(k - 1) * a.h[2];
-    const int in = (int)round((X0 - a.xc[0]) / a.hc[0]) + 1;
-    const int jn = (int)round((X1 - a.xc[1]) / a.hc[1]) + 1;
-    const int kn = (int)round((X2 - a.xc[2]) / a.hc[2]) + 1;
+// Gather form: one wave per coarse node k sums w(t, k) fbar(t) over the inner-box face nodes t
+// whose stencil (in(t) +- 1 on every axis) holds k -- a window of about 3 ncoarse fine nodes per
+// in-plane axis on each face within one coarse cell of k.  The lanes split each face's window
+// and a fixed butterfly adds their sums, so no atomics are needed and the result does not
+// depend on scheduling.
+constexpr int FIA_WAVES = 4;
+__global__ void __launch_bounds__(64 * FIA_WAVES) k_face_interp_adj(InterpArgs a, GView fb, GView cb, GView sb) {
+    const int64_t t = (int64_t)blockIdx.x * FIA_WAVES + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= (int64_t)a.cn[0] * a.cn[1] * a.cn[2]) return;  // whole waves
+    const int cc[3] = {(int)(t % a.cn[0]) + 1, (int)((t / a.cn[0]) % a.cn[1]) + 1,
+                       (int)(t / ((int64_t)a.cn[0] * a.cn[1])) + 1};
     auto inmap = [&](int p, int q, int r) {
         return p >= a.fp_lo[0] && p <= a.fp_hi[0] && q >= a.fp_lo[1] && q <= a.fp_hi[1] && r >= a.fp_lo[2] &&
                r <= a.fp_hi[2];
     };
-    const int nnb = (!inmap(in - 1, jn - 1, kn - 1) || !inmap(in + 1, jn + 1, kn + 1)) ? 1 : 0;
-    const int64_t o = fb.at(i - 1, j - 1, k - 1);
-    double f[3];
-    for (int q = 0; q < 3; q++) {
-        f[q] = fb.p[q * fb.cs + o];
-        phib.p[q * phib.cs + o] += f[q];
-    }
-    for (int kc = kn - 1; kc <= kn + 1; kc++)
-        for (int jc = jn - 1; jc <= jn + 1; jc++)
-            for (int ic = in - 1; ic <= in + 1; ic++) {
-                if (!(ic >= 1 && ic <= a.cn[0] && jc >= 1 && jc <= a.cn[1] && kc >= 1 && kc <= a.cn[2])) continue;
-                const double xc = (X0 - a.xc[0] - (double)(ic - 1) * a.hc[0]) / a.hc[0];
-                const double yc = (X1 - a.xc[1] - (double)(jc - 1) * a.hc[1]) / a.hc[1];
-                const double zc = (X2 - a.xc[2] - (double)(kc - 1) * a.hc[2]) / a.hc[2];
-                const double fw = tsc_w(xc) * tsc_w(yc) * tsc_w(zc);
-                const int64_t oc = cb.at(ic - 1, jc - 1, kc - 1);
-                for (int q = 0; q < 3; q++) {
-                    atomicAdd(&cb.p[q * cb.cs + oc], fw * f[q]);
-                    atomicAdd(&sb.p[q * sb.cs + oc], fw * f[q] * (double)(nnb - 1));
-                }
+    auto cnode = [&](int d, int i) {  // nearest coarse node (1-based) of fine node i on axis d
+        const double X = a.xbl[d] + (double)(i - 1) * a.h[d];
+        return (int)round((X - a.xc[d]) / a.hc[d]) + 1;
+    };
+    double acc_c[3] = {0, 0, 0}, acc_s[3] = {0, 0, 0};
+    for (int f = 0; f < 6; f++) {
+        const int ax = f >> 1, P = (f & 1) ? a.nbl[ax + 3] : a.nbl[ax];
+        if (abs(cnode(ax, P) - cc[ax]) > 1) continue;
+        const int u = ax == 0 ? 1 : 0, v = ax == 2 ? 1 : 2;
+        // face node ranges: X faces whole, Y faces without the x-edges, Z faces without x and y
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = a.nbl[d];
+            hi[d] = a.nbl[d + 3];
+        }
+        if (ax >= 1) { lo[0]++; hi[0]--; }
+        if (ax == 2) { lo[1]++; hi[1]--; }
+        int r0[3], r1[3];
+        for (int d : {u, v}) {
+            const double f0 = (a.xc[d] - a.xbl[d] + ((double)cc[d] - 2.5) * a.hc[d]) / a.h[d] + 1.0;
+            const double f1 = (a.xc[d] - a.xbl[d] + ((double)cc[d] + 0.5) * a.hc[d]) / a.h[d] + 1.0;
+            r0[d] = max(lo[d], (int)floor(f0) - 1);
+            r1[d] = min(hi[d], (int)ceil(f1) + 1);
+        }
+        const int nu = r1[u] - r0[u] + 1, nv = r1[v] - r0[v] + 1;
+        if (nu <= 0 || nv <= 0) continue;
+        for (int q0 = lane; q0 < nu * nv; q0 += 64) {
+            int n3[3];
+            n3[ax] = P;
+            n3[u] = r0[u] + q0 % nu;
+            n3[v] = r0[v] + q0 / nu;
+            const int in = cnode(0, n3[0]), jn = cnode(1, n3[1]), kn = cnode(2, n3[2]);
+            if (abs(in - cc[0]) > 1 || abs(jn - cc[1]) > 1 || abs(kn - cc[2]) > 1) continue;
+            const double X0 = a.xbl[0] + (double)(n3[0] - 1) * a.h[0];
+            const double X1 = a.xbl[1] + (double)(n3[1] - 1) * a.h[1];
+            const double X2 = a.xbl[2] + (double)(n3[2] - 1) * a.h[2];
+            const double xc = (X0 - a.xc[0] - (double)(cc[0] - 1) * a.hc[0]) / a.hc[0];
+            const double yc = (X1 - a.xc[1] - (double)(cc[1] - 1) * a.hc[1]) / a.hc[1];
+            const double zc = (X2 - a.xc[2] - (double)(cc[2] - 1) * a.hc[2]) / a.hc[2];
+            const double fw = tsc_w(xc) * tsc_w(yc) * tsc_w(zc);
+            const int nnb = (!inmap(in - 1, jn - 1, kn - 1) || !inmap(in + 1, jn + 1, kn + 1)) ? 1 : 0;
+            const int64_t o = fb.at(n3[0] - 1, n3[1] - 1, n3[2] - 1);
+            for (int q = 0; q < 3; q++) {
+                const double fv = fb.p[q * fb.cs + o];
+                acc_c[q] += fw * fv;
+                acc_s[q] += fw * fv * (double)(nnb - 1);
             }
+        }
+    }
+    for (int q = 0; q < 3; q++)
+        for (int m = 32; m >= 1; m >>= 1) {
+            acc_c[q] += __shfl_xor(acc_c[q], m, 64);
+            acc_s[q] += __shfl_xor(acc_s[q], m, 64);
+        }
+    if (lane != 0) return;
+    const int64_t oc = cb.at(cc[0] - 1, cc[1] - 1, cc[2] - 1), os = sb.at(cc[0] - 1, cc[1] - 1, cc[2] - 1);
+    for (int q = 0; q < 3; q++) {
+        cb.p[q * cb.cs + oc] += acc_c[q];
+        sb.p[q * sb.cs + os] += acc_s[q];
+    }
+}
+// phibar(t) += fbar(t) on the inner-box faces (the face value carries the fine solution).
+__global__ void k_face_carry_adj(int XS, int YS, int ZS, int nx, int ny, int nz, GView fb, GView phib) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nx * ny * nz) return;
+    const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+    if (!(i == 0 || i == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1)) return;
+    const int64_t o = fb.at(XS - 1 + i, YS - 1 + j, ZS - 1 + k), op = phib.at(XS - 1 + i, YS - 1 + j, ZS - 1 + k);
+    for (int q = 0; q < 3; q++) phib.p[q * phib.cs + op] += fb.p[q * fb.cs + o];
 }
 
 // ---------------------------------------------------------------- Lc^T, Samp^T
@@ -328,9 +365,49 @@ __global__ void k_grad_x(Grid g, const double *xp, const double *qp, int qs, int
 // ---------------------------------------------------------------- M2P^T
 // Forward (back_to_particles_3D, vpm_time.f90:30-107): UP_p = sum W u, GP_p = vol_p sum W s over
 // the 4 x 4 x 4 nodes around x_p, W the M4' weights.  With upstream adjoints
-// ubar_p = au * us_p and gbar_p = ag * gs_p: the grid adjoints ubar_grid += W ubar_p,
-// sbar_grid += W vol_p gbar_p (scatters) and the position adjoint
-// xbar_p += sum grad W (u . ubar_p + vol_p s . gbar_p) (accumulated).
+// ubar_p = au * us_p and gbar_p = ag * gs_p: the grid adjoints ubar_grid = sum_p W ubar_p and
+// sbar_grid = sum_p W vol_p gbar_p are projections of those particle values (the forward's
+// sorted gather P2M), and the position adjoint xbar_p += sum grad W (u . ubar_p + vol_p s .
+// gbar_p) is a gather (this kernel, with the scaled particle values written for the P2M).
+__global__ void k_m2p_adj_x(Grid g, const double *xp, const double *qp, int qs, int64_t n, const double *us,
+                            double au, const double *gs, double ag, GView vel, GView str, double *xacc, double *ubp,
+                            double *sbp) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double t[3], ub[3], gb[3];
+    int c0[3];
+    const double vol = qp[(int64_t)qs * p + (qs - 1)];
+    for (int d = 0; d < 3; d++) {
+        t[d] = (xp[3 * p + d] - g.x0[d]) / g.h[d];
+        c0[d] = (int)floor(t[d]);
+        ub[d] = au * us[3 * p + d];
+        gb[d] = ag * gs[3 * p + d] * vol;
+        ubp[3 * p + d] = ub[d];
+        sbp[3 * p + d] = gb[d];
+    }
+    double gx[3] = {0, 0, 0};
+    for (int kk = c0[2] - 1; kk <= c0[2] + 2; kk++) {
+        if (kk < 0 || kk >= g.n[2]) continue;
+        const double wz = m4_w(t[2] - kk), dz = m4_dw(t[2] - kk) / g.h[2];
+        for (int jj = c0[1] - 1; jj <= c0[1] + 2; jj++) {
+            if (jj < 0 || jj >= g.n[1]) continue;
+            const double wy = m4_w(t[1] - jj), dy = m4_dw(t[1] - jj) / g.h[1];
+            for (int ii = c0[0] - 1; ii <= c0[0] + 2; ii++) {
+                if (ii < 0 || ii >= g.n[0]) continue;
+                const double wx = m4_w(t[0] - ii), dxw = m4_dw(t[0] - ii) / g.h[0];
+                const int64_t ov = vel.at(ii, jj, kk), os = str.at(ii, jj, kk);
+                double sdot = 0.0;
+                for (int c = 0; c < 3; c++)
+                    sdot += vel.p[c * vel.cs + ov] * ub[c] + str.p[c * str.cs + os] * gb[c];
+                gx[0] += dxw * wy * wz * sdot;
+                gx[1] += wx * dy * wz * sdot;
+                gx[2] += wx * wy * dz * sdot;
+            }
+        }
+    }
+    for (int d = 0; d < 3; d++) xacc[3 * p + d] += gx[d];
+}
+// The scatter form of the same transpose (FP64 atomics), kept for checking the gather form.
 __global__ void k_m2p_adj(Grid g, const double *xp, const double *qp, int qs, int64_t n, const double *us, double au,
                           const double *gs, double ag, GView vel, GView str, GView ubg, GView sbg, double *xacc) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -506,9 +583,13 @@ int nderiv_adj(Ctx *c, const Grid &g, int64_t nb, const double *abar, const doub
     return VPMX_OK;
 }
 int face_interp_adj(Ctx *c, const InterpArgs &a, GView fb, GView phib, GView cb, GView sb) {
-    const int64_t nx = a.nbl[3] - a.nbl[0] + 1, ny = a.nbl[4] - a.nbl[1] + 1, nz = a.nbl[5] - a.nbl[2] + 1;
-    const int64_t nface = 2 * (ny * nz + (nx - 2) * nz + (nx - 2) * (ny - 2));
-    hipLaunchKernelGGL(k_face_interp_adj, dim3(blocks_of(nface)), dim3(256), 0, c->stream, a, nface, fb, phib, cb, sb);
+    const int nx = a.nbl[3] - a.nbl[0] + 1, ny = a.nbl[4] - a.nbl[1] + 1, nz = a.nbl[5] - a.nbl[2] + 1;
+    const int64_t nc = (int64_t)a.cn[0] * a.cn[1] * a.cn[2], nw = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_face_interp_adj, dim3((unsigned)((nc + FIA_WAVES - 1) / FIA_WAVES)), dim3(64 * FIA_WAVES), 0,
+                       c->stream, a, fb, cb, sb);
+    VPMX_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_face_carry_adj, dim3(blocks_of(nw)), dim3(256), 0, c->stream, a.nbl[0], a.nbl[1], a.nbl[2], nx,
+                       ny, nz, fb, phib);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
@@ -543,10 +624,26 @@ int grad_particles(Ctx *c, const Grid &g, const double *xp, const double *qp, in
 }
 int m2p_adj(Ctx *c, const Grid &g, const double *xp, const double *qp, int qs, int64_t n, const double *us, double au,
             const double *gs, double ag, GView vel, GView str, GView ubg, GView sbg, double *xacc) {
+    const int64_t G = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    VPMX_TRY(fill(c, ubg.p, 3 * G, 0.0));
+    VPMX_TRY(fill(c, sbg.p, 3 * G, 0.0));
     if (n <= 0) return VPMX_OK;
-    hipLaunchKernelGGL(k_m2p_adj, dim3(blocks_of(n)), dim3(256), 0, c->stream, g, xp, qp, qs, n, us, au, gs, ag, vel,
-                       str, ubg, sbg, xacc);
+    if (c->adj_m2p_scatter) {
+        hipLaunchKernelGGL(k_m2p_adj, dim3(blocks_of(n)), dim3(256), 0, c->stream, g, xp, qp, qs, n, us, au, gs, ag,
+                           vel, str, ubg, sbg, xacc);
+        VPMX_HIP(c, hipGetLastError());
+        return VPMX_OK;
+    }
+    int st = VPMX_OK;
+    double *ubp = (double *)ensure(c, c->adj_pu, sizeof(double) * 3 * (size_t)n, &st); if (st) return st;
+    double *sbp = (double *)ensure(c, c->adj_ps, sizeof(double) * 3 * (size_t)n, &st); if (st) return st;
+    hipLaunchKernelGGL(k_m2p_adj_x, dim3(blocks_of(n)), dim3(256), 0, c->stream, g, xp, qp, qs, n, us, au, gs, ag, vel,
+                       str, xacc, ubp, sbp);
     VPMX_HIP(c, hipGetLastError());
+    VPMX_TRY(sort_particles(c, g, xp, n, c->sort));
+    const int vn[3] = {g.n[0], g.n[1], g.n[2]}, wlo[3] = {0, 0, 0};
+    VPMX_TRY(p2m_gather(c, g, c->sort, xp, ubp, 3, 3, 4, 1.0, ubg, vn, wlo, -1, false, false, false));
+    VPMX_TRY(p2m_gather(c, g, c->sort, xp, sbp, 3, 3, 4, 1.0, sbg, vn, wlo, -1, false, false, false));
     return VPMX_OK;
 }
 int stretch_adj(Ctx *c, const int n[3], const double h[3], GView R, GView u, GView sb, GView rb, GView ub) {

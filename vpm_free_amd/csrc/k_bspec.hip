@@ -64,6 +64,7 @@ struct SpecCol {  // one column: a source face line family, one component
     int s_is_u;       // q runs along A's lower in-plane axis (else C does)
     int flipq;        // q -> K-1-q
     int comp;
+    int pair;         // B * 6 + A (the transposed sums read target face B's nodes)
 };
 
 struct SpecPair {  // where the contribution of source face A to target face B sits
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_toep_fft(SpecFft fq, SpecTa
 // complex sequence (r_w is real); results to Cm[2 col], Cm[2 col + 1] like the matrix
 // product's.  One w per sequence, one column per blockIdx.y.
 __global__ void __launch_bounds__(DST_THREADS) k_bsx_qconv(SpecFft fq, SpecTab t, const double2 *twq,
-                                                           const double2 *T2, const double *Bm, double *Cm) {
+                                                           const double2 *T2, const double *Bm, double *Cm, int cj) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
     const int Lq = fq.L, np = fq.npairs;
@@ -304,7 +305,11 @@ __global__ void __launch_bounds__(DST_THREADS) k_bsx_qconv(SpecFft fq, SpecTab t
     fft_reg<SPEC_MAXR>(A, twl, fq);
     for (int i = tid; i < np * Lq; i += DST_THREADS) {
         const int s = i / Lq, v = i - s * Lq, w = w0 + s;
-        if (w < t.nw) A[i] = cmul(A[i], T2[(int64_t)w * Lq + v]);
+        if (w < t.nw) {
+            double2 r = T2[(int64_t)w * Lq + v];
+            if (cj) r.y = -r.y;  // correlation: the transposed sums
+            A[i] = cmul(A[i], r);
+        }
     }
     __syncthreads();
     // inverse transform: FFT again, output n at index (Lq - n) mod Lq (1/Lq is in the table)
@@ -325,7 +330,7 @@ constexpr int QC_WAVES = 4, QC_LMAX = 576;
 template <int FFT>  // 0: the 576-point compile-time plan, -1: the runtime plan P
 __global__ void __launch_bounds__(64 * QC_WAVES) k_bsx_qconv_w(WavePlan P, SpecTab t, const double2 *twq,
                                                                const double2 *T2, const double *Bm, double *Cm,
-                                                               int ncol) {
+                                                               int ncol, int cj) {
 #pragma clang fp contract(off)
     extern __shared__ double2 lds[];
     const int Lq = P.L, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -343,7 +348,11 @@ __global__ void __launch_bounds__(64 * QC_WAVES) k_bsx_qconv_w(WavePlan P, SpecT
     if constexpr (FFT == 0) FixedFft<0>::template run<double2, double2 *, false>(A, twl, lane);
     else wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
     const double2 *tr = T2 + (int64_t)w * Lq;
-    for (int v = lane; v < Lq; v += 64) A[v] = cmul(A[v], tr[v]);
+    for (int v = lane; v < Lq; v += 64) {
+        double2 r = tr[v];
+        if (cj) r.y = -r.y;  // correlation: the transposed sums
+        A[v] = cmul(A[v], r);
+    }
     __builtin_amdgcn_wave_barrier();
     if constexpr (FFT == 0) FixedFft<0>::template run<double2, double2 *, false>(A, twl, lane);
     else wave_fft<SPEC_MAXR, double2, double2 *, QC_LMAX>(A, twl, P, lane);
@@ -407,6 +416,119 @@ __global__ void __launch_bounds__(64 * GATHER_WAVES) k_bsx_gather(Grid g, const 
     sol.p[2 * sol.cs + n] += acc2;
 }
 
+// ------------------------------------------------------------------ transposed sums
+// The adjoint of the sums: abar(c, s) = sum_t gbar(c, t) / |x_t - x_s| over every boundary
+// node t (each on its owner face, as the gather assigns it) for every face-cell source s.
+// Per (B, A) pair the forward is a convolution along C and a product (or convolution) over q;
+// the transpose is the correlation along C -- the conjugate kernel spectrum, exp(+pi i w / L) R
+// with the same real table R -- and the transposed product over p (R^T, or the conjugate of
+// the q-convolution's spectrum).  Columns and tables are the forward plan's.
+
+// Node lines p of the column's target face B (owned nodes, else 0), n < nC along C, ->
+// their half spectra: Bm[2 col][w][p] = Re, Bm[2 col + 1][w][p] = Im.
+template <int FFT>
+__global__ void __launch_bounds__(bsx_threads<FFT>()) k_bsxT_fwd(SpecFft f, WavePlan P, SpecTab t, const double2 *tw,
+                                                                  const SpecCol *cols, const SpecPair *pairs, Grid g,
+                                                                  GView gb, double *Bm) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    constexpr int NT = bsx_threads<FFT>();
+    const int L = f.L, np = FFT == -2 ? f.npairs : BW_WAVES, nl = 2 * np;
+    double2 *A = lds, *twl = lds + np * L;
+    double *Ad = (double *)A;
+    const int tid = threadIdx.x, p0 = blockIdx.x * nl, col = blockIdx.y;
+    const SpecCol cd = cols[col];
+    const SpecPair PP = pairs[cd.pair];
+    const int B = cd.pair / 6, ax = B >> 1, hi = B & 1;
+    const double *src = gb.p + (int64_t)cd.comp * gb.cs;
+    for (int i = tid; i < L; i += NT) twl[i] = tw[i];
+    for (int i = tid; i < nl * L; i += NT) {
+        const int l = i / L, r = i - l * L, p = p0 + l;
+        double v = 0.0;
+        if (r < t.nC && p < t.M) {
+            int c3[3];  // 0-based node coordinates
+            c3[ax] = hi ? g.n[ax] - 1 : 0;
+            c3[PP.tax] = PP.flipp ? t.M - 1 - p : p;
+            c3[PP.cax] = r;
+            // owner faces: X faces whole, Y faces without the x-edges, Z faces without x- and y-edges
+            const bool own = ax == 0 || (c3[0] >= 1 && c3[0] <= g.n[0] - 2 && (ax == 1 || (c3[1] >= 1 && c3[1] <= g.n[1] - 2)));
+            if (own) v = src[gb.at(c3[0], c3[1], c3[2])];
+        }
+        Ad[2 * ((l >> 1) * L + r) + (l & 1)] = v;
+    }
+    __syncthreads();
+    bsx_fft<FFT>(A, twl, f, P);
+    const int64_t cst = (int64_t)t.nw * t.M;
+    double *out = Bm + 2 * col * cst;
+    for (int i = tid; i < t.nw * nl; i += NT) {
+        const int k = i / nl, l = i - k * nl, p = p0 + l;
+        if (p < t.M) {
+            const double2 z = unpack_half(A, L, l >> 1, k, l & 1);
+            out[(int64_t)k * t.M + p] = z.x;
+            out[cst + (int64_t)k * t.M + p] = z.y;
+        }
+    }
+}
+
+// Inverse real FFT over w of exp(+pi i w / L) (Cm[2 col][w][q] + i Cm[2 col + 1][w][q]), the
+// first nC - 1 outputs (source cells along C) -> apart[B][comp][s] at the source's index in
+// face A's array (every (B, A) column writes its own slice, so no two columns meet).
+template <int FFT>
+__global__ void __launch_bounds__(bsx_threads<FFT>()) k_bsxT_inv(SpecFft f, WavePlan P, SpecTab t, const double2 *tw,
+                                                                  const SpecCol *cols, const double *Cm, int64_t ns,
+                                                                  double *apart) {
+#pragma clang fp contract(off)
+    extern __shared__ double2 lds[];
+    constexpr int NT = bsx_threads<FFT>();
+    const int L = f.L, np = FFT == -2 ? f.npairs : BW_WAVES, nl = 2 * np;
+    double2 *A = lds, *twl = lds + np * L;
+    const int tid = threadIdx.x, q0 = blockIdx.x * nl, col = blockIdx.y;
+    const SpecCol cd = cols[col];
+    const int64_t cst = (int64_t)t.nw * t.K;
+    const double *in = Cm + 2 * col * cst;
+    for (int i = tid; i < L; i += NT) twl[i] = tw[i];
+    for (int i = tid; i < t.nw * np; i += NT) {
+        const int k = i / np, pr = i - k * np, q = q0 + 2 * pr;
+        double2 ph = half_shift(k, L);
+        ph.y = -ph.y;  // exp(+pi i k / L)
+        const int64_t o = (int64_t)k * t.K + q;
+        const double xr = q < t.K ? in[o] : 0.0, xi = q < t.K ? in[cst + o] : 0.0;
+        const double yr = q + 1 < t.K ? in[o + 1] : 0.0, yi = q + 1 < t.K ? in[cst + o + 1] : 0.0;
+        const double2 X = make_double2(xr * ph.x - xi * ph.y, xr * ph.y + xi * ph.x);
+        const double2 Y = make_double2(yr * ph.x - yi * ph.y, yr * ph.y + yi * ph.x);
+        if (k == 0 || 2 * k == L) {
+            A[pr * L + k] = make_double2(X.x, Y.x);
+        } else {
+            A[pr * L + k] = make_double2(X.x - Y.y, X.y + Y.x);
+            A[pr * L + L - k] = make_double2(X.x + Y.y, Y.x - X.y);
+        }
+    }
+    __syncthreads();
+    bsx_fft<FFT>(A, twl, f, P);
+    const double invL = 1.0 / (double)L;
+    const int ncell = t.nC - 1;
+    double *out = apart + (int64_t)((cd.pair / 6) * 3 + cd.comp) * ns + cd.src_off;
+    for (int i = tid; i < nl * ncell; i += NT) {
+        const int l = i / ncell, r = i - l * ncell, q = q0 + l;
+        if (q >= t.K) continue;
+        const double2 w = A[(l >> 1) * L + (r == 0 ? 0 : L - r)];
+        const int qs = cd.flipq ? t.K - 1 - q : q;
+        const int64_t o = cd.s_is_u ? (int64_t)r * cd.ustride + qs : (int64_t)qs * cd.ustride + r;
+        out[o] = ((l & 1) ? w.y : w.x) * invL;
+    }
+}
+
+// abar(c, s) = sum over the six target faces B (fixed order) of apart[B][c][s].
+__global__ void k_bsxT_reduce(int64_t ns, const double *apart, double *abar) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    for (int c = 0; c < 3; c++) {
+        double v = 0.0;
+        for (int B = 0; B < 6; B++) v += apart[(int64_t)(B * 3 + c) * ns + s];
+        abar[(int64_t)c * ns + s] = v;
+    }
+}
+
 // ------------------------------------------------------------------ host plan
 struct SpecTable {
     SpecTab t{};
@@ -432,6 +554,8 @@ struct SpecPlan {
     double *Bm = nullptr, *Cm = nullptr;
     double *Ores = nullptr;
     double flops = 0.0;
+    double *BmT = nullptr, *CmT = nullptr, *apart = nullptr;  // transposed sums (allocated on first use)
+    int64_t apart_ns = 0;
 };
 
 static void free_plan(SpecPlan *P) {
@@ -445,6 +569,9 @@ static void free_plan(SpecPlan *P) {
     if (P->Bm) (void)hipFree(P->Bm);
     if (P->Cm) (void)hipFree(P->Cm);
     if (P->Ores) (void)hipFree(P->Ores);
+    if (P->BmT) (void)hipFree(P->BmT);
+    if (P->CmT) (void)hipFree(P->CmT);
+    if (P->apart) (void)hipFree(P->apart);
     delete P;
 }
 
@@ -581,7 +708,7 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
             const int uA = As == 0 ? 1 : 0;
             refs[B * 6 + A] = {ti, (int)T.cols.size(), tax, cax, flipp};
             for (int comp = 0; comp < 3; comp++)
-                T.cols.push_back({face_off[A], n[uA] - 1, sax == uA ? 1 : 0, flipq, comp});
+                T.cols.push_back({face_off[A], n[uA] - 1, sax == uA ? 1 : 0, flipq, comp, B * 6 + A});
         }
     // buffers
     int64_t nb = 0, ncm = 0, no = 0;
@@ -712,7 +839,8 @@ int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int
                       "dgemm_strided_batched");
 }
 
-int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, int slot) {
+// The plan of slot for grid g (built, or rebuilt when the grid changed).
+static int get_plan(Ctx *c, const Grid &g, int slot, SpecPlan **out) {
     SpecPlan *P = (SpecPlan *)c->bsx[slot];
     if (P && (P->n[0] != g.n[0] || P->n[1] != g.n[1] || P->n[2] != g.n[2] || P->h[0] != g.h[0] ||
               P->h[1] != g.h[1] || P->h[2] != g.h[2])) {
@@ -725,6 +853,13 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         if (st) return st;
         c->bsx[slot] = P;
     }
+    *out = P;
+    return VPMX_OK;
+}
+
+int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, int slot) {
+    SpecPlan *P = nullptr;
+    VPMX_TRY(get_plan(c, g, slot, &P));
     rocblas_handle hb = nullptr;
     VPMX_TRY(blas_handle(c, &hb));
     for (auto &T : P->tabs) {
@@ -747,16 +882,16 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
             const dim3 gq((unsigned)((nseq + QC_WAVES - 1) / QC_WAVES));
             if (T.t.Lq == 576)
                 hipLaunchKernelGGL(k_bsx_qconv_w<0>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, T.t, T.twq, T.T2,
-                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol);
+                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol, 0);
             else
                 hipLaunchKernelGGL(k_bsx_qconv_w<-1>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, T.t, T.twq, T.T2,
-                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol);
+                                   P->Bm + T.b_off, P->Cm + T.c_off, ncol, 0);
             VPMX_HIP(c, hipGetLastError());
         } else if (T.t.type == 1) {
             const size_t ldsq = sizeof(double2) * (size_t)(T.fq.npairs + 1) * T.t.Lq;
             hipLaunchKernelGGL(k_bsx_qconv, dim3((unsigned)((T.t.nw + T.fq.npairs - 1) / T.fq.npairs), (unsigned)ncol),
                                dim3(DST_THREADS), ldsq, c->stream, T.fq, T.t, T.twq, T.T2, P->Bm + T.b_off,
-                               P->Cm + T.c_off);
+                               P->Cm + T.c_off, 0);
             VPMX_HIP(c, hipGetLastError());
         } else {
             const double one = 1.0, zero = 0.0;
@@ -798,6 +933,96 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
         fft += per * (double)T.cols.size() * (T.t.K + T.t.M) / 2.0;
     }
     if (slot == 0) c->k_work[K_BOUNDSUM] = P->flops + fft;
+    return VPMX_OK;
+}
+
+int boundary_sums_spectral_adj(Ctx *c, const Grid &g, int64_t ns, GView gb, double *abar, int slot) {
+    SpecPlan *P = nullptr;
+    VPMX_TRY(get_plan(c, g, slot, &P));
+    int64_t nbT = 0, ncT = 0;
+    for (auto &T : P->tabs) {
+        const int64_t N = (int64_t)T.cols.size();
+        nbT += 2 * N * T.t.nw * T.t.M;
+        ncT += 2 * N * T.t.nw * T.t.K;
+    }
+    if (!P->BmT) {
+        VPMX_HIP(c, hipMalloc((void **)&P->BmT, sizeof(double) * nbT));
+        VPMX_HIP(c, hipMalloc((void **)&P->CmT, sizeof(double) * ncT));
+    }
+    if (P->apart_ns < ns) {
+        if (P->apart) {
+            VPMX_HIP(c, hipStreamSynchronize(c->stream));
+            (void)hipFree(P->apart);
+            P->apart = nullptr;
+        }
+        VPMX_HIP(c, hipMalloc((void **)&P->apart, sizeof(double) * 18 * (size_t)ns));
+        P->apart_ns = ns;
+    }
+    rocblas_handle hb = nullptr;
+    VPMX_TRY(blas_handle(c, &hb));
+    int64_t bo = 0, co = 0;
+    for (auto &T : P->tabs) {
+        const int np = T.form == -2 ? T.f.npairs : BW_WAVES, nl = 2 * np, ncol = (int)T.cols.size();
+        const size_t lds = sizeof(double2) * (size_t)(np + 1) * T.t.L;
+        double *Bm = P->BmT + bo, *Cm = P->CmT + co;
+        bo += 2 * (int64_t)ncol * T.t.nw * T.t.M;
+        co += 2 * (int64_t)ncol * T.t.nw * T.t.K;
+        const dim3 gf((unsigned)((T.t.M + nl - 1) / nl), (unsigned)ncol), gi((unsigned)((T.t.K + nl - 1) / nl), (unsigned)ncol);
+        if (T.form == 0)
+            hipLaunchKernelGGL(k_bsxT_fwd<0>, gf, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw, T.cols_d,
+                               P->pairs_d, g, gb, Bm);
+        else if (T.form == -1)
+            hipLaunchKernelGGL(k_bsxT_fwd<-1>, gf, dim3(bsx_threads<-1>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, P->pairs_d, g, gb, Bm);
+        else
+            hipLaunchKernelGGL(k_bsxT_fwd<-2>, gf, dim3(bsx_threads<-2>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, P->pairs_d, g, gb, Bm);
+        VPMX_HIP(c, hipGetLastError());
+        if (T.t.type == 1) {
+            // the q correlation: the forward's convolution with M and K exchanged and the
+            // conjugate spectrum
+            SpecTab tT = T.t;
+            tT.M = T.t.K;
+            tT.K = T.t.M;
+            if (T.wq.L > 0) {
+                const size_t ldsq = sizeof(double2) * (size_t)(QC_WAVES + 1) * T.t.Lq;
+                const int nseq = ncol * T.t.nw;
+                const dim3 gq((unsigned)((nseq + QC_WAVES - 1) / QC_WAVES));
+                if (T.t.Lq == 576)
+                    hipLaunchKernelGGL(k_bsx_qconv_w<0>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, tT, T.twq, T.T2,
+                                       Bm, Cm, ncol, 1);
+                else
+                    hipLaunchKernelGGL(k_bsx_qconv_w<-1>, gq, dim3(64 * QC_WAVES), ldsq, c->stream, T.wq, tT, T.twq,
+                                       T.T2, Bm, Cm, ncol, 1);
+            } else {
+                const size_t ldsq = sizeof(double2) * (size_t)(T.fq.npairs + 1) * T.t.Lq;
+                hipLaunchKernelGGL(k_bsx_qconv, dim3((unsigned)((T.t.nw + T.fq.npairs - 1) / T.fq.npairs), (unsigned)ncol),
+                                   dim3(DST_THREADS), ldsq, c->stream, T.fq, tT, T.twq, T.T2, Bm, Cm, 1);
+            }
+            VPMX_HIP(c, hipGetLastError());
+        } else {
+            const double one = 1.0, zero = 0.0;
+            VPMX_TRY(blas_check(c,
+                                rocblas_dgemm_strided_batched(hb, rocblas_operation_transpose, rocblas_operation_none,
+                                                              T.t.K, 2 * ncol, T.t.M, &one, T.G, T.t.M,
+                                                              (rocblas_stride)T.t.M * T.t.K, Bm, T.t.nw * T.t.M,
+                                                              (rocblas_stride)T.t.M, &zero, Cm, T.t.nw * T.t.K,
+                                                              (rocblas_stride)T.t.K, T.t.nw),
+                                "dgemm_strided_batched"));
+        }
+        if (T.form == 0)
+            hipLaunchKernelGGL(k_bsxT_inv<0>, gi, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw, T.cols_d,
+                               Cm, ns, P->apart);
+        else if (T.form == -1)
+            hipLaunchKernelGGL(k_bsxT_inv<-1>, gi, dim3(bsx_threads<-1>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, Cm, ns, P->apart);
+        else
+            hipLaunchKernelGGL(k_bsxT_inv<-2>, gi, dim3(bsx_threads<-2>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
+                               T.cols_d, Cm, ns, P->apart);
+        VPMX_HIP(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_bsxT_reduce, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, c->stream, ns, P->apart, abar);
+    VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
 

@@ -635,7 +635,7 @@ static int replace_with_remesh(Ctx *c) {
 //   rb = D0(pb_in) + D0(A^T BS^T (pb_faces - F^T D0(pb_in)))
 // (D0: zero-BC solve, symmetric; F: fold of face values; BS: boundary sums; A: normal
 // derivatives times -area/4pi).  t1, t2: scratch grids of g's size.
-static int pmesh_inf_adj(Ctx *c, const Grid &g, GView pb, GView rb, DBuf &t1, DBuf &t2) {
+static int pmesh_inf_adj(Ctx *c, const Grid &g, GView pb, GView rb, DBuf &t1, DBuf &t2, int slot) {
     int st = VPMX_OK;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     const int64_t G = (int64_t)n[0] * n[1] * n[2];
@@ -654,7 +654,10 @@ static int pmesh_inf_adj(Ctx *c, const Grid &g, GView pb, GView rb, DBuf &t1, DB
     int64_t nb = 0;
     // (the face-cell geometry: centroids and areas; the sources it also forms are not used)
     VPMX_TRY(normal_derivs(c, 3, g, view(y1, n), sig, cxyz, wgt, &nb));
-    VPMX_TRY(bsum_adj(c, g, nb, cxyz, view(w2, n), abar));
+    // BS^T: the spectral form on the forward's plan, else the direct sums
+    const int sp = c->adj_bs == 2 ? boundary_sums_spectral_adj(c, g, nb, view(w2, n), abar, slot) : VPMX_ERR_UNSUP;
+    if (sp == VPMX_ERR_UNSUP) VPMX_TRY(bsum_adj(c, g, nb, cxyz, view(w2, n), abar));
+    else VPMX_TRY(sp);
     VPMX_TRY(fill(c, w2, 3 * G, 0.0));
     VPMX_TRY(nderiv_adj(c, g, nb, abar, wgt + 3 * nb, view(w2, n)));
     VPMX_TRY(poisson_dirichlet(c, 3, n, h, view(w2, n), nullptr, rb, 0, SOLVE_FP64));
@@ -732,12 +735,12 @@ static int yaps_adj(Ctx *c, GView pbg, double **wb_out) {
     VPMX_TRY(face_interp_adj(c, ia, view(FB, nb), view(PB, nb), view(cb, cn), view(sb, cn)));
     // coarse solve^T, coarse Laplacian^T, sampling^T
     const Grid gc = make_grid(G.nn_coarse, G.xbound_coarse, G.dpm_coarse);
-    VPMX_TRY(pmesh_inf_adj(c, gc, view(cb, cn), view(rc, cn), c->adj_h, c->adj_i));
+    VPMX_TRY(pmesh_inf_adj(c, gc, view(cb, cn), view(rc, cn), c->adj_h, c->adj_i, 1));
     VPMX_TRY(coarse_laplacian_adj(c, cn, fp_lo, fp_hi, G.dpm_coarse, view(rc, cn), view(sb, cn)));
     VPMX_TRY(coarse_sample_adj(c, ns, ncs, cmap_d, view(sb, cn), view(PB, nb)));
     // fine block solve^T
     const Grid gb = make_grid(G.nnbl, &G.xbound_bl[0], G.dpm);
-    VPMX_TRY(pmesh_inf_adj(c, gb, view(PB, nb), view(YB, nb), c->adj_h, c->adj_i));
+    VPMX_TRY(pmesh_inf_adj(c, gb, view(PB, nb), view(YB, nb), c->adj_h, c->adj_i, 0));
     VPMX_TRY(axpy_view(c, nb, view(YB, nb), view(WB, nb)));
     *wb_out = WB;
     return VPMX_OK;
@@ -819,8 +822,6 @@ static int vpm2_adj(Ctx *c, const StageRef &S, const double *us, double au, cons
     double *sbg = (double *)ensure(c, c->adj_s, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
     double *rs = (double *)ensure(c, c->adj_r, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
     double *pbg = (double *)ensure(c, c->stage_b, sizeof(double) * 3 * (size_t)GG, &st); if (st) return st;
-    VPMX_TRY(fill(c, ubg, 3 * GG, 0.0));
-    VPMX_TRY(fill(c, sbg, 3 * GG, 0.0));
     VPMX_TRY(m2p_adj(c, gg, S.x, S.q, qs, S.n, us, au, gs, ag, view((double *)S.u, gn), view((double *)S.s, gn),
                      view(ubg, gn), view(sbg, gn), xacc));
     const GView Rg = view((double *)S.R, gn, w[0] - 1, w[1] - 1, w[2] - 1, nb);
@@ -957,7 +958,8 @@ static int rk2_step_adj(Ctx *c, const Ctx::AdjStep &rec, double dt) {
 // J of the probes after nsteps RK2 steps (each with its remesh) of the resident particles, and
 // its gradient with respect to the starting QP(1:3) and XP (in adj_qb / adj_xb, 3 per particle).
 // Checkpoints every ckpt steps; the backward pass recomputes one segment of steps at a time,
-// keeping its records.
+// keeping its records.  ckpt = 0: the forward pass keeps every step's record (no recomputation;
+// about 6 grids and 7 particle arrays of HBM per step).
 static int adjoint_rollout(Ctx *c, int nsteps, double dt, const double uinf[3], int ckpt, const double *pts,
                            int64_t np, double *J) {
     int st = VPMX_OK;
@@ -965,17 +967,22 @@ static int adjoint_rollout(Ctx *c, int nsteps, double dt, const double uinf[3], 
     VPMX_TRY(adjoint_supported(c));
     if (c->prm.ncell_rem != 1) return fail(c, VPMX_ERR_UNSUP, "ncell_rem must be 1");
     const int nr = c->neqpm + 1;
-    const int nseg = (nsteps + ckpt - 1) / ckpt;
+    const bool keep_all = ckpt <= 0;  // every step's record from the forward pass, no recomputation
+    if (keep_all) ckpt = nsteps > 0 ? nsteps : 1;
+    const int nseg = keep_all ? 0 : (nsteps + ckpt - 1) / ckpt;
     if ((int)c->adj_ck.size() < nseg) c->adj_ck.resize(nseg);
     if ((int)c->adj_steps.size() < ckpt) c->adj_steps.resize(ckpt);
     std::vector<int64_t> ckn(nseg), fwd_n(nsteps);
-    for (int k = 0; k < nsteps; k++) {
+    for (int k = 0; k < nsteps && keep_all; k++) VPMX_TRY(rk2_step_rec(c, dt, uinf, k + 1, &c->adj_steps[k]));
+    // the last segment keeps its records from this pass (the backward pass starts there)
+    const int klast = keep_all ? 0 : (nseg - 1) * ckpt;
+    for (int k = 0; k < nsteps && !keep_all; k++) {
         if (k % ckpt == 0) {
             ckn[k / ckpt] = c->np;
             VPMX_TRY(save_buf(c, c->adj_ck[k / ckpt][0], c->xp.p, sizeof(double) * 3 * (size_t)c->np));
             VPMX_TRY(save_buf(c, c->adj_ck[k / ckpt][1], c->qp.p, sizeof(double) * nr * (size_t)c->np));
         }
-        VPMX_TRY(rk2_step_rec(c, dt, uinf, k + 1, nullptr));
+        VPMX_TRY(rk2_step_rec(c, dt, uinf, k + 1, k >= klast ? &c->adj_steps[k - klast] : nullptr));
         fwd_n[k] = c->np;
     }
     const int64_t nK = c->np;
@@ -983,8 +990,13 @@ static int adjoint_rollout(Ctx *c, int nsteps, double dt, const double uinf[3], 
     double *XB = (double *)ensure(c, c->adj_xb, pbK, &st); if (st) return st;
     double *QB = (double *)ensure(c, c->adj_qb, pbK, &st); if (st) return st;
     VPMX_TRY(adjoint_probe_energy(c, (const double *)c->xp.p, (const double *)c->qp.p, nK, pts, np, J, QB, XB));
+    for (int k = nsteps - 1; k >= 0 && keep_all; k--) VPMX_TRY(rk2_step_adj(c, c->adj_steps[k], dt));
     for (int s = nseg - 1; s >= 0; s--) {
         const int k0 = s * ckpt, k1 = std::min(nsteps, k0 + ckpt);
+        if (s == nseg - 1) {  // recorded by the forward pass
+            for (int k = k1 - 1; k >= k0; k--) VPMX_TRY(rk2_step_adj(c, c->adj_steps[k - k0], dt));
+            continue;
+        }
         const int64_t n0 = ckn[s];
         double *dx = (double *)ensure(c, c->xp, sizeof(double) * 3 * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
         double *dq = (double *)ensure(c, c->qp, sizeof(double) * nr * (size_t)(n0 ? n0 : 1), &st); if (st) return st;
@@ -1095,7 +1107,8 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->sort.xs, &c->sort.qs, &c->sort.rowcnt, &c->dst_save, &c->fold_tab, &c->zt_prev, &c->dst_work64,
                     &c->tune_scratch, &c->adj_a, &c->adj_b, &c->adj_c, &c->adj_d, &c->adj_e, &c->adj_f,
                     &c->adj_g, &c->adj_h, &c->adj_i, &c->adj_p, &c->adj_q, &c->adj_u, &c->adj_s, &c->adj_r,
-                    &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm};
+                    &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm, &c->adj_pu,
+                    &c->adj_ps};
     for (DBuf *b : bufs) release(*b);
     for (auto &r : c->adj_steps)
         for (DBuf &b : r.b) release(b);
@@ -1158,6 +1171,11 @@ int32_t vpmx_set_option(vpmx_ctx *p, const char *name, int32_t value) {
     if (k == "boundsum") {
         if (value < 0 || value > 2) return fail(c, VPMX_ERR_ARG, "boundsum: 0 lattice, 1 direct, 2 spectral");
         c->bs_mode = value;
+    } else if (k == "adjoint_boundsum") {
+        if (value != 1 && value != 2) return fail(c, VPMX_ERR_ARG, "adjoint_boundsum: 1 direct, 2 spectral");
+        c->adj_bs = value;
+    } else if (k == "adjoint_m2p_scatter") {
+        c->adj_m2p_scatter = value != 0;
     } else if (k == "dst_reuse") {
         c->dst_reuse = value != 0;
     } else if (k == "tuning") {
@@ -1398,7 +1416,7 @@ int32_t vpmx_adjoint_rollout(vpmx_ctx *p, const double *xp, const double *qp, in
     Ctx *c = C(p);
     if (!c || !J || !dq || !dx || !uinf || (nprobe > 0 && !probes)) return VPMX_ERR_ARG;
     if (nprobe < 0 || nprobe > 0x7fffffffLL) return fail(c, VPMX_ERR_ARG, "probe count out of range");
-    if (nsteps < 0 || ckpt < 1) return fail(c, VPMX_ERR_ARG, "nsteps >= 0 and ckpt >= 1");
+    if (nsteps < 0 || ckpt < 0) return fail(c, VPMX_ERR_ARG, "nsteps >= 0 and ckpt >= 0");
     VPMX_HIP(c, hipSetDevice(c->device));
     if (!c->have_prm) return fail(c, VPMX_ERR_STATE, "no parameters set");
     VPMX_TRY(upload_particles(c, xp, qp, nvr, neqpm));

@@ -69,6 +69,9 @@ struct Ctx : vpmx_ctx {
         int64_t n0 = 0, n1 = 0;
     };
     std::vector<AdjStep> adj_steps;
+    int adj_bs = 2;  // transposed boundary sums: 2 spectral, 1 direct
+    bool adj_m2p_scatter = false;  // M2P^T by FP64 atomics instead of the sorted gather
+    DBuf adj_pu, adj_ps;
     std::vector<std::array<DBuf, 2>> adj_ck;
     int neqpm = 3;
     double ni = 0.0;  // kinematic viscosity NI of the diffusion mode (resident calls)

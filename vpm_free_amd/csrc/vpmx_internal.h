@@ -135,6 +135,9 @@ int boundary_sums_panels(Ctx *c, const Grid &g, int64_t nb, const double *sig, c
 // block's faces are too long for its in-LDS FFTs, so the caller can use the lattice kernel.
 // slot: 0 the fine block, 1 the coarse grid (each keeps its own plan across calls).
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *a, GView sol, int slot);
+// Its transpose: abar(c, s) = sum over boundary nodes t of gbar(c, t) / |x_t - x_s| (the adjoint
+// of the sources a); VPMX_ERR_UNSUP where the forward plan does not build.
+int boundary_sums_spectral_adj(Ctx *c, const Grid &g, int64_t nbound, GView gb, double *abar, int slot);
 void free_bspec(Ctx *c);
 // Column-major C_b = A_b op(B_b) for b < batch (FP64, rocBLAS on the context's stream).
 int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
