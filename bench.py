@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-stride", type=int, default=256,
                     help="boundary-sum target stride of the bounded CPU sample")
     ap.add_argument("--forms", default="fp64,mixed", help="forms to time: fp64, mixed or both")
+    ap.add_argument("--only", choices=("batch", "adjoint"), default=None,
+                    help="run one side leg alone and print its JSON object (for profiling; not the bench line)")
     ap.add_argument("--adjoint-steps", type=int, default=10, help="steps of the adjoint leg (0: skip)")
     ap.add_argument("--adjoint-ckpt", type=int, default=2, help="checkpoint spacing of the adjoint leg")
     ap.add_argument("--batch-steps", type=int, default=5,
@@ -267,6 +269,12 @@ def main():
         sys.exit(2)
     from vpm_free_amd import vpm as V
 
+    if args.only == "batch":
+        print(json.dumps(batch_leg(V, args.batch_steps)))
+        return
+    if args.only == "adjoint":
+        print(json.dumps(adjoint_leg(V, args.adjoint_steps, args.adjoint_ckpt)))
+        return
     t0 = time.perf_counter()
     xp, qp = workload(args.h)
     log(f"workload: {xp.shape[0]} particles generated in {time.perf_counter() - t0:.1f}s")

@@ -52,6 +52,9 @@ struct DstArgs {
     double norm;
     const double *sn;      // sin(j pi / N), j <= N/2 (half-length passes)
     int ncomp;             // components (one-wave kernels: all in one grid)
+    int order;             // one-wave kernels' batch order: 0 the two halves of a batch pair on
+                           // work-groups w and w+8 (one XCD) at once, 1 one after the other on
+                           // one work-group (the second half's loads find the lines in L2)
 };
 
 // Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
@@ -893,7 +896,10 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     // 128-byte row segment in FP32) go to work-groups w and w+8, which share an XCD (and its
     // L2); gridDim.x is a multiple of 16
     const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
-    auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
+    auto batch_of = [&](int it) {
+        return a.order ? (it >> 1) * 2 * G + 2 * (int)blockIdx.x + (it & 1)
+                       : 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1);
+    };
     const int nbt = nbatch * a.ncomp;
     int it = 0, gb = batch_of(0);
     int comp = gb / nbatch, b = gb - comp * nbatch;
@@ -1082,7 +1088,10 @@ __device__ __forceinline__ void dst_zmix_body(const DstArgs &a, const WavePlan &
     bool vok = false;
     double fx[2];
     const int G = gridDim.x, w8 = blockIdx.x & 7, slot = blockIdx.x >> 3, S2 = G >> 4;
-    auto batch_of = [&](int it) { return 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1); };
+    auto batch_of = [&](int it) {
+        return a.order ? (it >> 1) * 2 * G + 2 * (int)blockIdx.x + (it & 1)
+                       : 2 * (it * (G >> 1) + w8 * S2 + (slot >> 1)) + (slot & 1);
+    };
     const int nbt = nbatch * a.ncomp;
     int it = 0, gb = batch_of(0);
     int comp = gb / nbatch, b = gb - comp * nbatch;
@@ -1409,8 +1418,8 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
     // kernel variants: with a compile-time plan the registers capped for four waves per SIMD
     // (six for the FP32 x inverse of the 576-point plan) or not -- the tuner times both
-    const void *fns[2] = {nullptr, nullptr};
-    const char *names[2] = {"uncapped", "capped"};
+    const void *fns[4] = {nullptr, nullptr, nullptr, nullptr};
+    const char *names[4] = {"uncapped", "capped", "uncapped, batch halves in turn", "capped, batch halves in turn"};
     int nf = 1;
     constexpr bool dbl = sizeof(T) == 8;
     // (each length's plan is instantiated only for the form that uses it)
@@ -1445,6 +1454,11 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
                : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
                : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
     }
+    // each kernel in both batch orders: variants nf.. are the first nf with order 1
+    if (nf == 1) names[1] = names[2];
+    for (int q = 0; q < nf; q++) fns[nf + q] = fns[q];
+    const int nkern = nf;
+    nf *= 2;
     // eight padded sequences and the twiddles (the kernel's LDS layout)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const int S = wave_pad_shift(sizeof(T) == 8, MODE, fix);
@@ -1477,6 +1491,7 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             }
             DstArgs t = a;
             t.w = t.wo = (double *)c->tune_scratch.p;
+            t.order = fi >= nkern ? 1 : 0;
             WavePlan Pt = P;
             void *targs[] = {&t, &Pt};
             VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
@@ -1485,6 +1500,7 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         VPMX_TRY(tune(c, key, nf, fns, names, lds, DST_THREADS, trial, &ch));
     }
     const void *fn = fns[ch.fi];
+    a.order = ch.fi >= nkern ? 1 : 0;
     dim3 grid = grid_of(ch.per_cu);
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
@@ -1567,19 +1583,21 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
     char key[160];
     std::snprintf(key, sizeof key, "sine transforms z pass FP32/FP64 L=%d/%d m=%d lines=%lld comps=%d", f32->L, f64->L,
                   a.m, (long long)a.nlines, ncomp);
-    const char *names[1] = {"FP32 forward, FP64 inverse"};
+    const char *names[2] = {"FP32 forward, FP64 inverse", "FP32 forward, FP64 inverse, batch halves in turn"};
+    const void *fns2[2] = {fn, fn};
     Ctx::Tuned ch;
     {
         int st = VPMX_OK;
         // trials: FP32 input and FP64 output in scratch
         const size_t wb = (sizeof(float) + sizeof(double)) * (size_t)a.wcs * ncomp;
-        auto trial = [&](int, int pc) -> int {
+        auto trial = [&](int fi, int pc) -> int {
             if (!c->tune_scratch.p || c->tune_scratch.bytes < wb) {
                 void *p = ensure(c, c->tune_scratch, wb, &st);
                 if (!p) return st;
                 VPMX_TRY(fill(c, (double *)p, (int64_t)(c->tune_scratch.bytes / sizeof(double)), 0.0));
             }
             DstArgs t = a;
+            t.order = fi;
             t.w = (double *)c->tune_scratch.p;
             t.wo = (double *)((char *)c->tune_scratch.p + sizeof(float) * (size_t)a.wcs * ncomp);
             WavePlan p32 = P32, p64 = P64;
@@ -1588,8 +1606,9 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
             VPMX_HIP(c, hipLaunchKernel(fn, grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
             return VPMX_OK;
         };
-        VPMX_TRY(tune(c, key, 1, &fn, names, lds, DST_THREADS, trial, &ch));
+        VPMX_TRY(tune(c, key, 2, fns2, names, lds, DST_THREADS, trial, &ch));
     }
+    a.order = ch.fi;
     dim3 grid = grid_of(ch.per_cu);
     const double2 *tw64 = f64->tw;
     WavePlan p32 = P32, p64 = P64;
