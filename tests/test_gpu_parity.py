@@ -474,11 +474,13 @@ def test_boundary_sums_spectral_long_axis(vpmlib):
     assert rel_l2(out["spectral"], out["direct"]) <= TOL
 
 
-@pytest.mark.parametrize("prec,tol", [(0, TOL), (1, 1e-6)])
+@pytest.mark.parametrize("prec,tol", [(0, TOL), (1, 2e-6)])
 def test_pmesh_inf_reuse_full_size(vpmlib, prec, tol):
     """The second fine solve of pmesh_inf starts from the first one's x/y forward transforms
     less those of the folded Dirichlet values; at the 289-node block (compile-time plans, the
-    spectral boundary sums' 576-point path) it must agree with the two full solves."""
+    spectral boundary sums' 576-point path) it must agree with the two full solves.  In the
+    mixed form the two orders differ by FP32 rounding (measured 6e-7..1.0e-6 depending on how
+    the transforms form their twiddle powers); a wrong fold or transform shows as 1e-3 or more."""
     V = vpmlib
     nn = np.array([289, 289, 289], np.int32)
     dpm = np.array([1 / 64] * 3)

@@ -300,8 +300,16 @@ __device__ __forceinline__ void wave_stage(Buf buf, const C *__restrict__ tw, in
 #pragma unroll
             for (int r = 0; r < R; r++) v[i][r] = buf[j + r * nbf];
             if (k0) {
+                if constexpr (R == 4) {  // one table read, the higher powers in registers
+                    const C w1 = tw[k0 * twstep];
+                    const C w2 = cmul(w1, w1);
+                    v[i][1] = cmul(v[i][1], w1);
+                    v[i][2] = cmul(v[i][2], w2);
+                    v[i][3] = cmul(v[i][3], cmul(w2, w1));
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+                    for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+                }
             }
             if constexpr (R <= 4) dft_reg<R>(v[i]);
             dst[i] = q * Ns * R + k0;
@@ -377,8 +385,30 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
             for (int r = 0; r < R; r++) v[i][r] = buf[j + r * nbf];
             if constexpr (Ns > 1) {
                 // no branch on k0 == 0: tw[0] = (1, 0) leaves the value unchanged
+                if constexpr (R == 4) {
+                    // one table read per butterfly, the higher powers by multiplication in
+                    // registers: the passes are bound by LDS instructions, not by arithmetic
+                    const C w1 = tw[k0 * twstep];
+                    const C w2 = cmul(w1, w1);
+                    v[i][1] = cmul(v[i][1], w1);
+                    v[i][2] = cmul(v[i][2], w2);
+                    v[i][3] = cmul(v[i][3], cmul(w2, w1));
+                } else if constexpr (R == 9) {
+                    // two table reads (w, w^3), the other powers at most two products deep
+                    const C w1 = tw[k0 * twstep], w3 = tw[3 * k0 * twstep];
+                    const C w2 = cmul(w1, w1), w4 = cmul(w1, w3), w6 = cmul(w3, w3);
+                    v[i][1] = cmul(v[i][1], w1);
+                    v[i][2] = cmul(v[i][2], w2);
+                    v[i][3] = cmul(v[i][3], w3);
+                    v[i][4] = cmul(v[i][4], w4);
+                    v[i][5] = cmul(v[i][5], cmul(w2, w3));
+                    v[i][6] = cmul(v[i][6], w6);
+                    v[i][7] = cmul(v[i][7], cmul(w3, w4));
+                    v[i][8] = cmul(v[i][8], cmul(w4, w4));
+                } else {
 #pragma unroll
-                for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+                    for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
+                }
             }
             if constexpr (R <= 4 || R == 9) dft_reg<R>(v[i]);
             dst[i] = q * Ns * R + k0;
