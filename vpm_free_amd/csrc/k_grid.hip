@@ -96,6 +96,83 @@ int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView v
     return VPMX_OK;
 }
 
+// curl and stretching in one pass (calc_velocity_serial_3d, vpm_gcalc.f90:1-132): a work-group
+// owns a 64 x 4 column tile and marches through CS_KC planes of z, keeping the velocity of
+// three planes (with a one-node halo in x and y) in LDS; each plane's velocity is formed once
+// per tile (halo nodes by the neighbouring tiles too) with k_curl's expressions, and the
+// stretching term reads its velocity neighbours from LDS with k_stretch's expressions -- the
+// same values bit for bit, without writing the velocity grid and reading it back.
+constexpr int CS_TX = 64, CS_TY = 4, CS_KC = 32, CS_HX = CS_TX + 2, CS_HY = CS_TY + 2;
+__global__ void __launch_bounds__(256) k_curl_stretch(int nx, int ny, int nz, double ix2, double iy2, double iz2,
+                                                       GView psi, GView rhs, GView vel, GView out) {
+#pragma clang fp contract(off)
+    __shared__ double V[3][3][CS_HY][CS_HX];  // [plane mod 3][component][y][x]
+    const int i0 = blockIdx.x * CS_TX, j0 = blockIdx.y * CS_TY, kb = blockIdx.z * CS_KC;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
+    auto fill = [&](int kk) {  // velocity of plane kk on the tile and its halo
+        const int sl = (kk + 3) % 3;
+        for (int q = threadIdx.x; q < CS_HX * CS_HY; q += 256) {
+            const int hx = q % CS_HX, hy = q / CS_HX, i = i0 - 1 + hx, j = j0 - 1 + hy;
+            double u = 0.0, v = 0.0, w = 0.0;
+            if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && kk >= 1 && kk <= nz - 2) {
+                const int64_t o = psi.at(i, j, kk);
+                const int64_t xp = o + 1, xm = o - 1, yp = o + psi.py, ym = o - psi.py, zp = o + psi.pz, zm = o - psi.pz;
+                double dpsidx1 = (p1[xp] - p1[xm]) * ix2, dpsidx2 = (p2[xp] - p2[xm]) * ix2;
+                double dpsidy0 = (p0[yp] - p0[ym]) * iy2, dpsidy2 = (p2[yp] - p2[ym]) * iy2;
+                double dpsidz0 = (p0[zp] - p0[zm]) * iz2, dpsidz1 = (p1[zp] - p1[zm]) * iz2;
+                u = dpsidy2 - dpsidz1;
+                v = -(dpsidx2 - dpsidz0);
+                w = dpsidx1 - dpsidy0;
+            }
+            V[sl][0][hy][hx] = u;
+            V[sl][1][hy][hx] = v;
+            V[sl][2][hy][hx] = w;
+        }
+    };
+    fill(kb - 1);
+    fill(kb);
+    const int kend = kb + CS_KC < nz ? kb + CS_KC : nz;
+    const int i = i0 + tx, j = j0 + ty;
+    for (int k = kb; k < kend; k++) {
+        fill(k + 1);
+        __syncthreads();
+        const int sl = k % 3, sp = (k + 1) % 3, sm = (k + 2) % 3;
+        if (i < nx && j < ny) {
+            const int64_t ov = vel.at(i, j, k);
+            for (int c = 0; c < 3; c++) vel.p[c * vel.cs + ov] = V[sl][c][ty + 1][tx + 1];
+            double r[3] = {0.0, 0.0, 0.0};
+            if (i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3 && k >= 2 && k <= nz - 3) {
+                const double *R0 = rhs.p, *R1 = rhs.p + rhs.cs, *R2 = rhs.p + 2 * rhs.cs;
+                const int64_t ro = rhs.at(i, j, k);
+                const double a0p = R0[ro + 1], a0m = R0[ro - 1], a1p = R1[ro + rhs.py], a1m = R1[ro - rhs.py];
+                const double a2p = R2[ro + rhs.pz], a2m = R2[ro - rhs.pz];
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    double wdudx = -(a0p * V[sl][m][ty + 1][tx + 2] - (a0m)*V[sl][m][ty + 1][tx]) * ix2;
+                    double wdvdy = -(a1p * V[sl][m][ty + 2][tx + 1] - (a1m)*V[sl][m][ty][tx + 1]) * iy2;
+                    double wdwdz = -(a2p * V[sp][m][ty + 1][tx + 1] - (a2m)*V[sm][m][ty + 1][tx + 1]) * iz2;
+                    r[m] = wdudx + wdvdy + wdwdz;
+                }
+            }
+            const int64_t oo = out.at(i, j, k);
+            out.p[oo] = r[0];
+            out.p[out.cs + oo] = r[1];
+            out.p[2 * out.cs + oo] = r[2];
+        }
+        __syncthreads();  // the next iteration refills the slot of plane k - 1
+    }
+}
+int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView rhs, GView vel, GView out) {
+    if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
+    const dim3 grid((unsigned)((n[0] + CS_TX - 1) / CS_TX), (unsigned)((n[1] + CS_TY - 1) / CS_TY),
+                    (unsigned)((n[2] + CS_KC - 1) / CS_KC));
+    hipLaunchKernelGGL(k_curl_stretch, grid, dim3(256), 0, c->stream, n[0], n[1], n[2], 1.0 / (2 * h[0]),
+                       1.0 / (2 * h[1]), 1.0 / (2 * h[2]), psi, rhs, vel, out);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
 // diffuse_vort_3d (vpm_gcalc.f90:138-210), neqpm = 3: out = -NI * lap(RHS) per component on the
 // interior nodes (7-point second differences, the reference's term order), 0 elsewhere.
 __global__ void __launch_bounds__(256) k_diffuse(int nx, int ny, int nz, double dx2, double dy2, double dz2,

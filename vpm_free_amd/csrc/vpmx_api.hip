@@ -520,8 +520,8 @@ static int run_vpm(Ctx *c, int whattodo, const double *xp, const double *qp, dou
     GView psi = global_psi(c);
     GView rg = global_rhs(c);
     mark(c, ST_FD, 0);
-    VPMX_TRY(curl_kernel(c, gn, G.dpm, psi, view(vel, gn)));
-    if (whattodo == 2) VPMX_TRY(stretch_kernel(c, gn, G.dpm, rg, view(vel, gn), view(str, gn)));
+    if (whattodo == 2) VPMX_TRY(curl_stretch_kernel(c, gn, G.dpm, psi, rg, view(vel, gn), view(str, gn)));
+    else VPMX_TRY(curl_kernel(c, gn, G.dpm, psi, view(vel, gn)));
     mark(c, ST_FD, 1);
     used[ST_FD] = true;
     if (whattodo == 2) {

@@ -153,6 +153,8 @@ void free_tree(Ctx *c);
 int fill(Ctx *c, double *p, int64_t n, double v);
 int zero_faces(Ctx *c, const int n[3], GView v, int ncomp = 3);  // the six faces of a grid (k_poisson.hip)
 int curl_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView vel);
+// curl and stretching in one pass (the velocity grid written too); same values as the two kernels
+int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView rhs, GView vel, GView out);
 int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView vel, GView out);
 int diffuse_kernel(Ctx *c, const int n[3], const double h[3], double vis, GView rhs, GView out);
 int coarse_sample(Ctx *c, int ncomp, const int nsamp[3], int nc, const int *cmap_dev, GView fine, GView coarse,
