@@ -487,15 +487,15 @@ constexpr int DST_PF = (2 * DST_EMAX + DST_THREADS - 1) / DST_THREADS;
 // line DST_THREADS / nl neighbouring threads, which walk its points (the line index is
 // fixed per thread); y and z passes walk columns (neighbouring threads take neighbouring
 // lines, 8*nl contiguous bytes per point).  Both cover m <= DST_PF * DST_THREADS / nl.
-template <int MODE>
+template <int MODE, int NT = DST_THREADS>
 __device__ __forceinline__ void dst_elem(int m, int nl, int u, int *line, int *pt) {
     if (MODE == DST_XF || MODE == DST_XI) {
-        const int tpl = DST_THREADS / nl;
+        const int tpl = NT / nl;
         *line = threadIdx.x / tpl;
         *pt = (threadIdx.x & (tpl - 1)) + u * tpl;
     } else {
         *line = threadIdx.x & (nl - 1);
-        *pt = threadIdx.x / nl + u * (DST_THREADS / nl);
+        *pt = threadIdx.x / nl + u * (NT / nl);
     }
 }
 
@@ -756,11 +756,15 @@ __device__ __forceinline__ void wave_pre_odd(Buf B, int m, int lane, const doubl
 }
 
 
-// Addressing in the one-wave kernels (16 lines per batch).  dst_elem gives every thread one
-// line (x passes: 32 threads per line; y and z passes: line tid mod 16) and the points
-// pt0 + 32u of it, so per batch a thread forms its line's offsets once: element u is read at
-// src + u * src_step, written at dst + u * dst_step, and sits in LDS at a fixed stride.
+// Addressing in the one-wave kernels (NL = 16 or 32 lines per batch, 32 threads per line).
+// dst_elem gives every thread one line (x passes: 32 threads per line; y and z passes: line
+// tid mod NL) and the points pt0 + 32u of it, so per batch a thread forms its line's offsets
+// once: element u is read at src + u * src_step, written at dst + u * dst_step, and sits in
+// LDS at a fixed stride.  NL = 32 in FP32 gives every point of a y or z batch a whole
+// 128-byte row segment (16 lines leave the other half to another batch, which must find it in
+// L2).
 constexpr int WV_NL = 16, WV_PS = DST_THREADS / WV_NL;
+static_assert(WV_PS == 32, "32 threads per line");
 struct WaveAddr {
     int64_t src, src_step;  // work array, or the RHS view (x forward)
     int64_t dst, dst_step;  // work array, or the output view (x inverse)
@@ -788,10 +792,10 @@ __device__ __forceinline__ WaveAddr wave_addr(const DstArgs &a, int comp, int gl
 // whose element does not exist reads element 0 instead), so no branch separates the loads and
 // none has to wait for another; `ok` says whether the line exists, and the fill keeps only
 // the points below m.  PT: double for the x forward pass (the RHS), converted at the fill.
-template <int MODE, class T, class PT>
+template <int MODE, int NL, class T, class PT>
 __device__ __forceinline__ void wave_prefetch(const DstArgs &a, const T *w, int comp, int b, int line, int pt0,
                                               int m, PT (&v)[DST_PF], bool &ok, double (&fx)[2]) {
-    const WaveAddr ad = wave_addr<MODE>(a, comp, b * WV_NL + line, pt0);
+    const WaveAddr ad = wave_addr<MODE>(a, comp, b * NL + line, pt0);
     ok = ad.ok;
 #pragma unroll
     for (int u = 0; u < DST_PF; u++) {
@@ -801,8 +805,8 @@ __device__ __forceinline__ void wave_prefetch(const DstArgs &a, const T *w, int 
     }
     if (MODE == DST_XF) {
         fx[0] = fx[1] = 0.0;
-        const int gl = b * WV_NL + (int)threadIdx.x;
-        if (a.has_bc && (int)threadIdx.x < WV_NL && gl < a.nlines) {
+        const int gl = b * NL + (int)threadIdx.x;
+        if (a.has_bc && (int)threadIdx.x < NL && gl < a.nlines) {
             const int j = gl % a.my, k = gl / a.my;
             const double *bp = a.bc.p + comp * a.bc.cs;
             fx[0] = bp[a.bc.at(0, j + 1, k + 1)];
@@ -846,35 +850,35 @@ __host__ __device__ constexpr bool const_roots(bool dbl, int mode, int fix) {
 // T: arithmetic and work-array type (double: plain FP64 form, half-length transforms;
 // float: the mixed form, odd-extension transforms, whose RHS, Dirichlet values and output
 // stay FP64 and whose inner passes run in FP32).  P.L is the FFT length (m+1 or 2(m+1)).
-template <int MODE, int MAXR, class T, int FIX>
+template <int MODE, int MAXR, class T, int FIX, int NL>
 __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &P) {
 #pragma clang fp contract(off)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     extern __shared__ double2 lds_raw[];
     C *lds = (C *)lds_raw;
-    __shared__ double lamxy[16];
-    __shared__ double foldx[32];
+    __shared__ double lamxy[NL];
+    __shared__ double foldx[2 * NL];
     __shared__ double lzs[MODE == DST_Z ? WAVE_LMAX : 1];  // lambda_z, read once per kernel
-    constexpr int nl = WV_NL;
+    constexpr int nl = NL, NT = NL * WV_PS;
     constexpr bool ODD = sizeof(T) == 4;
     // N: FFT length (2(m+1) in the odd-extension form, m+1 in the half-length form); both
     // constants with a compile-time plan
     const int N = FIX >= 0 ? fix_len<FIX>() : P.L;
     const int m = FIX >= 0 ? (ODD ? N / 2 - 1 : N - 1) : a.m;
     const int nbatch = (int)((a.nlines + nl - 1) / nl);
-    // eight padded sequences (one per wave), then the twiddles
+    // NL / 2 padded sequences (one per wave), then the twiddles
     constexpr int S = wave_pad_shift(sizeof(T) == 8, MODE, FIX);
     constexpr bool CROOTS = const_roots(sizeof(T) == 8, MODE, FIX);
     const int Np = pad_len<S>(N);
     C *A = lds;
-    C *twl = lds + 8 * Np;  // exp(-2 pi i t / N)
+    C *twl = lds + (NL / 2) * Np;  // exp(-2 pi i t / N)
     T *Ad = (T *)A;
     // LDS scalar of point pt (element pt + 1) of line `line` in a batch: lines 2p and 2p+1 are
     // the real and imaginary parts of sequence p
     auto slot_of = [&](int line, int pt) { return 2 * ((line >> 1) * Np + pad_pos<S>(pt + 1)) + (line & 1); };
     // this thread's line and first point (dst_elem), and the LDS stride of its elements
-    const int tline = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x / WV_PS : (int)threadIdx.x & (WV_NL - 1);
-    const int tpt0 = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x & (WV_PS - 1) : (int)threadIdx.x / WV_NL;
+    const int tline = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x / WV_PS : (int)threadIdx.x & (NL - 1);
+    const int tpt0 = (MODE == DST_XF || MODE == DST_XI) ? (int)threadIdx.x & (WV_PS - 1) : (int)threadIdx.x / NL;
     constexpr int SSTEP = 2 * (WV_PS + (S ? WV_PS >> S : 0));
     const int slot0 = slot_of(tline, tpt0);
     // odd-extension form: the mirror element L - (pt + 1) of point pt, stepping down by SSTEP
@@ -885,9 +889,9 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     // (S = 0: the plain pointer, so those kernels compile exactly as unpadded code)
     using Buf = typename std::conditional<S == 0, C *, PadBuf<C, S>>::type;
     const Buf B{A + wave * Np};
-    for (int t = tid; t < N; t += DST_THREADS) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
+    for (int t = tid; t < N; t += NT) twl[t] = Cx<C>::mk((T)a.tw[t].x, (T)a.tw[t].y);
     if (MODE == DST_Z)
-        for (int t = tid; t < m; t += DST_THREADS) lzs[t] = a.lam[2][t];
+        for (int t = tid; t < m; t += NT) lzs[t] = a.lam[2][t];
     using PT = typename std::conditional<MODE == DST_XF, double, T>::type;
     PT v[DST_PF];
     bool vok = false;
@@ -903,7 +907,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     const int nbt = nbatch * a.ncomp;
     int it = 0, gb = batch_of(0);
     int comp = gb / nbatch, b = gb - comp * nbatch;
-    if (gb < nbt) wave_prefetch<MODE, T>(a, (T *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
+    if (gb < nbt) wave_prefetch<MODE, NL, T>(a, (T *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
     for (; gb < nbt; ++it) {
         T *w = (T *)a.wo + comp * a.wcs;
 #pragma unroll
@@ -918,14 +922,14 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
             // lines on the y or z faces of the window (a thread's line is fixed in x passes):
             // fold their Dirichlet neighbours; the x ends are folded in wave_pre
             int line, pt;
-            dst_elem<MODE>(m, nl, 0, &line, &pt);
+            dst_elem<MODE, NT>(m, nl, 0, &line, &pt);
             const int gl = b * nl + line;
             const int j = gl % a.my, k = gl / a.my;
             if (dst_line_ok<MODE>(a, gl) && (j == 0 || j == a.my - 1 || k == 0 || k == a.mz - 1)) {
                 const double *bp = a.bc.p + comp * a.bc.cs;
 #pragma unroll 1
                 for (int u = 0; u < DST_PF; u++) {
-                    dst_elem<MODE>(m, nl, u, &line, &pt);
+                    dst_elem<MODE, NT>(m, nl, u, &line, &pt);
                     if (pt >= m) continue;
                     double f = 0.0;
                     if (j == 0) f += bp[a.bc.at(pt + 1, 0, k + 1)] * a.ih2[1];
@@ -950,7 +954,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
         __syncthreads();
         const int gbn = batch_of(it + 1);
         const int compn = gbn / nbatch, bn = gbn - compn * nbatch;
-        if (gbn < nbt) wave_prefetch<MODE, T>(a, (T *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
+        if (gbn < nbt) wave_prefetch<MODE, NL, T>(a, (T *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
         const double *fold = (MODE == DST_XF && a.has_bc) ? foldx + 4 * wave : nullptr;
         if constexpr (ODD) {
             // the fill wrote the odd extension; with Dirichlet values the folds change
@@ -1016,9 +1020,9 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     }
 }
 
-template <int MODE, int MAXR, class T, int FIX>
-__global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P) {
-    dst_wave_body<MODE, MAXR, T, FIX>(a, P);
+template <int MODE, int MAXR, class T, int FIX, int NL = WV_NL>
+__global__ void __launch_bounds__(NL * WV_PS) k_dst_wave(DstArgs a, WavePlan P) {
+    dst_wave_body<MODE, MAXR, T, FIX, NL>(a, P);
 }
 // The same with the registers capped for W waves per SIMD (W = 4: 128 VGPRs, two 8-wave
 // work-groups per CU; W = 6: 80 VGPRs, three).  Measured on the 289- and 273-node blocks:
@@ -1026,10 +1030,10 @@ __global__ void __launch_bounds__(DST_THREADS) k_dst_wave(DstArgs a, WavePlan P)
 // the 273-node box gain from four waves despite a few spilled registers, the FP32 x inverse
 // pass of the 289-node block from six; the other passes lose more to the spills than the
 // occupancy brings (the FP32 y pass at six waves: 8 % slower), or already fit.
-template <int W, int MODE, int MAXR, class T, int FIX>
-__global__ void __launch_bounds__(DST_THREADS) __attribute__((amdgpu_waves_per_eu(W)))
+template <int W, int MODE, int MAXR, class T, int FIX, int NL = WV_NL>
+__global__ void __launch_bounds__(NL * WV_PS) __attribute__((amdgpu_waves_per_eu(W)))
 k_dst_wave_cap(DstArgs a, WavePlan P) {
-    dst_wave_body<MODE, MAXR, T, FIX>(a, P);
+    dst_wave_body<MODE, MAXR, T, FIX, NL>(a, P);
 }
 __host__ __device__ constexpr int wave_cap(bool dbl, int mode, int fix) {
     if (dbl) return ((fix == 2 && (mode == DST_XF || mode == DST_Y)) || (fix == 3 && mode == DST_XI)) ? 4 : 0;
@@ -1095,7 +1099,7 @@ __device__ __forceinline__ void dst_zmix_body(const DstArgs &a, const WavePlan &
     const int nbt = nbatch * a.ncomp;
     int it = 0, gb = batch_of(0);
     int comp = gb / nbatch, b = gb - comp * nbatch;
-    if (gb < nbt) wave_prefetch<DST_Z, float>(a, (const float *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
+    if (gb < nbt) wave_prefetch<DST_Z, WV_NL, float>(a, (const float *)a.w + comp * a.wcs, comp, b, tline, tpt0, m, v, vok, fx);
     for (; gb < nbt; ++it) {
 #pragma unroll
         for (int u = 0; u < DST_PF; u++)
@@ -1114,7 +1118,7 @@ __device__ __forceinline__ void dst_zmix_body(const DstArgs &a, const WavePlan &
         const int gbn = batch_of(it + 1);
         const int compn = gbn / nbatch, bn = gbn - compn * nbatch;
         if (gbn < nbt)
-            wave_prefetch<DST_Z, float>(a, (const float *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
+            wave_prefetch<DST_Z, WV_NL, float>(a, (const float *)a.w + compn * a.wcs, compn, bn, tline, tpt0, m, v, vok, fx);
         // forward transform in FP32
         if (lane == 0) B32[0] = B32[m + 1] = make_float2(0.f, 0.f);
         __builtin_amdgcn_wave_barrier();
@@ -1402,7 +1406,6 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         ns *= fp->radix[s];
     }
     a.npairs = 8;
-    const int64_t nbatch = (a.nlines + 15) / 16;
     // compile-time plans for the lengths of the common block sizes
     auto same = [&](const int *rr, int n) {
         if (fp->nstages != n) return false;
@@ -1417,57 +1420,88 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
     if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
     // kernel variants: with a compile-time plan the registers capped for four waves per SIMD
-    // (six for the FP32 x inverse of the 576-point plan) or not -- the tuner times both
-    const void *fns[4] = {nullptr, nullptr, nullptr, nullptr};
-    const char *names[4] = {"uncapped", "capped", "uncapped, batch halves in turn", "capped, batch halves in turn"};
-    int nf = 1;
+    // (six for the FP32 x inverse of the 576-point plan) or not, each in both batch orders;
+    // FP32 compile-time plans also with 32-line batches (1024 threads) -- the tuner times all
     constexpr bool dbl = sizeof(T) == 8;
     // (each length's plan is instantiated only for the form that uses it)
     constexpr int c2 = wave_cap(dbl, MODE, 2), c3 = wave_cap(dbl, MODE, 3), c0 = wave_cap(dbl, MODE, 0),
                   c5 = wave_cap(dbl, MODE, 5);
     constexpr int k2 = c2 ? c2 : 4, k3 = c3 ? c3 : 4, k0 = c0 ? c0 : 4, k5 = c5 ? c5 : 4;
+    struct Var {
+        const void *fn;
+        const char *name;
+        int nl, order;
+    };
+    Var vars[8];
+    int nv = 0;
+    auto add = [&](const void *fn, const char *name, int nl, int order) { vars[nv++] = {fn, name, nl, order}; };
+    const void *unc = nullptr, *cap = nullptr, *unc32 = nullptr, *cap32 = nullptr;
     if constexpr (dbl) {
         if (fix == 2) {
-            fns[0] = (const void *)k_dst_wave<MODE, 4, T, 2>;
-            fns[1] = (const void *)k_dst_wave_cap<k2, MODE, 4, T, 2>;
-            nf = 2;
+            unc = (const void *)k_dst_wave<MODE, 4, T, 2>;
+            cap = (const void *)k_dst_wave_cap<k2, MODE, 4, T, 2>;
         } else if (fix == 3) {
-            fns[0] = (const void *)k_dst_wave<MODE, 17, T, 3>;
-            fns[1] = (const void *)k_dst_wave_cap<k3, MODE, 17, T, 3>;
-            nf = 2;
+            unc = (const void *)k_dst_wave<MODE, 17, T, 3>;
+            cap = (const void *)k_dst_wave_cap<k3, MODE, 17, T, 3>;
         }
     } else {
         if (fix == 0) {
-            fns[0] = (const void *)k_dst_wave<MODE, 4, T, 0>;
-            fns[1] = (const void *)k_dst_wave_cap<k0, MODE, 4, T, 0>;
-            nf = 2;
+            unc = (const void *)k_dst_wave<MODE, 4, T, 0>;
+            cap = (const void *)k_dst_wave_cap<k0, MODE, 4, T, 0>;
+            unc32 = (const void *)k_dst_wave<MODE, 4, T, 0, 32>;
+            cap32 = (const void *)k_dst_wave_cap<k0, MODE, 4, T, 0, 32>;
         } else if (fix == 5) {
-            fns[0] = (const void *)k_dst_wave<MODE, 17, T, 5>;
-            fns[1] = (const void *)k_dst_wave_cap<k5, MODE, 17, T, 5>;
-            nf = 2;
+            unc = (const void *)k_dst_wave<MODE, 17, T, 5>;
+            cap = (const void *)k_dst_wave_cap<k5, MODE, 17, T, 5>;
+            unc32 = (const void *)k_dst_wave<MODE, 17, T, 5, 32>;
+            cap32 = (const void *)k_dst_wave_cap<k5, MODE, 17, T, 5, 32>;
         }
     }
-    if (!fns[0]) {
+    if (unc) {
+        add(unc, "uncapped", 16, 0);
+        add(cap, "capped", 16, 0);
+        add(unc, "uncapped, batch halves in turn", 16, 1);
+        add(cap, "capped, batch halves in turn", 16, 1);
+        if (unc32) {
+            add(unc32, "uncapped, 32-line batches", 32, 0);
+            add(cap32, "capped, 32-line batches", 32, 0);
+        }
+    } else {
         fix = -1;
-        nf = 1;
-        fns[0] = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
-               : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
-               : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
+        const void *f = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
+                      : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
+                      : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
+        add(f, "uncapped", 16, 0);
+        add(f, "uncapped, batch halves in turn", 16, 1);
     }
-    // each kernel in both batch orders: variants nf.. are the first nf with order 1
-    if (nf == 1) names[1] = names[2];
-    for (int q = 0; q < nf; q++) fns[nf + q] = fns[q];
-    const int nkern = nf;
-    nf *= 2;
-    // eight padded sequences and the twiddles (the kernel's LDS layout)
+    // NL / 2 padded sequences and the twiddles (the kernel's LDS layout)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
     const int S = wave_pad_shift(sizeof(T) == 8, MODE, fix);
     const int Np = S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
-    const size_t lds = sizeof(C) * (size_t)(8 * Np + a.L);
+    const void *fns[8];
+    const char *names[8];
+    size_t ldsv[8];
+    int thr[8];
+    {
+        int q = 0;
+        for (int r = 0; r < nv; r++) {
+            // (static LDS of the kernel: lambda_z and the per-line tables, about 9 KB)
+            const size_t l = sizeof(C) * (size_t)((vars[r].nl / 2) * Np + a.L);
+            if (vars[r].nl > 16 && (int64_t)(l + 12288) > c->max_lds) continue;  // does not fit a work-group
+            vars[q] = vars[r];
+            fns[q] = vars[q].fn;
+            names[q] = vars[q].name;
+            ldsv[q] = l;
+            thr[q] = vars[q].nl * WV_PS;
+            q++;
+        }
+        nv = q;
+    }
     a.ncomp = ncomp;
     // persistent grid: every component's batches in one grid, per_cu resident work-groups per
     // CU, a multiple of 16 (batch pairs on one XCD), no more than the batches need
-    auto grid_of = [&](int per_cu) {
+    auto grid_of = [&](int per_cu, int nl) {
+        const int64_t nbatch = (a.nlines + nl - 1) / nl;
         int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
         if (resident < 16) resident = 16;
         const int64_t need = (nbatch * (int64_t)ncomp + 15) / 16 * 16;
@@ -1491,17 +1525,17 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             }
             DstArgs t = a;
             t.w = t.wo = (double *)c->tune_scratch.p;
-            t.order = fi >= nkern ? 1 : 0;
+            t.order = vars[fi].order;
             WavePlan Pt = P;
             void *targs[] = {&t, &Pt};
-            VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
+            VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(pc, vars[fi].nl), dim3(thr[fi]), targs, ldsv[fi], c->stream));
             return VPMX_OK;
         };
-        VPMX_TRY(tune(c, key, nf, fns, names, lds, DST_THREADS, trial, &ch));
+        VPMX_TRY(tune_v(c, key, nv, fns, names, ldsv, thr, trial, &ch));
     }
-    const void *fn = fns[ch.fi];
-    a.order = ch.fi >= nkern ? 1 : 0;
-    dim3 grid = grid_of(ch.per_cu);
+    const Var &V = vars[ch.fi];
+    a.order = V.order;
+    dim3 grid = grid_of(ch.per_cu, V.nl);
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
     constexpr int kc = MODE == DST_Y ? K_DST : K_DSTZ;
@@ -1509,9 +1543,9 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         // the fine block's y and z passes: timed by their own dispatch events; work = bytes
         // moved (read + write of every interior point)
         c->k_work[kc] = 2.0 * sizeof(T) * (double)a.mx * (MODE == DST_Y ? a.mz : a.my) * a.m * ncomp;
-        VPMX_HIP(c, hipExtLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream, e0, e1, 0));
+        VPMX_HIP(c, hipExtLaunchKernel(V.fn, grid, dim3(thr[ch.fi]), args, ldsv[ch.fi], c->stream, e0, e1, 0));
     } else {
-        VPMX_HIP(c, hipLaunchKernel(fn, grid, dim3(DST_THREADS), args, lds, c->stream));
+        VPMX_HIP(c, hipLaunchKernel(V.fn, grid, dim3(thr[ch.fi]), args, ldsv[ch.fi], c->stream));
     }
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;

@@ -16,5 +16,8 @@ namespace vpmx {
 // context's stream; the fastest is kept.  Otherwise variant 0 at its occupancy.
 int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names, size_t lds,
          int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out);
+// The same with each variant's own dynamic LDS bytes and work-group size (for its occupancy).
+int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names,
+           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out);
 
 }  // namespace vpmx

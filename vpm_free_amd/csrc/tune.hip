@@ -11,6 +11,13 @@ namespace vpmx {
 
 int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names, size_t lds,
          int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out) {
+    std::vector<size_t> l((size_t)nf, lds);
+    std::vector<int> t((size_t)nf, threads);
+    return tune_v(c, key, nf, fns, names, l.data(), t.data(), trial, out);
+}
+
+int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names,
+           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out) {
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) {
         *out = it->second;
@@ -19,7 +26,7 @@ int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const c
     std::vector<int> occ((size_t)nf, 1);
     for (int fi = 0; fi < nf; fi++) {
         int o = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fns[fi], threads, lds) != hipSuccess || o < 1) o = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fns[fi], threads[fi], lds[fi]) != hipSuccess || o < 1) o = 1;
         occ[fi] = o;
     }
     Ctx::Tuned best;

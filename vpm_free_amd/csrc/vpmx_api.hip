@@ -1087,6 +1087,9 @@ int32_t vpmx_create(vpmx_ctx **out, int32_t device) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         c->num_cu = ncu;
+    int mlds = 0;
+    if (hipDeviceGetAttribute(&mlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && mlds > 0)
+        c->max_lds = mlds;
     const char *pm = getenv("VPMX_PRECISION");
     if (pm && std::string(pm) == "mixed") c->precision = 1;
 

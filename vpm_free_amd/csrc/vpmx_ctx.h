@@ -84,6 +84,7 @@ struct Ctx : vpmx_ctx {
     void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
     void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
     int num_cu = 256;       // compute units of the device
+    int64_t max_lds = 65536;  // LDS bytes a work-group may hold (device attribute)
     bool dst_reuse = true;  // second fine solve from the first one's forward transforms (option dst_reuse)
     int dst_kernel = 4;   // DST passes: 4 half-length, one wave per line pair (default), 3 half-length
                           // work-group FFTs, 2 odd extension, 1 two-buffer
