@@ -6,6 +6,9 @@
 //   coarse_sample   fine solution every ncoarse nodes       (yaps3d.f90:104-138)
 //   coarse_laplacian 7-point Laplacian of the sample        (yaps3d.f90:1990-2016)
 //   face_interp     TSC coarse correction on inner faces    (yaps3d.f90:340-395, 1914-1983)
+#include <cstdio>
+
+#include "tune.h"
 #include "vpmx_ctx.h"
 #include "vpmx_internal.h"
 
@@ -97,18 +100,22 @@ int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView v
 }
 
 // curl and stretching in one pass (calc_velocity_serial_3d, vpm_gcalc.f90:1-132): a work-group
-// owns a 64 x 4 column tile and marches through CS_KC planes of z, keeping the velocity of
-// three planes (with a one-node halo in x and y) in LDS; each plane's velocity is formed once
-// per tile (halo nodes by the neighbouring tiles too) with k_curl's expressions, and the
-// stretching term reads its velocity neighbours from LDS with k_stretch's expressions -- the
-// same values bit for bit, without writing the velocity grid and reading it back.
-constexpr int CS_TX = 64, CS_TY = 4, CS_KC = 32, CS_HX = CS_TX + 2, CS_HY = CS_TY + 2;
+// owns a TX x TY column tile (256 threads) and marches through CS_KC planes of z, keeping the
+// velocity of three planes (with a one-node halo in x and y) in LDS; each plane's velocity is
+// formed once per tile (halo nodes by the neighbouring tiles too) with k_curl's expressions,
+// and the stretching term reads its velocity neighbours from LDS with k_stretch's expressions
+// -- the same values bit for bit, without writing the velocity grid and reading it back.  The
+// tile shape is a tuner choice (the halo's share against the x extent of the rows).
+constexpr int CS_KC = 32;
+template <int TX, int TY>
 __global__ void __launch_bounds__(256) k_curl_stretch(int nx, int ny, int nz, double ix2, double iy2, double iz2,
                                                        GView psi, GView rhs, GView vel, GView out) {
 #pragma clang fp contract(off)
+    static_assert(TX * TY == 256, "one thread per column of the tile");
+    constexpr int CS_HX = TX + 2, CS_HY = TY + 2;
     __shared__ double V[3][3][CS_HY][CS_HX];  // [plane mod 3][component][y][x]
-    const int i0 = blockIdx.x * CS_TX, j0 = blockIdx.y * CS_TY, kb = blockIdx.z * CS_KC;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, kb = blockIdx.z * CS_KC;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
     auto fill = [&](int kk) {  // velocity of plane kk on the tile and its halo
         const int sl = (kk + 3) % 3;
@@ -165,10 +172,27 @@ __global__ void __launch_bounds__(256) k_curl_stretch(int nx, int ny, int nz, do
 }
 int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView rhs, GView vel, GView out) {
     if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
-    const dim3 grid((unsigned)((n[0] + CS_TX - 1) / CS_TX), (unsigned)((n[1] + CS_TY - 1) / CS_TY),
+    static const int shp[3][2] = {{64, 4}, {32, 8}, {128, 2}};
+    const void *fns[3] = {(const void *)k_curl_stretch<64, 4>, (const void *)k_curl_stretch<32, 8>,
+                          (const void *)k_curl_stretch<128, 2>};
+    const char *names[3] = {"64 x 4 column tiles", "32 x 8 column tiles", "128 x 2 column tiles"};
+    double ix2 = 1.0 / (2 * h[0]), iy2 = 1.0 / (2 * h[1]), iz2 = 1.0 / (2 * h[2]);
+    int nx = n[0], ny = n[1], nz = n[2];
+    void *args[] = {&nx, &ny, &nz, &ix2, &iy2, &iz2, &psi, &rhs, &vel, &out};
+    auto grid_of = [&](int fi) {
+        return dim3((unsigned)((n[0] + shp[fi][0] - 1) / shp[fi][0]), (unsigned)((n[1] + shp[fi][1] - 1) / shp[fi][1]),
                     (unsigned)((n[2] + CS_KC - 1) / CS_KC));
-    hipLaunchKernelGGL(k_curl_stretch, grid, dim3(256), 0, c->stream, n[0], n[1], n[2], 1.0 / (2 * h[0]),
-                       1.0 / (2 * h[1]), 1.0 / (2 * h[2]), psi, rhs, vel, out);
+    };
+    // trials write the real outputs: every variant writes the same values to every node
+    auto trial = [&](int fi, int) -> int {
+        VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(fi), dim3(256), args, 0, c->stream));
+        return VPMX_OK;
+    };
+    char key[128];
+    std::snprintf(key, sizeof key, "curl and stretching %dx%dx%d", n[0], n[1], n[2]);
+    Ctx::Tuned ch;
+    VPMX_TRY(tune(c, key, 3, fns, names, 0, 256, trial, &ch));
+    VPMX_HIP(c, hipLaunchKernel(fns[ch.fi], grid_of(ch.fi), dim3(256), args, 0, c->stream));
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }
