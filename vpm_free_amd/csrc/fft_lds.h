@@ -59,6 +59,24 @@ __device__ __forceinline__ void dft_reg(C (&v)[R]) {
         v[1] = cadd(d02, md);
         v[2] = csub(s02, s13);
         v[3] = csub(d02, md);
+    } else if constexpr (R == 8) {
+        // 8 = 2 x 4: the 4-point DFTs of the even and odd inputs, X_k = E_k + W^k O_k and
+        // X_{k+4} = E_k - W^k O_k, W = exp(-2 pi i / 8)
+        C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        dft_reg<4, C>(e);
+        dft_reg<4, C>(o);
+        const T r2 = (T)0.70710678118654752440;
+        const C t1 = Cx<C>::mk(r2 * (o[1].x + o[1].y), r2 * (o[1].y - o[1].x));  // W o1
+        const C t2 = Cx<C>::mk(o[2].y, -o[2].x);                                  // W^2 o2 = -i o2
+        const C t3 = Cx<C>::mk(r2 * (o[3].y - o[3].x), -r2 * (o[3].x + o[3].y));  // W^3 o3
+        v[0] = cadd(e[0], o[0]);
+        v[4] = csub(e[0], o[0]);
+        v[1] = cadd(e[1], t1);
+        v[5] = csub(e[1], t1);
+        v[2] = cadd(e[2], t2);
+        v[6] = csub(e[2], t2);
+        v[3] = cadd(e[3], t3);
+        v[7] = csub(e[3], t3);
     } else if constexpr (R == 9) {
         // 9 = 3 x 3 with n = 3a + b, k = k1 + 3 k2: a 3-point DFT over a for each b, the
         // twiddles exp(-2 pi i b k1 / 9), then a 3-point DFT over b for each k1
@@ -393,6 +411,17 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
                     v[i][1] = cmul(v[i][1], w1);
                     v[i][2] = cmul(v[i][2], w2);
                     v[i][3] = cmul(v[i][3], cmul(w2, w1));
+                } else if constexpr (R == 8) {
+                    // two table reads (w, w^4), the other powers at most two products deep
+                    const C w1 = tw[k0 * twstep], w4 = tw[4 * k0 * twstep];
+                    const C w2 = cmul(w1, w1), w3 = cmul(w1, w2);
+                    v[i][1] = cmul(v[i][1], w1);
+                    v[i][2] = cmul(v[i][2], w2);
+                    v[i][3] = cmul(v[i][3], w3);
+                    v[i][4] = cmul(v[i][4], w4);
+                    v[i][5] = cmul(v[i][5], cmul(w1, w4));
+                    v[i][6] = cmul(v[i][6], cmul(w2, w4));
+                    v[i][7] = cmul(v[i][7], cmul(w3, w4));
                 } else if constexpr (R == 9) {
                     // two table reads (w, w^3), the other powers at most two products deep
                     const C w1 = tw[k0 * twstep], w3 = tw[3 * k0 * twstep];
@@ -410,7 +439,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
                     for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
                 }
             }
-            if constexpr (R <= 4 || R == 9) dft_reg<R>(v[i]);
+            if constexpr (R <= 4 || R == 8 || R == 9) dft_reg<R>(v[i]);
             dst[i] = q * Ns * R + k0;
         }
     }
@@ -418,7 +447,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
 #pragma unroll
     for (int i = 0; i < BPL; i++)
         if (dst[i] >= 0) {
-            if constexpr (R <= 4 || R == 9) {
+            if constexpr (R <= 4 || R == 8 || R == 9) {
 #pragma unroll
                 for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
             } else if constexpr (R == 17 && CR) {
@@ -444,6 +473,13 @@ template <> struct FixedFft<0> {  // 576 = 4.4.4.9 (FP32, 289-node block)
     template <class C, class Buf, bool CR = false>
     static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
         wave_fft_s<C, Buf, CR, 576, 1, 4, 4, 4, 9>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<6> {  // 576 = 8.8.9 (FP32, 289-node block): three stages instead of four
+    static constexpr int L = 576;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 576, 1, 8, 8, 9>(b, tw, lane);
     }
 };
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)

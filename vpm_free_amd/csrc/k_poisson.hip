@@ -839,6 +839,9 @@ __host__ __device__ constexpr int fix_len() {
 // runtime plans lose more to the extra registers and address arithmetic than the conflicts
 // cost them.
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
+    // the 8.8.9 plan: one slot per 8 elements keeps its radix-8 stores (strides 8 and 64
+    // elements for groups of 8 lanes) on distinct banks in every pass
+    if (fix == 6) return 3;
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
@@ -1430,11 +1433,13 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     struct Var {
         const void *fn;
         const char *name;
-        int nl, order;
+        int nl, order, fix;
     };
-    Var vars[8];
+    Var vars[12];
     int nv = 0;
-    auto add = [&](const void *fn, const char *name, int nl, int order) { vars[nv++] = {fn, name, nl, order}; };
+    auto add = [&](const void *fn, const char *name, int nl, int order, int vfix) {
+        vars[nv++] = {fn, name, nl, order, vfix};
+    };
     const void *unc = nullptr, *cap = nullptr, *unc32 = nullptr, *cap32 = nullptr;
     if constexpr (dbl) {
         if (fix == 2) {
@@ -1458,35 +1463,46 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         }
     }
     if (unc) {
-        add(unc, "uncapped", 16, 0);
-        add(cap, "capped", 16, 0);
-        add(unc, "uncapped, batch halves in turn", 16, 1);
-        add(cap, "capped, batch halves in turn", 16, 1);
+        add(unc, "uncapped", 16, 0, fix);
+        add(cap, "capped", 16, 0, fix);
+        add(unc, "uncapped, batch halves in turn", 16, 1, fix);
+        add(cap, "capped, batch halves in turn", 16, 1, fix);
         if (unc32) {
-            add(unc32, "uncapped, 32-line batches", 32, 0);
-            add(cap32, "capped, 32-line batches", 32, 0);
+            add(unc32, "uncapped, 32-line batches", 32, 0, fix);
+            add(cap32, "capped, 32-line batches", 32, 0, fix);
+        }
+        if constexpr (!dbl) {
+            if (fix == 0) {  // the three-stage 8.8.9 plan of the same length
+                add((const void *)k_dst_wave<MODE, 9, T, 6>, "8.8.9 plan", 16, 0, 6);
+                add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 6>, "8.8.9 plan, capped", 16, 0, 6);
+                add((const void *)k_dst_wave<MODE, 9, T, 6>, "8.8.9 plan, batch halves in turn", 16, 1, 6);
+                add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 6>, "8.8.9 plan, capped, batch halves in turn", 16,
+                    1, 6);
+            }
         }
     } else {
         fix = -1;
         const void *f = rc == 4 ? (const void *)k_dst_wave<MODE, 4, T, -1>
                       : rc == 7 ? (const void *)k_dst_wave<MODE, 7, T, -1>
                       : rc == 17 ? (const void *)k_dst_wave<MODE, 17, T, -1> : (const void *)k_dst_wave<MODE, 37, T, -1>;
-        add(f, "uncapped", 16, 0);
-        add(f, "uncapped, batch halves in turn", 16, 1);
+        add(f, "uncapped", 16, 0, -1);
+        add(f, "uncapped, batch halves in turn", 16, 1, -1);
     }
     // NL / 2 padded sequences and the twiddles (the kernel's LDS layout)
     using C = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
-    const int S = wave_pad_shift(sizeof(T) == 8, MODE, fix);
-    const int Np = S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
-    const void *fns[8];
-    const char *names[8];
-    size_t ldsv[8];
-    int thr[8];
+    auto np_of = [&](int vfix) {
+        const int S = wave_pad_shift(sizeof(T) == 8, MODE, vfix);
+        return S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
+    };
+    const void *fns[12];
+    const char *names[12];
+    size_t ldsv[12];
+    int thr[12];
     {
         int q = 0;
         for (int r = 0; r < nv; r++) {
             // (static LDS of the kernel: lambda_z and the per-line tables, about 9 KB)
-            const size_t l = sizeof(C) * (size_t)((vars[r].nl / 2) * Np + a.L);
+            const size_t l = sizeof(C) * (size_t)((vars[r].nl / 2) * np_of(vars[r].fix) + a.L);
             if (vars[r].nl > 16 && (int64_t)(l + 12288) > c->max_lds) continue;  // does not fit a work-group
             vars[q] = vars[r];
             fns[q] = vars[q].fn;
