@@ -76,17 +76,29 @@ def stencil_nodes(xp, g):
     return int(occ.sum())
 
 
-def measured_traffic(mixed, axis):
-    """HBM bytes per launch of the y or z pass from the committed rocprofv3 PMC summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None."""
+def chosen_variant(tuning, mixed, axis):
+    """The tuner's choice for the y or z pass of the 289^3 block, as its report names it
+    ("8.8.9 plan, capped, 2 work-groups per CU"), or None."""
+    head = "sine transforms %s pass %s L=%d " % (axis, "FP32" if mixed else "FP64", 576 if mixed else 288)
+    for line in tuning:
+        if line.startswith(head) and " -> " in line:
+            return line.split(" -> ", 1)[1].rsplit(" (", 1)[0]
+    return None
+
+
+def measured_traffic(mixed, axis, variant):
+    """HBM bytes per launch of the y or z pass in the tuner's chosen variant, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), or None when
+    that variant was not profiled."""
     path = os.path.join(ROOT, "profiles", "dst_passes_289_traffic.md")
     key = "traffic_bytes_per_launch_" + axis
+    want = "float" if mixed else "double"
     try:
         for line in open(path):
-            if line.strip().startswith(key + " "):
-                kv = dict(t.split("=") for t in line.split()[1:])
-                return int(kv["float" if mixed else "double"])
-    except (OSError, KeyError, ValueError):
+            t = line.strip().split(None, 2)
+            if len(t) == 3 and t[0] == key and t[1].startswith(want + "=") and t[2] == variant:
+                return int(t[1].split("=")[1])
+    except (OSError, ValueError):
         return None
     return None
 
@@ -328,17 +340,19 @@ def main():
     dom = max(("dst_y_fine", "dst_z_fine"), key=lambda k: ks.get(k, {}).get("ms_per_step", -1.0))
     dk = ks.get(dom, {})
     axis = "y" if dom == "dst_y_fine" else "z"
+    variant = chosen_variant(head.get("tuning", []), mixed, axis)
     roofline = {
         "bound": "hbm",
         "achieved": dk.get("GB/s"),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": dk.get("frac"),
-        "traffic": measured_traffic(mixed, axis),
+        "traffic": measured_traffic(mixed, axis, variant),
         "traffic_source": "profiles/dst_passes_289_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-        "kernel": "k_dst_wave<DST_%s,4,%s>: %s pass of the 289^3 block's sine-transform Poisson solve (3 comps%s)"
+        "kernel": "k_dst_wave<DST_%s,%s>: %s pass of the 289^3 block's sine-transform Poisson solve (3 comps%s)"
                   % (axis.upper(), "float" if mixed else "double", axis,
                      "" if axis == "y" else "; forward transform, diagonal and inverse transform"),
+        "variant": variant,
         "ms_per_launch": dk.get("ms"),
         "launches_per_step": round(dk.get("launches", 0) / args.steps, 2),
         "algorithmic_bytes_per_launch": dk.get("algorithmic_bytes"),

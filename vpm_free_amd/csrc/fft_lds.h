@@ -482,6 +482,20 @@ template <> struct FixedFft<6> {  // 576 = 8.8.9 (FP32, 289-node block): three s
         wave_fft_s<C, Buf, CR, 576, 1, 8, 8, 9>(b, tw, lane);
     }
 };
+template <> struct FixedFft<7> {  // 288 = 8.4.9 (FP64, 289-node block): three stages instead of four
+    static constexpr int L = 288;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 288, 1, 8, 4, 9>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<8> {  // 544 = 17.8.4 (FP32, 273-node inner box): three stages instead of four
+    static constexpr int L = 544;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 544, 1, 17, 8, 4>(b, tw, lane);
+    }
+};
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)
     static constexpr int L = 288;
     template <class C, class Buf, bool CR = false>

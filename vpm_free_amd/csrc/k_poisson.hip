@@ -841,13 +841,13 @@ __host__ __device__ constexpr int fix_len() {
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
     // the 8.8.9 plan: one slot per 8 elements keeps its radix-8 stores (strides 8 and 64
     // elements for groups of 8 lanes) on distinct banks in every pass
-    if (fix == 6) return 3;
+    if (fix == 6 || fix == 7 || fix == 8) return 3;
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
 // the extra registers cost no occupancy).
 __host__ __device__ constexpr bool const_roots(bool dbl, int mode, int fix) {
-    return dbl ? (fix == 3 && mode != DST_Z) : (fix == 5 && (mode == DST_Z || mode == DST_XI));
+    return dbl ? (fix == 3 && mode != DST_Z) : ((fix == 5 || fix == 8) && (mode == DST_Z || mode == DST_XI));
 }
 
 // T: arithmetic and work-array type (double: plain FP64 form, half-length transforms;
@@ -1471,7 +1471,23 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             add(unc32, "uncapped, 32-line batches", 32, 0, fix);
             add(cap32, "capped, 32-line batches", 32, 0, fix);
         }
+        if constexpr (dbl) {
+            if (fix == 2) {  // the three-stage 8.4.9 plan of the same length
+                add((const void *)k_dst_wave<MODE, 9, T, 7>, "8.4.9 plan", 16, 0, 7);
+                add((const void *)k_dst_wave_cap<k2, MODE, 9, T, 7>, "8.4.9 plan, capped", 16, 0, 7);
+                add((const void *)k_dst_wave<MODE, 9, T, 7>, "8.4.9 plan, batch halves in turn", 16, 1, 7);
+                add((const void *)k_dst_wave_cap<k2, MODE, 9, T, 7>, "8.4.9 plan, capped, batch halves in turn", 16,
+                    1, 7);
+            }
+        }
         if constexpr (!dbl) {
+            if (fix == 5) {  // the three-stage 17.8.4 plan of the same length
+                add((const void *)k_dst_wave<MODE, 17, T, 8>, "17.8.4 plan", 16, 0, 8);
+                add((const void *)k_dst_wave_cap<k5, MODE, 17, T, 8>, "17.8.4 plan, capped", 16, 0, 8);
+                add((const void *)k_dst_wave<MODE, 17, T, 8>, "17.8.4 plan, batch halves in turn", 16, 1, 8);
+                add((const void *)k_dst_wave_cap<k5, MODE, 17, T, 8>, "17.8.4 plan, capped, batch halves in turn",
+                    16, 1, 8);
+            }
             if (fix == 0) {  // the three-stage 8.8.9 plan of the same length
                 add((const void *)k_dst_wave<MODE, 9, T, 6>, "8.8.9 plan", 16, 0, 6);
                 add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 6>, "8.8.9 plan, capped", 16, 0, 6);
