@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["vpmx_api.hip", "vpmx_batch.hip", "tune.hip", "k_adjoint.hip", "k_particles.hip", "k_poisson.hip", "k_boundary.hip", "k_bspec.hip", "k_grid.hip",
+HIP_SOURCES = ["vpmx_api.hip", "vpmx_batch.hip", "tune.hip", "k_adjoint.hip", "k_particles.hip", "k_poisson.hip", "k_boundary.hip", "k_bspec.hip", "k_dgemm.hip", "k_grid.hip",
                "k_tree.hip", "k_panel.hip"]
 CXX_SOURCES = ["geometry.cpp"]
 HIP_FLAGS = [
@@ -62,8 +62,7 @@ def build_hip(jobs=8):
         objs = list(ex.map(_compile_one, HIP_SOURCES + CXX_SOURCES))
     out = os.path.join(LIB, "libvpmx.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     return out
 

@@ -31,7 +31,6 @@
 // scheduling); the transforms run in LDS with the Stockham FFT of the Poisson passes.  A
 // final pass adds, for every boundary node, the six pair contributions of its owner face
 // in a fixed order (X faces own their edges, Y faces their z-edges).
-#include <rocblas/rocblas.h>
 
 #include <climits>
 #include <cmath>
@@ -580,8 +579,6 @@ void free_bspec(Ctx *c) {
         free_plan((SpecPlan *)c->bsx[q]);
         c->bsx[q] = nullptr;
     }
-    if (c->blas) (void)rocblas_destroy_handle((rocblas_handle)c->blas);
-    c->blas = nullptr;
 }
 
 static bool smooth7(int n) {
@@ -807,38 +804,6 @@ static int build_plan(Ctx *c, const Grid &g, SpecPlan **out) {
     return VPMX_OK;
 }
 
-static int blas_check(Ctx *c, rocblas_status s, const char *what) {
-    if (s == rocblas_status_success) return VPMX_OK;
-    return fail(c, VPMX_ERR_HIP, std::string("rocBLAS ") + what + ": " + rocblas_status_to_string(s));
-}
-
-// The context's rocBLAS handle on its stream (atomics off: results independent of scheduling).
-static int blas_handle(Ctx *c, rocblas_handle *out) {
-    if (!c->blas) {
-        rocblas_handle hb;
-        VPMX_TRY(blas_check(c, rocblas_create_handle(&hb), "create_handle"));
-        c->blas = hb;
-        VPMX_TRY(blas_check(c, rocblas_set_atomics_mode(hb, rocblas_atomics_not_allowed), "set_atomics_mode"));
-        VPMX_TRY(blas_check(c, rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host), "set_pointer_mode"));
-    }
-    rocblas_handle hb = (rocblas_handle)c->blas;
-    VPMX_TRY(blas_check(c, rocblas_set_stream(hb, c->stream), "set_stream"));
-    *out = hb;
-    return VPMX_OK;
-}
-
-int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
-                  int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch) {
-    rocblas_handle hb = nullptr;
-    VPMX_TRY(blas_handle(c, &hb));
-    const double one = 1.0, zero = 0.0;
-    return blas_check(c,
-                      rocblas_dgemm_strided_batched(hb, rocblas_operation_none,
-                                                    transb ? rocblas_operation_transpose : rocblas_operation_none, M, N,
-                                                    K, &one, A, lda, sa, B, ldb, sb, &zero, C, ldc, sc, batch),
-                      "dgemm_strided_batched");
-}
-
 // The plan of slot for grid g (built, or rebuilt when the grid changed).
 static int get_plan(Ctx *c, const Grid &g, int slot, SpecPlan **out) {
     SpecPlan *P = (SpecPlan *)c->bsx[slot];
@@ -860,8 +825,6 @@ static int get_plan(Ctx *c, const Grid &g, int slot, SpecPlan **out) {
 int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, GView sol, int slot) {
     SpecPlan *P = nullptr;
     VPMX_TRY(get_plan(c, g, slot, &P));
-    rocblas_handle hb = nullptr;
-    VPMX_TRY(blas_handle(c, &hb));
     for (auto &T : P->tabs) {
         const int np = T.form == -2 ? T.f.npairs : BW_WAVES, nl = 2 * np, ncol = (int)T.cols.size();
         const size_t lds = sizeof(double2) * (size_t)(np + 1) * T.t.L;
@@ -894,15 +857,9 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t ns, const double *a, G
                                P->Cm + T.c_off, 0);
             VPMX_HIP(c, hipGetLastError());
         } else {
-            const double one = 1.0, zero = 0.0;
-            VPMX_TRY(blas_check(c,
-                                rocblas_dgemm_strided_batched(hb, rocblas_operation_none, rocblas_operation_none, T.t.M,
-                                                              2 * ncol, T.t.K, &one, T.G, T.t.M,
-                                                              (rocblas_stride)T.t.M * T.t.K, P->Bm + T.b_off,
-                                                              T.t.nw * T.t.K, (rocblas_stride)T.t.K, &zero,
-                                                              P->Cm + T.c_off, T.t.nw * T.t.M, (rocblas_stride)T.t.M,
-                                                              T.t.nw),
-                                "dgemm_strided_batched"));
+            // per frequency w: C_w (M x 2 ncol) = G_w (M x K) B_w (K x 2 ncol)
+            VPMX_TRY(dgemm(c, false, false, T.t.M, 2 * ncol, T.t.K, T.G, T.t.M, (int64_t)T.t.M * T.t.K,
+                           P->Bm + T.b_off, T.t.nw * T.t.K, T.t.K, P->Cm + T.c_off, T.t.nw * T.t.M, T.t.M, T.t.nw));
         }
         if (T.form == 0)
             hipLaunchKernelGGL(k_bsx_inv<0>, gi, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw,
@@ -958,8 +915,6 @@ int boundary_sums_spectral_adj(Ctx *c, const Grid &g, int64_t ns, GView gb, doub
         VPMX_HIP(c, hipMalloc((void **)&P->apart, sizeof(double) * 18 * (size_t)ns));
         P->apart_ns = ns;
     }
-    rocblas_handle hb = nullptr;
-    VPMX_TRY(blas_handle(c, &hb));
     int64_t bo = 0, co = 0;
     for (auto &T : P->tabs) {
         const int np = T.form == -2 ? T.f.npairs : BW_WAVES, nl = 2 * np, ncol = (int)T.cols.size();
@@ -1001,14 +956,9 @@ int boundary_sums_spectral_adj(Ctx *c, const Grid &g, int64_t ns, GView gb, doub
             }
             VPMX_HIP(c, hipGetLastError());
         } else {
-            const double one = 1.0, zero = 0.0;
-            VPMX_TRY(blas_check(c,
-                                rocblas_dgemm_strided_batched(hb, rocblas_operation_transpose, rocblas_operation_none,
-                                                              T.t.K, 2 * ncol, T.t.M, &one, T.G, T.t.M,
-                                                              (rocblas_stride)T.t.M * T.t.K, Bm, T.t.nw * T.t.M,
-                                                              (rocblas_stride)T.t.M, &zero, Cm, T.t.nw * T.t.K,
-                                                              (rocblas_stride)T.t.K, T.t.nw),
-                                "dgemm_strided_batched"));
+            // per frequency w: C_w (K x 2 ncol) = G_w^T (K x M) B_w (M x 2 ncol)
+            VPMX_TRY(dgemm(c, true, false, T.t.K, 2 * ncol, T.t.M, T.G, T.t.M, (int64_t)T.t.M * T.t.K, Bm,
+                           T.t.nw * T.t.M, T.t.M, Cm, T.t.nw * T.t.K, T.t.K, T.t.nw));
         }
         if (T.form == 0)
             hipLaunchKernelGGL(k_bsxT_inv<0>, gi, dim3(bsx_threads<0>()), lds, c->stream, T.f, T.wf, T.t, T.tw, T.cols_d,

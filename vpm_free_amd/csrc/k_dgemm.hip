@@ -1,0 +1,209 @@
+// FP64 matrix products on the matrix cores (v_mfma_f64_16x16x4f64): the perpendicular-face
+// products of the spectral boundary sums (one real matrix per frequency, pmbound.f90:415-515
+// restated as convolutions) and their transposes, and the sine-matrix face transforms of the
+// forward-pass reuse.  Column-major C_b = op(A_b) op(B_b), b < batch, alpha 1, beta 0.
+//
+// A work-group of four waves computes a BM x BN tile of C; the waves sit WM x (4/WM) over the
+// tile and each owns MB x NB blocks of 16 x 16 (one 4-double accumulator per lane per block).
+// K advances 16 at a time through two LDS buffers (k-major rows, padded by one double): the
+// next step's elements are loaded into registers while the matrix cores work on the current
+// buffer.  Every element of C accumulates its K products in chunks of 4 in increasing k
+// whatever the tile shape, so all variants give the same bits.  Tiles are numbered m-fastest,
+// then n, then batch, and handed out XCD by XCD in contiguous chunks: the m tiles that share
+// a B tile run at the same time on one XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "tune.h"
+#include "vpmx_internal.h"
+
+namespace vpmx {
+
+typedef double dgemm_acc __attribute__((ext_vector_type(4)));
+
+struct GemmArgs {
+    const double *A;
+    const double *B;
+    double *C;
+    int M, N, K, lda, ldb, ldc;
+    int64_t sa, sb, sc;
+    int mt, nt;       // tiles along M and N
+    int64_t ntiles;   // mt * nt * batch
+    int64_t chunk;    // tiles per XCD
+};
+
+constexpr int DG_XCD = 8;
+
+template <bool TA, bool TB, int WM, int MB, int NB>
+__global__ void __launch_bounds__(256) k_dgemm(GemmArgs g) {
+    constexpr int WN = 4 / WM;
+    constexpr int BM = WM * MB * 16, BN = WN * NB * 16, BK = 16;
+    constexpr int LA = BM + 1, LB = BN + 1;
+    constexpr int NA = BM * BK / 256, NBL = BN * BK / 256;
+    static_assert(WM * WN == 4 && NA * 256 == BM * BK && NBL * 256 == BN * BK, "tile shape");
+    __shared__ double As[2][BK][LA];
+    __shared__ double Bs[2][BK][LB];
+    const int64_t bid = blockIdx.x;
+    const int64_t idx = (bid % DG_XCD) * g.chunk + bid / DG_XCD;
+    if (idx >= g.ntiles) return;
+    const int mt = (int)(idx % g.mt);
+    const int64_t rest = idx / g.mt;
+    const int nt = (int)(rest % g.nt);
+    const int64_t b = rest / g.nt;
+    const double *__restrict__ A = g.A + b * g.sa;
+    const double *__restrict__ B = g.B + b * g.sb;
+    double *__restrict__ C = g.C + b * g.sc;
+    const int m0 = mt * BM, n0 = nt * BN;
+    const int tid = threadIdx.x;
+    double ra[NA], rb[NBL];
+    // this thread's elements of the step at k0 (zeros outside the matrices)
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = tid + 256 * u;
+            const int i = TA ? e / BK : e % BM, k = TA ? e % BK : e / BM;
+            const int gi = m0 + i, gk = k0 + k;
+            double v = 0.0;
+            if (gi < g.M && gk < g.K) v = TA ? A[gk + (int64_t)gi * g.lda] : A[gi + (int64_t)gk * g.lda];
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NBL; u++) {
+            const int e = tid + 256 * u;
+            const int j = TB ? e % BN : e / BK, k = TB ? e / BN : e % BK;
+            const int gj = n0 + j, gk = k0 + k;
+            double v = 0.0;
+            if (gj < g.N && gk < g.K) v = TB ? B[gj + (int64_t)gk * g.ldb] : B[gk + (int64_t)gj * g.ldb];
+            rb[u] = v;
+        }
+    };
+    auto lstore = [&](int s) {
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+            const int e = tid + 256 * u;
+            const int i = TA ? e / BK : e % BM, k = TA ? e % BK : e / BM;
+            As[s][k][i] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NBL; u++) {
+            const int e = tid + 256 * u;
+            const int j = TB ? e % BN : e / BK, k = TB ? e / BN : e % BK;
+            Bs[s][k][j] = rb[u];
+        }
+    };
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave % WM, wn = wave / WM;
+    const int li = lane & 15, lk = lane >> 4;
+    dgemm_acc acc[MB][NB];
+#pragma unroll
+    for (int x = 0; x < MB; x++)
+#pragma unroll
+        for (int y = 0; y < NB; y++) acc[x][y] = dgemm_acc{0.0, 0.0, 0.0, 0.0};
+    const int nk = (g.K + BK - 1) / BK;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int s = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            // operands: lane l holds A(i = l % 16, k = l / 16) and B(k = l / 16, j = l % 16)
+            double a[MB], bv[NB];
+#pragma unroll
+            for (int x = 0; x < MB; x++) a[x] = As[s][kk + lk][(wm * MB + x) * 16 + li];
+#pragma unroll
+            for (int y = 0; y < NB; y++) bv[y] = Bs[s][kk + lk][(wn * NB + y) * 16 + li];
+#pragma unroll
+            for (int x = 0; x < MB; x++)
+#pragma unroll
+                for (int y = 0; y < NB; y++)
+                    acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], bv[y], acc[x][y], 0, 0, 0);
+        }
+        if (kt + 1 < nk) lstore(s ^ 1);
+        __syncthreads();
+    }
+    // result: lane l, entry r holds C(i = 4 r + l / 16, j = l % 16) of each block (the f64
+    // instruction's layout, checked on the device: rows i and i + 4 sit in consecutive entries)
+#pragma unroll
+    for (int x = 0; x < MB; x++)
+#pragma unroll
+        for (int y = 0; y < NB; y++) {
+            const int gj = n0 + (wn * NB + y) * 16 + li;
+            if (gj >= g.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gi = m0 + (wm * MB + x) * 16 + 4 * r + lk;
+                if (gi < g.M) C[gi + (int64_t)gj * g.ldc] = acc[x][y][r];
+            }
+        }
+}
+
+namespace {
+
+struct DgShape {
+    int wm, mb, nb;
+    const char *name;
+};
+constexpr int DG_NV = 4;
+constexpr DgShape dg_shapes[DG_NV] = {{4, 1, 9, "64 x 144 tiles"},
+                                      {4, 1, 4, "64 x 64 tiles, waves along M"},
+                                      {2, 2, 2, "64 x 64 tiles, 2 x 2 waves"},
+                                      {2, 1, 2, "32 x 64 tiles"}};
+
+template <bool TA, bool TB>
+void dg_fns(const void *fns[DG_NV]) {
+    fns[0] = (const void *)k_dgemm<TA, TB, 4, 1, 9>;
+    fns[1] = (const void *)k_dgemm<TA, TB, 4, 1, 4>;
+    fns[2] = (const void *)k_dgemm<TA, TB, 2, 2, 2>;
+    fns[3] = (const void *)k_dgemm<TA, TB, 2, 1, 2>;
+}
+
+}  // namespace
+
+int dgemm(Ctx *c, bool ta, bool tb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
+          int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch) {
+    if (M <= 0 || N <= 0 || batch <= 0) return VPMX_OK;
+    if (K <= 0) return fail(c, VPMX_ERR_ARG, "matrix product: K must be positive");
+    if (lda < (ta ? K : M) || ldb < (tb ? N : K) || ldc < M || sa < 0 || sb < 0 || sc < 0)
+        return fail(c, VPMX_ERR_ARG, "matrix product: leading dimension or stride too small");
+    const void *fns[DG_NV];
+    if (!ta && !tb) dg_fns<false, false>(fns);
+    else if (ta && !tb) dg_fns<true, false>(fns);
+    else if (!ta && tb) dg_fns<false, true>(fns);
+    else dg_fns<true, true>(fns);
+    const char *names[DG_NV];
+    for (int v = 0; v < DG_NV; v++) names[v] = dg_shapes[v].name;
+    auto args_of = [&](int v) {
+        const DgShape &s = dg_shapes[v];
+        const int bm = s.wm * s.mb * 16, bn = (4 / s.wm) * s.nb * 16;
+        GemmArgs g{A, B, C, M, N, K, lda, ldb, ldc, sa, sb, sc, (M + bm - 1) / bm, (N + bn - 1) / bn, 0, 0};
+        g.ntiles = (int64_t)g.mt * g.nt * batch;
+        g.chunk = (g.ntiles + DG_XCD - 1) / DG_XCD;
+        return g;
+    };
+    // trials write the real output: every variant writes the same bits to every element
+    auto trial = [&](int v, int) -> int {
+        GemmArgs g = args_of(v);
+        if (g.chunk * DG_XCD > 0x7fffffffLL) return fail(c, VPMX_ERR_UNSUP, "matrix product: too many tiles");
+        void *kargs[] = {&g};
+        VPMX_HIP(c, hipLaunchKernel(fns[v], dim3((unsigned)(g.chunk * DG_XCD)), dim3(256), kargs, 0, c->stream));
+        return VPMX_OK;
+    };
+    char key[160];
+    std::snprintf(key, sizeof key, "FP64 matrix products %c%c M=%d N=%d K=%d batch=%d", ta ? 'T' : 'N', tb ? 'T' : 'N',
+                  M, N, K, batch);
+    Ctx::Tuned ch;
+    VPMX_TRY(tune(c, key, DG_NV, fns, names, 0, 256, trial, &ch, false));
+    VPMX_TRY(trial(ch.fi, 0));
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
+int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
+                  int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch) {
+    return dgemm(c, false, transb, M, N, K, A, lda, sa, B, ldb, sb, C, ldc, sc, batch);
+}
+
+}  // namespace vpmx

@@ -191,7 +191,7 @@ int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GV
     char key[128];
     std::snprintf(key, sizeof key, "curl and stretching %dx%dx%d", n[0], n[1], n[2]);
     Ctx::Tuned ch;
-    VPMX_TRY(tune(c, key, 3, fns, names, 0, 256, trial, &ch));
+    VPMX_TRY(tune(c, key, 3, fns, names, 0, 256, trial, &ch, false));
     VPMX_HIP(c, hipLaunchKernel(fns[ch.fi], grid_of(ch.fi), dim3(256), args, 0, c->stream));
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;

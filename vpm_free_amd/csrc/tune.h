@@ -14,10 +14,13 @@ namespace vpmx {
 // being captured -- every candidate is run three times through trial(fi, per_cu), which queues
 // one launch whose results nothing reads (scratch arguments), timed with events on the
 // context's stream; the fastest is kept.  Otherwise variant 0 at its occupancy.
+// persistent = false: the launches are not persistent grids, so each variant is timed once
+// (three runs) and per_cu is not a choice.
 int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names, size_t lds,
-         int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out);
+         int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out, bool persistent = true);
 // The same with each variant's own dynamic LDS bytes and work-group size (for its occupancy).
 int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names,
-           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out);
+           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out,
+           bool persistent = true);
 
 }  // namespace vpmx

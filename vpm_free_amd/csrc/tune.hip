@@ -10,14 +10,15 @@
 namespace vpmx {
 
 int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names, size_t lds,
-         int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out) {
+         int threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out, bool persistent) {
     std::vector<size_t> l((size_t)nf, lds);
     std::vector<int> t((size_t)nf, threads);
-    return tune_v(c, key, nf, fns, names, l.data(), t.data(), trial, out);
+    return tune_v(c, key, nf, fns, names, l.data(), t.data(), trial, out, persistent);
 }
 
 int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names,
-           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out) {
+           const size_t *lds, const int *threads, const std::function<int(int, int)> &trial, Ctx::Tuned *out,
+           bool persistent) {
     auto it = c->tuned.find(key);
     if (it != c->tuned.end()) {
         *out = it->second;
@@ -35,7 +36,10 @@ int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const
     best.ms = -1.0;
     char buf[160];
     if (!c->tuning || c->capturing) {
-        std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (occupancy; not timed)", names[0], occ[0]);
+        if (persistent)
+            std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (occupancy; not timed)", names[0], occ[0]);
+        else
+            std::snprintf(buf, sizeof buf, "%s (not timed)", names[0]);
         best.what = buf;
         if (!c->capturing) c->tuned[key] = best;
         *out = best;
@@ -49,7 +53,8 @@ int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const
     }
     int st = VPMX_OK;
     for (int fi = 0; fi < nf && st == VPMX_OK; fi++) {
-        for (int pc = occ[fi]; pc >= 1 && pc > occ[fi] - 4 && st == VPMX_OK; pc--) {
+        const int pc_end = persistent ? occ[fi] - 4 : occ[fi] - 1;
+        for (int pc = occ[fi]; pc >= 1 && pc > pc_end && st == VPMX_OK; pc--) {
             float tmin = 1e30f;
             for (int rep = 0; rep < 3 && st == VPMX_OK; rep++) {
                 if (hipEventRecord(e0, c->stream) != hipSuccess) { st = fail(c, VPMX_ERR_HIP, "tuner: event"); break; }
@@ -73,7 +78,10 @@ int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (st) return st;
-    std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (%.4f ms)", names[best.fi], best.per_cu, best.ms);
+    if (persistent)
+        std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (%.4f ms)", names[best.fi], best.per_cu, best.ms);
+    else
+        std::snprintf(buf, sizeof buf, "%s (%.4f ms)", names[best.fi], best.ms);
     best.what = buf;
     c->tuned[key] = best;
     *out = best;

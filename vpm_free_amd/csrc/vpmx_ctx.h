@@ -81,7 +81,6 @@ struct Ctx : vpmx_ctx {
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c, panel_a;
     int bs_mode = 2;  // boundary sums (option boundsum): 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx[2] = {nullptr, nullptr};  // spectral boundary-sum plans: fine block, coarse grid
-    void *blas = nullptr;  // rocBLAS handle (spectral boundary sums)
     void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
     int num_cu = 256;       // compute units of the device
     int64_t max_lds = 65536;  // LDS bytes a work-group may hold (device attribute)

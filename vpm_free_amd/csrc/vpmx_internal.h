@@ -139,7 +139,11 @@ int boundary_sums_spectral(Ctx *c, const Grid &g, int64_t nbound, const double *
 // of the sources a); VPMX_ERR_UNSUP where the forward plan does not build.
 int boundary_sums_spectral_adj(Ctx *c, const Grid &g, int64_t nbound, GView gb, double *abar, int slot);
 void free_bspec(Ctx *c);
-// Column-major C_b = A_b op(B_b) for b < batch (FP64, rocBLAS on the context's stream).
+// Column-major C_b = op(A_b) op(B_b) for b < batch (FP64 on the matrix cores, k_dgemm.hip; the
+// tile shape is a tuner choice, every shape gives the same bits).
+int dgemm(Ctx *c, bool ta, bool tb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
+          int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch);
+// C_b = A_b op(B_b).
 int dgemm_batched(Ctx *c, bool transb, int M, int N, int K, const double *A, int lda, int64_t sa, const double *B,
                   int ldb, int64_t sb, double *C, int ldc, int64_t sc, int batch);
 // Tree form (k_tree.hip): sig (3 x nbound), cxyz (3 x nbound), ds (nbound) of normal_derivs.
