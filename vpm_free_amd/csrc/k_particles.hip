@@ -354,6 +354,20 @@ __global__ void k_row_counts(const int *__restrict__ start, int cnx, int64_t nro
 // fit; every thread then sums its node's 16 rows, the four cells of each, rows z-outer,
 // y-inner, cells and particles in cell order: deterministic, no atomics.  A row with more particles than a chunk holds is read straight from memory in the
 // same order.  Tiles without particles are written by k_p2m_zero_tiles.
+// Work-group order for XCD locality: the hardware hands work-group b to XCD b % 8, so within
+// each round of R work-groups (R a multiple of 8) the XCD of b takes the contiguous slice
+// [(b % 8) R / 8, (b % 8 + 1) R / 8) of the round instead of every 8th index: neighbouring tiles
+// then run at the same time on one XCD and share its L2 (halo nodes, the partial cache lines of
+// their rows).  A bijection of [0, grid); the last partial round keeps launch order.
+__device__ __forceinline__ int64_t xcd_slice(int64_t b, int64_t grid, int64_t R) {
+    const int64_t base = b / R * R;
+    if ((R & 7) || base + R > grid) return b;
+    const int64_t o = b - base;
+    return base + (o & 7) * (R >> 3) + (o >> 3);
+}
+// Round of the one-work-group-per-tile launches: 256 work-groups per XCD.
+constexpr int64_t XCD_ROUND = 2048;
+
 constexpr int PT_X = 16, PT_Y = 4, PT_Z = 4;
 constexpr int PC_X = PT_X + 3, PC_Y = PT_Y + 3, PC_Z = PT_Z + 3;
 constexpr int PT_ROWS = PC_Y * PC_Z;
@@ -472,8 +486,9 @@ __global__ void __launch_bounds__(256) k_p2m_tile(Grid g, const int *__restrict_
     __shared__ int any;
     const int tid = threadIdx.x;
     const int ntx = (vnx + PT_X - 1) / PT_X, nty = (vny + PT_Y - 1) / PT_Y;
-    if ((int)blockIdx.x >= *nlist) return;
-    const int tile = list[blockIdx.x];
+    const int64_t slot = xcd_slice(blockIdx.x, gridDim.x, XCD_ROUND);
+    if (slot >= *nlist) return;
+    const int tile = list[slot];
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int lx = tid & (PT_X - 1), ly = (tid / PT_X) & (PT_Y - 1), lz = tid / (PT_X * PT_Y);
     const int va = tx * PT_X + lx, vb = ty * PT_Y + ly, vc = tz * PT_Z + lz;
@@ -756,28 +771,16 @@ constexpr int M2P_TX = 8, M2P_TY = 8, M2P_TZ = 4;
 constexpr int M2P_NX = M2P_TX + 3, M2P_NY = M2P_TY + 3, M2P_NZ = M2P_TZ + 3;
 constexpr int M2P_ROWS = M2P_TY * M2P_TZ;
 
-// Tiles holding particles: flag per tile, then (after an exclusive scan) their list.
-// One 32-lane group per tile, one lane per cell row (loads in flight together).
-__global__ void k_m2p_tile_flags(int ntiles, const int *__restrict__ start, int cnx, int cny, int cnz,
-                                 int *__restrict__ flag) {
-    static_assert(M2P_ROWS == 32, "one lane per row of a 32-lane group");
-    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int tile = (int)(gt >> 5), r = (int)(gt & 31);
-    int cnt = 0;
-    if (tile < ntiles) {
-        const int ntx = (cnx + M2P_TX - 1) / M2P_TX, nty = (cny + M2P_TY - 1) / M2P_TY;
-        const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
-        const int cx0 = tx * M2P_TX, cx1 = min(cx0 + M2P_TX, cnx);
-        const int cyy = ty * M2P_TY + r % M2P_TY, czz = tz * M2P_TZ + r / M2P_TY;
-        if (cyy < cny && czz < cnz) {
-            const int64_t row = (int64_t)cnx * (cyy + (int64_t)cny * czz);
-            cnt = start[row + cx1] - start[row + cx0];
-        }
-    }
-    // any row of the group non-empty (lanes 0-31 and 32-63 are two groups of one wave)
-    const unsigned long long any = __ballot(cnt > 0);
-    const unsigned half = (threadIdx.x & 32) ? (unsigned)(any >> 32) : (unsigned)(any & 0xffffffffULL);
-    if (r == 0 && tile < ntiles) flag[tile] = half != 0 ? 1 : 0;
+// Tiles holding particles: flag per tile (from each particle's cell key; every writer of a
+// flag writes 1), then (after an exclusive scan) their list.
+__global__ void k_m2p_flag_particles(const int *__restrict__ keys, int64_t n, const int *dn, int cnx, int cny,
+                                     int ntx, int nty, int *__restrict__ flag) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pcount(n, dn)) return;
+    const int key = keys[p];
+    if (key < 0) return;
+    const int cx = key % cnx, cy = (key / cnx) % cny, cz = key / (cnx * cny);
+    flag[cx / M2P_TX + ntx * (cy / M2P_TY + nty * (cz / M2P_TZ))] = 1;
 }
 __global__ void k_m2p_tile_list(int ntiles, const int *__restrict__ flag, const int *__restrict__ pos,
                                 int *__restrict__ list) {
@@ -798,7 +801,7 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
     __shared__ int rs[M2P_ROWS + 1], rp[M2P_ROWS];
     const int ntx = (cnx + M2P_TX - 1) / M2P_TX, nty = (cny + M2P_TY - 1) / M2P_TY;
     const int ntot = *nlist;
-    for (int li = blockIdx.x; li < ntot; li += gridDim.x) {
+    for (int li = (int)xcd_slice(blockIdx.x, gridDim.x, gridDim.x); li < ntot; li += gridDim.x) {
     const int tile = list[li];
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     // cell indices (stored index = cell + 1) of the tile: [cx0, cx0 + TX) clipped to the grid
@@ -911,8 +914,9 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
         const bool timed = kernel_events(c, K_M2P, &e0, &e1);
         if (timed) VPMX_HIP(c, hipEventRecord(e0, c->stream));
         hipLaunchKernelGGL(k_m2p_outside, grid, dim3(256), 0, c->stream, keys, n, c->dn, up, gp);
-        hipLaunchKernelGGL(k_m2p_tile_flags, dim3((unsigned)(((int64_t)tiles * 32 + 255) / 256)), dim3(256), 0,
-                           c->stream, tiles, start, s.cn[0], s.cn[1], s.cn[2], flag);
+        hipLaunchKernelGGL(k_zero_int, dim3(gt), dim3(256), 0, c->stream, flag, (int64_t)tiles);
+        hipLaunchKernelGGL(k_m2p_flag_particles, grid, dim3(256), 0, c->stream, keys, n, c->dn, s.cn[0], s.cn[1], ntx,
+                           nty, flag);
         VPMX_TRY(exclusive_scan(c, flag, pos, tiles));
         hipLaunchKernelGGL(k_m2p_tile_list, dim3(gt), dim3(256), 0, c->stream, tiles, flag, pos, list);
         // persistent: the work-groups per CU the tuner picks, at most what one CU holds (LDS-bound:
