@@ -459,10 +459,74 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
     __builtin_amdgcn_wave_barrier();
 }
 
+// A radix-17 stage with at most 32 butterflies (the 272- and 544-point plans) leaves half or
+// three quarters of a wave's lanes idle.  Here SPL = 64 / nbf lanes share a butterfly: each
+// reads the 17 inputs (the same LDS addresses: broadcast reads), forms the conjugate-pair sums
+// p_r, m_r, and computes H / SPL of the H output pairs (lane 0 of the group also X_0), with the
+// same sums in the same order as dft_odd_store and its roots from the LDS table -- so the
+// stage's arithmetic and root reads are divided by SPL and its stores spread over the group.
+template <int L, int Ns, class C, class Buf>
+__device__ __forceinline__ void wave_stage_s17x(Buf buf, const C *__restrict__ tw, int lane) {
+    constexpr int R = 17, H = 8, nbf = L / R, twstep = L / (Ns * R);
+    static_assert(nbf * R == L && nbf <= 32, "split radix-17 stage: at most 32 butterflies");
+    constexpr int SPL = nbf <= 16 ? 4 : 2, KP = H / SPL;
+    using T = typename Cx<C>::R;
+    const int j = lane / SPL, sub = lane - j * SPL;
+    const bool act = j < nbf;
+    const int jj = act ? j : 0;  // idle lanes read butterfly 0 and store nothing
+    const int q = jj / Ns, k0 = jj - q * Ns;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = buf[jj + r * nbf];
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) v[r] = cmul(v[r], tw[r * k0 * twstep]);
+    }
+    C p[H], mm[H];
+    C x0 = v[0];
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        p[r - 1] = cadd(v[r], v[R - r]);
+        mm[r - 1] = csub(v[r], v[R - r]);
+        x0 = cadd(x0, p[r - 1]);
+    }
+    const int d = q * Ns * R + k0;
+    __builtin_amdgcn_wave_barrier();
+    if (act) {
+        if (sub == 0) buf[d] = x0;
+#pragma unroll
+        for (int kk = 0; kk < KP; kk++) {
+            const int k = sub * KP + kk + 1;
+            T X = v[0].x, Y = v[0].y, U = 0, V = 0;
+            int qq = 0;  // (r * k) mod R
+#pragma unroll
+            for (int r = 1; r <= H; r++) {
+                qq += k;
+                if (qq >= R) qq -= R;
+                const C w = tw[qq * nbf];  // (c, -s)
+                const T c = w.x, sn = -w.y;
+                X += c * p[r - 1].x;
+                Y += c * p[r - 1].y;
+                U += sn * mm[r - 1].y;
+                V += sn * mm[r - 1].x;
+            }
+            buf[d + k * Ns] = Cx<C>::mk(X + U, Y - V);
+            buf[d + (R - k) * Ns] = Cx<C>::mk(X - U, Y + V);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+// SPLIT: radix-17 stages as wave_stage_s17x.
+template <class C, class Buf, bool CR, bool SPLIT, int L, int Ns, int R, int... Rest>
+__device__ __forceinline__ void wave_fft_sx(Buf buf, const C *tw, int lane) {
+    if constexpr (R == 17 && SPLIT) wave_stage_s17x<L, Ns, C, Buf>(buf, tw, lane);
+    else wave_stage_s<R, L, Ns, C, Buf, CR>(buf, tw, lane);
+    if constexpr (sizeof...(Rest) > 0) wave_fft_sx<C, Buf, CR, SPLIT, L, Ns * R, Rest...>(buf, tw, lane);
+}
 template <class C, class Buf, bool CR, int L, int Ns, int R, int... Rest>
 __device__ __forceinline__ void wave_fft_s(Buf buf, const C *tw, int lane) {
-    wave_stage_s<R, L, Ns, C, Buf, CR>(buf, tw, lane);
-    if constexpr (sizeof...(Rest) > 0) wave_fft_s<C, Buf, CR, L, Ns * R, Rest...>(buf, tw, lane);
+    wave_fft_sx<C, Buf, CR, false, L, Ns, R, Rest...>(buf, tw, lane);
 }
 
 // Lengths with a compile-time plan (radices in the planner's order: 4s, 2s, odd primes).
@@ -494,6 +558,20 @@ template <> struct FixedFft<8> {  // 544 = 17.8.4 (FP32, 273-node inner box): th
     template <class C, class Buf, bool CR = false>
     static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
         wave_fft_s<C, Buf, CR, 544, 1, 17, 8, 4>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<9> {  // 272 = 4.4.17 with the radix-17 outputs split over 4 lanes
+    static constexpr int L = 272;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_sx<C, Buf, false, true, 272, 1, 4, 4, 17>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<10> {  // 544 = 17.8.4 with the radix-17 outputs split over 2 lanes
+    static constexpr int L = 544;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_sx<C, Buf, false, true, 544, 1, 17, 8, 4>(b, tw, lane);
     }
 };
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)

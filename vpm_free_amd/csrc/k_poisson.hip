@@ -841,7 +841,7 @@ __host__ __device__ constexpr int fix_len() {
 __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
     // the 8.8.9 plan: one slot per 8 elements keeps its radix-8 stores (strides 8 and 64
     // elements for groups of 8 lanes) on distinct banks in every pass
-    if (fix == 6 || fix == 7 || fix == 8) return 3;
+    if (fix == 6 || fix == 7 || fix == 8 || fix == 10) return 3;
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
@@ -1435,7 +1435,7 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         const char *name;
         int nl, order, fix;
     };
-    Var vars[12];
+    Var vars[16];
     int nv = 0;
     auto add = [&](const void *fn, const char *name, int nl, int order, int vfix) {
         vars[nv++] = {fn, name, nl, order, vfix};
@@ -1472,6 +1472,13 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             add(cap32, "capped, 32-line batches", 32, 0, fix);
         }
         if constexpr (dbl) {
+            if (fix == 3) {  // the same plan with the radix-17 outputs split over four lanes
+                add((const void *)k_dst_wave<MODE, 17, T, 9>, "radix-17 split", 16, 0, 9);
+                add((const void *)k_dst_wave_cap<k3, MODE, 17, T, 9>, "radix-17 split, capped", 16, 0, 9);
+                add((const void *)k_dst_wave<MODE, 17, T, 9>, "radix-17 split, batch halves in turn", 16, 1, 9);
+                add((const void *)k_dst_wave_cap<k3, MODE, 17, T, 9>, "radix-17 split, capped, batch halves in turn",
+                    16, 1, 9);
+            }
             if (fix == 2) {  // the three-stage 8.4.9 plan of the same length
                 add((const void *)k_dst_wave<MODE, 9, T, 7>, "8.4.9 plan", 16, 0, 7);
                 add((const void *)k_dst_wave_cap<k2, MODE, 9, T, 7>, "8.4.9 plan, capped", 16, 0, 7);
@@ -1481,6 +1488,11 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
             }
         }
         if constexpr (!dbl) {
+            if (fix == 5) {  // the 17.8.4 plan with the radix-17 outputs split over two lanes
+                add((const void *)k_dst_wave<MODE, 17, T, 10>, "17.8.4 plan, radix-17 split", 16, 0, 10);
+                add((const void *)k_dst_wave_cap<k5, MODE, 17, T, 10>, "17.8.4 plan, radix-17 split, capped", 16, 0,
+                    10);
+            }
             if (fix == 5) {  // the three-stage 17.8.4 plan of the same length
                 add((const void *)k_dst_wave<MODE, 17, T, 8>, "17.8.4 plan", 16, 0, 8);
                 add((const void *)k_dst_wave_cap<k5, MODE, 17, T, 8>, "17.8.4 plan, capped", 16, 0, 8);
@@ -1510,10 +1522,10 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         const int S = wave_pad_shift(sizeof(T) == 8, MODE, vfix);
         return S == 0 ? pad_len<0>(a.L) : S == 3 ? pad_len<3>(a.L) : pad_len<4>(a.L);
     };
-    const void *fns[12];
-    const char *names[12];
-    size_t ldsv[12];
-    int thr[12];
+    const void *fns[16];
+    const char *names[16];
+    size_t ldsv[16];
+    int thr[16];
     {
         int q = 0;
         for (int r = 0; r < nv; r++) {
@@ -1618,27 +1630,48 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         return true;
     };
     static const int r576[] = {4, 4, 4, 3, 3}, r288[] = {4, 4, 2, 3, 3}, r272[] = {4, 4, 17}, r544[] = {4, 4, 2, 17};
-    int fix32 = -1, fix64 = -1;
-    const void *fn = nullptr;
+    // kernels (compile-time plans of both forms, or runtime plans), each in both batch orders
+    struct ZK {
+        const void *fn;
+        int fix32, fix64;
+        const char *name;
+    };
+    ZK zk[2];
+    int nk = 1;
     if (f32->L == 576 && same(f32, r576, 5) && f64->L == 288 && same(f64, r288, 5)) {
-        fix32 = 0;
-        fix64 = 2;
-        fn = (const void *)k_dst_zmix<4, 0, 2>;
+        zk[0] = {(const void *)k_dst_zmix<4, 0, 2>, 0, 2, "FP32 forward, FP64 inverse"};
     } else if (f32->L == 544 && same(f32, r544, 4) && f64->L == 272 && same(f64, r272, 3)) {
-        fix32 = 5;
-        fix64 = 3;
-        fn = (const void *)k_dst_zmix<17, 5, 3>;
+        zk[0] = {(const void *)k_dst_zmix<17, 5, 3>, 5, 3, "FP32 forward, FP64 inverse"};
+        zk[1] = {(const void *)k_dst_zmix<17, 10, 9>, 10, 9, "FP32 forward (17.8.4), FP64 inverse, radix-17 split"};
+        nk = 2;
     } else {
-        fn = rc == 4 ? (const void *)k_dst_zmix<4, -1, -1>
-           : rc == 7 ? (const void *)k_dst_zmix<7, -1, -1>
-           : rc == 17 ? (const void *)k_dst_zmix<17, -1, -1> : (const void *)k_dst_zmix<37, -1, -1>;
+        zk[0] = {rc == 4 ? (const void *)k_dst_zmix<4, -1, -1>
+                 : rc == 7 ? (const void *)k_dst_zmix<7, -1, -1>
+                 : rc == 17 ? (const void *)k_dst_zmix<17, -1, -1> : (const void *)k_dst_zmix<37, -1, -1>,
+                 -1, -1, "FP32 forward, FP64 inverse"};
     }
     // the kernel's LDS layout: eight regions of Wq double2 slots, then both twiddle tables
-    const int S32 = wave_pad_shift(false, DST_Z, fix32), S64 = wave_pad_shift(true, DST_Z, fix64);
-    const int Np32 = S32 == 0 ? pad_len<0>(f32->L) : S32 == 3 ? pad_len<3>(f32->L) : pad_len<4>(f32->L);
-    const int Np64 = S64 == 0 ? pad_len<0>(f64->L) : S64 == 3 ? pad_len<3>(f64->L) : pad_len<4>(f64->L);
-    const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
-    const size_t lds = sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
+    auto lds_of = [&](int fix32, int fix64) {
+        const int S32 = wave_pad_shift(false, DST_Z, fix32), S64 = wave_pad_shift(true, DST_Z, fix64);
+        const int Np32 = S32 == 0 ? pad_len<0>(f32->L) : S32 == 3 ? pad_len<3>(f32->L) : pad_len<4>(f32->L);
+        const int Np64 = S64 == 0 ? pad_len<0>(f64->L) : S64 == 3 ? pad_len<3>(f64->L) : pad_len<4>(f64->L);
+        const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
+        return sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
+    };
+    const void *fns[4];
+    const char *names[4];
+    size_t ldsv[4];
+    int thr[4];
+    char nbuf[4][96];
+    for (int q = 0; q < nk; q++)
+        for (int o = 0; o < 2; o++) {
+            const int v = 2 * q + o;
+            fns[v] = zk[q].fn;
+            std::snprintf(nbuf[v], sizeof nbuf[v], "%s%s", zk[q].name, o ? ", batch halves in turn" : "");
+            names[v] = nbuf[v];
+            ldsv[v] = lds_of(zk[q].fix32, zk[q].fix64);
+            thr[v] = DST_THREADS;
+        }
     const int64_t nbatch = (a.nlines + 15) / 16;
     auto grid_of = [&](int per_cu) {
         int64_t resident = (int64_t)c->num_cu * per_cu / 16 * 16;
@@ -1649,8 +1682,6 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
     char key[160];
     std::snprintf(key, sizeof key, "sine transforms z pass FP32/FP64 L=%d/%d m=%d lines=%lld comps=%d", f32->L, f64->L,
                   a.m, (long long)a.nlines, ncomp);
-    const char *names[2] = {"FP32 forward, FP64 inverse", "FP32 forward, FP64 inverse, batch halves in turn"};
-    const void *fns2[2] = {fn, fn};
     Ctx::Tuned ch;
     {
         int st = VPMX_OK;
@@ -1663,18 +1694,20 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
                 VPMX_TRY(fill(c, (double *)p, (int64_t)(c->tune_scratch.bytes / sizeof(double)), 0.0));
             }
             DstArgs t = a;
-            t.order = fi;
+            t.order = fi & 1;
             t.w = (double *)c->tune_scratch.p;
             t.wo = (double *)((char *)c->tune_scratch.p + sizeof(float) * (size_t)a.wcs * ncomp);
             WavePlan p32 = P32, p64 = P64;
             const double2 *t64 = f64->tw;
             void *targs[] = {&t, &p32, &p64, &t64};
-            VPMX_HIP(c, hipLaunchKernel(fn, grid_of(pc), dim3(DST_THREADS), targs, lds, c->stream));
+            VPMX_HIP(c, hipLaunchKernel(fns[fi], grid_of(pc), dim3(DST_THREADS), targs, ldsv[fi], c->stream));
             return VPMX_OK;
         };
-        VPMX_TRY(tune(c, key, 2, fns2, names, lds, DST_THREADS, trial, &ch));
+        VPMX_TRY(tune_v(c, key, 2 * nk, fns, names, ldsv, thr, trial, &ch));
     }
-    a.order = ch.fi;
+    a.order = ch.fi & 1;
+    const void *fn = fns[ch.fi];
+    const size_t lds = ldsv[ch.fi];
     dim3 grid = grid_of(ch.per_cu);
     const double2 *tw64 = f64->tw;
     WavePlan p32 = P32, p64 = P64;
