@@ -542,9 +542,11 @@ def test_vpm_mode2_call_sequence_kept_zeros(vpmlib, oracle):
 
 
 def test_tuner_choices_keep_results(vpmlib):
-    """The autotuner picks kernel variants and persistent grid sizes by timing them; the
-    results are those of the fixed defaults bit for bit (the variants run the same
-    arithmetic, and every work item's sums do not depend on which work-group takes it)."""
+    """The autotuner picks kernel variants and persistent grid sizes by timing them; at h=1/16
+    (runtime FFT plans, so the variants differ in register caps, batch order and grid size
+    only) the results are those of the fixed defaults bit for bit: every work item's sums do
+    not depend on which work-group takes it.  (The plan variants of the 288-, 544- and
+    576-point lengths round differently; the full-size parity tests cover them.)"""
     V = vpmlib
     h = 1 / 16
     xp, qp = cases.single_ring(h=h, sigma=0.1)

@@ -1636,14 +1636,15 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         int fix32, fix64;
         const char *name;
     };
-    ZK zk[2];
+    ZK zk[3];
     int nk = 1;
     if (f32->L == 576 && same(f32, r576, 5) && f64->L == 288 && same(f64, r288, 5)) {
         zk[0] = {(const void *)k_dst_zmix<4, 0, 2>, 0, 2, "FP32 forward, FP64 inverse"};
     } else if (f32->L == 544 && same(f32, r544, 4) && f64->L == 272 && same(f64, r272, 3)) {
         zk[0] = {(const void *)k_dst_zmix<17, 5, 3>, 5, 3, "FP32 forward, FP64 inverse"};
         zk[1] = {(const void *)k_dst_zmix<17, 10, 9>, 10, 9, "FP32 forward (17.8.4), FP64 inverse, radix-17 split"};
-        nk = 2;
+        zk[2] = {(const void *)k_dst_zmix<17, 5, 9>, 5, 9, "FP32 forward, FP64 inverse with radix-17 split"};
+        nk = 3;
     } else {
         zk[0] = {rc == 4 ? (const void *)k_dst_zmix<4, -1, -1>
                  : rc == 7 ? (const void *)k_dst_zmix<7, -1, -1>
@@ -1658,11 +1659,11 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
         return sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
     };
-    const void *fns[4];
-    const char *names[4];
-    size_t ldsv[4];
-    int thr[4];
-    char nbuf[4][96];
+    const void *fns[6];
+    const char *names[6];
+    size_t ldsv[6];
+    int thr[6];
+    char nbuf[6][96];
     for (int q = 0; q < nk; q++)
         for (int o = 0; o < 2; o++) {
             const int v = 2 * q + o;

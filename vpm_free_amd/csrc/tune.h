@@ -12,8 +12,9 @@ namespace vpmx {
 // Candidates: variant fi < nf at each resident work-group count per CU from its occupancy
 // down (at most four counts).  The first time `key` is seen -- with tuning on and no graph
 // being captured -- every candidate is run three times through trial(fi, per_cu), which queues
-// one launch whose results nothing reads (scratch arguments), timed with events on the
-// context's stream; the fastest is kept.  Otherwise variant 0 at its occupancy.
+// one launch, timed with events on the context's stream; the fastest is kept.  A trial either
+// writes scratch arrays nothing reads, or, where every variant writes the same bits, the real
+// output.  Otherwise variant 0 at its occupancy.
 // persistent = false: the launches are not persistent grids, so each variant is timed once
 // (three runs) and per_cu is not a choice.
 int tune(Ctx *c, const std::string &key, int nf, const void *const *fns, const char *const *names, size_t lds,
