@@ -114,7 +114,9 @@ int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
  * defaults (clears the choices made so far); "adjoint_boundsum" -- the adjoint's transposed
  * boundary sums by 2 spectral correlation on the forward's plan (default), 1 direct sums;
  * "adjoint_m2p_scatter" -- 1: the adjoint's transposed interpolation by FP64 atomic scatters,
- * 0 (default): by the sorted gather projection. */
+ * 0 (default): by the sorted gather projection; "tune_variant" -- -1 (default): the tuner (or
+ * the defaults) decide, v >= 0: every launch takes its variant min(v, last) untimed (a check
+ * that the variants meant to give the same bits do; clears the choices made so far). */
 int32_t vpmx_set_option(vpmx_ctx *ctx, const char *name, int32_t value);
 /* The autotuner's choices so far, one line per launch key ("key -> choice (time)"), into buf
  * (NUL-terminated, cut to capacity). */

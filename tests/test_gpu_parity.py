@@ -568,3 +568,32 @@ def test_tuner_choices_keep_results(vpmlib):
     for prec in (0, 1):
         assert np.array_equal(out[(1, prec)][0], out[(0, prec)][0])
         assert np.array_equal(out[(1, prec)][1], out[(0, prec)][1])
+
+
+def test_tuner_variants_same_bits(vpmlib):
+    """Every launch forced to its variant v (option tune_variant) gives the same velocity and
+    stretching grids, UP and GP bit for bit, for every v and both forms, at h=1/16 (runtime FFT
+    plans, so the variants differ in scheduling only): the curl/stretch tile shapes, with the
+    velocity ring alone or with the stream function kept in registers and one LDS plane, the
+    FP64 matrix-product tile shapes, the sine-transform batch orders."""
+    V = vpmlib
+    h = 1 / 16
+    xp, qp = cases.single_ring(h=h, sigma=0.1)
+    ref = {}
+    for v in range(6):
+        with V.VPM(V.Params.make(h)) as c:
+            c.set_option("tune_variant", v)
+            for prec in (0, 1):
+                c.set_precision(prec)
+                up, gp = np.zeros_like(xp), np.zeros_like(xp)
+                c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+                c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+                got = (c.get_grid(3), c.get_grid(2), up, gp)
+                say("case: tune_variant", v, "precision", prec)
+                if v == 0:
+                    ref[prec] = got
+                    continue
+                for name, a, b in zip(("vel", "sol", "up", "gp"), got, ref[prec]):
+                    assert np.array_equal(a, b), (v, prec, name, float(np.max(np.abs(a - b))))
+            rep = c.tuning_report()
+        assert "forced" in rep

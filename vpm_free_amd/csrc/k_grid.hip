@@ -107,14 +107,20 @@ int stretch_kernel(Ctx *c, const int n[3], const double h[3], GView rhs, GView v
 // -- the same values bit for bit, without writing the velocity grid and reading it back.  The
 // tile shape is a tuner choice (the halo's share against the x extent of the rows).
 constexpr int CS_KC = 32;
+// Tiles go out XCD by XCD in contiguous slices of each round of R work-groups (xcd_slice; R
+// about the work-groups resident at once), so the tiles that share halo columns, and the
+// planes each of them reads three times, meet in one L2.
 template <int TX, int TY>
 __global__ void __launch_bounds__(256) k_curl_stretch(int nx, int ny, int nz, double ix2, double iy2, double iz2,
-                                                       GView psi, GView rhs, GView vel, GView out) {
+                                                       GView psi, GView rhs, GView vel, GView out, int R) {
 #pragma clang fp contract(off)
     static_assert(TX * TY == 256, "one thread per column of the tile");
     constexpr int CS_HX = TX + 2, CS_HY = TY + 2;
     __shared__ double V[3][3][CS_HY][CS_HX];  // [plane mod 3][component][y][x]
-    const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY, kb = blockIdx.z * CS_KC;
+    const int64_t gx = gridDim.x, gy = gridDim.y;
+    const int64_t lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int64_t slot = xcd_slice(lin, gx * gy * gridDim.z, R);
+    const int i0 = (int)(slot % gx) * TX, j0 = (int)((slot / gx) % gy) * TY, kb = (int)(slot / (gx * gy)) * CS_KC;
     const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
     const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
     auto fill = [&](int kk) {  // velocity of plane kk on the tile and its halo
@@ -170,15 +176,144 @@ __global__ void __launch_bounds__(256) k_curl_stretch(int nx, int ny, int nz, do
         __syncthreads();  // the next iteration refills the slot of plane k - 1
     }
 }
+// The same with each stream-function value read from memory once per tile: every thread owns
+// up to NS columns of the tile's region with a two-node halo ((TX + 4) x (TY + 4)) and keeps
+// psi_0 and psi_1 of its columns for three planes in registers (the z differences), while
+// each plane's three components go to one LDS plane the x and y differences read; the
+// stretching term keeps RHS_2 of its column for three planes the same way.  The expressions
+// and their operands are k_curl's and k_stretch's, so the values are the same bit for bit.
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_curl_stretch_w(int nx, int ny, int nz, double ix2, double iy2, double iz2,
+                                                         GView psi, GView rhs, GView vel, GView out, int R) {
+#pragma clang fp contract(off)
+    static_assert(TX * TY == 256, "one thread per column of the tile");
+    constexpr int CS_HX = TX + 2, CS_HY = TY + 2, RX = TX + 4, RY = TY + 4;
+    constexpr int NS = (RX * RY + 255) / 256;
+    __shared__ double V[3][3][CS_HY][CS_HX];  // velocity: [plane mod 3][component][y][x]
+    __shared__ double P[3][RY][RX];           // stream function of one plane: [component][y][x]
+    const int64_t gx = gridDim.x, gy = gridDim.y;
+    const int64_t lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int64_t slot = xcd_slice(lin, gx * gy * gridDim.z, R);
+    const int i0 = (int)(slot % gx) * TX, j0 = (int)((slot / gx) % gy) * TY, kb = (int)(slot / (gx * gy)) * CS_KC;
+    const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+    const double *p0 = psi.p, *p1 = psi.p + psi.cs, *p2 = psi.p + 2 * psi.cs;
+    // the thread's region columns: (hx, hy) <-> node (i0 - 2 + hx, j0 - 2 + hy)
+    int64_t co[NS];
+    bool cin[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const int q = tid + 256 * s, hx = q % RX, hy = q / RX, i = i0 - 2 + hx, j = j0 - 2 + hy;
+        cin[s] = q < RX * RY && i >= 0 && i < nx && j >= 0 && j < ny;
+        co[s] = cin[s] ? psi.at(i, j, 0) : 0;
+    }
+    auto ld = [&](const double *pc, int s, int k) -> double {
+        return (cin[s] && k >= 0 && k < nz) ? pc[co[s] + (int64_t)k * psi.pz] : 0.0;
+    };
+    // w0/w1[s][t]: psi_0 / psi_1 of column s at planes kk - 1 + t while plane kk is formed
+    double w0[NS][3], w1[NS][3];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        w0[s][1] = ld(p0, s, kb - 2);
+        w1[s][1] = ld(p1, s, kb - 2);
+        w0[s][2] = ld(p0, s, kb - 1);
+        w1[s][2] = ld(p1, s, kb - 1);
+    }
+    const int kend = kb + CS_KC < nz ? kb + CS_KC : nz;
+    const int i = i0 + tx, j = j0 + ty;
+    const bool tin = i < nx && j < ny;
+    const bool sint = i >= 2 && i <= nx - 3 && j >= 2 && j <= ny - 3;
+    const double *R0 = rhs.p, *R1 = rhs.p + rhs.cs, *R2 = rhs.p + 2 * rhs.cs;
+    const int64_t rc = tin ? rhs.at(i, j, 0) : 0;
+    auto ldr2 = [&](int k) -> double { return (tin && k >= 0 && k < nz) ? R2[rc + (int64_t)k * rhs.pz] : 0.0; };
+    // RHS_2 of the thread's column at planes k - 1, k, k + 1 of the stretching plane k = kk - 1
+    // (after the shift at the top of iteration kk; the first iteration has k = kb - 2)
+    double r2[3] = {0.0, ldr2(kb - 3), ldr2(kb - 2)};
+    for (int kk = kb - 1; kk <= kend; kk++) {
+        // plane kk into LDS, psi_0 / psi_1 of plane kk + 1 into the windows
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            w0[s][0] = w0[s][1];
+            w1[s][0] = w1[s][1];
+            w0[s][1] = w0[s][2];
+            w1[s][1] = w1[s][2];
+            w0[s][2] = ld(p0, s, kk + 1);
+            w1[s][2] = ld(p1, s, kk + 1);
+            const int q = tid + 256 * s;
+            if (q < RX * RY) {
+                const int hx = q % RX, hy = q / RX;
+                P[0][hy][hx] = w0[s][1];
+                P[1][hy][hx] = w1[s][1];
+                P[2][hy][hx] = ld(p2, s, kk);
+            }
+        }
+        __syncthreads();
+        // velocity of plane kk on the tile and its one-node halo
+        const int sl = (kk + 3) % 3;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const int q = tid + 256 * s, hx = q % RX, hy = q / RX;
+            if (q >= RX * RY || hx < 1 || hx > TX + 2 || hy < 1 || hy > TY + 2) continue;
+            const int gi = i0 - 2 + hx, gj = j0 - 2 + hy;
+            double u = 0.0, v = 0.0, w = 0.0;
+            if (gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= ny - 2 && kk >= 1 && kk <= nz - 2) {
+                double dpsidx1 = (P[1][hy][hx + 1] - P[1][hy][hx - 1]) * ix2;
+                double dpsidx2 = (P[2][hy][hx + 1] - P[2][hy][hx - 1]) * ix2;
+                double dpsidy0 = (P[0][hy + 1][hx] - P[0][hy - 1][hx]) * iy2;
+                double dpsidy2 = (P[2][hy + 1][hx] - P[2][hy - 1][hx]) * iy2;
+                double dpsidz0 = (w0[s][2] - w0[s][0]) * iz2, dpsidz1 = (w1[s][2] - w1[s][0]) * iz2;
+                u = dpsidy2 - dpsidz1;
+                v = -(dpsidx2 - dpsidz0);
+                w = dpsidx1 - dpsidy0;
+            }
+            V[sl][0][hy - 1][hx - 1] = u;
+            V[sl][1][hy - 1][hx - 1] = v;
+            V[sl][2][hy - 1][hx - 1] = w;
+        }
+        __syncthreads();
+        // stretching of plane k = kk - 1 (its velocity neighbours kk - 2 .. kk are in the ring)
+        const int k = kk - 1;
+        r2[0] = r2[1];
+        r2[1] = r2[2];
+        r2[2] = ldr2(k + 1);
+        if (k >= kb && k < kend && tin) {
+            const int s0 = k % 3, sp = (k + 1) % 3, sm = (k + 2) % 3;
+            const int64_t ov = vel.at(i, j, k);
+            for (int c = 0; c < 3; c++) vel.p[c * vel.cs + ov] = V[s0][c][ty + 1][tx + 1];
+            double r[3] = {0.0, 0.0, 0.0};
+            if (sint && k >= 2 && k <= nz - 3) {
+                const int64_t ro = rc + (int64_t)k * rhs.pz;
+                const double a0p = R0[ro + 1], a0m = R0[ro - 1], a1p = R1[ro + rhs.py], a1m = R1[ro - rhs.py];
+                const double a2p = r2[2], a2m = r2[0];
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    double wdudx = -(a0p * V[s0][m][ty + 1][tx + 2] - (a0m)*V[s0][m][ty + 1][tx]) * ix2;
+                    double wdvdy = -(a1p * V[s0][m][ty + 2][tx + 1] - (a1m)*V[s0][m][ty][tx + 1]) * iy2;
+                    double wdwdz = -(a2p * V[sp][m][ty + 1][tx + 1] - (a2m)*V[sm][m][ty + 1][tx + 1]) * iz2;
+                    r[m] = wdudx + wdvdy + wdwdz;
+                }
+            }
+            const int64_t oo = out.at(i, j, k);
+            out.p[oo] = r[0];
+            out.p[out.cs + oo] = r[1];
+            out.p[2 * out.cs + oo] = r[2];
+        }
+    }
+}
+
 int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GView rhs, GView vel, GView out) {
     if ((int64_t)n[0] * n[1] * n[2] <= 0) return VPMX_OK;
-    static const int shp[3][2] = {{64, 4}, {32, 8}, {128, 2}};
-    const void *fns[3] = {(const void *)k_curl_stretch<64, 4>, (const void *)k_curl_stretch<32, 8>,
-                          (const void *)k_curl_stretch<128, 2>};
-    const char *names[3] = {"64 x 4 column tiles", "32 x 8 column tiles", "128 x 2 column tiles"};
+    static const int shp[5][2] = {{64, 4}, {32, 8}, {128, 2}, {64, 4}, {32, 8}};
+    const void *fns[5] = {(const void *)k_curl_stretch<64, 4>, (const void *)k_curl_stretch<32, 8>,
+                          (const void *)k_curl_stretch<128, 2>, (const void *)k_curl_stretch_w<64, 4>,
+                          (const void *)k_curl_stretch_w<32, 8>};
+    const char *names[5] = {"64 x 4 column tiles", "32 x 8 column tiles", "128 x 2 column tiles",
+                            "64 x 4 column tiles, stream function once per tile",
+                            "32 x 8 column tiles, stream function once per tile"};
     double ix2 = 1.0 / (2 * h[0]), iy2 = 1.0 / (2 * h[1]), iz2 = 1.0 / (2 * h[2]);
     int nx = n[0], ny = n[1], nz = n[2];
-    void *args[] = {&nx, &ny, &nz, &ix2, &iy2, &iz2, &psi, &rhs, &vel, &out};
+    int R = c->num_cu * 4 / 8 * 8;  // about the work-groups resident at once (4 per CU)
+    if (R < 8) R = 8;
+    void *args[] = {&nx, &ny, &nz, &ix2, &iy2, &iz2, &psi, &rhs, &vel, &out, &R};
     auto grid_of = [&](int fi) {
         return dim3((unsigned)((n[0] + shp[fi][0] - 1) / shp[fi][0]), (unsigned)((n[1] + shp[fi][1] - 1) / shp[fi][1]),
                     (unsigned)((n[2] + CS_KC - 1) / CS_KC));
@@ -191,7 +326,7 @@ int curl_stretch_kernel(Ctx *c, const int n[3], const double h[3], GView psi, GV
     char key[128];
     std::snprintf(key, sizeof key, "curl and stretching %dx%dx%d", n[0], n[1], n[2]);
     Ctx::Tuned ch;
-    VPMX_TRY(tune(c, key, 3, fns, names, 0, 256, trial, &ch, false));
+    VPMX_TRY(tune(c, key, 5, fns, names, 0, 256, trial, &ch, false));
     VPMX_HIP(c, hipLaunchKernel(fns[ch.fi], grid_of(ch.fi), dim3(256), args, 0, c->stream));
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;

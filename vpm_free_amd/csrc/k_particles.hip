@@ -354,18 +354,7 @@ __global__ void k_row_counts(const int *__restrict__ start, int cnx, int64_t nro
 // fit; every thread then sums its node's 16 rows, the four cells of each, rows z-outer,
 // y-inner, cells and particles in cell order: deterministic, no atomics.  A row with more particles than a chunk holds is read straight from memory in the
 // same order.  Tiles without particles are written by k_p2m_zero_tiles.
-// Work-group order for XCD locality: the hardware hands work-group b to XCD b % 8, so within
-// each round of R work-groups (R a multiple of 8) the XCD of b takes the contiguous slice
-// [(b % 8) R / 8, (b % 8 + 1) R / 8) of the round instead of every 8th index: neighbouring tiles
-// then run at the same time on one XCD and share its L2 (halo nodes, the partial cache lines of
-// their rows).  A bijection of [0, grid); the last partial round keeps launch order.
-__device__ __forceinline__ int64_t xcd_slice(int64_t b, int64_t grid, int64_t R) {
-    const int64_t base = b / R * R;
-    if ((R & 7) || base + R > grid) return b;
-    const int64_t o = b - base;
-    return base + (o & 7) * (R >> 3) + (o >> 3);
-}
-// Round of the one-work-group-per-tile launches: 256 work-groups per XCD.
+// Round of the one-work-group-per-tile launches (xcd_slice): 256 work-groups per XCD.
 constexpr int64_t XCD_ROUND = 2048;
 
 constexpr int PT_X = 16, PT_Y = 4, PT_Z = 4;

@@ -35,6 +35,18 @@ int tune_v(Ctx *c, const std::string &key, int nf, const void *const *fns, const
     best.per_cu = occ[0];
     best.ms = -1.0;
     char buf[160];
+    if (c->tune_variant >= 0 && !c->capturing) {
+        best.fi = c->tune_variant < nf ? c->tune_variant : nf - 1;
+        best.per_cu = occ[best.fi];
+        if (persistent)
+            std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (forced)", names[best.fi], best.per_cu);
+        else
+            std::snprintf(buf, sizeof buf, "%s (forced)", names[best.fi]);
+        best.what = buf;
+        c->tuned[key] = best;
+        *out = best;
+        return VPMX_OK;
+    }
     if (!c->tuning || c->capturing) {
         if (persistent)
             std::snprintf(buf, sizeof buf, "%s, %d work-groups per CU (occupancy; not timed)", names[0], occ[0]);

@@ -1184,6 +1184,10 @@ int32_t vpmx_set_option(vpmx_ctx *p, const char *name, int32_t value) {
     } else if (k == "tuning") {
         c->tuning = value != 0;
         c->tuned.clear();
+    } else if (k == "tune_variant") {
+        if (value < -1) return fail(c, VPMX_ERR_ARG, "tune_variant: -1 (tuner) or a variant index");
+        c->tune_variant = value;
+        c->tuned.clear();
     } else {
         return fail(c, VPMX_ERR_ARG, "unknown option " + k);
     }

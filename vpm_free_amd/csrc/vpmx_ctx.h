@@ -53,6 +53,9 @@ struct Ctx : vpmx_ctx {
     // autotuner: per launch key (pass, precision, lengths, sizes) the kernel variant and the
     // resident work-groups per CU chosen by timing the candidates once
     bool tuning = true;
+    // >= 0: every launch key takes variant min(tune_variant, last) at its occupancy limit,
+    // untimed (for checking that variants meant to give the same bits do)
+    int tune_variant = -1;
     struct Tuned {
         int fi = 0, per_cu = 1;
         double ms = 0.0;

@@ -20,6 +20,19 @@ struct GView {
     }
 };
 
+// Work-group order for XCD locality: the hardware hands work-group b (linear id, x fastest) to
+// XCD b % 8, so within each round of R work-groups (R a multiple of 8) the XCD of b takes the
+// contiguous slice [(b % 8) R / 8, (b % 8 + 1) R / 8) of the round instead of every 8th index:
+// neighbouring tiles then run at the same time on one XCD and share its L2 (halo values, the
+// partial cache lines of their rows).  A bijection of [0, grid); the last partial round keeps
+// launch order.
+__device__ __forceinline__ int64_t xcd_slice(int64_t b, int64_t grid, int64_t R) {
+    const int64_t base = b / R * R;
+    if ((R & 7) || base + R > grid) return b;
+    const int64_t o = b - base;
+    return base + (o & 7) * (R >> 3) + (o >> 3);
+}
+
 // Geometry of a uniform node grid.
 struct Grid {
     int n[3];
