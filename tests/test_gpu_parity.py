@@ -196,13 +196,14 @@ def _lap7(u, h):
             (u[2:, 1:-1, 1:-1] - 2 * c + u[:-2, 1:-1, 1:-1]) / h[2] ** 2)
 
 
-@pytest.mark.parametrize("n", [289, 273])
+@pytest.mark.parametrize("n", [289, 273, 161, 145])
 def test_solve_full_size_plans(ctx, n):
-    """The block sizes of the 256-cubed workload, where the sine transforms run their
-    compile-time FFT plans (FP64: 288 = 4.4.2.9, 272 = 4.4.17; FP32: 576 = 4.4.4.9,
-    544 = 17.4.4.2), padded LDS sequences and register-capped kernels — sizes the oracle is
-    too slow for.  The FP64 solution satisfies the 7-point equations to rounding, and the
-    mixed form agrees with it to FP32 accuracy."""
+    """The block sizes of the 256-cubed and 128-cubed workloads, where the sine transforms run
+    their compile-time FFT plans (FP64: 288 = 4.4.2.9 or 8.4.9, 272 = 4.4.17 with the radix-17
+    stage on one lane or split, 160 = 8.4.5, 144 = 4.4.9; FP32: 576 = 4.4.4.9 or 8.8.9, 544 =
+    17.4.4.2 or 17.8.4), padded LDS sequences and register-capped kernels, as the tuner picks
+    them -- sizes the oracle is too slow for.  The FP64 solution satisfies the 7-point equations
+    to rounding, and the mixed form agrees with it to FP32 accuracy."""
     rng = np.random.default_rng(n)
     nn = np.array([n, n, n], np.int32)
     h = np.array([1 / 64] * 3)

@@ -574,6 +574,20 @@ template <> struct FixedFft<10> {  // 544 = 17.8.4 with the radix-17 outputs spl
         wave_fft_sx<C, Buf, false, true, 544, 1, 17, 8, 4>(b, tw, lane);
     }
 };
+template <> struct FixedFft<11> {  // 160 = 8.4.5 (FP64, 161-node block: the 128^3 class)
+    static constexpr int L = 160;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 160, 1, 8, 4, 5>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<12> {  // 144 = 4.4.9 (FP64, 145-node inner box of the 128^3 class)
+    static constexpr int L = 144;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 144, 1, 4, 4, 9>(b, tw, lane);
+    }
+};
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)
     static constexpr int L = 288;
     template <class C, class Buf, bool CR = false>

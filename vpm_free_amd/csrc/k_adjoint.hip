@@ -142,36 +142,44 @@ __global__ void __launch_bounds__(256) k_bsum_adj(Grid g, int64_t ns, const doub
 }
 
 // ---------------------------------------------------------------- A^T
-// Forward: a(c, s) = -(ds/4pi) * 1/4 sum_corners (1/h_n) sum_st coef_st s0(corner, layer st).
-// Transpose: scatter abar(c, s) * -(ds/4pi)/4/h_n * coef_st to those 20 nodes (atomics: a node
-// is a corner of up to four cells of a face, and near edges a node of several faces).
-__global__ void k_nderiv_adj(Grid g, int64_t nb, const double *__restrict__ abar, const double *__restrict__ ds,
-                             GView s0b) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nb) return;
+// Forward: a(c, s) = -(ds/4pi) * 1/4 sum_corners (1/h_n) sum_st coef_st s0(corner, layer st),
+// s a face cell, its corners nodes of the face plane and st = 0..4 the layers inward.
+// Transpose, as a gather: node t of layer st of face f gets coef_st times the sum of
+// -(ds/4pi)/4/h_n abar(c, s) over the (up to four) cells s of f it is a corner of, summed over
+// the faces whose five layers hold it (near edges, several) in face order -- every node of the
+// grid written (zeros outside the layers), no atomics, the same sum on every run.
+__global__ void __launch_bounds__(256) k_nderiv_adj(Grid g, int64_t nb, const double *__restrict__ abar,
+                                                    const double *__restrict__ ds, GView s0b) {
     const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)NX * NY * NZ) return;
+    const int crd[3] = {(int)(t % NX) + 1, (int)((t / NX) % NY) + 1, (int)(t / ((int64_t)NX * NY)) + 1};
     const int64_t fX = (int64_t)(NY - 1) * (NZ - 1), fY = (int64_t)(NX - 1) * (NZ - 1), fZ = (int64_t)(NX - 1) * (NY - 1);
     const double coef[5] = {25.0 / 12.0, -4.0, 3.0, -4.0 / 3.0, 0.25};
     const double PI4 = 4.0 * (4.0 * atan(1.0));
-    int face, i0, i1;
-    int64_t r;
-    if (s < 2 * fX) { face = (int)(s / fX); r = s % fX; i0 = (int)(r % (NY - 1)) + 1; i1 = (int)(r / (NY - 1)) + 1; }
-    else if (s < 2 * (fX + fY)) { int64_t u = s - 2 * fX; face = 2 + (int)(u / fY); r = u % fY; i0 = (int)(r % (NX - 1)) + 1; i1 = (int)(r / (NX - 1)) + 1; }
-    else { int64_t u = s - 2 * (fX + fY); face = 4 + (int)(u / fZ); r = u % fZ; i0 = (int)(r % (NX - 1)) + 1; i1 = (int)(r / (NX - 1)) + 1; }
-    const int axis = face / 2, hi = face & 1;
-    const int sg = hi ? -1 : 1;
-    const int pl = hi ? g.n[axis] : 1;
-    const double w = -(ds[s] / PI4) * 0.25 / g.h[axis];
     const int du[4] = {0, 1, 1, 0}, dv[4] = {0, 0, 1, 1};
-    for (int q = 0; q < 4; q++)
-        for (int st = 0; st < 5; st++) {
-            int ii, jj, kk;
-            if (axis == 0) { ii = pl + sg * st; jj = i0 + du[q]; kk = i1 + dv[q]; }
-            else if (axis == 1) { ii = i0 + du[q]; jj = pl + sg * st; kk = i1 + dv[q]; }
-            else { ii = i0 + du[q]; jj = i1 + dv[q]; kk = pl + sg * st; }
-            const int64_t o = s0b.at(ii - 1, jj - 1, kk - 1);
-            for (int c = 0; c < 3; c++) atomicAdd(&s0b.p[c * s0b.cs + o], w * coef[st] * abar[c * nb + s]);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int face = 0; face < 6; face++) {
+        const int axis = face / 2, hi = face & 1;
+        const int st = hi ? g.n[axis] - crd[axis] : crd[axis] - 1;
+        if (st < 0 || st > 4) continue;
+        // the face's in-plane node coordinates (the forward's i0 + du, i1 + dv) and counts
+        const int a = axis == 0 ? crd[1] : crd[0], b = axis == 2 ? crd[1] : crd[2];
+        const int N0 = axis == 0 ? NY : NX, N1 = axis == 2 ? NY : NZ;
+        const int64_t base = axis == 0 ? (int64_t)face * fX
+                             : axis == 1 ? 2 * fX + (int64_t)(face - 2) * fY
+                                         : 2 * (fX + fY) + (int64_t)(face - 4) * fZ;
+        for (int q = 0; q < 4; q++) {
+            const int i0 = a - du[q], i1 = b - dv[q];
+            if (i0 < 1 || i0 > N0 - 1 || i1 < 1 || i1 > N1 - 1) continue;
+            const int64_t s = base + (i0 - 1) + (int64_t)(i1 - 1) * (N0 - 1);
+            if (s >= nb) continue;
+            const double w = -(ds[s] / PI4) * 0.25 / g.h[axis];
+            for (int c = 0; c < 3; c++) acc[c] += w * coef[st] * abar[c * nb + s];
         }
+    }
+    const int64_t o = s0b.at(crd[0] - 1, crd[1] - 1, crd[2] - 1);
+    for (int c = 0; c < 3; c++) s0b.p[c * s0b.cs + o] = acc[c];
 }
 
 // ---------------------------------------------------------------- T^T
@@ -578,7 +586,8 @@ int bsum_adj(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, GView gb, do
     return VPMX_OK;
 }
 int nderiv_adj(Ctx *c, const Grid &g, int64_t nb, const double *abar, const double *ds, GView s0b) {
-    hipLaunchKernelGGL(k_nderiv_adj, dim3(blocks_of(nb)), dim3(256), 0, c->stream, g, nb, abar, ds, s0b);
+    const int64_t nt = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    hipLaunchKernelGGL(k_nderiv_adj, dim3(blocks_of(nt)), dim3(256), 0, c->stream, g, nb, abar, ds, s0b);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
 }

@@ -1422,6 +1422,9 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     if (sizeof(T) == 8 && fp->L == 288 && same(r288, 5)) fix = 2;
     if (sizeof(T) == 8 && fp->L == 272 && same(r272, 3)) fix = 3;
     if (sizeof(T) == 4 && fp->L == 544 && same(r544, 4)) fix = 5;
+    static const int r160[] = {4, 4, 2, 5}, r144[] = {4, 4, 3, 3};
+    if (sizeof(T) == 8 && fp->L == 160 && same(r160, 4)) fix = 11;
+    if (sizeof(T) == 8 && fp->L == 144 && same(r144, 4)) fix = 12;
     // kernel variants: with a compile-time plan the registers capped for four waves per SIMD
     // (six for the FP32 x inverse of the 576-point plan) or not, each in both batch orders;
     // FP32 compile-time plans also with 32-line batches (1024 threads) -- the tuner times all
@@ -1448,6 +1451,12 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
         } else if (fix == 3) {
             unc = (const void *)k_dst_wave<MODE, 17, T, 3>;
             cap = (const void *)k_dst_wave_cap<k3, MODE, 17, T, 3>;
+        } else if (fix == 11) {
+            unc = (const void *)k_dst_wave<MODE, 9, T, 11>;
+            cap = (const void *)k_dst_wave_cap<4, MODE, 9, T, 11>;
+        } else if (fix == 12) {
+            unc = (const void *)k_dst_wave<MODE, 9, T, 12>;
+            cap = (const void *)k_dst_wave_cap<4, MODE, 9, T, 12>;
         }
     } else {
         if (fix == 0) {

@@ -658,8 +658,7 @@ static int pmesh_inf_adj(Ctx *c, const Grid &g, GView pb, GView rb, DBuf &t1, DB
     const int sp = c->adj_bs == 2 ? boundary_sums_spectral_adj(c, g, nb, view(w2, n), abar, slot) : VPMX_ERR_UNSUP;
     if (sp == VPMX_ERR_UNSUP) VPMX_TRY(bsum_adj(c, g, nb, cxyz, view(w2, n), abar));
     else VPMX_TRY(sp);
-    VPMX_TRY(fill(c, w2, 3 * G, 0.0));
-    VPMX_TRY(nderiv_adj(c, g, nb, abar, wgt + 3 * nb, view(w2, n)));
+    VPMX_TRY(nderiv_adj(c, g, nb, abar, wgt + 3 * nb, view(w2, n)));  // writes every node of w2
     VPMX_TRY(poisson_dirichlet(c, 3, n, h, view(w2, n), nullptr, rb, 0, SOLVE_FP64));
     VPMX_TRY(axpy_view(c, n, view(y1, n), rb));
     return VPMX_OK;

@@ -223,6 +223,7 @@ int aos_to_soa(Ctx *c, int ncomp, const int n[3], const double *src, GView dst);
 int curl_adj(Ctx *c, const int n[3], const double h[3], GView ub, GView pb);
 int fold_adj(Ctx *c, const int n[3], const double h[3], GView pb, GView y, GView out);
 int bsum_adj(Ctx *c, const Grid &g, int64_t ns, const double *cxyz, GView gb, double *abar);
+// (writes every node of the grid g in s0b)
 int nderiv_adj(Ctx *c, const Grid &g, int64_t nb, const double *abar, const double *ds, GView s0b);
 int face_interp_adj(Ctx *c, const InterpArgs &a, GView fb, GView phib, GView cb, GView sb);
 int coarse_laplacian_adj(Ctx *c, const int cn[3], const int lo[3], const int hi[3], const double hc[3], GView rb,
