@@ -220,21 +220,25 @@ __global__ void k_cell_keys(Grid g, const double *xp, int64_t n, const int *dn, 
     }
     keys[p] = key;
 }
-__global__ void k_cell_scatter(const int *keys, int64_t n, const int *dn, const int *start, int *fill, int *perm) {
+// Each particle takes a slot of its cell by counting the cell's count back down, so the counts
+// are all zero again when the scatter has run (the next sort needs no clearing pass).
+__global__ void k_cell_scatter(const int *keys, int64_t n, const int *dn, const int *start, int *count, int *perm) {
     int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pcount(n, dn)) return;
     int key = keys[p];
     if (key < 0) return;
-    int pos = start[key] + atomicAdd(&fill[key], 1);
+    int pos = start[key] + atomicSub(&count[key], 1) - 1;
     perm[pos] = (int)p;
 }
-// Order the particles of each cell by particle index (the scatter above is unordered).
-__global__ void k_cell_order(const int *start, const int *count, int64_t ncells, int *perm) {
-    int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= ncells) return;
-    int cnt = count[cidx];
-    if (cnt < 2) return;
-    int s = start[cidx];
+// Order the particles of each cell by particle index (the scatter above is unordered): one
+// thread per sorted slot, the slot that opens a cell of two or more particles sorts it.
+__global__ void k_cell_order(const int *start, const int *keys, int64_t ncells, int *perm) {
+    const int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= (int64_t)start[ncells]) return;
+    const int cidx = keys[perm[sl]];
+    const int s = start[cidx];
+    const int cnt = start[cidx + 1] - s;
+    if (s != sl || cnt < 2) return;
     for (int a = 1; a < cnt; a++) {
         int v = perm[s + a];
         int b = a - 1;
@@ -253,19 +257,27 @@ int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState
     s.ncells = (int64_t)s.cn[0] * s.cn[1] * s.cn[2];
     int *keys = (int *)ensure(c, s.keys, sizeof(int) * (size_t)(n > 0 ? n : 1), &st); if (st) return st;
     int *perm = (int *)ensure(c, s.perm, sizeof(int) * (size_t)(n > 0 ? n : 1), &st); if (st) return st;
+    const void *was = s.count.p;
     int *count = (int *)ensure(c, s.count, sizeof(int) * (size_t)s.ncells, &st); if (st) return st;
     int *start = (int *)ensure(c, s.start, sizeof(int) * (size_t)(s.ncells + 1), &st); if (st) return st;
-    int *fill = (int *)ensure(c, s.fill, sizeof(int) * (size_t)s.ncells, &st); if (st) return st;
-    unsigned gc = (unsigned)((s.ncells + 255) / 256), gp = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, count, s.ncells);
-    hipLaunchKernelGGL(k_zero_int, dim3(gc), dim3(256), 0, c->stream, fill, s.ncells);
+    unsigned gp = (unsigned)((n + 255) / 256);
+    // the counts are zero after every completed sort (the scatter counts them back down); a new
+    // buffer, or a sort that stopped half way, clears them first
+    if (count != was || !s.count_zero) {
+        const int64_t words = (int64_t)(s.count.bytes / sizeof(int));
+        hipLaunchKernelGGL(k_zero_int, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, c->stream, count, words);
+    }
+    s.count_zero = false;
     if (n > 0)
         hipLaunchKernelGGL(k_cell_keys, dim3(gp), dim3(256), 0, c->stream, g, xp, n, c->dn, s.cn[0], s.cn[1], s.cn[2],
                            keys, count);
     VPMX_TRY(exclusive_scan(c, count, start, s.ncells));
-    if (n > 0) hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(256), 0, c->stream, keys, n, c->dn, start, fill, perm);
-    hipLaunchKernelGGL(k_cell_order, dim3(gc), dim3(256), 0, c->stream, start, count, s.ncells, perm);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_cell_scatter, dim3(gp), dim3(256), 0, c->stream, keys, n, c->dn, start, count, perm);
+        hipLaunchKernelGGL(k_cell_order, dim3(gp), dim3(256), 0, c->stream, start, keys, s.ncells, perm);
+    }
     VPMX_HIP(c, hipGetLastError());
+    s.count_zero = true;
     return VPMX_OK;
 }
 

@@ -72,6 +72,7 @@ struct SortState {
     DBuf rowcnt;  // particles per cell row (projection)
     int64_t ncells = 0;
     int cn[3] = {0, 0, 0};  // padded cell grid dims
+    bool count_zero = false;  // every word of count is zero (a completed sort leaves it so)
 };
 // Sort particles (xp AoS 3xn) into cells of grid g; cells are node indices inode in [-1, n+1].
 int sort_particles(Ctx *c, const Grid &g, const double *xp, int64_t n, SortState &s);
