@@ -77,6 +77,33 @@ __device__ __forceinline__ void dft_reg(C (&v)[R]) {
         v[6] = csub(e[2], t2);
         v[3] = cadd(e[3], t3);
         v[7] = csub(e[3], t3);
+    } else if constexpr (R == 16) {
+        // 16 = 4 x 4 with n = 4a + b, k = k1 + 4 k2: a 4-point DFT over a for each b, the
+        // twiddles exp(-2 pi i b k1 / 16), then a 4-point DFT over b for each k1
+        C y[4][4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+#pragma unroll
+            for (int a = 0; a < 4; a++) y[b][a] = v[4 * a + b];
+            dft_reg<4, C>(y[b]);
+        }
+        const T c1 = (T)0.92387953251128675613, s1 = (T)0.38268343236508977173, r2 = (T)0.70710678118654752440;
+        y[1][1] = cmul(y[1][1], Cx<C>::mk(c1, -s1));                                  // W^1
+        y[1][2] = Cx<C>::mk(r2 * (y[1][2].x + y[1][2].y), r2 * (y[1][2].y - y[1][2].x));  // W^2
+        y[1][3] = cmul(y[1][3], Cx<C>::mk(s1, -c1));                                  // W^3
+        y[2][1] = Cx<C>::mk(r2 * (y[2][1].x + y[2][1].y), r2 * (y[2][1].y - y[2][1].x));  // W^2
+        y[2][2] = Cx<C>::mk(y[2][2].y, -y[2][2].x);                                   // W^4 = -i
+        y[2][3] = Cx<C>::mk(r2 * (y[2][3].y - y[2][3].x), -r2 * (y[2][3].x + y[2][3].y));  // W^6
+        y[3][1] = cmul(y[3][1], Cx<C>::mk(s1, -c1));                                  // W^3
+        y[3][2] = Cx<C>::mk(r2 * (y[3][2].y - y[3][2].x), -r2 * (y[3][2].x + y[3][2].y));  // W^6
+        y[3][3] = cmul(y[3][3], Cx<C>::mk(-c1, s1));                                  // W^9
+#pragma unroll
+        for (int k1 = 0; k1 < 4; k1++) {
+            C z[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+            dft_reg<4, C>(z);
+#pragma unroll
+            for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = z[k2];
+        }
     } else if constexpr (R == 9) {
         // 9 = 3 x 3 with n = 3a + b, k = k1 + 3 k2: a 3-point DFT over a for each b, the
         // twiddles exp(-2 pi i b k1 / 9), then a 3-point DFT over b for each k1
@@ -439,7 +466,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
                     for (int r = 1; r < R; r++) v[i][r] = cmul(v[i][r], tw[r * k0 * twstep]);
                 }
             }
-            if constexpr (R <= 4 || R == 8 || R == 9) dft_reg<R>(v[i]);
+            if constexpr (R <= 4 || R == 8 || R == 9 || R == 16) dft_reg<R>(v[i]);
             dst[i] = q * Ns * R + k0;
         }
     }
@@ -447,7 +474,7 @@ __device__ __forceinline__ void wave_stage_s(Buf buf, const C *__restrict__ tw, 
 #pragma unroll
     for (int i = 0; i < BPL; i++)
         if (dst[i] >= 0) {
-            if constexpr (R <= 4 || R == 8 || R == 9) {
+            if constexpr (R <= 4 || R == 8 || R == 9 || R == 16) {
 #pragma unroll
                 for (int k = 0; k < R; k++) buf[dst[i] + k * Ns] = v[i][k];
             } else if constexpr (R == 17 && CR) {
@@ -544,6 +571,13 @@ template <> struct FixedFft<6> {  // 576 = 8.8.9 (FP32, 289-node block): three s
     template <class C, class Buf, bool CR = false>
     static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
         wave_fft_s<C, Buf, CR, 576, 1, 8, 8, 9>(b, tw, lane);
+    }
+};
+template <> struct FixedFft<13> {  // 576 = 16.4.9 (FP32, 289-node block): a radix-16 first stage (36 butterflies)
+    static constexpr int L = 576;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_s<C, Buf, CR, 576, 1, 16, 4, 9>(b, tw, lane);
     }
 };
 template <> struct FixedFft<7> {  // 288 = 8.4.9 (FP64, 289-node block): three stages instead of four

@@ -842,6 +842,9 @@ __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
     // the 8.8.9 plan: one slot per 8 elements keeps its radix-8 stores (strides 8 and 64
     // elements for groups of 8 lanes) on distinct banks in every pass
     if (fix == 6 || fix == 7 || fix == 8 || fix == 10) return 3;
+    // the 16.4.9 plan: its radix-16 stage writes 16 consecutive elements per lane (lanes 16
+    // elements apart), which one slot per 16 elements spreads over the banks
+    if (fix == 13) return 4;
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
@@ -1515,6 +1518,11 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
                 add((const void *)k_dst_wave<MODE, 9, T, 6>, "8.8.9 plan, batch halves in turn", 16, 1, 6);
                 add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 6>, "8.8.9 plan, capped, batch halves in turn", 16,
                     1, 6);
+                add((const void *)k_dst_wave<MODE, 9, T, 13>, "16.4.9 plan", 16, 0, 13);
+                add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 13>, "16.4.9 plan, capped", 16, 0, 13);
+                add((const void *)k_dst_wave<MODE, 9, T, 13>, "16.4.9 plan, batch halves in turn", 16, 1, 13);
+                add((const void *)k_dst_wave_cap<k0, MODE, 9, T, 13>, "16.4.9 plan, capped, batch halves in turn",
+                    16, 1, 13);
             }
         }
     } else {
