@@ -622,6 +622,13 @@ template <> struct FixedFft<12> {  // 144 = 4.4.9 (FP64, 145-node inner box of t
         wave_fft_s<C, Buf, CR, 144, 1, 4, 4, 9>(b, tw, lane);
     }
 };
+template <> struct FixedFft<14> {  // 272 = 16.17, the radix-17 outputs split over 4 lanes: two stages
+    static constexpr int L = 272;
+    template <class C, class Buf, bool CR = false>
+    static __device__ __forceinline__ void run(Buf b, const C *tw, int lane) {
+        wave_fft_sx<C, Buf, false, true, 272, 1, 16, 17>(b, tw, lane);
+    }
+};
 template <> struct FixedFft<2> {  // 288 = 4.4.2.9 (FP64, 289-node block)
     static constexpr int L = 288;
     template <class C, class Buf, bool CR = false>

@@ -842,9 +842,9 @@ __host__ __device__ constexpr int wave_pad_shift(bool dbl, int mode, int fix) {
     // the 8.8.9 plan: one slot per 8 elements keeps its radix-8 stores (strides 8 and 64
     // elements for groups of 8 lanes) on distinct banks in every pass
     if (fix == 6 || fix == 7 || fix == 8 || fix == 10) return 3;
-    // the 16.4.9 plan: its radix-16 stage writes 16 consecutive elements per lane (lanes 16
-    // elements apart), which one slot per 16 elements spreads over the banks
-    if (fix == 13) return 4;
+    // the 16.4.9 and 16.17 plans: their radix-16 stage writes 16 consecutive elements per lane
+    // (lanes 16 elements apart), which one slot per 16 elements spreads over the banks
+    if (fix == 13 || fix == 14) return 4;
     return fix < 0 ? 0 : (mode == DST_Y || (mode == DST_Z && (dbl || fix == 0)) || (!dbl && fix == 0 && mode == DST_XF)) ? (dbl ? 3 : 4) : 0;
 }
 // Radix-17 roots as constants in the 272- and 544-point plans (measured per pass: faster where
@@ -1490,6 +1490,10 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
                 add((const void *)k_dst_wave<MODE, 17, T, 9>, "radix-17 split, batch halves in turn", 16, 1, 9);
                 add((const void *)k_dst_wave_cap<k3, MODE, 17, T, 9>, "radix-17 split, capped, batch halves in turn",
                     16, 1, 9);
+                // and the two-stage 16.17 plan, radix-17 split
+                add((const void *)k_dst_wave<MODE, 17, T, 14>, "16.17 plan, radix-17 split", 16, 0, 14);
+                add((const void *)k_dst_wave<MODE, 17, T, 14>, "16.17 plan, radix-17 split, batch halves in turn", 16,
+                    1, 14);
             }
             if (fix == 2) {  // the three-stage 8.4.9 plan of the same length
                 add((const void *)k_dst_wave<MODE, 9, T, 7>, "8.4.9 plan", 16, 0, 7);
@@ -1653,7 +1657,7 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         int fix32, fix64;
         const char *name;
     };
-    ZK zk[3];
+    ZK zk[4];
     int nk = 1;
     if (f32->L == 576 && same(f32, r576, 5) && f64->L == 288 && same(f64, r288, 5)) {
         zk[0] = {(const void *)k_dst_zmix<4, 0, 2>, 0, 2, "FP32 forward, FP64 inverse"};
@@ -1661,7 +1665,8 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         zk[0] = {(const void *)k_dst_zmix<17, 5, 3>, 5, 3, "FP32 forward, FP64 inverse"};
         zk[1] = {(const void *)k_dst_zmix<17, 10, 9>, 10, 9, "FP32 forward (17.8.4), FP64 inverse, radix-17 split"};
         zk[2] = {(const void *)k_dst_zmix<17, 5, 9>, 5, 9, "FP32 forward, FP64 inverse with radix-17 split"};
-        nk = 3;
+        zk[3] = {(const void *)k_dst_zmix<17, 5, 14>, 5, 14, "FP32 forward, FP64 inverse 16.17 with radix-17 split"};
+        nk = 4;
     } else {
         zk[0] = {rc == 4 ? (const void *)k_dst_zmix<4, -1, -1>
                  : rc == 7 ? (const void *)k_dst_zmix<7, -1, -1>
@@ -1676,11 +1681,11 @@ static int launch_zmix(Ctx *c, DstArgs a, int ncomp, const AxisPlan &X, double *
         const int Wq = (Np32 + 1) / 2 > Np64 ? (Np32 + 1) / 2 : Np64;
         return sizeof(double2) * (size_t)(8 * Wq + f32->L / 2 + f64->L);
     };
-    const void *fns[6];
-    const char *names[6];
-    size_t ldsv[6];
-    int thr[6];
-    char nbuf[6][96];
+    const void *fns[8];
+    const char *names[8];
+    size_t ldsv[8];
+    int thr[8];
+    char nbuf[8][96];
     for (int q = 0; q < nk; q++)
         for (int o = 0; o < 2; o++) {
             const int v = 2 * q + o;
