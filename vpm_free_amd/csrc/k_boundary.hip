@@ -20,14 +20,53 @@ namespace vpmx {
 
 // Sources of face `f` (0 XMIN, 1 XMAX, 2 YMIN, 3 YMAX, 4 ZMIN, 5 ZMAX), in the reference's
 // order: k outer / j inner on X faces, k outer / i inner on Y faces, j outer / i inner on Z.
-__global__ void k_normal_derivs(Grid g, GView s0, int ncomp, int64_t nb, double *sig, double *cxyz, double *a,
-                                double *ds) {
+// Face-node offsets of the node-derivative table: face f holds ncomp x N0 x N1 values (N0, N1
+// its in-plane node counts: x faces (NY, NZ), y faces (NX, NZ), z faces (NX, NY)).
+struct FaceNodes {
+    int64_t off[7];
+};
+__host__ __device__ inline void face_dims(const int n[3], int f, int *N0, int *N1) {
+    const int axis = f / 2;
+    *N0 = axis == 0 ? n[1] : n[0];
+    *N1 = axis == 2 ? n[1] : n[2];
+}
+
+// The one-sided 5-point normal derivative (1/h_n)(25/12 f0 - 4 f1 + 3 f2 - 4/3 f3 + 1/4 f4) at
+// every node of every face, once per node (a node is a corner of up to four face cells):
+// thread per (face, node), the in-plane index with the faster memory stride first.
+__global__ void __launch_bounds__(256) k_face_node_derivs(Grid g, GView s0, int ncomp, FaceNodes fo, double *fd) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nn = fo.off[6] / ncomp;
+    if (t >= nn) return;
+    int f = 0;
+    while (f < 5 && t >= fo.off[f + 1] / ncomp) f++;
+    const int64_t r = t - fo.off[f] / ncomp;
+    int N0, N1;
+    face_dims(g.n, f, &N0, &N1);
+    const int a = (int)(r % N0) + 1, b = (int)(r / N0) + 1;
+    const int axis = f / 2, hi = f & 1, sg = hi ? -1 : 1, pl = hi ? g.n[axis] : 1;
+    const double a1 = 25.0 / 12.0, a2 = -4.0, a3 = 3.0, a4 = -4.0 / 3.0, a5 = 0.25;
+    const double hn = g.h[axis];
+    for (int c = 0; c < ncomp; c++) {
+        const double *sp = s0.p + c * s0.cs;
+        double v[5];
+        for (int st = 0; st < 5; st++) {
+            const int q = pl + sg * st;
+            const int ii = axis == 0 ? q : a, jj = axis == 1 ? q : (axis == 0 ? a : b), kk = axis == 2 ? q : b;
+            v[st] = sp[s0.at(ii - 1, jj - 1, kk - 1)];
+        }
+        fd[fo.off[f] + (int64_t)c * N0 * N1 + r] = (1.0 / hn) * ((((a1 * v[0] + a2 * v[1]) + a3 * v[2]) + a4 * v[3]) + a5 * v[4]);
+    }
+}
+
+__global__ void k_normal_derivs(Grid g, const double *__restrict__ fd, FaceNodes fo, int ncomp, int64_t nb, double *sig,
+                                double *cxyz, double *a, double *ds) {
 #pragma clang fp contract(off)
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nb) return;
     const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
     const int64_t fX = (int64_t)(NY - 1) * (NZ - 1), fY = (int64_t)(NX - 1) * (NZ - 1), fZ = (int64_t)(NX - 1) * (NY - 1);
-    const double a1 = 25.0 / 12.0, a2 = -4.0, a3 = 3.0, a4 = -4.0 / 3.0, a5 = 0.25;
     const double DX = g.h[0], DY = g.h[1], DZ = g.h[2];
     const double PI4 = 4.0 * (4.0 * atan(1.0));
     int face, i0, i1;  // face and the two in-plane 1-based cell indices
@@ -36,30 +75,16 @@ __global__ void k_normal_derivs(Grid g, GView s0, int ncomp, int64_t nb, double 
     else if (s < 2 * (fX + fY)) { int64_t u = s - 2 * fX; face = 2 + (int)(u / fY); r = u % fY; i0 = (int)(r % (NX - 1)) + 1; i1 = (int)(r / (NX - 1)) + 1; }
     else { int64_t u = s - 2 * (fX + fY); face = 4 + (int)(u / fZ); r = u % fZ; i0 = (int)(r % (NX - 1)) + 1; i1 = (int)(r / (NX - 1)) + 1; }
     const int axis = face / 2, hi = face & 1;
-    const int sg = hi ? -1 : 1;
     const int pl = hi ? g.n[axis] : 1;  // plane node index
-    // node index of corner (du, dv) -> (i,j,k) 1-based
-    auto node = [&](int du, int dv, int step, int *ii, int *jj, int *kk) {
-        if (axis == 0) { *ii = pl + sg * step; *jj = i0 + du; *kk = i1 + dv; }
-        else if (axis == 1) { *ii = i0 + du; *jj = pl + sg * step; *kk = i1 + dv; }
-        else { *ii = i0 + du; *jj = i1 + dv; *kk = pl + sg * step; }
-    };
     // corner order of psi1..psi4: X faces (j,k),(j1,k),(j1,k1),(j,k1); Y faces (i,k),(i1,k),(i1,k1),(i,k1);
-    // Z faces (i,j),(i1,j),(i1,j1),(i,j1)
+    // Z faces (i,j),(i1,j),(i1,j1),(i,j1); their derivatives from the face-node table
     const int du[4] = {0, 1, 1, 0}, dv[4] = {0, 0, 1, 1};
-    const double hn = g.h[axis];
+    int N0, N1;
+    face_dims(g.n, face, &N0, &N1);
     for (int c = 0; c < ncomp; c++) {
-        const double *sp = s0.p + c * s0.cs;
+        const double *fc = fd + fo.off[face] + (int64_t)c * N0 * N1;
         double p[4];
-        for (int q = 0; q < 4; q++) {
-            int ii, jj, kk;
-            double f[5];
-            for (int st = 0; st < 5; st++) {
-                node(du[q], dv[q], st, &ii, &jj, &kk);
-                f[st] = sp[s0.at(ii - 1, jj - 1, kk - 1)];
-            }
-            p[q] = (1.0 / hn) * ((((a1 * f[0] + a2 * f[1]) + a3 * f[2]) + a4 * f[3]) + a5 * f[4]);
-        }
+        for (int q = 0; q < 4; q++) p[q] = fc[(i0 + du[q] - 1) + (int64_t)(i1 + dv[q] - 1) * N0];
         double sv = 0.25 * (((p[0] + p[1]) + p[2]) + p[3]);
         sig[c * nb + s] = sv;
     }
@@ -102,9 +127,24 @@ int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, doubl
     const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
     if (NX < 6 || NY < 6 || NZ < 6) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
     int64_t nb = 2 * ((int64_t)(NY - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NY - 1));
+    // the derivative at each face node once, then each cell's four-corner average
+    FaceNodes fo;
+    fo.off[0] = 0;
+    for (int f = 0; f < 6; f++) {
+        int N0, N1;
+        face_dims(g.n, f, &N0, &N1);
+        fo.off[f + 1] = fo.off[f] + (int64_t)ncomp * N0 * N1;
+    }
+    int st = VPMX_OK;
+    double *fd = (double *)ensure(c, c->nd_face, sizeof(double) * (size_t)fo.off[6], &st);
+    if (st) return st;
+    const int64_t nn = fo.off[6] / ncomp;
+    hipLaunchKernelGGL(k_face_node_derivs, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, g, s0, ncomp,
+                       fo, fd);
+    VPMX_HIP(c, hipGetLastError());
     // wgt holds a (ncomp*nb) followed by ds (nb)
-    hipLaunchKernelGGL(k_normal_derivs, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g, s0, ncomp, nb,
-                       sig, cxyz, wgt, wgt + (int64_t)ncomp * nb);
+    hipLaunchKernelGGL(k_normal_derivs, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g, fd, fo, ncomp,
+                       nb, sig, cxyz, wgt, wgt + (int64_t)ncomp * nb);
     VPMX_HIP(c, hipGetLastError());
     *nbound = nb;
     return VPMX_OK;

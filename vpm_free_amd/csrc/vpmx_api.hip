@@ -1110,7 +1110,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->tune_scratch, &c->adj_a, &c->adj_b, &c->adj_c, &c->adj_d, &c->adj_e, &c->adj_f,
                     &c->adj_g, &c->adj_h, &c->adj_i, &c->adj_p, &c->adj_q, &c->adj_u, &c->adj_s, &c->adj_r,
                     &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm, &c->adj_pu,
-                    &c->adj_ps};
+                    &c->adj_ps, &c->nd_face};
     for (DBuf *b : bufs) release(*b);
     for (auto &r : c->adj_steps)
         for (DBuf &b : r.b) release(b);

@@ -82,6 +82,7 @@ struct Ctx : vpmx_ctx {
     // grids
     DBuf rhs_bl, sol_bl, sol0, vel, str;
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c, panel_a;
+    DBuf nd_face;  // normal derivatives at the face nodes (before the per-cell averages)
     int bs_mode = 2;  // boundary sums (option boundsum): 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx[2] = {nullptr, nullptr};  // spectral boundary-sum plans: fine block, coarse grid
     void *tree = nullptr;  // tree boundary-sum plans per (grid, levels)
