@@ -5,7 +5,7 @@
 //
 // A work-group of four waves computes a BM x BN tile of C; the waves sit WM x (4/WM) over the
 // tile and each owns MB x NB blocks of 16 x 16 (one 4-double accumulator per lane per block).
-// K advances 16 at a time through two LDS buffers (k-major rows, padded by one double): the
+// K advances BK (16 or 32) at a time through two LDS buffers (k-major rows, padded by one double): the
 // next step's elements are loaded into registers while the matrix cores work on the current
 // buffer.  Every element of C accumulates its K products in chunks of 4 in increasing k
 // whatever the tile shape, so all variants give the same bits.  Tiles are numbered m-fastest,
@@ -35,10 +35,10 @@ struct GemmArgs {
 
 constexpr int DG_XCD = 8;
 
-template <bool TA, bool TB, int WM, int MB, int NB>
+template <bool TA, bool TB, int WM, int MB, int NB, int BK = 16>
 __global__ void __launch_bounds__(256) k_dgemm(GemmArgs g) {
     constexpr int WN = 4 / WM;
-    constexpr int BM = WM * MB * 16, BN = WN * NB * 16, BK = 16;
+    constexpr int BM = WM * MB * 16, BN = WN * NB * 16;
     constexpr int LA = BM + 1, LB = BN + 1;
     constexpr int NA = BM * BK / 256, NBL = BN * BK / 256;
     static_assert(WM * WN == 4 && NA * 256 == BM * BK && NBL * 256 == BN * BK, "tile shape");
@@ -146,11 +146,13 @@ struct DgShape {
     int wm, mb, nb;
     const char *name;
 };
-constexpr int DG_NV = 4;
+constexpr int DG_NV = 6;
 constexpr DgShape dg_shapes[DG_NV] = {{4, 1, 9, "64 x 144 tiles"},
                                       {4, 1, 4, "64 x 64 tiles, waves along M"},
                                       {2, 2, 2, "64 x 64 tiles, 2 x 2 waves"},
-                                      {2, 1, 2, "32 x 64 tiles"}};
+                                      {2, 1, 2, "32 x 64 tiles"},
+                                      {4, 1, 9, "64 x 144 tiles, k-steps of 32"},
+                                      {4, 1, 4, "64 x 64 tiles, waves along M, k-steps of 32"}};
 
 template <bool TA, bool TB>
 void dg_fns(const void *fns[DG_NV]) {
@@ -158,6 +160,8 @@ void dg_fns(const void *fns[DG_NV]) {
     fns[1] = (const void *)k_dgemm<TA, TB, 4, 1, 4>;
     fns[2] = (const void *)k_dgemm<TA, TB, 2, 2, 2>;
     fns[3] = (const void *)k_dgemm<TA, TB, 2, 1, 2>;
+    fns[4] = (const void *)k_dgemm<TA, TB, 4, 1, 9, 32>;
+    fns[5] = (const void *)k_dgemm<TA, TB, 4, 1, 4, 32>;
 }
 
 }  // namespace
