@@ -771,6 +771,7 @@ __global__ void __launch_bounds__(256) k_m2p(Grid g, const double *xp, const dou
 constexpr int M2P_TX = 8, M2P_TY = 8, M2P_TZ = 4;
 constexpr int M2P_NX = M2P_TX + 3, M2P_NY = M2P_TY + 3, M2P_NZ = M2P_TZ + 3;
 constexpr int M2P_ROWS = M2P_TY * M2P_TZ;
+static_assert(M2P_ROWS <= 64, "the row scan runs in one wave");
 
 // Tiles holding particles: flag per tile (from each particle's cell key; every writer of a
 // flag writes 1), then (after an exclusive scan) their list.
@@ -809,21 +810,27 @@ __global__ void __launch_bounds__(256) k_m2p_tile(Grid g, const int *__restrict_
     const int cx0 = tx * M2P_TX, cy0 = ty * M2P_TY, cz0 = tz * M2P_TZ;
     const int cx1 = min(cx0 + M2P_TX, cnx);
     const int tid = threadIdx.x;
-    if (tid < M2P_ROWS) {
-        const int cyy = cy0 + tid % M2P_TY, czz = cz0 + tid / M2P_TY;
+    if (tid < 64) {  // the rows' first particles and counts, the counts scanned across the wave
         int cnt = 0, p0 = 0;
-        if (cyy < cny && czz < cnz) {
-            const int64_t row = (int64_t)cnx * (cyy + (int64_t)cny * czz);
-            p0 = start[row + cx0];
-            cnt = start[row + cx1] - p0;
+        if (tid < M2P_ROWS) {
+            const int cyy = cy0 + tid % M2P_TY, czz = cz0 + tid / M2P_TY;
+            if (cyy < cny && czz < cnz) {
+                const int64_t row = (int64_t)cnx * (cyy + (int64_t)cny * czz);
+                p0 = start[row + cx0];
+                cnt = start[row + cx1] - p0;
+            }
         }
-        rp[tid] = p0;
-        rs[tid + 1] = cnt;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        rs[0] = 0;
-        for (int r = 0; r < M2P_ROWS; r++) rs[r + 1] += rs[r];
+        int v = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(v, d, 64);
+            if (tid >= d) v += o;
+        }
+        if (tid < M2P_ROWS) {
+            rp[tid] = p0;
+            rs[tid + 1] = v;
+        }
+        if (tid == 0) rs[0] = 0;
     }
     __syncthreads();
     const int total = rs[M2P_ROWS];
