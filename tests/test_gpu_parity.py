@@ -498,6 +498,69 @@ def test_pmesh_inf_reuse_full_size(vpmlib, prec, tol):
     assert rel_l2(out["1"], out["0"]) <= tol
 
 
+@pytest.mark.parametrize("nn,levmax,ibctyp", [((33, 29, 31), -1, 2), ((65, 57, 49), -1, 2), ((43, 43, 43), 2, 2),
+                                               ((33, 29, 31), -1, 3)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_pmesh_inf_spectral_face_derivs(vpmlib, oracle, nn, levmax, ibctyp, prec):
+    """pmesh with the zero-BC solve's normal derivatives taken from its sine spectrum after the
+    z pass (option face_derivs = 1: face sums of the work array, then products with the sine
+    matrices; no y and x inverse passes) against the oracle, which differentiates the solved
+    field (calc_normalderiv_3d); point and panel sources, direct and tree sums.  FP64 at 1e-12;
+    the mixed form's FP32 passes within 2e-6 (the derivatives themselves are FP64 sums here)."""
+    V = vpmlib
+    nn = np.array(nn, np.int32)
+    dpm = np.array([1 / 32, 1 / 30, 1 / 28])
+    xb = np.array([-1.0, -0.95, -0.85, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.02, -0.03, 0.01), s=0.25)
+    sol = np.zeros_like(rhs)
+    say("case: spectral face derivatives", nn.tolist(), "levmax", levmax, "ibctyp", ibctyp, "precision", prec)
+    try:
+        oracle.set_ibctyp(ibctyp)
+        oracle.set_tree(levmax)
+        ref = oracle.pmesh_inf(xb, dpm, nn, rhs, sol)
+    finally:
+        oracle.set_ibctyp(2)
+        oracle.set_tree(-1)
+    out = {}
+    for fd in (1, 0):
+        with V.VPM(V.Params.make(0.125)) as c:
+            c.set_option("face_derivs", fd)
+            c.set_precision(prec)
+            out[fd] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol, itree=int(levmax >= 0), levmax=max(levmax, 1),
+                                        ibctyp=ibctyp)
+    err, err0 = rel_l2(out[1], ref), rel_l2(out[0], ref)
+    say("  rel L2 vs oracle: spectral %.2e, from the field %.2e" % (err, err0))
+    assert err <= (TOL if prec == 0 else 2e-6)
+
+
+def test_pmesh_inf_spectral_face_derivs_full_size(vpmlib):
+    """The spectral normal derivatives at the 289-node block (work-array rows of 287 points padded
+    to 288; the compile-time sine-transform plans): in FP64 against the stencil on the solved
+    field at 1e-12; in the mixed form both against the FP64 result, where the spectral ones
+    (FP64 sums of the FP32 spectrum) must stay within the mixed form's 1e-5 and no further off
+    than the stencil on the FP32-solved field."""
+    V = vpmlib
+    nn = np.array([289, 289, 289], np.int32)
+    dpm = np.array([1 / 64] * 3)
+    xb = np.array([-2.25, -2.25, -2.25, 0, 0, 0])
+    rhs = _blob_rhs(nn, xb, dpm, centre=(0.1, -0.05, 0.02), s=0.3)
+    sol = np.zeros_like(rhs)
+    say("case: spectral face derivatives, 289^3")
+    out = {}
+    for prec in (0, 1):
+        for fd in (1, 0):
+            with V.VPM(V.Params.make(0.125)) as c:
+                c.set_option("face_derivs", fd)
+                c.set_precision(prec)
+                out[prec, fd] = c.stage_pmesh_inf(xb, dpm, nn, rhs, sol)
+    e64 = rel_l2(out[0, 1], out[0, 0])
+    e1, e0 = rel_l2(out[1, 1], out[0, 0]), rel_l2(out[1, 0], out[0, 0])
+    say("  FP64 spectral vs field %.2e; mixed vs FP64: spectral %.2e, field %.2e" % (e64, e1, e0))
+    assert e64 <= TOL
+    assert e1 <= 1e-5
+    assert e1 <= 1.05 * e0
+
+
 def test_vpm_mode2_call_sequence_kept_zeros(vpmlib, oracle):
     """One single-block context, several vpm(2) calls at the geometry of the first one
     (idefine = 1): ring + blob; the same again (the block RHS keeps its zeros between calls

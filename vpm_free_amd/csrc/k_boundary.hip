@@ -123,31 +123,48 @@ __global__ void k_normal_derivs(Grid g, const double *__restrict__ fd, FaceNodes
     for (int c = 0; c < ncomp; c++) a[c * nb + s] = -(sig[c * nb + s] * dsv) / PI4;
 }
 
-int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt, int64_t *nbound) {
-    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
-    if (NX < 6 || NY < 6 || NZ < 6) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
-    int64_t nb = 2 * ((int64_t)(NY - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NY - 1));
-    // the derivative at each face node once, then each cell's four-corner average
+static FaceNodes face_nodes(int ncomp, const int n[3]) {
     FaceNodes fo;
     fo.off[0] = 0;
     for (int f = 0; f < 6; f++) {
         int N0, N1;
-        face_dims(g.n, f, &N0, &N1);
+        face_dims(n, f, &N0, &N1);
         fo.off[f + 1] = fo.off[f] + (int64_t)ncomp * N0 * N1;
     }
-    int st = VPMX_OK;
-    double *fd = (double *)ensure(c, c->nd_face, sizeof(double) * (size_t)fo.off[6], &st);
-    if (st) return st;
-    const int64_t nn = fo.off[6] / ncomp;
-    hipLaunchKernelGGL(k_face_node_derivs, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, g, s0, ncomp,
-                       fo, fd);
-    VPMX_HIP(c, hipGetLastError());
+    return fo;
+}
+
+double *normal_derivs_table(Ctx *c, int ncomp, const int n[3], int *st) {
+    const FaceNodes fo = face_nodes(ncomp, n);
+    return (double *)ensure(c, c->nd_face, sizeof(double) * (size_t)fo.off[6], st);
+}
+
+int normal_derivs_cells(Ctx *c, int ncomp, const Grid &g, const double *fd, double *sig, double *cxyz, double *wgt,
+                        int64_t *nbound) {
+    const int NX = g.n[0], NY = g.n[1], NZ = g.n[2];
+    if (NX < 6 || NY < 6 || NZ < 6) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
+    const int64_t nb = 2 * ((int64_t)(NY - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NZ - 1) + (int64_t)(NX - 1) * (NY - 1));
+    const FaceNodes fo = face_nodes(ncomp, g.n);
     // wgt holds a (ncomp*nb) followed by ds (nb)
     hipLaunchKernelGGL(k_normal_derivs, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, c->stream, g, fd, fo, ncomp,
                        nb, sig, cxyz, wgt, wgt + (int64_t)ncomp * nb);
     VPMX_HIP(c, hipGetLastError());
     *nbound = nb;
     return VPMX_OK;
+}
+
+int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt, int64_t *nbound) {
+    if (g.n[0] < 6 || g.n[1] < 6 || g.n[2] < 6) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
+    // the derivative at each face node once, then each cell's four-corner average
+    const FaceNodes fo = face_nodes(ncomp, g.n);
+    int st = VPMX_OK;
+    double *fd = normal_derivs_table(c, ncomp, g.n, &st);
+    if (st) return st;
+    const int64_t nn = fo.off[6] / ncomp;
+    hipLaunchKernelGGL(k_face_node_derivs, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, g, s0, ncomp,
+                       fo, fd);
+    VPMX_HIP(c, hipGetLastError());
+    return normal_derivs_cells(c, ncomp, g, fd, sig, cxyz, wgt, nbound);
 }
 
 __device__ __forceinline__ double rsqrt_f64(double r2) {

@@ -1254,6 +1254,10 @@ void free_fft_plans(Ctx *c) {
     c->sin_cache.clear();
     for (auto &kv : c->sinm_cache) (void)hipFree(kv.second);
     c->sinm_cache.clear();
+    for (auto &kv : c->sinx_cache) (void)hipFree(kv.second);
+    c->sinx_cache.clear();
+    for (auto &kv : c->fdw_cache) (void)hipFree(kv.second);
+    c->fdw_cache.clear();
 }
 
 // Eigenvalues 4 sin^2(pi k / (2(m+1))) / h^2 of the 1-D Dirichlet second difference,
@@ -1905,8 +1909,240 @@ static int fold_correct(Ctx *c, const DstArgs &a, int ncomp, bool f32, const GVi
     return VPMX_OK;
 }
 
+// ---------------------------------------------------------------- normal derivatives from the spectrum
+// The zero-BC solve of pmesh is read only through the one-sided normal derivative
+// (1/h)(25/12 f0 - 4 f1 + 3 f2 - 4/3 f3 + 1/4 f4) at every face node (f_s the value s nodes
+// inside, f0 = 0 on the face; pinfdomain.f90:294-547).  The solution is a sine series in x and y
+// once the z pass has run (work array in (kx, ky, z)), so on an x face the derivative is
+// sum_kx W_x(kx) (series in y) with W(k) = (1/h)(-4 sin(pi k/(m+1)) + 3 sin(2 pi k/(m+1))
+// - 4/3 sin(3 pi k/(m+1)) + 1/4 sin(4 pi k/(m+1))) on the low face and (-1)^(k+1) W(k) on the
+// high one; likewise on the y faces, and on the z faces the stencil acts on the four z planes
+// next to each face.  One read of the work array gives the three face sums (FP64); products
+// with the sine matrices (FP64, matrix cores) take them to the face nodes.  This replaces the
+// y and x inverse passes of that solve.
+
+constexpr int FR_COLS = 5;  // kx columns per lane: rows of at most 320 points
+
+// x and y face sums of one (z plane, component): waves take rows ky = wave + 4 j (so a wave's
+// rows share their parity), R rows at a time with all their loads in flight; lanes take columns
+// kx = lane + 64 q (the parity of kx is the lane's, so the low and high x sums are the even-lane
+// sum plus / minus the odd-lane sum).  xp[b][k][ky], yp[b][k][kx] with b = component (low
+// face) or ncomp + component (high face) and k = z + 1; the columns k = 0 and mz + 1 (the nodes
+// on the z faces) are zero.
+template <typename T, int R>
+__global__ void __launch_bounds__(256) k_face_reduce_xy(const T *__restrict__ w, int64_t wcs, int px, int mx, int my,
+                                                        int mz, int ncomp, const double *__restrict__ wx,
+                                                        const double *__restrict__ wy, double *__restrict__ xp,
+                                                        double *__restrict__ yp) {
+    __shared__ double part[4][FR_COLS * 64];
+    const int z = blockIdx.x, comp = blockIdx.y;
+    const int NZ = mz + 2;
+    const T *__restrict__ pl = w + comp * wcs + (int64_t)z * px * my;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double wxl[FR_COLS], ys[FR_COLS];
+#pragma unroll
+    for (int q = 0; q < FR_COLS; q++) {
+        const int kx = lane + 64 * q;
+        wxl[q] = kx < mx ? wx[kx] : 0.0;
+        ys[q] = 0.0;
+    }
+    double *__restrict__ xlo = xp + ((int64_t)comp * NZ + z + 1) * my;
+    double *__restrict__ xhi = xp + ((int64_t)(ncomp + comp) * NZ + z + 1) * my;
+    for (int ky0 = wave; ky0 < my; ky0 += 4 * R) {
+        T v[R][FR_COLS];
+        // every load in bounds (indexes clamped), values outside the rows masked afterwards, so
+        // no load waits behind a branch
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ky = ky0 + 4 * r;
+            const T *__restrict__ row = pl + (int64_t)min(ky, my - 1) * px;
+#pragma unroll
+            for (int q = 0; q < FR_COLS; q++) v[r][q] = row[min(lane + 64 * q, mx - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < FR_COLS; q++)
+                if (ky0 + 4 * r >= my || lane + 64 * q >= mx) v[r][q] = (T)0;
+        double s[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ky = ky0 + 4 * r;
+            const double wyk = wy[min(ky, my - 1)];  // (the masked rows add zeros)
+            s[r] = 0.0;
+#pragma unroll
+            for (int q = 0; q < FR_COLS; q++) {
+                s[r] += wxl[q] * (double)v[r][q];
+                ys[q] += wyk * (double)v[r][q];
+            }
+        }
+#pragma unroll
+        for (int d = 2; d < 64; d <<= 1)
+#pragma unroll
+            for (int r = 0; r < R; r++) s[r] += __shfl_xor(s[r], d, 64);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int ky = ky0 + 4 * r;
+            const double o = __shfl_xor(s[r], 1, 64);
+            if (lane == 0 && ky < my) {
+                xlo[ky] = s[r] + o;
+                xhi[ky] = s[r] - o;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < FR_COLS; q++) part[wave][lane + 64 * q] = ys[q];
+    __syncthreads();
+    // waves 0 and 2 hold the even rows, 1 and 3 the odd ones
+    for (int kx = tid; kx < mx; kx += 256) {
+        const double e = part[0][kx] + part[2][kx], o = part[1][kx] + part[3][kx];
+        yp[((int64_t)comp * NZ + z + 1) * mx + kx] = e + o;
+        yp[((int64_t)(ncomp + comp) * NZ + z + 1) * mx + kx] = e - o;
+    }
+    for (int side = 0; side < 2; side++) {
+        if (z != (side ? mz - 1 : 0)) continue;
+        const int k = side ? NZ - 1 : 0;
+        for (int b = comp; b < 2 * ncomp; b += ncomp) {
+            for (int t = tid; t < my; t += 256) xp[((int64_t)b * NZ + k) * my + t] = 0.0;
+            for (int t = tid; t < mx; t += 256) yp[((int64_t)b * NZ + k) * mx + t] = 0.0;
+        }
+    }
+}
+
+// z face sums: the stencil on the four planes next to each z face, zp[b][ky][kx].
+template <typename T>
+__global__ void __launch_bounds__(256) k_face_reduce_z(const T *__restrict__ w, int64_t wcs, int px, int mx, int my,
+                                                       int mz, int ncomp, double ihz, double *__restrict__ zp) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int comp = blockIdx.y;
+    if (t >= (int64_t)mx * my) return;
+    const int kx = (int)(t % mx), ky = (int)(t / mx);
+    const T *__restrict__ p = w + comp * wcs + (int64_t)ky * px + kx;
+    const int64_t pz = (int64_t)px * my;
+    const double a2 = -4.0, a3 = 3.0, a4 = -4.0 / 3.0, a5 = 0.25;
+    for (int side = 0; side < 2; side++) {
+        const int z0 = side ? mz - 1 : 0, dz = side ? -1 : 1;
+        const double v1 = (double)p[z0 * pz], v2 = (double)p[(z0 + dz) * pz], v3 = (double)p[(z0 + 2 * dz) * pz],
+                     v4 = (double)p[(z0 + 3 * dz) * pz];
+        zp[((int64_t)(side * ncomp + comp) * my + ky) * mx + kx] = ihz * (((a2 * v1 + a3 * v2) + a4 * v3) + a5 * v4);
+    }
+}
+
+// W(k), k = 1..m (index k - 1), uploaded once per (m, h).
+static int face_weights(Ctx *c, int m, double h, const double **out) {
+    auto key = std::make_pair(m, h);
+    auto it = c->fdw_cache.find(key);
+    if (it != c->fdw_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    const long double PI = 3.141592653589793238462643383279502884L;
+    const long double a[4] = {-4.0L, 3.0L, -4.0L / 3.0L, 0.25L};
+    std::vector<double> v(m);
+    for (int k = 1; k <= m; k++) {
+        long double s = 0.0L;
+        for (int q = 1; q <= 4; q++) {
+            const long long r = ((long long)k * q) % (2LL * (m + 1));
+            s += a[q - 1] * sinl(PI * (long double)r / (long double)(m + 1));
+        }
+        v[k - 1] = (double)(s / (long double)h);
+    }
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * m));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    c->fdw_cache[key] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
+}
+
+// The DST-I matrix with a zero row above and below, sin(pi r (k+1) / (m+1)) for r = 0..m+1,
+// k < m, column-major (m+2) x m (row r = node r of the line), uploaded once per m.
+static int sine_matrix_ext(Ctx *c, int m, const double **out) {
+    auto it = c->sinx_cache.find(m);
+    if (it != c->sinx_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    const int N = m + 2;
+    std::vector<double> v((size_t)N * m);
+    const long double PI = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < m; k++)
+        for (int r = 0; r < N; r++) {
+            const long long q = ((long long)r * (k + 1)) % (2LL * (m + 1));
+            v[(size_t)k * N + r] = q % (m + 1) == 0 ? 0.0 : (double)sinl(PI * (long double)q / (long double)(m + 1));
+        }
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * v.size()));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    c->sinx_cache[m] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
+}
+
+// The face-node table fd (faces XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX; each ncomp x N0 x N1 with
+// (N0, N1) = (NY, NZ), (NX, NZ), (NX, NY), the first index fastest) from the work array after
+// the z pass (a.w, FP32 when f32).  Every entry is written (zero on the face edges).
+static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, const double h[3], double *fd) {
+    int st = VPMX_OK;
+    const int mx = a.mx, my = a.my, mz = a.mz, np = 2 * ncomp;
+    const int NX = mx + 2, NY = my + 2, NZ = mz + 2;
+    if (mx > 64 * FR_COLS) return fail(c, VPMX_ERR_UNSUP, "spectral normal derivatives: rows of at most 320 points");
+    if (mx < 4 || my < 4 || mz < 4) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
+    const double *wx = nullptr, *wy = nullptr, *SX = nullptr, *SY = nullptr;
+    VPMX_TRY(face_weights(c, mx, h[0], &wx));
+    VPMX_TRY(face_weights(c, my, h[1], &wy));
+    VPMX_TRY(sine_matrix_ext(c, mx, &SX));
+    VPMX_TRY(sine_matrix_ext(c, my, &SY));
+    const int64_t nxp = (int64_t)np * NZ * my, nyp = (int64_t)np * NZ * mx, nzp = (int64_t)np * my * mx,
+                  nv = (int64_t)np * my * NX;
+    double *xp = (double *)ensure(c, c->fd_spec, sizeof(double) * (size_t)(nxp + nyp + nzp + nv), &st);
+    if (st) return st;
+    double *yp = xp + nxp, *zp = yp + nyp, *v = zp + nzp;
+    const dim3 gz((unsigned)(((int64_t)mx * my + 255) / 256), (unsigned)ncomp);
+    // rows in flight per wave: two candidates per precision, same bits (the row order of every
+    // sum is fixed)
+    const void *fns[2];
+    if (f32) {
+        fns[0] = (const void *)k_face_reduce_xy<float, 4>;
+        fns[1] = (const void *)k_face_reduce_xy<float, 8>;
+    } else {
+        fns[0] = (const void *)k_face_reduce_xy<double, 2>;
+        fns[1] = (const void *)k_face_reduce_xy<double, 4>;
+    }
+    const char *names[2] = {f32 ? "4 rows in flight" : "2 rows in flight", f32 ? "8 rows in flight" : "4 rows in flight"};
+    const void *wp = a.w;
+    int64_t wcs = a.wcs;
+    int px = a.px, ncv = ncomp;
+    auto trial = [&](int var, int) -> int {
+        void *kargs[] = {(void *)&wp, &wcs, &px, (void *)&mx, (void *)&my, (void *)&mz, &ncv, &wx, &wy, &xp, &yp};
+        VPMX_HIP(c, hipLaunchKernel(fns[var], dim3((unsigned)mz, (unsigned)ncomp), dim3(256), kargs, 0, c->stream));
+        return VPMX_OK;
+    };
+    char key[128];
+    std::snprintf(key, sizeof key, "face sums %s m=%d,%d,%d", f32 ? "FP32" : "FP64", mx, my, mz);
+    Ctx::Tuned ch;
+    VPMX_TRY(tune(c, key, 2, fns, names, 0, 256, trial, &ch, false));
+    VPMX_TRY(trial(ch.fi, 0));
+    if (f32)
+        hipLaunchKernelGGL(k_face_reduce_z<float>, gz, dim3(256), 0, c->stream, (const float *)a.w, a.wcs, a.px, mx, my,
+                           mz, ncomp, 1.0 / h[2], zp);
+    else
+        hipLaunchKernelGGL(k_face_reduce_z<double>, gz, dim3(256), 0, c->stream, (const double *)a.w, a.wcs, a.px, mx,
+                           my, mz, ncomp, 1.0 / h[2], zp);
+    VPMX_HIP(c, hipGetLastError());
+    const int64_t oy = (int64_t)np * NY * NZ, oz = oy + (int64_t)np * NX * NZ;
+    // x faces: fd[b][k][j] = sum_ky S_y(j, ky) xp[b][k][ky]; y faces likewise with S_x
+    VPMX_TRY(dgemm(c, false, false, NY, np * NZ, my, SY, NY, 0, xp, my, 0, fd, NY, 0, 1));
+    VPMX_TRY(dgemm(c, false, false, NX, np * NZ, mx, SX, NX, 0, yp, mx, 0, fd + oy, NX, 0, 1));
+    // z faces: v[b][ky][i] = sum_kx S_x(i, kx) zp[b][ky][kx], fd[b][j][i] = sum_ky v[b][ky][i] S_y(j, ky)
+    VPMX_TRY(dgemm(c, false, false, NX, np * my, mx, SX, NX, 0, zp, mx, 0, v, NX, 0, 1));
+    VPMX_TRY(dgemm(c, false, true, NX, NY, my, v, NX, (int64_t)NX * my, SY, NY, 0, fd + oz, NX, (int64_t)NX * NY, np));
+    return VPMX_OK;
+}
+
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out,
-                      int reuse, int form) {
+                      int reuse, int form, double *face_nd) {
     int st = VPMX_OK;
     int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
     if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(c, VPMX_ERR_ARG, "Poisson window needs >= 3 nodes per axis");
@@ -1989,6 +2225,10 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     } else {
         VPMX_TRY(run_pass<DST_Z>(c, a, ncomp, ax[2], f32));
         a.w = a.wo;
+    }
+    if (face_nd) {
+        if (inv64 || bc) return fail(c, VPMX_ERR_ARG, "Poisson: face derivatives of a zero-BC solve only");
+        return face_derivs_spectral(c, a, ncomp, f32, h, face_nd);
     }
     const bool fi = f32 && !inv64;
     // y inverse

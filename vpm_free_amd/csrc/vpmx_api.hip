@@ -99,6 +99,7 @@ static void collect_times(Ctx *, const bool *) {}
 // SOL(faces) += boundary sums (ibctyp 2 point sources, 3 panels; tree form with levmax
 // levels when levmax >= 0, pmlib.f90:69 ibctyp/itree/levmax), then the Dirichlet solve of
 // the whole grid.
+constexpr int64_t FACE_SPECTRAL_MIN = 96 * 96 * 96;  // grid nodes from which the spectral derivatives pay
 static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax, int ibctyp) {
     int st = VPMX_OK;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
@@ -110,13 +111,23 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     double h[3];
     window_h(g.x0, g.h, win, h);
     const int ru = c->dst_reuse ? 1 : 0;
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, ru));  // keeps its forward transforms
     int64_t nbmax = 2 * ((int64_t)n[0] * n[1] + (int64_t)n[0] * n[2] + (int64_t)n[1] * n[2]);
     double *sig = (double *)ensure(c, c->src_sig, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
     double *cxyz = (double *)ensure(c, c->src_cxyz, sizeof(double) * 3 * (size_t)nbmax, &st); if (st) return st;
     double *wgt = (double *)ensure(c, c->src_w, sizeof(double) * 4 * (size_t)nbmax, &st); if (st) return st;
     int64_t nb = 0;
-    VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
+    // SOL_0 is read only through its normal derivatives at the faces: on large grids they come
+    // straight from its spectrum (no inverse y and x passes), else from the solved field
+    const bool spectral = c->face_derivs == 1 || (c->face_derivs < 0 && G >= FACE_SPECTRAL_MIN && n[0] <= 322);
+    if (spectral) {
+        double *fd = normal_derivs_table(c, 3, n, &st);
+        if (st) return st;
+        VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, ru, SOLVE_CTX, fd));  // keeps its forward transforms
+        VPMX_TRY(normal_derivs_cells(c, 3, g, fd, sig, cxyz, wgt, &nb));
+    } else {
+        VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, nullptr, s0v, ru));  // keeps its forward transforms
+        VPMX_TRY(normal_derivs(c, 3, g, s0v, sig, cxyz, wgt, &nb));
+    }
     // SOL_0 is dead once the sources exist: scratch of the panel sums
     VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax, ibctyp, &s0v));
     VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2 * ru));  // same RHS: reuses them
@@ -1110,7 +1121,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->tune_scratch, &c->adj_a, &c->adj_b, &c->adj_c, &c->adj_d, &c->adj_e, &c->adj_f,
                     &c->adj_g, &c->adj_h, &c->adj_i, &c->adj_p, &c->adj_q, &c->adj_u, &c->adj_s, &c->adj_r,
                     &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm, &c->adj_pu,
-                    &c->adj_ps, &c->nd_face};
+                    &c->adj_ps, &c->nd_face, &c->fd_spec};
     for (DBuf *b : bufs) release(*b);
     for (auto &r : c->adj_steps)
         for (DBuf &b : r.b) release(b);
@@ -1180,6 +1191,9 @@ int32_t vpmx_set_option(vpmx_ctx *p, const char *name, int32_t value) {
         c->adj_m2p_scatter = value != 0;
     } else if (k == "dst_reuse") {
         c->dst_reuse = value != 0;
+    } else if (k == "face_derivs") {
+        if (value < -1 || value > 1) return fail(c, VPMX_ERR_ARG, "face_derivs: -1 by grid size, 0 solved field, 1 spectral");
+        c->face_derivs = value;
     } else if (k == "tuning") {
         c->tuning = value != 0;
         c->tuned.clear();

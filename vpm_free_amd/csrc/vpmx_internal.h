@@ -119,8 +119,10 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
 // mixed form), SOLVE_FP64, SOLVE_FP32, SOLVE_INV64 (forward passes and diagonal in FP32, the
 // inverse passes in FP64: the accuracy of an FP64 solve where second differences are taken).
 enum SolveForm { SOLVE_CTX = -1, SOLVE_FP64 = 0, SOLVE_FP32 = 1, SOLVE_INV64 = 2 };
+// face_nd (zero BCs only): instead of the solution, the one-sided normal derivative at every
+// node of every face into that table (the layout of normal_derivs_table), from the spectrum.
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
-                      GView out, int reuse = 0, int form = SOLVE_CTX);
+                      GView out, int reuse = 0, int form = SOLVE_CTX, double *face_nd = nullptr);
 
 // Complex FFT plan of length L (Stockham radices, device twiddle table exp(-2 pi i t / L)),
 // cached per context.
@@ -133,6 +135,13 @@ int fft_shape(Ctx *c, int L, FftShape *out);
 // Boundary (k_boundary.hip)
 int normal_derivs(Ctx *c, int ncomp, const Grid &g, GView s0, double *sig, double *cxyz, double *wgt,
                   int64_t *nbound);
+// The same in two steps: the face-node table (ncomp values per node of each face, faces XMIN,
+// XMAX, YMIN, YMAX, ZMIN, ZMAX, each ncomp x N0 x N1 with (N0, N1) = (NY, NZ), (NX, NZ),
+// (NX, NY), first index fastest; the context keeps its buffer), filled by the caller, then the
+// per-cell sources from it.
+double *normal_derivs_table(Ctx *c, int ncomp, const int n[3], int *st);
+int normal_derivs_cells(Ctx *c, int ncomp, const Grid &g, const double *fd, double *sig, double *cxyz, double *wgt,
+                        int64_t *nbound);
 // levmax >= 0: the tree form with that many levels (pm.inp tree = 1), else the direct sums;
 // ibctyp 2 point sources, 3 constant-source panels (needs `scratch`, a grid of sol's shape
 // whose boundary nodes it may overwrite, for the direct form).
