@@ -111,7 +111,10 @@ int32_t vpmx_set_precision(vpmx_ctx *ctx, int32_t mode);
  * "dst_reuse" -- 1 (default): the second fine solve of pmesh starts from the first one's
  * forward transforms, 0: both solves in full; "face_derivs" -- the normal derivatives of the
  * zero-BC solve of pmesh (calc_normalderiv_3d): -1 (default) by grid size, 1 from its sine
- * spectrum after the z pass (no inverse y and x passes), 0 from the solved field; "tuning" -- 1 (default): the autotuner picks
+ * spectrum after the z pass (no inverse y and x passes), 0 from the solved field;
+ * "fine_partial" -- the second fine solve of a single yaps block: -1 (default) by grid size, 1 only
+ * the nodes that yaps reads (the inner box's faces and the coarse samples, from the spectrum after
+ * the z pass; the block's other nodes are left unset), 0 every node; "tuning" -- 1 (default): the autotuner picks
  * kernel variants and persistent grid sizes by timing them once per launch key, 0: fixed
  * defaults (clears the choices made so far); "adjoint_boundsum" -- the adjoint's transposed
  * boundary sums by 2 spectral correlation on the forward's plan (default), 1 direct sums;

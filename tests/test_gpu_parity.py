@@ -132,6 +132,43 @@ def test_vpm_mode2_ring(vpmlib, oracle, h, sigma):
         assert v <= TOL, (k, v)
 
 
+@pytest.mark.parametrize("h,sigma,tree", [(0.125, 0.2, 0), (1 / 16, 0.1, 0), (1 / 16, 0.1, 1)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vpm_mode2_spectral_shortcuts(vpmlib, oracle, h, sigma, tree, prec):
+    """vpm(2) with both spectral shortcuts of the fine solves forced on small blocks (they are
+    taken by default from 96^3 nodes): the zero-BC solve's normal derivatives from its spectrum
+    (face_derivs = 1), and the second solve evaluated only on the inner box's faces and at the
+    coarse samples (fine_partial = 1) -- the nodes yaps reads.  Every grid, UP and GP against the
+    oracle at 1e-12 in FP64, at the mixed form's 1e-5 in the mixed form."""
+    V = vpmlib
+    xp, qp = _ring_case(h, sigma)
+    say("case: vpm mode 2 spectral shortcuts h=%g tree=%d precision %d n=%d" % (h, tree, prec, xp.shape[0]))
+    po = oracle.Params.make(h, iyntree=tree, ilevmax=3)
+    go = oracle.define_sizes(po, xp)
+    up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
+    out = {}
+    for on in (1, 0):
+        with V.VPM(V.Params.make(h, iyntree=tree, ilevmax=3)) as c:
+            c.set_option("face_derivs", on)
+            c.set_option("fine_partial", on)
+            c.set_precision(prec)
+            up, gp = np.zeros_like(xp), np.zeros_like(xp)
+            c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
+            c.vpm(xp, qp, up, gp, xp.shape[0], 3, 2, 1)
+            out[on] = {"psi": c.get_grid(1), "vel": c.get_grid(3), "sol": c.get_grid(2), "up": up.copy(),
+                       "gp": gp.copy()}
+    ref = {"psi": f["psi"], "vel": f["vel"], "sol": f["sol"], "up": up_r, "gp": gp_r}
+    errs = {k: rel_l2(out[1][k], ref[k]) for k in ref}
+    say("  rel L2 vs oracle:", errs)
+    if prec == 0:
+        for k, v in errs.items():
+            assert v <= TOL, (k, v)
+    else:
+        say("  (without them:", {k: rel_l2(out[0][k], ref[k]) for k in ref}, ")")
+        for k, v in errs.items():
+            assert v <= 1e-5, (k, v)
+
+
 def test_remesh_exact(vpmlib, oracle):
     V = vpmlib
     h = 0.125

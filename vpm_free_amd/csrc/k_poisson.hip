@@ -1258,6 +1258,10 @@ void free_fft_plans(Ctx *c) {
     c->sinx_cache.clear();
     for (auto &kv : c->fdw_cache) (void)hipFree(kv.second);
     c->fdw_cache.clear();
+    for (auto &kv : c->sinr_cache) (void)hipFree(kv.second);
+    c->sinr_cache.clear();
+    for (auto &kv : c->sinl_cache) (void)hipFree(kv.second);
+    c->sinl_cache.clear();
 }
 
 // Eigenvalues 4 sin^2(pi k / (2(m+1))) / h^2 of the 1-D Dirichlet second difference,
@@ -2141,8 +2145,235 @@ static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, c
     return VPMX_OK;
 }
 
+// ---------------------------------------------------------------- selected values of a solve
+// The second fine solve of a single yaps block is read only on the inner box's faces (the TSC
+// correction adds to them, the final solve takes them as Dirichlet values) and at the coarse
+// sampling nodes (every ncoarse-th node); the final solve overwrites the rest of the inner box
+// and nothing reads the margin (yaps3d.f90:104-138,254-261,340-395).  From the work array after
+// the z pass (kx, ky, z): the x-face values are sum_kx sin(pi kx p / (mx+1)) (series in y) --
+// face sums with sine weights, whose high-face weights are (-1)^(kx+1) times the low ones when
+// the margins are equal -- then S_y; the y faces likewise; the z faces and the sample planes are
+// planes of the work array, taken through S_x and S_y (full rows for the faces, the sample rows
+// for the samples).  FP64 products on the matrix cores replace the y and x inverse passes.
+
+// sin(pi p k / (m+1)), k = 1..m (index k - 1), uploaded once per (m, p).
+static int sine_row(Ctx *c, int m, int p, const double **out) {
+    auto key = std::make_pair(m, p);
+    auto it = c->sinr_cache.find(key);
+    if (it != c->sinr_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    const long double PI = 3.141592653589793238462643383279502884L;
+    std::vector<double> v(m);
+    for (int k = 1; k <= m; k++) {
+        const long long q = ((long long)p * k) % (2LL * (m + 1));
+        v[k - 1] = q % (m + 1) == 0 ? 0.0 : (double)sinl(PI * (long double)q / (long double)(m + 1));
+    }
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * m));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * m, hipMemcpyHostToDevice));
+    c->sinr_cache[key] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
+}
+
+// Rows p = a s (a = 1..m/s) of the DST-I matrix, sin(pi p (k+1) / (m+1)), column-major
+// (m/s) x m, uploaded once per (m, s).
+static int sine_rows(Ctx *c, int m, int step, const double **out) {
+    auto key = std::make_pair(m, step);
+    auto it = c->sinl_cache.find(key);
+    if (it != c->sinl_cache.end()) {
+        *out = (const double *)it->second;
+        return VPMX_OK;
+    }
+    const int nr = m / step;
+    const long double PI = 3.141592653589793238462643383279502884L;
+    std::vector<double> v((size_t)nr * m);
+    for (int k = 0; k < m; k++)
+        for (int r = 0; r < nr; r++) {
+            const long long q = ((long long)(r + 1) * step * (k + 1)) % (2LL * (m + 1));
+            v[(size_t)k * nr + r] = q % (m + 1) == 0 ? 0.0 : (double)sinl(PI * (long double)q / (long double)(m + 1));
+        }
+    void *d = nullptr;
+    VPMX_HIP(c, hipMalloc(&d, sizeof(double) * v.size()));
+    VPMX_HIP(c, hipMemcpy(d, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+    c->sinl_cache[key] = d;
+    *out = (const double *)d;
+    return VPMX_OK;
+}
+
+// Planes z = z0 + t dz (t < np) of the work array, FP64: out[t][comp][ky][kx].
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_planes(const T *__restrict__ w, int64_t wcs, int px, int mx, int my,
+                                                       int ncomp, int z0, int dz, double *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t pl = (int64_t)mx * my;
+    if (e >= pl) return;
+    const int b = blockIdx.y, t = b / ncomp, comp = b % ncomp;
+    const int kx = (int)(e % mx), ky = (int)(e / mx);
+    out[(int64_t)b * pl + e] = (double)w[comp * wcs + (int64_t)(z0 + t * dz) * px * my + (int64_t)ky * px + kx];
+}
+
+struct PartialTabs {
+    const double *tx, *ty, *tz, *ps;  // x / y / z face values, sample values
+    int xs[2], ys[2], zs[2];           // face nodes (1-based)
+    int step, nlx, nly, nlz;           // sample step and interior sample counts
+};
+
+// Writes the selected nodes of `out` once each: the x faces, the y faces off the x faces, the
+// z faces off both, the interior samples off every face.
+__global__ void __launch_bounds__(256) k_partial_scatter(PartialTabs t, int nx, int ny, int nz, int ncomp, GView out) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int mx = nx - 2, my = ny - 2, mz = nz - 2;
+    const int64_t nxf = (int64_t)2 * ncomp * my * mz, nyf = (int64_t)2 * ncomp * mx * mz, nzf = (int64_t)2 * ncomp * mx * my;
+    const int64_t nsm = (int64_t)ncomp * t.nlx * t.nly * t.nlz;
+    auto onx = [&](int i) { return i == t.xs[0] || i == t.xs[1]; };
+    auto ony = [&](int j) { return j == t.ys[0] || j == t.ys[1]; };
+    auto onz = [&](int k) { return k == t.zs[0] || k == t.zs[1]; };
+    int i, j, k, comp;
+    double v;
+    if (e < nxf) {
+        j = 2 + (int)(e % my); e /= my;
+        k = 2 + (int)(e % mz); e /= mz;
+        const int b = (int)e, side = b / ncomp;
+        comp = b % ncomp;
+        i = t.xs[side];
+        v = t.tx[((int64_t)b * nz + (k - 1)) * ny + (j - 1)];
+    } else if ((e -= nxf) < nyf) {
+        i = 2 + (int)(e % mx); e /= mx;
+        k = 2 + (int)(e % mz); e /= mz;
+        const int b = (int)e, side = b / ncomp;
+        comp = b % ncomp;
+        j = t.ys[side];
+        if (onx(i)) return;
+        v = t.ty[((int64_t)b * nz + (k - 1)) * nx + (i - 1)];
+    } else if ((e -= nyf) < nzf) {
+        i = 2 + (int)(e % mx); e /= mx;
+        j = 2 + (int)(e % my); e /= my;
+        const int b = (int)e, side = b / ncomp;
+        comp = b % ncomp;
+        k = t.zs[side];
+        if (onx(i) || ony(j)) return;
+        v = t.tz[((int64_t)b * ny + (j - 1)) * nx + (i - 1)];
+    } else if ((e -= nzf) < nsm) {
+        const int ax = (int)(e % t.nlx); e /= t.nlx;
+        const int ay = (int)(e % t.nly); e /= t.nly;
+        const int b = (int)e;  // plane * ncomp + comp
+        const int az = b / ncomp;
+        comp = b % ncomp;
+        i = 1 + (ax + 1) * t.step;
+        j = 1 + (ay + 1) * t.step;
+        k = 1 + (az + 1) * t.step;
+        if (onx(i) || ony(j) || onz(k)) return;
+        v = t.ps[((int64_t)b * t.nly + ay) * t.nlx + ax];
+    } else {
+        return;
+    }
+    out.p[comp * out.cs + out.at(i - 1, j - 1, k - 1)] = v;
+}
+
+static int partial_values(Ctx *c, const DstArgs &a, int ncomp, bool f32, const SolvePartial &sp, GView out) {
+    int st = VPMX_OK;
+    const int mx = a.mx, my = a.my, mz = a.mz, np = 2 * ncomp;
+    const int NX = mx + 2, NY = my + 2, NZ = mz + 2;
+    const int step = sp.step;
+    if (mx > 64 * FR_COLS) return fail(c, VPMX_ERR_UNSUP, "selected solve values: rows of at most 320 points");
+    for (int d = 0; d < 3; d++) {
+        const int n = d == 0 ? NX : (d == 1 ? NY : NZ);
+        const int *f = d == 0 ? sp.xs : (d == 1 ? sp.ys : sp.zs);
+        if (f[0] < 2 || f[1] > n - 1 || f[0] >= f[1]) return fail(c, VPMX_ERR_ARG, "selected solve values: faces inside the grid");
+        if (d < 2 && f[1] != n + 1 - f[0]) return fail(c, VPMX_ERR_ARG, "selected solve values: equal margins in x and y");
+    }
+    if (step < 1) return fail(c, VPMX_ERR_ARG, "selected solve values: sample step");
+    const int nlx = mx / step, nly = my / step, nlz = mz / step;
+    const double *wx = nullptr, *wy = nullptr, *SX = nullptr, *SY = nullptr, *LX = nullptr, *LY = nullptr;
+    VPMX_TRY(sine_row(c, mx, sp.xs[0] - 1, &wx));
+    VPMX_TRY(sine_row(c, my, sp.ys[0] - 1, &wy));
+    VPMX_TRY(sine_matrix_ext(c, mx, &SX));
+    VPMX_TRY(sine_matrix_ext(c, my, &SY));
+    if (nlx > 0 && nly > 0 && nlz > 0) {
+        VPMX_TRY(sine_rows(c, mx, step, &LX));
+        VPMX_TRY(sine_rows(c, my, step, &LY));
+    }
+    // face sums, gathered planes, and the products' outputs
+    const int64_t nxp = (int64_t)np * NZ * my, nyp = (int64_t)np * NZ * mx, nzc = (int64_t)np * my * mx;
+    const int64_t nsc = (int64_t)ncomp * nlz * my * mx, ntx = (int64_t)np * NZ * NY, nty = (int64_t)np * NZ * NX;
+    const int64_t nv = (int64_t)np * my * NX, ntz = (int64_t)np * NY * NX, nt5 = (int64_t)ncomp * nlz * my * nlx;
+    const int64_t nps = (int64_t)ncomp * nlz * nly * nlx;
+    double *xp = (double *)ensure(c, c->fp_spec,
+                                  sizeof(double) * (size_t)(nxp + nyp + nzc + nsc + ntx + nty + nv + ntz + nt5 + nps), &st);
+    if (st) return st;
+    double *yp = xp + nxp, *zc = yp + nyp, *sc = zc + nzc, *tx = sc + nsc, *ty = tx + ntx, *v = ty + nty, *tz = v + nv;
+    double *t5 = tz + ntz, *ps = t5 + nt5;
+    // x and y face sums (the kernel of the normal derivatives, with sine weights; its tuning key)
+    {
+        const void *fns[2];
+        if (f32) {
+            fns[0] = (const void *)k_face_reduce_xy<float, 4>;
+            fns[1] = (const void *)k_face_reduce_xy<float, 8>;
+        } else {
+            fns[0] = (const void *)k_face_reduce_xy<double, 2>;
+            fns[1] = (const void *)k_face_reduce_xy<double, 4>;
+        }
+        const char *names[2] = {f32 ? "4 rows in flight" : "2 rows in flight", f32 ? "8 rows in flight" : "4 rows in flight"};
+        const void *wp = a.w;
+        int64_t wcs = a.wcs;
+        int px = a.px, ncv = ncomp, mxv = mx, myv = my, mzv = mz;
+        auto trial = [&](int var, int) -> int {
+            void *kargs[] = {(void *)&wp, &wcs, &px, &mxv, &myv, &mzv, &ncv, &wx, &wy, &xp, &yp};
+            VPMX_HIP(c, hipLaunchKernel(fns[var], dim3((unsigned)mz, (unsigned)ncomp), dim3(256), kargs, 0, c->stream));
+            return VPMX_OK;
+        };
+        char key[128];
+        std::snprintf(key, sizeof key, "face sums %s m=%d,%d,%d", f32 ? "FP32" : "FP64", mx, my, mz);
+        Ctx::Tuned ch;
+        VPMX_TRY(tune(c, key, 2, fns, names, 0, 256, trial, &ch, false));
+        VPMX_TRY(trial(ch.fi, 0));
+    }
+    // the z-face planes and the interior sample planes
+    const int64_t pl = (int64_t)mx * my;
+    const unsigned gpl = (unsigned)((pl + 255) / 256);
+    if (f32) {
+        hipLaunchKernelGGL(k_gather_planes<float>, dim3(gpl, (unsigned)np), dim3(256), 0, c->stream, (const float *)a.w,
+                           a.wcs, a.px, mx, my, ncomp, sp.zs[0] - 2, sp.zs[1] - sp.zs[0], zc);
+        if (nlz > 0 && nlx > 0 && nly > 0)
+            hipLaunchKernelGGL(k_gather_planes<float>, dim3(gpl, (unsigned)(ncomp * nlz)), dim3(256), 0, c->stream,
+                               (const float *)a.w, a.wcs, a.px, mx, my, ncomp, step - 1, step, sc);
+    } else {
+        hipLaunchKernelGGL(k_gather_planes<double>, dim3(gpl, (unsigned)np), dim3(256), 0, c->stream, (const double *)a.w,
+                           a.wcs, a.px, mx, my, ncomp, sp.zs[0] - 2, sp.zs[1] - sp.zs[0], zc);
+        if (nlz > 0 && nlx > 0 && nly > 0)
+            hipLaunchKernelGGL(k_gather_planes<double>, dim3(gpl, (unsigned)(ncomp * nlz)), dim3(256), 0, c->stream,
+                               (const double *)a.w, a.wcs, a.px, mx, my, ncomp, step - 1, step, sc);
+    }
+    VPMX_HIP(c, hipGetLastError());
+    // x faces tx[b][k][j] = sum_ky S_y(j, ky) xp[b][k][ky]; y faces ty[b][k][i]
+    VPMX_TRY(dgemm(c, false, false, NY, np * NZ, my, SY, NY, 0, xp, my, 0, tx, NY, 0, 1));
+    VPMX_TRY(dgemm(c, false, false, NX, np * NZ, mx, SX, NX, 0, yp, mx, 0, ty, NX, 0, 1));
+    // z faces: v[b][ky][i] = sum_kx S_x(i, kx) zc[b][ky][kx]; tz[b][j][i] = sum_ky v[b][ky][i] S_y(j, ky)
+    VPMX_TRY(dgemm(c, false, false, NX, np * my, mx, SX, NX, 0, zc, mx, 0, v, NX, 0, 1));
+    VPMX_TRY(dgemm(c, false, true, NX, NY, my, v, NX, (int64_t)NX * my, SY, NY, 0, tz, NX, (int64_t)NX * NY, np));
+    // samples: t5[b][ky][a] = sum_kx L_x(a, kx) sc[b][ky][kx]; ps[b][ay][ax] = sum_ky t5[b][ky][ax] L_y(ay, ky)
+    if (nlx > 0 && nly > 0 && nlz > 0) {
+        VPMX_TRY(dgemm(c, false, false, nlx, ncomp * nlz * my, mx, LX, nlx, 0, sc, mx, 0, t5, nlx, 0, 1));
+        VPMX_TRY(dgemm(c, false, true, nlx, nly, my, t5, nlx, (int64_t)nlx * my, LY, nly, 0, ps, nlx,
+                       (int64_t)nlx * nly, ncomp * nlz));
+    }
+    PartialTabs t;
+    t.tx = tx; t.ty = ty; t.tz = tz; t.ps = ps;
+    for (int q = 0; q < 2; q++) { t.xs[q] = sp.xs[q]; t.ys[q] = sp.ys[q]; t.zs[q] = sp.zs[q]; }
+    t.step = step;
+    t.nlx = nlx; t.nly = nly; t.nlz = nlz;
+    const int64_t tot = (int64_t)np * (my * (int64_t)mz + mx * (int64_t)mz + mx * (int64_t)my) + nps;
+    hipLaunchKernelGGL(k_partial_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, t, NX, NY, NZ,
+                       ncomp, out);
+    VPMX_HIP(c, hipGetLastError());
+    return VPMX_OK;
+}
+
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc, GView out,
-                      int reuse, int form, double *face_nd) {
+                      int reuse, int form, double *face_nd, const SolvePartial *part) {
     int st = VPMX_OK;
     int m[3] = {wn[0] - 2, wn[1] - 2, wn[2] - 2};
     if (m[0] < 1 || m[1] < 1 || m[2] < 1) return fail(c, VPMX_ERR_ARG, "Poisson window needs >= 3 nodes per axis");
@@ -2229,6 +2460,10 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     if (face_nd) {
         if (inv64 || bc) return fail(c, VPMX_ERR_ARG, "Poisson: face derivatives of a zero-BC solve only");
         return face_derivs_spectral(c, a, ncomp, f32, h, face_nd);
+    }
+    if (part) {
+        if (inv64) return fail(c, VPMX_ERR_ARG, "Poisson: selected values of the FP32/FP64 passes only");
+        return partial_values(c, a, ncomp, f32, *part, out);
     }
     const bool fi = f32 && !inv64;
     // y inverse

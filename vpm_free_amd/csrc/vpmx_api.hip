@@ -100,7 +100,8 @@ static void collect_times(Ctx *, const bool *) {}
 // levels when levmax >= 0, pmlib.f90:69 ibctyp/itree/levmax), then the Dirichlet solve of
 // the whole grid.
 constexpr int64_t FACE_SPECTRAL_MIN = 96 * 96 * 96;  // grid nodes from which the spectral derivatives pay
-static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax, int ibctyp) {
+static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, bool fine, int levmax, int ibctyp,
+                     const SolvePartial *part = nullptr) {
     int st = VPMX_OK;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     int64_t G = (int64_t)n[0] * n[1] * n[2];
@@ -130,7 +131,8 @@ static int pmesh_inf(Ctx *c, const Grid &g, GView rhs, GView sol, DBuf &s0buf, b
     }
     // SOL_0 is dead once the sources exist: scratch of the panel sums
     VPMX_TRY(boundary_sums(c, 3, g, nb, sig, cxyz, wgt, sol, fine, levmax, ibctyp, &s0v));
-    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2 * ru));  // same RHS: reuses them
+    // same RHS: reuses them (part: only the nodes the caller reads)
+    VPMX_TRY(poisson_dirichlet(c, 3, n, h, rhs, &sol, sol, 2 * ru, SOLVE_CTX, nullptr, part));
     return VPMX_OK;
 }
 
@@ -178,7 +180,23 @@ static int yaps_single(Ctx *c, bool *used) {
     const bool tree = P.iyntree == 1;
     if (tree && G.ilevmax < 1)
         return fail(c, VPMX_ERR_UNSUP, "tree = 1 needs at least one level on the coarse grid (ilevmax >= 1)");
-    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true, tree ? G.ilevmax : -1, P.ibctyp));
+    // the block solution is read only on the inner box's faces and at the coarse samples (the
+    // final solve overwrites the rest of the inner box, and the margin is never read): on large
+    // blocks with equal x and y margins only those nodes are evaluated
+    const int32_t *wi = G.nnbl_bl;
+    SolvePartial part;
+    for (int q = 0; q < 2; q++) {
+        part.xs[q] = wi[3 * q];
+        part.ys[q] = wi[3 * q + 1];
+        part.zs[q] = wi[3 * q + 2];
+    }
+    part.step = P.ncoarse;
+    const bool part_ok = part.xs[0] >= 2 && part.ys[0] >= 2 && part.zs[0] >= 2 && part.xs[1] <= nb[0] - 1 &&
+                         part.ys[1] <= nb[1] - 1 && part.zs[1] <= nb[2] - 1 && part.xs[0] < part.xs[1] &&
+                         part.ys[0] < part.ys[1] && part.zs[0] < part.zs[1] && part.xs[1] == nb[0] + 1 - part.xs[0] &&
+                         part.ys[1] == nb[1] + 1 - part.ys[0] && nb[0] <= 322 && P.ncoarse >= 1;
+    const bool use_part = part_ok && (c->fine_partial == 1 || (c->fine_partial < 0 && B >= FACE_SPECTRAL_MIN));
+    VPMX_TRY(pmesh_inf(c, gb, rv, sv, c->sol0, true, tree ? G.ilevmax : -1, P.ibctyp, use_part ? &part : nullptr));
     mark(c, ST_FINE, 1);
     used[ST_FINE] = used[ST_BOUNDS] = true;
     // 2. coarse correction
@@ -1121,7 +1139,7 @@ void vpmx_destroy(vpmx_ctx *p) {
                     &c->tune_scratch, &c->adj_a, &c->adj_b, &c->adj_c, &c->adj_d, &c->adj_e, &c->adj_f,
                     &c->adj_g, &c->adj_h, &c->adj_i, &c->adj_p, &c->adj_q, &c->adj_u, &c->adj_s, &c->adj_r,
                     &c->adj_xb, &c->adj_qb, &c->adj_xb2, &c->adj_qb2, &c->adj_xm, &c->adj_qm, &c->adj_pu,
-                    &c->adj_ps, &c->nd_face, &c->fd_spec};
+                    &c->adj_ps, &c->nd_face, &c->fd_spec, &c->fp_spec};
     for (DBuf *b : bufs) release(*b);
     for (auto &r : c->adj_steps)
         for (DBuf &b : r.b) release(b);
@@ -1191,6 +1209,9 @@ int32_t vpmx_set_option(vpmx_ctx *p, const char *name, int32_t value) {
         c->adj_m2p_scatter = value != 0;
     } else if (k == "dst_reuse") {
         c->dst_reuse = value != 0;
+    } else if (k == "fine_partial") {
+        if (value < -1 || value > 1) return fail(c, VPMX_ERR_ARG, "fine_partial: -1 by grid size, 0 every node, 1 faces and samples");
+        c->fine_partial = value;
     } else if (k == "face_derivs") {
         if (value < -1 || value > 1) return fail(c, VPMX_ERR_ARG, "face_derivs: -1 by grid size, 0 solved field, 1 spectral");
         c->face_derivs = value;

@@ -84,6 +84,8 @@ struct Ctx : vpmx_ctx {
     DBuf src_sig, src_cxyz, src_w, bs_part, lat_pairs, lat_pairs_c, panel_a;
     DBuf nd_face;  // normal derivatives at the face nodes (before the per-cell averages)
     DBuf fd_spec;  // the face sums of the spectral normal derivatives and their partial transforms
+    DBuf fp_spec;  // the second fine solve's face and sample values (sums, gathered planes, products)
+    int fine_partial = -1;  // second fine solve of a single block: -1 by grid size, 0 every node, 1 faces and samples
     int face_derivs = -1;  // normal derivatives of pmesh: -1 by grid size, 0 from the solved field, 1 spectral
     int bs_mode = 2;  // boundary sums (option boundsum): 2 spectral (default), 0 lattice tables, 1 direct LDS source tiles
     void *bsx[2] = {nullptr, nullptr};  // spectral boundary-sum plans: fine block, coarse grid
@@ -108,6 +110,8 @@ struct Ctx : vpmx_ctx {
     std::map<int, void *> sinm_cache;  // sin(pi (t+1)(k+1) / (m+1)), t, k < m, per m
     std::map<int, void *> sinx_cache;  // sin(pi r (k+1) / (m+1)), r <= m+1, k < m, per m
     std::map<std::pair<int, double>, void *> fdw_cache;  // normal-derivative weights per (m, h)
+    std::map<std::pair<int, int>, void *> sinr_cache;  // sin(pi p k / (m+1)), k = 1..m, per (m, p)
+    std::map<std::pair<int, int>, void *> sinl_cache;  // rows p = a s, a = 1..m/s, of the DST-I matrix, per (m, s)
     DBuf scan_tmp, red_tmp, rm_flag, rm_pos, rm_x, rm_q, rm_grid;
     DBuf tile_flag, tile_pos, tile_list;  // interpolation: cell tiles holding particles
     // projection without zeros for empty tiles: tile_flag maps the node tiles of a view of

@@ -121,8 +121,16 @@ int remesh_compact(Ctx *c, GView w, const int vn[3], const int lo[3], const int 
 enum SolveForm { SOLVE_CTX = -1, SOLVE_FP64 = 0, SOLVE_FP32 = 1, SOLVE_INV64 = 2 };
 // face_nd (zero BCs only): instead of the solution, the one-sided normal derivative at every
 // node of every face into that table (the layout of normal_derivs_table), from the spectrum.
+// part: only the nodes on the planes x = xs[0], xs[1], y = ys[...], z = zs[...] (interior
+// nodes; equal margins in x and y: xs[1] = nx + 1 - xs[0]) and the interior nodes 1 + a step
+// (a >= 1) in all three axes are written; the rest of the interior keeps its old values.
+struct SolvePartial {
+    int xs[2], ys[2], zs[2];
+    int step;
+};
 int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GView rhs, const GView *bc,
-                      GView out, int reuse = 0, int form = SOLVE_CTX, double *face_nd = nullptr);
+                      GView out, int reuse = 0, int form = SOLVE_CTX, double *face_nd = nullptr,
+                      const SolvePartial *part = nullptr);
 
 // Complex FFT plan of length L (Stockham radices, device twiddle table exp(-2 pi i t / L)),
 // cached per context.

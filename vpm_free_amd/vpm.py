@@ -222,7 +222,7 @@ class VPM:
         self._check(self._lib.vpmx_set_block_weights(self._h, _wp(self._w[0]), _wp(self._w[1]), _wp(self._w[2])))
 
     def set_option(self, name, value):
-        """boundsum (0 lattice, 1 direct, 2 spectral), dst_reuse (0/1), face_derivs (-1/0/1), tuning (0/1),
+        """boundsum (0 lattice, 1 direct, 2 spectral), dst_reuse (0/1), face_derivs (-1/0/1), fine_partial (-1/0/1), tuning (0/1),
         tune_variant (-1 or a variant)."""
         self._check(self._lib.vpmx_set_option(self._h, name.encode(), int(value)))
 
