@@ -384,11 +384,14 @@ def test_vpm_mode2_panels(vpmlib, oracle, h, sigma, tree):
 @pytest.mark.parametrize("h,sigma,blocks,tree,ibctyp", [
     (0.125, 0.2, (2, 1, 1), 0, 2), (0.125, 0.2, (1, 2, 1), 0, 2), (0.125, 0.2, (2, 2, 2), 0, 2),
     (0.125, 0.2, (3, 2, 1), 1, 2), (0.125, 0.2, (2, 1, 2), 0, 3), (1 / 16, 0.1, (6, 1, 1), 1, 2)])
-def test_vpm_mode2_blocks(vpmlib, oracle, h, sigma, blocks, tree, ibctyp):
+@pytest.mark.parametrize("shortcuts", [0, 1])
+def test_vpm_mode2_blocks(vpmlib, oracle, h, sigma, blocks, tree, ibctyp, shortcuts):
     """vpm(2) with NBI x NBJ x NBK yaps blocks on one device (rhsscat_3d, every block's fine
     solve, the coarse problem from all blocks' samples, interp_stencil with the neighbours'
     fine solutions, solget_3d) against the oracle; (6, 1, 1) with tree = 1, 3 levels is the
-    shipped pm.inp's combination."""
+    shipped pm.inp's combination.  `shortcuts`: the spectral normal derivatives and the
+    selected-node second solve forced on (several blocks keep every node of the second solve,
+    as the neighbours read it)."""
     V = vpmlib
     xp, qp = _ring_case(h, sigma)
     say("case: vpm mode 2 blocks %s h=%g tree=%d ibctyp=%d n=%d" % (blocks, h, tree, ibctyp, xp.shape[0]))
@@ -397,6 +400,8 @@ def test_vpm_mode2_blocks(vpmlib, oracle, h, sigma, blocks, tree, ibctyp):
     go = oracle.define_sizes(po, xp)
     up_r, gp_r, f = oracle.vpm2(po, go, xp, qp, fields=True)
     with V.VPM(V.Params.make(h, **kw)) as c:
+        c.set_option("face_derivs", shortcuts)
+        c.set_option("fine_partial", shortcuts)
         up, gp = np.zeros_like(xp), np.zeros_like(xp)
         c.vpm(xp, qp, up, gp, xp.shape[0], 3, 0, 0)
         assert bytes(c.geometry) == bytes(go)
