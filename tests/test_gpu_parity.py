@@ -679,16 +679,20 @@ def test_tuner_choices_keep_results(vpmlib):
 def test_tuner_variants_same_bits(vpmlib):
     """Every launch forced to its variant v (option tune_variant) gives the same velocity and
     stretching grids, UP and GP bit for bit, for every v and both forms, at h=1/16 (runtime FFT
-    plans, so the variants differ in scheduling only): the curl/stretch tile shapes, with the
-    velocity ring alone or with the stream function kept in registers and one LDS plane, the
-    FP64 matrix-product tile shapes, the sine-transform batch orders."""
+    plans, so the variants differ in scheduling only), with the fine solves' spectral shortcuts
+    on: the curl/stretch tile shapes, with the velocity ring alone or with the stream function
+    kept in registers and one LDS plane, the FP64
+    matrix-product tile shapes, the face sums' rows in flight, the sine-transform batch
+    orders."""
     V = vpmlib
     h = 1 / 16
     xp, qp = cases.single_ring(h=h, sigma=0.1)
     ref = {}
-    for v in range(6):
+    for v in range(7):
         with V.VPM(V.Params.make(h)) as c:
             c.set_option("tune_variant", v)
+            c.set_option("face_derivs", 1)
+            c.set_option("fine_partial", 1)
             for prec in (0, 1):
                 c.set_precision(prec)
                 up, gp = np.zeros_like(xp), np.zeros_like(xp)
