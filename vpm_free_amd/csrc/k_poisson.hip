@@ -55,6 +55,11 @@ struct DstArgs {
     int order;             // one-wave kernels' batch order: 0 the two halves of a batch pair on
                            // work-groups w and w+8 (one XCD) at once, 1 one after the other on
                            // one work-group (the second half's loads find the lines in L2)
+    // one-wave z pass with zsum set: forward transform and diagonal only (the spectrum stored),
+    // and per line sum_k zw[k-1] u_k (low z face) and sum_k (-1)^(k+1) zw[k-1] u_k (high) into
+    // zsum[side][comp][ky][kx]
+    double *zsum;
+    const double *zw;
 };
 
 // Mixed-radix Stockham FFT of npairs sequences of length L held in LDS; returns the
@@ -892,6 +897,7 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
     // (every component's batches in one persistent grid: global batch gb is batch
     // gb mod nbatch of component gb / nbatch)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool zf = MODE == DST_Z && a.zsum != nullptr;  // forward transform and diagonal only
     // (S = 0: the plain pointer, so those kernels compile exactly as unpadded code)
     using Buf = typename std::conditional<S == 0, C *, PadBuf<C, S>>::type;
     const Buf B{A + wave * Np};
@@ -979,9 +985,11 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
         if (!ODD) wave_post<C>(B, N, lane);
         if (MODE == DST_Z) {
             const double la = lamxy[2 * wave], lb = lamxy[2 * wave + 1];
+            double so[2] = {0.0, 0.0}, se[2] = {0.0, 0.0};  // z-face sums over odd / even modes
             for (int j = 1 + lane; j <= m; j += 64) {
                 const double lz = lzs[j - 1];
                 const C z = B[j];
+                C g;
                 if constexpr (ODD) {
                     // odd-extension form: the transform's outputs F = (-Im Z/2, Re Z/2), the
                     // diagonal (factors from the FP32 reciprocal, 1 ulp) and the odd extension
@@ -989,33 +997,65 @@ __device__ __forceinline__ void dst_wave_body(const DstArgs &a, const WavePlan &
                     const float nz = (float)a.norm, h = 0.5f;
                     const float fa = -nz * __builtin_amdgcn_rcpf((float)(la + lz));
                     const float fb = -nz * __builtin_amdgcn_rcpf((float)(lb + lz));
-                    const C g = Cx<C>::mk((-h * z.y) * fa, (h * z.x) * fb);
+                    g = Cx<C>::mk((-h * z.y) * fa, (h * z.x) * fb);
                     B[j] = g;
-                    B[N - j] = Cx<C>::mk(-g.x, -g.y);
+                    if (!zf) B[N - j] = Cx<C>::mk(-g.x, -g.y);
                 } else {
-                    B[j] = Cx<C>::mk(z.x * neg_quot(a.norm, la + lz), z.y * neg_quot(a.norm, lb + lz));
+                    g = Cx<C>::mk(z.x * neg_quot(a.norm, la + lz), z.y * neg_quot(a.norm, lb + lz));
+                    B[j] = g;
+                }
+                if (zf) {
+                    const double wj = a.zw[j - 1];
+                    if (j & 1) {
+                        so[0] += wj * (double)g.x;
+                        so[1] += wj * (double)g.y;
+                    } else {
+                        se[0] += wj * (double)g.x;
+                        se[1] += wj * (double)g.y;
+                    }
                 }
             }
-            if constexpr (ODD) {
-                if (lane == 0) B[0] = B[m + 1] = Cx<C>::mk(0, 0);
-                __builtin_amdgcn_wave_barrier();
+            if (zf) {
+                // the line pair's sums over the wave: low face odd + even modes, high face odd - even
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+                    for (int d = 1; d < 64; d <<= 1) {
+                        so[t] += __shfl_xor(so[t], d, 64);
+                        se[t] += __shfl_xor(se[t], d, 64);
+                    }
+                if (lane < 2) {
+                    const int gl = b * nl + 2 * wave + lane;
+                    if (dst_line_ok<MODE>(a, gl)) {
+                        const int kx = gl % a.px, ky = gl / a.px;
+                        const double o = lane ? so[1] : so[0], e = lane ? se[1] : se[0];
+                        a.zsum[((int64_t)comp * a.my + ky) * a.mx + kx] = o + e;
+                        a.zsum[((int64_t)(a.ncomp + comp) * a.my + ky) * a.mx + kx] = o - e;
+                    }
+                }
             } else {
-                __builtin_amdgcn_wave_barrier();
-                wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
+                if constexpr (ODD) {
+                    if (lane == 0) B[0] = B[m + 1] = Cx<C>::mk(0, 0);
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    __builtin_amdgcn_wave_barrier();
+                    wave_pre<C>(B, a.sn, N, lane, nullptr, 0.0);
+                }
+                if constexpr (FIX >= 0) FixedFft<FIX>::template run<C, decltype(B), CROOTS>(B, twl, lane);
+                else wave_fft<MAXR>(B, twl, P, lane);
+                if (!ODD) wave_post<C>(B, N, lane);
             }
-            if constexpr (FIX >= 0) FixedFft<FIX>::template run<C, decltype(B), CROOTS>(B, twl, lane);
-            else wave_fft<MAXR>(B, twl, P, lane);
-            if (!ODD) wave_post<C>(B, N, lane);
         }
         __syncthreads();
         const WaveAddr ad = wave_addr<MODE>(a, comp, b * nl + tline, tpt0);
         // odd-extension form: point k of line 2p is -Im Z_k / 2, of line 2p+1 Re Z_k / 2 (the
-        // other part of the same complex slot)
+        // other part of the same complex slot); a forward-only z pass stores the diagonal's
+        // outputs as they are (line 2p in the real part)
         const T hs = ODD ? ((tline & 1) ? (T)0.5 : (T)-0.5) : (T)1;
 #pragma unroll
         for (int u = 0; u < DST_PF; u++) {
             if (!ad.ok || tpt0 + u * WV_PS >= m) continue;
-            const T val = ODD ? (hs * Ad[(slot0 + u * SSTEP) ^ 1]) * (T)a.scale : Ad[slot0 + u * SSTEP] * (T)a.scale;
+            const T val = (ODD && !zf) ? (hs * Ad[(slot0 + u * SSTEP) ^ 1]) * (T)a.scale
+                                       : Ad[slot0 + u * SSTEP] * (T)a.scale;
             if (MODE == DST_XI) a.out.p[ad.dst + u * ad.dst_step] = (double)val;
             else w[ad.dst + u * ad.dst_step] = val;
         }
@@ -1584,20 +1624,23 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     // real launch then overwrites)
     static const char *mname[4] = {"x forward", "y", "z", "x inverse"};
     char key[160];
-    std::snprintf(key, sizeof key, "sine transforms %s pass %s L=%d m=%d lines=%lld comps=%d", mname[MODE],
-                  dbl ? "FP64" : "FP32", a.L, a.m, (long long)a.nlines, ncomp);
+    std::snprintf(key, sizeof key, "sine transforms %s pass %s L=%d m=%d lines=%lld comps=%d%s", mname[MODE],
+                  dbl ? "FP64" : "FP32", a.L, a.m, (long long)a.nlines, ncomp,
+                  a.zsum ? " (forward and diagonal, z-face sums)" : "");
     Ctx::Tuned ch;
     {
         int st = VPMX_OK;
-        const size_t wb = sizeof(T) * (size_t)a.wcs * ncomp;
+        const size_t wb = (sizeof(T) * (size_t)a.wcs * ncomp + 255) / 256 * 256;
+        const size_t zb = a.zsum ? sizeof(double) * 2 * (size_t)ncomp * a.mx * a.my : 0;
         auto trial = [&](int fi, int pc) -> int {
-            if (!c->tune_scratch.p || c->tune_scratch.bytes < wb) {
-                void *p = ensure(c, c->tune_scratch, wb, &st);
+            if (!c->tune_scratch.p || c->tune_scratch.bytes < wb + zb) {
+                void *p = ensure(c, c->tune_scratch, wb + zb, &st);
                 if (!p) return st;
                 VPMX_TRY(fill(c, (double *)p, (int64_t)(c->tune_scratch.bytes / sizeof(double)), 0.0));
             }
             DstArgs t = a;
             t.w = t.wo = (double *)c->tune_scratch.p;
+            if (a.zsum) t.zsum = (double *)((char *)c->tune_scratch.p + wb);
             t.order = vars[fi].order;
             WavePlan Pt = P;
             void *targs[] = {&t, &Pt};
@@ -1612,7 +1655,7 @@ static int launch_wave(Ctx *c, DstArgs a, int ncomp, int rc, const FftPlan *fp) 
     hipEvent_t e0, e1;
     void *args[] = {&a, &P};
     constexpr int kc = MODE == DST_Y ? K_DST : K_DSTZ;
-    if ((MODE == DST_Y || MODE == DST_Z) && a.m == c->dst_timed_m && kernel_events(c, kc, &e0, &e1)) {
+    if ((MODE == DST_Y || MODE == DST_Z) && a.m == c->dst_timed_m && !a.zsum && kernel_events(c, kc, &e0, &e1)) {
         // the fine block's y and z passes: timed by their own dispatch events; work = bytes
         // moved (read + write of every interior point)
         c->k_work[kc] = 2.0 * sizeof(T) * (double)a.mx * (MODE == DST_Y ? a.mz : a.my) * a.m * ncomp;
@@ -2084,26 +2127,46 @@ static int sine_matrix_ext(Ctx *c, int m, const double **out) {
     return VPMX_OK;
 }
 
+// Work buffers of the spectral normal derivatives (the context keeps them): x / y face sums,
+// z face sums, the z faces' partial products, and the x / y faces' partial products when the z
+// pass left z in sine space.
+struct FaceBufs {
+    double *xp, *yp, *zp, *v, *tx, *ty;
+};
+static int face_bufs(Ctx *c, int mx, int my, int mz, int ncomp, FaceBufs *fb) {
+    int st = VPMX_OK;
+    const int np = 2 * ncomp, NX = mx + 2, NY = my + 2, NZ = mz + 2;
+    const int64_t nxp = (int64_t)np * NZ * my, nyp = (int64_t)np * NZ * mx, nzp = (int64_t)np * my * mx,
+                  nv = (int64_t)np * my * NX, ntx = (int64_t)np * NZ * NY, nty = (int64_t)np * NZ * NX;
+    double *p = (double *)ensure(c, c->fd_spec, sizeof(double) * (size_t)(nxp + nyp + nzp + nv + ntx + nty), &st);
+    if (st) return st;
+    fb->xp = p;
+    fb->yp = fb->xp + nxp;
+    fb->zp = fb->yp + nyp;
+    fb->v = fb->zp + nzp;
+    fb->tx = fb->v + nv;
+    fb->ty = fb->tx + ntx;
+    return VPMX_OK;
+}
+
 // The face-node table fd (faces XMIN, XMAX, YMIN, YMAX, ZMIN, ZMAX; each ncomp x N0 x N1 with
 // (N0, N1) = (NY, NZ), (NX, NZ), (NX, NY), the first index fastest) from the work array after
-// the z pass (a.w, FP32 when f32).  Every entry is written (zero on the face edges).
-static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, const double h[3], double *fd) {
-    int st = VPMX_OK;
+// the z pass (a.w, FP32 when f32).  zspec: the z pass stopped after the diagonal (z still in
+// sine space) and left the z faces' sums in fb.zp; else the z faces are summed here from the
+// planes next to them.  Every entry is written (zero on the face edges).
+static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, const double h[3], double *fd,
+                                bool zspec, const FaceBufs &fb) {
     const int mx = a.mx, my = a.my, mz = a.mz, np = 2 * ncomp;
     const int NX = mx + 2, NY = my + 2, NZ = mz + 2;
     if (mx > 64 * FR_COLS) return fail(c, VPMX_ERR_UNSUP, "spectral normal derivatives: rows of at most 320 points");
     if (mx < 4 || my < 4 || mz < 4) return fail(c, VPMX_ERR_ARG, "normal derivatives need >= 6 nodes per axis");
-    const double *wx = nullptr, *wy = nullptr, *SX = nullptr, *SY = nullptr;
+    const double *wx = nullptr, *wy = nullptr, *SX = nullptr, *SY = nullptr, *SZ = nullptr;
     VPMX_TRY(face_weights(c, mx, h[0], &wx));
     VPMX_TRY(face_weights(c, my, h[1], &wy));
     VPMX_TRY(sine_matrix_ext(c, mx, &SX));
     VPMX_TRY(sine_matrix_ext(c, my, &SY));
-    const int64_t nxp = (int64_t)np * NZ * my, nyp = (int64_t)np * NZ * mx, nzp = (int64_t)np * my * mx,
-                  nv = (int64_t)np * my * NX;
-    double *xp = (double *)ensure(c, c->fd_spec, sizeof(double) * (size_t)(nxp + nyp + nzp + nv), &st);
-    if (st) return st;
-    double *yp = xp + nxp, *zp = yp + nyp, *v = zp + nzp;
-    const dim3 gz((unsigned)(((int64_t)mx * my + 255) / 256), (unsigned)ncomp);
+    if (zspec) VPMX_TRY(sine_matrix_ext(c, mz, &SZ));
+    double *xp = fb.xp, *yp = fb.yp, *zp = fb.zp, *v = fb.v;
     // rows in flight per wave: two candidates per precision, same bits (the row order of every
     // sum is fixed)
     const void *fns[2];
@@ -2117,9 +2180,9 @@ static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, c
     const char *names[2] = {f32 ? "4 rows in flight" : "2 rows in flight", f32 ? "8 rows in flight" : "4 rows in flight"};
     const void *wp = a.w;
     int64_t wcs = a.wcs;
-    int px = a.px, ncv = ncomp;
+    int px = a.px, ncv = ncomp, mxv = mx, myv = my, mzv = mz;
     auto trial = [&](int var, int) -> int {
-        void *kargs[] = {(void *)&wp, &wcs, &px, (void *)&mx, (void *)&my, (void *)&mz, &ncv, &wx, &wy, &xp, &yp};
+        void *kargs[] = {(void *)&wp, &wcs, &px, &mxv, &myv, &mzv, &ncv, &wx, &wy, &xp, &yp};
         VPMX_HIP(c, hipLaunchKernel(fns[var], dim3((unsigned)mz, (unsigned)ncomp), dim3(256), kargs, 0, c->stream));
         return VPMX_OK;
     };
@@ -2128,17 +2191,31 @@ static int face_derivs_spectral(Ctx *c, const DstArgs &a, int ncomp, bool f32, c
     Ctx::Tuned ch;
     VPMX_TRY(tune(c, key, 2, fns, names, 0, 256, trial, &ch, false));
     VPMX_TRY(trial(ch.fi, 0));
-    if (f32)
-        hipLaunchKernelGGL(k_face_reduce_z<float>, gz, dim3(256), 0, c->stream, (const float *)a.w, a.wcs, a.px, mx, my,
-                           mz, ncomp, 1.0 / h[2], zp);
-    else
-        hipLaunchKernelGGL(k_face_reduce_z<double>, gz, dim3(256), 0, c->stream, (const double *)a.w, a.wcs, a.px, mx,
-                           my, mz, ncomp, 1.0 / h[2], zp);
-    VPMX_HIP(c, hipGetLastError());
+    if (!zspec) {
+        const dim3 gz((unsigned)(((int64_t)mx * my + 255) / 256), (unsigned)ncomp);
+        if (f32)
+            hipLaunchKernelGGL(k_face_reduce_z<float>, gz, dim3(256), 0, c->stream, (const float *)a.w, a.wcs, a.px, mx,
+                               my, mz, ncomp, 1.0 / h[2], zp);
+        else
+            hipLaunchKernelGGL(k_face_reduce_z<double>, gz, dim3(256), 0, c->stream, (const double *)a.w, a.wcs, a.px,
+                               mx, my, mz, ncomp, 1.0 / h[2], zp);
+        VPMX_HIP(c, hipGetLastError());
+    }
     const int64_t oy = (int64_t)np * NY * NZ, oz = oy + (int64_t)np * NX * NZ;
-    // x faces: fd[b][k][j] = sum_ky S_y(j, ky) xp[b][k][ky]; y faces likewise with S_x
-    VPMX_TRY(dgemm(c, false, false, NY, np * NZ, my, SY, NY, 0, xp, my, 0, fd, NY, 0, 1));
-    VPMX_TRY(dgemm(c, false, false, NX, np * NZ, mx, SX, NX, 0, yp, mx, 0, fd + oy, NX, 0, 1));
+    if (zspec) {
+        // x faces: tx[b][kz][j] = sum_ky S_y(j, ky) xp[b][kz][ky], fd[b][k][j] = sum_kz tx[b][kz][j] S_z(k, kz)
+        VPMX_TRY(dgemm(c, false, false, NY, np * NZ, my, SY, NY, 0, xp, my, 0, fb.tx, NY, 0, 1));
+        VPMX_TRY(dgemm(c, false, true, NY, NZ, mz, fb.tx + NY, NY, (int64_t)NY * NZ, SZ, NZ, 0, fd, NY,
+                       (int64_t)NY * NZ, np));
+        // y faces likewise with S_x
+        VPMX_TRY(dgemm(c, false, false, NX, np * NZ, mx, SX, NX, 0, yp, mx, 0, fb.ty, NX, 0, 1));
+        VPMX_TRY(dgemm(c, false, true, NX, NZ, mz, fb.ty + NX, NX, (int64_t)NX * NZ, SZ, NZ, 0, fd + oy, NX,
+                       (int64_t)NX * NZ, np));
+    } else {
+        // x faces: fd[b][k][j] = sum_ky S_y(j, ky) xp[b][k][ky]; y faces likewise with S_x
+        VPMX_TRY(dgemm(c, false, false, NY, np * NZ, my, SY, NY, 0, xp, my, 0, fd, NY, 0, 1));
+        VPMX_TRY(dgemm(c, false, false, NX, np * NZ, mx, SX, NX, 0, yp, mx, 0, fd + oy, NX, 0, 1));
+    }
     // z faces: v[b][ky][i] = sum_kx S_x(i, kx) zp[b][ky][kx], fd[b][j][i] = sum_ky v[b][ky][i] S_y(j, ky)
     VPMX_TRY(dgemm(c, false, false, NX, np * my, mx, SX, NX, 0, zp, mx, 0, v, NX, 0, 1));
     VPMX_TRY(dgemm(c, false, true, NX, NY, my, v, NX, (int64_t)NX * my, SY, NY, 0, fd + oz, NX, (int64_t)NX * NY, np));
@@ -2445,8 +2522,19 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
             std::copy(key, key + 6, c->save_key);
         }
     }
-    // z forward + diagonal + inverse (into wo), then the inverse passes in place there
+    // z forward + diagonal + inverse (into wo), then the inverse passes in place there; for
+    // the face derivatives alone, with the one-wave kernel, z forward + diagonal and the z
+    // faces' sums
     a.nlines = (int64_t)a.px * m[1];
+    FaceBufs fb{};
+    const bool zspec = face_nd && !inv64 && (f32 || (c->dst_kernel == 4 && wave_class(ax[2], false)));
+    if (face_nd) {
+        VPMX_TRY(face_bufs(c, m[0], m[1], m[2], ncomp, &fb));
+        if (zspec) {
+            a.zsum = fb.zp;
+            VPMX_TRY(face_weights(c, m[2], h[2], &a.zw));
+        }
+    }
     if (inv64) {
         // FP32 forward transforms and diagonal; the inverse passes in FP64
         double *w64 = (double *)ensure(c, c->dst_work64, sizeof(double) * (size_t)a.wcs * ncomp, &st);
@@ -2459,7 +2547,8 @@ int poisson_dirichlet(Ctx *c, int ncomp, const int wn[3], const double h[3], GVi
     }
     if (face_nd) {
         if (inv64 || bc) return fail(c, VPMX_ERR_ARG, "Poisson: face derivatives of a zero-BC solve only");
-        return face_derivs_spectral(c, a, ncomp, f32, h, face_nd);
+        a.zsum = nullptr;
+        return face_derivs_spectral(c, a, ncomp, f32, h, face_nd, zspec, fb);
     }
     if (part) {
         if (inv64) return fail(c, VPMX_ERR_ARG, "Poisson: selected values of the FP32/FP64 passes only");
