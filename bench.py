@@ -81,7 +81,8 @@ def chosen_variant(tuning, mixed, axis):
     ("8.8.9 plan, capped, 2 work-groups per CU"), or None."""
     head = "sine transforms %s pass %s L=%d " % (axis, "FP32" if mixed else "FP64", 576 if mixed else 288)
     for line in tuning:
-        if line.startswith(head) and " -> " in line:
+        # (not the zero-BC solve's z pass, which stops after the diagonal)
+        if line.startswith(head) and " -> " in line and "(forward and diagonal" not in line:
             return line.split(" -> ", 1)[1].rsplit(" (", 1)[0]
     return None
 
@@ -351,7 +352,8 @@ def main():
         "traffic_source": "profiles/dst_passes_289_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         "kernel": "k_dst_wave<DST_%s,%s>: %s pass of the 289^3 block's sine-transform Poisson solve (3 comps%s)"
                   % (axis.upper(), "float" if mixed else "double", axis,
-                     "" if axis == "y" else "; forward transform, diagonal and inverse transform"),
+                     "" if axis == "y" else "; forward transform, diagonal and inverse transform -- the second"
+                     " fine solve's; the zero-BC solve's z pass stops after the diagonal and is not counted"),
         "variant": variant,
         "ms_per_launch": dk.get("ms"),
         "launches_per_step": round(dk.get("launches", 0) / args.steps, 2),
