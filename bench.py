@@ -286,8 +286,8 @@ def adjoint_leg(V, nsteps, ckpt, dt=0.01):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this library runs on one GPU (the yaps blocks would be the units of a "
+                                   "sharded run; it is not launched)", "n_gpus": args.gpus}))
         sys.exit(2)
     from vpm_free_amd import vpm as V
 
