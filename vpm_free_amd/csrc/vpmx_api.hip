@@ -1058,12 +1058,32 @@ int step_rk2_enqueue(Ctx *c, double dt, const double uinf[3], int ntime, int mre
     double *q0 = (double *)ensure(c, c->q0, sizeof(double) * nr * (size_t)(n ? n : 1), &st); if (st) return st;
     double *xp = (double *)c->xp.p, *qp = (double *)c->qp.p, *up = (double *)c->up.p, *gp = (double *)c->gp.p;
     TimingScope timing(c);  // the stage times of the whole step
-    if (n > 0) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    VPMX_HIP(c, hipStreamIsCapturing(c->stream, &cap));
+    const bool swap = !c->dn && cap == hipStreamCaptureStatusNone;
+    if (swap) {
+        // the step's start particles become (x0, q0) by exchanging the buffers (both hold n
+        // particles), and the first update writes the resident arrays from them
+        std::swap(c->x0, c->xp);
+        std::swap(c->q0, c->qp);
+        x0 = (double *)c->x0.p;
+        q0 = (double *)c->q0.p;
+        xp = (double *)c->xp.p;
+        qp = (double *)c->qp.p;
+    } else if (n > 0) {
+        // (a batch's cloud, or under capture: the buffers' roles stay fixed for replays)
         VPMX_HIP(c, hipMemcpyAsync(x0, xp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, c->stream));
         VPMX_HIP(c, hipMemcpyAsync(q0, qp, sizeof(double) * nr * n, hipMemcpyDeviceToDevice, c->stream));
     }
-    // stage 1 at (x0, q0)
-    VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
+    // stage 1 at (x0, q0); if it fails, the resident arrays get their particles back
+    const int r1 = run_vpm(c, 2, x0, q0, up, gp, n);
+    if (r1 != VPMX_OK) {
+        if (swap) {
+            std::swap(c->x0, c->xp);
+            std::swap(c->q0, c->qp);
+        }
+        return r1;
+    }
     VPMX_TRY(euler_update(c, xp, qp, nr, up, gp, n, 0.5 * dt, uinf, x0, q0));
     // stage 2 at the midpoint
     VPMX_TRY(run_vpm(c, 2, xp, qp, up, gp, n));
