@@ -969,17 +969,24 @@ int m2p_gather(Ctx *c, const Grid &g, const SortState &s, const double *xp, cons
 __global__ void k_euler(double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, const int *dn,
                         double dt, double ux, double uy, double uz, const double *x0, const double *q0) {
 #pragma clang fp contract(off)
-    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pcount(n, dn)) return;
-    double ui[3] = {ux, uy, uz};
-    for (int d = 0; d < 3; d++) xp[3 * p + d] = x0[3 * p + d] + (up[3 * p + d] + ui[d]) * dt;
-    for (int d = 0; d < 3; d++) qp[(int64_t)qs * p + d] = q0[(int64_t)qs * p + d] - gp[3 * p + d] * dt;
-    for (int d = 3; d < qs; d++) qp[(int64_t)qs * p + d] = q0[(int64_t)qs * p + d];
+    // one thread per array element (positions, then strengths), so neighbouring threads touch
+    // neighbouring doubles
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = pcount(n, dn), nx = 3 * np;
+    if (e < nx) {
+        const int d = (int)(e % 3);
+        const double ui = d == 0 ? ux : (d == 1 ? uy : uz);
+        xp[e] = x0[e] + (up[e] + ui) * dt;
+    } else if (e - nx < (int64_t)qs * np) {
+        const int64_t f = e - nx, p = f / qs;
+        const int d = (int)(f - p * qs);
+        qp[f] = d < 3 ? q0[f] - gp[3 * p + d] * dt : q0[f];
+    }
 }
 int euler_update(Ctx *c, double *xp, double *qp, int qs, const double *up, const double *gp, int64_t n, double dt,
                  const double uinf[3], const double *x0, const double *q0) {
     if (n <= 0) return VPMX_OK;
-    hipLaunchKernelGGL(k_euler, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, xp, qp, qs, up, gp, n, c->dn,
+    hipLaunchKernelGGL(k_euler, dim3((unsigned)(((3 + qs) * n + 255) / 256)), dim3(256), 0, c->stream, xp, qp, qs, up, gp, n, c->dn,
                        dt, uinf[0], uinf[1], uinf[2], x0, q0);
     VPMX_HIP(c, hipGetLastError());
     return VPMX_OK;
